@@ -1,0 +1,172 @@
+/*
+ * libsds_hip -- the MI355X (gfx950) numeric phase of SparseDirectSolver.jl's
+ * multifrontal solver behind a plain C ABI.
+ *
+ * SparseDirectSolver.jl has no C ABI of its own: the numeric phase is reached
+ * from `execute!` through `_numeric_phase!` (src/solver.jl:717-741) and the
+ * solve phase through `_solve_phase!` (src/solver.jl:810-871), both on Julia
+ * objects (`Symbolic`, `Numeric`, `SolveWorkspace`). Each entry point below
+ * replaces one of those Julia calls; a Julia package extension binds them with
+ * `ccall` (INTEGRATION.md shows the binding).
+ *
+ * Conventions
+ *  - One handle per solver, bound to one HIP stream; every call is
+ *    asynchronous on that stream except where a host result is returned.
+ *    One host thread per handle (a `DirectSolver` is an unlocked mutable
+ *    handle, src/solver.jl:187).
+ *  - Integer arrays of the analysis are passed exactly as the Julia
+ *    `Symbolic` holds them: 1-based, element offsets, Int32 or Int64
+ *    (`index_bits`).
+ *  - Device pointers (nzval, B, X) are borrowed for the duration of the call;
+ *    the library owns the factor, the contribution-block pool and the solve
+ *    and refinement workspaces (`allocate_numeric`, src/numeric/storage.jl:205-256;
+ *    `allocate_solve`, src/solve/sweeps.jl:164-172).
+ *  - Return value: SDS_SUCCESS or an error code mapping to the reference's
+ *    exception types (src/errors.jl:10-57); sds_last_error() has the message.
+ *    A numerical failure is not an error: it is reported through `info`
+ *    (src/numeric/factorize.jl:300-302).
+ */
+#ifndef SDS_HIP_H
+#define SDS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sds_handle* sds_handle_t;
+
+enum {
+    SDS_SUCCESS = 0,
+    SDS_INVALID_VALUE = 1,       /* InvalidValueError  (src/errors.jl:18) */
+    SDS_NOT_SUPPORTED = 2,       /* NotSupportedError  (src/errors.jl:30) */
+    SDS_FACTORIZATION_ERROR = 3, /* FactorizationError (src/errors.jl:42), e.g. phase order */
+    SDS_DEVICE_ERROR = 4,        /* a HIP runtime error */
+    SDS_INTERRUPTED = 5          /* InterruptedError   (src/logging.jl:59-63) */
+};
+
+/* element types of the factor (the input values are always Float64) */
+enum { SDS_R64 = 0, SDS_R32 = 1 };
+
+/* structures: cudssMatrixType_t values (src/types.jl:17-23) */
+enum { SDS_STRUCTURE_G = 0, SDS_STRUCTURE_S = 1, SDS_STRUCTURE_H = 2, SDS_STRUCTURE_SPD = 3, SDS_STRUCTURE_HPD = 4 };
+
+/* solve phases: cudssPhase_t bits (src/types.jl:56-71) */
+enum {
+    SDS_PHASE_SOLVE_FWD_PERM = 16,
+    SDS_PHASE_SOLVE_FWD = 32,
+    SDS_PHASE_SOLVE_DIAG = 64,
+    SDS_PHASE_SOLVE_BWD = 128,
+    SDS_PHASE_SOLVE_BWD_PERM = 256,
+    SDS_PHASE_SOLVE_REFINEMENT = 512,
+    SDS_PHASE_SOLVE = 1008
+};
+
+/*
+ * Create a handle on `stream` (a hipStream_t; NULL = the default stream) and
+ * the current HIP device. Replaces the backend binding that `DirectSolver`
+ * makes at construction (src/solver.jl:226-244).
+ */
+int sds_create(sds_handle_t* handle, void* stream);
+int sds_destroy(sds_handle_t handle);
+const char* sds_last_error(sds_handle_t handle);
+int sds_set_stream(sds_handle_t handle, void* stream);
+
+/*
+ * The device maps of an analysis: `DEVICE_MAPS` of src/symbolic/maps.jl:16-18
+ * (the entries the numeric and solve phases read), 1-based as in `Symbolic`,
+ * each array of `index_bits` (32 or 64) integers; `amap` entries are signed
+ * (negative = mirrored entry: conjugate for "H"/"HPD", the U^T structure for
+ * "G"). The refinement map (`refinement_map`, src/solve/refinement.jl:67-101:
+ * CSR of the full matrix over the user's nzval, src < 0 = conjugated) may be
+ * NULL when no refinement is wanted; for "G" it holds 2n rows (M, then M^T).
+ */
+typedef struct {
+    int64_t n, nnz, ns;
+    int32_t structure; /* SDS_STRUCTURE_* */
+    int32_t index_bits;
+    const void *perm, *super_ptr, *snparent, *rowptr, *rowval;
+    const void *front_ptr, *front_nrows, *front_ncols;
+    const void *child_ptr, *child_list, *relind_ptr, *relind;
+    const void *amap, *amap_ptr, *amap_src;
+    int64_t full_nrows, full_nnz;
+    const void *full_rowptr, *full_colval, *full_src;
+} sds_symbolic_desc;
+
+/*
+ * Upload an analysis and build the device schedule (tree levels, size
+ * classes, contribution-block placement). Replaces `adapt(backend, S, INT)`
+ * (src/symbolic/maps.jl:288-294) plus the launch plan of `NumericPlan`
+ * (src/numeric/storage.jl:36-104).
+ */
+int sds_upload_symbolic(sds_handle_t handle, const sds_symbolic_desc* desc);
+
+/*
+ * Allocate the numeric storage for factors of element type `dtype`
+ * (SDS_R64, or SDS_R32 = FP32 factor with FP64 refinement) and `nbatch`
+ * members (1). Replaces `allocate_numeric` (src/numeric/storage.jl:205-256).
+ */
+int sds_alloc_numeric(sds_handle_t handle, int32_t dtype, int32_t nbatch);
+
+/* pivoting parameters of one factorization (src/options.jl:155-222) */
+typedef struct {
+    int32_t pivot_type;        /* cudssPivotType_t value (src/types.jl:84-92) */
+    int32_t pivot_epsilon_alg; /* 0 default, 1 "algo1" (scaled by max |a_ij|), 2 "algo2" */
+    double pivot_threshold;
+    double pivot_epsilon;      /* < 0: the default for the factor precision (src/options.jl:57-61) */
+    const int8_t* pivot_sign;  /* host, n entries (original rows), or NULL */
+    const volatile int32_t* interrupt; /* host flag polled between launch groups, or NULL */
+} sds_factor_params;
+
+/*
+ * Numeric factorization of the values `nzval` (device, Float64, nnz entries in
+ * the order of the user's CSR). Replaces `factorize!` / `factorize_cholesky!`
+ * / `factorize_ldlt!` / `factorize_lu!` (src/numeric/factorize.jl:315-360,
+ * src/numeric/ldlt.jl:1447-1475, src/numeric/lu.jl:543-555), including the
+ * one device-to-host read of the reduced status: `*info` is 0 or the original
+ * column (1-based) of the first failed pivot (src/numeric/factorize.jl:383-417).
+ */
+int sds_factorize(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int64_t* info);
+
+/*
+ * Solve phases with the current factor: `phase` is SDS_PHASE_SOLVE or an OR
+ * of its sub-phases, run in order fwd_perm, fwd, diag, bwd, bwd_perm,
+ * refinement (src/solver.jl:846-869). B and X are device, column-major,
+ * Float64, n x nrhs with leading dimensions ldb/ldx (X == B allowed except
+ * with refinement). `solve_mode` 0 solves A x = b, 1 A^T x = b
+ * (src/solve/refinement.jl:10-15). Refinement (`refine!`,
+ * src/solve/refinement.jl:359-404) runs `ir_n_steps` steps of
+ * X += A^-1 (B - A X) with the residual in Float64 on the values `nzval`
+ * (device; may be NULL without refinement), stopping early when
+ * ir_tol > 0 and every ||R_k||_2 <= ir_tol ||B_k||_2; the steps done go to
+ * *ir_steps (may be NULL). Replaces permute_rhs!, forward_sweep!,
+ * diagonal_sweep!, backward_sweep!, unpermute_solution! and refine!
+ * (src/solve/permute.jl:117-150, src/solve/sweeps.jl:908-1039).
+ */
+int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const double* B, int64_t ldb, double* X,
+              int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
+
+/*
+ * Data parameters after a factorization (src/solver.jl getparam, the
+ * `Numeric` fields of src/numeric/storage.jl:161-179):
+ *  stats: 6 * ns Int64 per-front (npos, nneg, nzero, nperturbed, n2x2, info);
+ *  totals: 6 Int64 (sums; totals[5] = fronts with a failure);
+ *  inertia = (totals[0], totals[1]); npivots = totals[3]
+ *  (src/reference/ldlt.jl:570-616).
+ */
+int sds_get_stats(sds_handle_t handle, int64_t* stats, int64_t* totals);
+/* local pivot order (n Int32, 1-based factor columns, `numeric.piv`) and pivot kinds (n Int8) */
+int sds_get_pivots(sds_handle_t handle, int32_t* piv, int8_t* kind);
+/* D (2n entries of the factor element type: diagonal, then 2x2 subdiagonals); "diag" */
+int sds_get_diag(sds_handle_t handle, void* d);
+/* host copy of the factor panels (factor_len entries of the factor element type);
+ * which = 0: `factor` (L), 1: `ufactor` (U^T of "G") */
+int sds_get_factor(sds_handle_t handle, int32_t which, void* out);
+/* "memory_estimates" (src/symbolic/maps.jl:338-356): 16 Int64, bytes */
+int sds_memory_estimates(sds_handle_t handle, int64_t* est);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDS_HIP_H */

@@ -1,0 +1,892 @@
+// Host symbolic analysis of SparseDirectSolver.jl, restated in C++ so that the
+// numeric phase can be driven without a Julia runtime. The analysis is the
+// input producer of the hot path: it turns a CSR pattern and an ordering into
+// the supernodal partition, the panel layout and the assembly/extend-add maps
+// that the device numeric phase and the CPU oracle both consume.
+//
+// Every output array is 1-based, exactly as the Julia `Symbolic` holds it
+// (src/symbolic/maps.jl:62-95), so the arrays can be handed to the C ABI of the
+// numeric library unchanged.
+//
+// Steps and the reference functions they follow:
+//   symmetric pattern           src/symbolic/pattern.jl:138-164 (view filter :113-120)
+//   elimination tree            src/symbolic/etree.jl:14-36
+//   postorder                   src/symbolic/etree.jl:45-78
+//   column counts (GNP)         src/symbolic/etree.jl:81-147
+//   fundamental supernodes      src/symbolic/supernodes.jl:90-112
+//   relaxed amalgamation        src/symbolic/supernodes.jl:142-228
+//   supernodal rows / flops     src/symbolic/supernodes.jl:285-339
+//   panel layout (panel_ptr)    src/symbolic/layout.jl:271-276
+//   relind                      src/symbolic/maps.jl:110-131
+//   amap + owner grouping       src/symbolic/maps.jl:145-187
+//   child lists                 src/symbolic/maps.jl:204-217
+//   full-pattern residual map   src/symbolic/pattern.jl:243-290, src/solve/refinement.jl:67-101
+// Orderings: natural, user permutation, geometric nested dissection of grids
+// (METIS/AMD are not available in this environment) and an approximate minimum
+// degree of our own (src/symbolic/ordering.jl picks AMD.jl/Metis.jl there).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using i64 = int64_t;
+using V = std::vector<i64>;
+
+enum Structure { ST_G = 0, ST_S = 1, ST_H = 2, ST_SPD = 3, ST_HPD = 4 };
+enum View { VW_F = 0, VW_L = 1, VW_U = 2 };
+
+struct Analysis {
+    i64 n = 0, nnz = 0, ns = 0;
+    int structure = 0, view = 0;
+    i64 nnz_stored = 0, nnz_L = 0;
+    double flops = 0.0, flops_true = 0.0;
+    std::map<std::string, V> f;  // named 1-based integer arrays
+    std::string err;
+};
+
+thread_local std::string g_err;
+
+// ----------------------------------------------------------------------------
+// symmetric pattern (0-based adjacency lists, sorted, unique, no diagonal)
+
+bool keep_entry(int structure, int view, i64 r, i64 c) {
+    if (structure == ST_G) return true;
+    if (view == VW_U) return c >= r;
+    return c <= r;  // 'L' and 'F' read the lower triangle
+}
+
+struct Pattern {
+    i64 n;
+    V colptr, rowval;
+    const i64* nb(i64 j) const { return rowval.data() + colptr[j]; }
+    i64 deg(i64 j) const { return colptr[j + 1] - colptr[j]; }
+};
+
+Pattern symmetric_pattern(i64 n, const V& rp, const V& cv, int structure, int view) {
+    V cnt(n + 1, 0);
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            i64 c = cv[p];
+            if (c == r || !keep_entry(structure, view, r, c)) continue;
+            cnt[r + 1]++;
+            cnt[c + 1]++;
+        }
+    for (i64 j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+    V rv(cnt[n]);
+    V next(cnt.begin(), cnt.end() - 1);
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            i64 c = cv[p];
+            if (c == r || !keep_entry(structure, view, r, c)) continue;
+            rv[next[c]++] = r;
+            rv[next[r]++] = c;
+        }
+    Pattern P;
+    P.n = n;
+    P.colptr.assign(n + 1, 0);
+    i64 out = 0;
+    for (i64 j = 0; j < n; ++j) {
+        auto b = rv.begin() + cnt[j], e = rv.begin() + cnt[j + 1];
+        std::sort(b, e);
+        i64 prev = -1;
+        for (auto it = b; it != e; ++it) {
+            if (*it == prev) continue;
+            rv[out++] = *it;
+            prev = *it;
+        }
+        P.colptr[j + 1] = out;
+    }
+    rv.resize(out);
+    P.rowval = std::move(rv);
+    return P;
+}
+
+// ----------------------------------------------------------------------------
+// elimination tree, postorder, column counts (0-based, -1 = root)
+
+V etree(const Pattern& P, const V& perm, const V& iperm) {
+    i64 n = P.n;
+    V parent(n, -1), anc(n, -1);
+    for (i64 k = 0; k < n; ++k) {
+        const i64* nb = P.nb(perm[k]);
+        for (i64 t = 0; t < P.deg(perm[k]); ++t) {
+            i64 i = iperm[nb[t]];
+            if (i >= k) continue;
+            while (i != -1 && i < k) {  // climb with path compression towards k
+                i64 inext = anc[i];
+                anc[i] = k;
+                if (inext == -1) parent[i] = k;
+                i = inext;
+            }
+        }
+    }
+    return parent;
+}
+
+V postorder(const V& parent) {
+    i64 n = (i64)parent.size();
+    V head(n, -1), next(n, -1);
+    for (i64 j = n - 1; j >= 0; --j) {  // reverse insertion: increasing child lists
+        i64 p = parent[j];
+        if (p < 0) continue;
+        next[j] = head[p];
+        head[p] = j;
+    }
+    V post(n);
+    V stack;
+    i64 k = 0;
+    for (i64 root = 0; root < n; ++root) {
+        if (parent[root] >= 0) continue;
+        stack.push_back(root);
+        while (!stack.empty()) {
+            i64 p = stack.back();
+            i64 c = head[p];
+            if (c < 0) {
+                stack.pop_back();
+                post[k++] = p;
+            } else {
+                head[p] = next[c];
+                stack.push_back(c);
+            }
+        }
+    }
+    if (k != n) throw std::runtime_error("parent does not describe a forest");
+    return post;
+}
+
+V colcounts(const Pattern& P, const V& perm, const V& iperm, const V& parent, const V& post) {
+    i64 n = P.n;
+    V delta(n, 0), first(n, -1), maxfirst(n, -1), prevleaf(n, -1), anc(n);
+    std::iota(anc.begin(), anc.end(), 0);
+    for (i64 k = 0; k < n; ++k) {
+        i64 j = post[k];
+        delta[j] = first[j] == -1 ? 1 : 0;
+        while (j != -1 && first[j] == -1) {
+            first[j] = k;
+            j = parent[j];
+        }
+    }
+    for (i64 k = 0; k < n; ++k) {
+        i64 j = post[k];
+        if (parent[j] != -1) delta[parent[j]]--;
+        const i64* nb = P.nb(perm[j]);
+        for (i64 t = 0; t < P.deg(perm[j]); ++t) {
+            i64 i = iperm[nb[t]];
+            // Gilbert-Ng-Peyton leaf test of row subtree i
+            if (i <= j || first[j] <= maxfirst[i]) continue;
+            maxfirst[i] = first[j];
+            i64 jprev = prevleaf[i];
+            prevleaf[i] = j;
+            delta[j]++;
+            if (jprev == -1) continue;  // first leaf of the row subtree
+            i64 q = jprev;
+            while (q != anc[q]) q = anc[q];
+            for (i64 s = jprev; s != q;) {
+                i64 sp = anc[s];
+                anc[s] = q;
+                s = sp;
+            }
+            delta[q]--;  // subsequent leaf: q is the least common ancestor
+        }
+        if (parent[j] != -1) anc[j] = parent[j];
+    }
+    for (i64 k = 0; k < n; ++k) {
+        i64 j = post[k];
+        if (parent[j] != -1) delta[parent[j]] += delta[j];
+    }
+    return delta;
+}
+
+// ----------------------------------------------------------------------------
+// supernodes
+
+struct ColumnPartition {
+    V order;      // supernodal column k = etree column order[k]
+    V super_ptr;  // 0-based, ns + 1
+    V snparent;   // -1 = root
+};
+
+V col2sn_of(const V& sptr) {
+    i64 ns = (i64)sptr.size() - 1;
+    V c2s(sptr.back());
+    for (i64 s = 0; s < ns; ++s)
+        for (i64 j = sptr[s]; j < sptr[s + 1]; ++j) c2s[j] = s;
+    return c2s;
+}
+
+V relabel_parent(const V& parent, const V& order) {
+    i64 n = (i64)order.size();
+    V inv(n);
+    for (i64 k = 0; k < n; ++k) inv[order[k]] = k;
+    V par(n);
+    for (i64 k = 0; k < n; ++k) par[k] = parent[order[k]] < 0 ? -1 : inv[parent[order[k]]];
+    return par;
+}
+
+i64 trapezoid(i64 w, i64 f) { return w * f - (w * (w - 1)) / 2; }
+
+ColumnPartition fundamental_supernodes(const V& parent, const V& post, const V& counts) {
+    i64 n = (i64)parent.size();
+    V par = relabel_parent(parent, post);
+    V cnt(n);
+    for (i64 k = 0; k < n; ++k) cnt[k] = counts[post[k]];
+    V nchild(n, 0);
+    for (i64 k = 0; k < n; ++k)
+        if (par[k] >= 0) nchild[par[k]]++;
+    ColumnPartition cp;
+    cp.order = post;
+    cp.super_ptr.push_back(0);
+    for (i64 k = 1; k < n; ++k) {
+        bool joins = par[k - 1] == k && nchild[k] == 1 && cnt[k - 1] == cnt[k] + 1;
+        if (!joins) cp.super_ptr.push_back(k);
+    }
+    if (n > 0) cp.super_ptr.push_back(n);
+    V c2s = col2sn_of(cp.super_ptr);
+    i64 ns = (i64)cp.super_ptr.size() - 1;
+    cp.snparent.assign(ns, -1);
+    for (i64 s = 0; s < ns; ++s) {
+        i64 pl = par[cp.super_ptr[s + 1] - 1];
+        cp.snparent[s] = pl < 0 ? -1 : c2s[pl];
+    }
+    return cp;
+}
+
+ColumnPartition amalgamate(const ColumnPartition& sn, const V& parent, const V& counts, i64 max_width,
+                           double zero_fraction, i64 min_width) {
+    if (max_width < 1) throw std::runtime_error("amalgamate: max_width must be >= 1");
+    i64 n = (i64)sn.order.size();
+    V cnt(n);
+    for (i64 k = 0; k < n; ++k) cnt[k] = counts[sn.order[k]];
+    const V& uptr = sn.super_ptr;
+    i64 nu = (i64)uptr.size() - 1;
+    for (i64 u = 0; u < nu; ++u)
+        for (i64 j = uptr[u]; j < uptr[u + 1]; ++j)
+            if (cnt[j] != cnt[uptr[u]] - (j - uptr[u]))
+                throw std::runtime_error("amalgamate needs exact supernodes");
+    V par = relabel_parent(parent, sn.order);
+    V c2u = col2sn_of(uptr);
+    V uparent(nu);
+    for (i64 u = 0; u < nu; ++u) {
+        i64 pl = par[uptr[u + 1] - 1];
+        uparent[u] = pl < 0 ? -1 : c2u[pl];
+        if (uparent[u] >= 0 && uparent[u] <= u) throw std::runtime_error("order is not topological");
+    }
+    V width(nu), rows(nu), truennz(nu), zeros(nu, 0), into(nu, -1);
+    i64 total = 0;
+    for (i64 u = 0; u < nu; ++u) {
+        width[u] = uptr[u + 1] - uptr[u];
+        rows[u] = cnt[uptr[u]];
+        truennz[u] = trapezoid(width[u], rows[u]);
+        total += truennz[u];
+    }
+    i64 budget = (i64)std::floor(zero_fraction * (double)total);
+    i64 used = 0;
+    std::vector<V> children(nu);
+    for (i64 u = 0; u < nu; ++u)
+        if (uparent[u] >= 0) children[uparent[u]].push_back(u);
+    for (i64 p = 0; p < nu; ++p) {
+        V& kids = children[p];
+        // most shared rows first, then narrow first, then by index
+        std::sort(kids.begin(), kids.end(), [&](i64 a, i64 b) {
+            i64 ka = -(rows[a] - width[a]), kb = -(rows[b] - width[b]);
+            if (ka != kb) return ka < kb;
+            if (width[a] != width[b]) return width[a] < width[b];
+            return a < b;
+        });
+        for (i64 c : kids) {
+            i64 w = width[c] + width[p];
+            if (w > max_width) continue;
+            i64 added = width[c] * (width[c] + rows[p] - rows[c]);
+            if (used + added > budget) continue;
+            i64 z = zeros[c] + zeros[p] + added;
+            i64 t = truennz[c] + truennz[p];
+            if (!((double)z <= zero_fraction * (double)t || w <= min_width)) continue;
+            into[c] = p;
+            used += added;
+            width[p] = w;
+            rows[p] += width[c];
+            zeros[p] = z;
+            truennz[p] = t;
+        }
+    }
+    V rep(nu);
+    std::iota(rep.begin(), rep.end(), 0);
+    for (i64 u = nu - 1; u >= 0; --u)
+        if (into[u] >= 0) rep[u] = rep[into[u]];
+    V ids(nu, -1);
+    i64 ns = 0;
+    for (i64 u = 0; u < nu; ++u)
+        if (into[u] < 0) ids[u] = ns++;
+    V gparent(ns, -1);
+    std::vector<V> members(ns);
+    for (i64 u = 0; u < nu; ++u) {
+        members[ids[rep[u]]].push_back(u);
+        if (into[u] < 0 && uparent[u] >= 0) gparent[ids[u]] = ids[rep[uparent[u]]];
+    }
+    V gpost = postorder(gparent);
+    ColumnPartition cp;
+    V neworder;
+    neworder.reserve(n);
+    cp.super_ptr.push_back(0);
+    for (i64 g : gpost) {
+        for (i64 u : members[g])
+            for (i64 j = uptr[u]; j < uptr[u + 1]; ++j) neworder.push_back(j);
+        cp.super_ptr.push_back((i64)neworder.size());
+    }
+    V gpos(ns);
+    for (i64 k = 0; k < ns; ++k) gpos[gpost[k]] = k;
+    cp.snparent.resize(ns);
+    for (i64 k = 0; k < ns; ++k) {
+        i64 g = gpost[k];
+        cp.snparent[k] = gparent[g] < 0 ? -1 : gpos[gparent[g]];
+    }
+    cp.order.resize(n);
+    for (i64 k = 0; k < n; ++k) cp.order[k] = sn.order[neworder[k]];
+    return cp;
+}
+
+// ----------------------------------------------------------------------------
+// geometric nested dissection of an nx x ny x nz grid (natural row-major
+// numbering x + nx (y + ny z)): the middle plane of the longest side is the
+// separator, ordered after both halves; boxes of at most `leaf` points are
+// numbered naturally
+
+void nd_box(i64 nx, i64 ny, i64 x0, i64 x1, i64 y0, i64 y1, i64 z0, i64 z1, i64 leaf, V& out) {
+    i64 lx = x1 - x0, ly = y1 - y0, lz = z1 - z0;
+    if (lx <= 0 || ly <= 0 || lz <= 0) return;
+    if (lx * ly * lz <= leaf || (lx <= 2 && ly <= 2 && lz <= 2)) {
+        for (i64 z = z0; z < z1; ++z)
+            for (i64 y = y0; y < y1; ++y)
+                for (i64 x = x0; x < x1; ++x) out.push_back(x + nx * (y + ny * z));
+        return;
+    }
+    if (lx >= ly && lx >= lz) {
+        i64 m = x0 + lx / 2;
+        nd_box(nx, ny, x0, m, y0, y1, z0, z1, leaf, out);
+        nd_box(nx, ny, m + 1, x1, y0, y1, z0, z1, leaf, out);
+        nd_box(nx, ny, m, m + 1, y0, y1, z0, z1, 1 << 30, out);
+    } else if (ly >= lz) {
+        i64 m = y0 + ly / 2;
+        nd_box(nx, ny, x0, x1, y0, m, z0, z1, leaf, out);
+        nd_box(nx, ny, x0, x1, m + 1, y1, z0, z1, leaf, out);
+        nd_box(nx, ny, x0, x1, m, m + 1, z0, z1, 1 << 30, out);
+    } else {
+        i64 m = z0 + lz / 2;
+        nd_box(nx, ny, x0, x1, y0, y1, z0, m, leaf, out);
+        nd_box(nx, ny, x0, x1, y0, y1, m + 1, z1, leaf, out);
+        nd_box(nx, ny, x0, x1, y0, y1, m, m + 1, 1 << 30, out);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// approximate minimum degree on the quotient graph (element absorption,
+// approximate external degrees, mass elimination of indistinguishable nodes
+// by hashing). A compact implementation of the method of Amestoy, Davis and
+// Duff (SIAM J. Matrix Anal. Appl. 17, 1996); no dense-row detection and no
+// aggressive absorption.
+
+V amd_order(const Pattern& P) {
+    const i64 n = P.n;
+    if (n == 0) return V();
+    // element/variable adjacency: for variable i, the list of elements (first
+    // elen[i] entries) followed by the variables it is adjacent to
+    std::vector<V> adj(n);
+    for (i64 j = 0; j < n; ++j) adj[j].assign(P.nb(j), P.nb(j) + P.deg(j));
+    V elen(n, 0);           // number of elements in adj[i]
+    V nv(n, 1);             // supervariable size (0 = absorbed into another)
+    V degree(n);            // approximate external degree
+    std::vector<char> status(n, 0);  // 0 variable, 1 element, 2 absorbed/dead
+    std::vector<V> lelem(n);          // variables of element e (live members)
+    V w(n, 0);
+    i64 wflg = 1;
+    for (i64 i = 0; i < n; ++i) degree[i] = P.deg(i);
+    // degree buckets
+    V head(n + 1, -1), next(n, -1), prev(n, -1);
+    auto bucket_insert = [&](i64 i) {
+        i64 d = std::min(degree[i], n);
+        next[i] = head[d];
+        prev[i] = -1;
+        if (head[d] >= 0) prev[head[d]] = i;
+        head[d] = i;
+    };
+    auto bucket_remove = [&](i64 i) {
+        i64 d = std::min(degree[i], n);
+        if (prev[i] >= 0) next[prev[i]] = next[i];
+        else head[d] = next[i];
+        if (next[i] >= 0) prev[next[i]] = prev[i];
+        next[i] = prev[i] = -1;
+    };
+    for (i64 i = 0; i < n; ++i) bucket_insert(i);
+    V order;
+    order.reserve(n);
+    std::vector<V> merged(n);  // variables merged into supervariable i
+    i64 mindeg = 0, nel = 0;
+    V Lme;
+    auto bump = [&]() {
+        if (++wflg > (i64)1 << 60) {
+            std::fill(w.begin(), w.end(), 0);
+            wflg = 1;
+        }
+    };
+    while (nel < n) {
+        while (mindeg <= n && head[mindeg] < 0) ++mindeg;
+        i64 me = head[mindeg];
+        bucket_remove(me);
+        // the new element me: union of its variables and its elements' variables
+        bump();
+        w[me] = wflg;
+        Lme.clear();
+        for (i64 t = 0; t < (i64)adj[me].size(); ++t) {
+            i64 e = adj[me][t];
+            if (t < elen[me]) {
+                if (status[e] != 1) continue;
+                for (i64 v : lelem[e])
+                    if (status[v] == 0 && nv[v] > 0 && w[v] != wflg) {
+                        w[v] = wflg;
+                        Lme.push_back(v);
+                    }
+                status[e] = 2;  // absorbed into me
+                lelem[e].clear();
+            } else if (status[e] == 0 && nv[e] > 0 && w[e] != wflg) {
+                w[e] = wflg;
+                Lme.push_back(e);
+            }
+        }
+        status[me] = 1;
+        order.push_back(me);
+        nel += nv[me];
+        for (i64 v : merged[me]) order.push_back(v);
+        merged[me].clear();
+        adj[me].clear();
+        elen[me] = 0;
+        lelem[me] = Lme;
+        // update the variables of me: prune absorbed elements and me's members
+        for (i64 v : Lme) bucket_remove(v);
+        // |Le \ Lme| for every element e adjacent to a variable of Lme
+        bump();
+        i64 tag = wflg;
+        for (i64 v : Lme) w[v] = tag;  // mark Lme
+        std::vector<std::pair<i64, i64>> ext;  // not needed beyond this scope
+        std::map<i64, i64> we;                  // element -> |Le \ Lme|
+        for (i64 v : Lme)
+            for (i64 t = 0; t < elen[v]; ++t) {
+                i64 e = adj[v][t];
+                if (status[e] != 1 || e == me) continue;
+                auto it = we.find(e);
+                if (it == we.end()) {
+                    i64 c = 0;
+                    for (i64 u : lelem[e])
+                        if (status[u] == 0 && nv[u] > 0) c += nv[u];
+                    it = we.emplace(e, c).first;
+                }
+                it->second -= nv[v];
+            }
+        i64 lme_size = 0;
+        for (i64 v : Lme) lme_size += nv[v];
+        for (i64 v : Lme) {
+            // rebuild adjacency: elements (me first), then variables not in Lme
+            V nadj;
+            nadj.push_back(me);
+            i64 deg_e = 0;
+            for (i64 t = 0; t < elen[v]; ++t) {
+                i64 e = adj[v][t];
+                if (status[e] != 1 || e == me) continue;
+                i64 ext_e = we.count(e) ? we[e] : 0;
+                if (ext_e <= 0) {  // element e is covered by me: absorb
+                    status[e] = 2;
+                    lelem[e].clear();
+                    continue;
+                }
+                nadj.push_back(e);
+                deg_e += ext_e;
+            }
+            i64 ne = (i64)nadj.size();
+            i64 deg_v = 0;
+            for (i64 t = elen[v]; t < (i64)adj[v].size(); ++t) {
+                i64 u = adj[v][t];
+                if (status[u] != 0 || nv[u] == 0 || w[u] == tag || u == me) continue;
+                nadj.push_back(u);
+                deg_v += nv[u];
+            }
+            adj[v] = std::move(nadj);
+            elen[v] = ne;
+            i64 d = std::min(degree[v], (lme_size - nv[v]) + deg_e + deg_v);
+            d = std::min(d, (n - nel) - nv[v]);
+            degree[v] = std::max<i64>(d, 0);
+        }
+        // supervariable detection among Lme by hashing the adjacency lists
+        std::map<std::pair<i64, i64>, V> hbuckets;
+        for (i64 v : Lme) {
+            if (nv[v] == 0) continue;
+            V a = adj[v];
+            std::sort(a.begin() + 1, a.begin() + elen[v]);
+            std::sort(a.begin() + elen[v], a.end());
+            adj[v] = a;
+            i64 h = 0;
+            for (i64 x : a) h = h * 1000003 + x;
+            hbuckets[{(i64)a.size(), h}].push_back(v);
+        }
+        for (auto& kv : hbuckets) {
+            V& vs = kv.second;
+            for (size_t a = 0; a < vs.size(); ++a) {
+                i64 i = vs[a];
+                if (nv[i] == 0) continue;
+                for (size_t b = a + 1; b < vs.size(); ++b) {
+                    i64 j = vs[b];
+                    if (nv[j] == 0 || elen[i] != elen[j] || adj[i] != adj[j]) continue;
+                    nv[i] += nv[j];
+                    nv[j] = 0;
+                    status[j] = 2;
+                    merged[i].push_back(j);
+                    for (i64 x : merged[j]) merged[i].push_back(x);
+                    merged[j].clear();
+                    adj[j].clear();
+                    elen[j] = 0;
+                }
+            }
+        }
+        V live;
+        for (i64 v : Lme)
+            if (nv[v] > 0) {
+                degree[v] = std::max<i64>(0, std::min(degree[v], (n - nel) - nv[v]));
+                live.push_back(v);
+                bucket_insert(v);
+                mindeg = std::min(mindeg, std::min(degree[v], n));
+            }
+        lelem[me] = live;
+    }
+    return order;
+}
+
+// ----------------------------------------------------------------------------
+
+Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structure, int view, int index_base,
+                  const i64* perm_in, int ordering, i64 max_width, double zero_fraction, i64 min_width,
+                  int use_superpanels) {
+    auto A = new Analysis();
+    A->n = n;
+    A->structure = structure;
+    A->view = view;
+    i64 base = index_base;  // 0 or 1
+    V rp(n + 1), cv;
+    for (i64 i = 0; i <= n; ++i) rp[i] = rowptr_in[i] - base;
+    if (rp[0] != 0) throw std::runtime_error("rowptr must start at the index base");
+    for (i64 r = 0; r < n; ++r)
+        if (rp[r] > rp[r + 1]) throw std::runtime_error("rowptr is not nondecreasing");
+    i64 nnz = rp[n];
+    A->nnz = nnz;
+    cv.resize(nnz);
+    for (i64 p = 0; p < nnz; ++p) {
+        cv[p] = colval_in[p] - base;
+        if (cv[p] < 0 || cv[p] >= n) throw std::runtime_error("column index out of range");
+    }
+    Pattern P = symmetric_pattern(n, rp, cv, structure, view);
+
+    // ordering: etree numbering column k = original perm0[k]
+    V perm0(n);
+    if (perm_in) {
+        std::vector<char> seen(n, 0);
+        for (i64 k = 0; k < n; ++k) {
+            perm0[k] = perm_in[k] - 1;
+            if (perm0[k] < 0 || perm0[k] >= n || seen[perm0[k]]) throw std::runtime_error("user_perm is not a permutation");
+            seen[perm0[k]] = 1;
+        }
+    } else if (ordering == 3) {
+        perm0 = amd_order(P);
+    } else {
+        std::iota(perm0.begin(), perm0.end(), 0);
+    }
+    V iperm0(n);
+    for (i64 k = 0; k < n; ++k) iperm0[perm0[k]] = k;
+
+    V parent = etree(P, perm0, iperm0);
+    V post = postorder(parent);
+    V counts = colcounts(P, perm0, iperm0, parent, post);
+    ColumnPartition cp = fundamental_supernodes(parent, post, counts);
+    if (use_superpanels) cp = amalgamate(cp, parent, counts, max_width, zero_fraction, min_width);
+
+    // supernodal symbolic factorization
+    V fperm(n), fiperm(n);
+    for (i64 k = 0; k < n; ++k) fperm[k] = perm0[cp.order[k]];
+    for (i64 k = 0; k < n; ++k) fiperm[fperm[k]] = k;
+    V par = relabel_parent(parent, cp.order);
+    V cnt(n);
+    for (i64 k = 0; k < n; ++k) cnt[k] = counts[cp.order[k]];
+    i64 ns = (i64)cp.super_ptr.size() - 1;
+    A->ns = ns;
+    const V& sptr = cp.super_ptr;
+    V col2sn = col2sn_of(sptr);
+    const V& snparent = cp.snparent;
+    std::vector<V> children(ns);
+    for (i64 s = 0; s < ns; ++s)
+        if (snparent[s] >= 0) children[snparent[s]].push_back(s);
+    V snrowptr(ns + 1, 0), snrowval;
+    V mark(n, -1), below;
+    i64 nnz_stored = 0;
+    double flops = 0;
+    for (i64 s = 0; s < ns; ++s) {
+        i64 first = sptr[s], last = sptr[s + 1] - 1;
+        below.clear();
+        for (i64 j = first; j <= last; ++j) {
+            const i64* nb = P.nb(fperm[j]);
+            for (i64 t = 0; t < P.deg(fperm[j]); ++t) {
+                i64 i = fiperm[nb[t]];
+                if (i > last && mark[i] != s) {
+                    mark[i] = s;
+                    below.push_back(i);
+                }
+            }
+        }
+        for (i64 c : children[s])
+            for (i64 q = snrowptr[c]; q < snrowptr[c + 1]; ++q) {
+                i64 i = snrowval[q];
+                if (i > last && mark[i] != s) {
+                    mark[i] = s;
+                    below.push_back(i);
+                }
+            }
+        std::sort(below.begin(), below.end());
+        for (i64 j = first; j <= last; ++j) snrowval.push_back(j);
+        for (i64 i : below) snrowval.push_back(i);
+        snrowptr[s + 1] = (i64)snrowval.size();
+        i64 w = last - first + 1, f = w + (i64)below.size();
+        nnz_stored += trapezoid(w, f);
+        for (i64 k = 0; k < w; ++k) flops += (double)(f - k) * (double)(f - k);
+    }
+    V snpost = postorder(snparent);
+    A->nnz_stored = nnz_stored;
+    A->flops = flops;
+    i64 nnzl = 0;
+    double ft = 0;
+    for (i64 k = 0; k < n; ++k) {
+        nnzl += cnt[k];
+        ft += (double)cnt[k] * (double)cnt[k];
+    }
+    A->nnz_L = nnzl;
+    A->flops_true = ft;
+
+    // layout: panel_ptr (1-based element offsets)
+    V frows(ns), fcols(ns), panel_ptr(ns + 1);
+    panel_ptr[0] = 1;
+    for (i64 s = 0; s < ns; ++s) {
+        fcols[s] = sptr[s + 1] - sptr[s];
+        frows[s] = snrowptr[s + 1] - snrowptr[s];
+        panel_ptr[s + 1] = panel_ptr[s] + frows[s] * fcols[s];
+    }
+
+    // relind: positions (1-based) of a child's rows below its width among its parent's rows
+    V relind_ptr(ns + 1), relind;
+    relind_ptr[0] = 1;
+    for (i64 c = 0; c < ns; ++c) {
+        i64 p = snparent[c];
+        if (p >= 0) {
+            i64 k = snrowptr[p], kend = snrowptr[p + 1];
+            for (i64 q = snrowptr[c] + fcols[c]; q < snrowptr[c + 1]; ++q) {
+                i64 i = snrowval[q];
+                while (k < kend && snrowval[k] < i) ++k;
+                if (k >= kend || snrowval[k] != i) throw std::runtime_error("child row missing from its parent");
+                relind.push_back(k - snrowptr[p] + 1);
+            }
+        }
+        relind_ptr[c + 1] = (i64)relind.size() + 1;
+    }
+
+    // amap: factor offset of every stored entry; negative = conjugate / U^T structure
+    bool neg = structure == ST_H || structure == ST_HPD || structure == ST_G;
+    V amap(nnz, 0);
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            i64 c = cv[p];
+            if (!keep_entry(structure, view, r, c)) continue;
+            i64 i = fiperm[r], j = fiperm[c];
+            bool flip = i < j;
+            if (flip) std::swap(i, j);
+            i64 s = col2sn[j];
+            auto b = snrowval.begin() + snrowptr[s], e = snrowval.begin() + snrowptr[s + 1];
+            auto it = std::lower_bound(b, e, i);
+            if (it == e || *it != i) throw std::runtime_error("entry outside the symbolic factor");
+            i64 pos = it - b;
+            i64 off = panel_ptr[s] + (j - sptr[s]) * frows[s] + pos;
+            amap[p] = (flip && neg) ? -off : off;
+        }
+    // owner-pull grouping: sorted by destination (L before U^T for "G"), then position
+    bool signed_ = structure == ST_G;
+    V order;
+    order.reserve(nnz);
+    for (i64 p = 0; p < nnz; ++p)
+        if (amap[p] != 0) order.push_back(p);
+    std::stable_sort(order.begin(), order.end(), [&](i64 a, i64 b) {
+        i64 oa = std::llabs(amap[a]), ob = std::llabs(amap[b]);
+        if (oa != ob) return oa < ob;
+        bool na = signed_ && amap[a] < 0, nb_ = signed_ && amap[b] < 0;
+        if (na != nb_) return !na;
+        return a < b;
+    });
+    V amap_ptr(ns + 1, 0);
+    for (i64 p : order) {
+        i64 off = std::llabs(amap[p]);
+        i64 owner = std::upper_bound(panel_ptr.begin(), panel_ptr.end(), off) - panel_ptr.begin() - 1;
+        amap_ptr[owner + 1]++;
+    }
+    amap_ptr[0] = 1;
+    for (i64 s = 0; s < ns; ++s) amap_ptr[s + 1] += amap_ptr[s];
+
+    // children in increasing id
+    V child_ptr(ns + 1, 0), child_list;
+    for (i64 s = 0; s < ns; ++s)
+        if (snparent[s] >= 0) child_ptr[snparent[s] + 1]++;
+    child_ptr[0] = 1;
+    for (i64 s = 0; s < ns; ++s) child_ptr[s + 1] += child_ptr[s];
+    child_list.resize(child_ptr[ns] - 1);
+    {
+        V nxt(child_ptr.begin(), child_ptr.end() - 1);
+        for (i64 s = 0; s < ns; ++s)
+            if (snparent[s] >= 0) child_list[(nxt[snparent[s]]++) - 1] = s + 1;
+    }
+
+    // full-pattern residual map of the stored matrix: row-major, (row, col, src) order,
+    // src negative for a conjugated (mirrored Hermitian) contribution; duplicates kept.
+    // For "G" the rows of the transpose follow as rows n+1..2n.
+    struct Contrib { i64 r, c, p; bool cj; };
+    std::vector<Contrib> contrib;
+    bool herm = structure == ST_H || structure == ST_HPD;
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            i64 c = cv[p];
+            if (!keep_entry(structure, view, r, c)) continue;
+            contrib.push_back({r, c, p, false});
+            if (structure != ST_G && r != c) contrib.push_back({c, r, p, herm});
+        }
+    std::sort(contrib.begin(), contrib.end(), [](const Contrib& a, const Contrib& b) {
+        if (a.r != b.r) return a.r < b.r;
+        if (a.c != b.c) return a.c < b.c;
+        return a.p < b.p;
+    });
+    i64 nrows_map = structure == ST_G ? 2 * n : n;
+    V full_rowptr(nrows_map + 1, 0), full_colval, full_src;
+    for (auto& t : contrib) {
+        full_rowptr[t.r + 1]++;
+        full_colval.push_back(t.c + 1);
+        full_src.push_back(t.cj ? -(t.p + 1) : (t.p + 1));
+    }
+    if (structure == ST_G) {
+        // rows of M^T: the entries (i, j) of M by column j, rows i in order
+        std::vector<Contrib> tr(contrib);
+        std::stable_sort(tr.begin(), tr.end(), [](const Contrib& a, const Contrib& b) { return a.c < b.c; });
+        for (auto& t : tr) {
+            full_rowptr[n + t.c + 1]++;
+            full_colval.push_back(t.r + 1);
+            full_src.push_back(t.cj ? -(t.p + 1) : (t.p + 1));
+        }
+    }
+    full_rowptr[0] = 1;
+    for (i64 i = 0; i < nrows_map; ++i) full_rowptr[i + 1] += full_rowptr[i];
+
+    // 1-based outputs
+    auto one = [](const V& v) {
+        V o(v.size());
+        for (size_t k = 0; k < v.size(); ++k) o[k] = v[k] + 1;
+        return o;
+    };
+    auto& F = A->f;
+    F["perm"] = one(fperm);
+    F["iperm"] = one(fiperm);
+    F["parent"] = V(par.size());
+    for (size_t k = 0; k < par.size(); ++k) F["parent"][k] = par[k] + 1;  // 0 = root
+    F["counts"] = cnt;
+    F["super_ptr"] = one(sptr);
+    F["col2sn"] = one(col2sn);
+    F["snparent"] = V(ns);
+    for (i64 s = 0; s < ns; ++s) F["snparent"][s] = snparent[s] + 1;
+    F["snpost"] = one(snpost);
+    F["rowptr"] = one(snrowptr);
+    F["rowval"] = one(snrowval);
+    F["front_ptr"] = panel_ptr;
+    F["front_nrows"] = frows;
+    F["front_ncols"] = fcols;
+    F["child_ptr"] = child_ptr;
+    F["child_list"] = child_list;
+    F["relind_ptr"] = relind_ptr;
+    F["relind"] = relind;
+    F["amap"] = amap;
+    F["amap_ptr"] = amap_ptr;
+    F["amap_src"] = one(order);
+    F["full_rowptr"] = full_rowptr;
+    F["full_colval"] = full_colval;
+    F["full_src"] = full_src;
+    F["etree_perm"] = one(perm0);
+    return A;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Run the analysis. ordering: 0 natural, 1 user permutation (perm must be
+// given), 3 approximate minimum degree. Returns NULL on error
+// (sdsa_last_error() holds the message).
+void* sdsa_analyze(int64_t n, const int64_t* rowptr, const int64_t* colval, int structure, int view,
+                   int index_base, const int64_t* perm, int ordering, int64_t max_width, double zero_fraction,
+                   int64_t min_width, int use_superpanels) {
+    try {
+        return analyze(n, rowptr, colval, structure, view, index_base, perm, ordering, max_width, zero_fraction,
+                       min_width, use_superpanels);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+const char* sdsa_last_error(void) { return g_err.c_str(); }
+
+void sdsa_free(void* h) { delete static_cast<Analysis*>(h); }
+
+// length of a named array, -1 if unknown
+int64_t sdsa_field_len(void* h, const char* name) {
+    auto A = static_cast<Analysis*>(h);
+    auto it = A->f.find(name);
+    return it == A->f.end() ? -1 : (int64_t)it->second.size();
+}
+
+int sdsa_field_copy(void* h, const char* name, int64_t* dst) {
+    auto A = static_cast<Analysis*>(h);
+    auto it = A->f.find(name);
+    if (it == A->f.end()) return -1;
+    std::memcpy(dst, it->second.data(), it->second.size() * sizeof(int64_t));
+    return 0;
+}
+
+// scalars: 0 n, 1 nnz, 2 ns, 3 nnz_stored, 4 nnz_L, 5 stored-panel flops, 6 true flops (sum c_j^2)
+double sdsa_scalar(void* h, int which) {
+    auto A = static_cast<Analysis*>(h);
+    switch (which) {
+        case 0: return (double)A->n;
+        case 1: return (double)A->nnz;
+        case 2: return (double)A->ns;
+        case 3: return (double)A->nnz_stored;
+        case 4: return (double)A->nnz_L;
+        case 5: return A->flops;
+        case 6: return A->flops_true;
+    }
+    return -1;
+}
+
+// geometric nested dissection of an nx x ny x nz grid; perm (1-based) of length nx ny nz
+int sdsa_grid_nd(int64_t nx, int64_t ny, int64_t nz, int64_t leaf, int64_t* perm) {
+    V out;
+    out.reserve(nx * ny * nz);
+    nd_box(nx, ny, 0, nx, 0, ny, 0, nz, leaf < 1 ? 1 : leaf, out);
+    if ((int64_t)out.size() != nx * ny * nz) return -1;
+    for (size_t k = 0; k < out.size(); ++k) perm[k] = out[k] + 1;
+    return 0;
+}
+
+}  // extern "C"

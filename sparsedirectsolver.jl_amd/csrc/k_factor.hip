@@ -1,0 +1,398 @@
+// Numeric factorization kernels (gfx950). Each front writes only its own panel
+// and contribution block; the extend-add is owner-pull (children in child_list
+// order) and never uses atomics, so a factorization is bitwise reproducible.
+//
+// Small fronts (the whole f x f front fits the LDS of one workgroup) are
+// assembled and factored by one fused kernel, one workgroup per front, all the
+// small fronts of a tree level in one launch. Large fronts are assembled in
+// global memory (panel in the factor buffer, contribution block in the CB pool)
+// by column-chunk tasks, then factored block column by block column (NB wide):
+// diagonal block in LDS, row-tile triangular solves, and Schur updates as 64x64
+// tiles on the matrix cores; the contribution block gets one rank-w update with
+// K = w after the panel is done.
+//
+// Reference counterparts: regime B fused kernel src/numeric/front.jl:154-206,
+// assembly src/numeric/assembly.jl:18-252, regime C panel src/numeric/factorize.jl:240-269,
+// statistics src/numeric/factorize.jl:13-48.
+#include "sds_common.h"
+#include "sds_kernels.h"
+
+namespace sds {
+
+// Cholesky pivot acceptance on the FP64 value of the pivot (src/numeric/front.jl:46-50):
+// a pivot in [0, eps] is raised to eps (cuDSS pivot_epsilon), negative or NaN fails.
+__device__ inline bool chol_accept(double a, double eps) { return a > eps || (eps > 0.0 && a >= 0.0); }
+
+// ---------------------------------------------------------------------------
+// small fronts: fused assembly + dense Cholesky of the whole front in LDS
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_small_chol(DevSym S, const int32_t* __restrict__ fronts,
+                                                         const double* __restrict__ nz, T* __restrict__ factor,
+                                                         T* __restrict__ cbpool, int32_t* __restrict__ status,
+                                                         double eps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
+    __shared__ int32_t st;
+    const int s = fronts[blockIdx.x];
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < f * f; q += nt) F[q] = T(0);
+    if (tid == 0) st = 0;
+    __syncthreads();
+    // A: one thread sums each run of entries with the same destination, in nzval order
+    const int64_t a0 = S.asm_ptr[s], a1 = S.asm_ptr[s + 1];
+    for (int64_t k = a0 + tid; k < a1; k += nt) {
+        const int64_t d = S.asm_dst[k];
+        if (k > a0 && S.asm_dst[k - 1] == d) continue;
+        double v = 0.0;
+        for (int64_t q = k; q < a1 && S.asm_dst[q] == d; ++q) v += nz[S.asm_src[q]];
+        const int64_t loc = d >= 0 ? d : -(d + 1);
+        F[loc] += T(v);
+    }
+    __syncthreads();
+    // extend-add of the children's lower-triangular contribution blocks, in order
+    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
+        const int c = S.child_list[k];
+        const int64_t co = S.cb_off[c];
+        if (co < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* C = cbpool + co;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        for (int q = tid; q < mc * mc; q += nt) {
+            const int jj = q / mc, ii = q - jj * mc;
+            if (ii < jj) continue;
+            F[ri[jj] * f + ri[ii]] += C[q];
+        }
+        __syncthreads();
+    }
+    // right-looking Cholesky of the w pivot columns over the whole front
+    for (int j = 0; j < w; ++j) {
+        if (tid == 0) {
+            const double a = (double)F[j * f + j];
+            if (chol_accept(a, eps))
+                F[j * f + j] = T(sqrt(fmax(a, eps)));
+            else
+                st = j + 1;
+        }
+        __syncthreads();
+        if (st) break;
+        const T d = F[j * f + j];
+        for (int i = j + 1 + tid; i < f; i += nt) F[j * f + i] /= d;
+        __syncthreads();
+        const int r = f - j - 1;
+        for (int q = tid; q < r * r; q += nt) {
+            const int kk = q / r, ii = q - kk * r;
+            if (ii < kk) continue;
+            F[(j + 1 + kk) * f + j + 1 + ii] -= F[j * f + j + 1 + ii] * F[j * f + j + 1 + kk];
+        }
+        __syncthreads();
+    }
+    T* P = factor + S.panel_off[s];
+    for (int q = tid; q < f * w; q += nt) {
+        const int j = q / f, i = q - j * f;
+        P[q] = i >= j ? F[q] : T(0);
+    }
+    const int64_t co = S.cb_off[s];
+    if (co >= 0) {
+        T* C = cbpool + co;
+        for (int q = tid; q < m * m; q += nt) {
+            const int jj = q / m, ii = q - jj * m;
+            if (ii >= jj) C[q] = F[(w + jj) * f + w + ii];
+        }
+    }
+    if (tid == 0) status[s] = st;
+}
+
+// ---------------------------------------------------------------------------
+// large fronts
+
+// address of front entry (i, j), i >= j: panel column or contribution block
+template <typename T>
+__device__ inline T* front_at(T* P, T* C, int f, int w, int i, int j) {
+    return j < w ? P + (int64_t)j * f + i : C + (int64_t)(j - w) * (f - w) + (i - w);
+}
+
+// zero a chunk of columns of a large front and add A into its panel columns
+// task: (s, col0, ncols)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* __restrict__ tasks,
+                                                         const double* __restrict__ nz, T* __restrict__ factor,
+                                                         T* __restrict__ cbpool) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    T* P = factor + S.panel_off[s];
+    T* C = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int j1 = min(t.a + t.b, f);
+    for (int j = t.a; j < j1; ++j) {
+        if (j < w) {
+            for (int i = tid; i < f; i += nt) P[(int64_t)j * f + i] = T(0);
+        } else {
+            for (int i = tid; i < m; i += nt) C[(int64_t)(j - w) * m + i] = T(0);
+        }
+    }
+    __syncthreads();
+    const int c0 = S.super_ptr[s];
+    for (int j = t.a; j < min(j1, w); ++j) {
+        const int64_t a0 = S.acol_ptr[c0 + j], a1 = S.acol_ptr[c0 + j + 1];
+        for (int64_t k = a0 + tid; k < a1; k += nt) {
+            const int64_t d = S.asm_dst[k];
+            if (k > a0 && S.asm_dst[k - 1] == d) continue;
+            double v = 0.0;
+            for (int64_t q = k; q < a1 && S.asm_dst[q] == d; ++q) v += nz[S.asm_src[q]];
+            const int64_t loc = d >= 0 ? d : -(d + 1);
+            P[loc] += T(v);
+        }
+    }
+}
+
+// extend-add of one child per front (the launch covers child rank r of every front);
+// task: (s, child, jj0, njj)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __restrict__ tasks,
+                                                       T* __restrict__ factor, T* __restrict__ cbpool) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    const int c = t.a, mc = S.fnrows[c] - S.fncols[c];
+    T* P = factor + S.panel_off[s];
+    T* Cs = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
+    const T* Cc = cbpool + S.cb_off[c];
+    const int32_t* ri = S.relind + S.relind_ptr[c];
+    const int jend = min(t.b + t.c, mc);
+    for (int jj = t.b; jj < jend; ++jj) {
+        const int pj = ri[jj];
+        for (int ii = jj + threadIdx.x; ii < mc; ii += blockDim.x)
+            *front_at(P, Cs, f, w, ri[ii], pj) += Cc[(int64_t)jj * mc + ii];
+    }
+}
+
+// Cholesky of the diagonal block of block column k (nb = min(NB, w - k)); task: (s, k)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_potrf(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                  int32_t* __restrict__ status, double eps) {
+    __shared__ T D[NB][NB + 1];
+    __shared__ int32_t st;
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a;
+    if (status[s] != 0) return;
+    const int nb = min(NB, w - k);
+    T* P = factor + S.panel_off[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < nb * nb; q += nt) {
+        const int j = q / nb, i = q - j * nb;
+        D[j][i] = P[(int64_t)(k + j) * f + k + i];
+    }
+    if (tid == 0) st = 0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (tid == 0) {
+            const double a = (double)D[j][j];
+            if (chol_accept(a, eps))
+                D[j][j] = T(sqrt(fmax(a, eps)));
+            else
+                st = k + j + 1;
+        }
+        __syncthreads();
+        if (st) break;
+        const T d = D[j][j];
+        for (int i = j + 1 + tid; i < nb; i += nt) D[j][i] /= d;
+        __syncthreads();
+        const int r = nb - j - 1;
+        for (int q = tid; q < r * r; q += nt) {
+            const int kk = q / r, ii = q - kk * r;
+            if (ii < kk) continue;
+            D[j + 1 + kk][j + 1 + ii] -= D[j][j + 1 + ii] * D[j][j + 1 + kk];
+        }
+        __syncthreads();
+    }
+    for (int q = tid; q < nb * nb; q += nt) {
+        const int j = q / nb, i = q - j * nb;
+        if (i >= j) P[(int64_t)(k + j) * f + k + i] = D[j][i];
+    }
+    if (tid == 0 && st) status[s] = st;
+}
+
+// rows [row0, row0 + 64) of block column k: X <- X L11^-T; task: (s, k, row0); 64 threads
+template <typename T>
+__global__ void __launch_bounds__(64) k_big_trsm(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                const int32_t* __restrict__ status) {
+    __shared__ T L[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];  // X[col][row]
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
+    if (status[s] != 0) return;
+    const int nb = min(NB, w - k);
+    const int nr = min(64, f - row0);
+    T* P = factor + S.panel_off[s];
+    const int tid = threadIdx.x;
+    for (int j = 0; j < nb; ++j) {
+        L[j][tid] = tid < nb ? P[(int64_t)(k + j) * f + k + tid] : T(0);
+        X[j][tid] = tid < nr ? P[(int64_t)(k + j) * f + row0 + tid] : T(0);
+    }
+    __syncthreads();
+    if (tid < nr) {
+        for (int kk = 0; kk < nb; ++kk) {
+            T v = X[kk][tid];
+            for (int jj = 0; jj < kk; ++jj) v -= X[jj][tid] * L[jj][kk];
+            X[kk][tid] = v / L[kk][kk];
+        }
+    }
+    __syncthreads();
+    if (tid < nr)
+        for (int j = 0; j < nb; ++j) P[(int64_t)(k + j) * f + row0 + tid] = X[j][tid];
+}
+
+// Schur update of one 64x64 tile: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T on the lower
+// triangle of the front, A the panel columns k0..k1-1. mode 0: step update of the
+// panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
+// task: (s, I0, J0, k0)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_update(DevSym S, const Task* __restrict__ tasks,
+                                                   T* __restrict__ factor, T* __restrict__ cbpool,
+                                                   const int32_t* __restrict__ status, int mode) {
+    constexpr int KC = 16;
+    __shared__ T As[KC][TILE + 4];
+    __shared__ T Bs[KC][TILE + 4];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    if (status[s] != 0) return;
+    const int I0 = t.a, J0 = t.b;
+    const int k0 = mode == 0 ? t.c : 0;
+    const int k1 = mode == 0 ? min(t.c + NB, w) : w;
+    const int jlim = mode == 0 ? w : f;
+    T* P = factor + S.panel_off[s];
+    T* C = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int kc = k0; kc < k1; kc += KC) {
+        for (int q = tid; q < KC * TILE; q += 256) {
+            const int kk = q / TILE, r = q - kk * TILE;
+            const int kcol = kc + kk;
+            const bool kin = kcol < k1;
+            As[kk][r] = (kin && I0 + r < f) ? P[(int64_t)kcol * f + I0 + r] : T(0);
+            Bs[kk][r] = (kin && J0 + r < f) ? P[(int64_t)kcol * f + J0 + r] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            T av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = I0 + ty + 16 * a;
+        if (i >= f) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = J0 + tx + 16 * b;
+            if (j >= jlim || j > i || j < (mode == 0 ? k1 : w)) continue;
+            T* dst = front_at(P, C, f, w, i, j);
+            *dst -= acc[a][b];
+        }
+    }
+}
+
+// per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and the smallest
+// failed factor column (1-based, 0 = none) into info[0]
+__global__ void __launch_bounds__(256) k_chol_stats(DevSym S, const int32_t* __restrict__ status,
+                                                   int64_t* __restrict__ stats, int64_t* __restrict__ info) {
+    __shared__ int64_t best[256];
+    int64_t m = INT64_MAX;
+    for (int s = threadIdx.x; s < S.ns; s += blockDim.x) {
+        const int64_t fi = status[s];
+        int64_t* st = stats + (int64_t)s * NF;
+        st[0] = fi == 0 ? (int64_t)S.fncols[s] : fi - 1;
+        st[1] = st[2] = st[3] = st[4] = 0;
+        st[5] = fi;
+        if (fi > 0) m = min(m, (int64_t)S.super_ptr[s] + fi);
+    }
+    best[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) best[threadIdx.x] = min(best[threadIdx.x], best[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) info[0] = best[0] == INT64_MAX ? 0 : best[0];
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+template <typename T>
+void launch_front_small_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                             const double* nz, T* factor, T* cbpool, int32_t* status, double eps) {
+    size_t lds = (size_t)fmax * fmax * sizeof(T);
+    hipLaunchKernelGGL(k_front_small_chol<T>, dim3(nfronts), dim3(256), lds, st, S, fronts, nz, factor, cbpool,
+                       status, eps);
+}
+
+template <typename T>
+void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks, int n, const double* nz, T* factor,
+                             T* cbpool) {
+    hipLaunchKernelGGL(k_big_zero_scatter<T>, dim3(n), dim3(256), 0, st, S, tasks, nz, factor, cbpool);
+}
+
+template <typename T>
+void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool) {
+    hipLaunchKernelGGL(k_big_extend_add<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool);
+}
+
+template <typename T>
+void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* status,
+                      double eps) {
+    hipLaunchKernelGGL(k_big_potrf<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, status, eps);
+}
+
+template <typename T>
+void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* status) {
+    hipLaunchKernelGGL(k_big_trsm<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, status);
+}
+
+template <typename T>
+void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
+                       const int32_t* status, int mode) {
+    hipLaunchKernelGGL(k_big_update<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+}
+
+void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info) {
+    hipLaunchKernelGGL(k_chol_stats, dim3(1), dim3(256), 0, st, S, status, stats, info);
+}
+
+void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+    if (n <= 0) return;
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, st, p, n, v);
+}
+
+#define INST(T)                                                                                                   \
+    template void launch_front_small_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
+                                             T*, T*, int32_t*, double);                                           \
+    template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
+    template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
+    template void launch_big_potrf<T>(hipStream_t, const DevSym&, const Task*, int, T*, int32_t*, double);         \
+    template void launch_big_trsm<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int32_t*);           \
+    template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace sds

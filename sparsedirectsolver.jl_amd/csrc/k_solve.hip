@@ -1,0 +1,278 @@
+// Solve-phase kernels (gfx950): right-hand-side permutation, level-batched
+// multifrontal forward and backward sweeps, the D^-1 step of LDL^T, and the
+// FP64 residual of iterative refinement.
+//
+// Work vectors are row-interleaved (Y[k * nrhs + r]), so a gather of one row
+// reads all its right-hand sides from one cache line. The forward sweep is the
+// deterministic owner-pull form: a front pulls its children's update vectors
+// (m entries per right-hand side, "to subtract") in child_list order, solves
+// with L11 and leaves L21 x (plus what it pulled for its rows below) as its own
+// update vector; no atomics. The backward sweep gathers the final values of a
+// front's rows below from Y.
+//
+// Reference counterparts: src/solve/permute.jl:68-90, src/solve/sweeps.jl:438-570,
+// src/solve/refinement.jl:136-200.
+#include "sds_common.h"
+#include "sds_kernels.h"
+
+namespace sds {
+
+template <typename T>
+__global__ void k_permute_in(int n, int nrhs, const int32_t* __restrict__ perm, const double* __restrict__ B,
+                             int64_t ldb, T* __restrict__ Y) {
+    const int64_t total = (int64_t)n * nrhs;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
+        Y[q] = T(B[perm[k] + r * ldb]);
+    }
+}
+
+// X[perm[k], r] = Y[k, r] (add = 0) or X[perm[k], r] += Y[k, r] (add = 1)
+template <typename T>
+__global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm, const T* __restrict__ Y,
+                              double* __restrict__ X, int64_t ldx, int add) {
+    const int64_t total = (int64_t)n * nrhs;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
+        double* x = X + perm[k] + r * ldx;
+        *x = add ? *x + (double)Y[q] : (double)Y[q];
+    }
+}
+
+// forward sweep over the fronts of one level, one workgroup per front.
+// unit: unit diagonal (LDL^T, LU); piv (nullable): local pivot order of each
+// supernode (global factor columns), applied to its slice first through the
+// scratch Z (same shape as Y; a front only touches its own columns there).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd(DevSym S, const int32_t* __restrict__ fronts,
+                                            const T* __restrict__ factor, T* __restrict__ Y, T* __restrict__ U,
+                                            T* __restrict__ Z, int nrhs, int unit, const int32_t* __restrict__ piv) {
+    const int s = fronts[blockIdx.x];
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const T* P = factor + S.panel_off[s];
+    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    if (us)
+        for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
+    __syncthreads();
+    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
+        const int c = S.child_list[k];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        for (int q = tid; q < mc * nrhs; q += nt) {
+            const int ii = q / nrhs, r = q - ii * nrhs;
+            const int pos = ri[ii];
+            if (pos < w)
+                Y[(int64_t)(c0 + pos) * nrhs + r] -= uc[q];
+            else
+                us[(pos - w) * nrhs + r] += uc[q];
+        }
+        __syncthreads();
+    }
+    T* y = Y + (int64_t)c0 * nrhs;
+    if (piv) {
+        T* z = Z + (int64_t)c0 * nrhs;
+        for (int q = tid; q < w * nrhs; q += nt) {
+            const int kk = q / nrhs, r = q - kk * nrhs;
+            z[q] = Y[(int64_t)piv[c0 + kk] * nrhs + r];
+        }
+        __syncthreads();
+        for (int q = tid; q < w * nrhs; q += nt) y[q] = z[q];
+        __syncthreads();
+    }
+    for (int j0 = 0; j0 < w; j0 += 64) {
+        const int j1 = min(j0 + 64, w);
+        for (int j = j0; j < j1; ++j) {
+            if (!unit) {
+                if (tid < nrhs) y[j * nrhs + tid] /= P[(int64_t)j * f + j];
+                __syncthreads();
+            }
+            for (int q = tid; q < (j1 - j - 1) * nrhs; q += nt) {
+                const int i = j + 1 + q / nrhs, r = q % nrhs;
+                y[i * nrhs + r] -= P[(int64_t)j * f + i] * y[j * nrhs + r];
+            }
+            __syncthreads();
+        }
+        // rows below the chunk: in L11 (subtract now) or below the supernode (update vector)
+        for (int q = tid; q < (f - j1) * nrhs; q += nt) {
+            const int i = j1 + q / nrhs, r = q % nrhs;
+            T t = T(0);
+            for (int j = j0; j < j1; ++j) t += P[(int64_t)j * f + i] * y[j * nrhs + r];
+            if (i < w)
+                y[i * nrhs + r] -= t;
+            else if (us)
+                us[(i - w) * nrhs + r] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// backward sweep over the fronts of one level, one workgroup (4 waves) per front:
+// x_j = (y_j - sum_{i > j} L[i, j] x_i) / L[j, j], columns in chunks of 64 from the
+// last; the dot products over the rows past the chunk one wave per column, the
+// substitution inside the chunk on LDS copies. piv (nullable): undo the local pivot
+// order afterwards through the scratch Z.
+template <typename T, int RB>
+__global__ void __launch_bounds__(256) k_bwd(DevSym S, const int32_t* __restrict__ fronts,
+                                            const T* __restrict__ factor, T* __restrict__ Y, T* __restrict__ Z,
+                                            int nrhs, int unit, const int32_t* __restrict__ piv) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* xs = reinterpret_cast<T*>(smem_raw);  // 64 * nrhs: the chunk's values
+    T* ts = xs + 64 * nrhs;                  // 64 * nrhs: dot products past the chunk
+    const int s = fronts[blockIdx.x];
+    const int f = S.fnrows[s], w = S.fncols[s];
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const T* P = factor + S.panel_off[s];
+    const int32_t* rows = S.rowval + S.rowptr[s];
+    T* y = Y + (int64_t)c0 * nrhs;
+    for (int j1 = w; j1 > 0; j1 -= 64) {
+        const int j0 = max(j1 - 64, 0);
+        for (int j = j0 + wv; j < j1; j += 4) {
+            for (int rb = 0; rb < nrhs; rb += RB) {
+                T acc[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc[r] = T(0);
+                for (int i = j1 + lane; i < f; i += 64) {
+                    const T l = P[(int64_t)j * f + i];
+                    const T* yr = Y + (int64_t)rows[i] * nrhs + rb;
+#pragma unroll
+                    for (int r = 0; r < RB; ++r)
+                        if (rb + r < nrhs) acc[r] += l * yr[r];
+                }
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    T v = acc[r];
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    if (lane == 0 && rb + r < nrhs) ts[(j - j0) * nrhs + rb + r] = v;
+                }
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < (j1 - j0) * nrhs; q += 256) xs[q] = y[j0 * nrhs + q] - ts[q];
+        __syncthreads();
+        if (wv == 0) {
+            for (int j = j1 - 1; j >= j0; --j) {
+                const int i = j + 1 + lane;
+                const T l = i < j1 ? P[(int64_t)j * f + i] : T(0);
+                const T dgl = unit ? T(1) : P[(int64_t)j * f + j];
+                for (int r = 0; r < nrhs; ++r) {
+                    T v = i < j1 ? l * xs[(i - j0) * nrhs + r] : T(0);
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    if (lane == 0) {
+                        T x = xs[(j - j0) * nrhs + r] - v;
+                        if (!unit) x /= dgl;
+                        xs[(j - j0) * nrhs + r] = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < (j1 - j0) * nrhs; q += 256) y[j0 * nrhs + q] = xs[q];
+        __syncthreads();
+    }
+    if (piv) {
+        T* z = Z + (int64_t)c0 * nrhs;
+        for (int q = tid; q < w * nrhs; q += 256) z[q] = y[q];
+        __syncthreads();
+        for (int q = tid; q < w * nrhs; q += 256) {
+            const int kk = q / nrhs, r = q - kk * nrhs;
+            Y[(int64_t)piv[c0 + kk] * nrhs + r] = z[q];
+        }
+    }
+}
+
+// Y <- D^-1 Y with 1x1 and 2x2 blocks (src/solve/sweeps.jl:549-570); one thread per (row, rhs)
+template <typename T>
+__global__ void k_diag(int n, int nrhs, const T* __restrict__ d, const int8_t* __restrict__ kind, T* __restrict__ Y) {
+    const int64_t total = (int64_t)n * nrhs;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
+        const int8_t kd = kind ? kind[k] : KIND_1X1;
+        if (kd == KIND_2X2_SECOND) continue;
+        if (kd == KIND_2X2_FIRST) {
+            const T a = d[k], b = d[n + k], c = d[k + 1];
+            const T det = a * c - b * b;
+            const T y1 = Y[(int64_t)k * nrhs + r], y2 = Y[(int64_t)(k + 1) * nrhs + r];
+            Y[(int64_t)k * nrhs + r] = (c * y1 - b * y2) / det;
+            Y[(int64_t)(k + 1) * nrhs + r] = (a * y2 - b * y1) / det;
+        } else {
+            Y[q] /= d[k];
+        }
+    }
+}
+
+// R = B - M X over the full-pattern map (rowptr/colval/src 0-based; src < 0: -(p + 1), conjugate),
+// FP64; one thread per (row, rhs); `roff` selects the rows of M^T of a "G" map
+__global__ void k_residual(int n, int nrhs, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+                           const int64_t* __restrict__ src, const double* __restrict__ nz, const double* __restrict__ X,
+                           int64_t ldx, const double* __restrict__ B, int64_t ldb, double* __restrict__ R) {
+    const int64_t total = (int64_t)n * nrhs;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(q / n), i = (int)(q - (int64_t)r * n);
+        double acc = 0.0;
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            const int64_t p = src[e];
+            const double v = nz[p >= 0 ? p : -(p + 1)];
+            acc += v * X[colval[e] + r * ldx];
+        }
+        R[q] = B[i + r * ldb] - acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+static int grid_for(int64_t total) { return (int)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 4096); }
+
+template <typename T>
+void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y) {
+    hipLaunchKernelGGL(k_permute_in<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, B, ldb, Y);
+}
+
+template <typename T>
+void launch_permute_out(hipStream_t st, int n, int nrhs, const int32_t* perm, const T* Y, double* X, int64_t ldx,
+                        int add) {
+    hipLaunchKernelGGL(k_permute_out<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, Y, X,
+                       ldx, add);
+}
+
+template <typename T>
+void launch_fwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const T* factor, T* Y, T* U, T* Z,
+                int nrhs, int unit, const int32_t* piv) {
+    hipLaunchKernelGGL(k_fwd<T>, dim3(nfronts), dim3(256), 0, st, S, fronts, factor, Y, U, Z, nrhs, unit, piv);
+}
+
+template <typename T>
+void launch_bwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const T* factor, T* Y, T* Z,
+                int nrhs, int unit, const int32_t* piv) {
+    size_t lds = (size_t)128 * nrhs * sizeof(T);
+    hipLaunchKernelGGL((k_bwd<T, 8>), dim3(nfronts), dim3(256), lds, st, S, fronts, factor, Y, Z, nrhs, unit, piv);
+}
+
+template <typename T>
+void launch_diag(hipStream_t st, int n, int nrhs, const T* d, const int8_t* kind, T* Y) {
+    hipLaunchKernelGGL(k_diag<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, d, kind, Y);
+}
+
+void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,
+                     const double* nz, const double* X, int64_t ldx, const double* B, int64_t ldb, double* R) {
+    hipLaunchKernelGGL(k_residual, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, rowptr, colval, src,
+                       nz, X, ldx, B, ldb, R);
+}
+
+#define INST(T)                                                                                                     \
+    template void launch_permute_in<T>(hipStream_t, int, int, const int32_t*, const double*, int64_t, T*);          \
+    template void launch_permute_out<T>(hipStream_t, int, int, const int32_t*, const T*, double*, int64_t, int);    \
+    template void launch_fwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, T*, int, int,    \
+                                const int32_t*);                                                                    \
+    template void launch_bwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, int, int,        \
+                                const int32_t*);                                                                    \
+    template void launch_diag<T>(hipStream_t, int, int, const T*, const int8_t*, T*);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace sds

@@ -1,0 +1,54 @@
+// Shared device-side definitions of libsds_hip: the symbolic maps as the
+// kernels see them (0-based, converted once at upload from the 1-based arrays
+// of the analysis), task records of the level-batched launches, and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sds {
+
+// Device view of the analysis (src/symbolic/maps.jl:62-95), 0-based.
+struct DevSym {
+    int32_t n, ns;
+    const int32_t* perm;        // n: original row of factor column k
+    const int32_t* super_ptr;   // ns + 1
+    const int32_t* fnrows;      // ns: f
+    const int32_t* fncols;      // ns: w
+    const int64_t* panel_off;   // ns: element offset of the f x w panel in the factor buffer
+    const int64_t* rowptr;      // ns + 1: rows of supernode s are rowval[rowptr[s] .. rowptr[s+1])
+    const int32_t* rowval;      // supernodal row numbers
+    const int32_t* child_ptr;   // ns + 1
+    const int32_t* child_list;  // children in increasing id (owner-pull order)
+    const int64_t* relind_ptr;  // ns + 1
+    const int32_t* relind;      // positions of a child's CB rows among its parent's rows
+    const int64_t* asm_ptr;     // ns + 1: entries of A assembled into front s
+    const int64_t* asm_src;     // position of the entry in nzval
+    const int64_t* asm_dst;     // local panel offset j f + i; -(offset + 1) for a mirrored (U^T) entry
+    const int64_t* acol_ptr;    // n + 1: the entries of global factor column j start at acol_ptr[j]
+    const int64_t* cb_off;      // ns: contribution block (m x m, ld m) in the CB pool, -1 = none
+    const int64_t* uv_off;      // ns: solve update vector (m entries per right-hand side), -1 = none
+    int32_t unsym;              // structure "G": full fronts, L and U^T panels
+};
+
+// One task of a level-batched launch.
+struct Task {
+    int32_t s;  // front
+    int32_t a, b, c;
+};
+
+// pivot kinds (src/numeric/storage.jl:182-185)
+enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4 };
+
+constexpr int NB = 64;     // pivot block of the blocked front factorization
+constexpr int TILE = 64;   // Schur-update tile
+constexpr int NF = 6;      // per-front statistics fields
+
+#define SDS_HIP_CHECK(x)                                      \
+    do {                                                      \
+        hipError_t e_ = (x);                                  \
+        if (e_ != hipSuccess) return ::sds::hip_fail(e_, #x); \
+    } while (0)
+
+int hip_fail(hipError_t e, const char* what);
+
+}  // namespace sds

@@ -1,0 +1,753 @@
+// Host side of libsds_hip: the C ABI of include/sds_hip.h, the device schedule
+// built from an uploaded analysis, and the drivers of the numeric and solve
+// phases. All device memory is allocated at upload/allocation time; the phases
+// themselves allocate nothing (except growing the solve workspace when a call
+// brings more right-hand sides than any before it).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sds_hip.h"
+#include "sds_common.h"
+#include "sds_kernels.h"
+
+namespace sds {
+
+int hip_fail(hipError_t e, const char* what) {
+    (void)e;
+    (void)what;
+    return SDS_DEVICE_ERROR;
+}
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// launch kinds of the large-front plan
+enum LaunchKind { L_ZERO_SCATTER, L_EXTEND_ADD, L_POTRF, L_TRSM, L_UPDATE_PANEL, L_UPDATE_CB };
+
+struct Launch {
+    int kind;
+    int64_t off;
+    int n;
+};
+
+struct Level {
+    std::vector<int32_t> small, big, all;
+    int fmax_small = 0;
+    int64_t off_small = 0, off_all = 0;  // offsets into the device front lists
+    std::vector<Launch> plan;           // large-front launches of this level
+};
+
+}  // namespace
+}  // namespace sds
+
+using namespace sds;
+
+struct sds_handle {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    // analysis (host copies, 0-based)
+    bool have_sym = false;
+    int64_t n = 0, nnz = 0, ns = 0, factor_len = 0;
+    int structure = 3;
+    std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
+    std::vector<int64_t> panel_off;
+    std::vector<DevBuf> symbufs;  // device maps
+    DevSym dsym{};
+    int64_t full_nrows = 0;
+    int64_t* full_rowptr = nullptr;
+    int32_t* full_colval = nullptr;
+    int64_t* full_src = nullptr;
+    // schedule + numeric
+    int dtype = -1;
+    std::vector<Level> levels;
+    std::vector<DevBuf> numbufs;
+    int32_t* d_fronts = nullptr;  // per-level front lists (small lists, then all lists)
+    Task* d_tasks = nullptr;
+    int64_t cb_len = 0, uv_len = 0;
+    int64_t* d_cb_off = nullptr;
+    int64_t* d_uv_off = nullptr;
+    void* factor = nullptr;
+    void* cbpool = nullptr;
+    int32_t* status = nullptr;
+    int64_t* stats = nullptr;
+    int64_t* d_info = nullptr;
+    int64_t* h_info = nullptr;  // pinned
+    bool factorized = false;
+    // solve workspace
+    int cap_nrhs = 0;
+    DevBuf Y, Z, U, R, Bc, norms;
+    double* h_norms = nullptr;  // pinned
+    int fsmall = 0;
+};
+
+namespace {
+
+int fail(sds_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHK(h, x)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(h, SDS_DEVICE_ERROR, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int dev_alloc(sds_handle* h, std::vector<DevBuf>& owner, size_t bytes, void** out) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        return fail(h, SDS_DEVICE_ERROR,
+                    "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    owner.push_back({p, bytes});
+    *out = p;
+    return SDS_SUCCESS;
+}
+
+void free_all(std::vector<DevBuf>& bufs) {
+    for (auto& b : bufs) (void)hipFree(b.p);
+    bufs.clear();
+}
+
+template <typename V>
+int upload(sds_handle* h, std::vector<DevBuf>& owner, const std::vector<V>& v, V** out) {
+    void* p;
+    int rc = dev_alloc(h, owner, v.size() * sizeof(V), &p);
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    *out = static_cast<V*>(p);
+    return SDS_SUCCESS;
+}
+
+struct IdxReader {
+    const void* p;
+    int bits;
+    int64_t operator[](int64_t i) const {
+        return bits == 32 ? (int64_t) static_cast<const int32_t*>(p)[i] : static_cast<const int64_t*>(p)[i];
+    }
+};
+
+// first-fit placement of blocks with lifetimes [first, last] (levels) on an unbounded buffer:
+// level by level, release the blocks whose last level has passed, then place the new ones
+// (largest first). Returns the high-water mark.
+int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& first, const std::vector<int>& last,
+                     int nlev, std::vector<int64_t>& off) {
+    int64_t nb = (int64_t)len.size();
+    off.assign(nb, -1);
+    std::vector<std::vector<int64_t>> born(nlev + 1), dies(nlev + 2);
+    for (int64_t b = 0; b < nb; ++b)
+        if (len[b] > 0) {
+            born[first[b]].push_back(b);
+            dies[last[b] + 1].push_back(b);
+        }
+    std::vector<std::pair<int64_t, int64_t>> freel;  // sorted (offset, length) holes
+    int64_t top = 0;
+    auto release = [&](int64_t o, int64_t l) {
+        auto it = std::lower_bound(freel.begin(), freel.end(), std::make_pair(o, l));
+        it = freel.insert(it, {o, l});
+        size_t k = it - freel.begin();
+        if (k + 1 < freel.size() && freel[k].first + freel[k].second == freel[k + 1].first) {
+            freel[k].second += freel[k + 1].second;
+            freel.erase(freel.begin() + k + 1);
+        }
+        if (k > 0 && freel[k - 1].first + freel[k - 1].second == freel[k].first) {
+            freel[k - 1].second += freel[k].second;
+            freel.erase(freel.begin() + k);
+        }
+        if (!freel.empty() && freel.back().first + freel.back().second == top) {
+            top = freel.back().first;
+            freel.pop_back();
+        }
+    };
+    int64_t hw = 0;
+    for (int L = 0; L <= nlev; ++L) {
+        for (int64_t b : dies[L]) release(off[b], len[b]);
+        std::vector<int64_t>& nbk = born[L];
+        std::sort(nbk.begin(), nbk.end(), [&](int64_t a, int64_t b) { return len[a] > len[b]; });
+        for (int64_t b : nbk) {
+            int64_t l = len[b];
+            bool placed = false;
+            for (size_t k = 0; k < freel.size(); ++k)
+                if (freel[k].second >= l) {
+                    off[b] = freel[k].first;
+                    if (freel[k].second == l)
+                        freel.erase(freel.begin() + k);
+                    else
+                        freel[k] = {freel[k].first + l, freel[k].second - l};
+                    placed = true;
+                    break;
+                }
+            if (!placed) {
+                off[b] = top;
+                top += l;
+            }
+            hw = std::max(hw, top);
+        }
+    }
+    return hw;
+}
+
+void free_numeric(sds_handle* h) {
+    free_all(h->numbufs);
+    if (h->h_info) (void)hipHostFree(h->h_info);
+    if (h->h_norms) (void)hipHostFree(h->h_norms);
+    h->h_info = nullptr;
+    h->h_norms = nullptr;
+    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms}) {
+        if (b->p) (void)hipFree(b->p);
+        *b = DevBuf{};
+    }
+    h->cap_nrhs = 0;
+    h->levels.clear();
+    h->dtype = -1;
+    h->factorized = false;
+}
+
+int build_schedule(sds_handle* h, int dtype) {
+    const int64_t ns = h->ns;
+    const size_t es = dtype == SDS_R64 ? 8 : 4;
+    // largest front whose dense f x f square fits 64 KiB of LDS next to the kernel's static words
+    int fsmall = (int)std::floor(std::sqrt((65536.0 - 256.0) / (double)es));
+    h->fsmall = fsmall;
+    // tree levels by height: leaves 0, a parent one above its highest child
+    std::vector<int> lev(ns, 0);
+    for (int64_t s = 0; s < ns; ++s)
+        if (h->snparent[s] >= 0) lev[h->snparent[s]] = std::max(lev[h->snparent[s]], lev[s] + 1);
+    int nlev = 0;
+    for (int64_t s = 0; s < ns; ++s) nlev = std::max(nlev, lev[s] + 1);
+    h->levels.assign(nlev, Level{});
+    std::vector<char> big(ns, 0);
+    for (int64_t s = 0; s < ns; ++s) {
+        Level& L = h->levels[lev[s]];
+        L.all.push_back((int32_t)s);
+        if (h->fnrows[s] <= fsmall) {
+            L.small.push_back((int32_t)s);
+            L.fmax_small = std::max(L.fmax_small, h->fnrows[s]);
+        } else {
+            L.big.push_back((int32_t)s);
+            big[s] = 1;
+        }
+    }
+    // contribution blocks (m x m) and solve update vectors (m per rhs): live from
+    // their front's level to their parent's level
+    std::vector<int64_t> cblen(ns, 0), uvlen(ns, 0);
+    std::vector<int> first(ns), last(ns);
+    for (int64_t s = 0; s < ns; ++s) {
+        int64_t m = h->fnrows[s] - h->fncols[s];
+        first[s] = lev[s];
+        last[s] = h->snparent[s] >= 0 ? lev[h->snparent[s]] : lev[s];
+        if (m > 0 && (h->snparent[s] >= 0 || big[s])) cblen[s] = m * m;
+        if (m > 0 && h->snparent[s] >= 0) uvlen[s] = m;
+    }
+    std::vector<int64_t> cb_off, uv_off;
+    h->cb_len = place_blocks(cblen, first, last, nlev, cb_off);
+    h->uv_len = place_blocks(uvlen, first, last, nlev, uv_off);
+    // front lists
+    std::vector<int32_t> flist;
+    for (auto& L : h->levels) {
+        L.off_small = (int64_t)flist.size();
+        flist.insert(flist.end(), L.small.begin(), L.small.end());
+    }
+    for (auto& L : h->levels) {
+        L.off_all = (int64_t)flist.size();
+        flist.insert(flist.end(), L.all.begin(), L.all.end());
+    }
+    // task lists of the large fronts
+    std::vector<Task> tasks;
+    auto push_launch = [&](Level& L, int kind, const std::vector<Task>& t) {
+        if (t.empty()) return;
+        L.plan.push_back({kind, (int64_t)tasks.size(), (int)t.size()});
+        tasks.insert(tasks.end(), t.begin(), t.end());
+    };
+    for (auto& L : h->levels) {
+        if (L.big.empty()) continue;
+        std::vector<Task> t;
+        for (int32_t s : L.big)
+            for (int c = 0; c < h->fnrows[s]; c += 8) t.push_back({s, c, 8, 0});
+        push_launch(L, L_ZERO_SCATTER, t);
+        int maxch = 0;
+        for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
+        for (int r = 0; r < maxch; ++r) {
+            t.clear();
+            for (int32_t s : L.big) {
+                if (h->child_ptr[s] + r >= h->child_ptr[s + 1]) continue;
+                int32_t c = h->child_list[h->child_ptr[s] + r];
+                if (cb_off[c] < 0) continue;
+                int mc = h->fnrows[c] - h->fncols[c];
+                for (int jj = 0; jj < mc; jj += 16) t.push_back({s, c, jj, 16});
+            }
+            push_launch(L, L_EXTEND_ADD, t);
+        }
+        int wmax = 0;
+        for (int32_t s : L.big) wmax = std::max(wmax, h->fncols[s]);
+        for (int k = 0; k < wmax; k += NB) {
+            std::vector<Task> tp, tt, tu;
+            for (int32_t s : L.big) {
+                int f = h->fnrows[s], w = h->fncols[s];
+                if (k >= w) continue;
+                int nb = std::min(NB, w - k);
+                tp.push_back({s, k, 0, 0});
+                for (int r0 = k + nb; r0 < f; r0 += 64) tt.push_back({s, k, r0, 0});
+                int base = k + nb;
+                if (base < w)
+                    for (int I0 = base; I0 < f; I0 += TILE)
+                        for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) tu.push_back({s, I0, J0, k});
+            }
+            push_launch(L, L_POTRF, tp);
+            push_launch(L, L_TRSM, tt);
+            push_launch(L, L_UPDATE_PANEL, tu);
+        }
+        t.clear();
+        for (int32_t s : L.big) {
+            int f = h->fnrows[s], w = h->fncols[s];
+            if (f == w || cb_off[s] < 0 || h->snparent[s] < 0) continue;
+            for (int I0 = w; I0 < f; I0 += TILE)
+                for (int J0 = w; J0 <= I0; J0 += TILE) t.push_back({s, I0, J0, 0});
+        }
+        push_launch(L, L_UPDATE_CB, t);
+    }
+    void* p;
+    int rc;
+    if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
+    if ((rc = upload(h, h->numbufs, tasks, &h->d_tasks))) return rc;
+    if ((rc = upload(h, h->numbufs, cb_off, &h->d_cb_off))) return rc;
+    if ((rc = upload(h, h->numbufs, uv_off, &h->d_uv_off))) return rc;
+    h->dsym.cb_off = h->d_cb_off;
+    h->dsym.uv_off = h->d_uv_off;
+    (void)p;
+    return SDS_SUCCESS;
+}
+
+template <typename T>
+int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volatile int32_t* interrupt) {
+    hipStream_t st = h->stream;
+    T* factor = static_cast<T*>(h->factor);
+    T* cb = static_cast<T*>(h->cbpool);
+    launch_fill_i32(st, h->status, h->ns, 0);
+    for (auto& L : h->levels) {
+        if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
+        if (!L.small.empty())
+            launch_front_small_chol<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small, nz,
+                                       factor, cb, h->status, eps);
+        for (const Launch& ln : L.plan) {
+            const Task* t = h->d_tasks + ln.off;
+            switch (ln.kind) {
+                case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
+                case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
+                case L_POTRF: launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, h->status, eps); break;
+                case L_TRSM: launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, h->status); break;
+                case L_UPDATE_PANEL: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0); break;
+                case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
+            }
+        }
+    }
+    launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info);
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
+}
+
+int ensure_solve_workspace(sds_handle* h, int nrhs) {
+    if (nrhs <= h->cap_nrhs) return SDS_SUCCESS;
+    size_t es = h->dtype == SDS_R64 ? 8 : 4;
+    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms}) {
+        if (b->p) (void)hipFree(b->p);
+        *b = DevBuf{};
+    }
+    auto get = [&](DevBuf& b, size_t bytes) -> int {
+        bytes = std::max<size_t>(bytes, 8);
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) return fail(h, SDS_DEVICE_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+        b.bytes = bytes;
+        return SDS_SUCCESS;
+    };
+    int rc;
+    if ((rc = get(h->Y, es * h->n * nrhs))) return rc;
+    if ((rc = get(h->Z, es * h->n * nrhs))) return rc;
+    if ((rc = get(h->U, es * std::max<int64_t>(h->uv_len, 1) * nrhs))) return rc;
+    if ((rc = get(h->R, 8 * h->n * nrhs))) return rc;
+    if ((rc = get(h->Bc, 8 * h->n * nrhs))) return rc;
+    if ((rc = get(h->norms, 16 * nrhs))) return rc;
+    h->cap_nrhs = nrhs;
+    return SDS_SUCCESS;
+}
+
+template <typename T>
+void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd) {
+    hipStream_t st = h->stream;
+    const T* factor = static_cast<const T*>(h->factor);
+    T* Y = static_cast<T*>(h->Y.p);
+    T* Z = static_cast<T*>(h->Z.p);
+    T* U = static_cast<T*>(h->U.p);
+    const bool unit = false;  // Cholesky
+    if (fwd)
+        for (auto& L : h->levels)
+            launch_fwd<T>(st, h->dsym, h->d_fronts + L.off_all, (int)L.all.size(), factor, Y, U, Z, nrhs, unit,
+                          nullptr);
+    (void)diag;
+    if (bwd)
+        for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it)
+            launch_bwd<T>(st, h->dsym, h->d_fronts + it->off_all, (int)it->all.size(), factor, Y, Z, nrhs, unit,
+                          nullptr);
+}
+
+template <typename T>
+int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
+            const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
+    hipStream_t st = h->stream;
+    int rc = ensure_solve_workspace(h, nrhs);
+    if (rc) return rc;
+    T* Y = static_cast<T*>(h->Y.p);
+    const int n = (int)h->n;
+    const bool refine = (phase & SDS_PHASE_SOLVE_REFINEMENT) && ir_n_steps > 0;
+    const double* Bsrc = B;
+    if (refine && X == B) {
+        HIPCHK(h, hipMemcpy2DAsync(h->Bc.p, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                   hipMemcpyDeviceToDevice, st));
+        Bsrc = static_cast<const double*>(h->Bc.p);
+    }
+    if (phase & SDS_PHASE_SOLVE_FWD_PERM) launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y);
+    sweeps<T>(h, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD);
+    if (phase & SDS_PHASE_SOLVE_BWD_PERM) launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0);
+    int done = 0;
+    if (refine) {
+        if (!h->full_rowptr) return fail(h, SDS_INVALID_VALUE, "refinement needs the full-pattern map at upload");
+        if (!nz) return fail(h, SDS_INVALID_VALUE, "refinement needs the matrix values");
+        double* R = static_cast<double*>(h->R.p);
+        for (int it = 0; it < ir_n_steps; ++it) {
+            launch_residual(st, n, nrhs, h->full_rowptr, h->full_colval, h->full_src, nz, X, ldx, Bsrc,
+                            Bsrc == B ? ldb : n, R);
+            if (ir_tol > 0) {
+                // host copies of the residual and rhs norms (one synchronization per step)
+                std::vector<double> hr((size_t)n * nrhs), hb((size_t)n * nrhs);
+                HIPCHK(h, hipMemcpyAsync(hr.data(), R, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost, st));
+                HIPCHK(h, hipMemcpy2DAsync(hb.data(), sizeof(double) * n, Bsrc, sizeof(double) * (Bsrc == B ? ldb : n),
+                                           sizeof(double) * n, nrhs, hipMemcpyDeviceToHost, st));
+                HIPCHK(h, hipStreamSynchronize(st));
+                bool ok = true;
+                for (int r = 0; r < nrhs; ++r) {
+                    double a = 0, b = 0;
+                    for (int i = 0; i < n; ++i) {
+                        a += hr[(size_t)r * n + i] * hr[(size_t)r * n + i];
+                        b += hb[(size_t)r * n + i] * hb[(size_t)r * n + i];
+                    }
+                    if (std::sqrt(a) > ir_tol * std::sqrt(b)) ok = false;
+                }
+                if (ok) break;
+            }
+            launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y);
+            sweeps<T>(h, nrhs, 1, 1, 1);
+            launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1);
+            ++done;
+        }
+    }
+    if (ir_steps) *ir_steps = done;
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sds_create(sds_handle_t* out, void* stream) {
+    if (!out) return SDS_INVALID_VALUE;
+    auto h = new sds_handle();
+    h->stream = static_cast<hipStream_t>(stream);
+    (void)hipGetDevice(&h->device);
+    *out = h;
+    return SDS_SUCCESS;
+}
+
+int sds_destroy(sds_handle_t h) {
+    if (!h) return SDS_SUCCESS;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    else (void)hipDeviceSynchronize();
+    free_numeric(h);
+    free_all(h->symbufs);
+    delete h;
+    return SDS_SUCCESS;
+}
+
+const char* sds_last_error(sds_handle_t h) { return h ? h->err.c_str() : "null handle"; }
+
+int sds_set_stream(sds_handle_t h, void* stream) {
+    if (!h) return SDS_INVALID_VALUE;
+    h->stream = static_cast<hipStream_t>(stream);
+    return SDS_SUCCESS;
+}
+
+int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
+    if (!h || !d) return SDS_INVALID_VALUE;
+    if (d->index_bits != 32 && d->index_bits != 64) return fail(h, SDS_INVALID_VALUE, "index_bits must be 32 or 64");
+    if (d->structure != SDS_STRUCTURE_SPD && d->structure != SDS_STRUCTURE_HPD)
+        return fail(h, SDS_NOT_SUPPORTED, "only the Cholesky structures \"SPD\"/\"HPD\" are available in this build");
+    free_numeric(h);
+    free_all(h->symbufs);
+    h->have_sym = false;
+    const int b = d->index_bits;
+    const int64_t n = d->n, ns = d->ns, nnz = d->nnz;
+    h->n = n;
+    h->ns = ns;
+    h->nnz = nnz;
+    h->structure = d->structure;
+    IdxReader perm{d->perm, b}, sptr{d->super_ptr, b}, spar{d->snparent, b}, rptr{d->rowptr, b}, rval{d->rowval, b},
+        fptr{d->front_ptr, b}, fnr{d->front_nrows, b}, fnc{d->front_ncols, b}, cptr{d->child_ptr, b},
+        clist{d->child_list, b}, riptr{d->relind_ptr, b}, rind{d->relind, b}, amap{d->amap, b},
+        aptr{d->amap_ptr, b}, asrc{d->amap_src, b};
+    h->perm.resize(n);
+    for (int64_t k = 0; k < n; ++k) h->perm[k] = (int32_t)(perm[k] - 1);
+    h->super_ptr.resize(ns + 1);
+    for (int64_t s = 0; s <= ns; ++s) h->super_ptr[s] = (int32_t)(sptr[s] - 1);
+    h->fnrows.resize(ns);
+    h->fncols.resize(ns);
+    h->snparent.resize(ns);
+    h->panel_off.resize(ns);
+    for (int64_t s = 0; s < ns; ++s) {
+        h->fnrows[s] = (int32_t)fnr[s];
+        h->fncols[s] = (int32_t)fnc[s];
+        h->snparent[s] = (int32_t)(spar[s] - 1);
+        h->panel_off[s] = fptr[s] - 1;
+        if (h->snparent[s] >= 0 && h->snparent[s] <= s)
+            return fail(h, SDS_INVALID_VALUE, "snparent must point to a later supernode");
+    }
+    h->factor_len = fptr[ns] - 1;
+    std::vector<int64_t> rowptr(ns + 1);
+    for (int64_t s = 0; s <= ns; ++s) rowptr[s] = rptr[s] - 1;
+    std::vector<int32_t> rowval(rowptr[ns]);
+    for (int64_t q = 0; q < rowptr[ns]; ++q) rowval[q] = (int32_t)(rval[q] - 1);
+    h->child_ptr.resize(ns + 1);
+    for (int64_t s = 0; s <= ns; ++s) h->child_ptr[s] = (int32_t)(cptr[s] - 1);
+    h->child_list.resize(h->child_ptr[ns]);
+    for (int64_t q = 0; q < h->child_ptr[ns]; ++q) h->child_list[q] = (int32_t)(clist[q] - 1);
+    std::vector<int64_t> relind_ptr(ns + 1);
+    for (int64_t s = 0; s <= ns; ++s) relind_ptr[s] = riptr[s] - 1;
+    std::vector<int32_t> relind(relind_ptr[ns]);
+    for (int64_t q = 0; q < relind_ptr[ns]; ++q) relind[q] = (int32_t)(rind[q] - 1);
+    // assembly entries grouped by front (owner-pull), with local panel offsets
+    std::vector<int64_t> asm_ptr(ns + 1), asm_src, asm_dst, acol_ptr(n + 1, 0);
+    for (int64_t s = 0; s <= ns; ++s) asm_ptr[s] = aptr[s] - 1;
+    asm_src.resize(asm_ptr[ns]);
+    asm_dst.resize(asm_ptr[ns]);
+    for (int64_t s = 0; s < ns; ++s) {
+        const int64_t f = h->fnrows[s];
+        const int32_t c0 = h->super_ptr[s], w = h->fncols[s];
+        int32_t col = 0;
+        for (int64_t k = asm_ptr[s]; k < asm_ptr[s + 1]; ++k) {
+            int64_t p = asrc[k] - 1;
+            if (p < 0 || p >= nnz) return fail(h, SDS_INVALID_VALUE, "amap_src out of range");
+            int64_t off = amap[p];
+            int64_t loc = (off < 0 ? -off : off) - 1 - h->panel_off[s];
+            if (loc < 0 || loc >= f * w) return fail(h, SDS_INVALID_VALUE, "amap offset outside its front");
+            asm_src[k] = p;
+            asm_dst[k] = off > 0 ? loc : -(loc + 1);
+            int32_t j = (int32_t)(loc / f);
+            while (col <= j) acol_ptr[c0 + col++] = k;
+        }
+        while (col <= w) {
+            acol_ptr[c0 + col] = asm_ptr[s + 1];
+            ++col;
+        }
+    }
+    acol_ptr[n] = asm_ptr[ns];
+    int rc;
+    DevSym& D = h->dsym;
+    D = DevSym{};
+    D.n = (int32_t)n;
+    D.ns = (int32_t)ns;
+    D.unsym = d->structure == SDS_STRUCTURE_G;
+    int32_t *p_perm, *p_sptr, *p_fnr, *p_fnc, *p_rval, *p_cptr, *p_clist, *p_rind;
+    int64_t *p_poff, *p_rptr, *p_riptr, *p_aptr, *p_asrc, *p_adst, *p_acol;
+    if ((rc = upload(h, h->symbufs, h->perm, &p_perm))) return rc;
+    if ((rc = upload(h, h->symbufs, h->super_ptr, &p_sptr))) return rc;
+    if ((rc = upload(h, h->symbufs, h->fnrows, &p_fnr))) return rc;
+    if ((rc = upload(h, h->symbufs, h->fncols, &p_fnc))) return rc;
+    if ((rc = upload(h, h->symbufs, h->panel_off, &p_poff))) return rc;
+    if ((rc = upload(h, h->symbufs, rowptr, &p_rptr))) return rc;
+    if ((rc = upload(h, h->symbufs, rowval, &p_rval))) return rc;
+    if ((rc = upload(h, h->symbufs, h->child_ptr, &p_cptr))) return rc;
+    if ((rc = upload(h, h->symbufs, h->child_list, &p_clist))) return rc;
+    if ((rc = upload(h, h->symbufs, relind_ptr, &p_riptr))) return rc;
+    if ((rc = upload(h, h->symbufs, relind, &p_rind))) return rc;
+    if ((rc = upload(h, h->symbufs, asm_ptr, &p_aptr))) return rc;
+    if ((rc = upload(h, h->symbufs, asm_src, &p_asrc))) return rc;
+    if ((rc = upload(h, h->symbufs, asm_dst, &p_adst))) return rc;
+    if ((rc = upload(h, h->symbufs, acol_ptr, &p_acol))) return rc;
+    D.perm = p_perm;
+    D.super_ptr = p_sptr;
+    D.fnrows = p_fnr;
+    D.fncols = p_fnc;
+    D.panel_off = p_poff;
+    D.rowptr = p_rptr;
+    D.rowval = p_rval;
+    D.child_ptr = p_cptr;
+    D.child_list = p_clist;
+    D.relind_ptr = p_riptr;
+    D.relind = p_rind;
+    D.asm_ptr = p_aptr;
+    D.asm_src = p_asrc;
+    D.asm_dst = p_adst;
+    D.acol_ptr = p_acol;
+    h->full_rowptr = nullptr;
+    h->full_colval = nullptr;
+    h->full_src = nullptr;
+    h->full_nrows = 0;
+    if (d->full_rowptr && d->full_colval && d->full_src) {
+        IdxReader frp{d->full_rowptr, b}, fcv{d->full_colval, b}, fsrc{d->full_src, b};
+        int64_t nr = d->full_nrows, fnnz = d->full_nnz;
+        std::vector<int64_t> rp(nr + 1), src(fnnz);
+        std::vector<int32_t> cv(fnnz);
+        for (int64_t i = 0; i <= nr; ++i) rp[i] = frp[i] - 1;
+        for (int64_t e = 0; e < fnnz; ++e) {
+            cv[e] = (int32_t)(fcv[e] - 1);
+            int64_t sv = fsrc[e];
+            src[e] = sv > 0 ? sv - 1 : sv;
+        }
+        if ((rc = upload(h, h->symbufs, rp, &h->full_rowptr))) return rc;
+        if ((rc = upload(h, h->symbufs, cv, &h->full_colval))) return rc;
+        if ((rc = upload(h, h->symbufs, src, &h->full_src))) return rc;
+        h->full_nrows = nr;
+    }
+    h->have_sym = true;
+    return SDS_SUCCESS;
+}
+
+int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->have_sym) return fail(h, SDS_FACTORIZATION_ERROR, "no analysis uploaded");
+    if (dtype != SDS_R64 && dtype != SDS_R32) return fail(h, SDS_INVALID_VALUE, "dtype must be SDS_R64 or SDS_R32");
+    if (nbatch != 1) return fail(h, SDS_NOT_SUPPORTED, "uniform batches are not available in this build");
+    free_numeric(h);
+    int rc = build_schedule(h, dtype);
+    if (rc) return rc;
+    size_t es = dtype == SDS_R64 ? 8 : 4;
+    void* p;
+    if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->factor))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->cb_len, 1), &h->cbpool))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
+    h->status = static_cast<int32_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * NF * (h->ns + 1), &p))) return rc;
+    h->stats = static_cast<int64_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * 8, &p))) return rc;
+    h->d_info = static_cast<int64_t*>(p);
+    HIPCHK(h, hipHostMalloc((void**)&h->h_info, sizeof(int64_t) * 8, hipHostMallocDefault));
+    h->dtype = dtype;
+    return SDS_SUCCESS;
+}
+
+int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int64_t* info) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (h->dtype < 0) return fail(h, SDS_FACTORIZATION_ERROR, "numeric storage not allocated");
+    if (!nzval && h->nnz > 0) return fail(h, SDS_INVALID_VALUE, "nzval is NULL");
+    double eps = h->dtype == SDS_R64 ? 1e-13 : 1e-5;  // default_pivot_epsilon (src/options.jl:57-61)
+    const volatile int32_t* intr = nullptr;
+    if (prm) {
+        if (prm->pivot_epsilon >= 0) eps = prm->pivot_epsilon;
+        intr = prm->interrupt;
+    }
+    h->factorized = false;
+    int rc = h->dtype == SDS_R64 ? factorize_cholesky<double>(h, nzval, eps, intr)
+                                 : factorize_cholesky<float>(h, nzval, eps, intr);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int64_t col = h->h_info[0];
+    if (info) *info = col > 0 ? (int64_t)h->perm[col - 1] + 1 : 0;
+    h->factorized = true;
+    return SDS_SUCCESS;
+}
+
+int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B, int64_t ldb, double* X, int64_t ldx,
+              int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->factorized) return fail(h, SDS_FACTORIZATION_ERROR, "solve before a factorization");
+    if (nrhs < 1 || nrhs > 64) return fail(h, SDS_NOT_SUPPORTED, "nrhs must be in 1..64");
+    if (ldb < h->n || ldx < h->n) return fail(h, SDS_INVALID_VALUE, "leading dimension smaller than n");
+    if (solve_mode != 0 && solve_mode != 1 && solve_mode != 2)
+        return fail(h, SDS_INVALID_VALUE, "solve_mode must be 0, 1 or 2");
+    if ((phase & ~SDS_PHASE_SOLVE) != 0 || phase == 0) return fail(h, SDS_INVALID_VALUE, "not a solve phase");
+    // symmetric structures: A^T = A, and for real values A^H = A
+    return h->dtype == SDS_R64 ? solve_t<double>(h, phase, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
+                               : solve_t<float>(h, phase, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+}
+
+int sds_get_stats(sds_handle_t h, int64_t* stats, int64_t* totals) {
+    if (!h || h->dtype < 0) return SDS_INVALID_VALUE;
+    std::vector<int64_t> st((size_t)NF * h->ns);
+    HIPCHK(h, hipMemcpyAsync(st.data(), h->stats, sizeof(int64_t) * st.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (stats) std::memcpy(stats, st.data(), sizeof(int64_t) * st.size());
+    if (totals) {
+        for (int q = 0; q < NF; ++q) totals[q] = 0;
+        for (int64_t s = 0; s < h->ns; ++s) {
+            for (int q = 0; q < NF - 1; ++q) totals[q] += st[s * NF + q];
+            totals[NF - 1] += st[s * NF + NF - 1] != 0;
+        }
+    }
+    return SDS_SUCCESS;
+}
+
+int sds_get_pivots(sds_handle_t h, int32_t* piv, int8_t* kind) {
+    if (!h || h->dtype < 0) return SDS_INVALID_VALUE;
+    // Cholesky: no pivoting, every pivot a 1x1 in place
+    for (int64_t k = 0; k < h->n; ++k) {
+        if (piv) piv[k] = (int32_t)(k + 1);
+        if (kind) kind[k] = KIND_1X1;
+    }
+    return SDS_SUCCESS;
+}
+
+int sds_get_diag(sds_handle_t h, void* d) {
+    if (!h || h->dtype < 0 || !d) return SDS_INVALID_VALUE;
+    // Cholesky: the diagonal of L, read from the panels
+    size_t es = h->dtype == SDS_R64 ? 8 : 4;
+    std::memset(d, 0, es * 2 * h->n);
+    for (int64_t s = 0; s < h->ns; ++s) {
+        int64_t f = h->fnrows[s];
+        for (int32_t j = 0; j < h->fncols[s]; ++j)
+            HIPCHK(h, hipMemcpy(static_cast<char*>(d) + es * (h->super_ptr[s] + j),
+                                static_cast<char*>(h->factor) + es * (h->panel_off[s] + (int64_t)j * f + j), es,
+                                hipMemcpyDeviceToHost));
+    }
+    return SDS_SUCCESS;
+}
+
+int sds_get_factor(sds_handle_t h, int32_t which, void* out) {
+    if (!h || h->dtype < 0 || !out) return SDS_INVALID_VALUE;
+    if (which != 0) return fail(h, SDS_NOT_SUPPORTED, "no U^T factor for this structure");
+    size_t es = h->dtype == SDS_R64 ? 8 : 4;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->factor, es * h->factor_len, hipMemcpyDeviceToHost));
+    return SDS_SUCCESS;
+}
+
+int sds_memory_estimates(sds_handle_t h, int64_t* est) {
+    if (!h || !est) return SDS_INVALID_VALUE;
+    for (int q = 0; q < 16; ++q) est[q] = 0;
+    size_t es = h->dtype == SDS_R32 ? 4 : 8;
+    int64_t maps = 0;
+    for (auto& b : h->symbufs) maps += (int64_t)b.bytes;
+    est[6] = (int64_t)es * h->factor_len;
+    est[7] = (int64_t)es * 2 * h->n;
+    est[8] = (int64_t)es * h->cb_len;
+    est[9] = maps;
+    est[10] = (int64_t)(h->ns + 1) * NF * 8 + (h->ns + 1) * 4 + h->n * 6;
+    est[0] = est[6] + est[7] + est[9] + est[10];
+    est[1] = est[0] + est[8];
+    est[4] = est[1];
+    return SDS_SUCCESS;
+}
+
+}  // extern "C"

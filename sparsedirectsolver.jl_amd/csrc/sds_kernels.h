@@ -1,0 +1,42 @@
+// Launchers of the device kernels (defined in k_factor.hip, k_solve.hip).
+#pragma once
+#include "sds_common.h"
+
+namespace sds {
+
+template <typename T>
+void launch_front_small_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                             const double* nz, T* factor, T* cbpool, int32_t* status, double eps);
+template <typename T>
+void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks, int n, const double* nz, T* factor,
+                             T* cbpool);
+template <typename T>
+void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool);
+template <typename T>
+void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* status,
+                      double eps);
+template <typename T>
+void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* status);
+template <typename T>
+void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
+                       const int32_t* status, int mode);
+void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info);
+void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
+
+template <typename T>
+void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y);
+template <typename T>
+void launch_permute_out(hipStream_t st, int n, int nrhs, const int32_t* perm, const T* Y, double* X, int64_t ldx,
+                        int add);
+template <typename T>
+void launch_fwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const T* factor, T* Y, T* U, T* Z,
+                int nrhs, int unit, const int32_t* piv);
+template <typename T>
+void launch_bwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const T* factor, T* Y, T* Z,
+                int nrhs, int unit, const int32_t* piv);
+template <typename T>
+void launch_diag(hipStream_t st, int n, int nrhs, const T* d, const int8_t* kind, T* Y);
+void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,
+                     const double* nz, const double* X, int64_t ldx, const double* B, int64_t ldb, double* R);
+
+}  // namespace sds

@@ -1,0 +1,47 @@
+"""ctypes signatures of the C ABI in include/sds_hip.h."""
+import ctypes
+
+c_i32, c_i64, c_dbl, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+# the exported entry points, in the order of include/sds_hip.h
+EXPORTS = ("sds_create", "sds_destroy", "sds_last_error", "sds_set_stream", "sds_upload_symbolic",
+           "sds_alloc_numeric", "sds_factorize", "sds_solve", "sds_get_stats", "sds_get_pivots", "sds_get_diag",
+           "sds_get_factor", "sds_memory_estimates")
+
+SDS_SUCCESS, SDS_INVALID_VALUE, SDS_NOT_SUPPORTED, SDS_FACTORIZATION_ERROR, SDS_DEVICE_ERROR, SDS_INTERRUPTED = range(6)
+SDS_R64, SDS_R32 = 0, 1
+
+
+class SymbolicDesc(ctypes.Structure):
+    _fields_ = [("n", c_i64), ("nnz", c_i64), ("ns", c_i64), ("structure", c_i32), ("index_bits", c_i32)] + [
+        (k, vp) for k in ("perm", "super_ptr", "snparent", "rowptr", "rowval", "front_ptr", "front_nrows",
+                          "front_ncols", "child_ptr", "child_list", "relind_ptr", "relind", "amap", "amap_ptr",
+                          "amap_src")] + [("full_nrows", c_i64), ("full_nnz", c_i64), ("full_rowptr", vp),
+                                          ("full_colval", vp), ("full_src", vp)]
+
+
+class FactorParams(ctypes.Structure):
+    _fields_ = [("pivot_type", c_i32), ("pivot_epsilon_alg", c_i32), ("pivot_threshold", c_dbl),
+                ("pivot_epsilon", c_dbl), ("pivot_sign", vp), ("interrupt", vp)]
+
+
+def bind(lib):
+    P = ctypes.POINTER
+    lib.sds_create.argtypes = [P(vp), vp]
+    lib.sds_destroy.argtypes = [vp]
+    lib.sds_last_error.argtypes = [vp]
+    lib.sds_last_error.restype = ctypes.c_char_p
+    lib.sds_set_stream.argtypes = [vp, vp]
+    lib.sds_upload_symbolic.argtypes = [vp, P(SymbolicDesc)]
+    lib.sds_alloc_numeric.argtypes = [vp, c_i32, c_i32]
+    lib.sds_factorize.argtypes = [vp, vp, P(FactorParams), P(c_i64)]
+    lib.sds_solve.argtypes = [vp, c_i32, c_i32, vp, c_i64, vp, c_i64, c_i32, vp, c_i32, c_dbl, P(c_i32)]
+    lib.sds_get_stats.argtypes = [vp, vp, vp]
+    lib.sds_get_pivots.argtypes = [vp, vp, vp]
+    lib.sds_get_diag.argtypes = [vp, vp]
+    lib.sds_get_factor.argtypes = [vp, c_i32, vp]
+    lib.sds_memory_estimates.argtypes = [vp, vp]
+    for name in EXPORTS:
+        if name != "sds_last_error":
+            getattr(lib, name).restype = ctypes.c_int
+    return lib

@@ -1,0 +1,355 @@
+"""The solver handle and its phases, mirroring SparseDirectSolver.jl's API.
+
+``DirectSolver``, ``execute``, ``setparam``, ``getparam`` and ``update`` keep the
+names, phase strings, parameter names and error behaviour of the reference
+(src/solver.jl:187-966, src/options.jl:155-657) so that a caller of the
+reference (or of CUDSS.jl) finds the same surface. The host analysis runs in
+``libsds_analysis.so``; every numeric and solve phase runs on the GPU through
+the C ABI of ``libsds_hip.so`` (include/sds_hip.h). There is no CPU path: without
+the HIP library the phases raise.
+
+Device arrays are torch tensors on ``cuda`` (plumbing for HIP memory and streams).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._native import hip_lib
+from .analysis import symbolic_analysis
+from .enums import PHASES, PIVOT_TYPES, STRUCTURES, VIEWS
+from .errors import FactorizationError, InterruptedError_, InvalidValueError, NotSupportedError
+
+CONFIG_PARAMETERS = ("reordering_alg", "factorization_alg", "solve_alg", "matching_alg", "solve_mode", "ir_n_steps",
+                     "ir_tol", "pivot_type", "pivot_threshold", "pivot_epsilon", "max_lu_nnz", "hybrid_memory_mode",
+                     "hybrid_device_memory_limit", "use_cuda_register_memory", "host_nthreads",
+                     "hybrid_execute_mode", "pivot_epsilon_alg", "nd_nlevels", "ubatch_size", "ubatch_index",
+                     "use_superpanels", "device_count", "device_indices", "schur_mode", "deterministic_mode",
+                     "nd_ubfactor", "nd_nseps", "nd_seed")
+EXTRA_CONFIG = ("ir_mode", "factor_precision", "amalgamation", "schedule", "pivot_pairs", "pivot_pair_tolerance")
+USER_DATA = ("user_perm", "user_host_interrupt", "pivot_sign", "user_schur_indices", "user_nd_partition_tree",
+             "ubatch_mask")
+DATA_PARAMETERS = ("info", "lu_nnz", "npivots", "inertia", "perm_reorder_row", "perm_reorder_col", "perm_row",
+                   "perm_col", "diag", "user_perm", "hybrid_device_memory_min", "comm_device", "comm_host",
+                   "memory_estimates", "perm_matching", "scale_row", "scale_col", "nsuperpanels",
+                   "user_schur_indices", "schur_shape", "schur_matrix", "user_nd_partition_tree",
+                   "nd_partition_tree", "user_host_interrupt", "ir_n_steps", "ubatch_mask", "flops", "pivot_sign",
+                   "pivot_stats")
+
+_EPS_ALGS = {"default": 0, "algo1": 1, "algo2": 2}
+
+STAGE_NONE, STAGE_REORDERED, STAGE_ANALYZED, STAGE_FACTORIZED = range(4)
+
+
+def _defaults():
+    return {
+        "reordering_alg": "default", "factorization_alg": "default", "solve_alg": "default",
+        "matching_alg": "default", "solve_mode": 0, "ir_n_steps": 0, "ir_tol": 0.0, "pivot_type": "A",
+        "pivot_threshold": 0.01, "pivot_epsilon": None, "pivot_epsilon_alg": "default", "max_lu_nnz": -1,
+        "hybrid_memory_mode": 0, "hybrid_device_memory_limit": 0, "use_cuda_register_memory": 1,
+        "host_nthreads": 0, "hybrid_execute_mode": 0, "nd_nlevels": 10, "ubatch_size": 0, "ubatch_index": -1,
+        "use_superpanels": 1, "device_count": 1, "device_indices": None, "schur_mode": 0,
+        "deterministic_mode": 0, "nd_ubfactor": -1, "nd_nseps": -1, "nd_seed": -1,
+        "ir_mode": "plain", "factor_precision": None, "amalgamation": (32, 0.25, 8), "schedule": "auto",
+        "pivot_pairs": "default", "pivot_pair_tolerance": 1e-6,
+        "user_perm": None, "user_host_interrupt": None, "pivot_sign": None, "user_schur_indices": None,
+        "user_nd_partition_tree": None, "ubatch_mask": None,
+    }
+
+
+def _check(lib, h, rc):
+    if rc == _abi.SDS_SUCCESS:
+        return
+    msg = lib.sds_last_error(h).decode()
+    if rc == _abi.SDS_INVALID_VALUE:
+        raise InvalidValueError(msg)
+    if rc == _abi.SDS_NOT_SUPPORTED:
+        raise NotSupportedError(msg)
+    if rc == _abi.SDS_FACTORIZATION_ERROR:
+        raise FactorizationError(0, msg)
+    if rc == _abi.SDS_INTERRUPTED:
+        raise InterruptedError_(msg)
+    raise RuntimeError(f"libsds_hip: {msg}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class DirectSolver:
+    """Solver handle for one sparse matrix (src/solver.jl:187-244).
+
+    ``DirectSolver(A, structure, view)`` with ``A`` a scipy CSR matrix (host), or
+    ``DirectSolver((rowptr, colval, nzval, n), structure, view, index="O")`` with
+    the CSR arrays; ``nzval`` may be a torch tensor on ``cuda`` (used in place)."""
+
+    def __init__(self, A, structure, view="F", index="O"):
+        torch = _torch()
+        if structure not in STRUCTURES:
+            raise InvalidValueError(f'structure must be "G", "S", "H", "SPD" or "HPD", got {structure!r}')
+        if view not in VIEWS:
+            raise InvalidValueError(f"view must be 'L', 'U' or 'F', got {view!r}")
+        if isinstance(A, tuple):
+            rowptr, colval, nzval, n = A
+            base = 1 if index == "O" else 0
+        else:
+            A = A.tocsr()
+            A.sort_indices()
+            if A.shape[0] != A.shape[1]:
+                raise InvalidValueError(f"the matrix must be square, got {A.shape[0]} × {A.shape[1]}")
+            rowptr, colval, nzval, n = A.indptr + 1, A.indices + 1, A.data, A.shape[0]
+            base, index = 1, "O"
+        self.n = int(n)
+        self.rowptr = np.asarray(rowptr, dtype=np.int64)
+        self.colval = np.asarray(colval, dtype=np.int64)
+        self.index = index
+        self._base = base
+        self.structure = structure
+        self.view = view
+        self.options = _defaults()
+        self.stage = STAGE_NONE
+        self.info = 0
+        self.ir_steps = -1
+        self.symbolic = None
+        self._h = None
+        self._lib = None
+        self._keep = []
+        self._set_values(nzval)
+
+    def _set_values(self, nzval):
+        torch = _torch()
+        if isinstance(nzval, torch.Tensor):
+            if nzval.dtype != torch.float64:
+                raise NotSupportedError("matrix values must be Float64 (factor_precision selects an FP32 factor)")
+            self.nzval = nzval if nzval.is_cuda else nzval.cuda()
+        else:
+            self.nzval = torch.as_tensor(np.ascontiguousarray(nzval, dtype=np.float64)).cuda()
+        if self.nzval.numel() != self.colval.shape[0]:
+            raise InvalidValueError(f"nzval has {self.nzval.numel()} entries, the pattern has {self.colval.shape[0]}")
+
+    def __repr__(self):
+        st = ("created", "reordered", "analyzed", "factorized")[self.stage]
+        return f'DirectSolver(n = {self.n}, nnz = {self.colval.shape[0]}, structure = "{self.structure}", ' \
+               f"view = '{self.view}', {st})"
+
+    def __del__(self):
+        if self._h is not None and self._lib is not None:
+            try:
+                self._lib.sds_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- phases ---------------------------------------------------------------------------------------
+    def _reorder(self):
+        o = self.options
+        if self.structure not in ("SPD", "HPD"):
+            raise NotSupportedError(f'structure "{self.structure}" is not available in this build')
+        if o["schur_mode"] != 0:
+            raise NotSupportedError("schur_mode = 1 is not available in this build")
+        if o["matching_alg"] not in ("default", "none"):
+            raise NotSupportedError("matching is not available in this build")
+        ordering = {"default": "default", "algo3": "algo3", "algo5": "algo5"}.get(o["reordering_alg"])
+        if ordering is None and o["user_perm"] is None:
+            raise NotSupportedError(f"reordering_alg = {o['reordering_alg']!r} is not available in this build")
+        self.symbolic = symbolic_analysis(self.rowptr, self.colval, self.n, self.structure, self.view,
+                                          self.index, perm=o["user_perm"], ordering=ordering or "default",
+                                          amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"])
+        self.stage = STAGE_REORDERED
+
+    def _symbolic_factorization(self):
+        if self.stage < STAGE_REORDERED:
+            raise FactorizationError(0, 'phase "symbolic_factorization": needs "reordering"')
+        lib = hip_lib()
+        if self._h is None:
+            h = ctypes.c_void_p()
+            torch = _torch()
+            stream = torch.cuda.current_stream().cuda_stream
+            _check(lib, None, lib.sds_create(ctypes.byref(h), ctypes.c_void_p(stream)))
+            self._h, self._lib = h, lib
+        S = self.symbolic
+        keep = {k: np.ascontiguousarray(S.maps[k], dtype=np.int64) for k in S.maps}
+        d = _abi.SymbolicDesc()
+        d.n, d.nnz, d.ns = S.n, S.nnz, S.ns
+        d.structure = STRUCTURES[self.structure]
+        d.index_bits = 64
+        for k in ("perm", "super_ptr", "snparent", "rowptr", "rowval", "front_ptr", "front_nrows", "front_ncols",
+                  "child_ptr", "child_list", "relind_ptr", "relind", "amap", "amap_ptr", "amap_src"):
+            setattr(d, k, keep[k].ctypes.data)
+        d.full_nrows = keep["full_rowptr"].shape[0] - 1
+        d.full_nnz = keep["full_colval"].shape[0]
+        d.full_rowptr, d.full_colval, d.full_src = (keep["full_rowptr"].ctypes.data, keep["full_colval"].ctypes.data,
+                                                    keep["full_src"].ctypes.data)
+        _check(lib, self._h, lib.sds_upload_symbolic(self._h, ctypes.byref(d)))
+        fp = self.options["factor_precision"]
+        dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64
+        _check(lib, self._h, lib.sds_alloc_numeric(self._h, dtype, 1))
+        self._dtype = dtype
+        self.stage = STAGE_ANALYZED
+
+    def _factorize(self, phase):
+        if self.stage < STAGE_ANALYZED:
+            raise FactorizationError(0, f'phase "{phase}": needs "analysis"')
+        if phase == "refactorization" and self.stage < STAGE_FACTORIZED:
+            raise FactorizationError(0, 'phase "refactorization": needs a previous "factorization"')
+        o = self.options
+        p = _abi.FactorParams()
+        p.pivot_type = PIVOT_TYPES[o["pivot_type"]]
+        p.pivot_epsilon_alg = _EPS_ALGS[o["pivot_epsilon_alg"]]
+        p.pivot_threshold = float(o["pivot_threshold"])
+        p.pivot_epsilon = -1.0 if o["pivot_epsilon"] is None else float(o["pivot_epsilon"])
+        flag = o["user_host_interrupt"]
+        p.interrupt = ctypes.addressof(flag) if flag is not None else None
+        info = ctypes.c_int64(0)
+        self.info = 0
+        self.stage = STAGE_ANALYZED
+        _check(self._lib, self._h, self._lib.sds_factorize(self._h, ctypes.c_void_p(self.nzval.data_ptr()),
+                                                            ctypes.byref(p), ctypes.byref(info)))
+        self.info = int(info.value)
+        self.stage = STAGE_FACTORIZED
+
+    def _solve(self, phase, X, B):
+        torch = _torch()
+        if self.stage < STAGE_FACTORIZED:
+            raise FactorizationError(0, f'phase "{phase}": needs a "factorization"')
+        bits = PHASES[phase]
+        vec = B.dim() == 1
+        nrhs = 1 if vec else B.shape[1]
+        if B.shape[0] != self.n or X.shape[0] != self.n or (not vec and X.shape[1] != nrhs):
+            raise InvalidValueError(f"A is {self.n}×{self.n}, X is {tuple(X.shape)}, B is {tuple(B.shape)}")
+        # column-major n x nrhs views for the C ABI
+        Bc = B.reshape(self.n, nrhs).t().contiguous().to(torch.float64)
+        Xc = X.reshape(self.n, nrhs).t().contiguous().to(torch.float64)
+        o = self.options
+        steps = ctypes.c_int32(0)
+        nsteps = int(o["ir_n_steps"]) if bits & PHASES["solve_refinement"] else 0
+        _check(self._lib, self._h, self._lib.sds_solve(
+            self._h, bits, int(o["solve_mode"]), ctypes.c_void_p(Bc.data_ptr()), self.n,
+            ctypes.c_void_p(Xc.data_ptr()), self.n, nrhs, ctypes.c_void_p(self.nzval.data_ptr()), nsteps,
+            float(o["ir_tol"]), ctypes.byref(steps)))
+        if bits & PHASES["solve_refinement"]:
+            self.ir_steps = int(steps.value)
+        X.copy_(Xc.t().reshape(X.shape))
+
+
+def execute(phase, solver, X=None, B=None, asynchronous=True):
+    """Run `phase` on `solver` (src/solver.jl:436-460)."""
+    if phase not in PHASES:
+        raise ValueError(f'unknown phase "{phase}"; expected one of {", ".join(repr(p) for p in PHASES)}')
+    if phase == "reordering":
+        solver._reorder()
+    elif phase == "symbolic_factorization":
+        solver._symbolic_factorization()
+    elif phase == "analysis":
+        solver._reorder()
+        solver._symbolic_factorization()
+    elif phase in ("factorization", "refactorization"):
+        solver._factorize(phase)
+    else:
+        solver._solve(phase, X, B)
+    if not asynchronous:
+        _torch().cuda.synchronize()
+
+
+def update(solver, nzval):
+    """Point the solver at new values with the same pattern (src/solver.jl:309-321)."""
+    solver._set_values(nzval)
+
+
+def setparam(solver, name, value):
+    """Set a configuration or user data parameter (src/options.jl:610)."""
+    if name not in CONFIG_PARAMETERS + EXTRA_CONFIG + USER_DATA:
+        if name in DATA_PARAMETERS:
+            raise InvalidValueError(f'"{name}" is a data parameter computed by the solver; it cannot be set')
+        raise ValueError(f'unknown parameter "{name}"')
+    o = solver.options
+    if name == "pivot_type":
+        if value not in PIVOT_TYPES:
+            raise InvalidValueError(f"pivot_type must be one of {', '.join(PIVOT_TYPES)}, got {value!r}")
+        if value in ("C", "R"):
+            raise NotSupportedError("global pivoting ('C', 'R') is not supported")
+    elif name in ("ir_n_steps",) and (not isinstance(value, (int, np.integer)) or value < 0):
+        raise InvalidValueError("ir_n_steps must be a nonnegative integer")
+    elif name in ("ir_tol", "pivot_threshold") and not (float(value) >= 0):
+        raise InvalidValueError(f"{name} must be nonnegative")
+    elif name == "pivot_epsilon" and value is not None and not (float(value) >= 0):
+        raise InvalidValueError("pivot_epsilon must be nonnegative")
+    elif name == "pivot_epsilon_alg" and value not in _EPS_ALGS:
+        raise InvalidValueError(f"pivot_epsilon_alg must be one of {', '.join(_EPS_ALGS)}")
+    elif name == "solve_mode" and value not in (0, 1, 2):
+        raise InvalidValueError("solve_mode must be 0, 1 or 2")
+    elif name == "user_perm" and value is not None:
+        v = np.asarray(value, dtype=np.int64)
+        if v.shape != (solver.n,) or not np.array_equal(np.sort(v), np.arange(1, solver.n + 1)):
+            raise InvalidValueError("user_perm must be a permutation of 1:n")
+        value = v
+    elif name == "amalgamation":
+        mw, zf, mn = value
+        if int(mw) < 1 or not (0 <= float(zf)) or int(mn) < 0:
+            raise InvalidValueError("amalgamation must be (max_width ≥ 1, zero_fraction ≥ 0, min_width ≥ 0)")
+        value = (int(mw), float(zf), int(mn))
+    elif name == "factor_precision" and value not in (None, "Float64", "Float32", np.float32, np.float64):
+        raise InvalidValueError("factor_precision must be Float64 or Float32")
+    o[name] = value
+
+
+def getparam(solver, name):
+    """Read a data or configuration parameter (src/options.jl:648, src/solver.jl getparam)."""
+    if name in solver.options and name not in ("ir_n_steps", "user_perm"):
+        return solver.options[name]
+    if name == "user_perm":
+        return solver.options["user_perm"]
+    if name == "ir_n_steps":
+        return solver.ir_steps
+    if name == "info":
+        return solver.info
+    S = solver.symbolic
+    if name in ("perm_reorder_row", "perm_reorder_col", "perm_row", "perm_col"):
+        if S is None:
+            raise FactorizationError(0, f'"{name}" needs the "reordering" phase')
+        return S.perm.copy()
+    if name == "nsuperpanels":
+        return S.ns
+    if name == "flops":
+        return S.flops
+    if name == "lu_nnz":
+        return S.nnz_stored
+    if solver.stage < STAGE_ANALYZED:
+        raise FactorizationError(0, f'"{name}" needs the analysis')
+    lib, h = solver._lib, solver._h
+    if name == "memory_estimates":
+        est = np.zeros(16, dtype=np.int64)
+        _check(lib, h, lib.sds_memory_estimates(h, est.ctypes.data))
+        return est
+    if solver.stage < STAGE_FACTORIZED:
+        raise FactorizationError(0, f'"{name}" needs a "factorization"')
+    if name in ("inertia", "npivots", "pivot_stats"):
+        tot = np.zeros(6, dtype=np.int64)
+        _check(lib, h, lib.sds_get_stats(h, None, tot.ctypes.data))
+        if name == "inertia":
+            return (int(tot[0]), int(tot[1]))
+        if name == "npivots":
+            return int(tot[3])
+        return dict(npos=int(tot[0]), nneg=int(tot[1]), nzero=int(tot[2]), nperturbed=int(tot[3]), n2x2=int(tot[4]))
+    if name == "diag":
+        d = np.zeros(2 * solver.n, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
+        _check(lib, h, lib.sds_get_diag(h, d.ctypes.data))
+        return d[:solver.n].astype(np.float64)
+    raise ValueError(f'unknown parameter "{name}"')
+
+
+# test and debugging glue (the reference's extract.jl host copies)
+def numeric_factor(solver, which=0):
+    """Host copy of the factor panels."""
+    S = solver.symbolic
+    a = np.zeros(S.factor_len, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
+    _check(solver._lib, solver._h, solver._lib.sds_get_factor(solver._h, which, a.ctypes.data))
+    return a
+
+
+def numeric_stats(solver):
+    st = np.zeros(6 * solver.symbolic.ns, dtype=np.int64)
+    tot = np.zeros(6, dtype=np.int64)
+    _check(solver._lib, solver._h, solver._lib.sds_get_stats(solver._h, st.ctypes.data, tot.ctypes.data))
+    return st, tot

@@ -1,0 +1,118 @@
+"""GPU Cholesky (C ABI of libsds_hip) against the CPU oracle.
+
+Contract of the reference's device-vs-oracle tests (test/test_numeric_cholesky_a.jl:69-94,
+test/utils.jl:230, :377): panel error max|L_gpu - L_ref| / max|L_ref| <= 100 eps, per-front
+statistics equal, solution relative residual <= sqrt(eps), and a second factorization
+bitwise identical to the first.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sparsedirectsolver as sds
+from sparsedirectsolver import matrices as M
+from sparsedirectsolver.solver import numeric_factor, numeric_stats
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _solver(A, perm=None, amalgamation=(32, 0.25, 8), fp=None):
+    L = M.lower_csr(A)
+    rp, cv, nz = M.csr_arrays(L)
+    s = sds.DirectSolver((rp, cv, nz, A.shape[0]), "SPD", "L")
+    if perm is not None:
+        sds.setparam(s, "user_perm", perm)
+    sds.setparam(s, "amalgamation", amalgamation)
+    if fp:
+        sds.setparam(s, "factor_precision", fp)
+    sds.execute("analysis", s)
+    return s, nz
+
+
+CASES = [
+    ("lap2d_10x10_natural", lambda: (M.laplacian2d(10, 10), None, (32, 0.25, 8))),
+    ("lap2d_30x20_nd", lambda: (M.laplacian2d(30, 20), sds.grid_nd_perm(30, 20, 1), (32, 0.25, 8))),
+    ("lap3d_12_nd", lambda: (M.laplacian3d(12, 12, 12), sds.grid_nd_perm(12, 12, 12), (32, 0.25, 8))),
+    ("lap3d_16_nd_wide", lambda: (M.laplacian3d(16, 16, 16), sds.grid_nd_perm(16, 16, 16), (512, 0.25, 8))),
+    ("varlap3d_14_nd", lambda: (M.variable_laplacian((14, 13, 12), seed=3), sds.grid_nd_perm(14, 13, 12),
+                                (128, 0.25, 8))),
+    ("lap3d_20_natural", lambda: (M.laplacian3d(20, 20, 20), None, (64, 0.25, 8))),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_factor_matches_oracle(name, make):
+    import torch
+    A, perm, amal = make()
+    print(f"case {name} n={A.shape[0]}", flush=True)
+    s, nz = _solver(A, perm, amal)
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 0
+    info, ref, st_ref = oracle.cholesky(s.symbolic, nz)
+    assert info == 0
+    got = numeric_factor(s)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 100 * EPS, f"panel error {err:.3e}"
+    st, tot = numeric_stats(s)
+    assert np.array_equal(st, st_ref)
+    # determinism: a refactorization gives bitwise identical panels
+    sds.execute("refactorization", s)
+    assert np.array_equal(numeric_factor(s), got)
+    # solve, several right-hand sides
+    n = A.shape[0]
+    rng = np.random.default_rng(666)
+    B = torch.tensor(rng.standard_normal((n, 3)), dtype=torch.float64, device="cuda")
+    X = torch.zeros_like(B)
+    sds.execute("solve", s, X, B)
+    Xh, Bh = X.cpu().numpy(), B.cpu().numpy()
+    Xr = oracle.solve_cholesky(s.symbolic, ref, Bh)
+    assert np.abs(Xh - Xr).max() / np.abs(Xr).max() <= 1e-10
+    relres = np.linalg.norm(A @ Xh - Bh, axis=0) / np.maximum(np.linalg.norm(Bh, axis=0), 1)
+    assert relres.max() <= np.sqrt(EPS)
+
+
+def test_failed_pivot_info():
+    """[4 0 0 0; 0 4 0 0; 0 0 1 2; 0 0 2 1] has its first non-positive pivot at column 4
+    (test/test_reference_cholesky.jl:121-124)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(np.array([[4, 0, 0, 0], [0, 4, 0, 0], [0, 0, 1, 2], [0, 0, 2, 1]], dtype=float))
+    s, nz = _solver(A, perm=np.arange(1, 5))
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 4
+    info, _, _ = oracle.cholesky(s.symbolic, nz)
+    assert info == 4
+
+
+def test_sub_phases_equal_solve():
+    import torch
+    A = M.laplacian3d(10, 9, 8)
+    s, nz = _solver(A, sds.grid_nd_perm(10, 9, 8), (64, 0.25, 8))
+    sds.execute("factorization", s)
+    n = A.shape[0]
+    B = torch.tensor(np.random.default_rng(1).standard_normal((n, 2)), dtype=torch.float64, device="cuda")
+    X1 = torch.zeros_like(B)
+    sds.execute("solve", s, X1, B)
+    X2 = torch.zeros_like(B)
+    for ph in ("solve_fwd_perm", "solve_fwd", "solve_diag", "solve_bwd", "solve_bwd_perm"):
+        sds.execute(ph, s, X2, B)
+    assert torch.equal(X1, X2)
+
+
+def test_refinement_fp64_and_fp32():
+    import torch
+    A = M.laplacian3d(16, 16, 16)
+    perm = sds.grid_nd_perm(16, 16, 16)
+    n = A.shape[0]
+    B = torch.tensor(np.random.default_rng(2).standard_normal((n, 8)), dtype=torch.float64, device="cuda")
+    Bh = B.cpu().numpy()
+    for fp, steps, tol in ((None, 2, 1e-14), ("Float32", 6, 1e-10)):
+        s, nz = _solver(A, perm, (512, 0.25, 8), fp=fp)
+        sds.execute("factorization", s)
+        sds.setparam(s, "ir_n_steps", steps)
+        X = torch.zeros_like(B)
+        sds.execute("solve", s, X, B)
+        Xh = X.cpu().numpy()
+        berr = np.abs(A @ Xh - Bh).max(axis=0) / (abs(A).sum(axis=1).max() * np.abs(Xh).max(axis=0) +
+                                                  np.abs(Bh).max(axis=0))
+        assert berr.max() <= tol, (fp, berr.max())
