@@ -1,0 +1,216 @@
+"""Benchmark of the numeric phase on one MI355X.
+
+Workload (one "step" = refactorization + solve with 8 right-hand sides):
+3-D 7-point Laplacian on a 64^3 grid (n = 262,144), SPD, Float64 values,
+geometric nested dissection, amalgamation (512, 0.25, 8), B ~ N(0, 1) n x 8.
+Two forms are timed: the native FP64 factor + solve, and the FP32 factor with
+FP64 iterative refinement (the residual and the update of x in FP64) run for
+the number of steps that reaches a normwise backward error <= 1e-10.
+
+`value` is the FP64 form's refactorize+solve milliseconds per step (lower is
+better); the FP32+IR form, factor TFLOP/s, backward errors, the dominant
+kernel's roofline (HIP events inside the library, on the solver's stream) and
+a CPU baseline (the C oracle on a bounded sample, see `cpu_baseline.sample`)
+are on the same JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sparsedirectsolver.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
+MFMA_KINDS = ("update_panel", "update_cb", "front_small", "potrf", "trsm")
+
+
+def backward_error(A, X, B):
+    """max over right-hand sides of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
+    R = B - A @ X
+    an = abs(A).sum(axis=1).max()
+    return float(np.max(np.abs(R).max(axis=0) / (an * np.abs(X).max(axis=0) + np.abs(B).max(axis=0))))
+
+
+def make_workload(k, nrhs, seed=666):
+    import sparsedirectsolver as sds
+    from sparsedirectsolver import matrices as M
+    A = M.laplacian3d(k, k, k)
+    L = M.lower_csr(A)
+    rp, cv, nz = M.csr_arrays(L)
+    perm = sds.grid_nd_perm(k, k, k)
+    B = np.random.default_rng(seed).standard_normal((A.shape[0], nrhs))
+    return A, (rp, cv, nz), perm, B
+
+
+def make_solver(csr, n, perm, amalgamation, factor_precision=None):
+    import sparsedirectsolver as sds
+    rp, cv, nz = csr
+    s = sds.DirectSolver((rp, cv, nz, n), "SPD", "L")
+    sds.setparam(s, "user_perm", perm)
+    sds.setparam(s, "amalgamation", amalgamation)
+    if factor_precision:
+        sds.setparam(s, "factor_precision", factor_precision)
+    sds.execute("analysis", s)
+    sds.execute("factorization", s)
+    return s
+
+
+def time_steps(s, X, B, steps, warmup, ir_steps):
+    """Median-free total timing of `steps` refactorization+solve steps (after `warmup`)."""
+    import torch
+    import sparsedirectsolver as sds
+    sds.setparam(s, "ir_n_steps", ir_steps)
+    for _ in range(warmup):
+        sds.execute("refactorization", s)
+        sds.execute("solve", s, X, B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sds.execute("refactorization", s)
+        sds.execute("solve", s, X, B)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def time_phase(s, steps, fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def cpu_baseline(S, nz, factor_gpu, B, budget_flops):
+    """The C oracle (a single-threaded restatement of src/reference/cholesky.jl) on the host:
+    the factorization of the postorder prefix whose flops stay under `budget_flops` (a union
+    of whole subtrees), scaled to the whole tree by flops; plus the full oracle solve with all
+    right-hand sides on the same factor."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    f = S.front_nrows.astype(np.float64)
+    w = S.front_ncols.astype(np.float64)
+    fl = np.array([np.sum((fi - np.arange(wi)) ** 2) for fi, wi in zip(f, w)])
+    post = S.snpost - 1
+    cum = np.cumsum(fl[post])
+    q1 = int(np.searchsorted(cum, budget_flops, side="right"))
+    q1 = max(q1, 1)
+    t0 = time.perf_counter()
+    oracle.cholesky_prefix(S, nz, q1)
+    t_sample = time.perf_counter() - t0
+    sample_flops = float(cum[q1 - 1])
+    t_factor = t_sample * fl.sum() / sample_flops
+    t0 = time.perf_counter()
+    oracle.solve_cholesky(S, factor_gpu, B)
+    t_solve = time.perf_counter() - t0
+    return {
+        "value": round((t_factor + t_solve) * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+        "sample": (f"oracle factorization of the first {q1} of {S.ns} supernodes in postorder "
+                   f"({sample_flops:.3e} of {fl.sum():.3e} flops, {t_sample:.2f} s, "
+                   f"{sample_flops / t_sample / 1e9:.2f} GFLOP/s) scaled by flops to the whole tree, "
+                   f"plus the full oracle solve with {B.shape[1]} right-hand sides ({t_solve:.2f} s)"),
+        "factor_ms_est": round(t_factor * 1e3, 1), "solve_ms": round(t_solve * 1e3, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--nrhs", type=int, default=8)
+    ap.add_argument("--cpu-budget", type=float, default=5e10, help="flops of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the numeric phase of one "
+              "system runs on one MI355X, and multi-GPU sharding of the tree is not built).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import sparsedirectsolver as sds
+    from sparsedirectsolver.solver import numeric_factor, profile, profile_read
+
+    amal = (512, 0.25, 8)
+    A, csr, perm, Bh = make_workload(args.grid, args.nrhs)
+    n = A.shape[0]
+    B = torch.tensor(Bh, dtype=torch.float64, device="cuda")
+    X = torch.zeros_like(B)
+
+    # -- FP64 form
+    s64 = make_solver(csr, n, perm, amal)
+    S = s64.symbolic
+    t_step64 = time_steps(s64, X, B, args.steps, args.warmup, 0)
+    berr64 = backward_error(A, X.cpu().numpy(), Bh)
+    # profiled pass over the same steps: per-kernel-class HIP events on the solver's stream
+    profile(s64, True)
+    t_refact64 = time_phase(s64, args.steps, lambda: sds.execute("refactorization", s64))
+    t_solve64 = time_phase(s64, args.steps, lambda: sds.execute("solve", s64, X, B))
+    prof = profile_read(s64)
+    profile(s64, False)
+    flops_true = S.flops_true
+    res = {
+        "metric": "refactor+solve ms & factor TFLOP/s, FP32+IR vs FP64 at ≤1e-10 backward err",
+        "value": round(t_step64, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_step64, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3-D 7-point Laplacian {args.grid}^3 (n = {n}), SPD Cholesky, "
+                               f"{args.nrhs} right-hand sides, refactorization + solve",
+                   "ordering": "geometric nested dissection", "amalgamation": list(amal), "nrhs": args.nrhs,
+                   "nnz_L": S.nnz_L, "nnz_stored": S.nnz_stored, "supernodes": S.ns,
+                   "factor_flops": flops_true, "parallelism": "single GPU"},
+        "fp64": {"refactor_ms": round(t_refact64, 3), "solve_ms": round(t_solve64, 3),
+                 "factor_tflops": round(flops_true / (t_refact64 * 1e-3) / 1e12, 3),
+                 "backward_error": berr64},
+    }
+    # dominant kernel class of the profiled region and its roofline
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, nl, fl, by = prof[dom]
+    if dom in MFMA_KINDS and fl > 0:
+        ach = fl / (ms * 1e-3) / 1e12
+        res["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp64_tflops"],
+                           "unit": "TFLOP/s", "frac": round(ach / PEAK["fp64_tflops"], 4), "traffic": None,
+                           "launches": nl, "avg_launch_ms": round(ms / nl, 4)}
+    else:
+        ach = by / (ms * 1e-3) / 1e9
+        res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK["hbm_gbs"],
+                           "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4), "traffic": None,
+                           "launches": nl, "avg_launch_ms": round(ms / nl, 4)}
+    res["profile_ms"] = {k: round(v[0] / args.steps, 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+
+    # -- FP32 factor + FP64 refinement
+    if not args.no_fp32:
+        s32 = make_solver(csr, n, perm, amal, "Float32")
+        steps_needed = None
+        for k in range(1, 11):
+            sds.setparam(s32, "ir_n_steps", k)
+            sds.execute("solve", s32, X, B)
+            if backward_error(A, X.cpu().numpy(), Bh) <= 1e-10:
+                steps_needed = k
+                break
+        if steps_needed is not None:
+            t_step32 = time_steps(s32, X, B, args.steps, args.warmup, steps_needed)
+            berr32 = backward_error(A, X.cpu().numpy(), Bh)
+            t_refact32 = time_phase(s32, args.steps, lambda: sds.execute("refactorization", s32))
+            res["fp32_ir"] = {"ms_per_step": round(t_step32, 3), "ir_steps": steps_needed,
+                              "refactor_ms": round(t_refact32, 3),
+                              "factor_tflops": round(flops_true / (t_refact32 * 1e-3) / 1e12, 3),
+                              "backward_error": berr32, "speedup_vs_fp64": round(t_step64 / t_step32, 3)}
+        else:
+            res["fp32_ir"] = {"ms_per_step": None, "note": "backward error 1e-10 not reached in 10 refinement steps"}
+        del s32
+
+    # -- CPU baseline (rank 0, N = 1)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(S, csr[2], numeric_factor(s64), Bh, args.cpu_budget)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

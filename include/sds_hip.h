@@ -166,6 +166,33 @@ int sds_get_factor(sds_handle_t handle, int32_t which, void* out);
 /* "memory_estimates" (src/symbolic/maps.jl:338-356): 16 Int64, bytes */
 int sds_memory_estimates(sds_handle_t handle, int64_t* est);
 
+/*
+ * Per-kernel-class timing. No counterpart in the reference, which profiles from
+ * outside (CUPTI traces of bench/profile_phases.jl). With profiling on, every
+ * launch of the phases is bracketed by HIP events on the handle's stream and
+ * its algorithmic flops and bytes are recorded; sds_profile_read() waits for
+ * the recorded launches and returns the totals of one class since
+ * sds_profile(h, 1) (which clears them). sds_profile(h, 0) stops recording.
+ */
+enum {
+    SDS_K_FRONT_SMALL = 0, /* fused small-front assembly + factorization */
+    SDS_K_ZERO_SCATTER,    /* large fronts: zero + assembly of A */
+    SDS_K_EXTEND_ADD,      /* large fronts: extend-add of one child rank */
+    SDS_K_POTRF,           /* large fronts: diagonal-block Cholesky */
+    SDS_K_TRSM,            /* large fronts: panel triangular solve */
+    SDS_K_UPDATE_PANEL,    /* large fronts: Schur update of the panel columns */
+    SDS_K_UPDATE_CB,       /* large fronts: contribution-block update (K = w) */
+    SDS_K_STATS,           /* statistics / info reduction */
+    SDS_K_PERMUTE,         /* right-hand-side permutation in / out */
+    SDS_K_FWD,             /* forward sweep (one launch per tree level) */
+    SDS_K_BWD,             /* backward sweep (one launch per tree level) */
+    SDS_K_DIAG,            /* D^-1 */
+    SDS_K_RESIDUAL,        /* refinement residual (FP64 SpMV) */
+    SDS_K_COUNT
+};
+int sds_profile(sds_handle_t handle, int32_t enable);
+int sds_profile_read(sds_handle_t handle, int32_t kind, double* ms, int64_t* launches, double* flops, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

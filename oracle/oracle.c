@@ -114,7 +114,19 @@ static int64_t potrf_lower(double *F, int64_t f, int64_t w) {
     return 0;
 }
 
+static int64_t cholesky_range(const oracle_sym *S, const double *nz, double *factor, int64_t *stats, int64_t q1);
+
 int64_t oracle_cholesky(const oracle_sym *S, const double *nz, double *factor, int64_t *stats) {
+    return cholesky_range(S, nz, factor, stats, S->ns);
+}
+
+/* the first q1 supernodes of the postorder only (a union of whole subtrees): the bounded
+ * sample the CPU baseline times */
+int64_t oracle_cholesky_prefix(const oracle_sym *S, const double *nz, double *factor, int64_t *stats, int64_t q1) {
+    return cholesky_range(S, nz, factor, stats, q1 < S->ns ? q1 : S->ns);
+}
+
+static int64_t cholesky_range(const oracle_sym *S, const double *nz, double *factor, int64_t *stats, int64_t q1) {
     int64_t ns = S->ns;
     memset(factor, 0, sizeof(double) * (size_t)S->factor_len);
     memset(stats, 0, sizeof(int64_t) * (size_t)(NF * ns));
@@ -124,7 +136,7 @@ int64_t oracle_cholesky(const oracle_sym *S, const double *nz, double *factor, i
     for (int64_t s = 0; s < ns; ++s) fmax = S->front_nrows[s] > fmax ? S->front_nrows[s] : fmax;
     double *F = (double *)malloc(sizeof(double) * (size_t)(fmax * fmax + 1));
     int64_t ret = 0;
-    for (int64_t q = 0; q < ns; ++q) {
+    for (int64_t q = 0; q < q1; ++q) {
         int64_t s = S->snpost[q] - 1;
         int64_t f = S->front_nrows[s], w = S->front_ncols[s], m = f - w, base = S->front_ptr[s] - 1;
         assemble(S, s, nz, F, &cbs, 0, 0);

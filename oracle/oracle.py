@@ -43,6 +43,8 @@ def lib():
         vp = ctypes.c_void_p
         L.oracle_cholesky.restype = ctypes.c_int64
         L.oracle_cholesky.argtypes = [vp, vp, vp, vp]
+        L.oracle_cholesky_prefix.restype = ctypes.c_int64
+        L.oracle_cholesky_prefix.argtypes = [vp, vp, vp, vp, ctypes.c_int64]
         L.oracle_solve_cholesky.argtypes = [vp, vp, vp, vp, ctypes.c_int64]
         L.oracle_ldlt.restype = ctypes.c_int64
         L.oracle_ldlt.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp, vp, vp]
@@ -92,6 +94,17 @@ def cholesky(S, nzval):
     factor = np.zeros(S.factor_len)
     stats = np.zeros(6 * S.ns, dtype=np.int64)
     info = lib().oracle_cholesky(o.ptr, _p(nz), _p(factor), _p(stats))
+    return int(info), factor, stats
+
+
+def cholesky_prefix(S, nzval, q1):
+    """The oracle Cholesky on the first q1 supernodes of the postorder only (a union of whole
+    subtrees); used to time a bounded sample of a large workload."""
+    o = OracleSym(S)
+    nz = np.ascontiguousarray(nzval, dtype=np.float64)
+    factor = np.zeros(S.factor_len)
+    stats = np.zeros(6 * S.ns, dtype=np.int64)
+    info = lib().oracle_cholesky_prefix(o.ptr, _p(nz), _p(factor), _p(stats), int(q1))
     return int(info), factor, stats
 
 

@@ -518,7 +518,9 @@ V amd_order(const Pattern& P) {
             }
             adj[v] = std::move(nadj);
             elen[v] = ne;
-            i64 d = std::min(degree[v], (lme_size - nv[v]) + deg_e + deg_v);
+            // approximate external degree (Amestoy-Davis-Duff): the smallest of the old degree
+            // grown by the new element, the element-sum bound, and the variables left
+            i64 d = std::min(degree[v] + (lme_size - nv[v]), (lme_size - nv[v]) + deg_e + deg_v);
             d = std::min(d, (n - nel) - nv[v]);
             degree[v] = std::max<i64>(d, 0);
         }

@@ -185,6 +185,217 @@ __global__ void __launch_bounds__(256) k_bwd(DevSym S, const int32_t* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// large fronts: the sweeps of one front are split into NB-row diagonal blocks
+// and run as level-batched launches with many workgroups per front.
+
+// solve the diagonal block [k, k + nb) in LDS: xs[i * nrhs + r] <- L_bb^-1 xs (lower = 1) or
+// L_bb^-T xs (lower = 0); Lb[j][i] = L(k + i, k + j). One wave per right-hand side group,
+// lane i holds row i; the pivot value moves between lanes by shuffles (no barriers).
+template <typename T>
+__device__ void block_solve(const T (*Lb)[NB + 1], T* xs, int nb, int nrhs, int unit, int lower) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r = wv; r < nrhs; r += nw) {
+        T v = lane < nb ? xs[lane * nrhs + r] : T(0);
+        if (lower) {
+            for (int j = 0; j < nb; ++j) {
+                T xj = __shfl(v, j);
+                if (!unit) xj /= Lb[j][j];
+                if (lane == j) v = xj;
+                if (lane > j && lane < nb) v -= Lb[j][lane] * xj;
+            }
+        } else {
+            for (int j = nb - 1; j >= 0; --j) {
+                T xj = __shfl(v, j);
+                if (!unit) xj /= Lb[j][j];
+                if (lane == j) v = xj;
+                if (lane < j) v -= Lb[lane][j] * xj;
+            }
+        }
+        if (lane < nb) xs[lane * nrhs + r] = v;
+    }
+}
+
+// forward, before the block steps: zero the update vector, pull the children's update
+// vectors (child_list order), then apply the local pivot order (piv) through Z.
+// task: (s); one workgroup per front
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_pull(DevSym S, const Task* __restrict__ tasks, T* __restrict__ Y,
+                                                 T* __restrict__ U, T* __restrict__ Z, int nrhs,
+                                                 const int32_t* __restrict__ piv) {
+    const int s = tasks[blockIdx.x].s;
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    if (us)
+        for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
+    __syncthreads();
+    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
+        const int c = S.child_list[k];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        for (int q = tid; q < mc * nrhs; q += nt) {
+            const int ii = q / nrhs, r = q - ii * nrhs;
+            const int pos = ri[ii];
+            if (pos < w)
+                Y[(int64_t)(c0 + pos) * nrhs + r] -= uc[q];
+            else
+                us[(pos - w) * nrhs + r] += uc[q];
+        }
+        __syncthreads();
+    }
+    if (piv) {
+        T* z = Z + (int64_t)c0 * nrhs;
+        for (int q = tid; q < w * nrhs; q += nt) {
+            const int kk = q / nrhs, r = q - kk * nrhs;
+            z[q] = Y[(int64_t)piv[c0 + kk] * nrhs + r];
+        }
+        __syncthreads();
+        for (int q = tid; q < w * nrhs; q += nt) Y[(int64_t)c0 * nrhs + q] = z[q];
+    }
+}
+
+// forward block step: every workgroup solves the diagonal block [k, k + nb) (from Y) and
+// applies it to its tile of 256 rows below: rows < w of Y, rows >= w of the update vector.
+// The solved block goes to Z (written by the task with c = 1). task: (s, k, row0, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_step(DevSym S, const Task* __restrict__ tasks,
+                                                 const T* __restrict__ factor, T* __restrict__ Y, T* __restrict__ U,
+                                                 T* __restrict__ Z, int nrhs, int unit) {
+    __shared__ T Lb[NB][NB + 1];
+    __shared__ T xs[NB * MAX_NRHS];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
+    const int nb = min(NB, w - k);
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    const T* P = factor + S.panel_off[s];
+    for (int q = tid; q < nb * nb; q += 256) {
+        const int j = q / nb, i = q - j * nb;
+        Lb[j][i] = P[(int64_t)(k + j) * f + k + i];
+    }
+    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = Y[(int64_t)(c0 + k) * nrhs + q];
+    __syncthreads();
+    block_solve<T>(Lb, xs, nb, nrhs, unit, 1);
+    __syncthreads();
+    if (t.c)
+        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = xs[q];
+    const int i = row0 + tid;
+    if (i < min(row0 + 256, f)) {
+        T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+        for (int rb = 0; rb < nrhs; rb += 8) {
+            T acc[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] = T(0);
+            for (int j = 0; j < nb; ++j) {
+                const T l = P[(int64_t)(k + j) * f + i];
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if (rb + r < nrhs) acc[r] += l * xs[j * nrhs + rb + r];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                if (rb + r >= nrhs) continue;
+                if (i < w)
+                    Y[(int64_t)(c0 + i) * nrhs + rb + r] -= acc[r];
+                else if (us)
+                    us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
+            }
+        }
+    }
+}
+
+// after the block steps of a level: the solved values of the fronts' own columns from Z
+// to Y (fwd), or to their pivot positions (bwd, piv). task: (s, col0)
+template <typename T>
+__global__ void __launch_bounds__(256) k_finish(DevSym S, const Task* __restrict__ tasks, T* __restrict__ Y,
+                                               const T* __restrict__ Z, int nrhs, const int32_t* __restrict__ piv) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, w = S.fncols[s], c0 = S.super_ptr[s];
+    const int j1 = min(t.a + 256, w);
+    for (int q = threadIdx.x; q < (j1 - t.a) * nrhs; q += 256) {
+        const int kk = t.a + q / nrhs, r = q % nrhs;
+        const int dst = piv ? piv[c0 + kk] : c0 + kk;
+        Y[(int64_t)dst * nrhs + r] = Z[(int64_t)(c0 + kk) * nrhs + r];
+    }
+}
+
+// backward, before the block steps: y_j -= sum_{i >= w} L[i, j] y[rows[i]] for the columns
+// [col0, col0 + 64) of the front; one wave per column. task: (s, col0)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restrict__ tasks,
+                                                const T* __restrict__ factor, T* __restrict__ Y, int nrhs) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const T* P = factor + S.panel_off[s];
+    const int32_t* rows = S.rowval + S.rowptr[s];
+    for (int j = t.a + wv; j < min(t.a + 64, w); j += 4) {
+        for (int rb = 0; rb < nrhs; rb += 8) {
+            T acc[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] = T(0);
+            for (int i = w + lane; i < f; i += 64) {
+                const T l = P[(int64_t)j * f + i];
+                const T* yr = Y + (int64_t)rows[i] * nrhs + rb;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if (rb + r < nrhs) acc[r] += l * yr[r];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                T v = acc[r];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                if (lane == 0 && rb + r < nrhs) Y[(int64_t)(c0 + j) * nrhs + rb + r] -= v;
+            }
+        }
+    }
+}
+
+// backward block step: every workgroup solves the diagonal block [k, k + nb) with L_bb^T
+// (from Y) and applies it to its tile of 64 earlier columns [col0, col0 + 64) ∩ [0, k):
+// y_j -= sum_{i in block} L[i, j] x_i. The solved block goes to Z (task with c = 1).
+// task: (s, k, col0, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_step(DevSym S, const Task* __restrict__ tasks,
+                                                 const T* __restrict__ factor, T* __restrict__ Y, T* __restrict__ Z,
+                                                 int nrhs, int unit) {
+    __shared__ T Lb[NB][NB + 1];
+    __shared__ T Lt[NB][NB + 1];  // Lt[jj][i] = L(k + i, col0 + jj)
+    __shared__ T xs[NB * MAX_NRHS];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, col0 = t.b;
+    const int nb = min(NB, w - k);
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    const T* P = factor + S.panel_off[s];
+    const int ncol = max(0, min(col0 + 64, k) - col0);
+    for (int q = tid; q < nb * nb; q += 256) {
+        const int j = q / nb, i = q - j * nb;
+        Lb[j][i] = P[(int64_t)(k + j) * f + k + i];
+    }
+    for (int q = tid; q < ncol * nb; q += 256) {
+        const int jj = q / nb, i = q - jj * nb;
+        Lt[jj][i] = P[(int64_t)(col0 + jj) * f + k + i];
+    }
+    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = Y[(int64_t)(c0 + k) * nrhs + q];
+    __syncthreads();
+    block_solve<T>(Lb, xs, nb, nrhs, unit, 0);
+    __syncthreads();
+    if (t.c)
+        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = xs[q];
+    const int jj = tid & 63;
+    if (jj < ncol)
+        for (int r = tid >> 6; r < nrhs; r += 4) {
+            T acc = T(0);
+            for (int i = 0; i < nb; ++i) acc += Lt[jj][i] * xs[i * nrhs + r];
+            Y[(int64_t)(c0 + col0 + jj) * nrhs + r] -= acc;
+        }
+}
+
 // Y <- D^-1 Y with 1x1 and 2x2 blocks (src/solve/sweeps.jl:549-570); one thread per (row, rhs)
 template <typename T>
 __global__ void k_diag(int n, int nrhs, const T* __restrict__ d, const int8_t* __restrict__ kind, T* __restrict__ Y) {
@@ -253,6 +464,35 @@ void launch_bwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfro
 }
 
 template <typename T>
+void launch_fwd_pull(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, T* U, T* Z, int nrhs,
+                     const int32_t* piv) {
+    hipLaunchKernelGGL(k_fwd_pull<T>, dim3(n), dim3(256), 0, st, S, tasks, Y, U, Z, nrhs, piv);
+}
+
+template <typename T>
+void launch_fwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* U, T* Z,
+                     int nrhs, int unit) {
+    hipLaunchKernelGGL(k_fwd_step<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, U, Z, nrhs, unit);
+}
+
+template <typename T>
+void launch_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, const T* Z, int nrhs,
+                   const int32_t* piv) {
+    hipLaunchKernelGGL(k_finish<T>, dim3(n), dim3(256), 0, st, S, tasks, Y, Z, nrhs, piv);
+}
+
+template <typename T>
+void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, int nrhs) {
+    hipLaunchKernelGGL(k_bwd_l21<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, nrhs);
+}
+
+template <typename T>
+void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
+                     int unit) {
+    hipLaunchKernelGGL(k_bwd_step<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, Z, nrhs, unit);
+}
+
+template <typename T>
 void launch_diag(hipStream_t st, int n, int nrhs, const T* d, const int8_t* kind, T* Y) {
     hipLaunchKernelGGL(k_diag<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, d, kind, Y);
 }
@@ -270,7 +510,12 @@ void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, con
                                 const int32_t*);                                                                    \
     template void launch_bwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, int, int,        \
                                 const int32_t*);                                                                    \
-    template void launch_diag<T>(hipStream_t, int, int, const T*, const int8_t*, T*);
+    template void launch_diag<T>(hipStream_t, int, int, const T*, const int8_t*, T*);                              \
+    template void launch_fwd_pull<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, T*, int, const int32_t*); \
+    template void launch_fwd_step<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, T*, T*, int, int);  \
+    template void launch_finish<T>(hipStream_t, const DevSym&, const Task*, int, T*, const T*, int, const int32_t*); \
+    template void launch_bwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, int);               \
+    template void launch_bwd_step<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, T*, int, int);
 INST(double)
 INST(float)
 #undef INST

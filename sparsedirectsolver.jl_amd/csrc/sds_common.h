@@ -42,6 +42,7 @@ enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4
 constexpr int NB = 64;     // pivot block of the blocked front factorization
 constexpr int TILE = 64;   // Schur-update tile
 constexpr int NF = 6;      // per-front statistics fields
+constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
 
 #define SDS_HIP_CHECK(x)                                      \
     do {                                                      \

@@ -31,20 +31,59 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// launch kinds of the large-front plan
-enum LaunchKind { L_ZERO_SCATTER, L_EXTEND_ADD, L_POTRF, L_TRSM, L_UPDATE_PANEL, L_UPDATE_CB };
+// launch kinds of the large-front plans
+enum LaunchKind {
+    L_ZERO_SCATTER,
+    L_EXTEND_ADD,
+    L_POTRF,
+    L_TRSM,
+    L_UPDATE_PANEL,
+    L_UPDATE_CB,
+    S_FWD_PULL,
+    S_FWD_STEP,
+    S_FWD_FINISH,
+    S_BWD_L21,
+    S_BWD_STEP,
+    S_BWD_FINISH
+};
+
+// a front the solve gives one workgroup (no more than one diagonal block, moderate height)
+inline bool solve_small(int f, int w) { return w <= NB && f <= 2048; }
 
 struct Launch {
     int kind;
     int64_t off;
     int n;
+    double flops, bytes;  // algorithmic work of the launch
 };
 
 struct Level {
     std::vector<int32_t> small, big, all;
+    std::vector<int32_t> ssmall;        // fronts the solve handles one workgroup each
     int fmax_small = 0;
-    int64_t off_small = 0, off_all = 0;  // offsets into the device front lists
+    int64_t off_small = 0, off_all = 0, off_ssmall = 0;  // offsets into the device front lists
     std::vector<Launch> plan;           // large-front launches of this level
+    std::vector<Launch> fwd_plan, bwd_plan;  // large-front solve launches (per right-hand side work)
+    double small_flops = 0, small_bytes = 0;
+    double fwd_bytes = 0, fwd_flops = 0;  // factor bytes read by a sweep, flops per right-hand side
+    double rhs_rows = 0;                  // rows of the level's fronts (right-hand-side entries touched)
+};
+
+// plan launch kind -> profiling kind of include/sds_hip.h
+int prof_kind(int kind) {
+    switch (kind) {
+        case L_ZERO_SCATTER: return SDS_K_ZERO_SCATTER;
+        case L_EXTEND_ADD: return SDS_K_EXTEND_ADD;
+        case L_POTRF: return SDS_K_POTRF;
+        case L_TRSM: return SDS_K_TRSM;
+        case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
+        default: return SDS_K_UPDATE_CB;
+    }
+}
+
+struct ProfRec {
+    int kind, e0, e1;
+    double flops, bytes;
 };
 
 }  // namespace
@@ -61,7 +100,7 @@ struct sds_handle {
     int64_t n = 0, nnz = 0, ns = 0, factor_len = 0;
     int structure = 3;
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
-    std::vector<int64_t> panel_off;
+    std::vector<int64_t> panel_off, asm_count;
     std::vector<DevBuf> symbufs;  // device maps
     DevSym dsym{};
     int64_t full_nrows = 0;
@@ -89,6 +128,14 @@ struct sds_handle {
     DevBuf Y, Z, U, R, Bc, norms;
     double* h_norms = nullptr;  // pinned
     int fsmall = 0;
+    // profiling (HIP events around every launch on the handle's stream)
+    bool prof = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evnext = 0;
+    std::vector<ProfRec> recs;
+    double acc_ms[SDS_K_COUNT] = {}, acc_flops[SDS_K_COUNT] = {}, acc_bytes[SDS_K_COUNT] = {};
+    int64_t acc_n[SDS_K_COUNT] = {};
+    int64_t nnz_full = 0;
 };
 
 namespace {
@@ -120,6 +167,49 @@ int dev_alloc(sds_handle* h, std::vector<DevBuf>& owner, size_t bytes, void** ou
 void free_all(std::vector<DevBuf>& bufs) {
     for (auto& b : bufs) (void)hipFree(b.p);
     bufs.clear();
+}
+
+int next_event(sds_handle* h) {
+    if (h->evnext == h->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        h->evpool.push_back(e);
+    }
+    return (int)h->evnext++;
+}
+
+// run `fn` (one kernel launch) bracketed by events when profiling is on
+template <class F>
+void timed(sds_handle* h, int kind, double flops, double bytes, F&& fn) {
+    if (!h->prof) {
+        fn();
+        return;
+    }
+    int e0 = next_event(h), e1 = next_event(h);
+    if (e0 < 0 || e1 < 0) {
+        fn();
+        return;
+    }
+    (void)hipEventRecord(h->evpool[e0], h->stream);
+    fn();
+    (void)hipEventRecord(h->evpool[e1], h->stream);
+    h->recs.push_back({kind, e0, e1, flops, bytes});
+}
+
+// wait for the recorded launches and accumulate their durations
+void harvest(sds_handle* h) {
+    if (h->recs.empty()) return;
+    (void)hipStreamSynchronize(h->stream);
+    for (const ProfRec& r : h->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->evpool[r.e0], h->evpool[r.e1]) != hipSuccess) continue;
+        h->acc_ms[r.kind] += ms;
+        h->acc_flops[r.kind] += r.flops;
+        h->acc_bytes[r.kind] += r.bytes;
+        h->acc_n[r.kind] += 1;
+    }
+    h->recs.clear();
+    h->evnext = 0;
 }
 
 template <typename V>
@@ -265,62 +355,210 @@ int build_schedule(sds_handle* h, int dtype) {
         L.off_all = (int64_t)flist.size();
         flist.insert(flist.end(), L.all.begin(), L.all.end());
     }
+    for (auto& L : h->levels) {
+        for (int32_t s : L.all)
+            if (solve_small(h->fnrows[s], h->fncols[s])) L.ssmall.push_back(s);
+        L.off_ssmall = (int64_t)flist.size();
+        flist.insert(flist.end(), L.ssmall.begin(), L.ssmall.end());
+    }
+    // algorithmic work: Cholesky flops of a front sum_{k<w} (f-k)^2 (the CHOLMOD
+    // convention of the reference, src/symbolic/etree.jl:153-160), bytes = what the
+    // front must move at least: its panel and contribution block written once, A read
+    // (value + map), its children's contribution blocks read once
+    const double dbl_es = (double)es;
+    auto front_flops = [&](int32_t s) {
+        double f = h->fnrows[s], w = h->fncols[s], fl = 0;
+        for (int k = 0; k < (int)w; ++k) fl += (f - k) * (f - k);
+        return fl;
+    };
+    auto front_bytes = [&](int32_t s) {
+        double f = h->fnrows[s], w = h->fncols[s], m = f - w;
+        double b = dbl_es * (f * w - w * (w - 1) / 2 + m * (m + 1) / 2);
+        b += 16.0 * (double)(h->asm_count[s]);
+        for (int k = h->child_ptr[s]; k < h->child_ptr[s + 1]; ++k) {
+            double mc = h->fnrows[h->child_list[k]] - h->fncols[h->child_list[k]];
+            b += dbl_es * mc * (mc + 1) / 2;
+        }
+        return b;
+    };
+    for (auto& L : h->levels) {
+        for (int32_t s : L.small) {
+            L.small_flops += front_flops(s);
+            L.small_bytes += front_bytes(s);
+        }
+        for (int32_t s : L.ssmall) {
+            double f = h->fnrows[s], w = h->fncols[s];
+            L.fwd_bytes += dbl_es * (f * w - w * (w - 1) / 2);
+            L.fwd_flops += 2.0 * (f * w - w * (w - 1) / 2);
+            L.rhs_rows += f;
+        }
+    }
+    // entries (i, j) of a tile with j <= i, rows < f, columns in [jmin, jlim)
+    auto tile_entries = [](int I0, int J0, int f, int jmin, int jlim) {
+        double c = 0;
+        for (int j = std::max(J0, jmin); j < std::min(J0 + TILE, jlim); ++j) {
+            int i0 = std::max(I0, j), i1 = std::min(I0 + TILE, f);
+            if (i1 > i0) c += i1 - i0;
+        }
+        return c;
+    };
     // task lists of the large fronts
     std::vector<Task> tasks;
-    auto push_launch = [&](Level& L, int kind, const std::vector<Task>& t) {
+    auto push_launch = [&](Level& L, int kind, const std::vector<Task>& t, double flops, double bytes) {
         if (t.empty()) return;
-        L.plan.push_back({kind, (int64_t)tasks.size(), (int)t.size()});
+        L.plan.push_back({kind, (int64_t)tasks.size(), (int)t.size(), flops, bytes});
         tasks.insert(tasks.end(), t.begin(), t.end());
     };
     for (auto& L : h->levels) {
         if (L.big.empty()) continue;
         std::vector<Task> t;
-        for (int32_t s : L.big)
+        double fl = 0, by = 0;
+        for (int32_t s : L.big) {
+            double f = h->fnrows[s], w = h->fncols[s], m = f - w;
             for (int c = 0; c < h->fnrows[s]; c += 8) t.push_back({s, c, 8, 0});
-        push_launch(L, L_ZERO_SCATTER, t);
+            by += dbl_es * (f * w + m * m) + 16.0 * (double)h->asm_count[s];
+        }
+        push_launch(L, L_ZERO_SCATTER, t, 0, by);
         int maxch = 0;
         for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
         for (int r = 0; r < maxch; ++r) {
             t.clear();
+            by = 0;
             for (int32_t s : L.big) {
                 if (h->child_ptr[s] + r >= h->child_ptr[s + 1]) continue;
                 int32_t c = h->child_list[h->child_ptr[s] + r];
                 if (cb_off[c] < 0) continue;
                 int mc = h->fnrows[c] - h->fncols[c];
                 for (int jj = 0; jj < mc; jj += 16) t.push_back({s, c, jj, 16});
+                by += 3.0 * dbl_es * (double)mc * (mc + 1) / 2;  // read the child's block, read+write the parent
             }
-            push_launch(L, L_EXTEND_ADD, t);
+            push_launch(L, L_EXTEND_ADD, t, 0, by);
         }
         int wmax = 0;
         for (int32_t s : L.big) wmax = std::max(wmax, h->fncols[s]);
         for (int k = 0; k < wmax; k += NB) {
             std::vector<Task> tp, tt, tu;
+            double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0;
             for (int32_t s : L.big) {
                 int f = h->fnrows[s], w = h->fncols[s];
                 if (k >= w) continue;
                 int nb = std::min(NB, w - k);
                 tp.push_back({s, k, 0, 0});
+                fp += (double)nb * nb * nb / 3.0;
+                bp += 2.0 * dbl_es * nb * (nb + 1) / 2;
                 for (int r0 = k + nb; r0 < f; r0 += 64) tt.push_back({s, k, r0, 0});
+                ft += (double)(f - k - nb) * nb * nb;
+                bt += 2.0 * dbl_es * (double)(f - k - nb) * nb;
                 int base = k + nb;
                 if (base < w)
                     for (int I0 = base; I0 < f; I0 += TILE)
-                        for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) tu.push_back({s, I0, J0, k});
+                        for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
+                            tu.push_back({s, I0, J0, k});
+                            double e = tile_entries(I0, J0, f, base, w);
+                            fu += 2.0 * e * nb;
+                            bu += 2.0 * dbl_es * e;
+                        }
             }
-            push_launch(L, L_POTRF, tp);
-            push_launch(L, L_TRSM, tt);
-            push_launch(L, L_UPDATE_PANEL, tu);
+            push_launch(L, L_POTRF, tp, fp, bp);
+            push_launch(L, L_TRSM, tt, ft, bt);
+            push_launch(L, L_UPDATE_PANEL, tu, fu, bu);
         }
         t.clear();
+        fl = by = 0;
         for (int32_t s : L.big) {
             int f = h->fnrows[s], w = h->fncols[s];
             if (f == w || cb_off[s] < 0 || h->snparent[s] < 0) continue;
             for (int I0 = w; I0 < f; I0 += TILE)
-                for (int J0 = w; J0 <= I0; J0 += TILE) t.push_back({s, I0, J0, 0});
+                for (int J0 = w; J0 <= I0; J0 += TILE) {
+                    t.push_back({s, I0, J0, 0});
+                    double e = tile_entries(I0, J0, f, w, f);
+                    fl += 2.0 * e * w;
+                    by += 2.0 * dbl_es * e;
+                }
+            by += dbl_es * (double)(f - w) * w;  // L21 read once
         }
-        push_launch(L, L_UPDATE_CB, t);
+        push_launch(L, L_UPDATE_CB, t, fl, by);
     }
     void* p;
     int rc;
+    // solve plans of the fronts that are not solve_small: per level, the forward sweep is
+    // pull, NB-row block steps (row tiles of 256 below the block), finish; the backward sweep
+    // is the L21 gather, NB-row block steps from the last (column tiles of 64), finish.
+    // Flops and bytes are per right-hand side (bytes: the factor entries read).
+    auto push_solve = [&](std::vector<Launch>& plan, int kind, const std::vector<Task>& t, double flops,
+                          double bytes) {
+        if (t.empty()) return;
+        plan.push_back({kind, (int64_t)tasks.size(), (int)t.size(), flops, bytes});
+        tasks.insert(tasks.end(), t.begin(), t.end());
+    };
+    for (auto& L : h->levels) {
+        std::vector<int32_t> bs;
+        for (int32_t s : L.all)
+            if (!solve_small(h->fnrows[s], h->fncols[s])) bs.push_back(s);
+        if (bs.empty()) continue;
+        std::vector<Task> t;
+        int wmax = 0;
+        for (int32_t s : bs) {
+            t.push_back({s, 0, 0, 0});
+            wmax = std::max(wmax, h->fncols[s]);
+        }
+        push_solve(L.fwd_plan, S_FWD_PULL, t, 0, 0);
+        for (int k = 0; k < wmax; k += NB) {
+            t.clear();
+            double fl = 0, by = 0;
+            for (int32_t s : bs) {
+                int f = h->fnrows[s], w = h->fncols[s];
+                if (k >= w) continue;
+                int nb = std::min(NB, w - k);
+                int first = 1;
+                for (int r0 = k + nb; r0 < f; r0 += 256) {
+                    t.push_back({s, k, r0, first});
+                    first = 0;
+                }
+                if (first) t.push_back({s, k, f, 1});
+                fl += 2.0 * ((double)(f - k - nb) * nb + (double)nb * nb / 2);
+                by += dbl_es * ((double)(f - k - nb) * nb + (double)nb * (nb + 1) / 2);
+            }
+            push_solve(L.fwd_plan, S_FWD_STEP, t, fl, by);
+        }
+        t.clear();
+        for (int32_t s : bs)
+            for (int c = 0; c < h->fncols[s]; c += 256) t.push_back({s, c, 0, 0});
+        push_solve(L.fwd_plan, S_FWD_FINISH, t, 0, 0);
+        // backward
+        t.clear();
+        double fl = 0, by = 0;
+        for (int32_t s : bs) {
+            int f = h->fnrows[s], w = h->fncols[s];
+            if (f == w) continue;
+            for (int c = 0; c < w; c += 64) t.push_back({s, c, 0, 0});
+            fl += 2.0 * (double)(f - w) * w;
+            by += dbl_es * (double)(f - w) * w;
+        }
+        push_solve(L.bwd_plan, S_BWD_L21, t, fl, by);
+        for (int k = ((wmax - 1) / NB) * NB; k >= 0; k -= NB) {
+            t.clear();
+            fl = by = 0;
+            for (int32_t s : bs) {
+                int w = h->fncols[s];
+                if (k >= w) continue;
+                int nb = std::min(NB, w - k);
+                int first = 1;
+                for (int c = 0; c < k; c += 64) {
+                    t.push_back({s, k, c, first});
+                    first = 0;
+                }
+                if (first) t.push_back({s, k, 0, 1});
+                fl += 2.0 * ((double)k * nb + (double)nb * nb / 2);
+                by += dbl_es * ((double)k * nb + (double)nb * (nb + 1) / 2);
+            }
+            push_solve(L.bwd_plan, S_BWD_STEP, t, fl, by);
+        }
+        t.clear();
+        for (int32_t s : bs)
+            for (int c = 0; c < h->fncols[s]; c += 256) t.push_back({s, c, 0, 0});
+        push_solve(L.bwd_plan, S_BWD_FINISH, t, 0, 0);
+    }
     if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
     if ((rc = upload(h, h->numbufs, tasks, &h->d_tasks))) return rc;
     if ((rc = upload(h, h->numbufs, cb_off, &h->d_cb_off))) return rc;
@@ -340,21 +578,27 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     for (auto& L : h->levels) {
         if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
         if (!L.small.empty())
-            launch_front_small_chol<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small, nz,
-                                       factor, cb, h->status, eps);
+            timed(h, SDS_K_FRONT_SMALL, L.small_flops, L.small_bytes, [&] {
+                launch_front_small_chol<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small,
+                                           nz, factor, cb, h->status, eps);
+            });
         for (const Launch& ln : L.plan) {
             const Task* t = h->d_tasks + ln.off;
-            switch (ln.kind) {
-                case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
-                case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
-                case L_POTRF: launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, h->status, eps); break;
-                case L_TRSM: launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, h->status); break;
-                case L_UPDATE_PANEL: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0); break;
-                case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
-            }
+            timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
+                switch (ln.kind) {
+                    case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
+                    case L_POTRF: launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, h->status, eps); break;
+                    case L_TRSM: launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, h->status); break;
+                    case L_UPDATE_PANEL:
+                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
+                        break;
+                    case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
+                }
+            });
         }
     }
-    launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info);
+    timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
     HIPCHK(h, hipGetLastError());
     return SDS_SUCCESS;
 }
@@ -391,16 +635,41 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd) {
     T* Y = static_cast<T*>(h->Y.p);
     T* Z = static_cast<T*>(h->Z.p);
     T* U = static_cast<T*>(h->U.p);
-    const bool unit = false;  // Cholesky
+    const int unit = 0;  // Cholesky
+    const int32_t* piv = nullptr;
+    const double es = sizeof(T);
+    auto run = [&](const Launch& ln, int kind) {
+        const Task* t = h->d_tasks + ln.off;
+        timed(h, kind, ln.flops * nrhs, ln.bytes + (ln.flops > 0 ? es * nrhs * ln.flops / 16 : 0), [&] {
+            switch (ln.kind) {
+                case S_FWD_PULL: launch_fwd_pull<T>(st, h->dsym, t, ln.n, Y, U, Z, nrhs, piv); break;
+                case S_FWD_STEP: launch_fwd_step<T>(st, h->dsym, t, ln.n, factor, Y, U, Z, nrhs, unit); break;
+                case S_FWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, nullptr); break;
+                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, factor, Y, nrhs); break;
+                case S_BWD_STEP: launch_bwd_step<T>(st, h->dsym, t, ln.n, factor, Y, Z, nrhs, unit); break;
+                case S_BWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, piv); break;
+            }
+        });
+    };
     if (fwd)
-        for (auto& L : h->levels)
-            launch_fwd<T>(st, h->dsym, h->d_fronts + L.off_all, (int)L.all.size(), factor, Y, U, Z, nrhs, unit,
-                          nullptr);
+        for (auto& L : h->levels) {
+            if (!L.ssmall.empty())
+                timed(h, SDS_K_FWD, L.fwd_flops * nrhs, L.fwd_bytes + es * nrhs * L.rhs_rows, [&] {
+                    launch_fwd<T>(st, h->dsym, h->d_fronts + L.off_ssmall, (int)L.ssmall.size(), factor, Y, U, Z,
+                                  nrhs, unit, piv);
+                });
+            for (const Launch& ln : L.fwd_plan) run(ln, SDS_K_FWD);
+        }
     (void)diag;
     if (bwd)
-        for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it)
-            launch_bwd<T>(st, h->dsym, h->d_fronts + it->off_all, (int)it->all.size(), factor, Y, Z, nrhs, unit,
-                          nullptr);
+        for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
+            if (!it->ssmall.empty())
+                timed(h, SDS_K_BWD, it->fwd_flops * nrhs, it->fwd_bytes + es * nrhs * it->rhs_rows, [&] {
+                    launch_bwd<T>(st, h->dsym, h->d_fronts + it->off_ssmall, (int)it->ssmall.size(), factor, Y, Z,
+                                  nrhs, unit, piv);
+                });
+            for (const Launch& ln : it->bwd_plan) run(ln, SDS_K_BWD);
+        }
 }
 
 template <typename T>
@@ -418,17 +687,24 @@ int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* 
                                    hipMemcpyDeviceToDevice, st));
         Bsrc = static_cast<const double*>(h->Bc.p);
     }
-    if (phase & SDS_PHASE_SOLVE_FWD_PERM) launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y);
+    const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
+    if (phase & SDS_PHASE_SOLVE_FWD_PERM)
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y); });
     sweeps<T>(h, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD);
-    if (phase & SDS_PHASE_SOLVE_BWD_PERM) launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0);
+    if (phase & SDS_PHASE_SOLVE_BWD_PERM)
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0); });
     int done = 0;
     if (refine) {
         if (!h->full_rowptr) return fail(h, SDS_INVALID_VALUE, "refinement needs the full-pattern map at upload");
         if (!nz) return fail(h, SDS_INVALID_VALUE, "refinement needs the matrix values");
         double* R = static_cast<double*>(h->R.p);
         for (int it = 0; it < ir_n_steps; ++it) {
-            launch_residual(st, n, nrhs, h->full_rowptr, h->full_colval, h->full_src, nz, X, ldx, Bsrc,
-                            Bsrc == B ? ldb : n, R);
+            // one SpMV: values + column + source index per entry, X gathered, B read, R written
+            timed(h, SDS_K_RESIDUAL, 2.0 * h->nnz_full * nrhs,
+                  (double)h->nnz_full * (8 + 4 + 8) + 3.0 * 8 * n * nrhs, [&] {
+                      launch_residual(st, n, nrhs, h->full_rowptr, h->full_colval, h->full_src, nz, X, ldx, Bsrc,
+                                      Bsrc == B ? ldb : n, R);
+                  });
             if (ir_tol > 0) {
                 // host copies of the residual and rhs norms (one synchronization per step)
                 std::vector<double> hr((size_t)n * nrhs), hb((size_t)n * nrhs);
@@ -447,9 +723,10 @@ int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* 
                 }
                 if (ok) break;
             }
-            launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y);
+            timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y); });
             sweeps<T>(h, nrhs, 1, 1, 1);
-            launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1);
+            timed(h, SDS_K_PERMUTE, 0, pbytes + 8.0 * n * nrhs,
+                  [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1); });
             ++done;
         }
     }
@@ -477,6 +754,7 @@ int sds_destroy(sds_handle_t h) {
     else (void)hipDeviceSynchronize();
     free_numeric(h);
     free_all(h->symbufs);
+    for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
     delete h;
     return SDS_SUCCESS;
 }
@@ -539,6 +817,8 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     // assembly entries grouped by front (owner-pull), with local panel offsets
     std::vector<int64_t> asm_ptr(ns + 1), asm_src, asm_dst, acol_ptr(n + 1, 0);
     for (int64_t s = 0; s <= ns; ++s) asm_ptr[s] = aptr[s] - 1;
+    h->asm_count.resize(ns);
+    for (int64_t s = 0; s < ns; ++s) h->asm_count[s] = asm_ptr[s + 1] - asm_ptr[s];
     asm_src.resize(asm_ptr[ns]);
     asm_dst.resize(asm_ptr[ns]);
     for (int64_t s = 0; s < ns; ++s) {
@@ -619,6 +899,7 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
         if ((rc = upload(h, h->symbufs, cv, &h->full_colval))) return rc;
         if ((rc = upload(h, h->symbufs, src, &h->full_src))) return rc;
         h->full_nrows = nr;
+        h->nnz_full = fnnz;
     }
     h->have_sym = true;
     return SDS_SUCCESS;
@@ -730,6 +1011,29 @@ int sds_get_factor(sds_handle_t h, int32_t which, void* out) {
     size_t es = h->dtype == SDS_R64 ? 8 : 4;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(out, h->factor, es * h->factor_len, hipMemcpyDeviceToHost));
+    return SDS_SUCCESS;
+}
+
+int sds_profile(sds_handle_t h, int32_t enable) {
+    if (!h) return SDS_INVALID_VALUE;
+    harvest(h);
+    if (enable) {
+        for (int k = 0; k < SDS_K_COUNT; ++k) {
+            h->acc_ms[k] = h->acc_flops[k] = h->acc_bytes[k] = 0;
+            h->acc_n[k] = 0;
+        }
+    }
+    h->prof = enable != 0;
+    return SDS_SUCCESS;
+}
+
+int sds_profile_read(sds_handle_t h, int32_t kind, double* ms, int64_t* launches, double* flops, double* bytes) {
+    if (!h || kind < 0 || kind >= SDS_K_COUNT) return SDS_INVALID_VALUE;
+    harvest(h);
+    if (ms) *ms = h->acc_ms[kind];
+    if (launches) *launches = h->acc_n[kind];
+    if (flops) *flops = h->acc_flops[kind];
+    if (bytes) *bytes = h->acc_bytes[kind];
     return SDS_SUCCESS;
 }
 

@@ -36,6 +36,20 @@ void launch_bwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfro
                 int nrhs, int unit, const int32_t* piv);
 template <typename T>
 void launch_diag(hipStream_t st, int n, int nrhs, const T* d, const int8_t* kind, T* Y);
+template <typename T>
+void launch_fwd_pull(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, T* U, T* Z, int nrhs,
+                     const int32_t* piv);
+template <typename T>
+void launch_fwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* U, T* Z,
+                     int nrhs, int unit);
+template <typename T>
+void launch_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, const T* Z, int nrhs,
+                   const int32_t* piv);
+template <typename T>
+void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, int nrhs);
+template <typename T>
+void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
+                     int unit);
 void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,
                      const double* nz, const double* X, int64_t ldx, const double* B, int64_t ldb, double* R);
 

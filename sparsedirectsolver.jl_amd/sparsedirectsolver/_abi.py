@@ -6,7 +6,11 @@ c_i32, c_i64, c_dbl, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctype
 # the exported entry points, in the order of include/sds_hip.h
 EXPORTS = ("sds_create", "sds_destroy", "sds_last_error", "sds_set_stream", "sds_upload_symbolic",
            "sds_alloc_numeric", "sds_factorize", "sds_solve", "sds_get_stats", "sds_get_pivots", "sds_get_diag",
-           "sds_get_factor", "sds_memory_estimates")
+           "sds_get_factor", "sds_memory_estimates", "sds_profile", "sds_profile_read")
+
+# profiling kinds (SDS_K_* of include/sds_hip.h)
+PROFILE_KINDS = ("front_small", "zero_scatter", "extend_add", "potrf", "trsm", "update_panel", "update_cb", "stats",
+                 "permute", "fwd", "bwd", "diag", "residual")
 
 SDS_SUCCESS, SDS_INVALID_VALUE, SDS_NOT_SUPPORTED, SDS_FACTORIZATION_ERROR, SDS_DEVICE_ERROR, SDS_INTERRUPTED = range(6)
 SDS_R64, SDS_R32 = 0, 1
@@ -41,6 +45,8 @@ def bind(lib):
     lib.sds_get_diag.argtypes = [vp, vp]
     lib.sds_get_factor.argtypes = [vp, c_i32, vp]
     lib.sds_memory_estimates.argtypes = [vp, vp]
+    lib.sds_profile.argtypes = [vp, c_i32]
+    lib.sds_profile_read.argtypes = [vp, c_i32, P(c_dbl), P(c_i64), P(c_dbl), P(c_dbl)]
     for name in EXPORTS:
         if name != "sds_last_error":
             getattr(lib, name).restype = ctypes.c_int

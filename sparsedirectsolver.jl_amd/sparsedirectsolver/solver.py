@@ -353,3 +353,20 @@ def numeric_stats(solver):
     tot = np.zeros(6, dtype=np.int64)
     _check(solver._lib, solver._h, solver._lib.sds_get_stats(solver._h, st.ctypes.data, tot.ctypes.data))
     return st, tot
+
+
+def profile(solver, enable=True):
+    """Start (clearing the totals) or stop per-kernel-class timing on the solver's stream."""
+    _check(solver._lib, solver._h, solver._lib.sds_profile(solver._h, 1 if enable else 0))
+
+
+def profile_read(solver):
+    """{kind: (ms, launches, flops, bytes)} accumulated since profile(solver, True)."""
+    out = {}
+    for k, name in enumerate(_abi.PROFILE_KINDS):
+        ms, n, fl, by = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _check(solver._lib, solver._h, solver._lib.sds_profile_read(solver._h, k, ctypes.byref(ms), ctypes.byref(n),
+                                                                     ctypes.byref(fl), ctypes.byref(by)))
+        if n.value:
+            out[name] = (ms.value, n.value, fl.value, by.value)
+    return out
