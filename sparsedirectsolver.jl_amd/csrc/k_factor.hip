@@ -160,87 +160,133 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     const T* Cc = cbpool + S.cb_off[c];
     const int32_t* ri = S.relind + S.relind_ptr[c];
     const int jend = min(t.b + t.c, mc);
+    constexpr int U = 8;  // entries per thread in flight: all loads issued before the stores
     for (int jj = t.b; jj < jend; ++jj) {
         const int pj = ri[jj];
-        for (int ii = jj + threadIdx.x; ii < mc; ii += blockDim.x)
-            *front_at(P, Cs, f, w, ri[ii], pj) += Cc[(int64_t)jj * mc + ii];
+        for (int ii0 = jj + threadIdx.x; ii0 < mc; ii0 += 256 * U) {
+            T v[U], old[U];
+            T* dst[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ii = ii0 + 256 * u;
+                dst[u] = ii < mc ? front_at(P, Cs, f, w, ri[ii], pj) : nullptr;
+                v[u] = ii < mc ? Cc[(int64_t)jj * mc + ii] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) old[u] = dst[u] ? *dst[u] : T(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (dst[u]) *dst[u] = old[u] + v[u];
+        }
     }
 }
 
-// Cholesky of the diagonal block of block column k (nb = min(NB, w - k)); task: (s, k)
+// Block step of a large front, part 1: Cholesky of the (updated) diagonal block [k, k + nb)
+// with one wave: lane i holds row i in registers and the pivot and column j move between
+// lanes with v_readlane (register broadcasts). The factored block goes to DB (NB entries per
+// factor column); a failure sets the front's status. task: (s, k)
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_potrf(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
-                                                  int32_t* __restrict__ status, double eps) {
-    __shared__ T D[NB][NB + 1];
-    __shared__ int32_t st;
+__global__ void __launch_bounds__(64) k_big_potrf(DevSym S, const Task* __restrict__ tasks,
+                                                 const T* __restrict__ factor, T* __restrict__ DB,
+                                                 int32_t* __restrict__ status, double eps) {
+    __shared__ T D[NB][NB + 1];  // D[j][i] = L(k + i, k + j)
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a;
     if (status[s] != 0) return;
     const int nb = min(NB, w - k);
-    T* P = factor + S.panel_off[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < nb * nb; q += nt) {
-        const int j = q / nb, i = q - j * nb;
-        D[j][i] = P[(int64_t)(k + j) * f + k + i];
+    const T* P = factor + S.panel_off[s];
+    const int i = threadIdx.x;
+#pragma unroll
+    for (int q = i; q < NB * NB; q += 64) {
+        const int j = q / NB, r = q - j * NB;
+        D[j][r] = (r < nb && j < nb && r >= j) ? P[(int64_t)(k + j) * f + k + r] : (r == j ? T(1) : T(0));
     }
-    if (tid == 0) st = 0;
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (tid == 0) {
-            const double a = (double)D[j][j];
-            if (chol_accept(a, eps))
-                D[j][j] = T(sqrt(fmax(a, eps)));
-            else
-                st = k + j + 1;
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) a[c] = D[c][i];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const double ajj = (double)lane_bcast(a[j], j);
+        if (j < nb && !chol_accept(ajj, eps)) {
+            if (i == 0) status[s] = k + j + 1;
+            return;
         }
-        __syncthreads();
-        if (st) break;
-        const T d = D[j][j];
-        for (int i = j + 1 + tid; i < nb; i += nt) D[j][i] /= d;
-        __syncthreads();
-        const int r = nb - j - 1;
-        for (int q = tid; q < r * r; q += nt) {
-            const int kk = q / r, ii = q - kk * r;
-            if (ii < kk) continue;
-            D[j + 1 + kk][j + 1 + ii] -= D[j][j + 1 + ii] * D[j][j + 1 + kk];
-        }
-        __syncthreads();
+        const T d = j < nb ? T(sqrt(fmax(ajj, eps))) : T(1);
+        const T l = i > j ? a[j] / d : (i == j ? d : a[j]);
+        a[j] = l;
+        const T li = i > j ? l : T(0);
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c) a[c] -= li * lane_bcast(l, c);
+        __builtin_amdgcn_sched_barrier(0);  // one column at a time: bounds register pressure
     }
-    for (int q = tid; q < nb * nb; q += nt) {
-        const int j = q / nb, i = q - j * nb;
-        if (i >= j) P[(int64_t)(k + j) * f + k + i] = D[j][i];
-    }
-    if (tid == 0 && st) status[s] = st;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) D[c][i] = c <= i ? a[c] : T(0);
+    __syncthreads();
+    T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
+#pragma unroll
+    for (int q = i; q < NB * NB; q += 64)
+        if (q < nb * NB) db[q] = D[q / NB][q % NB];
 }
 
-// rows [row0, row0 + 64) of block column k: X <- X L11^-T; task: (s, k, row0); 64 threads
+// Block step of a large front, part 2: a tile of PANEL_ROWS panel rows below the diagonal
+// block, X <- X L_bb^-T with L_bb from DB; the tile is staged through LDS with coalesced
+// loads and stores and each thread solves its row in registers. The task with c = 1 also
+// copies the factored block into the panel (nobody in this launch reads it there).
+// task: (s, k, row0, c)
 template <typename T>
-__global__ void __launch_bounds__(64) k_big_trsm(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
-                                                const int32_t* __restrict__ status) {
-    __shared__ T L[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];  // X[col][row]
+__global__ void __launch_bounds__(PANEL_ROWS) k_big_trsm(DevSym S, const Task* __restrict__ tasks,
+                                                        T* __restrict__ factor, const T* __restrict__ DB,
+                                                        const int32_t* __restrict__ status) {
+    __shared__ T D[NB][NB + 1];
+    __shared__ T Xs[NB][PANEL_ROWS + 1];  // Xs[c][rr] = panel row row0 + rr, column k + c
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
     if (status[s] != 0) return;
     const int nb = min(NB, w - k);
-    const int nr = min(64, f - row0);
     T* P = factor + S.panel_off[s];
+    const T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
     const int tid = threadIdx.x;
-    for (int j = 0; j < nb; ++j) {
-        L[j][tid] = tid < nb ? P[(int64_t)(k + j) * f + k + tid] : T(0);
-        X[j][tid] = tid < nr ? P[(int64_t)(k + j) * f + row0 + tid] : T(0);
+#pragma unroll
+    for (int q = tid; q < NB * NB; q += PANEL_ROWS) {
+        const int j = q / NB, i = q - j * NB;
+        D[j][i] = (i < nb && j < nb) ? db[q] : (i == j ? T(1) : T(0));
     }
-    __syncthreads();
-    if (tid < nr) {
-        for (int kk = 0; kk < nb; ++kk) {
-            T v = X[kk][tid];
-            for (int jj = 0; jj < kk; ++jj) v -= X[jj][tid] * L[jj][kk];
-            X[kk][tid] = v / L[kk][kk];
+    if (t.c)
+        for (int q = tid; q < nb * nb; q += PANEL_ROWS) {
+            const int j = q / nb, i = q - j * nb;
+            if (i >= j) P[(int64_t)(k + j) * f + k + i] = db[j * NB + i];
         }
+    // columns past nb are zero (and L_bb is padded with the identity), so the register row
+    // needs no guards (runtime guards on it make the compiler spill)
+    const int nr = min(PANEL_ROWS, f - row0);
+#pragma unroll
+    for (int q = tid; q < NB * PANEL_ROWS; q += PANEL_ROWS) {
+        const int c = q / PANEL_ROWS, rr = q - c * PANEL_ROWS;
+        Xs[c][rr] = (rr < nr && c < nb) ? P[(int64_t)(k + c) * f + row0 + rr] : T(0);
     }
     __syncthreads();
-    if (tid < nr)
-        for (int j = 0; j < nb; ++j) P[(int64_t)(k + j) * f + row0 + tid] = X[j][tid];
+    if (nr > 0) {
+        T x[NB];
+#pragma unroll
+        for (int kk = 0; kk < NB; ++kk) x[kk] = Xs[kk][tid];
+#pragma unroll
+        for (int kk = 0; kk < NB; ++kk) {
+            T v = x[kk];
+#pragma unroll
+            for (int jj = 0; jj < kk; ++jj) v -= x[jj] * D[jj][kk];
+            x[kk] = v / D[kk][kk];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < NB; ++kk) Xs[kk][tid] = x[kk];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = tid; q < NB * PANEL_ROWS; q += PANEL_ROWS) {
+        const int c = q / PANEL_ROWS, rr = q - c * PANEL_ROWS;
+        if (rr < nr && c < nb) P[(int64_t)(k + c) * f + row0 + rr] = Xs[c][rr];
+    }
 }
 
 // Schur update of one 64x64 tile: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T on the lower
@@ -306,6 +352,109 @@ __global__ void __launch_bounds__(256) k_big_update(DevSym S, const Task* __rest
     }
 }
 
+// The same tile update on the matrix cores. 256 threads = 4 waves, each wave a 32x32
+// quadrant of the 64x64 tile as 2x2 blocks of 16x16 accumulators; K in chunks of 16
+// staged through LDS, the next chunk prefetched into registers while the current one is
+// multiplied. MFMA 16x16x4 operand layout (checked on gfx950): lane l gives A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator element q of lane l is D[row(l / 16, q)][l % 16] with
+// row = 4 (l / 16) + q for the f32 instruction and 4 q + l / 16 for the f64 one.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ inline acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l4, int q) { return 4 * q + l4; }
+};
+template <>
+struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ inline acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l4, int q) { return 4 * l4 + q; }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
+                                                        T* __restrict__ factor, T* __restrict__ cbpool,
+                                                        const int32_t* __restrict__ status, int mode) {
+    constexpr int KC = 16;
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[KC][TILE + 4];
+    __shared__ T Bs[KC][TILE + 4];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    if (status[s] != 0) return;
+    const int I0 = t.a, J0 = t.b;
+    const int k0 = mode == 0 ? t.c : 0;
+    const int k1 = mode == 0 ? min(t.c + NB, w) : w;
+    const int jlim = mode == 0 ? w : f;
+    const int jmin = mode == 0 ? k1 : w;
+    T* P = factor + S.panel_off[s];
+    T* C = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    // this thread's share of a chunk: 4 entries of A and of B, (kk, r) = (q / 64, q % 64)
+    T pa[4], pb[4];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
+            const int kcol = kc + kk;
+            const bool kin = kcol < k1;
+            pa[u] = (kin && I0 + r < f) ? P[(int64_t)kcol * f + I0 + r] : T(0);
+            pb[u] = (kin && J0 + r < f) ? P[(int64_t)kcol * f + J0 + r] : T(0);
+        }
+    };
+    fetch(k0);
+    for (int kc = k0; kc < k1; kc += KC) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
+            As[kk][r] = pa[u];
+            Bs[kk][r] = pb[u];
+        }
+        __syncthreads();
+        if (kc + KC < k1) fetch(kc + KC);
+#pragma unroll
+        for (int k4 = 0; k4 < KC; k4 += 4) {
+            const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
+            const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
+            acc[0][0] = Mfma<T>::mac(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::mac(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::mac(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::mac(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = J0 + wn * 32 + b * 16 + l16;
+            if (j >= jlim || j < jmin) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = I0 + wm * 32 + a * 16 + Mfma<T>::row(l4, q);
+                if (i >= f || i < j) continue;
+                T* dst = front_at(P, C, f, w, i, j);
+                *dst -= acc[a][b][q];
+            }
+        }
+}
+
 // per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and the smallest
 // failed factor column (1-based, 0 = none) into info[0]
 __global__ void __launch_bounds__(256) k_chol_stats(DevSym S, const int32_t* __restrict__ status,
@@ -357,20 +506,21 @@ void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, i
 }
 
 template <typename T>
-void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* status,
+void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                       double eps) {
-    hipLaunchKernelGGL(k_big_potrf<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, status, eps);
+    hipLaunchKernelGGL(k_big_potrf<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, DB, status, eps);
 }
 
 template <typename T>
-void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* status) {
-    hipLaunchKernelGGL(k_big_trsm<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, status);
+void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const T* DB,
+                     const int32_t* status) {
+    hipLaunchKernelGGL(k_big_trsm<T>, dim3(n), dim3(PANEL_ROWS), 0, st, S, tasks, factor, DB, status);
 }
 
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode) {
-    hipLaunchKernelGGL(k_big_update<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+    hipLaunchKernelGGL(k_big_update_mfma<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
 }
 
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info) {
@@ -388,8 +538,8 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
                                              T*, T*, int32_t*, double);                                           \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
-    template void launch_big_potrf<T>(hipStream_t, const DevSym&, const Task*, int, T*, int32_t*, double);         \
-    template void launch_big_trsm<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int32_t*);           \
+    template void launch_big_potrf<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);    \
+    template void launch_big_trsm<T>(hipStream_t, const DevSym&, const Task*, int, T*, const T*, const int32_t*); \
     template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int);
 INST(double)
 INST(float)

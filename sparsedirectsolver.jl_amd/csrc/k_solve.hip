@@ -39,6 +39,46 @@ __global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm,
     }
 }
 
+// stage the ni x nj block of panel P (leading dimension f) at (row i0, column j0) into LDS
+// as B[jj][ii]; the loop has a compile-time trip count and is unrolled so that each thread
+// has all its loads in flight at once (blockDim.x = 256)
+template <typename T>
+__device__ inline void load_block(T (*B)[NB + 1], const T* __restrict__ P, int f, int i0, int j0, int ni, int nj) {
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+        const int q = threadIdx.x + 256 * u, jj = q / NB, ii = q - jj * NB;
+        if (ii < ni && jj < nj) B[jj][ii] = P[(int64_t)(j0 + jj) * f + i0 + ii];
+    }
+}
+
+// solve a diagonal block held in LDS: xs[i * nrhs + r] <- L_bb^-1 xs (lower = 1) or
+// L_bb^-T xs (lower = 0); Lb[j][i] = L(k + i, k + j). One wave per right-hand side group,
+// lane i holds row i in a register; the solved value moves between lanes by shuffles.
+// xs is read before and written after the solve only (callers put barriers around it).
+template <typename T>
+__device__ void block_solve(const T (*Lb)[NB + 1], T* xs, int nb, int nrhs, int unit, int lower) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r = wv; r < nrhs; r += nw) {
+        T v = lane < nb ? xs[lane * nrhs + r] : T(0);
+        if (lower) {
+            for (int j = 0; j < nb; ++j) {
+                T xj = __shfl(v, j);
+                if (!unit) xj /= Lb[j][j];
+                if (lane == j) v = xj;
+                if (lane > j && lane < nb) v -= Lb[j][lane] * xj;
+            }
+        } else {
+            for (int j = nb - 1; j >= 0; --j) {
+                T xj = __shfl(v, j);
+                if (!unit) xj /= Lb[j][j];
+                if (lane == j) v = xj;
+                if (lane < j) v -= Lb[lane][j] * xj;
+            }
+        }
+        if (lane < nb) xs[lane * nrhs + r] = v;
+    }
+}
+
 // forward sweep over the fronts of one level, one workgroup per front.
 // unit: unit diagonal (LDL^T, LU); piv (nullable): local pivot order of each
 // supernode (global factor columns), applied to its slice first through the
@@ -83,24 +123,22 @@ __global__ void __launch_bounds__(256) k_fwd(DevSym S, const int32_t* __restrict
         for (int q = tid; q < w * nrhs; q += nt) y[q] = z[q];
         __syncthreads();
     }
-    for (int j0 = 0; j0 < w; j0 += 64) {
-        const int j1 = min(j0 + 64, w);
-        for (int j = j0; j < j1; ++j) {
-            if (!unit) {
-                if (tid < nrhs) y[j * nrhs + tid] /= P[(int64_t)j * f + j];
-                __syncthreads();
-            }
-            for (int q = tid; q < (j1 - j - 1) * nrhs; q += nt) {
-                const int i = j + 1 + q / nrhs, r = q % nrhs;
-                y[i * nrhs + r] -= P[(int64_t)j * f + i] * y[j * nrhs + r];
-            }
-            __syncthreads();
-        }
+    __shared__ T Lb[NB][NB + 1];
+    __shared__ T xs[NB * MAX_NRHS];
+    for (int j0 = 0; j0 < w; j0 += NB) {
+        const int nb = min(NB, w - j0), j1 = j0 + nb;
+        load_block<T>(Lb, P, f, j0, j0, nb, nb);
+        for (int q = tid; q < nb * nrhs; q += nt) xs[q] = y[j0 * nrhs + q];
+        __syncthreads();
+        block_solve<T>(Lb, xs, nb, nrhs, unit, 1);
+        __syncthreads();
+        for (int q = tid; q < nb * nrhs; q += nt) y[j0 * nrhs + q] = xs[q];
         // rows below the chunk: in L11 (subtract now) or below the supernode (update vector)
         for (int q = tid; q < (f - j1) * nrhs; q += nt) {
             const int i = j1 + q / nrhs, r = q % nrhs;
             T t = T(0);
-            for (int j = j0; j < j1; ++j) t += P[(int64_t)j * f + i] * y[j * nrhs + r];
+#pragma unroll 16
+            for (int j = 0; j < nb; ++j) t += P[(int64_t)(j0 + j) * f + i] * xs[j * nrhs + r];
             if (i < w)
                 y[i * nrhs + r] -= t;
             else if (us)
@@ -122,6 +160,7 @@ __global__ void __launch_bounds__(256) k_bwd(DevSym S, const int32_t* __restrict
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* xs = reinterpret_cast<T*>(smem_raw);  // 64 * nrhs: the chunk's values
     T* ts = xs + 64 * nrhs;                  // 64 * nrhs: dot products past the chunk
+    __shared__ T Lb[NB][NB + 1];             // the chunk's diagonal block
     const int s = fronts[blockIdx.x];
     const int f = S.fnrows[s], w = S.fncols[s];
     const int c0 = S.super_ptr[s];
@@ -151,25 +190,11 @@ __global__ void __launch_bounds__(256) k_bwd(DevSym S, const int32_t* __restrict
                 }
             }
         }
+        load_block<T>(Lb, P, f, j0, j0, j1 - j0, j1 - j0);
         __syncthreads();
         for (int q = tid; q < (j1 - j0) * nrhs; q += 256) xs[q] = y[j0 * nrhs + q] - ts[q];
         __syncthreads();
-        if (wv == 0) {
-            for (int j = j1 - 1; j >= j0; --j) {
-                const int i = j + 1 + lane;
-                const T l = i < j1 ? P[(int64_t)j * f + i] : T(0);
-                const T dgl = unit ? T(1) : P[(int64_t)j * f + j];
-                for (int r = 0; r < nrhs; ++r) {
-                    T v = i < j1 ? l * xs[(i - j0) * nrhs + r] : T(0);
-                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                    if (lane == 0) {
-                        T x = xs[(j - j0) * nrhs + r] - v;
-                        if (!unit) x /= dgl;
-                        xs[(j - j0) * nrhs + r] = x;
-                    }
-                }
-            }
-        }
+        block_solve<T>(Lb, xs, j1 - j0, nrhs, unit, 0);
         __syncthreads();
         for (int q = tid; q < (j1 - j0) * nrhs; q += 256) y[j0 * nrhs + q] = xs[q];
         __syncthreads();
@@ -188,33 +213,6 @@ __global__ void __launch_bounds__(256) k_bwd(DevSym S, const int32_t* __restrict
 // ---------------------------------------------------------------------------
 // large fronts: the sweeps of one front are split into NB-row diagonal blocks
 // and run as level-batched launches with many workgroups per front.
-
-// solve the diagonal block [k, k + nb) in LDS: xs[i * nrhs + r] <- L_bb^-1 xs (lower = 1) or
-// L_bb^-T xs (lower = 0); Lb[j][i] = L(k + i, k + j). One wave per right-hand side group,
-// lane i holds row i; the pivot value moves between lanes by shuffles (no barriers).
-template <typename T>
-__device__ void block_solve(const T (*Lb)[NB + 1], T* xs, int nb, int nrhs, int unit, int lower) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int r = wv; r < nrhs; r += nw) {
-        T v = lane < nb ? xs[lane * nrhs + r] : T(0);
-        if (lower) {
-            for (int j = 0; j < nb; ++j) {
-                T xj = __shfl(v, j);
-                if (!unit) xj /= Lb[j][j];
-                if (lane == j) v = xj;
-                if (lane > j && lane < nb) v -= Lb[j][lane] * xj;
-            }
-        } else {
-            for (int j = nb - 1; j >= 0; --j) {
-                T xj = __shfl(v, j);
-                if (!unit) xj /= Lb[j][j];
-                if (lane == j) v = xj;
-                if (lane < j) v -= Lb[lane][j] * xj;
-            }
-        }
-        if (lane < nb) xs[lane * nrhs + r] = v;
-    }
-}
 
 // forward, before the block steps: zero the update vector, pull the children's update
 // vectors (child_list order), then apply the local pivot order (piv) through Z.
@@ -273,10 +271,7 @@ __global__ void __launch_bounds__(256) k_fwd_step(DevSym S, const Task* __restri
     const int c0 = S.super_ptr[s];
     const int tid = threadIdx.x;
     const T* P = factor + S.panel_off[s];
-    for (int q = tid; q < nb * nb; q += 256) {
-        const int j = q / nb, i = q - j * nb;
-        Lb[j][i] = P[(int64_t)(k + j) * f + k + i];
-    }
+    load_block<T>(Lb, P, f, k, k, nb, nb);
     for (int q = tid; q < nb * nrhs; q += 256) xs[q] = Y[(int64_t)(c0 + k) * nrhs + q];
     __syncthreads();
     block_solve<T>(Lb, xs, nb, nrhs, unit, 1);
@@ -290,6 +285,7 @@ __global__ void __launch_bounds__(256) k_fwd_step(DevSym S, const Task* __restri
             T acc[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) acc[r] = T(0);
+#pragma unroll 16
             for (int j = 0; j < nb; ++j) {
                 const T l = P[(int64_t)(k + j) * f + i];
 #pragma unroll
@@ -325,32 +321,45 @@ __global__ void __launch_bounds__(256) k_finish(DevSym S, const Task* __restrict
 
 // backward, before the block steps: y_j -= sum_{i >= w} L[i, j] y[rows[i]] for the columns
 // [col0, col0 + 64) of the front; one wave per column. task: (s, col0)
+// The 64 x 64 tiles of L21 and the matching 64 rows of Y (gathered through rowval, 8
+// right-hand sides at a time) are staged in LDS, so L21 is read from memory once and Y once
+// per 64 columns; thread (column jj, rhs pair) accumulates over the rows.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restrict__ tasks,
                                                 const T* __restrict__ factor, T* __restrict__ Y, int nrhs) {
+    __shared__ T Lt[64][65];  // Lt[jj][ii] = L(i0 + ii, col0 + jj)
+    __shared__ T Yc[64][8];
     const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s], col0 = t.a;
+    const int ncol = min(64, w - col0);
+    const int tid = threadIdx.x, jj = tid & 63, rr = (tid >> 6) * 2;
     const T* P = factor + S.panel_off[s];
     const int32_t* rows = S.rowval + S.rowptr[s];
-    for (int j = t.a + wv; j < min(t.a + 64, w); j += 4) {
-        for (int rb = 0; rb < nrhs; rb += 8) {
-            T acc[8];
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        T acc0 = T(0), acc1 = T(0);
+        for (int i0 = w; i0 < f; i0 += 64) {
+            const int ni = min(64, f - i0);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] = T(0);
-            for (int i = w + lane; i < f; i += 64) {
-                const T l = P[(int64_t)j * f + i];
-                const T* yr = Y + (int64_t)rows[i] * nrhs + rb;
-#pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if (rb + r < nrhs) acc[r] += l * yr[r];
+            for (int q = tid; q < 64 * 64; q += 256) {
+                const int c = q >> 6, ii = q & 63;
+                Lt[c][ii] = (c < ncol && ii < ni) ? P[(int64_t)(col0 + c) * f + i0 + ii] : T(0);
             }
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                T v = acc[r];
-                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                if (lane == 0 && rb + r < nrhs) Y[(int64_t)(c0 + j) * nrhs + rb + r] -= v;
+            for (int q = tid; q < 64 * 8; q += 256) {
+                const int ii = q >> 3, r = q & 7;
+                Yc[ii][r] = (ii < ni && rb + r < nrhs) ? Y[(int64_t)rows[i0 + ii] * nrhs + rb + r] : T(0);
             }
+            __syncthreads();
+#pragma unroll 8
+            for (int ii = 0; ii < 64; ++ii) {
+                const T l = Lt[jj][ii];
+                acc0 += l * Yc[ii][rr];
+                acc1 += l * Yc[ii][rr + 1];
+            }
+            __syncthreads();
+        }
+        if (jj < ncol) {
+            if (rb + rr < nrhs) Y[(int64_t)(c0 + col0 + jj) * nrhs + rb + rr] -= acc0;
+            if (rb + rr + 1 < nrhs) Y[(int64_t)(c0 + col0 + jj) * nrhs + rb + rr + 1] -= acc1;
         }
     }
 }
@@ -373,14 +382,8 @@ __global__ void __launch_bounds__(256) k_bwd_step(DevSym S, const Task* __restri
     const int tid = threadIdx.x;
     const T* P = factor + S.panel_off[s];
     const int ncol = max(0, min(col0 + 64, k) - col0);
-    for (int q = tid; q < nb * nb; q += 256) {
-        const int j = q / nb, i = q - j * nb;
-        Lb[j][i] = P[(int64_t)(k + j) * f + k + i];
-    }
-    for (int q = tid; q < ncol * nb; q += 256) {
-        const int jj = q / nb, i = q - jj * nb;
-        Lt[jj][i] = P[(int64_t)(col0 + jj) * f + k + i];
-    }
+    load_block<T>(Lb, P, f, k, k, nb, nb);
+    load_block<T>(Lt, P, f, k, col0, nb, ncol);
     for (int q = tid; q < nb * nrhs; q += 256) xs[q] = Y[(int64_t)(c0 + k) * nrhs + q];
     __syncthreads();
     block_solve<T>(Lb, xs, nb, nrhs, unit, 0);
@@ -391,6 +394,7 @@ __global__ void __launch_bounds__(256) k_bwd_step(DevSym S, const Task* __restri
     if (jj < ncol)
         for (int r = tid >> 6; r < nrhs; r += 4) {
             T acc = T(0);
+#pragma unroll 16
             for (int i = 0; i < nb; ++i) acc += Lt[jj][i] * xs[i * nrhs + r];
             Y[(int64_t)(c0 + col0 + jj) * nrhs + r] -= acc;
         }

@@ -41,8 +41,20 @@ enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4
 
 constexpr int NB = 64;     // pivot block of the blocked front factorization
 constexpr int TILE = 64;   // Schur-update tile
+constexpr int PANEL_ROWS = 128;  // panel rows per workgroup of the fused block step
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
+
+// value of `v` in lane `lane` (wave-uniform) of the calling wave, as a register broadcast
+__device__ inline double lane_bcast(double v, int lane) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_readlane(x.x, lane);
+    x.y = __builtin_amdgcn_readlane(x.y, lane);
+    return __builtin_bit_cast(double, x);
+}
+__device__ inline float lane_bcast(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
 
 #define SDS_HIP_CHECK(x)                                      \
     do {                                                      \

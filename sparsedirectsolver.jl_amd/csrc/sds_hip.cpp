@@ -118,6 +118,7 @@ struct sds_handle {
     int64_t* d_uv_off = nullptr;
     void* factor = nullptr;
     void* cbpool = nullptr;
+    void* dbuf = nullptr;  // factored diagonal blocks of the large fronts (NB entries per column)
     int32_t* status = nullptr;
     int64_t* stats = nullptr;
     int64_t* d_info = nullptr;
@@ -445,10 +446,15 @@ int build_schedule(sds_handle* h, int dtype) {
                 int nb = std::min(NB, w - k);
                 tp.push_back({s, k, 0, 0});
                 fp += (double)nb * nb * nb / 3.0;
-                bp += 2.0 * dbl_es * nb * (nb + 1) / 2;
-                for (int r0 = k + nb; r0 < f; r0 += 64) tt.push_back({s, k, r0, 0});
+                bp += dbl_es * (double)nb * (nb + 1);
+                int first = 1;
+                for (int r0 = k + nb; r0 < f; r0 += PANEL_ROWS) {
+                    tt.push_back({s, k, r0, first});
+                    first = 0;
+                }
+                if (first) tt.push_back({s, k, f, 1});
                 ft += (double)(f - k - nb) * nb * nb;
-                bt += 2.0 * dbl_es * (double)(f - k - nb) * nb;
+                bt += dbl_es * ((double)nb * (nb + 1) / 2 + 2.0 * (double)(f - k - nb) * nb);
                 int base = k + nb;
                 if (base < w)
                     for (int I0 = base; I0 < f; I0 += TILE)
@@ -588,8 +594,12 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
                     case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
-                    case L_POTRF: launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, h->status, eps); break;
-                    case L_TRSM: launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, h->status); break;
+                    case L_POTRF:
+                        launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
+                        break;
+                    case L_TRSM:
+                        launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, static_cast<const T*>(h->dbuf), h->status);
+                        break;
                     case L_UPDATE_PANEL:
                         launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
                         break;
@@ -917,6 +927,7 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     void* p;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->factor))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->cb_len, 1), &h->cbpool))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->dbuf))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
     h->status = static_cast<int32_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * NF * (h->ns + 1), &p))) return rc;

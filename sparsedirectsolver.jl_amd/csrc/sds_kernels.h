@@ -13,10 +13,11 @@ void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks,
 template <typename T>
 void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool);
 template <typename T>
-void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* status,
+void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                       double eps);
 template <typename T>
-void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* status);
+void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const T* DB,
+                     const int32_t* status);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode);
