@@ -14,6 +14,7 @@
 // Reference counterparts: regime B fused kernel src/numeric/front.jl:154-206,
 // assembly src/numeric/assembly.jl:18-252, regime C panel src/numeric/factorize.jl:240-269,
 // statistics src/numeric/factorize.jl:13-48.
+#include "k_front.h"
 #include "sds_common.h"
 #include "sds_kernels.h"
 
@@ -37,35 +38,8 @@ __global__ void __launch_bounds__(256) k_front_small_chol(DevSym S, const int32_
     const int s = fronts[blockIdx.x];
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < f * f; q += nt) F[q] = T(0);
     if (tid == 0) st = 0;
-    __syncthreads();
-    // A: one thread sums each run of entries with the same destination, in nzval order
-    const int64_t a0 = S.asm_ptr[s], a1 = S.asm_ptr[s + 1];
-    for (int64_t k = a0 + tid; k < a1; k += nt) {
-        const int64_t d = S.asm_dst[k];
-        if (k > a0 && S.asm_dst[k - 1] == d) continue;
-        double v = 0.0;
-        for (int64_t q = k; q < a1 && S.asm_dst[q] == d; ++q) v += nz[S.asm_src[q]];
-        const int64_t loc = d >= 0 ? d : -(d + 1);
-        F[loc] += T(v);
-    }
-    __syncthreads();
-    // extend-add of the children's lower-triangular contribution blocks, in order
-    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
-        const int c = S.child_list[k];
-        const int64_t co = S.cb_off[c];
-        if (co < 0) continue;
-        const int mc = S.fnrows[c] - S.fncols[c];
-        const T* C = cbpool + co;
-        const int32_t* ri = S.relind + S.relind_ptr[c];
-        for (int q = tid; q < mc * mc; q += nt) {
-            const int jj = q / mc, ii = q - jj * mc;
-            if (ii < jj) continue;
-            F[ri[jj] * f + ri[ii]] += C[q];
-        }
-        __syncthreads();
-    }
+    assemble_dense<T>(S, s, nz, cbpool, F, f, 0);
     // right-looking Cholesky of the w pivot columns over the whole front
     for (int j = 0; j < w; ++j) {
         if (tid == 0) {

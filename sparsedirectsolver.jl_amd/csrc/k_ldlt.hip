@@ -1,0 +1,669 @@
+// LDL^T factorization kernels (gfx950) with diagonal and Bunch-Kaufman pivoting inside
+// each front's fully summed block, threshold test against the whole remaining column,
+// best-1x1 fallback and static perturbation (+-eps, optionally with requested signs).
+//
+// Every floating-point operation of the factorization is the one the CPU reference
+// performs, in the same order, with no fused multiply-add (the file is compiled with
+// contraction off): a front's pivot decisions, factor, D and contribution block are
+// bitwise those of src/reference/ldlt.jl in FP64. The pivot search reductions
+// (column maxima, argmax with the first index on ties) are order-independent, so they
+// run as parallel reductions.
+//
+// Small fronts (f x f fits the LDS) are assembled, factored and their contribution
+// block formed in one workgroup; large fronts are assembled in global memory by the
+// shared column-chunk and extend-add kernels, factored by one workgroup of 1024 threads
+// on the panel in place (the full w x w fully summed block lives in the panel's top
+// rows, the upper triangle being the mirror the reference keeps), then their
+// contribution block is updated by 64 x 64 tiles, each entry subtracting the terms
+// W[i, k] L[j, k] in the order k = 0, 1, ... of the reference.
+//
+// Reference counterparts: ref_ldlt! and _ldlt_front! (src/reference/ldlt.jl:121-202,
+// :384-454), _choose_pivot (:280-325), _best_1x1 (:264-276), the threshold test
+// (:252-260), the 2x2 test (:240-248), W = L21 D and the Schur update (:206-231);
+// device statistics src/numeric/ldlt.jl:1276-1356.
+#pragma clang fp contract(off)
+#include "k_front.h"
+#include "sds_common.h"
+#include "sds_kernels.h"
+
+namespace sds {
+
+enum { PV_NONE = 0, PV_DIAG = 1, PV_BK = 2 };
+enum { ST_NPOS = 0, ST_NNEG = 1, ST_NZERO = 2, ST_NPERT = 3, ST_N2X2 = 4, ST_INFO = 5 };
+
+// max that propagates NaN, as Julia's max
+__device__ inline double jmax(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+// (value, index) pairs of the argmax reductions: index < 0 = no candidate; a larger value
+// wins, equal values keep the smaller index (the first one a sequential scan meets)
+__device__ inline bool vi_better(double ov, int oi, double v, int i) {
+    if (oi < 0) return false;
+    if (i < 0) return true;
+    return ov > v || (ov == v && oi < i);
+}
+
+template <int NT>
+struct RedBuf {
+    double v[NT / 64];
+    int i[NT / 64];
+};
+
+__device__ inline double wave_jmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = jmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ inline void wave_argmax(double& v, int& i) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oi = __shfl_xor(i, o);
+        if (vi_better(ov, oi, v, i)) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+
+// block-wide reductions: every thread gets the same result
+template <int NT>
+__device__ double block_jmax(double v, RedBuf<NT>& rb) {
+    v = wave_jmax(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) rb.v[wv] = v;
+    __syncthreads();
+    double r = rb.v[0];
+    for (int q = 1; q < NT / 64; ++q) r = jmax(r, rb.v[q]);
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ void block_argmax(double& v, int& i, RedBuf<NT>& rb) {
+    wave_argmax(v, i);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        rb.v[wv] = v;
+        rb.i[wv] = i;
+    }
+    __syncthreads();
+    v = rb.v[0];
+    i = rb.i[0];
+    for (int q = 1; q < NT / 64; ++q)
+        if (vi_better(rb.v[q], rb.i[q], v, i)) {
+            v = rb.v[q];
+            i = rb.i[q];
+        }
+    __syncthreads();
+}
+
+struct LdltPrm {
+    int pivot;
+    double u, eps;
+};
+
+// the front view of one factorization: F is f x f column-major (LDS or the panel), only
+// columns < w are touched; lp is the local pivot order (original positions)
+template <typename T, int NT>
+struct Front {
+    T* F;
+    int f, w;
+    int* lp;
+    RedBuf<NT>* rb;
+    __device__ double at(int i, int j) const { return (double)F[(int64_t)j * f + i]; }
+    __device__ T& ref(int i, int j) const { return F[(int64_t)j * f + i]; }
+
+    // largest |F[i, c]| over the rows k..f-1 but s1 and s2, NaN-propagating (ldlt.jl:240-260)
+    __device__ double colmax(int c, int k, int s1, int s2) const {
+        double m = 0.0;
+        for (int i = k + threadIdx.x; i < f; i += NT)
+            if (i != s1 && i != s2) m = jmax(m, fabs(at(i, c)));
+        return block_jmax<NT>(m, *rb);
+    }
+    // d >= u max_{i in k..f-1, i != c} |F[i, c]| (the early exit of the reference is the same
+    // predicate: the running maximum only grows)
+    __device__ bool accept_1x1(int c, int k, const LdltPrm& p) const {
+        const double d = fabs(at(c, c));
+        if (!(d >= p.eps)) return false;
+        return d >= p.u * colmax(c, k, c, -1);
+    }
+    __device__ bool nonsingular_2x2(int k, int r, const LdltPrm& p) const {
+        const double det = at(k, k) * at(r, r) - at(k, r) * at(r, k);
+        const double mx = jmax(jmax(fabs(at(k, k)), fabs(at(r, k))), fabs(at(r, r)));
+        return det != 0.0 && isfinite(det) && mx >= p.eps;
+    }
+    __device__ bool accept_2x2(int k, int r, const LdltPrm& p) const {
+        if (!nonsingular_2x2(k, r, p)) return false;
+        const double det = at(k, k) * at(r, r) - at(k, r) * at(r, k);
+        const double e11 = fabs(at(r, r) / det), e12 = fabs(at(k, r) / det), e21 = fabs(at(r, k) / det),
+                     e22 = fabs(at(k, k) / det);
+        const double m1 = colmax(k, k, k, r), m2 = colmax(r, k, r, k);
+        return p.u * (e11 * m1 + e21 * m2) <= 1.0 && p.u * (e12 * m1 + e22 * m2) <= 1.0;
+    }
+    // acceptable 1x1 pivot of the block k..w-1 with the largest |a_jj|, first on ties; -1 if
+    // none (ldlt.jl:264-276). One wave per candidate column; a column that cannot beat its
+    // wave's current best is not tested (it cannot be the sequential scan's choice either).
+    __device__ int best_1x1(int k, const LdltPrm& p) const {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        double bv = 0.0;
+        int bi = -1;
+        for (int j = k + wv; j < w; j += NT / 64) {
+            const double a = fabs(at(j, j));
+            if (!(a >= p.eps)) continue;
+            if (bi >= 0 && !(a > bv)) continue;
+            double m = 0.0;
+            for (int i = k + lane; i < f; i += 64)
+                if (i != j) m = jmax(m, fabs(at(i, j)));
+            m = wave_jmax(m);
+            if (!(a >= p.u * m)) continue;
+            bv = a;
+            bi = j;
+        }
+        // each wave's candidate is wave-uniform; combine across waves
+        if (lane == 0) {
+            rb->v[wv] = bv;
+            rb->i[wv] = bi;
+        }
+        __syncthreads();
+        double v = rb->v[0];
+        int i = rb->i[0];
+        for (int q = 1; q < NT / 64; ++q)
+            if (vi_better(rb->v[q], rb->i[q], v, i)) {
+                v = rb->v[q];
+                i = rb->i[q];
+            }
+        __syncthreads();
+        return i;
+    }
+    // pivot choice at column k (ldlt.jl:280-325): c = pivot column, r = 2x2 partner or -1
+    __device__ void choose(int k, const LdltPrm& p, int& c, int& r) const {
+        r = -1;
+        if (p.pivot == PV_NONE) {
+            c = k;
+            return;
+        }
+        if (p.pivot == PV_DIAG) {
+            if (accept_1x1(k, k, p)) {
+                c = k;
+                return;
+            }
+            const int best = best_1x1(k, p);
+            if (best >= 0) {
+                c = best;
+                return;
+            }
+            // the largest diagonal entry, first on ties; a NaN |a_kk| keeps k
+            double v = fabs(at(k, k));
+            int bi = k;
+            for (int j = k + 1 + threadIdx.x; j < w; j += NT) {
+                const double a = fabs(at(j, j));
+                if (a > v) {
+                    v = a;
+                    bi = j;
+                }
+            }
+            block_argmax<NT>(v, bi, *rb);
+            c = fabs(at(bi, bi)) >= p.eps ? bi : k;
+            return;
+        }
+        const double alpha = (1.0 + sqrt(17.0)) / 8.0;  // Bunch-Kaufman constant (ldlt.jl:22)
+        double lambda = 0.0;
+        int rr = -1;
+        for (int i = k + 1 + threadIdx.x; i < w; i += NT) {
+            const double a = fabs(at(i, k));
+            if (a > lambda) {
+                lambda = a;
+                rr = i;
+            }
+        }
+        block_argmax<NT>(lambda, rr, *rb);
+        if (rr < 0) lambda = 0.0;
+        const double akk = fabs(at(k, k));
+        int bc = k, br = -1;
+        if (rr < 0 || akk >= alpha * lambda) {
+            bc = k;
+        } else {
+            double sigma = 0.0;
+            for (int i = k + threadIdx.x; i < w; i += NT)
+                if (i != rr) sigma = jmax(sigma, fabs(at(i, rr)));
+            sigma = block_jmax<NT>(sigma, *rb);
+            if (akk * sigma >= alpha * lambda * lambda)
+                bc = k;
+            else if (fabs(at(rr, rr)) >= alpha * sigma)
+                bc = rr;
+            else {
+                bc = k;
+                br = rr;
+            }
+        }
+        const bool ok = br < 0 ? accept_1x1(bc, k, p) : accept_2x2(k, br, p);
+        if (ok) {
+            c = bc;
+            r = br;
+            return;
+        }
+        const int best = best_1x1(k, p);
+        if (best >= 0) {
+            c = best;
+            return;
+        }
+        if (br >= 0 && nonsingular_2x2(k, br, p)) {
+            c = bc;
+            r = br;
+            return;
+        }
+        c = bc;
+    }
+    // symmetric interchange of positions a and b: columns over all f rows, then rows over the
+    // w fully summed columns (ldlt.jl:328-340)
+    __device__ void swap(int a, int b) const {
+        __syncthreads();  // the pivot choice's last reads of F are done
+        if (a == b) return;
+        for (int i = threadIdx.x; i < f; i += NT) {
+            const T t = ref(i, a);
+            ref(i, a) = ref(i, b);
+            ref(i, b) = t;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < w; j += NT) {
+            const T t = ref(a, j);
+            ref(a, j) = ref(b, j);
+            ref(b, j) = t;
+        }
+        if (threadIdx.x == 0) {
+            const int t = lp[a];
+            lp[a] = lp[b];
+            lp[b] = t;
+        }
+        __syncthreads();
+    }
+};
+
+// Per-front outputs of the pivot loop.
+template <typename T>
+struct PivOut {
+    int8_t* kind;  // w
+    T* dd;         // w: D (1x1 value, or the 2x2 diagonal entries)
+    T* ee;         // w: 2x2 subdiagonal at the first position
+    int* st;       // NF statistics (shared)
+};
+
+// The pivot loop of one front (_ldlt_front!, ldlt.jl:384-454) run by all NT threads.
+// sreq (nullable): requested sign of each original local position (psign of its row).
+template <typename T, int NT>
+__device__ void ldlt_front(const Front<T, NT>& Fr, const LdltPrm& p, const int8_t* sreq_row,
+                           const int32_t* perm_c0, PivOut<T> out, double* sh) {
+    const int f = Fr.f, w = Fr.w, tid = threadIdx.x;
+    int k = 0;
+    while (k < w) {
+        int c, r;
+        Fr.choose(k, p, c, r);
+        if (r < 0) {
+            Fr.swap(k, c);
+            if (tid == 0) {
+                double d = Fr.at(k, k);
+                int8_t kd = KIND_1X1;
+                if (!(fabs(d) >= p.eps) || d == 0.0) {
+                    if (d == 0.0) out.st[ST_NZERO]++;
+                    const int req = sreq_row ? (int)sreq_row[perm_c0[Fr.lp[k]]] : 0;
+                    const double sg = req != 0 ? (double)req : (d < 0.0 ? -1.0 : 1.0);
+                    d = sg * p.eps;
+                    kd = KIND_PERTURBED;
+                    out.st[ST_NPERT]++;
+                    if (d == 0.0 && out.st[ST_INFO] == 0) out.st[ST_INFO] = Fr.lp[k] + 1;
+                }
+                Fr.ref(k, k) = T(d);
+                out.kind[k] = kd;
+                out.dd[k] = T(d);
+                out.ee[k] = T(0);
+                if (d > 0.0)
+                    out.st[ST_NPOS]++;
+                else if (d < 0.0)
+                    out.st[ST_NNEG]++;
+                sh[0] = d;
+            }
+            __syncthreads();
+            const T d = T(sh[0]);
+            for (int i = k + 1 + tid; i < f; i += NT) Fr.ref(i, k) = Fr.ref(i, k) / d;
+            __syncthreads();
+            const int R = f - k - 1, C = w - k - 1;
+            for (int q = tid; q < R * C; q += NT) {
+                const int j = k + 1 + q / R, i = k + 1 + q % R;
+                Fr.ref(i, j) = Fr.ref(i, j) - Fr.ref(i, k) * Fr.ref(k, j);
+            }
+            __syncthreads();
+            k += 1;
+        } else {
+            Fr.swap(k + 1, r);
+            if (tid == 0) {
+                const T a = Fr.ref(k, k), b = Fr.ref(k + 1, k), c2 = Fr.ref(k + 1, k + 1);
+                const T up = b;
+                Fr.ref(k, k + 1) = up;
+                const T det = a * c2 - up * b;
+                sh[0] = (double)(c2 / det);
+                sh[1] = (double)(-up / det);
+                sh[2] = (double)(-b / det);
+                sh[3] = (double)(a / det);
+                out.kind[k] = KIND_2X2_FIRST;
+                out.kind[k + 1] = KIND_2X2_SECOND;
+                out.dd[k] = a;
+                out.dd[k + 1] = c2;
+                out.ee[k] = b;
+                out.ee[k + 1] = T(0);
+                out.st[ST_N2X2]++;
+                if (det < T(0)) {
+                    out.st[ST_NPOS]++;
+                    out.st[ST_NNEG]++;
+                } else if (a > T(0)) {
+                    out.st[ST_NPOS] += 2;
+                } else {
+                    out.st[ST_NNEG] += 2;
+                }
+                Fr.ref(k + 1, k) = T(0);  // the reference clears it; nobody reads it below
+            }
+            __syncthreads();
+            const T e11 = T(sh[0]), e12 = T(sh[1]), e21 = T(sh[2]), e22 = T(sh[3]);
+            for (int i = k + 2 + tid; i < f; i += NT) {
+                const T x1 = Fr.ref(i, k), x2 = Fr.ref(i, k + 1);
+                Fr.ref(i, k) = x1 * e11 + x2 * e21;
+                Fr.ref(i, k + 1) = x1 * e12 + x2 * e22;
+            }
+            __syncthreads();
+            const int R = f - k - 2, C = w - k - 2;
+            for (int q = tid; q < R * C; q += NT) {
+                const int j = k + 2 + q / R, i = k + 2 + q % R;
+                Fr.ref(i, j) = Fr.ref(i, j) - (Fr.ref(i, k) * Fr.ref(k, j) + Fr.ref(i, k + 1) * Fr.ref(k + 1, j));
+            }
+            __syncthreads();
+            k += 2;
+        }
+    }
+}
+
+// W[i, k] = (L21 D)[i, k] for row i of L (column-major, ld f) given the pivot kinds and D
+// (ldlt.jl:206-231); k is a factor column of the front
+template <typename T>
+__device__ inline T w_entry(const T* L, int64_t f, int i, int k, const int8_t* kind, const T* dd, const T* ee) {
+    const int8_t kd = kind[k];
+    if (kd == KIND_2X2_FIRST) {
+        const T x1 = L[(int64_t)k * f + i], x2 = L[(int64_t)(k + 1) * f + i];
+        return x1 * dd[k] + x2 * ee[k];
+    }
+    if (kd == KIND_2X2_SECOND) {
+        const T x1 = L[(int64_t)(k - 1) * f + i], x2 = L[(int64_t)k * f + i];
+        return x1 * ee[k - 1] + x2 * dd[k];
+    }
+    return L[(int64_t)k * f + i] * dd[k];
+}
+
+template <typename T>
+__device__ inline double eps_eff(double eps, int scaled, const double* aux) {
+    if (!scaled) return eps;
+    const double a = aux[0];
+    return a > 0 ? eps * a : eps;
+}
+
+// ---------------------------------------------------------------------------
+// small fronts: one workgroup, the dense front in LDS
+
+constexpr int LDLT_SMALL_W = 128;  // largest front of the small class (f <= this)
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_small_ldlt(DevSym S, const int32_t* __restrict__ fronts,
+                                                         const double* __restrict__ nz, T* __restrict__ factor,
+                                                         T* __restrict__ cbpool, T* __restrict__ dvec,
+                                                         int32_t* __restrict__ piv, int8_t* __restrict__ pkind,
+                                                         int64_t* __restrict__ stats, LdltPrm prm, int scaled,
+                                                         const double* __restrict__ aux,
+                                                         const int8_t* __restrict__ psign) {
+    constexpr int NT = 256;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
+    __shared__ RedBuf<NT> rb;
+    __shared__ int lp[LDLT_SMALL_W];
+    __shared__ int8_t kind[LDLT_SMALL_W];
+    __shared__ T dd[LDLT_SMALL_W], ee[LDLT_SMALL_W];
+    __shared__ int st[NF];
+    __shared__ double sh[4];
+    const int s = fronts[blockIdx.x];
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    prm.eps = eps_eff<T>(prm.eps, scaled, aux);
+    for (int k = tid; k < w; k += NT) {
+        lp[k] = k;
+        kind[k] = 0;
+        dd[k] = ee[k] = T(0);
+    }
+    if (tid < NF) st[tid] = 0;
+    assemble_dense<T>(S, s, nz, cbpool, F, f, 0);
+    // the upper triangle of the fully summed block mirrors the lower one
+    for (int q = tid; q < w * w; q += NT) {
+        const int j = q / w, i = q - j * w;
+        if (i > j) F[i * f + j] = F[j * f + i];
+    }
+    __syncthreads();
+    Front<T, NT> Fr{F, f, w, lp, &rb};
+    ldlt_front<T, NT>(Fr, prm, psign, S.perm + c0, PivOut<T>{kind, dd, ee, st}, sh);
+    // W = L21 D, then the lower triangle of F22 <- F22 - W L21^T, k in order
+    const int64_t co = S.cb_off[s];
+    if (co >= 0) {
+        T* C = cbpool + co;
+        for (int q = tid; q < m * m; q += NT) {
+            const int jj = q / m, ii = q - jj * m;
+            if (ii < jj) continue;
+            T acc = F[(w + jj) * f + w + ii];
+            for (int k = 0; k < w; ++k) acc = acc - w_entry<T>(F, f, w + ii, k, kind, dd, ee) * F[k * f + w + jj];
+            C[q] = acc;
+        }
+    }
+    T* P = factor + S.panel_off[s];
+    for (int q = tid; q < f * w; q += NT) {
+        const int j = q / f, i = q - j * f;
+        P[q] = i > j ? F[q] : (i == j ? T(1) : T(0));
+    }
+    for (int k = tid; k < w; k += NT) {
+        const int g = c0 + k;
+        piv[g] = c0 + lp[k];
+        pkind[g] = kind[k];
+        dvec[g] = dd[k];
+        dvec[S.n + g] = ee[k];
+    }
+    if (tid < NF) stats[(int64_t)s * NF + tid] = st[tid];
+}
+
+// ---------------------------------------------------------------------------
+// large fronts: the panel in place, one workgroup of 1024 threads per front.
+// task: (s). The local pivot order is kept in piv (c0 + k) during the loop.
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_big_ldlt_panel(DevSym S, const Task* __restrict__ tasks,
+                                                        T* __restrict__ factor, T* __restrict__ dvec,
+                                                        int32_t* __restrict__ piv, int8_t* __restrict__ pkind,
+                                                        int64_t* __restrict__ stats, LdltPrm prm, int scaled,
+                                                        const double* __restrict__ aux,
+                                                        const int8_t* __restrict__ psign) {
+    constexpr int NT = 1024;
+    __shared__ RedBuf<NT> rb;
+    __shared__ int st[NF];
+    __shared__ double sh[4];
+    const int s = tasks[blockIdx.x].s;
+    const int f = S.fnrows[s], w = S.fncols[s];
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    prm.eps = eps_eff<T>(prm.eps, scaled, aux);
+    T* P = factor + S.panel_off[s];
+    int* lp = piv + c0;
+    int8_t* kind = pkind + c0;
+    T* dd = dvec + c0;
+    T* ee = dvec + S.n + c0;
+    for (int k = tid; k < w; k += NT) {
+        lp[k] = k;
+        kind[k] = 0;
+        dd[k] = ee[k] = T(0);
+    }
+    if (tid < NF) st[tid] = 0;
+    for (int q = tid; q < w * w; q += NT) {
+        const int j = q / w, i = q - j * w;
+        if (i > j) P[(int64_t)i * f + j] = P[(int64_t)j * f + i];
+    }
+    __syncthreads();
+    Front<T, NT> Fr{P, f, w, lp, &rb};
+    ldlt_front<T, NT>(Fr, prm, psign, S.perm + c0, PivOut<T>{kind, dd, ee, st}, sh);
+    // unit diagonal, empty upper triangle (the contribution-block kernel reads L21 only)
+    for (int q = tid; q < w * w; q += NT) {
+        const int j = q / w, i = q - j * w;
+        if (i <= j) P[(int64_t)j * f + i] = i == j ? T(1) : T(0);
+    }
+    for (int k = tid; k < w; k += NT) lp[k] = c0 + lp[k];
+    if (tid < NF) stats[(int64_t)s * NF + tid] = st[tid];
+}
+
+// Contribution block of a large front: the 64 x 64 tile (I0, J0) (front rows, I0 >= J0 >= w)
+// of the lower triangle of F22 <- F22 - W L21^T, each entry subtracting W[i, k] L[j, k] for
+// k = 0, 1, ..., w - 1 in order. Chunks of 16 columns of W and L21 are staged in LDS.
+// Thread (ty, tx) owns rows I0 + 4 ty + a and columns J0 + 4 tx + b (a, b < 4).
+template <typename T>
+__global__ void __launch_bounds__(256) k_ldlt_cb(DevSym S, const Task* __restrict__ tasks,
+                                                const T* __restrict__ factor, T* __restrict__ cbpool,
+                                                const T* __restrict__ dvec, const int8_t* __restrict__ pkind) {
+    constexpr int KC = 16;
+    __shared__ T Ws[KC][TILE];
+    __shared__ T Ls[KC][TILE];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int I0 = t.a, J0 = t.b;
+    const int c0 = S.super_ptr[s];
+    const T* L = factor + S.panel_off[s];
+    T* C = cbpool + S.cb_off[s];
+    const int8_t* kind = pkind + c0;
+    const T* dd = dvec + c0;
+    const T* ee = dvec + S.n + c0;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = I0 + 4 * ty + a, j = J0 + 4 * tx + b;
+            acc[a][b] = (i < f && j <= i) ? C[(int64_t)(j - w) * m + (i - w)] : T(0);
+        }
+    for (int k0 = 0; k0 < w; k0 += KC) {
+        for (int q = tid; q < KC * TILE; q += 256) {
+            const int kk = q / TILE, c = q - kk * TILE, k = k0 + kk;
+            const int i = I0 + c, j = J0 + c;
+            Ws[kk][c] = (k < w && i < f) ? w_entry<T>(L, f, i, k, kind, dd, ee) : T(0);
+            Ls[kk][c] = (k < w && j < f) ? L[(int64_t)k * f + j] : T(0);
+        }
+        __syncthreads();
+        const int kn = min(KC, w - k0);
+        for (int kk = 0; kk < kn; ++kk) {
+            T wv[4], lv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) wv[a] = Ws[kk][4 * ty + a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) lv[b] = Ls[kk][4 * tx + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] - wv[a] * lv[b];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = I0 + 4 * ty + a, j = J0 + 4 * tx + b;
+            if (i < f && j <= i) C[(int64_t)(j - w) * m + (i - w)] = acc[a][b];
+        }
+}
+
+// aux[0] = max |nzval| (the scale of pivot_epsilon_alg = "algo1"): non-negative doubles
+// order like their bit patterns, so an integer atomic max is exact; a NaN wins, as the
+// reference's NaN-propagating max. aux[0] must be zero on entry.
+__global__ void k_absmax(int64_t nnz, const double* __restrict__ nz, unsigned long long* __restrict__ aux) {
+    unsigned long long m = 0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long b = __double_as_longlong(fabs(nz[q]));
+        m = b > m ? b : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(aux, m);
+}
+
+// info[0] = the smallest failed factor column (1-based; 0 = none) over the fronts'
+// statistics (ldlt.jl:208-216)
+__global__ void __launch_bounds__(256) k_info_from_stats(DevSym S, const int64_t* __restrict__ stats,
+                                                        int64_t* __restrict__ info) {
+    __shared__ int64_t best[256];
+    int64_t mn = INT64_MAX;
+    for (int s = threadIdx.x; s < S.ns; s += blockDim.x) {
+        const int64_t fi = stats[(int64_t)s * NF + ST_INFO];
+        if (fi > 0) mn = min(mn, (int64_t)S.super_ptr[s] + fi);
+    }
+    best[threadIdx.x] = mn;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) best[threadIdx.x] = min(best[threadIdx.x], best[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) info[0] = best[0] == INT64_MAX ? 0 : best[0];
+}
+
+// ---------------------------------------------------------------------------
+
+template <typename T>
+void launch_front_small_ldlt(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                             const double* nz, T* factor, T* cbpool, T* dvec, int32_t* piv, int8_t* kind,
+                             int64_t* stats, int pivot, double u, double eps, int scaled, const double* aux,
+                             const int8_t* psign) {
+    size_t lds = (size_t)fmax * fmax * sizeof(T);
+    LdltPrm p{pivot, u, eps};
+    hipLaunchKernelGGL(k_front_small_ldlt<T>, dim3(nfronts), dim3(256), lds, st, S, fronts, nz, factor, cbpool, dvec,
+                       piv, kind, stats, p, scaled, aux, psign);
+}
+
+template <typename T>
+void launch_big_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* dvec,
+                           int32_t* piv, int8_t* kind, int64_t* stats, int pivot, double u, double eps, int scaled,
+                           const double* aux, const int8_t* psign) {
+    LdltPrm p{pivot, u, eps};
+    hipLaunchKernelGGL(k_big_ldlt_panel<T>, dim3(n), dim3(1024), 0, st, S, tasks, factor, dvec, piv, kind, stats, p,
+                       scaled, aux, psign);
+}
+
+template <typename T>
+void launch_ldlt_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* cbpool,
+                    const T* dvec, const int8_t* kind) {
+    hipLaunchKernelGGL(k_ldlt_cb<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, dvec, kind);
+}
+
+void launch_absmax(hipStream_t st, int64_t nnz, const double* nz, double* aux) {
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)std::min<int64_t>(std::max<int64_t>((nnz + 255) / 256, 1), 1024)),
+                       dim3(256), 0, st, nnz, nz, reinterpret_cast<unsigned long long*>(aux));
+}
+
+void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stats, int64_t* info) {
+    hipLaunchKernelGGL(k_info_from_stats, dim3(1), dim3(256), 0, st, S, stats, info);
+}
+
+#define INST(T)                                                                                                    \
+    template void launch_front_small_ldlt<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
+                                             T*, T*, T*, int32_t*, int8_t*, int64_t*, int, double, double, int,   \
+                                             const double*, const int8_t*);                                        \
+    template void launch_big_ldlt_panel<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*,        \
+                                           int8_t*, int64_t*, int, double, double, int, const double*,             \
+                                           const int8_t*);                                                         \
+    template void launch_ldlt_cb<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, const T*,         \
+                                    const int8_t*);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace sds

@@ -39,6 +39,13 @@ enum LaunchKind {
     L_TRSM,
     L_UPDATE_PANEL,
     L_UPDATE_CB,
+    L_LDLT_PANEL,
+    L_LDLT_CB,
+    L_LU_ZERO,
+    L_LU_SCATTER,
+    L_LU_EXTEND,
+    L_LU_PANEL,
+    L_LU_CB,
     S_FWD_PULL,
     S_FWD_STEP,
     S_FWD_FINISH,
@@ -77,9 +84,17 @@ int prof_kind(int kind) {
         case L_POTRF: return SDS_K_POTRF;
         case L_TRSM: return SDS_K_TRSM;
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
+        case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
+        case L_LU_PANEL: return SDS_K_PIVOT_PANEL;
+        case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
+        case L_LU_SCATTER: return SDS_K_ZERO_SCATTER;
+        case L_LU_EXTEND: return SDS_K_EXTEND_ADD;
         default: return SDS_K_UPDATE_CB;
     }
 }
+
+// factorization method of a structure
+enum Method { M_CHOLESKY = 0, M_LDLT = 1, M_LU = 2 };
 
 struct ProfRec {
     int kind, e0, e1;
@@ -99,6 +114,7 @@ struct sds_handle {
     bool have_sym = false;
     int64_t n = 0, nnz = 0, ns = 0, factor_len = 0;
     int structure = 3;
+    int method = M_CHOLESKY;
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
     std::vector<DevBuf> symbufs;  // device maps
@@ -117,9 +133,15 @@ struct sds_handle {
     int64_t* d_cb_off = nullptr;
     int64_t* d_uv_off = nullptr;
     void* factor = nullptr;
+    void* ufactor = nullptr;  // U^T panels of "G"
     void* cbpool = nullptr;
     void* dbuf = nullptr;  // factored diagonal blocks of the large fronts (NB entries per column)
     int32_t* status = nullptr;
+    void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
+    int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
+    int8_t* pkind = nullptr;     // n: pivot kinds
+    int8_t* psign = nullptr;     // n: requested pivot signs by original row (pivot_sign)
+    double* aux = nullptr;       // max |a_ij| of the values (pivot_epsilon_alg = "algo1")
     int64_t* stats = nullptr;
     int64_t* d_info = nullptr;
     int64_t* h_info = nullptr;  // pinned
@@ -293,6 +315,7 @@ int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& fi
 
 void free_numeric(sds_handle* h) {
     free_all(h->numbufs);
+    h->ufactor = nullptr;
     if (h->h_info) (void)hipHostFree(h->h_info);
     if (h->h_norms) (void)hipHostFree(h->h_norms);
     h->h_info = nullptr;
@@ -311,7 +334,9 @@ int build_schedule(sds_handle* h, int dtype) {
     const int64_t ns = h->ns;
     const size_t es = dtype == SDS_R64 ? 8 : 4;
     // largest front whose dense f x f square fits 64 KiB of LDS next to the kernel's static words
-    int fsmall = (int)std::floor(std::sqrt((65536.0 - 256.0) / (double)es));
+    // (the pivoting kernels keep per-column pivot data in LDS too)
+    int fsmall = h->method == M_CHOLESKY ? (int)std::floor(std::sqrt((65536.0 - 256.0) / (double)es))
+                                         : std::min(128, (int)std::floor(std::sqrt((65536.0 - 4096.0) / (double)es)));
     h->fsmall = fsmall;
     // tree levels by height: leaves 0, a parent one above its highest child
     std::vector<int> lev(ns, 0);
@@ -419,7 +444,16 @@ int build_schedule(sds_handle* h, int dtype) {
             for (int c = 0; c < h->fnrows[s]; c += 8) t.push_back({s, c, 8, 0});
             by += dbl_es * (f * w + m * m) + 16.0 * (double)h->asm_count[s];
         }
-        push_launch(L, L_ZERO_SCATTER, t, 0, by);
+        const bool lu = h->method == M_LU;
+        if (lu) {
+            push_launch(L, L_LU_ZERO, t, 0, by);
+            t.clear();
+            for (int32_t s : L.big)
+                for (int c = 0; c < h->fncols[s]; c += 8) t.push_back({s, c, 8, 0});
+            push_launch(L, L_LU_SCATTER, t, 0, 0);
+        } else {
+            push_launch(L, L_ZERO_SCATTER, t, 0, by);
+        }
         int maxch = 0;
         for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
         for (int r = 0; r < maxch; ++r) {
@@ -431,12 +465,26 @@ int build_schedule(sds_handle* h, int dtype) {
                 if (cb_off[c] < 0) continue;
                 int mc = h->fnrows[c] - h->fncols[c];
                 for (int jj = 0; jj < mc; jj += 16) t.push_back({s, c, jj, 16});
-                by += 3.0 * dbl_es * (double)mc * (mc + 1) / 2;  // read the child's block, read+write the parent
+                // read the child's block, read + write the parent
+                by += 3.0 * dbl_es * (lu ? (double)mc * mc : (double)mc * (mc + 1) / 2);
             }
-            push_launch(L, L_EXTEND_ADD, t, 0, by);
+            push_launch(L, lu ? L_LU_EXTEND : L_EXTEND_ADD, t, 0, by);
         }
         int wmax = 0;
         for (int32_t s : L.big) wmax = std::max(wmax, h->fncols[s]);
+        if (h->method != M_CHOLESKY) {
+            // pivoted fronts: the whole panel by one workgroup per front
+            t.clear();
+            fl = by = 0;
+            for (int32_t s : L.big) {
+                double f = h->fnrows[s], w = h->fncols[s];
+                t.push_back({s, 0, 0, 0});
+                for (int k = 0; k < (int)w; ++k) fl += 2.0 * (f - k) * (w - k);
+                by += 2.0 * dbl_es * f * w;
+            }
+            push_launch(L, lu ? L_LU_PANEL : L_LDLT_PANEL, t, fl, by);
+            wmax = 0;
+        }
         for (int k = 0; k < wmax; k += NB) {
             std::vector<Task> tp, tt, tu;
             double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0;
@@ -475,15 +523,16 @@ int build_schedule(sds_handle* h, int dtype) {
             int f = h->fnrows[s], w = h->fncols[s];
             if (f == w || cb_off[s] < 0 || h->snparent[s] < 0) continue;
             for (int I0 = w; I0 < f; I0 += TILE)
-                for (int J0 = w; J0 <= I0; J0 += TILE) {
+                for (int J0 = w; J0 < (lu ? f : I0 + 1); J0 += TILE) {
                     t.push_back({s, I0, J0, 0});
-                    double e = tile_entries(I0, J0, f, w, f);
+                    double e = lu ? (double)std::min(TILE, f - I0) * std::min(TILE, f - J0)
+                                  : tile_entries(I0, J0, f, w, f);
                     fl += 2.0 * e * w;
                     by += 2.0 * dbl_es * e;
                 }
             by += dbl_es * (double)(f - w) * w;  // L21 read once
         }
-        push_launch(L, L_UPDATE_CB, t, fl, by);
+        push_launch(L, h->method == M_CHOLESKY ? L_UPDATE_CB : lu ? L_LU_CB : L_LDLT_CB, t, fl, by);
     }
     void* p;
     int rc;
@@ -613,6 +662,89 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     return SDS_SUCCESS;
 }
 
+// LDL^T (src/numeric/ldlt.jl:1447-1475 drives the same steps): pivot 0 none, 1 diagonal,
+// 2 Bunch-Kaufman; `scaled`: eps times max |a_ij| (pivot_epsilon_alg = "algo1")
+template <typename T>
+int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double eps, int scaled,
+                   const int8_t* psign, const volatile int32_t* interrupt) {
+    hipStream_t st = h->stream;
+    T* factor = static_cast<T*>(h->factor);
+    T* cb = static_cast<T*>(h->cbpool);
+    T* dv = static_cast<T*>(h->dvec);
+    if (scaled) {
+        launch_fill_i32(st, reinterpret_cast<int32_t*>(h->aux), 2, 0);
+        launch_absmax(st, h->nnz, nz, h->aux);
+    }
+    for (auto& L : h->levels) {
+        if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
+        if (!L.small.empty())
+            timed(h, SDS_K_FRONT_SMALL, L.small_flops, L.small_bytes, [&] {
+                launch_front_small_ldlt<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small,
+                                           nz, factor, cb, dv, h->piv, h->pkind, h->stats, pivot, u, eps, scaled,
+                                           h->aux, psign);
+            });
+        for (const Launch& ln : L.plan) {
+            const Task* t = h->d_tasks + ln.off;
+            timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
+                switch (ln.kind) {
+                    case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
+                    case L_LDLT_PANEL:
+                        launch_big_ldlt_panel<T>(st, h->dsym, t, ln.n, factor, dv, h->piv, h->pkind, h->stats, pivot,
+                                                 u, eps, scaled, h->aux, psign);
+                        break;
+                    case L_LDLT_CB: launch_ldlt_cb<T>(st, h->dsym, t, ln.n, factor, cb, dv, h->pkind); break;
+                }
+            });
+        }
+    }
+    timed(h, SDS_K_STATS, 0, 0, [&] { launch_info_from_stats(st, h->dsym, h->stats, h->d_info); });
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
+}
+
+// LU of "G" (src/numeric/lu.jl:543-555 drives the same steps)
+template <typename T>
+int factorize_lu(sds_handle* h, const double* nz, int pivoting, double u, double eps, int scaled,
+                 const volatile int32_t* interrupt) {
+    hipStream_t st = h->stream;
+    T* factor = static_cast<T*>(h->factor);
+    T* uf = static_cast<T*>(h->ufactor);
+    T* cb = static_cast<T*>(h->cbpool);
+    T* dv = static_cast<T*>(h->dvec);
+    if (scaled) {
+        launch_fill_i32(st, reinterpret_cast<int32_t*>(h->aux), 2, 0);
+        launch_absmax(st, h->nnz, nz, h->aux);
+    }
+    for (auto& L : h->levels) {
+        if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
+        if (!L.small.empty())
+            timed(h, SDS_K_FRONT_SMALL, L.small_flops, L.small_bytes, [&] {
+                launch_front_small_lu<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small,
+                                         nz, factor, uf, cb, dv, h->piv, h->pkind, h->stats, pivoting, u, eps, scaled,
+                                         h->aux);
+            });
+        for (const Launch& ln : L.plan) {
+            const Task* t = h->d_tasks + ln.off;
+            timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
+                switch (ln.kind) {
+                    case L_LU_ZERO: launch_big_lu_zero<T>(st, h->dsym, t, ln.n, factor, uf, cb); break;
+                    case L_LU_SCATTER: launch_big_lu_scatter<T>(st, h->dsym, t, ln.n, nz, factor, uf); break;
+                    case L_LU_EXTEND: launch_big_lu_extend_add<T>(st, h->dsym, t, ln.n, factor, uf, cb); break;
+                    case L_LU_PANEL:
+                        launch_big_lu_panel<T>(st, h->dsym, t, ln.n, factor, uf, dv, h->piv, h->pkind, h->stats,
+                                               pivoting, u, eps, scaled, h->aux);
+                        break;
+                    case L_LU_CB: launch_lu_cb<T>(st, h->dsym, t, ln.n, factor, uf, cb, dv); break;
+                }
+            });
+        }
+    }
+    timed(h, SDS_K_STATS, 0, 0, [&] { launch_info_from_stats(st, h->dsym, h->stats, h->d_info); });
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
+}
+
 int ensure_solve_workspace(sds_handle* h, int nrhs) {
     if (nrhs <= h->cap_nrhs) return SDS_SUCCESS;
     size_t es = h->dtype == SDS_R64 ? 8 : 4;
@@ -638,26 +770,35 @@ int ensure_solve_workspace(sds_handle* h, int nrhs) {
     return SDS_SUCCESS;
 }
 
+// The forward sweep runs with the lower panels `ffac` (the local row order `fpiv` applied
+// first), the backward sweep with the transposes of `bfac` (`bpiv` undone last):
+// Cholesky L / L; LDL^T L + piv / L + piv; LU L + piv / U^T; LU transposed U^T / L + piv.
 template <typename T>
-void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd) {
+void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed) {
     hipStream_t st = h->stream;
-    const T* factor = static_cast<const T*>(h->factor);
+    const T* L = static_cast<const T*>(h->factor);
+    const T* Ut = h->ufactor ? static_cast<const T*>(h->ufactor) : L;
     T* Y = static_cast<T*>(h->Y.p);
     T* Z = static_cast<T*>(h->Z.p);
     T* U = static_cast<T*>(h->U.p);
-    const int unit = 0;  // Cholesky
-    const int32_t* piv = nullptr;
+    const int unit = h->method == M_CHOLESKY ? 0 : 1;
+    const int32_t* pv = h->method == M_CHOLESKY ? nullptr : h->piv;
+    const bool lu = h->method == M_LU;
+    const T* ffac = lu && transposed ? Ut : L;
+    const T* bfac = lu && !transposed ? Ut : L;
+    const int32_t* fpiv = lu && transposed ? nullptr : pv;
+    const int32_t* bpiv = lu && !transposed ? nullptr : pv;
     const double es = sizeof(T);
     auto run = [&](const Launch& ln, int kind) {
         const Task* t = h->d_tasks + ln.off;
         timed(h, kind, ln.flops * nrhs, ln.bytes + (ln.flops > 0 ? es * nrhs * ln.flops / 16 : 0), [&] {
             switch (ln.kind) {
-                case S_FWD_PULL: launch_fwd_pull<T>(st, h->dsym, t, ln.n, Y, U, Z, nrhs, piv); break;
-                case S_FWD_STEP: launch_fwd_step<T>(st, h->dsym, t, ln.n, factor, Y, U, Z, nrhs, unit); break;
+                case S_FWD_PULL: launch_fwd_pull<T>(st, h->dsym, t, ln.n, Y, U, Z, nrhs, fpiv); break;
+                case S_FWD_STEP: launch_fwd_step<T>(st, h->dsym, t, ln.n, ffac, Y, U, Z, nrhs, unit); break;
                 case S_FWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, nullptr); break;
-                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, factor, Y, nrhs); break;
-                case S_BWD_STEP: launch_bwd_step<T>(st, h->dsym, t, ln.n, factor, Y, Z, nrhs, unit); break;
-                case S_BWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, piv); break;
+                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, bfac, Y, nrhs); break;
+                case S_BWD_STEP: launch_bwd_step<T>(st, h->dsym, t, ln.n, bfac, Y, Z, nrhs, unit); break;
+                case S_BWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, bpiv); break;
             }
         });
     };
@@ -665,25 +806,28 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd) {
         for (auto& L : h->levels) {
             if (!L.ssmall.empty())
                 timed(h, SDS_K_FWD, L.fwd_flops * nrhs, L.fwd_bytes + es * nrhs * L.rhs_rows, [&] {
-                    launch_fwd<T>(st, h->dsym, h->d_fronts + L.off_ssmall, (int)L.ssmall.size(), factor, Y, U, Z,
-                                  nrhs, unit, piv);
+                    launch_fwd<T>(st, h->dsym, h->d_fronts + L.off_ssmall, (int)L.ssmall.size(), ffac, Y, U, Z,
+                                  nrhs, unit, fpiv);
                 });
             for (const Launch& ln : L.fwd_plan) run(ln, SDS_K_FWD);
         }
-    (void)diag;
+    if (diag && h->method != M_CHOLESKY)
+        timed(h, SDS_K_DIAG, 2.0 * h->n * nrhs, es * (2.0 * h->n + 2.0 * h->n * nrhs) + h->n, [&] {
+            launch_diag<T>(st, (int)h->n, nrhs, static_cast<const T*>(h->dvec), lu ? nullptr : h->pkind, Y);
+        });
     if (bwd)
         for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
             if (!it->ssmall.empty())
                 timed(h, SDS_K_BWD, it->fwd_flops * nrhs, it->fwd_bytes + es * nrhs * it->rhs_rows, [&] {
-                    launch_bwd<T>(st, h->dsym, h->d_fronts + it->off_ssmall, (int)it->ssmall.size(), factor, Y, Z,
-                                  nrhs, unit, piv);
+                    launch_bwd<T>(st, h->dsym, h->d_fronts + it->off_ssmall, (int)it->ssmall.size(), bfac, Y, Z,
+                                  nrhs, unit, bpiv);
                 });
             for (const Launch& ln : it->bwd_plan) run(ln, SDS_K_BWD);
         }
 }
 
 template <typename T>
-int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
+int solve_t(sds_handle* h, int32_t phase, int transposed, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
             const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
     hipStream_t st = h->stream;
     int rc = ensure_solve_workspace(h, nrhs);
@@ -700,7 +844,8 @@ int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* 
     const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
     if (phase & SDS_PHASE_SOLVE_FWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y); });
-    sweeps<T>(h, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD);
+    sweeps<T>(h, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD,
+              transposed);
     if (phase & SDS_PHASE_SOLVE_BWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0); });
     int done = 0;
@@ -712,8 +857,9 @@ int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* 
             // one SpMV: values + column + source index per entry, X gathered, B read, R written
             timed(h, SDS_K_RESIDUAL, 2.0 * h->nnz_full * nrhs,
                   (double)h->nnz_full * (8 + 4 + 8) + 3.0 * 8 * n * nrhs, [&] {
-                      launch_residual(st, n, nrhs, h->full_rowptr, h->full_colval, h->full_src, nz, X, ldx, Bsrc,
-                                      Bsrc == B ? ldb : n, R);
+                      // the rows n..2n-1 of a "G" map are those of M^T
+                      launch_residual(st, n, nrhs, h->full_rowptr + (transposed && h->method == M_LU ? n : 0),
+                                      h->full_colval, h->full_src, nz, X, ldx, Bsrc, Bsrc == B ? ldb : n, R);
                   });
             if (ir_tol > 0) {
                 // host copies of the residual and rhs norms (one synchronization per step)
@@ -734,7 +880,7 @@ int solve_t(sds_handle* h, int32_t phase, const double* B, int64_t ldb, double* 
                 if (ok) break;
             }
             timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y); });
-            sweeps<T>(h, nrhs, 1, 1, 1);
+            sweeps<T>(h, nrhs, 1, 1, 1, transposed);
             timed(h, SDS_K_PERMUTE, 0, pbytes + 8.0 * n * nrhs,
                   [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1); });
             ++done;
@@ -780,8 +926,8 @@ int sds_set_stream(sds_handle_t h, void* stream) {
 int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     if (!h || !d) return SDS_INVALID_VALUE;
     if (d->index_bits != 32 && d->index_bits != 64) return fail(h, SDS_INVALID_VALUE, "index_bits must be 32 or 64");
-    if (d->structure != SDS_STRUCTURE_SPD && d->structure != SDS_STRUCTURE_HPD)
-        return fail(h, SDS_NOT_SUPPORTED, "only the Cholesky structures \"SPD\"/\"HPD\" are available in this build");
+    if (d->structure < SDS_STRUCTURE_G || d->structure > SDS_STRUCTURE_HPD)
+        return fail(h, SDS_INVALID_VALUE, "unknown structure");
     free_numeric(h);
     free_all(h->symbufs);
     h->have_sym = false;
@@ -791,6 +937,9 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     h->ns = ns;
     h->nnz = nnz;
     h->structure = d->structure;
+    h->method = (d->structure == SDS_STRUCTURE_SPD || d->structure == SDS_STRUCTURE_HPD) ? M_CHOLESKY
+                : d->structure == SDS_STRUCTURE_G                                      ? M_LU
+                                                                                       : M_LDLT;
     IdxReader perm{d->perm, b}, sptr{d->super_ptr, b}, spar{d->snparent, b}, rptr{d->rowptr, b}, rval{d->rowval, b},
         fptr{d->front_ptr, b}, fnr{d->front_nrows, b}, fnc{d->front_ncols, b}, cptr{d->child_ptr, b},
         clist{d->child_list, b}, riptr{d->relind_ptr, b}, rind{d->relind, b}, amap{d->amap, b},
@@ -927,9 +1076,22 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     void* p;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->factor))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->cb_len, 1), &h->cbpool))) return rc;
+    h->ufactor = nullptr;
+    if (h->method == M_LU &&
+        (rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->ufactor)))
+        return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->dbuf))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
     h->status = static_cast<int32_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, es * 2 * std::max<int64_t>(h->n, 1), &h->dvec))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * std::max<int64_t>(h->n, 1), &p))) return rc;
+    h->piv = static_cast<int32_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, std::max<int64_t>(h->n, 1), &p))) return rc;
+    h->pkind = static_cast<int8_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, std::max<int64_t>(h->n, 1), &p))) return rc;
+    h->psign = static_cast<int8_t*>(p);
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(double) * 2, &p))) return rc;
+    h->aux = static_cast<double*>(p);
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * NF * (h->ns + 1), &p))) return rc;
     h->stats = static_cast<int64_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * 8, &p))) return rc;
@@ -945,13 +1107,51 @@ int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* 
     if (!nzval && h->nnz > 0) return fail(h, SDS_INVALID_VALUE, "nzval is NULL");
     double eps = h->dtype == SDS_R64 ? 1e-13 : 1e-5;  // default_pivot_epsilon (src/options.jl:57-61)
     const volatile int32_t* intr = nullptr;
+    int ptype = 0, scaled = 0;
+    double u = 0.01;
+    const int8_t* psign = nullptr;
     if (prm) {
         if (prm->pivot_epsilon >= 0) eps = prm->pivot_epsilon;
         intr = prm->interrupt;
+        ptype = prm->pivot_type;
+        u = prm->pivot_threshold;
+        scaled = prm->pivot_epsilon_alg == 1;
+        if (prm->pivot_epsilon_alg < 0 || prm->pivot_epsilon_alg > 2)
+            return fail(h, SDS_INVALID_VALUE, "pivot_epsilon_alg must be 0, 1 or 2");
+        if (!(u >= 0.0 && u <= 1.0)) return fail(h, SDS_INVALID_VALUE, "pivot_threshold must be in [0, 1]");
+        if (prm->pivot_sign) {
+            HIPCHK(h, hipMemcpyAsync(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice, h->stream));
+            psign = h->psign;
+        }
     }
     h->factorized = false;
-    int rc = h->dtype == SDS_R64 ? factorize_cholesky<double>(h, nzval, eps, intr)
+    int rc;
+    if (h->method == M_LDLT) {
+        // pivot_type (src/types.jl:84-92): 'N' none, 'D' diagonal, 'A'/'L'/'B' Bunch-Kaufman
+        // inside the fully summed block (src/reference/ldlt.jl:44-51)
+        int pivot;
+        switch (ptype) {
+            case 1: pivot = 0; break;
+            case 4: pivot = 1; break;
+            case 0: case 5: case 6: pivot = 2; break;
+            default: return fail(h, SDS_NOT_SUPPORTED, "pivot_type is not supported by LDL^T");
+        }
+        rc = h->dtype == SDS_R64 ? factorize_ldlt<double>(h, nzval, pivot, u, eps, scaled, psign, intr)
+                                 : factorize_ldlt<float>(h, nzval, pivot, u, eps, scaled, psign, intr);
+    } else if (h->method == M_LU) {
+        // 'N' and 'D' keep the diagonal pivots, 'A'/'L'/'B' pivot by rows (src/reference/lu.jl:26-38)
+        int pivoting;
+        switch (ptype) {
+            case 1: case 4: pivoting = 0; break;
+            case 0: case 5: case 6: pivoting = 1; break;
+            default: return fail(h, SDS_NOT_SUPPORTED, "pivot_type is not supported by LU");
+        }
+        rc = h->dtype == SDS_R64 ? factorize_lu<double>(h, nzval, pivoting, u, eps, scaled, intr)
+                                 : factorize_lu<float>(h, nzval, pivoting, u, eps, scaled, intr);
+    } else {
+        rc = h->dtype == SDS_R64 ? factorize_cholesky<double>(h, nzval, eps, intr)
                                  : factorize_cholesky<float>(h, nzval, eps, intr);
+    }
     if (rc) return rc;
     HIPCHK(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -970,9 +1170,11 @@ int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B
     if (solve_mode != 0 && solve_mode != 1 && solve_mode != 2)
         return fail(h, SDS_INVALID_VALUE, "solve_mode must be 0, 1 or 2");
     if ((phase & ~SDS_PHASE_SOLVE) != 0 || phase == 0) return fail(h, SDS_INVALID_VALUE, "not a solve phase");
-    // symmetric structures: A^T = A, and for real values A^H = A
-    return h->dtype == SDS_R64 ? solve_t<double>(h, phase, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
-                               : solve_t<float>(h, phase, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+    // symmetric structures: A^T = A; real values: A^H = A^T (src/solve/refinement.jl:10-15)
+    const int tr = h->method == M_LU && solve_mode != 0;
+    return h->dtype == SDS_R64
+               ? solve_t<double>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
+               : solve_t<float>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
 }
 
 int sds_get_stats(sds_handle_t h, int64_t* stats, int64_t* totals) {
@@ -993,6 +1195,15 @@ int sds_get_stats(sds_handle_t h, int64_t* stats, int64_t* totals) {
 
 int sds_get_pivots(sds_handle_t h, int32_t* piv, int8_t* kind) {
     if (!h || h->dtype < 0) return SDS_INVALID_VALUE;
+    if (h->method != M_CHOLESKY) {
+        std::vector<int32_t> p(h->n);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipMemcpy(p.data(), h->piv, sizeof(int32_t) * h->n, hipMemcpyDeviceToHost));
+        if (piv)
+            for (int64_t k = 0; k < h->n; ++k) piv[k] = p[k] + 1;
+        if (kind) HIPCHK(h, hipMemcpy(kind, h->pkind, h->n, hipMemcpyDeviceToHost));
+        return SDS_SUCCESS;
+    }
     // Cholesky: no pivoting, every pivot a 1x1 in place
     for (int64_t k = 0; k < h->n; ++k) {
         if (piv) piv[k] = (int32_t)(k + 1);
@@ -1003,8 +1214,13 @@ int sds_get_pivots(sds_handle_t h, int32_t* piv, int8_t* kind) {
 
 int sds_get_diag(sds_handle_t h, void* d) {
     if (!h || h->dtype < 0 || !d) return SDS_INVALID_VALUE;
-    // Cholesky: the diagonal of L, read from the panels
     size_t es = h->dtype == SDS_R64 ? 8 : 4;
+    if (h->method != M_CHOLESKY) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipMemcpy(d, h->dvec, es * 2 * h->n, hipMemcpyDeviceToHost));
+        return SDS_SUCCESS;
+    }
+    // Cholesky: the diagonal of L, read from the panels
     std::memset(d, 0, es * 2 * h->n);
     for (int64_t s = 0; s < h->ns; ++s) {
         int64_t f = h->fnrows[s];
@@ -1018,10 +1234,10 @@ int sds_get_diag(sds_handle_t h, void* d) {
 
 int sds_get_factor(sds_handle_t h, int32_t which, void* out) {
     if (!h || h->dtype < 0 || !out) return SDS_INVALID_VALUE;
-    if (which != 0) return fail(h, SDS_NOT_SUPPORTED, "no U^T factor for this structure");
+    if (which != 0 && (which != 1 || !h->ufactor)) return fail(h, SDS_NOT_SUPPORTED, "no U^T factor for this structure");
     size_t es = h->dtype == SDS_R64 ? 8 : 4;
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    HIPCHK(h, hipMemcpy(out, h->factor, es * h->factor_len, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(out, which ? h->ufactor : h->factor, es * h->factor_len, hipMemcpyDeviceToHost));
     return SDS_SUCCESS;
 }
 

@@ -24,6 +24,43 @@ void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info);
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 
+// LDL^T (k_ldlt.hip)
+template <typename T>
+void launch_front_small_ldlt(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                             const double* nz, T* factor, T* cbpool, T* dvec, int32_t* piv, int8_t* kind,
+                             int64_t* stats, int pivot, double u, double eps, int scaled, const double* aux,
+                             const int8_t* psign);
+template <typename T>
+void launch_big_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* dvec,
+                           int32_t* piv, int8_t* kind, int64_t* stats, int pivot, double u, double eps, int scaled,
+                           const double* aux, const int8_t* psign);
+template <typename T>
+void launch_ldlt_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* cbpool,
+                    const T* dvec, const int8_t* kind);
+void launch_absmax(hipStream_t st, int64_t nnz, const double* nz, double* aux);
+
+// LU (k_lu.hip)
+template <typename T>
+void launch_front_small_lu(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                           const double* nz, T* factor, T* ufactor, T* cbpool, T* dvec, int32_t* piv, int8_t* kind,
+                           int64_t* stats, int pivoting, double u, double eps, int scaled, const double* aux);
+template <typename T>
+void launch_big_lu_zero(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor, T* cbpool);
+template <typename T>
+void launch_big_lu_scatter(hipStream_t st, const DevSym& S, const Task* tasks, int n, const double* nz, T* factor,
+                           T* ufactor);
+template <typename T>
+void launch_big_lu_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor,
+                              T* cbpool);
+template <typename T>
+void launch_big_lu_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor, T* dvec,
+                         int32_t* piv, int8_t* kind, int64_t* stats, int pivoting, double u, double eps, int scaled,
+                         const double* aux);
+template <typename T>
+void launch_lu_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* ufactor,
+                  T* cbpool, const T* dvec);
+void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stats, int64_t* info);
+
 template <typename T>
 void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y);
 template <typename T>

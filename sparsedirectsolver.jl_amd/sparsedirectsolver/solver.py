@@ -144,7 +144,7 @@ class DirectSolver:
     # -- phases ---------------------------------------------------------------------------------------
     def _reorder(self):
         o = self.options
-        if self.structure not in ("SPD", "HPD"):
+        if self.structure not in ("SPD", "HPD", "S", "H", "G"):
             raise NotSupportedError(f'structure "{self.structure}" is not available in this build')
         if o["schur_mode"] != 0:
             raise NotSupportedError("schur_mode = 1 is not available in this build")
@@ -201,6 +201,12 @@ class DirectSolver:
         p.pivot_epsilon = -1.0 if o["pivot_epsilon"] is None else float(o["pivot_epsilon"])
         flag = o["user_host_interrupt"]
         p.interrupt = ctypes.addressof(flag) if flag is not None else None
+        sign = o["pivot_sign"]
+        if sign is not None:
+            sign = np.ascontiguousarray(sign, dtype=np.int8)
+            if sign.shape != (self.n,) or np.any(np.abs(sign) > 1):
+                raise InvalidValueError("pivot_sign must hold n entries in {-1, 0, 1}")
+            p.pivot_sign = sign.ctypes.data
         info = ctypes.c_int64(0)
         self.info = 0
         self.stage = STAGE_ANALYZED
@@ -346,6 +352,21 @@ def numeric_factor(solver, which=0):
     a = np.zeros(S.factor_len, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
     _check(solver._lib, solver._h, solver._lib.sds_get_factor(solver._h, which, a.ctypes.data))
     return a
+
+
+def numeric_pivots(solver):
+    """(piv, kind): the local pivot order as 1-based factor columns and the pivot kinds."""
+    piv = np.zeros(solver.n, dtype=np.int32)
+    kind = np.zeros(solver.n, dtype=np.int8)
+    _check(solver._lib, solver._h, solver._lib.sds_get_pivots(solver._h, piv.ctypes.data, kind.ctypes.data))
+    return piv, kind
+
+
+def numeric_diag(solver):
+    """D as stored: 2n entries (the diagonal, then the 2x2 subdiagonals at the first position)."""
+    d = np.zeros(2 * solver.n, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
+    _check(solver._lib, solver._h, solver._lib.sds_get_diag(solver._h, d.ctypes.data))
+    return d
 
 
 def numeric_stats(solver):

@@ -35,6 +35,28 @@ def random_symindef(n, density, rng):
     return (S + sp.diags(signs * (r + 1))).tocsr()
 
 
+def random_symmetric(n, density, rng, diag=0.0):
+    """B + B^T with N(0, 1) entries and a small N(0, diag^2) diagonal (stored, possibly zero):
+    strongly indefinite, so Bunch-Kaufman takes many 2x2 pivots."""
+    B = sprand(rng, n, n, density)
+    B.data = rng.standard_normal(B.data.size)
+    S = (B + B.T).tocsr()
+    S = S + sp.diags(diag * rng.standard_normal(n))
+    S = (S + sp.csr_matrix((np.zeros(n), (np.arange(n), np.arange(n))), shape=(n, n))).tocsr()
+    S.sort_indices()
+    return S
+
+
+def random_unsym(n, density, rng, diag=1.0):
+    """Unsymmetric sparse B with N(0, 1) entries plus an N(0, diag^2) diagonal (stored)."""
+    B = sprand(rng, n, n, density)
+    B.data = rng.standard_normal(B.data.size)
+    A = B + sp.diags(diag * rng.standard_normal(n))
+    A = (A + sp.csr_matrix((np.zeros(n), (np.arange(n), np.arange(n))), shape=(n, n))).tocsr()
+    A.sort_indices()
+    return A
+
+
 def kkt_matrix(nh, nj, delta, rng, hessian="spd", hessian_scale=1.0):
     """[H J^T; J -delta I] with J of full row rank (test/matrices.jl:143-158)."""
     H = random_spd(nh, min(1.0, 5 / nh), rng) if hessian == "spd" else random_symindef(nh, min(1.0, 5 / nh), rng)
