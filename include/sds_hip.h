@@ -148,6 +148,20 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * Adjoint of X = A^-1 B with respect to the stored values of A (no counterpart in
+ * the reference, whose Project.toml has no ChainRules; the differentiation rule is
+ * this library's addition): grad[p] = -sum_r lambda[i, r] X[j, r] for the stored
+ * entry p at (i, j) of the user's matrix, where lambda = A^-T Xbar comes from
+ * sds_solve with solve_mode = 1 on the same factor. A mirrored entry of a symmetric
+ * structure ("S", "H", "SPD", "HPD") stands for (i, j) and (j, i) and receives both
+ * terms; an entry the solver ignores (the other triangle of a full view) receives 0.
+ * lambda, X (n x nrhs, column-major) and grad (nnz) are device Float64 buffers;
+ * asynchronous on the handle's stream. Needs the full-pattern map at upload.
+ */
+int sds_adjoint_scatter(sds_handle_t handle, const double* lambda, int64_t ldl, const double* X, int64_t ldx,
+                        int32_t nrhs, double* grad);
+
+/*
  * Data parameters after a factorization (src/solver.jl getparam, the
  * `Numeric` fields of src/numeric/storage.jl:161-179):
  *  stats: 6 * ns Int64 per-front (npos, nneg, nzero, nperturbed, n2x2, info);

@@ -438,6 +438,27 @@ __global__ void k_residual(int n, int nrhs, const int64_t* __restrict__ rowptr, 
     }
 }
 
+// Adjoint of x = A^-1 b with respect to the stored values: grad[p] = -sum_r lambda_i x_j over
+// the right-hand sides r, for the entry p at (i, j); a mirrored entry of a symmetric structure
+// stands for (i, j) and (j, i) and gets both terms; an entry the solver ignores gets 0.
+// One thread per stored entry, no atomics.
+__global__ void k_adjoint_scatter(int64_t nnz, int nrhs, const int32_t* __restrict__ ai, const int32_t* __restrict__ aj,
+                                  const int8_t* __restrict__ aflag, const double* __restrict__ lam, int64_t ldl,
+                                  const double* __restrict__ X, int64_t ldx, double* __restrict__ grad) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const int8_t fl = aflag[p];
+        double g = 0.0;
+        if (fl) {
+            const int i = ai[p], j = aj[p];
+            for (int r = 0; r < nrhs; ++r) {
+                g += lam[i + r * ldl] * X[j + r * ldx];
+                if (fl == 2) g += lam[j + r * ldl] * X[i + r * ldx];
+            }
+        }
+        grad[p] = -g;
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 static int grid_for(int64_t total) { return (int)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 4096); }
@@ -499,6 +520,13 @@ void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 template <typename T>
 void launch_diag(hipStream_t st, int n, int nrhs, const T* d, const int8_t* kind, T* Y) {
     hipLaunchKernelGGL(k_diag<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, d, kind, Y);
+}
+
+void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
+                            const int8_t* aflag, const double* lam, int64_t ldl, const double* X, int64_t ldx,
+                            double* grad) {
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(grid_for(nnz)), dim3(256), 0, st, nnz, nrhs, ai, aj, aflag, lam, ldl, X,
+                       ldx, grad);
 }
 
 void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,

@@ -123,6 +123,11 @@ struct sds_handle {
     int64_t* full_rowptr = nullptr;
     int32_t* full_colval = nullptr;
     int64_t* full_src = nullptr;
+    // adjoint map: stored entry p at (row, col) of the permuted-back matrix, 0 unused /
+    // 1 one position / 2 a mirrored symmetric entry (both positions)
+    int32_t* adj_i = nullptr;
+    int32_t* adj_j = nullptr;
+    int8_t* adj_flag = nullptr;
     // schedule + numeric
     int dtype = -1;
     std::vector<Level> levels;
@@ -1043,6 +1048,8 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     h->full_colval = nullptr;
     h->full_src = nullptr;
     h->full_nrows = 0;
+    h->adj_i = h->adj_j = nullptr;
+    h->adj_flag = nullptr;
     if (d->full_rowptr && d->full_colval && d->full_src) {
         IdxReader frp{d->full_rowptr, b}, fcv{d->full_colval, b}, fsrc{d->full_src, b};
         int64_t nr = d->full_nrows, fnnz = d->full_nnz;
@@ -1059,6 +1066,24 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
         if ((rc = upload(h, h->symbufs, src, &h->full_src))) return rc;
         h->full_nrows = nr;
         h->nnz_full = fnnz;
+        // the adjoint map from the rows of M (the first n rows of the map)
+        std::vector<int32_t> ai(nnz, 0), aj(nnz, 0);
+        std::vector<int8_t> af(nnz, 0);
+        for (int64_t r = 0; r < std::min<int64_t>(nr, n); ++r)
+            for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+                const int64_t sv = src[e], p = sv >= 0 ? sv : -(sv + 1);
+                if (p < 0 || p >= nnz) return fail(h, SDS_INVALID_VALUE, "full_src out of range");
+                if (af[p] == 0) {
+                    ai[p] = (int32_t)r;
+                    aj[p] = cv[e];
+                    af[p] = 1;
+                } else if (ai[p] == cv[e] && aj[p] == (int32_t)r && ai[p] != aj[p]) {
+                    af[p] = 2;
+                }
+            }
+        if ((rc = upload(h, h->symbufs, ai, &h->adj_i))) return rc;
+        if ((rc = upload(h, h->symbufs, aj, &h->adj_j))) return rc;
+        if ((rc = upload(h, h->symbufs, af, &h->adj_flag))) return rc;
     }
     h->have_sym = true;
     return SDS_SUCCESS;
@@ -1175,6 +1200,17 @@ int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B
     return h->dtype == SDS_R64
                ? solve_t<double>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
                : solve_t<float>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+}
+
+int sds_adjoint_scatter(sds_handle_t h, const double* lambda, int64_t ldl, const double* X, int64_t ldx, int32_t nrhs,
+                        double* grad) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->adj_flag) return fail(h, SDS_INVALID_VALUE, "the adjoint needs the full-pattern map at upload");
+    if (!lambda || !X || !grad || nrhs < 1) return fail(h, SDS_INVALID_VALUE, "null buffer or nrhs < 1");
+    if (ldl < h->n || ldx < h->n) return fail(h, SDS_INVALID_VALUE, "leading dimension smaller than n");
+    launch_adjoint_scatter(h->stream, h->nnz, nrhs, h->adj_i, h->adj_j, h->adj_flag, lambda, ldl, X, ldx, grad);
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
 }
 
 int sds_get_stats(sds_handle_t h, int64_t* stats, int64_t* totals) {

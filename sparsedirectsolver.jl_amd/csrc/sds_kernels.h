@@ -1,4 +1,4 @@
-// Launchers of the device kernels (defined in k_factor.hip, k_solve.hip).
+// Launchers of the device kernels (defined in k_factor.hip, k_ldlt.hip, k_lu.hip, k_solve.hip).
 #pragma once
 #include "sds_common.h"
 
@@ -88,6 +88,9 @@ void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, c
 template <typename T>
 void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
                      int unit);
+void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
+                            const int8_t* aflag, const double* lam, int64_t ldl, const double* X, int64_t ldx,
+                            double* grad);
 void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,
                      const double* nz, const double* X, int64_t ldx, const double* B, int64_t ldb, double* R);
 

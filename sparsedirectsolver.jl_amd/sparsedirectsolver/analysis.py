@@ -106,3 +106,70 @@ def symbolic_analysis(rowptr, colval, n, structure, view="F", index="O", perm=No
         lib.sdsa_free(h)
     return Symbolic(n=int(sc[0]), nnz=int(sc[1]), ns=int(sc[2]), structure=structure, view=view,
                     nnz_stored=int(sc[3]), nnz_L=int(sc[4]), flops=sc[5], flops_true=sc[6], maps=maps)
+
+
+def merge_symbolic(parts):
+    """The analysis of a block-diagonal matrix from its diagonal blocks' analyses: the members'
+    arrays concatenated with their column, supernode, factor, row-list and value offsets, so
+    that each member keeps exactly its own supernodes, fronts and maps (the elimination forest
+    is the union of the members' trees). The members must share structure and view."""
+    if not parts:
+        raise ValueError("merge_symbolic needs at least one analysis")
+    st, vw = parts[0].structure, parts[0].view
+    if any(p.structure != st or p.view != vw for p in parts):
+        raise ValueError("the members of a batch must share structure and view")
+    gen = st == "G"
+    cat = {k: [] for k in _FIELDS}
+    N = NS = NZ = FL = RV = CL = RI = AS = 0
+    full_m, full_t = [], []  # (rowptr deltas, colval, src) of M rows and M^T rows
+    for S in parts:
+        m = S.maps
+        cat["perm"].append(m["perm"] + N)
+        cat["iperm"].append(m["iperm"] + N)
+        cat["parent"].append(np.where(m["parent"] > 0, m["parent"] + N, 0))
+        cat["counts"].append(m["counts"])
+        cat["super_ptr"].append(m["super_ptr"][:-1] + N)
+        cat["col2sn"].append(m["col2sn"] + NS)
+        cat["snparent"].append(np.where(m["snparent"] > 0, m["snparent"] + NS, 0))
+        cat["snpost"].append(m["snpost"] + NS)
+        cat["rowptr"].append(m["rowptr"][:-1] + RV)
+        cat["rowval"].append(m["rowval"] + N)
+        cat["front_ptr"].append(m["front_ptr"][:-1] + FL)
+        cat["front_nrows"].append(m["front_nrows"])
+        cat["front_ncols"].append(m["front_ncols"])
+        cat["child_ptr"].append(m["child_ptr"][:-1] + CL)
+        cat["child_list"].append(m["child_list"] + NS)
+        cat["relind_ptr"].append(m["relind_ptr"][:-1] + RI)
+        cat["relind"].append(m["relind"])
+        a = m["amap"]
+        cat["amap"].append(np.where(a != 0, np.sign(a) * (np.abs(a) + FL), 0))
+        cat["amap_ptr"].append(m["amap_ptr"][:-1] + AS)
+        cat["amap_src"].append(m["amap_src"] + NZ)
+        cat["etree_perm"].append(m["etree_perm"] + N)
+        rp, cv, src = m["full_rowptr"], m["full_colval"], m["full_src"]
+        srcs = np.sign(src) * (np.abs(src) + NZ)
+        rows_m = S.n
+        full_m.append((np.diff(rp[:rows_m + 1]), cv[:rp[rows_m] - 1] + N, srcs[:rp[rows_m] - 1]))
+        if gen:
+            full_t.append((np.diff(rp[rows_m:]), cv[rp[rows_m] - 1:] + N, srcs[rp[rows_m] - 1:]))
+        N += S.n
+        NS += S.ns
+        NZ += S.nnz
+        FL += S.factor_len
+        RV += m["rowval"].shape[0]
+        CL += m["child_list"].shape[0]
+        RI += m["relind"].shape[0]
+        AS += m["amap_src"].shape[0]
+    maps = {k: np.concatenate(v) if v else np.zeros(0, dtype=np.int64) for k, v in cat.items()
+            if k not in ("full_rowptr", "full_colval", "full_src")}
+    for k, last in (("super_ptr", N + 1), ("rowptr", RV + 1), ("front_ptr", FL + 1), ("child_ptr", CL + 1),
+                    ("relind_ptr", RI + 1), ("amap_ptr", AS + 1)):
+        maps[k] = np.concatenate([maps[k], [last]]).astype(np.int64)
+    blocks = full_m + full_t
+    counts = np.concatenate([b[0] for b in blocks])
+    maps["full_rowptr"] = np.concatenate([[1], 1 + np.cumsum(counts)]).astype(np.int64)
+    maps["full_colval"] = np.concatenate([b[1] for b in blocks]).astype(np.int64)
+    maps["full_src"] = np.concatenate([b[2] for b in blocks]).astype(np.int64)
+    return Symbolic(n=N, nnz=NZ, ns=NS, structure=st, view=vw, nnz_stored=sum(p.nnz_stored for p in parts),
+                    nnz_L=sum(p.nnz_L for p in parts), flops=float(sum(p.flops for p in parts)),
+                    flops_true=float(sum(p.flops_true for p in parts)), maps=maps)

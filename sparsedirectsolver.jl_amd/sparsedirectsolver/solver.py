@@ -115,6 +115,7 @@ class DirectSolver:
         self._h = None
         self._lib = None
         self._keep = []
+        self._analysis = None
         self._set_values(nzval)
 
     def _set_values(self, nzval):
@@ -150,6 +151,11 @@ class DirectSolver:
             raise NotSupportedError("schur_mode = 1 is not available in this build")
         if o["matching_alg"] not in ("default", "none"):
             raise NotSupportedError("matching is not available in this build")
+        if self._analysis is not None:
+            # an analysis assembled elsewhere (the block-diagonal forest of an uneven batch)
+            self.symbolic = self._analysis(o)
+            self.stage = STAGE_REORDERED
+            return
         ordering = {"default": "default", "algo3": "algo3", "algo5": "algo5"}.get(o["reordering_alg"])
         if ordering is None and o["user_perm"] is None:
             raise NotSupportedError(f"reordering_alg = {o['reordering_alg']!r} is not available in this build")
