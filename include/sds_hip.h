@@ -203,6 +203,7 @@ enum {
     SDS_K_DIAG,            /* D^-1 */
     SDS_K_RESIDUAL,        /* refinement residual (FP64 SpMV) */
     SDS_K_PIVOT_PANEL,     /* large fronts: pivoted panel factorization (LDL^T, LU) */
+    SDS_K_INVERSE,         /* Cholesky: inverses of the diagonal blocks (for the sweeps) */
     SDS_K_COUNT
 };
 int sds_profile(sds_handle_t handle, int32_t enable);

@@ -155,175 +155,91 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     }
 }
 
-// Block step of a large front, part 1: Cholesky of the (updated) diagonal block [k, k + nb)
-// with one wave: lane i holds row i in registers and the pivot and column j move between
-// lanes with v_readlane (register broadcasts). The factored block goes to DB (NB entries per
-// factor column); a failure sets the front's status. task: (s, k)
-template <typename T>
-__global__ void __launch_bounds__(64) k_big_potrf(DevSym S, const Task* __restrict__ tasks,
-                                                 const T* __restrict__ factor, T* __restrict__ DB,
-                                                 int32_t* __restrict__ status, double eps) {
-    __shared__ T D[NB][NB + 1];  // D[j][i] = L(k + i, k + j)
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a;
-    if (status[s] != 0) return;
-    const int nb = min(NB, w - k);
-    const T* P = factor + S.panel_off[s];
-    const int i = threadIdx.x;
-#pragma unroll
-    for (int q = i; q < NB * NB; q += 64) {
-        const int j = q / NB, r = q - j * NB;
-        D[j][r] = (r < nb && j < nb && r >= j) ? P[(int64_t)(k + j) * f + k + r] : (r == j ? T(1) : T(0));
-    }
-    __syncthreads();
-    T a[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) a[c] = D[c][i];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const double ajj = (double)lane_bcast(a[j], j);
-        if (j < nb && !chol_accept(ajj, eps)) {
-            if (i == 0) status[s] = k + j + 1;
-            return;
-        }
-        const T d = j < nb ? T(sqrt(fmax(ajj, eps))) : T(1);
-        const T l = i > j ? a[j] / d : (i == j ? d : a[j]);
-        a[j] = l;
-        const T li = i > j ? l : T(0);
-#pragma unroll
-        for (int c = j + 1; c < NB; ++c) a[c] -= li * lane_bcast(l, c);
-        __builtin_amdgcn_sched_barrier(0);  // one column at a time: bounds register pressure
-    }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) D[c][i] = c <= i ? a[c] : T(0);
-    __syncthreads();
-    T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
-#pragma unroll
-    for (int q = i; q < NB * NB; q += 64)
-        if (q < nb * NB) db[q] = D[q / NB][q % NB];
-}
-
-// Block step of a large front, part 2: a tile of PANEL_ROWS panel rows below the diagonal
-// block, X <- X L_bb^-T with L_bb from DB; the tile is staged through LDS with coalesced
-// loads and stores and each thread solves its row in registers. The task with c = 1 also
-// copies the factored block into the panel (nobody in this launch reads it there).
+// Block step of a large front: Cholesky of the (updated) diagonal block [k, k + nb) and the
+// triangular solve of a tile of PANEL_ROWS panel rows below it, X <- X L_bb^-T. Every
+// workgroup of the step factors the 64 x 64 block in LDS itself (two barriers per column:
+// each lane recomputes the pivot's square root, no broadcast needed), so the step is one
+// launch; the task with c = 1 keeps the factored block in DB (NB entries per factor column)
+// and records a failed pivot. The panel's own diagonal block keeps its unfactored values
+// until the factorization's last launch copies DB into it: the blocks are read by nobody in
+// between, and no workgroup of the step writes what another one reads.
+// Each thread then solves its row in registers, right-looking (column jj scaled by the
+// pivot's reciprocal, then subtracted from the later columns: 63 independent updates); the
+// row is loaded and stored straight from the panel (consecutive threads, consecutive rows).
 // task: (s, k, row0, c)
 template <typename T>
-__global__ void __launch_bounds__(PANEL_ROWS) k_big_trsm(DevSym S, const Task* __restrict__ tasks,
-                                                        T* __restrict__ factor, const T* __restrict__ DB,
-                                                        const int32_t* __restrict__ status) {
-    __shared__ T D[NB][NB + 1];
-    __shared__ T Xs[NB][PANEL_ROWS + 1];  // Xs[c][rr] = panel row row0 + rr, column k + c
+__global__ void __launch_bounds__(PANEL_ROWS) k_big_panel(DevSym S, const Task* __restrict__ tasks,
+                                                         T* __restrict__ factor, T* __restrict__ DB,
+                                                         int32_t* __restrict__ status, double eps) {
+    __shared__ T D[NB][NB + 1];  // D[j][i] = L(k + i, k + j)
+    __shared__ T rd[NB];         // reciprocals of the pivots
+    __shared__ int fail;
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
     if (status[s] != 0) return;
     const int nb = min(NB, w - k);
     T* P = factor + S.panel_off[s];
-    const T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
     const int tid = threadIdx.x;
+    const int i = row0 + tid;
+    // the diagonal block (lower part, identity past nb)
 #pragma unroll
     for (int q = tid; q < NB * NB; q += PANEL_ROWS) {
-        const int j = q / NB, i = q - j * NB;
-        D[j][i] = (i < nb && j < nb) ? db[q] : (i == j ? T(1) : T(0));
+        const int j = q / NB, ii = q - j * NB;
+        D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? P[(int64_t)(k + j) * f + k + ii] : T(0)) : (ii == j ? T(1) : T(0));
     }
-    if (t.c)
-        for (int q = tid; q < nb * nb; q += PANEL_ROWS) {
-            const int j = q / nb, i = q - j * nb;
-            if (i >= j) P[(int64_t)(k + j) * f + k + i] = db[j * NB + i];
-        }
-    // columns past nb are zero (and L_bb is padded with the identity), so the register row
-    // needs no guards (runtime guards on it make the compiler spill)
-    const int nr = min(PANEL_ROWS, f - row0);
-#pragma unroll
-    for (int q = tid; q < NB * PANEL_ROWS; q += PANEL_ROWS) {
-        const int c = q / PANEL_ROWS, rr = q - c * PANEL_ROWS;
-        Xs[c][rr] = (rr < nr && c < nb) ? P[(int64_t)(k + c) * f + row0 + rr] : T(0);
-    }
+    if (tid == 0) fail = 0;
     __syncthreads();
-    if (nr > 0) {
-        T x[NB];
-#pragma unroll
-        for (int kk = 0; kk < NB; ++kk) x[kk] = Xs[kk][tid];
-#pragma unroll
-        for (int kk = 0; kk < NB; ++kk) {
-            T v = x[kk];
-#pragma unroll
-            for (int jj = 0; jj < kk; ++jj) v -= x[jj] * D[jj][kk];
-            x[kk] = v / D[kk][kk];
-            __builtin_amdgcn_sched_barrier(0);
+    // right-looking Cholesky of D
+    for (int j = 0; j < nb; ++j) {
+        const double a = (double)D[j][j];
+        if (!chol_accept(a, eps)) {
+            if (tid == 0) fail = j + 1;
+            break;  // uniform: every thread sees the same pivot
         }
-#pragma unroll
-        for (int kk = 0; kk < NB; ++kk) Xs[kk][tid] = x[kk];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = tid; q < NB * PANEL_ROWS; q += PANEL_ROWS) {
-        const int c = q / PANEL_ROWS, rr = q - c * PANEL_ROWS;
-        if (rr < nr && c < nb) P[(int64_t)(k + c) * f + row0 + rr] = Xs[c][rr];
-    }
-}
-
-// Schur update of one 64x64 tile: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T on the lower
-// triangle of the front, A the panel columns k0..k1-1. mode 0: step update of the
-// panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
-// task: (s, I0, J0, k0)
-template <typename T>
-__global__ void __launch_bounds__(256) k_big_update(DevSym S, const Task* __restrict__ tasks,
-                                                   T* __restrict__ factor, T* __restrict__ cbpool,
-                                                   const int32_t* __restrict__ status, int mode) {
-    constexpr int KC = 16;
-    __shared__ T As[KC][TILE + 4];
-    __shared__ T Bs[KC][TILE + 4];
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
-    if (status[s] != 0) return;
-    const int I0 = t.a, J0 = t.b;
-    const int k0 = mode == 0 ? t.c : 0;
-    const int k1 = mode == 0 ? min(t.c + NB, w) : w;
-    const int jlim = mode == 0 ? w : f;
-    T* P = factor + S.panel_off[s];
-    T* C = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    T acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-    for (int kc = k0; kc < k1; kc += KC) {
-        for (int q = tid; q < KC * TILE; q += 256) {
-            const int kk = q / TILE, r = q - kk * TILE;
-            const int kcol = kc + kk;
-            const bool kin = kcol < k1;
-            As[kk][r] = (kin && I0 + r < f) ? P[(int64_t)kcol * f + I0 + r] : T(0);
-            Bs[kk][r] = (kin && J0 + r < f) ? P[(int64_t)kcol * f + J0 + r] : T(0);
+        const T d = T(sqrt(fmax(a, eps)));
+        const T r = T(1) / d;
+        __syncthreads();  // every thread has read D[j][j]
+        if (tid < NB) {
+            if (tid == j) D[j][j] = d;
+            if (tid > j) D[j][tid] *= r;
+            if (tid == 0) rd[j] = r;
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KC; ++kk) {
-            T av[4], bv[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        const int R = NB - 1 - j;
+        for (int q = tid; q < R * R; q += PANEL_ROWS) {
+            const int jj = j + 1 + q / R, ii = j + 1 + q % R;
+            if (ii >= jj) D[jj][ii] = fma(-D[j][ii], D[j][jj], D[jj][ii]);
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int i = I0 + ty + 16 * a;
-        if (i >= f) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = J0 + tx + 16 * b;
-            if (j >= jlim || j > i || j < (mode == 0 ? k1 : w)) continue;
-            T* dst = front_at(P, C, f, w, i, j);
-            *dst -= acc[a][b];
-        }
+    __syncthreads();
+    if (fail) {
+        if (t.c && tid == 0) status[s] = k + fail;
+        return;
     }
+    for (int j = nb + tid; j < NB; j += PANEL_ROWS) rd[j] = T(1);
+    if (t.c) {
+        T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
+        for (int q = tid; q < nb * NB; q += PANEL_ROWS) db[q] = D[q / NB][q % NB];
+    }
+    __syncthreads();
+    if (i >= f) return;
+    // this thread's row. Per-column guards on the register row make the compiler spill it,
+    // so columns past nb load column nb - 1 (they solve to garbage that nothing reads: the
+    // padding of D keeps them out of the real columns) and the stores run from the last
+    // column down, so that column nb - 1 is written last with its real value.
+    T x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = P[(int64_t)(k + min(c, nb - 1)) * f + i];
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        x[jj] = x[jj] * rd[jj];
+        const T xj = x[jj];
+#pragma unroll
+        for (int kk = jj + 1; kk < NB; ++kk) x[kk] = fma(-xj, D[jj][kk], x[kk]);
+    }
+#pragma unroll
+    for (int c = NB - 1; c >= 0; --c) P[(int64_t)(k + min(c, nb - 1)) * f + i] = x[c];
 }
 
 // The same tile update on the matrix cores. 256 threads = 4 waves, each wave a 32x32
@@ -480,15 +396,9 @@ void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, i
 }
 
 template <typename T>
-void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
+void launch_big_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                       double eps) {
-    hipLaunchKernelGGL(k_big_potrf<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, DB, status, eps);
-}
-
-template <typename T>
-void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const T* DB,
-                     const int32_t* status) {
-    hipLaunchKernelGGL(k_big_trsm<T>, dim3(n), dim3(PANEL_ROWS), 0, st, S, tasks, factor, DB, status);
+    hipLaunchKernelGGL(k_big_panel<T>, dim3(n), dim3(PANEL_ROWS), 0, st, S, tasks, factor, DB, status, eps);
 }
 
 template <typename T>
@@ -512,8 +422,7 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
                                              T*, T*, int32_t*, double);                                           \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
-    template void launch_big_potrf<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);    \
-    template void launch_big_trsm<T>(hipStream_t, const DevSym&, const Task*, int, T*, const T*, const int32_t*); \
+    template void launch_big_panel<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);    \
     template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int);
 INST(double)
 INST(float)

@@ -41,9 +41,13 @@ enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4
 
 constexpr int NB = 64;     // pivot block of the blocked front factorization
 constexpr int TILE = 64;   // Schur-update tile
-constexpr int PANEL_ROWS = 128;  // panel rows per workgroup of the fused block step
+constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
+// Cholesky sweeps with inverted diagonal blocks: fronts with more than FUSE_ROWS rows below
+// their fully summed block get separate L21 launches, with L21^T partial sums per L21T_CHUNK rows
+constexpr int FUSE_ROWS = 256;
+constexpr int L21T_CHUNK = 1024;
 
 // value of `v` in lane `lane` (wave-uniform) of the calling wave, as a register broadcast
 __device__ inline double lane_bcast(double v, int lane) {

@@ -74,6 +74,10 @@ struct Level {
     double small_flops = 0, small_bytes = 0;
     double fwd_bytes = 0, fwd_flops = 0;  // factor bytes read by a sweep, flops per right-hand side
     double rhs_rows = 0;                  // rows of the level's fronts (right-hand-side entries touched)
+    // Cholesky sweeps with inverted diagonal blocks: L21 launches of the tall fronts
+    Launch inv_fwd{S_FWD_STEP, 0, 0, 0, 0}, inv_bwd{S_BWD_L21, 0, 0, 0, 0};
+    double sweep_bytes = 0, sweep_flops = 0;  // all fronts of the level: factor bytes, flops per rhs
+    double all_rows = 0;                      // rows of all the level's fronts
 };
 
 // plan launch kind -> profiling kind of include/sds_hip.h
@@ -117,6 +121,10 @@ struct sds_handle {
     int method = M_CHOLESKY;
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
+    Launch inv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the diagonal blocks (Cholesky)
+    int64_t* d_poff = nullptr;               // per front: partial-sum offset of its L21^T (units of nrhs)
+    int64_t part_units = 0;                  // partial-sum buffer, units of nrhs
+    DevBuf part;
     std::vector<DevBuf> symbufs;  // device maps
     DevSym dsym{};
     int64_t full_nrows = 0;
@@ -325,7 +333,7 @@ void free_numeric(sds_handle* h) {
     if (h->h_norms) (void)hipHostFree(h->h_norms);
     h->h_info = nullptr;
     h->h_norms = nullptr;
-    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms}) {
+    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms, &h->part}) {
         if (b->p) (void)hipFree(b->p);
         *b = DevBuf{};
     }
@@ -491,13 +499,12 @@ int build_schedule(sds_handle* h, int dtype) {
             wmax = 0;
         }
         for (int k = 0; k < wmax; k += NB) {
-            std::vector<Task> tp, tt, tu;
+            std::vector<Task> tt, tu;
             double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0;
             for (int32_t s : L.big) {
                 int f = h->fnrows[s], w = h->fncols[s];
                 if (k >= w) continue;
                 int nb = std::min(NB, w - k);
-                tp.push_back({s, k, 0, 0});
                 fp += (double)nb * nb * nb / 3.0;
                 bp += dbl_es * (double)nb * (nb + 1);
                 int first = 1;
@@ -518,8 +525,7 @@ int build_schedule(sds_handle* h, int dtype) {
                             bu += 2.0 * dbl_es * e;
                         }
             }
-            push_launch(L, L_POTRF, tp, fp, bp);
-            push_launch(L, L_TRSM, tt, ft, bt);
+            push_launch(L, L_TRSM, tt, ft + fp, bt + bp);
             push_launch(L, L_UPDATE_PANEL, tu, fu, bu);
         }
         t.clear();
@@ -619,6 +625,51 @@ int build_schedule(sds_handle* h, int dtype) {
             for (int c = 0; c < h->fncols[s]; c += 256) t.push_back({s, c, 0, 0});
         push_solve(L.bwd_plan, S_BWD_FINISH, t, 0, 0);
     }
+    // Cholesky: the diagonal-block inverses and the inverse-block sweep plans
+    std::vector<int64_t> poff(ns, -1);
+    h->part_units = 0;
+    if (h->method == M_CHOLESKY) {
+        std::vector<Task> t;
+        double fl = 0, by = 0;
+        for (int64_t s = 0; s < ns; ++s)
+            for (int k = 0; k < h->fncols[s]; k += NB) {
+                t.push_back({(int32_t)s, k, 0, big[s] ? 1 : 0});
+                const double nb = std::min(NB, h->fncols[s] - k);
+                fl += nb * nb * nb / 3.0;
+                by += dbl_es * (nb * (nb + 1) / 2 + (double)NB * nb);
+            }
+        h->inv_launch = {L_POTRF, (int64_t)tasks.size(), (int)t.size(), fl, by};
+        tasks.insert(tasks.end(), t.begin(), t.end());
+        for (auto& L : h->levels) {
+            for (int32_t s : L.all) {
+                const double f = h->fnrows[s], w = h->fncols[s];
+                const double e = f * w - w * (w - 1) / 2 + w * NB;  // panel + inverse blocks
+                L.sweep_bytes += dbl_es * e;
+                L.sweep_flops += 2.0 * e;
+                L.all_rows += f;
+            }
+            std::vector<Task> tf, tb;
+            int64_t units = 0;
+            for (int32_t s : L.all) {
+                const int f = h->fnrows[s], w = h->fncols[s], m = f - w;
+                if (m <= FUSE_ROWS || uv_off[s] < 0) continue;
+                for (int r0 = w; r0 < f; r0 += 256) tf.push_back({s, r0, 0, 0});
+                poff[s] = units;
+                const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
+                for (int c = 0; c < w; c += 64)
+                    for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, (int32_t)units});
+                units += (int64_t)nch * w;
+            }
+            h->part_units = std::max(h->part_units, units);
+            if (!tf.empty()) {
+                L.inv_fwd = {S_FWD_STEP, (int64_t)tasks.size(), (int)tf.size(), 0, 0};
+                tasks.insert(tasks.end(), tf.begin(), tf.end());
+                L.inv_bwd = {S_BWD_L21, (int64_t)tasks.size(), (int)tb.size(), 0, 0};
+                tasks.insert(tasks.end(), tb.begin(), tb.end());
+            }
+        }
+    }
+    if ((rc = upload(h, h->numbufs, poff, &h->d_poff))) return rc;
     if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
     if ((rc = upload(h, h->numbufs, tasks, &h->d_tasks))) return rc;
     if ((rc = upload(h, h->numbufs, cb_off, &h->d_cb_off))) return rc;
@@ -648,11 +699,8 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
                     case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
-                    case L_POTRF:
-                        launch_big_potrf<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
-                        break;
                     case L_TRSM:
-                        launch_big_trsm<T>(st, h->dsym, t, ln.n, factor, static_cast<const T*>(h->dbuf), h->status);
+                        launch_big_panel<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
                         break;
                     case L_UPDATE_PANEL:
                         launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
@@ -663,6 +711,13 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
         }
     }
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
+    // the inverses of the diagonal blocks for the sweeps (a failed factorization gives
+    // inverses of whatever the panels hold; its solves are not meaningful anyway)
+    if (h->inv_launch.n)
+        timed(h, SDS_K_INVERSE, h->inv_launch.flops, h->inv_launch.bytes, [&] {
+            launch_block_inverse<T>(st, h->dsym, h->d_tasks + h->inv_launch.off, h->inv_launch.n, factor,
+                                    static_cast<T*>(h->dbuf));
+        });
     HIPCHK(h, hipGetLastError());
     return SDS_SUCCESS;
 }
@@ -753,7 +808,7 @@ int factorize_lu(sds_handle* h, const double* nz, int pivoting, double u, double
 int ensure_solve_workspace(sds_handle* h, int nrhs) {
     if (nrhs <= h->cap_nrhs) return SDS_SUCCESS;
     size_t es = h->dtype == SDS_R64 ? 8 : 4;
-    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms}) {
+    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms, &h->part}) {
         if (b->p) (void)hipFree(b->p);
         *b = DevBuf{};
     }
@@ -771,6 +826,7 @@ int ensure_solve_workspace(sds_handle* h, int nrhs) {
     if ((rc = get(h->R, 8 * h->n * nrhs))) return rc;
     if ((rc = get(h->Bc, 8 * h->n * nrhs))) return rc;
     if ((rc = get(h->norms, 16 * nrhs))) return rc;
+    if ((rc = get(h->part, es * std::max<int64_t>(h->part_units, 1) * nrhs))) return rc;
     h->cap_nrhs = nrhs;
     return SDS_SUCCESS;
 }
@@ -794,6 +850,33 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
     const int32_t* fpiv = lu && transposed ? nullptr : pv;
     const int32_t* bpiv = lu && !transposed ? nullptr : pv;
     const double es = sizeof(T);
+    if (h->method == M_CHOLESKY) {
+        // inverted diagonal blocks (DB) and level launches over all fronts
+        const T* Li = static_cast<const T*>(h->dbuf);
+        T* part = static_cast<T*>(h->part.p);
+        if (fwd)
+            for (auto& Lv : h->levels) {
+                timed(h, SDS_K_FWD, Lv.sweep_flops * nrhs, Lv.sweep_bytes + 3.0 * es * nrhs * Lv.all_rows, [&] {
+                    launch_fwd_front<T>(st, h->dsym, h->d_fronts + Lv.off_all, (int)Lv.all.size(), L, Li, Y, U, nrhs);
+                });
+                if (Lv.inv_fwd.n)
+                    timed(h, SDS_K_FWD, 0, 0, [&] {
+                        launch_fwd_l21<T>(st, h->dsym, h->d_tasks + Lv.inv_fwd.off, Lv.inv_fwd.n, L, Y, U, nrhs);
+                    });
+            }
+        if (bwd)
+            for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
+                if (it->inv_bwd.n)
+                    timed(h, SDS_K_BWD, 0, 0, [&] {
+                        launch_bwd_l21t<T>(st, h->dsym, h->d_tasks + it->inv_bwd.off, it->inv_bwd.n, L, Y, part, nrhs);
+                    });
+                timed(h, SDS_K_BWD, it->sweep_flops * nrhs, it->sweep_bytes + 3.0 * es * nrhs * it->all_rows, [&] {
+                    launch_bwd_front<T>(st, h->dsym, h->d_fronts + it->off_all, (int)it->all.size(), h->d_poff, L, Li,
+                                        Y, part, nrhs);
+                });
+            }
+        return;
+    }
     auto run = [&](const Launch& ln, int kind) {
         const Task* t = h->d_tasks + ln.off;
         timed(h, kind, ln.flops * nrhs, ln.bytes + (ln.flops > 0 ? es * nrhs * ln.flops / 16 : 0), [&] {

@@ -13,11 +13,8 @@ void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks,
 template <typename T>
 void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool);
 template <typename T>
-void launch_big_potrf(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
+void launch_big_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                       double eps);
-template <typename T>
-void launch_big_trsm(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const T* DB,
-                     const int32_t* status);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode);
@@ -88,6 +85,21 @@ void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, c
 template <typename T>
 void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
                      int unit);
+// Cholesky sweeps with inverted diagonal blocks (k_solve_inv.hip)
+template <typename T>
+void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB);
+template <typename T>
+void launch_fwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const T* factor, const T* DB, T* Y,
+                      T* U, int nrhs);
+template <typename T>
+void launch_fwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* U,
+                    int nrhs);
+template <typename T>
+void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* part,
+                     int nrhs);
+template <typename T>
+void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const int64_t* poff,
+                      const T* factor, const T* DB, T* Y, const T* part, int nrhs);
 void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
                             const int8_t* aflag, const double* lam, int64_t ldl, const double* X, int64_t ldx,
                             double* grad);
