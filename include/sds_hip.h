@@ -148,6 +148,27 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * HIP graph of one refactorization followed by one solve (no counterpart in the
+ * reference, which replays nothing; it is this library's form of the
+ * "refactorize + solve captured in a graph" of uneven batches and serving loops).
+ * sds_graph_capture records, on a private stream, exactly what sds_factorize and
+ * sds_solve would launch for these arguments (values nzval, params, phase,
+ * solve_mode, B, X, nrhs, ir_n_steps with ir_tol = 0: no host synchronization),
+ * without running it; the pointers are baked into the graph, so each replay reads
+ * the current contents of nzval and B and writes X. pivot_sign (if any) is copied
+ * once at capture. sds_graph_launch replays it on the handle's stream
+ * (asynchronous); sds_graph_info waits and returns the replay's info, as
+ * sds_factorize does. Capturing again replaces the graph; sds_alloc_numeric and
+ * sds_destroy free it.
+ */
+int sds_graph_capture(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t phase,
+                      int32_t solve_mode, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t nrhs,
+                      int32_t ir_n_steps);
+int sds_graph_launch(sds_handle_t handle);
+int sds_graph_info(sds_handle_t handle, int64_t* info);
+int sds_graph_destroy(sds_handle_t handle);
+
+/*
  * Adjoint of X = A^-1 B with respect to the stored values of A (no counterpart in
  * the reference, whose Project.toml has no ChainRules; the differentiation rule is
  * this library's addition): grad[p] = -sum_r lambda[i, r] X[j, r] for the stored

@@ -242,10 +242,14 @@ __global__ void __launch_bounds__(PANEL_ROWS) k_big_panel(DevSym S, const Task* 
     for (int c = NB - 1; c >= 0; --c) P[(int64_t)(k + min(c, nb - 1)) * f + i] = x[c];
 }
 
-// The same tile update on the matrix cores. 256 threads = 4 waves, each wave a 32x32
-// quadrant of the 64x64 tile as 2x2 blocks of 16x16 accumulators; K in chunks of 16
-// staged through LDS, the next chunk prefetched into registers while the current one is
-// multiplied. MFMA 16x16x4 operand layout (checked on gfx950): lane l gives A[l % 16][l / 16]
+// Schur update of one 64x64 tile on the matrix cores: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T
+// on the lower triangle of the front, A the panel columns k0..k1-1. mode 0: step update of
+// the panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
+// task: (s, I0, J0, k0). 256 threads = 4 waves, each wave a 32x32 quadrant of the tile as
+// 2x2 blocks of 16x16 accumulators; K in chunks of 64 staged through LDS (one chunk holds a
+// whole step), the next chunk prefetched into registers while the current one is multiplied,
+// and the tile's old values loaded at the start. MFMA 16x16x4 operand layout (checked on
+// gfx950): lane l gives A[l % 16][l / 16]
 // and B[l / 16][l % 16]; accumulator element q of lane l is D[row(l / 16, q)][l % 16] with
 // row = 4 (l / 16) + q for the f32 instruction and 4 q + l / 16 for the f64 one.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -274,10 +278,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
                                                         T* __restrict__ factor, T* __restrict__ cbpool,
                                                         const int32_t* __restrict__ status, int mode) {
-    constexpr int KC = 16;
+    constexpr int KC = 64;                 // a step's whole K (mode 0) in one chunk
+    constexpr int PER = KC * TILE / 256;   // entries of A (and of B) per thread and chunk
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As[KC][TILE + 4];
-    __shared__ T Bs[KC][TILE + 4];
+    __shared__ T As[KC][TILE];
+    __shared__ T Bs[KC][TILE];
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
     if (status[s] != 0) return;
@@ -291,16 +296,29 @@ __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* _
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int l16 = lane & 15, l4 = lane >> 4;
+    // the destination values, read now so that their latency hides under the products
+    T old[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = J0 + wn * 32 + b * 16 + l16;
+                const int i = I0 + wm * 32 + a * 16 + Mfma<T>::row(l4, q);
+                const bool in = j < jlim && j >= jmin && i < f && i >= j;
+                old[a][b][q] = in ? *front_at(P, C, f, w, i, j) : T(0);
+            }
     acc_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-    // this thread's share of a chunk: 4 entries of A and of B, (kk, r) = (q / 64, q % 64)
-    T pa[4], pb[4];
+    // this thread's share of a chunk: PER entries of A and of B, (kk, r) = (q / 64, q % 64)
+    T pa[PER], pb[PER];
     auto fetch = [&](int kc) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < PER; ++u) {
             const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
             const int kcol = kc + kk;
             const bool kin = kcol < k1;
@@ -311,15 +329,15 @@ __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* _
     fetch(k0);
     for (int kc = k0; kc < k1; kc += KC) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < PER; ++u) {
             const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
             As[kk][r] = pa[u];
             Bs[kk][r] = pb[u];
         }
         __syncthreads();
         if (kc + KC < k1) fetch(kc + KC);
-#pragma unroll
-        for (int k4 = 0; k4 < KC; k4 += 4) {
+        const int kn = min(KC, k1 - kc);
+        for (int k4 = 0; k4 < kn; k4 += 4) {
             const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
             const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
             acc[0][0] = Mfma<T>::mac(a0, b0, acc[0][0]);
@@ -339,8 +357,7 @@ __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* _
             for (int q = 0; q < 4; ++q) {
                 const int i = I0 + wm * 32 + a * 16 + Mfma<T>::row(l4, q);
                 if (i >= f || i < j) continue;
-                T* dst = front_at(P, C, f, w, i, j);
-                *dst -= acc[a][b][q];
+                *front_at(P, C, f, w, i, j) = old[a][b][q] - acc[a][b][q];
             }
         }
 }

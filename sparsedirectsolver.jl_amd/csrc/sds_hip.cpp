@@ -164,6 +164,10 @@ struct sds_handle {
     DevBuf Y, Z, U, R, Bc, norms;
     double* h_norms = nullptr;  // pinned
     int fsmall = 0;
+    // captured refactorization + solve (sds_graph_capture), on a private capture stream
+    hipStream_t cstream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     // profiling (HIP events around every launch on the handle's stream)
     bool prof = false;
     std::vector<hipEvent_t> evpool;
@@ -326,7 +330,15 @@ int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& fi
     return hw;
 }
 
+void graph_free(sds_handle* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+}
+
 void free_numeric(sds_handle* h) {
+    graph_free(h);  // a captured graph refers to the buffers
     free_all(h->numbufs);
     h->ufactor = nullptr;
     if (h->h_info) (void)hipHostFree(h->h_info);
@@ -996,6 +1008,8 @@ int sds_destroy(sds_handle_t h) {
     if (!h) return SDS_SUCCESS;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     else (void)hipDeviceSynchronize();
+    graph_free(h);
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
     free_numeric(h);
     free_all(h->symbufs);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
@@ -1209,8 +1223,14 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     return SDS_SUCCESS;
 }
 
-int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int64_t* info) {
-    if (!h) return SDS_INVALID_VALUE;
+}  // extern "C"
+
+namespace {
+
+// the launches of one factorization on the handle's stream, without the host read of info.
+// sign_sync: upload pivot_sign with a blocking copy (before a graph capture) instead of an
+// asynchronous one on the stream.
+int factorize_enqueue(sds_handle* h, const double* nzval, const sds_factor_params* prm, bool sign_sync) {
     if (h->dtype < 0) return fail(h, SDS_FACTORIZATION_ERROR, "numeric storage not allocated");
     if (!nzval && h->nnz > 0) return fail(h, SDS_INVALID_VALUE, "nzval is NULL");
     double eps = h->dtype == SDS_R64 ? 1e-13 : 1e-5;  // default_pivot_epsilon (src/options.jl:57-61)
@@ -1228,7 +1248,10 @@ int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* 
             return fail(h, SDS_INVALID_VALUE, "pivot_epsilon_alg must be 0, 1 or 2");
         if (!(u >= 0.0 && u <= 1.0)) return fail(h, SDS_INVALID_VALUE, "pivot_threshold must be in [0, 1]");
         if (prm->pivot_sign) {
-            HIPCHK(h, hipMemcpyAsync(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice, h->stream));
+            if (sign_sync)
+                HIPCHK(h, hipMemcpy(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice));
+            else
+                HIPCHK(h, hipMemcpyAsync(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice, h->stream));
             psign = h->psign;
         }
     }
@@ -1260,12 +1283,99 @@ int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* 
         rc = h->dtype == SDS_R64 ? factorize_cholesky<double>(h, nzval, eps, intr)
                                  : factorize_cholesky<float>(h, nzval, eps, intr);
     }
-    if (rc) return rc;
+    return rc;
+}
+
+// info of the last factorization: the one device-to-host read (src/numeric/factorize.jl:383-417)
+int read_info(sds_handle* h, int64_t* info) {
     HIPCHK(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     int64_t col = h->h_info[0];
     if (info) *info = col > 0 ? (int64_t)h->perm[col - 1] + 1 : 0;
+    return SDS_SUCCESS;
+}
+
+int check_solve_args(sds_handle* h, int32_t phase, int32_t solve_mode, int64_t ldb, int64_t ldx, int32_t nrhs) {
+    if (nrhs < 1 || nrhs > MAX_NRHS) return fail(h, SDS_NOT_SUPPORTED, "nrhs must be in 1..64");
+    if (ldb < h->n || ldx < h->n) return fail(h, SDS_INVALID_VALUE, "leading dimension smaller than n");
+    if (solve_mode != 0 && solve_mode != 1 && solve_mode != 2)
+        return fail(h, SDS_INVALID_VALUE, "solve_mode must be 0, 1 or 2");
+    if ((phase & ~SDS_PHASE_SOLVE) != 0 || phase == 0) return fail(h, SDS_INVALID_VALUE, "not a solve phase");
+    return SDS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int64_t* info) {
+    if (!h) return SDS_INVALID_VALUE;
+    h->factorized = false;
+    int rc = factorize_enqueue(h, nzval, prm, false);
+    if (rc) return rc;
+    if ((rc = read_info(h, info))) return rc;
     h->factorized = true;
+    return SDS_SUCCESS;
+}
+
+int sds_graph_capture(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int32_t phase,
+                      int32_t solve_mode, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t nrhs,
+                      int32_t ir_n_steps) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (h->dtype < 0) return fail(h, SDS_FACTORIZATION_ERROR, "numeric storage not allocated");
+    int rc = check_solve_args(h, phase, solve_mode, ldb, ldx, nrhs);
+    if (rc) return rc;
+    if (!B || !X) return fail(h, SDS_INVALID_VALUE, "B and X are needed");
+    // every allocation happens before the capture
+    if ((rc = ensure_solve_workspace(h, nrhs))) return rc;
+    graph_free(h);
+    if (!h->cstream) HIPCHK(h, hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    hipStream_t user = h->stream;
+    const bool prof = h->prof;
+    h->prof = false;
+    h->stream = h->cstream;
+    hipError_t e = hipStreamBeginCapture(h->cstream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = factorize_enqueue(h, nzval, prm, true);
+        const int tr = h->method == M_LU && solve_mode != 0;
+        if (!rc)
+            rc = h->dtype == SDS_R64 ? solve_t<double>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
+                                                       nullptr)
+                                     : solve_t<float>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
+                                                      nullptr);
+        hipError_t e2 = hipStreamEndCapture(h->cstream, &h->graph);
+        if (!rc && e2 != hipSuccess) rc = fail(h, SDS_DEVICE_ERROR, std::string("hipStreamEndCapture: ") +
+                                                                        hipGetErrorString(e2));
+    } else {
+        rc = fail(h, SDS_DEVICE_ERROR, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    h->stream = user;
+    h->prof = prof;
+    if (rc) {
+        graph_free(h);
+        return rc;
+    }
+    HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    return SDS_SUCCESS;
+}
+
+int sds_graph_launch(sds_handle_t h) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->gexec) return fail(h, SDS_INVALID_VALUE, "no captured graph");
+    HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    h->factorized = true;
+    return SDS_SUCCESS;
+}
+
+int sds_graph_info(sds_handle_t h, int64_t* info) {
+    if (!h) return SDS_INVALID_VALUE;
+    return read_info(h, info);
+}
+
+int sds_graph_destroy(sds_handle_t h) {
+    if (!h) return SDS_INVALID_VALUE;
+    graph_free(h);
     return SDS_SUCCESS;
 }
 
@@ -1273,11 +1383,8 @@ int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B
               int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
     if (!h) return SDS_INVALID_VALUE;
     if (!h->factorized) return fail(h, SDS_FACTORIZATION_ERROR, "solve before a factorization");
-    if (nrhs < 1 || nrhs > 64) return fail(h, SDS_NOT_SUPPORTED, "nrhs must be in 1..64");
-    if (ldb < h->n || ldx < h->n) return fail(h, SDS_INVALID_VALUE, "leading dimension smaller than n");
-    if (solve_mode != 0 && solve_mode != 1 && solve_mode != 2)
-        return fail(h, SDS_INVALID_VALUE, "solve_mode must be 0, 1 or 2");
-    if ((phase & ~SDS_PHASE_SOLVE) != 0 || phase == 0) return fail(h, SDS_INVALID_VALUE, "not a solve phase");
+    int rc = check_solve_args(h, phase, solve_mode, ldb, ldx, nrhs);
+    if (rc) return rc;
     // symmetric structures: A^T = A; real values: A^H = A^T (src/solve/refinement.jl:10-15)
     const int tr = h->method == M_LU && solve_mode != 0;
     return h->dtype == SDS_R64

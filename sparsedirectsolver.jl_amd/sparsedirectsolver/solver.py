@@ -199,6 +199,18 @@ class DirectSolver:
             raise FactorizationError(0, f'phase "{phase}": needs "analysis"')
         if phase == "refactorization" and self.stage < STAGE_FACTORIZED:
             raise FactorizationError(0, 'phase "refactorization": needs a previous "factorization"')
+        p, keep = self._factor_params()
+        info = ctypes.c_int64(0)
+        self.info = 0
+        self.stage = STAGE_ANALYZED
+        _check(self._lib, self._h, self._lib.sds_factorize(self._h, ctypes.c_void_p(self.nzval.data_ptr()),
+                                                            ctypes.byref(p), ctypes.byref(info)))
+        del keep
+        self.info = int(info.value)
+        self.stage = STAGE_FACTORIZED
+
+    def _factor_params(self):
+        """(sds_factor_params, objects to keep alive during the call)"""
         o = self.options
         p = _abi.FactorParams()
         p.pivot_type = PIVOT_TYPES[o["pivot_type"]]
@@ -213,13 +225,7 @@ class DirectSolver:
             if sign.shape != (self.n,) or np.any(np.abs(sign) > 1):
                 raise InvalidValueError("pivot_sign must hold n entries in {-1, 0, 1}")
             p.pivot_sign = sign.ctypes.data
-        info = ctypes.c_int64(0)
-        self.info = 0
-        self.stage = STAGE_ANALYZED
-        _check(self._lib, self._h, self._lib.sds_factorize(self._h, ctypes.c_void_p(self.nzval.data_ptr()),
-                                                            ctypes.byref(p), ctypes.byref(info)))
-        self.info = int(info.value)
-        self.stage = STAGE_FACTORIZED
+        return p, (sign, flag)
 
     def _solve(self, phase, X, B):
         torch = _torch()
@@ -262,6 +268,47 @@ def execute(phase, solver, X=None, B=None, asynchronous=True):
         solver._solve(phase, X, B)
     if not asynchronous:
         _torch().cuda.synchronize()
+
+
+def graph_capture(solver, nrhs=1, phase="solve"):
+    """Capture one refactorization + `phase` of the solver into a HIP graph (sds_graph_capture).
+
+    The graph reads the solver's current values tensor and two device buffers the solver keeps
+    (n x nrhs right-hand sides and solutions); graph_replay copies B in, replays and copies X
+    out. The values are read at each replay: update them in place (solver.nzval) between
+    replays, or call update() and capture again."""
+    torch = _torch()
+    if solver.stage < STAGE_ANALYZED:
+        raise FactorizationError(0, "graph_capture needs the analysis")
+    n = solver.n
+    solver._gB = torch.zeros((nrhs, n), dtype=torch.float64, device=solver.nzval.device)
+    solver._gX = torch.zeros((nrhs, n), dtype=torch.float64, device=solver.nzval.device)
+    p, keep = solver._factor_params()
+    o = solver.options
+    nsteps = int(o["ir_n_steps"]) if PHASES[phase] & PHASES["solve_refinement"] else 0
+    _check(solver._lib, solver._h, solver._lib.sds_graph_capture(
+        solver._h, ctypes.c_void_p(solver.nzval.data_ptr()), ctypes.byref(p), PHASES[phase], int(o["solve_mode"]),
+        ctypes.c_void_p(solver._gB.data_ptr()), n, ctypes.c_void_p(solver._gX.data_ptr()), n, nrhs, nsteps))
+    del keep
+    solver._gnrhs = nrhs
+
+
+def graph_replay(solver, X=None, B=None, check_info=False):
+    """Replay the captured graph: B (n x nrhs or n) is copied into the graph's buffer first and
+    the solution into X afterwards (both optional). check_info: wait and return info."""
+    n, nrhs = solver.n, solver._gnrhs
+    if B is not None:
+        solver._gB.copy_(B.reshape(n, nrhs).t())
+    _check(solver._lib, solver._h, solver._lib.sds_graph_launch(solver._h))
+    solver.stage = STAGE_FACTORIZED
+    if X is not None:
+        X.copy_(solver._gX.t().reshape(X.shape))
+    if check_info:
+        info = ctypes.c_int64(0)
+        _check(solver._lib, solver._h, solver._lib.sds_graph_info(solver._h, ctypes.byref(info)))
+        solver.info = int(info.value)
+        return solver.info
+    return None
 
 
 def update(solver, nzval):
