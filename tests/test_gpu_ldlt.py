@@ -121,15 +121,15 @@ def test_ldlt_sub_phases_equal_solve():
 
 
 def test_ldlt_perturbation_and_sign():
-    """Zero diagonal blocks with 'N' pivoting: every zero pivot is perturbed to +-eps with the
-    requested sign; with eps = 0 the first zero pivot is reported through info."""
-    import torch
+    """The zero (1,1)-block of a KKT matrix with delta = 0 eliminated first with 'N' pivoting:
+    every zero pivot is perturbed to +-eps with the requested sign; with eps = 0 the first
+    zero pivot is reported through info."""
     nh, nj = 60, 30
     K = kkt_matrix(nh, nj, 0.0, _rng(12))
     n = nh + nj
     sign = np.zeros(n, dtype=np.int8)
     sign[nh:] = -1
-    perm = np.arange(1, n + 1)
+    perm = np.concatenate([np.arange(nh + 1, n + 1), np.arange(1, nh + 1)])  # the duals first
     s, nz = _solver(K, perm, (32, 0.25, 8), "N", 0.01, eps=1e-10, sign=sign)
     sds.execute("factorization", s)
     ref = oracle.ldlt(s.symbolic, nz, pivot_type="N", pivot_threshold=0.01, eps=1e-10, pivot_sign=sign)
@@ -141,7 +141,6 @@ def test_ldlt_perturbation_and_sign():
     ref0 = oracle.ldlt(s0.symbolic, nz0, pivot_type="N", pivot_threshold=0.01, eps=0.0)
     assert ref0["info"] > 0
     assert sds.getparam(s0, "info") == ref0["info"]
-    del torch
 
 
 def test_ldlt_fp32_refinement():

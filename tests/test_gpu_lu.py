@@ -113,9 +113,11 @@ def test_lu_cudss_doc_example_ones():
 
 
 def test_lu_fp32_refinement_both_modes():
+    """FP32 factor (row choices on FP32 values) + FP64 refinement on a moderately conditioned
+    (cond ~ 5e3) unsymmetric matrix with row interchanges and large fronts."""
     import torch
-    A = _small_diag(M.convection_diffusion3d(12, peclet=10.0), 9, 1e-2)
-    n = A.shape[0]
+    n = 600
+    A = (random_unsym(n, 0.01, _rng(3), 1.0) + 2.0 * sp.identity(n)).tocsr()
     s, _ = _solver(A, None, (128, 0.5, 8), "A", 0.5, fp="Float32")
     sds.execute("factorization", s)
     sds.setparam(s, "ir_n_steps", 6)
