@@ -24,6 +24,17 @@ namespace sds {
 // a pivot in [0, eps] is raised to eps (cuDSS pivot_epsilon), negative or NaN fails.
 __device__ inline bool chol_accept(double a, double eps) { return a > eps || (eps > 0.0 && a >= 0.0); }
 
+// 1/sqrt(a), a > 0: the hardware estimate refined by two Newton steps (the pivots of the
+// large fronts' diagonal blocks; the result is within an ulp or two, well inside the
+// factor's accuracy)
+__device__ inline double rsqrt_nr(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 // ---------------------------------------------------------------------------
 // small fronts: fused assembly + dense Cholesky of the whole front in LDS
 
@@ -189,26 +200,48 @@ __global__ void __launch_bounds__(PANEL_ROWS) k_big_panel(DevSym S, const Task* 
     }
     if (tid == 0) fail = 0;
     __syncthreads();
-    // right-looking Cholesky of D
-    for (int j = 0; j < nb; ++j) {
-        const double a = (double)D[j][j];
-        if (!chol_accept(a, eps)) {
-            if (tid == 0) fail = j + 1;
-            break;  // uniform: every thread sees the same pivot
-        }
-        const T d = T(sqrt(fmax(a, eps)));
-        const T r = T(1) / d;
-        __syncthreads();  // every thread has read D[j][j]
-        if (tid < NB) {
-            if (tid == j) D[j][j] = d;
-            if (tid > j) D[j][tid] *= r;
-            if (tid == 0) rd[j] = r;
+    // blocked Cholesky of D: per 16-column panel, wave 0 factors the panel's rows in registers
+    // (lane i holds row kb + i of the panel; the pivot and the column entries move between
+    // lanes by v_readlane, no barrier inside the panel; 1/sqrt is the hardware estimate with
+    // two Newton steps), then all four waves apply the panel to the trailing block from LDS.
+    const int ii = tid & (NB - 1), part = tid / NB;
+    static_assert(PANEL_ROWS == 4 * NB, "four waves of NB rows");
+    for (int kb = 0; kb < nb; kb += 16) {
+        if (part == 0) {
+            const int row = kb + ii;
+            T p[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) p[c] = row < NB ? D[kb + c][row] : T(0);
+            int bad = 0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const double piv = (double)lane_bcast(p[c], c);
+                if (kb + c < nb && !chol_accept(piv, eps) && !bad) bad = kb + c + 1;
+                const T r = T(rsqrt_nr(fmax(piv, eps)));
+                const T d = T(fmax(piv, eps)) * r;
+                p[c] = ii > c ? p[c] * r : (ii == c ? d : p[c]);
+                if (ii == 0) rd[kb + c] = r;
+#pragma unroll
+                for (int cc = c + 1; cc < 16; ++cc) p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (row < NB) D[kb + c][row] = p[c];
+            if (ii == 0 && bad) fail = bad;
         }
         __syncthreads();
-        const int R = NB - 1 - j;
-        for (int q = tid; q < R * R; q += PANEL_ROWS) {
-            const int jj = j + 1 + q / R, ii = j + 1 + q % R;
-            if (ii >= jj) D[jj][ii] = fma(-D[j][ii], D[j][jj], D[jj][ii]);
+        if (fail) break;  // uniform after the barrier
+        const int j0 = kb + 16;
+        if (ii >= j0) {
+            T lr[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lr[c] = D[kb + c][ii];
+            for (int jj = j0 + part; jj <= ii; jj += 4) {
+                T acc = D[jj][ii];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) acc = fma(-lr[c], D[kb + c][jj], acc);
+                D[jj][ii] = acc;
+            }
         }
         __syncthreads();
     }

@@ -8,13 +8,21 @@
 // partitioned-inverse form of the reference's solve_alg = "algo1" (src/solver.jl:834,
 // src/solve/fused.jl) applied per diagonal block.
 //
-// Per tree level:
+// Per tree level, for the narrow fronts (w <= NB, one diagonal block):
 //   forward : k_fwd_front (one workgroup per front: pull the children's update vectors, solve
-//             L11 block by block, and for fronts with few rows below, their L21 update too),
-//             then k_fwd_l21 for the tall fronts (row tiles of L21 times the solved top).
-//   backward: k_bwd_l21t for the tall fronts (column blocks of L21^T times the gathered
+//             the block, and for fronts with few rows below, their L21 update too), then
+//             k_fwd_l21 for the tall ones (row tiles of L21 times the solved top);
+//   backward: k_bwd_l21t for the tall ones (column blocks of L21^T times the gathered
 //             solution, partial sums per row chunk), then k_bwd_front (sum the partials in
-//             chunk order, solve L11^T block by block).
+//             chunk order, or form L21^T x itself, and solve the block).
+// The wide fronts (w > NB) run their blocks as level-batched steps, many workgroups per front:
+//   forward : k_fwd_front pulls the children's vectors, then per block k one k_fwd_wstep launch
+//             (every workgroup forms y_k = Linv_kk y_k and applies it to its tile of 256 rows
+//             below; the task with c = 1 keeps y_k in Z), then the solved blocks Z -> Y;
+//   backward: k_bwd_l21t partials, k_bwd_front sums them into the top rows, then per block from
+//             the last one k_bwd_wstep launch (every workgroup forms x_k = Linv_kk^T y_k and
+//             subtracts its contribution from a tile of 64 earlier columns; c = 1 keeps x_k in
+//             Z), then Z -> Y.
 // Every entry is produced by one thread with a fixed summation order: no atomics, and the
 // result is bitwise reproducible.
 #include "sds_common.h"
@@ -128,6 +136,7 @@ __global__ void __launch_bounds__(256) k_fwd_front(DevSym S, const int32_t* __re
         }
         __syncthreads();
     }
+    if (w > NB) return;  // a wide front: its blocks are the step launches' work
     T* y = Y + (int64_t)c0 * nrhs;
     const bool fuse = m <= FUSE_ROWS;
     const int rend = fuse ? f : w;  // rows this workgroup updates
@@ -273,15 +282,41 @@ __global__ void __launch_bounds__(256) k_bwd_front(DevSym S, const int32_t* __re
                 y[q] -= acc;
             }
         } else {
-            for (int q = tid; q < w * nrhs; q += nt) {
-                const int j = q / nrhs, r = q - j * nrhs;
-                T acc = T(0);
-                for (int i = w; i < f; ++i) acc += P[(int64_t)j * f + i] * Y[(int64_t)rows[i] * nrhs + r];
-                y[q] -= acc;
+            // L21^T x over 64-row tiles staged in LDS: thread (column jj, right-hand-side pair)
+            // (w <= NB here: a wide front's partial sums are always computed apart)
+            __shared__ T Lt[NB][NB + 1];
+            __shared__ T Yc[NB][8];
+            const int jj = tid & (NB - 1), rr = (tid / NB) * 2;
+            for (int rb = 0; rb < nrhs; rb += 8) {
+                T acc0 = T(0), acc1 = T(0);
+                for (int i0 = w; i0 < f; i0 += NB) {
+                    const int ni = min(NB, f - i0);
+                    for (int q = tid; q < NB * NB; q += nt) {
+                        const int c = q / NB, ii = q % NB;
+                        Lt[c][ii] = (c < w && ii < ni) ? P[(int64_t)c * f + i0 + ii] : T(0);
+                    }
+                    for (int q = tid; q < NB * 8; q += nt) {
+                        const int ii = q >> 3, r = q & 7;
+                        Yc[ii][r] = (ii < ni && rb + r < nrhs) ? Y[(int64_t)rows[i0 + ii] * nrhs + rb + r] : T(0);
+                    }
+                    __syncthreads();
+#pragma unroll 8
+                    for (int ii = 0; ii < NB; ++ii) {
+                        const T l = Lt[jj][ii];
+                        acc0 += l * Yc[ii][rr];
+                        acc1 += l * Yc[ii][rr + 1];
+                    }
+                    __syncthreads();
+                }
+                if (jj < w) {
+                    if (rb + rr < nrhs) y[jj * nrhs + rb + rr] -= acc0;
+                    if (rb + rr + 1 < nrhs) y[jj * nrhs + rb + rr + 1] -= acc1;
+                }
             }
         }
         __syncthreads();
     }
+    if (w > NB) return;  // a wide front: its blocks are the step launches' work
     const int kl = ((w - 1) / NB) * NB;
     for (int k = kl; k >= 0; k -= NB) {
         const int nb = min(NB, w - k);
@@ -297,6 +332,93 @@ __global__ void __launch_bounds__(256) k_bwd_front(DevSym S, const int32_t* __re
         __syncthreads();
         for (int q = tid; q < nb * nrhs; q += nt) y[k * nrhs + q] = ys[q];
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// wide fronts: block steps
+
+// forward block step k of a wide front: y_k = Linv_kk y_k (every workgroup), the task with
+// c = 1 keeps it in Z; this workgroup's 256 rows below the block get L(rows, block) y_k:
+// rows < w of Y, rows >= w of the front's update vector. task: (s, k, row0, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_wstep(DevSym S, const Task* __restrict__ tasks,
+                                                  const T* __restrict__ factor, const T* __restrict__ DB,
+                                                  T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Z, int nrhs) {
+    __shared__ T xs[NB * MAX_NRHS];
+    __shared__ T ys[NB * MAX_NRHS];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
+    const int nb = min(NB, w - k);
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    const T* P = factor + S.panel_off[s];
+    T* y = Y + (int64_t)c0 * nrhs;
+    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = y[k * nrhs + q];
+    __syncthreads();
+    block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 0);
+    __syncthreads();
+    if (t.c)
+        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = ys[q];
+    const int i = row0 + tid;
+    if (i >= f) return;
+    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    if (i >= w && !us) return;
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        T acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = T(0);
+#pragma unroll 8
+        for (int j = 0; j < nb; ++j) {
+            const T l = P[(int64_t)(k + j) * f + i];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] += l * ys[j * nrhs + min(rb + r, nrhs - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (rb + r >= nrhs) break;
+            if (i < w)
+                y[(int64_t)i * nrhs + rb + r] -= acc[r];
+            else
+                us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
+        }
+    }
+}
+
+// backward block step k of a wide front: x_k = Linv_kk^T y_k (every workgroup), the task with
+// c = 1 keeps it in Z; this workgroup's 64 earlier columns [col0, col0 + 64) ∩ [0, k) get
+// y_j -= sum_i L(k + i, j) x_i. task: (s, k, col0, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_wstep(DevSym S, const Task* __restrict__ tasks,
+                                                  const T* __restrict__ factor, const T* __restrict__ DB,
+                                                  T* __restrict__ Y, T* __restrict__ Z, int nrhs) {
+    __shared__ T xs[NB * MAX_NRHS];
+    __shared__ T ys[NB * MAX_NRHS];
+    __shared__ T Lt[NB][NB + 1];  // Lt[jj][i] = L(k + i, col0 + jj)
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, col0 = t.b;
+    const int nb = min(NB, w - k);
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    const T* P = factor + S.panel_off[s];
+    T* y = Y + (int64_t)c0 * nrhs;
+    const int ncol = max(0, min(col0 + NB, k) - col0);
+    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = y[k * nrhs + q];
+    for (int q = tid; q < NB * NB; q += 256) {
+        const int c = q / NB, i = q % NB;
+        Lt[c][i] = (c < ncol && i < nb) ? P[(int64_t)(col0 + c) * f + k + i] : T(0);
+    }
+    __syncthreads();
+    block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 1);
+    __syncthreads();
+    if (t.c)
+        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = ys[q];
+    for (int q = tid; q < ncol * nrhs; q += 256) {
+        const int jj = q / nrhs, r = q - jj * nrhs;
+        T acc = T(0);
+#pragma unroll 8
+        for (int i = 0; i < nb; ++i) acc += Lt[jj][i] * ys[i * nrhs + r];
+        y[(int64_t)(col0 + jj) * nrhs + r] -= acc;
     }
 }
 
@@ -326,6 +448,18 @@ void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 }
 
 template <typename T>
+void launch_fwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
+                      T* U, T* Z, int nrhs) {
+    hipLaunchKernelGGL(k_fwd_wstep<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, DB, Y, U, Z, nrhs);
+}
+
+template <typename T>
+void launch_bwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
+                      T* Z, int nrhs) {
+    hipLaunchKernelGGL(k_bwd_wstep<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, DB, Y, Z, nrhs);
+}
+
+template <typename T>
 void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const int64_t* poff,
                       const T* factor, const T* DB, T* Y, const T* part, int nrhs) {
     hipLaunchKernelGGL(k_bwd_front<T>, dim3(n), dim3(256), 0, st, S, fronts, poff, factor, DB, Y, part, nrhs);
@@ -338,7 +472,10 @@ void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, in
     template void launch_fwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);    \
     template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);   \
     template void launch_bwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, const int64_t*, const T*,   \
-                                      const T*, T*, const T*, int);
+                                      const T*, T*, const T*, int);                                                \
+    template void launch_fwd_wstep<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, T*, T*, \
+                                      int);                                                                        \
+    template void launch_bwd_wstep<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, T*, int);
 INST(double)
 INST(float)
 #undef INST

@@ -74,8 +74,11 @@ struct Level {
     double small_flops = 0, small_bytes = 0;
     double fwd_bytes = 0, fwd_flops = 0;  // factor bytes read by a sweep, flops per right-hand side
     double rhs_rows = 0;                  // rows of the level's fronts (right-hand-side entries touched)
-    // Cholesky sweeps with inverted diagonal blocks: L21 launches of the tall fronts
+    // Cholesky sweeps with inverted diagonal blocks: L21 launches of the tall narrow fronts
+    // (and the L21^T partial sums of the wide ones), block steps and finish of the wide fronts
     Launch inv_fwd{S_FWD_STEP, 0, 0, 0, 0}, inv_bwd{S_BWD_L21, 0, 0, 0, 0};
+    std::vector<Launch> wf_steps, wb_steps;
+    Launch wfin{S_FWD_FINISH, 0, 0, 0, 0};
     double sweep_bytes = 0, sweep_flops = 0;  // all fronts of the level: factor bytes, flops per rhs
     double all_rows = 0;                      // rows of all the level's fronts
 };
@@ -489,7 +492,7 @@ int build_schedule(sds_handle* h, int dtype) {
                 int32_t c = h->child_list[h->child_ptr[s] + r];
                 if (cb_off[c] < 0) continue;
                 int mc = h->fnrows[c] - h->fncols[c];
-                for (int jj = 0; jj < mc; jj += 16) t.push_back({s, c, jj, 16});
+                for (int jj = 0; jj < mc; jj += 4) t.push_back({s, c, jj, 4});
                 // read the child's block, read + write the parent
                 by += 3.0 * dbl_es * (lu ? (double)mc * mc : (double)mc * (mc + 1) / 2);
             }
@@ -660,24 +663,65 @@ int build_schedule(sds_handle* h, int dtype) {
                 L.sweep_flops += 2.0 * e;
                 L.all_rows += f;
             }
-            std::vector<Task> tf, tb;
+            std::vector<Task> tf, tb, fin;
+            std::vector<int32_t> wide;
             int64_t units = 0;
+            int wmax = 0;
             for (int32_t s : L.all) {
                 const int f = h->fnrows[s], w = h->fncols[s], m = f - w;
-                if (m <= FUSE_ROWS || uv_off[s] < 0) continue;
-                for (int r0 = w; r0 < f; r0 += 256) tf.push_back({s, r0, 0, 0});
-                poff[s] = units;
-                const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
-                for (int c = 0; c < w; c += 64)
-                    for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, (int32_t)units});
-                units += (int64_t)nch * w;
+                if (w > NB) {
+                    wide.push_back(s);
+                    wmax = std::max(wmax, w);
+                    for (int c = 0; c < w; c += 256) fin.push_back({s, c, 0, 0});
+                }
+                const bool tall = w <= NB && m > FUSE_ROWS && uv_off[s] >= 0;
+                if (tall)
+                    for (int r0 = w; r0 < f; r0 += 256) tf.push_back({s, r0, 0, 0});
+                if (tall || (w > NB && m > 0)) {
+                    poff[s] = units;
+                    const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
+                    for (int c = 0; c < w; c += 64)
+                        for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, (int32_t)units});
+                    units += (int64_t)nch * w;
+                }
             }
             h->part_units = std::max(h->part_units, units);
-            if (!tf.empty()) {
-                L.inv_fwd = {S_FWD_STEP, (int64_t)tasks.size(), (int)tf.size(), 0, 0};
-                tasks.insert(tasks.end(), tf.begin(), tf.end());
-                L.inv_bwd = {S_BWD_L21, (int64_t)tasks.size(), (int)tb.size(), 0, 0};
-                tasks.insert(tasks.end(), tb.begin(), tb.end());
+            auto put = [&](std::vector<Task>& v, int kind) {
+                Launch ln{kind, (int64_t)tasks.size(), (int)v.size(), 0, 0};
+                tasks.insert(tasks.end(), v.begin(), v.end());
+                return ln;
+            };
+            if (!tf.empty()) L.inv_fwd = put(tf, S_FWD_STEP);
+            if (!tb.empty()) L.inv_bwd = put(tb, S_BWD_L21);
+            if (!fin.empty()) L.wfin = put(fin, S_FWD_FINISH);
+            for (int k = 0; k < wmax; k += NB) {
+                std::vector<Task> t;
+                for (int32_t s : wide) {
+                    const int f = h->fnrows[s], w = h->fncols[s];
+                    if (k >= w) continue;
+                    const int nb = std::min(NB, w - k);
+                    int first = 1;
+                    for (int r0 = k + nb; r0 < f; r0 += 256) {
+                        t.push_back({s, k, r0, first});
+                        first = 0;
+                    }
+                    if (first) t.push_back({s, k, f, 1});
+                }
+                L.wf_steps.push_back(put(t, S_FWD_STEP));
+            }
+            for (int k = ((wmax - 1) / NB) * NB; wmax > 0 && k >= 0; k -= NB) {
+                std::vector<Task> t;
+                for (int32_t s : wide) {
+                    const int w = h->fncols[s];
+                    if (k >= w) continue;
+                    int first = 1;
+                    for (int c = 0; c < k; c += NB) {
+                        t.push_back({s, k, c, first});
+                        first = 0;
+                    }
+                    if (first) t.push_back({s, k, 0, 1});
+                }
+                L.wb_steps.push_back(put(t, S_BWD_STEP));
             }
         }
     }
@@ -875,6 +919,14 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
                     timed(h, SDS_K_FWD, 0, 0, [&] {
                         launch_fwd_l21<T>(st, h->dsym, h->d_tasks + Lv.inv_fwd.off, Lv.inv_fwd.n, L, Y, U, nrhs);
                     });
+                for (const Launch& ln : Lv.wf_steps)
+                    timed(h, SDS_K_FWD, 0, 0, [&] {
+                        launch_fwd_wstep<T>(st, h->dsym, h->d_tasks + ln.off, ln.n, L, Li, Y, U, Z, nrhs);
+                    });
+                if (Lv.wfin.n)
+                    timed(h, SDS_K_FWD, 0, 0, [&] {
+                        launch_finish<T>(st, h->dsym, h->d_tasks + Lv.wfin.off, Lv.wfin.n, Y, Z, nrhs, nullptr);
+                    });
             }
         if (bwd)
             for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
@@ -886,6 +938,14 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
                     launch_bwd_front<T>(st, h->dsym, h->d_fronts + it->off_all, (int)it->all.size(), h->d_poff, L, Li,
                                         Y, part, nrhs);
                 });
+                for (const Launch& ln : it->wb_steps)
+                    timed(h, SDS_K_BWD, 0, 0, [&] {
+                        launch_bwd_wstep<T>(st, h->dsym, h->d_tasks + ln.off, ln.n, L, Li, Y, Z, nrhs);
+                    });
+                if (it->wfin.n)
+                    timed(h, SDS_K_BWD, 0, 0, [&] {
+                        launch_finish<T>(st, h->dsym, h->d_tasks + it->wfin.off, it->wfin.n, Y, Z, nrhs, nullptr);
+                    });
             }
         return;
     }

@@ -98,6 +98,12 @@ template <typename T>
 void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* part,
                      int nrhs);
 template <typename T>
+void launch_fwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
+                      T* U, T* Z, int nrhs);
+template <typename T>
+void launch_bwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
+                      T* Z, int nrhs);
+template <typename T>
 void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const int64_t* poff,
                       const T* factor, const T* DB, T* Y, const T* part, int nrhs);
 void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
