@@ -8,23 +8,24 @@
 // partitioned-inverse form of the reference's solve_alg = "algo1" (src/solver.jl:834,
 // src/solve/fused.jl) applied per diagonal block.
 //
-// Per tree level, for the narrow fronts (w <= NB, one diagonal block):
-//   forward : k_fwd_front (one workgroup per front: pull the children's update vectors, solve
-//             the block, and for fronts with few rows below, their L21 update too), then
-//             k_fwd_l21 for the tall ones (row tiles of L21 times the solved top);
-//   backward: k_bwd_l21t for the tall ones (column blocks of L21^T times the gathered
-//             solution, partial sums per row chunk), then k_bwd_front (sum the partials in
-//             chunk order, or form L21^T x itself, and solve the block).
-// The wide fronts (w > NB) run their blocks as level-batched steps, many workgroups per front:
-//   forward : k_fwd_front pulls the children's vectors, then per block k one k_fwd_wstep launch
-//             (every workgroup forms y_k = Linv_kk y_k and applies it to its tile of 256 rows
-//             below; the task with c = 1 keeps y_k in Z), then the solved blocks Z -> Y;
-//   backward: k_bwd_l21t partials, k_bwd_front sums them into the top rows, then per block from
-//             the last one k_bwd_wstep launch (every workgroup forms x_k = Linv_kk^T y_k and
-//             subtracts its contribution from a tile of 64 earlier columns; c = 1 keeps x_k in
-//             Z), then Z -> Y.
-// Every entry is produced by one thread with a fixed summation order: no atomics, and the
-// result is bitwise reproducible.
+// Per tree level (narrow fronts: w <= NB columns, one diagonal block; wide fronts: w > NB):
+//   forward : k_fwd_front, one workgroup per narrow front: pull the children's update
+//             vectors, y_top = Linv y_top, and, when the front has at most FUSE_ROWS rows below
+//             its columns, its update vector u = L21 y_top; k_fwd_sf, one wave per diagonal
+//             block of the wide fronts: block k waits for the blocks before it (flags) and forms
+//             y_k = Linv_kk (y_k - sum_j L_kj y_j); k_fwd_l21, one wave per 64 rows below the
+//             columns of the remaining fronts: pull the children's entries of those rows and add
+//             L21 y_top.
+//   backward: k_bwd_l21t, partial sums of L21^T x per chunk of L21T_CHUNK rows (one wave per
+//             group of columns, lanes along the rows, a reduce-scatter across the lanes at the
+//             end); k_bwd_front, one workgroup per narrow front: y_top -= L21^T x (partials in
+//             chunk order, or LDS-staged rows), x_top = Linv^T y_top; k_bwd_sf, one wave per
+//             diagonal block of the wide fronts from the last: x_k = Linv_kk^T (y_k - partials
+//             - sum_{j>k} L_jk^T x_j).
+// The flags of the wide fronts' blocks are cleared at the start of each sweep; a block waits
+// only for blocks with lower workgroup index in the same launch, and workgroups are dispatched
+// in index order, so every wait ends (the spin is bounded all the same: on a timeout the wave
+// records an error word and goes on).
 #include "sds_common.h"
 #include "sds_kernels.h"
 
@@ -82,6 +83,41 @@ __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __re
 // ---------------------------------------------------------------------------
 // helpers
 
+extern __shared__ __align__(16) unsigned char sweep_lds[];
+
+// first index in the ascending ri[0, n) with ri[idx] >= v
+__device__ inline int lower_bound_i32(const int32_t* __restrict__ ri, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ri[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr int SPIN_LIMIT = 1 << 22;
+
+// wait (all lanes of the wave) until flag != 0 with agent-scope acquire ordering
+__device__ inline void wait_flag(const int32_t* flag, int32_t* err) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// publish the block's results (all lanes have stored them) with agent-scope release ordering
+__device__ inline void set_flag(int32_t* flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ys[j * nrhs + r] (j < nb) = sum_{i < nb} Linv[j][i] xs[i * nrhs + r] (trans = 0), or
 // sum_i Linv[i][j] xs[i] (trans = 1); Linv column-major ld NB in global memory (L2 resident).
 template <typename T>
@@ -98,35 +134,37 @@ __device__ inline void block_gemv(const T* __restrict__ Li, const T* xs, T* ys, 
     }
 }
 
-constexpr int L21_ROWS = 256;  // rows of L21 per workgroup of k_fwd_l21
-
 // ---------------------------------------------------------------------------
 // forward
 
-// one workgroup per front of the level. fronts: the level's fronts
+// one workgroup per narrow front of the level (w <= NB). Fused fronts (m <= FUSE_ROWS) also
+// form their update vector; the others leave it to k_fwd_l21 (rows >= w, pulls included).
+// LDS: 2 NB nrhs entries.
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_front(DevSym S, const int32_t* __restrict__ fronts,
                                                   const T* __restrict__ factor, const T* __restrict__ DB,
                                                   T* __restrict__ Y, T* __restrict__ U, int nrhs) {
-    __shared__ T xs[NB * MAX_NRHS];
-    __shared__ T ys[NB * MAX_NRHS];
+    T* xs = reinterpret_cast<T*>(sweep_lds);
+    T* ys = xs + NB * nrhs;
     const int s = fronts[blockIdx.x];
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int c0 = S.super_ptr[s];
     const int tid = threadIdx.x, nt = blockDim.x;
     const T* P = factor + S.panel_off[s];
-    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    const bool fused = m <= FUSE_ROWS;
+    T* us = fused && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
     if (us)
         for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
     __syncthreads();
-    // the children's update vectors, in child_list order
+    // the children's update vectors, in child_list order (only the rows < w when not fused)
     for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
         const int c = S.child_list[kc];
         if (S.uv_off[c] < 0) continue;
         const int mc = S.fnrows[c] - S.fncols[c];
         const T* uc = U + S.uv_off[c] * nrhs;
         const int32_t* ri = S.relind + S.relind_ptr[c];
-        for (int q = tid; q < mc * nrhs; q += nt) {
+        const int hi = us ? mc : lower_bound_i32(ri, mc, w);
+        for (int q = tid; q < hi * nrhs; q += nt) {
             const int ii = q / nrhs, r = q - ii * nrhs;
             const int pos = ri[ii];
             if (pos < w)
@@ -136,68 +174,146 @@ __global__ void __launch_bounds__(256) k_fwd_front(DevSym S, const int32_t* __re
         }
         __syncthreads();
     }
-    if (w > NB) return;  // a wide front: its blocks are the step launches' work
     T* y = Y + (int64_t)c0 * nrhs;
-    const bool fuse = m <= FUSE_ROWS;
-    const int rend = fuse ? f : w;  // rows this workgroup updates
-    for (int k = 0; k < w; k += NB) {
-        const int nb = min(NB, w - k);
-        for (int q = tid; q < nb * nrhs; q += nt) xs[q] = y[k * nrhs + q];
-        __syncthreads();
-        block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 0);
-        __syncthreads();
-        for (int q = tid; q < nb * nrhs; q += nt) y[k * nrhs + q] = ys[q];
-        // rows below the block: in L11 (subtract now) or below the front (update vector)
-        for (int q = tid; q < (rend - k - nb) * nrhs; q += nt) {
-            const int i = k + nb + q / nrhs, r = q % nrhs;
-            T acc = T(0);
+    for (int q = tid; q < w * nrhs; q += nt) xs[q] = y[q];
+    __syncthreads();
+    block_gemv<T>(DB + (int64_t)c0 * NB, xs, ys, w, nrhs, 0);
+    __syncthreads();
+    for (int q = tid; q < w * nrhs; q += nt) y[q] = ys[q];
+    if (!us) return;
+    // update vector rows, one per thread, 8 right-hand sides per pass over the row's entries
+    for (int i = w + tid; i < f; i += nt)
+        for (int rb = 0; rb < nrhs; rb += 8) {
+            T acc[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] = T(0);
 #pragma unroll 8
-            for (int j = 0; j < nb; ++j) acc += P[(int64_t)(k + j) * f + i] * ys[j * nrhs + r];
-            if (i < w)
-                y[i * nrhs + r] -= acc;
-            else if (us)
-                us[(i - w) * nrhs + r] += acc;
+            for (int j = 0; j < w; ++j) {
+                const T l = P[(int64_t)j * f + i];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc[r] += l * ys[j * nrhs + min(rb + r, nrhs - 1)];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (rb + r < nrhs) us[(i - w) * nrhs + rb + r] += acc[r];
+        }
+}
+
+// one wave per diagonal block of the wide fronts: task (s, k, flag base, 0). The block's rows
+// get their children's entries, then y_k -= L_kj y_j for every earlier block j (after its
+// flag), then y_k = Linv_kk y_k, and the block's flag is set. LDS: NB nrhs + NB 8 entries.
+template <typename T>
+__global__ void __launch_bounds__(64) k_fwd_sf(DevSym S, const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                              const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ U,
+                                              int32_t* __restrict__ flags, int32_t* __restrict__ err, int nrhs) {
+    T* ts = reinterpret_cast<T*>(sweep_lds);  // NB x nrhs
+    T* yj = ts + NB * nrhs;                   // NB x 8
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, k = t.a;
+    const int f = S.fnrows[s], w = S.fncols[s];
+    const int c0 = S.super_ptr[s];
+    const int nb = min(NB, w - k);
+    const int lane = threadIdx.x;
+    const T* P = factor + S.panel_off[s];
+    T* y = Y + (int64_t)c0 * nrhs;
+    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+        const int c = S.child_list[kc];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        const int lo = lower_bound_i32(ri, mc, k), hi = lower_bound_i32(ri, mc, k + nb);
+        for (int q = lane; q < (hi - lo) * nrhs; q += 64) {
+            const int ii = lo + q / nrhs, r = q % nrhs;
+            y[(int64_t)ri[ii] * nrhs + r] -= uc[(int64_t)ii * nrhs + r];
         }
         __syncthreads();
     }
+    const int i = k + min(lane, nb - 1);  // this lane's row (clamped; results of lanes >= nb unused)
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        T acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = T(0);
+        for (int j = 0; j < k; j += NB) {
+            T l[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) l[c] = P[(int64_t)(j + c) * f + i];
+            wait_flag(flags + t.b + j / NB, err);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) yj[lane * 8 + r] = rb + r < nrhs ? y[(int64_t)(j + lane) * nrhs + rb + r] : T(0);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc[r] += l[c] * yj[c * 8 + r];
+            __syncthreads();
+        }
+        if (lane < nb)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (rb + r < nrhs) ts[lane * nrhs + rb + r] = y[(int64_t)i * nrhs + rb + r] - acc[r];
+    }
+    __syncthreads();
+    const T* Li = DB + (int64_t)(c0 + k) * NB;
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        T acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = T(0);
+        for (int ii = 0; ii < nb; ++ii) {
+            const T li = Li[(int64_t)ii * NB + lane];  // Linv(lane, ii): zero above the diagonal
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] += li * ts[ii * nrhs + min(rb + r, nrhs - 1)];
+        }
+        if (lane < nb)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (rb + r < nrhs) y[(int64_t)(k + lane) * nrhs + rb + r] = acc[r];
+    }
+    set_flag(flags + t.b + k / NB);
 }
 
-// U[rows >= w] += L21 y_top for the tall fronts: a tile of L21_ROWS rows per workgroup, one
-// row per thread, all nrhs right-hand sides 8 at a time. task: (s, row0)
+// the update vector rows [r0, r0 + 64) (r0 >= w) of a front that is wide or tall, one wave:
+// zero them, add the children's entries of those rows (child_list order), add L21 y_top.
+// task: (s, r0)
 template <typename T>
-__global__ void __launch_bounds__(L21_ROWS) k_fwd_l21(DevSym S, const Task* __restrict__ tasks,
-                                                     const T* __restrict__ factor, const T* __restrict__ Y,
-                                                     T* __restrict__ U, int nrhs) {
-    __shared__ T yt[NB * MAX_NRHS];
+__global__ void __launch_bounds__(64) k_fwd_l21(DevSym S, const Task* __restrict__ tasks,
+                                               const T* __restrict__ factor, const T* __restrict__ Y,
+                                               T* __restrict__ U, int nrhs) {
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
     const int c0 = S.super_ptr[s];
-    const int i = t.a + threadIdx.x;
+    const int r0 = t.a, r1 = min(f, r0 + 64);
+    const int lane = threadIdx.x;
     const T* P = factor + S.panel_off[s];
     T* us = U + S.uv_off[s] * nrhs;
+    for (int q = lane; q < (r1 - r0) * nrhs; q += 64) us[(int64_t)(r0 - w) * nrhs + q] = T(0);
+    __syncthreads();
+    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+        const int c = S.child_list[kc];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        const int lo = lower_bound_i32(ri, mc, r0), hi = lower_bound_i32(ri, mc, r1);
+        for (int q = lane; q < (hi - lo) * nrhs; q += 64) {
+            const int ii = lo + q / nrhs, r = q % nrhs;
+            us[(int64_t)(ri[ii] - w) * nrhs + r] += uc[(int64_t)ii * nrhs + r];
+        }
+        __syncthreads();
+    }
+    const int i = min(r0 + lane, f - 1);
     const T* y = Y + (int64_t)c0 * nrhs;
     for (int rb = 0; rb < nrhs; rb += 8) {
         T acc[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) acc[r] = T(0);
-        for (int k = 0; k < w; k += NB) {
-            const int nb = min(NB, w - k);
-            __syncthreads();
-            for (int q = threadIdx.x; q < nb * 8; q += L21_ROWS) {
-                const int j = q >> 3, r = q & 7;
-                yt[q] = rb + r < nrhs ? y[(k + j) * nrhs + rb + r] : T(0);
-            }
-            __syncthreads();
-            if (i < f) {
-#pragma unroll 8
-                for (int j = 0; j < nb; ++j) {
-                    const T l = P[(int64_t)(k + j) * f + i];
+#pragma unroll 16
+        for (int j = 0; j < w; ++j) {
+            const T l = P[(int64_t)j * f + i];
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) acc[r] += l * yt[j * 8 + r];
-                }
-            }
+            for (int r = 0; r < 8; ++r) acc[r] += l * y[(int64_t)j * nrhs + min(rb + r, nrhs - 1)];
         }
-        if (i < f)
+        if (r0 + lane < f)
 #pragma unroll
             for (int r = 0; r < 8; ++r)
                 if (rb + r < nrhs) us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
@@ -207,63 +323,69 @@ __global__ void __launch_bounds__(L21_ROWS) k_fwd_l21(DevSym S, const Task* __re
 // ---------------------------------------------------------------------------
 // backward
 
-// partial sums of L21^T x over a chunk of L21_T_CHUNK rows for a block of NB columns of a tall
-// front: part[(chunk * w + j) * nrhs + r]. Thread (column jj, rhs pair) accumulates over the
-// rows; L21 tiles and the gathered solution rows are staged in LDS. task: (s, col0, chunk, poff)
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_l21t(DevSym S, const Task* __restrict__ tasks,
-                                                 const T* __restrict__ factor, const T* __restrict__ Y,
-                                                 T* __restrict__ part, int nrhs) {
-    __shared__ T Lt[64][65];
-    __shared__ T Yc[64][8];
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], col0 = t.a, chunk = t.b;
-    const int ncol = min(64, w - col0);
-    const int tid = threadIdx.x, jj = tid & 63, rr = (tid >> 6) * 2;
+// partial sums of L21^T x over the chunk of L21T_CHUNK rows of a task (s, col0, chunk): a wave
+// per CW = 64 / RW columns; lanes along the rows; RW right-hand sides per pass. The 64 sums
+// (CW columns x RW right-hand sides) are reduce-scattered across the lanes (recursive
+// halving: lane l ends with sum l). part[(poff + chunk w + j) nrhs + r].
+template <typename T, int RW>
+__global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const Task* __restrict__ tasks,
+                                                const int64_t* __restrict__ poff, const T* __restrict__ factor,
+                                                const T* __restrict__ Y, T* __restrict__ part, int nrhs) {
+    constexpr int CW = 64 / RW;
+    const Task t = tasks[blockIdx.x / RW];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], chunk = t.b;
+    const int col0 = t.a + (blockIdx.x % RW) * CW;
+    if (col0 >= w) return;
+    const int lane = threadIdx.x;
     const T* P = factor + S.panel_off[s];
     const int32_t* rows = S.rowval + S.rowptr[s];
     const int r0 = w + chunk * L21T_CHUNK, r1 = min(f, r0 + L21T_CHUNK);
-    T* out = part + (int64_t)t.c * nrhs;  // this front's partials start at t.c (in units of nrhs)
-    for (int rb = 0; rb < nrhs; rb += 8) {
-        T acc0 = T(0), acc1 = T(0);
-        for (int i0 = r0; i0 < r1; i0 += 64) {
-            const int ni = min(64, r1 - i0);
+    T* out = part + (poff[s] + (int64_t)chunk * w) * nrhs;
+    for (int rb = 0; rb < nrhs; rb += RW) {
+        T a[64];
 #pragma unroll
-            for (int q = tid; q < 64 * 64; q += 256) {
-                const int c = q >> 6, ii = q & 63;
-                Lt[c][ii] = (c < ncol && ii < ni) ? P[(int64_t)(col0 + c) * f + i0 + ii] : T(0);
+        for (int v = 0; v < 64; ++v) a[v] = T(0);
+#pragma unroll(RW >= 4 ? 2 : 1)
+        for (int i = r0 + lane; i < r1; i += 64) {
+            const T* yr = Y + (int64_t)rows[i] * nrhs;
+            T x[RW];
+#pragma unroll
+            for (int r = 0; r < RW; ++r) x[r] = yr[min(rb + r, nrhs - 1)];
+            const T* pc = P + (int64_t)col0 * f + i;  // column col0 + c; columns past w read w - 1
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const T l = *pc;
+                pc += col0 + c + 1 < w ? f : 0;
+#pragma unroll
+                for (int r = 0; r < RW; ++r) a[c * RW + r] += l * x[r];
             }
-            for (int q = tid; q < 64 * 8; q += 256) {
-                const int ii = q >> 3, r = q & 7;
-                Yc[ii][r] = (ii < ni && rb + r < nrhs) ? Y[(int64_t)rows[i0 + ii] * nrhs + rb + r] : T(0);
-            }
-            __syncthreads();
-#pragma unroll 8
-            for (int ii = 0; ii < 64; ++ii) {
-                const T l = Lt[jj][ii];
-                acc0 += l * Yc[ii][rr];
-                acc1 += l * Yc[ii][rr + 1];
-            }
-            __syncthreads();
         }
-        if (jj < ncol) {
-            const int64_t o = ((int64_t)chunk * w + col0 + jj) * nrhs + rb + rr;
-            if (rb + rr < nrhs) out[o] = acc0;
-            if (rb + rr + 1 < nrhs) out[o + 1] = acc1;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const bool up = (lane & d) != 0;
+#pragma unroll
+            for (int v = 0; v < d; ++v) {
+                const T send = up ? a[v] : a[v + d];
+                const T keep = up ? a[v + d] : a[v];
+                a[v] = keep + __shfl_xor(send, d);
+            }
         }
+        const int c = lane / RW, r = lane % RW;
+        if (col0 + c < w && rb + r < nrhs) out[(int64_t)(col0 + c) * nrhs + rb + r] = a[0];
     }
 }
 
-// one workgroup per front of the level: y_top -= L21^T x_bottom (from the partials of the tall
-// fronts, in chunk order, or computed here), then x_top = L11^-T y_top block by block from the
-// last. poff[s]: the front's partial-sum offset (units of nrhs), -1 when computed here.
+// one workgroup per narrow front of the level: y_top -= L21^T x (partials in chunk order, or
+// from LDS-staged solution rows when fused), then x_top = Linv^T y_top.
+// LDS: 3 NB nrhs entries.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_front(DevSym S, const int32_t* __restrict__ fronts,
                                                   const int64_t* __restrict__ poff, const T* __restrict__ factor,
                                                   const T* __restrict__ DB, T* __restrict__ Y,
                                                   const T* __restrict__ part, int nrhs) {
-    __shared__ T xs[NB * MAX_NRHS];
-    __shared__ T ys[NB * MAX_NRHS];
+    T* ts = reinterpret_cast<T*>(sweep_lds);
+    T* ys = ts + NB * nrhs;
+    T* Xs = ys + NB * nrhs;
     const int s = fronts[blockIdx.x];
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int c0 = S.super_ptr[s];
@@ -271,158 +393,130 @@ __global__ void __launch_bounds__(256) k_bwd_front(DevSym S, const int32_t* __re
     const T* P = factor + S.panel_off[s];
     const int32_t* rows = S.rowval + S.rowptr[s];
     T* y = Y + (int64_t)c0 * nrhs;
-    const int64_t po = poff[s];
+    for (int q = tid; q < w * nrhs; q += nt) ts[q] = y[q];
+    __syncthreads();
     if (m > 0) {
+        const int64_t po = poff[s];
         if (po >= 0) {
             const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
             const T* pp = part + po * nrhs;
             for (int q = tid; q < w * nrhs; q += nt) {
-                T acc = T(0);
-                for (int ch = 0; ch < nch; ++ch) acc += pp[(int64_t)ch * w * nrhs + q];
-                y[q] -= acc;
+                T v = ts[q];
+                for (int ch = 0; ch < nch; ++ch) v -= pp[(int64_t)ch * w * nrhs + q];
+                ts[q] = v;
             }
         } else {
-            // L21^T x over 64-row tiles staged in LDS: thread (column jj, right-hand-side pair)
-            // (w <= NB here: a wide front's partial sums are always computed apart)
-            __shared__ T Lt[NB][NB + 1];
-            __shared__ T Yc[NB][8];
-            const int jj = tid & (NB - 1), rr = (tid / NB) * 2;
-            for (int rb = 0; rb < nrhs; rb += 8) {
-                T acc0 = T(0), acc1 = T(0);
-                for (int i0 = w; i0 < f; i0 += NB) {
-                    const int ni = min(NB, f - i0);
-                    for (int q = tid; q < NB * NB; q += nt) {
-                        const int c = q / NB, ii = q % NB;
-                        Lt[c][ii] = (c < w && ii < ni) ? P[(int64_t)c * f + i0 + ii] : T(0);
-                    }
-                    for (int q = tid; q < NB * 8; q += nt) {
-                        const int ii = q >> 3, r = q & 7;
-                        Yc[ii][r] = (ii < ni && rb + r < nrhs) ? Y[(int64_t)rows[i0 + ii] * nrhs + rb + r] : T(0);
-                    }
-                    __syncthreads();
-#pragma unroll 8
-                    for (int ii = 0; ii < NB; ++ii) {
-                        const T l = Lt[jj][ii];
-                        acc0 += l * Yc[ii][rr];
-                        acc1 += l * Yc[ii][rr + 1];
-                    }
-                    __syncthreads();
+            for (int i0 = w; i0 < f; i0 += 64) {
+                const int ni = min(64, f - i0);
+                for (int q = tid; q < ni * nrhs; q += nt) {
+                    const int ii = q / nrhs, r = q - ii * nrhs;
+                    Xs[q] = Y[(int64_t)rows[i0 + ii] * nrhs + r];
                 }
-                if (jj < w) {
-                    if (rb + rr < nrhs) y[jj * nrhs + rb + rr] -= acc0;
-                    if (rb + rr + 1 < nrhs) y[jj * nrhs + rb + rr + 1] -= acc1;
+                __syncthreads();
+                for (int q = tid; q < w * nrhs; q += nt) {
+                    const int j = q / nrhs, r = q - j * nrhs;
+                    const T* pj = P + (int64_t)j * f + i0;
+                    T acc = T(0);
+                    for (int ii = 0; ii < ni; ++ii) acc += pj[ii] * Xs[ii * nrhs + r];
+                    ts[q] -= acc;
                 }
+                __syncthreads();
             }
         }
         __syncthreads();
     }
-    if (w > NB) return;  // a wide front: its blocks are the step launches' work
-    const int kl = ((w - 1) / NB) * NB;
-    for (int k = kl; k >= 0; k -= NB) {
-        const int nb = min(NB, w - k);
-        // the block's right-hand side minus L(rows below in L11, block)^T x
-        for (int q = tid; q < nb * nrhs; q += nt) {
-            const int j = q / nrhs, r = q - j * nrhs;
-            T acc = T(0);
-            for (int i = k + nb; i < w; ++i) acc += P[(int64_t)(k + j) * f + i] * y[i * nrhs + r];
-            xs[q] = y[k * nrhs + q] - acc;
-        }
-        __syncthreads();
-        block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 1);
-        __syncthreads();
-        for (int q = tid; q < nb * nrhs; q += nt) y[k * nrhs + q] = ys[q];
-        __syncthreads();
-    }
+    block_gemv<T>(DB + (int64_t)c0 * NB, ts, ys, w, nrhs, 1);
+    __syncthreads();
+    for (int q = tid; q < w * nrhs; q += nt) y[q] = ys[q];
 }
 
-// ---------------------------------------------------------------------------
-// wide fronts: block steps
-
-// forward block step k of a wide front: y_k = Linv_kk y_k (every workgroup), the task with
-// c = 1 keeps it in Z; this workgroup's 256 rows below the block get L(rows, block) y_k:
-// rows < w of Y, rows >= w of the front's update vector. task: (s, k, row0, c)
+// one wave per diagonal block of the wide fronts, the blocks of a front from the last: task
+// (s, k, flag base, 0). t = y_k - L21^T x partials (chunk order) - sum_{j > k, descending}
+// L_jk^T x_j (after block j's flag; L_jk staged through LDS), x_k = Linv_kk^T t, flag set.
+// LDS: NB nrhs + NB (NB + 1) + NB 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_wstep(DevSym S, const Task* __restrict__ tasks,
-                                                  const T* __restrict__ factor, const T* __restrict__ DB,
-                                                  T* __restrict__ Y, T* __restrict__ U, T* __restrict__ Z, int nrhs) {
-    __shared__ T xs[NB * MAX_NRHS];
-    __shared__ T ys[NB * MAX_NRHS];
+__global__ void __launch_bounds__(64) k_bwd_sf(DevSym S, const Task* __restrict__ tasks,
+                                              const int64_t* __restrict__ poff, const T* __restrict__ factor,
+                                              const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ part,
+                                              int32_t* __restrict__ flags, int32_t* __restrict__ err, int nrhs) {
+    T* ts = reinterpret_cast<T*>(sweep_lds);  // NB x nrhs
+    T(*Lt)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(ts + NB * nrhs);
+    T* xj = ts + NB * nrhs + NB * (NB + 1);  // NB x 8
     const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
-    const int nb = min(NB, w - k);
+    const int s = t.s, k = t.a;
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int c0 = S.super_ptr[s];
-    const int tid = threadIdx.x;
+    const int nb = min(NB, w - k);
+    const int lane = threadIdx.x;
     const T* P = factor + S.panel_off[s];
     T* y = Y + (int64_t)c0 * nrhs;
-    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = y[k * nrhs + q];
-    __syncthreads();
-    block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 0);
-    __syncthreads();
-    if (t.c)
-        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = ys[q];
-    const int i = row0 + tid;
-    if (i >= f) return;
-    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
-    if (i >= w && !us) return;
+    const int64_t po = poff[s];
+    const int nch = m > 0 && po >= 0 ? (m + L21T_CHUNK - 1) / L21T_CHUNK : 0;
+    for (int q = lane; q < nb * nrhs; q += 64) {
+        T v = y[(int64_t)k * nrhs + q];
+        for (int ch = 0; ch < nch; ++ch) v -= part[(po + (int64_t)ch * w + k) * nrhs + q];
+        ts[q] = v;
+    }
+    const int cl = min(lane, nb - 1);  // this lane's column (clamped; lanes >= nb unused)
+    const int kl = ((w - 1) / NB) * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
         T acc[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) acc[r] = T(0);
-#pragma unroll 8
-        for (int j = 0; j < nb; ++j) {
-            const T l = P[(int64_t)(k + j) * f + i];
+        for (int j = kl; j > k; j -= NB) {
+            const int njb = min(NB, w - j);
+            const int ir = j + min(lane, njb - 1);
+            __syncthreads();
+#pragma unroll 16
+            for (int c = 0; c < NB; ++c) Lt[c][lane] = lane < njb ? P[(int64_t)(k + min(c, nb - 1)) * f + ir] : T(0);
+            wait_flag(flags + t.b + j / NB, err);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] += l * ys[j * nrhs + min(rb + r, nrhs - 1)];
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            if (rb + r >= nrhs) break;
-            if (i < w)
-                y[(int64_t)i * nrhs + rb + r] -= acc[r];
-            else
-                us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
-        }
-    }
-}
-
-// backward block step k of a wide front: x_k = Linv_kk^T y_k (every workgroup), the task with
-// c = 1 keeps it in Z; this workgroup's 64 earlier columns [col0, col0 + 64) ∩ [0, k) get
-// y_j -= sum_i L(k + i, j) x_i. task: (s, k, col0, c)
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_wstep(DevSym S, const Task* __restrict__ tasks,
-                                                  const T* __restrict__ factor, const T* __restrict__ DB,
-                                                  T* __restrict__ Y, T* __restrict__ Z, int nrhs) {
-    __shared__ T xs[NB * MAX_NRHS];
-    __shared__ T ys[NB * MAX_NRHS];
-    __shared__ T Lt[NB][NB + 1];  // Lt[jj][i] = L(k + i, col0 + jj)
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, col0 = t.b;
-    const int nb = min(NB, w - k);
-    const int c0 = S.super_ptr[s];
-    const int tid = threadIdx.x;
-    const T* P = factor + S.panel_off[s];
-    T* y = Y + (int64_t)c0 * nrhs;
-    const int ncol = max(0, min(col0 + NB, k) - col0);
-    for (int q = tid; q < nb * nrhs; q += 256) xs[q] = y[k * nrhs + q];
-    for (int q = tid; q < NB * NB; q += 256) {
-        const int c = q / NB, i = q % NB;
-        Lt[c][i] = (c < ncol && i < nb) ? P[(int64_t)(col0 + c) * f + k + i] : T(0);
-    }
-    __syncthreads();
-    block_gemv<T>(DB + (int64_t)(c0 + k) * NB, xs, ys, nb, nrhs, 1);
-    __syncthreads();
-    if (t.c)
-        for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = ys[q];
-    for (int q = tid; q < ncol * nrhs; q += 256) {
-        const int jj = q / nrhs, r = q - jj * nrhs;
-        T acc = T(0);
+            for (int r = 0; r < 8; ++r) xj[lane * 8 + r] = rb + r < nrhs ? y[(int64_t)ir * nrhs + rb + r] : T(0);
+            __syncthreads();
 #pragma unroll 8
-        for (int i = 0; i < nb; ++i) acc += Lt[jj][i] * ys[i * nrhs + r];
-        y[(int64_t)(col0 + jj) * nrhs + r] -= acc;
+            for (int ii = 0; ii < NB; ++ii) {
+                const T l = Lt[cl][ii];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc[r] += l * xj[ii * 8 + r];
+            }
+        }
+        __syncthreads();
+        if (lane < nb)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (rb + r < nrhs) ts[lane * nrhs + rb + r] -= acc[r];
     }
+    // Linv^T: Lt[c][i] = Linv(i, c)
+    const T* Li = DB + (int64_t)(c0 + k) * NB;
+    __syncthreads();
+#pragma unroll 16
+    for (int c = 0; c < NB; ++c) Lt[c][lane] = c < nb ? Li[(int64_t)c * NB + lane] : T(0);
+    __syncthreads();
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        T acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = T(0);
+        for (int ii = 0; ii < nb; ++ii) {
+            const T l = Lt[cl][ii];  // zero above the diagonal
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] += l * ts[ii * nrhs + min(rb + r, nrhs - 1)];
+        }
+        if (lane < nb)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (rb + r < nrhs) y[(int64_t)(k + lane) * nrhs + rb + r] = acc[r];
+    }
+    set_flag(flags + t.b + k / NB);
 }
 
 // ---------------------------------------------------------------------------
+
+// dynamic LDS beyond 64 KiB has to be allowed per kernel (once per process and kernel)
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
 
 template <typename T>
 void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB) {
@@ -430,52 +524,68 @@ void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, in
 }
 
 template <typename T>
-void launch_fwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const T* factor, const T* DB, T* Y,
-                      T* U, int nrhs) {
-    hipLaunchKernelGGL(k_fwd_front<T>, dim3(n), dim3(256), 0, st, S, fronts, factor, DB, Y, U, nrhs);
+void launch_fwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const T* factor,
+                      const T* DB, T* Y, T* U, int nrhs) {
+    const size_t lds = 2 * NB * nrhs * sizeof(T);
+    allow_lds(k_fwd_front<T>, lds);
+    hipLaunchKernelGGL(k_fwd_front<T>, dim3(n), dim3(threads), lds, st, S, fronts, factor, DB, Y, U, nrhs);
+}
+
+template <typename T>
+void launch_fwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
+                   const T* U, int32_t* flags, int32_t* err, int nrhs) {
+    hipLaunchKernelGGL(k_fwd_sf<T>, dim3(n), dim3(64), (NB * nrhs + NB * 8) * sizeof(T), st, S, tasks, factor, DB, Y,
+                       U, flags, err, nrhs);
 }
 
 template <typename T>
 void launch_fwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* U,
                     int nrhs) {
-    hipLaunchKernelGGL(k_fwd_l21<T>, dim3(n), dim3(L21_ROWS), 0, st, S, tasks, factor, Y, U, nrhs);
+    hipLaunchKernelGGL(k_fwd_l21<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, Y, U, nrhs);
 }
 
 template <typename T>
-void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* part,
-                     int nrhs) {
-    hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, part, nrhs);
+void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
+                     const T* Y, T* part, int nrhs) {
+    if (nrhs >= 8)
+        hipLaunchKernelGGL((k_bwd_l21t<T, 8>), dim3(n * 8), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
+    else if (nrhs >= 4)
+        hipLaunchKernelGGL((k_bwd_l21t<T, 4>), dim3(n * 4), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
+    else if (nrhs >= 2)
+        hipLaunchKernelGGL((k_bwd_l21t<T, 2>), dim3(n * 2), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
+    else
+        hipLaunchKernelGGL((k_bwd_l21t<T, 1>), dim3(n), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
 }
 
 template <typename T>
-void launch_fwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
-                      T* U, T* Z, int nrhs) {
-    hipLaunchKernelGGL(k_fwd_wstep<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, DB, Y, U, Z, nrhs);
-}
-
-template <typename T>
-void launch_bwd_wstep(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
-                      T* Z, int nrhs) {
-    hipLaunchKernelGGL(k_bwd_wstep<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, DB, Y, Z, nrhs);
-}
-
-template <typename T>
-void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, const int64_t* poff,
+void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const int64_t* poff,
                       const T* factor, const T* DB, T* Y, const T* part, int nrhs) {
-    hipLaunchKernelGGL(k_bwd_front<T>, dim3(n), dim3(256), 0, st, S, fronts, poff, factor, DB, Y, part, nrhs);
+    const size_t lds = 3 * NB * nrhs * sizeof(T);
+    allow_lds(k_bwd_front<T>, lds);
+    hipLaunchKernelGGL(k_bwd_front<T>, dim3(n), dim3(threads), lds, st, S, fronts, poff, factor, DB, Y, part, nrhs);
+}
+
+template <typename T>
+void launch_bwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
+                   const T* DB, T* Y, const T* part, int32_t* flags, int32_t* err, int nrhs) {
+    const size_t lds = (NB * nrhs + NB * (NB + 1) + NB * 8) * sizeof(T);
+    allow_lds(k_bwd_sf<T>, lds);
+    hipLaunchKernelGGL(k_bwd_sf<T>, dim3(n), dim3(64), lds, st, S, tasks, poff, factor, DB, Y, part, flags, err, nrhs);
 }
 
 #define INST(T)                                                                                                     \
-    template void launch_block_inverse<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);             \
-    template void launch_fwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, const T*, T*, T*, \
-                                      int);                                                                        \
+    template void launch_block_inverse<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                   \
+    template void launch_fwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const T*, const T*, T*, \
+                                      T*, int);                                                                    \
+    template void launch_fwd_sf<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, const T*,  \
+                                   int32_t*, int32_t*, int);                                                       \
     template void launch_fwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);    \
-    template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);   \
-    template void launch_bwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, const int64_t*, const T*,   \
-                                      const T*, T*, const T*, int);                                                \
-    template void launch_fwd_wstep<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, T*, T*, \
-                                      int);                                                                        \
-    template void launch_bwd_wstep<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, T*, int);
+    template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const Task*, int, const int64_t*, const T*,       \
+                                     const T*, T*, int);                                                           \
+    template void launch_bwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const int64_t*,        \
+                                      const T*, const T*, T*, const T*, int);                                      \
+    template void launch_bwd_sf<T>(hipStream_t, const DevSym&, const Task*, int, const int64_t*, const T*,         \
+                                   const T*, T*, const T*, int32_t*, int32_t*, int);
 INST(double)
 INST(float)
 #undef INST

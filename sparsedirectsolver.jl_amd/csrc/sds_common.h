@@ -44,10 +44,11 @@ constexpr int TILE = 64;   // Schur-update tile
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
-// Cholesky sweeps with inverted diagonal blocks: fronts with more than FUSE_ROWS rows below
-// their fully summed block get separate L21 launches, with L21^T partial sums per L21T_CHUNK rows
+// Cholesky sweeps with inverted diagonal blocks: narrow fronts (at most NB columns) with at most
+// FUSE_ROWS rows below their columns are solved by one workgroup each; the others get separate
+// L21 launches, with L21^T partial sums per L21T_CHUNK rows
 constexpr int FUSE_ROWS = 256;
-constexpr int L21T_CHUNK = 1024;
+constexpr int L21T_CHUNK = 512;
 
 // value of `v` in lane `lane` (wave-uniform) of the calling wave, as a register broadcast
 __device__ inline double lane_bcast(double v, int lane) {

@@ -74,13 +74,15 @@ struct Level {
     double small_flops = 0, small_bytes = 0;
     double fwd_bytes = 0, fwd_flops = 0;  // factor bytes read by a sweep, flops per right-hand side
     double rhs_rows = 0;                  // rows of the level's fronts (right-hand-side entries touched)
-    // Cholesky sweeps with inverted diagonal blocks: L21 launches of the tall narrow fronts
-    // (and the L21^T partial sums of the wide ones), block steps and finish of the wide fronts
+    // Cholesky sweeps with inverted diagonal blocks: the narrow fronts (one workgroup each, of
+    // narrow_threads threads), the diagonal blocks of the wide fronts (sf_fwd, sf_bwd), and the
+    // L21 row tiles (inv_fwd) and L21^T partial sums (inv_bwd) of the wide and tall fronts
+    std::vector<int32_t> narrow;
+    int64_t off_narrow = 0;
+    int narrow_threads = 256;
+    Launch narrow_fwd{S_FWD_STEP, 0, 0, 0, 0}, narrow_bwd{S_BWD_STEP, 0, 0, 0, 0};
+    Launch sf_fwd{S_FWD_STEP, 0, 0, 0, 0}, sf_bwd{S_BWD_STEP, 0, 0, 0, 0};
     Launch inv_fwd{S_FWD_STEP, 0, 0, 0, 0}, inv_bwd{S_BWD_L21, 0, 0, 0, 0};
-    std::vector<Launch> wf_steps, wb_steps;
-    Launch wfin{S_FWD_FINISH, 0, 0, 0, 0};
-    double sweep_bytes = 0, sweep_flops = 0;  // all fronts of the level: factor bytes, flops per rhs
-    double all_rows = 0;                      // rows of all the level's fronts
 };
 
 // plan launch kind -> profiling kind of include/sds_hip.h
@@ -127,6 +129,8 @@ struct sds_handle {
     Launch inv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the diagonal blocks (Cholesky)
     int64_t* d_poff = nullptr;               // per front: partial-sum offset of its L21^T (units of nrhs)
     int64_t part_units = 0;                  // partial-sum buffer, units of nrhs
+    int64_t nflags = 0;                      // diagonal-block flags of the wide fronts' sweeps
+    int32_t* d_flags = nullptr;              // nflags + 1 words: the flags, then the spin-timeout word
     DevBuf part;
     std::vector<DevBuf> symbufs;  // device maps
     DevSym dsym{};
@@ -655,75 +659,61 @@ int build_schedule(sds_handle* h, int dtype) {
             }
         h->inv_launch = {L_POTRF, (int64_t)tasks.size(), (int)t.size(), fl, by};
         tasks.insert(tasks.end(), t.begin(), t.end());
+        int64_t units = 0, nfl = 0;
+        auto put = [&](std::vector<Task>& v, int kind, double fl2, double by2) {
+            Launch ln{kind, (int64_t)tasks.size(), (int)v.size(), fl2, by2};
+            tasks.insert(tasks.end(), v.begin(), v.end());
+            return ln;
+        };
         for (auto& L : h->levels) {
-            for (int32_t s : L.all) {
-                const double f = h->fnrows[s], w = h->fncols[s];
-                const double e = f * w - w * (w - 1) / 2 + w * NB;  // panel + inverse blocks
-                L.sweep_bytes += dbl_es * e;
-                L.sweep_flops += 2.0 * e;
-                L.all_rows += f;
-            }
-            std::vector<Task> tf, tb, fin;
-            std::vector<int32_t> wide;
-            int64_t units = 0;
-            int wmax = 0;
+            std::vector<Task> sff, sfb, tf, tb;
+            double nfl_ = 0, nby = 0, sfl = 0, sby = 0, ffl = 0, fby = 0;
+            int fmax = 0;
+            units = 0;
             for (int32_t s : L.all) {
                 const int f = h->fnrows[s], w = h->fncols[s], m = f - w;
-                if (w > NB) {
-                    wide.push_back(s);
-                    wmax = std::max(wmax, w);
-                    for (int c = 0; c < w; c += 256) fin.push_back({s, c, 0, 0});
+                // factor entries a sweep reads: L11 lower triangle + L21, and the inverse blocks
+                const double e11 = (double)w * (w + 1) / 2 + (double)NB * w, e21 = (double)m * w;
+                const bool narrow = w <= NB;
+                const bool fused = narrow && m <= FUSE_ROWS;
+                if (narrow) {
+                    L.narrow.push_back(s);
+                    fmax = std::max(fmax, f);
+                    nfl_ += 2.0 * (e11 + (fused ? e21 : 0));
+                    nby += dbl_es * (e11 + (fused ? e21 : 0));
+                } else {
+                    for (int k = 0; k < w; k += NB) sff.push_back({s, k, (int32_t)nfl, 0});
+                    for (int k = ((w - 1) / NB) * NB; k >= 0; k -= NB) sfb.push_back({s, k, (int32_t)nfl, 0});
+                    nfl += (w + NB - 1) / NB;
+                    sfl += 2.0 * e11;
+                    sby += dbl_es * e11;
                 }
-                const bool tall = w <= NB && m > FUSE_ROWS && uv_off[s] >= 0;
-                if (tall)
-                    for (int r0 = w; r0 < f; r0 += 256) tf.push_back({s, r0, 0, 0});
-                if (tall || (w > NB && m > 0)) {
+                if (!fused && m > 0 && uv_off[s] >= 0) {
+                    for (int r0 = w; r0 < f; r0 += 64) tf.push_back({s, r0, 0, 0});
                     poff[s] = units;
                     const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
                     for (int c = 0; c < w; c += 64)
-                        for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, (int32_t)units});
+                        for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, 0});
                     units += (int64_t)nch * w;
+                    ffl += 2.0 * e21;
+                    fby += dbl_es * e21;
                 }
             }
+            L.narrow_threads = fmax <= 64 ? 64 : 256;
+            L.narrow_fwd = {S_FWD_STEP, 0, (int)L.narrow.size(), nfl_, nby};
+            L.narrow_bwd = {S_BWD_STEP, 0, (int)L.narrow.size(), nfl_, nby};
             h->part_units = std::max(h->part_units, units);
-            auto put = [&](std::vector<Task>& v, int kind) {
-                Launch ln{kind, (int64_t)tasks.size(), (int)v.size(), 0, 0};
-                tasks.insert(tasks.end(), v.begin(), v.end());
-                return ln;
-            };
-            if (!tf.empty()) L.inv_fwd = put(tf, S_FWD_STEP);
-            if (!tb.empty()) L.inv_bwd = put(tb, S_BWD_L21);
-            if (!fin.empty()) L.wfin = put(fin, S_FWD_FINISH);
-            for (int k = 0; k < wmax; k += NB) {
-                std::vector<Task> t;
-                for (int32_t s : wide) {
-                    const int f = h->fnrows[s], w = h->fncols[s];
-                    if (k >= w) continue;
-                    const int nb = std::min(NB, w - k);
-                    int first = 1;
-                    for (int r0 = k + nb; r0 < f; r0 += 256) {
-                        t.push_back({s, k, r0, first});
-                        first = 0;
-                    }
-                    if (first) t.push_back({s, k, f, 1});
-                }
-                L.wf_steps.push_back(put(t, S_FWD_STEP));
-            }
-            for (int k = ((wmax - 1) / NB) * NB; wmax > 0 && k >= 0; k -= NB) {
-                std::vector<Task> t;
-                for (int32_t s : wide) {
-                    const int w = h->fncols[s];
-                    if (k >= w) continue;
-                    int first = 1;
-                    for (int c = 0; c < k; c += NB) {
-                        t.push_back({s, k, c, first});
-                        first = 0;
-                    }
-                    if (first) t.push_back({s, k, 0, 1});
-                }
-                L.wb_steps.push_back(put(t, S_BWD_STEP));
-            }
+            if (!sff.empty()) L.sf_fwd = put(sff, S_FWD_STEP, sfl, sby);
+            if (!sfb.empty()) L.sf_bwd = put(sfb, S_BWD_STEP, sfl, sby);
+            if (!tf.empty()) L.inv_fwd = put(tf, S_FWD_STEP, ffl, fby);
+            if (!tb.empty()) L.inv_bwd = put(tb, S_BWD_L21, ffl, fby);
         }
+        h->nflags = nfl;
+        for (auto& L : h->levels) {
+            L.off_narrow = (int64_t)flist.size();
+            flist.insert(flist.end(), L.narrow.begin(), L.narrow.end());
+        }
+        if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (nfl + 1), (void**)&h->d_flags))) return rc;
     }
     if ((rc = upload(h, h->numbufs, poff, &h->d_poff))) return rc;
     if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
@@ -910,43 +900,48 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
         // inverted diagonal blocks (DB) and level launches over all fronts
         const T* Li = static_cast<const T*>(h->dbuf);
         T* part = static_cast<T*>(h->part.p);
-        if (fwd)
+        int32_t* flags = h->d_flags;
+        int32_t* err = h->d_flags + h->nflags;
+        const double rb = es * nrhs;  // right-hand-side bytes per row
+        if (fwd) {
+            if (h->nflags) launch_fill_i32(st, flags, h->nflags + 1, 0);
             for (auto& Lv : h->levels) {
-                timed(h, SDS_K_FWD, Lv.sweep_flops * nrhs, Lv.sweep_bytes + 3.0 * es * nrhs * Lv.all_rows, [&] {
-                    launch_fwd_front<T>(st, h->dsym, h->d_fronts + Lv.off_all, (int)Lv.all.size(), L, Li, Y, U, nrhs);
-                });
+                if (!Lv.narrow.empty())
+                    timed(h, SDS_K_FWD, Lv.narrow_fwd.flops * nrhs, Lv.narrow_fwd.bytes, [&] {
+                        launch_fwd_front<T>(st, h->dsym, h->d_fronts + Lv.off_narrow, (int)Lv.narrow.size(),
+                                            Lv.narrow_threads, L, Li, Y, U, nrhs);
+                    });
+                if (Lv.sf_fwd.n)
+                    timed(h, SDS_K_FWD, Lv.sf_fwd.flops * nrhs, Lv.sf_fwd.bytes, [&] {
+                        launch_fwd_sf<T>(st, h->dsym, h->d_tasks + Lv.sf_fwd.off, Lv.sf_fwd.n, L, Li, Y, U, flags, err,
+                                         nrhs);
+                    });
                 if (Lv.inv_fwd.n)
-                    timed(h, SDS_K_FWD, 0, 0, [&] {
+                    timed(h, SDS_K_FWD, Lv.inv_fwd.flops * nrhs, Lv.inv_fwd.bytes + rb * 64.0 * Lv.inv_fwd.n, [&] {
                         launch_fwd_l21<T>(st, h->dsym, h->d_tasks + Lv.inv_fwd.off, Lv.inv_fwd.n, L, Y, U, nrhs);
                     });
-                for (const Launch& ln : Lv.wf_steps)
-                    timed(h, SDS_K_FWD, 0, 0, [&] {
-                        launch_fwd_wstep<T>(st, h->dsym, h->d_tasks + ln.off, ln.n, L, Li, Y, U, Z, nrhs);
-                    });
-                if (Lv.wfin.n)
-                    timed(h, SDS_K_FWD, 0, 0, [&] {
-                        launch_finish<T>(st, h->dsym, h->d_tasks + Lv.wfin.off, Lv.wfin.n, Y, Z, nrhs, nullptr);
-                    });
             }
-        if (bwd)
+        }
+        if (bwd) {
+            if (h->nflags) launch_fill_i32(st, flags, h->nflags + 1, 0);
             for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
                 if (it->inv_bwd.n)
-                    timed(h, SDS_K_BWD, 0, 0, [&] {
-                        launch_bwd_l21t<T>(st, h->dsym, h->d_tasks + it->inv_bwd.off, it->inv_bwd.n, L, Y, part, nrhs);
+                    timed(h, SDS_K_BWD, it->inv_bwd.flops * nrhs, it->inv_bwd.bytes, [&] {
+                        launch_bwd_l21t<T>(st, h->dsym, h->d_tasks + it->inv_bwd.off, it->inv_bwd.n, h->d_poff, L, Y,
+                                           part, nrhs);
                     });
-                timed(h, SDS_K_BWD, it->sweep_flops * nrhs, it->sweep_bytes + 3.0 * es * nrhs * it->all_rows, [&] {
-                    launch_bwd_front<T>(st, h->dsym, h->d_fronts + it->off_all, (int)it->all.size(), h->d_poff, L, Li,
-                                        Y, part, nrhs);
-                });
-                for (const Launch& ln : it->wb_steps)
-                    timed(h, SDS_K_BWD, 0, 0, [&] {
-                        launch_bwd_wstep<T>(st, h->dsym, h->d_tasks + ln.off, ln.n, L, Li, Y, Z, nrhs);
+                if (!it->narrow.empty())
+                    timed(h, SDS_K_BWD, it->narrow_bwd.flops * nrhs, it->narrow_bwd.bytes, [&] {
+                        launch_bwd_front<T>(st, h->dsym, h->d_fronts + it->off_narrow, (int)it->narrow.size(),
+                                            it->narrow_threads, h->d_poff, L, Li, Y, part, nrhs);
                     });
-                if (it->wfin.n)
-                    timed(h, SDS_K_BWD, 0, 0, [&] {
-                        launch_finish<T>(st, h->dsym, h->d_tasks + it->wfin.off, it->wfin.n, Y, Z, nrhs, nullptr);
+                if (it->sf_bwd.n)
+                    timed(h, SDS_K_BWD, it->sf_bwd.flops * nrhs, it->sf_bwd.bytes, [&] {
+                        launch_bwd_sf<T>(st, h->dsym, h->d_tasks + it->sf_bwd.off, it->sf_bwd.n, h->d_poff, L, Li, Y,
+                                         part, flags, err, nrhs);
                     });
             }
+        }
         return;
     }
     auto run = [&](const Launch& ln, int kind) {
