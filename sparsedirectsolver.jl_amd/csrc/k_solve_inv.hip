@@ -8,28 +8,27 @@
 // partitioned-inverse form of the reference's solve_alg = "algo1" (src/solver.jl:834,
 // src/solve/fused.jl) applied per diagonal block.
 //
-// Per tree level (narrow fronts: w <= NB columns, one diagonal block; wide fronts: w > NB):
-//   forward : k_fwd_front, one workgroup per narrow front: pull the children's update
-//             vectors, y_top = Linv y_top, and, when the front has at most FUSE_ROWS rows below
-//             its columns, its update vector u = L21 y_top; k_fwd_sf, one wave per diagonal
-//             block of the wide fronts: block k waits for the blocks before it (flags) and forms
-//             y_k = Linv_kk (y_k - sum_j L_kj y_j); k_fwd_l21, one wave per 64 rows below the
-//             columns of the remaining fronts: pull the children's entries of those rows and add
-//             L21 y_top.
-//   backward: k_bwd_l21t, partial sums of L21^T x per chunk of L21T_CHUNK rows (one wave per
-//             group of columns, lanes along the rows, a reduce-scatter across the lanes at the
-//             end); k_bwd_front, one workgroup per narrow front: y_top -= L21^T x (partials in
-//             chunk order, or LDS-staged rows), x_top = Linv^T y_top; k_bwd_sf, one wave per
-//             diagonal block of the wide fronts from the last: x_k = Linv_kk^T (y_k - partials
-//             - sum_{j>k} L_jk^T x_j).
-// The flags of the wide fronts' blocks are cleared at the start of each sweep; a block waits
-// only for blocks with lower workgroup index in the same launch, and workgroups are dispatched
-// in index order, so every wait ends (the spin is bounded all the same: on a timeout the wave
-// records an error word and goes on).
+// A front's fully summed columns are cut into solve steps of at most SOLVE_STEP columns
+// [c0, c1) (one step unless the front is wider). Per tree level and step:
+//   forward : k_fwd_diag, one workgroup per front: (first step) pull the children's update
+//             vectors; then block by block y_k = Linv_kk t_k and the step's rows below the block
+//             get t_i -= L_ik y_k, one row per thread, the step's right-hand side in LDS. Narrow
+//             fronts with at most FUSE_ROWS rows below their columns form their update vector
+//             here too; for the others k_fwd_rows adds L(rows, step) y_step to the rows below
+//             the step (64 per wave; the update vector rows are zeroed and get the children's
+//             entries of their rows first).
+//   backward: k_bwd_l21t, partial sums of L(rows below the step, step)^T x per chunk of
+//             L21T_CHUNK rows (one wave per group of columns, lanes along the rows, a
+//             reduce-scatter across the lanes at the end); k_bwd_diag, one workgroup per front:
+//             t = y_step - partials, then block by block from the last x_k = Linv_kk^T t_k and
+//             the step's columns left of the block get t_j -= L_kj^T x_k, one column per thread.
 #include "sds_common.h"
 #include "sds_kernels.h"
 
 namespace sds {
+
+// the solve kernels contract multiply-adds (their results are compared with a tolerance)
+#pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
 // Linv of the diagonal block [k, k + nb) of front s: one wave; lane c computes column c of
@@ -98,425 +97,635 @@ __device__ inline int lower_bound_i32(const int32_t* __restrict__ ri, int n, int
     return lo;
 }
 
-constexpr int SPIN_LIMIT = 1 << 22;
-
-// wait (all lanes of the wave) until flag != 0 with agent-scope acquire ordering
-__device__ inline void wait_flag(const int32_t* flag, int32_t* err) {
-    int spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        if (++spins > SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+// one recursive-halving exchange at lane distance D: the lanes with bit D set keep the upper
+// halves a[D, 2D), the others the lower, each adding its partner's copy
+template <int D, typename T>
+__device__ __attribute__((always_inline)) inline void halving_step(T (&a)[64], int lane) {
+    const bool up = (lane & D) != 0;
+#pragma unroll
+    for (int v = 0; v < D; ++v) {
+        const T send = up ? a[v] : a[v + D];
+        const T keep = up ? a[v + D] : a[v];
+        a[v] = keep + __shfl_xor(send, D);
     }
 }
 
-// publish the block's results (all lanes have stored them) with agent-scope release ordering
-__device__ inline void set_flag(int32_t* flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ys[j * nrhs + r] (j < nb) = sum_{i < nb} Linv[j][i] xs[i * nrhs + r] (trans = 0), or
-// sum_i Linv[i][j] xs[i] (trans = 1); Linv column-major ld NB in global memory (L2 resident).
+// sum over the 64 lanes of a[v] for v < 64, reduce-scattered: lane l returns the sum of a[l]
+// (recursive halving, 63 exchanges; a fixed summation order)
 template <typename T>
-__device__ inline void block_gemv(const T* __restrict__ Li, const T* xs, T* ys, int nb, int nrhs, int trans) {
-    for (int q = threadIdx.x; q < nb * nrhs; q += blockDim.x) {
-        const int j = q / nrhs, r = q - j * nrhs;
-        T acc = T(0);
-        if (!trans) {
-            for (int i = 0; i <= j; ++i) acc += Li[(int64_t)i * NB + j] * xs[i * nrhs + r];
-        } else {
-            for (int i = j; i < nb; ++i) acc += Li[(int64_t)j * NB + i] * xs[i * nrhs + r];
-        }
-        ys[q] = acc;
-    }
+__device__ __attribute__((always_inline)) inline T wave_reduce_scatter64(T (&a)[64]) {
+    const int lane = threadIdx.x & 63;
+    halving_step<32>(a, lane);
+    halving_step<16>(a, lane);
+    halving_step<8>(a, lane);
+    halving_step<4>(a, lane);
+    halving_step<2>(a, lane);
+    halving_step<1>(a, lane);
+    return a[0];
 }
 
 // ---------------------------------------------------------------------------
 // forward
 
-// one workgroup per narrow front of the level (w <= NB). Fused fronts (m <= FUSE_ROWS) also
-// form their update vector; the others leave it to k_fwd_l21 (rows >= w, pulls included).
-// LDS: 2 NB nrhs entries.
+// one workgroup per front step: task (step, fused); wave v owns diagonal block v of the step
+// (64 rows, lane = row, its right-hand side t in registers). Stage s: every wave first loads
+// 64 entries into registers (wave s: its Linv row; a later wave v: its row of the tile L_vs,
+// coalesced across the lanes), then wave s forms y_s = Linv_ss t_s and publishes it in LDS,
+// and every later wave v subtracts L_vs y_s from its t. A fused front (one block, at most
+// FUSE_ROWS rows below) also forms its update vector u_i = L(i, front) y, one row per thread.
+// LDS: (waves + 1) 64 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_front(DevSym S, const int32_t* __restrict__ fronts,
-                                                  const T* __restrict__ factor, const T* __restrict__ DB,
-                                                  T* __restrict__ Y, T* __restrict__ U, int nrhs) {
-    T* xs = reinterpret_cast<T*>(sweep_lds);
-    T* ys = xs + NB * nrhs;
-    const int s = fronts[blockIdx.x];
+__global__ void __launch_bounds__(512) k_fwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                 const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U,
+                                                 int nrhs) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    T* yx = reinterpret_cast<T*>(sweep_lds);  // (nt / 64) blocks x 64 rows x 8: the blocks' solutions
+    T* tb = yx + (nt >> 6) * 64 * 8;          // 64 x 8: the block being solved
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1;
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
-    const int c0 = S.super_ptr[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int cg = S.super_ptr[s];
     const T* P = factor + S.panel_off[s];
-    const bool fused = m <= FUSE_ROWS;
-    T* us = fused && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
-    if (us)
-        for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
-    __syncthreads();
-    // the children's update vectors, in child_list order (only the rows < w when not fused)
-    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-        const int c = S.child_list[kc];
-        if (S.uv_off[c] < 0) continue;
-        const int mc = S.fnrows[c] - S.fncols[c];
-        const T* uc = U + S.uv_off[c] * nrhs;
-        const int32_t* ri = S.relind + S.relind_ptr[c];
-        const int hi = us ? mc : lower_bound_i32(ri, mc, w);
-        for (int q = tid; q < hi * nrhs; q += nt) {
-            const int ii = q / nrhs, r = q - ii * nrhs;
-            const int pos = ri[ii];
-            if (pos < w)
-                Y[(int64_t)(c0 + pos) * nrhs + r] -= uc[q];
-            else
-                us[(pos - w) * nrhs + r] += uc[q];
-        }
+    const bool fused = tk.a != 0;  // single step and single block; rows below the front done here
+    T* us = fused && m > 0 && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    if (c0 == 0) {
+        if (us)
+            for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
         __syncthreads();
-    }
-    T* y = Y + (int64_t)c0 * nrhs;
-    for (int q = tid; q < w * nrhs; q += nt) xs[q] = y[q];
-    __syncthreads();
-    block_gemv<T>(DB + (int64_t)c0 * NB, xs, ys, w, nrhs, 0);
-    __syncthreads();
-    for (int q = tid; q < w * nrhs; q += nt) y[q] = ys[q];
-    if (!us) return;
-    // update vector rows, one per thread, 8 right-hand sides per pass over the row's entries
-    for (int i = w + tid; i < f; i += nt)
-        for (int rb = 0; rb < nrhs; rb += 8) {
-            T acc[8];
-#pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] = T(0);
-#pragma unroll 8
-            for (int j = 0; j < w; ++j) {
-                const T l = P[(int64_t)j * f + i];
-#pragma unroll
-                for (int r = 0; r < 8; ++r) acc[r] += l * ys[j * nrhs + min(rb + r, nrhs - 1)];
+        // the children's update vectors, in child_list order (only the rows < w unless fused)
+        for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+            const int c = S.child_list[kc];
+            if (S.uv_off[c] < 0) continue;
+            const int mc = S.fnrows[c] - S.fncols[c];
+            const T* uc = U + S.uv_off[c] * nrhs;
+            const int32_t* ri = S.relind + S.relind_ptr[c];
+            const int hi = us ? mc : lower_bound_i32(ri, mc, w);
+            for (int q = tid; q < hi * nrhs; q += nt) {
+                const int ii = q / nrhs, r = q - ii * nrhs;
+                const int pos = ri[ii];
+                if (pos < w)
+                    Y[(int64_t)(cg + pos) * nrhs + r] -= uc[q];
+                else
+                    us[(pos - w) * nrhs + r] += uc[q];
             }
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) us[(i - w) * nrhs + rb + r] += acc[r];
+            __syncthreads();
         }
-}
-
-// one wave per diagonal block of the wide fronts: task (s, k, flag base, 0). The block's rows
-// get their children's entries, then y_k -= L_kj y_j for every earlier block j (after its
-// flag), then y_k = Linv_kk y_k, and the block's flag is set. LDS: NB nrhs + NB 8 entries.
-template <typename T>
-__global__ void __launch_bounds__(64) k_fwd_sf(DevSym S, const Task* __restrict__ tasks, const T* __restrict__ factor,
-                                              const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ U,
-                                              int32_t* __restrict__ flags, int32_t* __restrict__ err, int nrhs) {
-    T* ts = reinterpret_cast<T*>(sweep_lds);  // NB x nrhs
-    T* yj = ts + NB * nrhs;                   // NB x 8
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, k = t.a;
-    const int f = S.fnrows[s], w = S.fncols[s];
-    const int c0 = S.super_ptr[s];
-    const int nb = min(NB, w - k);
-    const int lane = threadIdx.x;
-    const T* P = factor + S.panel_off[s];
-    T* y = Y + (int64_t)c0 * nrhs;
-    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-        const int c = S.child_list[kc];
-        if (S.uv_off[c] < 0) continue;
-        const int mc = S.fnrows[c] - S.fncols[c];
-        const T* uc = U + S.uv_off[c] * nrhs;
-        const int32_t* ri = S.relind + S.relind_ptr[c];
-        const int lo = lower_bound_i32(ri, mc, k), hi = lower_bound_i32(ri, mc, k + nb);
-        for (int q = lane; q < (hi - lo) * nrhs; q += 64) {
-            const int ii = lo + q / nrhs, r = q % nrhs;
-            y[(int64_t)ri[ii] * nrhs + r] -= uc[(int64_t)ii * nrhs + r];
-        }
-        __syncthreads();
     }
-    const int i = k + min(lane, nb - 1);  // this lane's row (clamped; results of lanes >= nb unused)
+    const int nw = c1 - c0, nblk = (nw + NB - 1) / NB;
+    const bool active = wv < nblk;
+    const int kb = c0 + NB * wv;  // this wave's block
+    const int nbw = active ? min(NB, c1 - kb) : 1;
+    const int kbs = active ? kb : c0;  // a valid block for the loads of an idle wave
+    const int row = kb + min(lane, nbw - 1);  // this lane's row (clamped)
+    T* y = Y + (int64_t)cg * nrhs;
     for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
         T acc[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] = T(0);
-        for (int j = 0; j < k; j += NB) {
+        for (int r = 0; r < 8; ++r) acc[r] = active && r < nr ? y[(int64_t)row * nrhs + rb + r] : T(0);
+        for (int sb = 0; sb < nblk; ++sb) {
+            const int ks = c0 + NB * sb, nbs = min(NB, c1 - ks);
+            // this stage's registers: the tile row L(row, block s) (unconditional loads with a
+            // chosen base: a conditional reload of a register array makes the compiler spill)
+            const bool tile = active && wv > sb;
+            const T* base = tile ? P + (int64_t)ks * f + row : DB + (int64_t)(cg + kbs) * NB + min(lane, nbw - 1);
+            const int64_t stride = tile ? f : NB;
+            const int lim = tile ? NB : nbw;
             T l[NB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) l[c] = P[(int64_t)(j + c) * f + i];
-            wait_flag(flags + t.b + j / NB, err);
+            for (int c = 0; c < NB; ++c) {
+                const T v = base[min(c, lim - 1) * stride];
+                l[c] = c < lim ? v : T(0);
+            }
+            if (wv == sb)
 #pragma unroll
-            for (int r = 0; r < 8; ++r) yj[lane * 8 + r] = rb + r < nrhs ? y[(int64_t)(j + lane) * nrhs + rb + r] : T(0);
+                for (int r = 0; r < 8; ++r) tb[lane * 8 + r] = acc[r];
             __syncthreads();
+            if (wv == sb) {
+                T yv[8];
 #pragma unroll
-            for (int c = 0; c < NB; ++c)
+                for (int r = 0; r < 8; ++r) yv[r] = T(0);
 #pragma unroll
-                for (int r = 0; r < 8; ++r) acc[r] += l[c] * yj[c * 8 + r];
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) yv[r] += l[i] * tb[i * 8 + r];  // Linv(lane, i): zero for i > lane
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const T v = lane < nbs ? yv[r] : T(0);
+                    yx[(sb * NB + lane) * 8 + r] = v;
+                }
+            }
             __syncthreads();
+            if (tile) {
+                const T* ysb = yx + sb * NB * 8;
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) acc[r] -= l[c] * ysb[c * 8 + r];
+            }
         }
-        if (lane < nb)
+        // the step's solution to global memory (a store inside the stage loop makes the
+        // compiler spill the register tiles)
+        for (int q = tid; q < nw * 8; q += nt)
+            if ((q & 7) < nr) y[(int64_t)(c0 + (q >> 3)) * nrhs + rb + (q & 7)] = yx[q];
+        if (us) {
+            // update vector rows, one per thread: u_i += L(i, 0:w) y
+            for (int i = w + tid; i < f; i += nt) {
+                T a[8];
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) ts[lane * nrhs + rb + r] = y[(int64_t)i * nrhs + rb + r] - acc[r];
-    }
-    __syncthreads();
-    const T* Li = DB + (int64_t)(c0 + k) * NB;
-    for (int rb = 0; rb < nrhs; rb += 8) {
-        T acc[8];
+                for (int r = 0; r < 8; ++r) a[r] = T(0);
+#pragma unroll 16
+                for (int j = 0; j < w; ++j) {
+                    const T lv = P[(int64_t)j * f + i];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] = T(0);
-        for (int ii = 0; ii < nb; ++ii) {
-            const T li = Li[(int64_t)ii * NB + lane];  // Linv(lane, ii): zero above the diagonal
+                    for (int r = 0; r < 8; ++r) a[r] += lv * yx[j * 8 + r];
+                }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] += li * ts[ii * nrhs + min(rb + r, nrhs - 1)];
-        }
-        if (lane < nb)
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) y[(int64_t)(k + lane) * nrhs + rb + r] = acc[r];
-    }
-    set_flag(flags + t.b + k / NB);
-}
-
-// the update vector rows [r0, r0 + 64) (r0 >= w) of a front that is wide or tall, one wave:
-// zero them, add the children's entries of those rows (child_list order), add L21 y_top.
-// task: (s, r0)
-template <typename T>
-__global__ void __launch_bounds__(64) k_fwd_l21(DevSym S, const Task* __restrict__ tasks,
-                                               const T* __restrict__ factor, const T* __restrict__ Y,
-                                               T* __restrict__ U, int nrhs) {
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
-    const int c0 = S.super_ptr[s];
-    const int r0 = t.a, r1 = min(f, r0 + 64);
-    const int lane = threadIdx.x;
-    const T* P = factor + S.panel_off[s];
-    T* us = U + S.uv_off[s] * nrhs;
-    for (int q = lane; q < (r1 - r0) * nrhs; q += 64) us[(int64_t)(r0 - w) * nrhs + q] = T(0);
-    __syncthreads();
-    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-        const int c = S.child_list[kc];
-        if (S.uv_off[c] < 0) continue;
-        const int mc = S.fnrows[c] - S.fncols[c];
-        const T* uc = U + S.uv_off[c] * nrhs;
-        const int32_t* ri = S.relind + S.relind_ptr[c];
-        const int lo = lower_bound_i32(ri, mc, r0), hi = lower_bound_i32(ri, mc, r1);
-        for (int q = lane; q < (hi - lo) * nrhs; q += 64) {
-            const int ii = lo + q / nrhs, r = q % nrhs;
-            us[(int64_t)(ri[ii] - w) * nrhs + r] += uc[(int64_t)ii * nrhs + r];
+                for (int r = 0; r < 8; ++r)
+                    if (r < nr) us[(i - w) * nrhs + rb + r] += a[r];
+            }
         }
         __syncthreads();
     }
-    const int i = min(r0 + lane, f - 1);
-    const T* y = Y + (int64_t)c0 * nrhs;
+}
+
+// rows [r0, r1) below a front step (64 per workgroup of 8 waves, within L11 or within the update
+// vector): y_i -= L(i, step) y_step, or u_i += L(i, step) y_step. At the front's first step the
+// rows get the children's entries of those rows first (child_list order; update vector rows are
+// zeroed before). Wave v takes the step's 16-column batches v, v + 8, ...; y_step is staged in
+// LDS; the eight partial sums are added in wave order. task: (step, r0).
+// LDS: SOLVE_STEP 8 + 8 64 8 entries.
+template <typename T>
+__global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __restrict__ steps,
+                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                 T* __restrict__ Y, T* __restrict__ U, int nrhs) {
+    constexpr int NWV = 8;
+    T* ys = reinterpret_cast<T*>(sweep_lds);  // step columns x 8
+    T* red = ys + SOLVE_STEP * 8;              // NWV waves x 64 rows x 8
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1;
+    const int f = S.fnrows[s], w = S.fncols[s];
+    const int cg = S.super_ptr[s];
+    const int r0 = tk.a, r1 = min(r0 + 64, r0 < w ? w : f);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = NWV * 64;
+    const T* P = factor + S.panel_off[s];
+    const bool below = r0 >= w;
+    T* us = below ? U + S.uv_off[s] * nrhs : nullptr;
+    if (c0 == 0) {
+        if (below)
+            for (int q = tid; q < (r1 - r0) * nrhs; q += nt) us[(int64_t)(r0 - w) * nrhs + q] = T(0);
+        __syncthreads();
+        for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+            const int c = S.child_list[kc];
+            if (S.uv_off[c] < 0) continue;
+            const int mc = S.fnrows[c] - S.fncols[c];
+            const T* uc = U + S.uv_off[c] * nrhs;
+            const int32_t* ri = S.relind + S.relind_ptr[c];
+            const int lo = lower_bound_i32(ri, mc, r0), hi = lower_bound_i32(ri, mc, r1);
+            for (int q = tid; q < (hi - lo) * nrhs; q += nt) {
+                const int ii = lo + q / nrhs, r = q % nrhs;
+                const T v = uc[(int64_t)ii * nrhs + r];
+                if (below)
+                    us[(int64_t)(ri[ii] - w) * nrhs + r] += v;
+                else
+                    Y[(int64_t)(cg + ri[ii]) * nrhs + r] -= v;
+            }
+            __syncthreads();
+        }
+    }
+    const int nw = c1 - c0;
+    const int i = min(r0 + lane, r1 - 1);
+    const T* pi = P + i;
     for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
+        for (int q = tid; q < nw * 8; q += nt) {
+            const int j = q >> 3, r = q & 7;
+            ys[q] = r < nr ? Y[(int64_t)(cg + c0 + j) * nrhs + rb + r] : T(0);
+        }
+        __syncthreads();
         T acc[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) acc[r] = T(0);
-#pragma unroll 16
-        for (int j = 0; j < w; ++j) {
-            const T l = P[(int64_t)j * f + i];
+        for (int jb = 16 * wv; jb < nw; jb += 16 * NWV) {
+            T l[16];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] += l * y[(int64_t)j * nrhs + min(rb + r, nrhs - 1)];
+            for (int jj = 0; jj < 16; ++jj) l[jj] = pi[(int64_t)(c0 + min(jb + jj, nw - 1)) * f];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const T lv = jb + jj < nw ? l[jj] : T(0);
+                const T* yj = ys + min(jb + jj, nw - 1) * 8;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc[r] += lv * yj[r];
+            }
         }
-        if (r0 + lane < f)
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
+        for (int r = 0; r < 8; ++r) red[(wv * 64 + lane) * 8 + r] = acc[r];
+        __syncthreads();
+        if (wv == 0 && r0 + lane < r1) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                if (r >= nr) break;
+                T v = red[lane * 8 + r];
+#pragma unroll
+                for (int u = 1; u < NWV; ++u) v += red[(u * 64 + lane) * 8 + r];
+                if (below)
+                    us[(int64_t)(i - w) * nrhs + rb + r] += v;
+                else
+                    Y[(int64_t)(cg + i) * nrhs + rb + r] -= v;
+            }
+        }
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // backward
 
-// partial sums of L21^T x over the chunk of L21T_CHUNK rows of a task (s, col0, chunk): a wave
-// per CW = 64 / RW columns; lanes along the rows; RW right-hand sides per pass. The 64 sums
-// (CW columns x RW right-hand sides) are reduce-scattered across the lanes (recursive
-// halving: lane l ends with sum l). part[(poff + chunk w + j) nrhs + r].
-template <typename T, int RW>
-__global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const Task* __restrict__ tasks,
-                                                const int64_t* __restrict__ poff, const T* __restrict__ factor,
+// partial sums of L(rows below the step, step)^T x over the chunk of L21T_CHUNK rows of a task
+// (step, col0, chunk): one wave for the 8 columns from col0; lanes along the rows; 8
+// right-hand sides per pass. The 64 sums are reduce-scattered across the lanes (lane l: column
+// l / 8, right-hand side l % 8). part[(poff + chunk nw + j - c0) nrhs + r], nw = c1 - c0.
+template <typename T>
+__global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __restrict__ steps,
+                                                const Task* __restrict__ tasks, const T* __restrict__ factor,
                                                 const T* __restrict__ Y, T* __restrict__ part, int nrhs) {
-    constexpr int CW = 64 / RW;
-    const Task t = tasks[blockIdx.x / RW];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], chunk = t.b;
-    const int col0 = t.a + (blockIdx.x % RW) * CW;
-    if (col0 >= w) return;
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1, chunk = tk.b;
+    const int col0 = tk.a;
+    const int f = S.fnrows[s];
     const int lane = threadIdx.x;
     const T* P = factor + S.panel_off[s];
     const int32_t* rows = S.rowval + S.rowptr[s];
-    const int r0 = w + chunk * L21T_CHUNK, r1 = min(f, r0 + L21T_CHUNK);
-    T* out = part + (poff[s] + (int64_t)chunk * w) * nrhs;
-    for (int rb = 0; rb < nrhs; rb += RW) {
+    const int r0 = c1 + chunk * L21T_CHUNK, r1 = min(f, r0 + L21T_CHUNK);
+    T* out = part + (st.poff + (int64_t)chunk * (c1 - c0)) * nrhs;
+    for (int rb = 0; rb < nrhs; rb += 8) {
         T a[64];
 #pragma unroll
         for (int v = 0; v < 64; ++v) a[v] = T(0);
-#pragma unroll(RW >= 4 ? 2 : 1)
+#pragma unroll 2
         for (int i = r0 + lane; i < r1; i += 64) {
             const T* yr = Y + (int64_t)rows[i] * nrhs;
-            T x[RW];
+            T x[8];
 #pragma unroll
-            for (int r = 0; r < RW; ++r) x[r] = yr[min(rb + r, nrhs - 1)];
-            const T* pc = P + (int64_t)col0 * f + i;  // column col0 + c; columns past w read w - 1
+            for (int r = 0; r < 8; ++r) x[r] = yr[min(rb + r, nrhs - 1)];
+            const T* pc = P + (int64_t)col0 * f + i;  // column col0 + c; columns past c1 read c1 - 1
 #pragma unroll
-            for (int c = 0; c < CW; ++c) {
+            for (int c = 0; c < 8; ++c) {
                 const T l = *pc;
-                pc += col0 + c + 1 < w ? f : 0;
+                pc += col0 + c + 1 < c1 ? f : 0;
 #pragma unroll
-                for (int r = 0; r < RW; ++r) a[c * RW + r] += l * x[r];
+                for (int r = 0; r < 8; ++r) a[c * 8 + r] += l * x[r];
             }
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            const bool up = (lane & d) != 0;
-#pragma unroll
-            for (int v = 0; v < d; ++v) {
-                const T send = up ? a[v] : a[v + d];
-                const T keep = up ? a[v + d] : a[v];
-                a[v] = keep + __shfl_xor(send, d);
-            }
-        }
-        const int c = lane / RW, r = lane % RW;
-        if (col0 + c < w && rb + r < nrhs) out[(int64_t)(col0 + c) * nrhs + rb + r] = a[0];
+        const T v = wave_reduce_scatter64<T>(a);
+        const int c = lane >> 3, r = lane & 7;
+        if (col0 + c < c1 && rb + r < nrhs) out[(int64_t)(col0 + c - c0) * nrhs + rb + r] = v;
     }
 }
 
-// one workgroup per narrow front of the level: y_top -= L21^T x (partials in chunk order, or
-// from LDS-staged solution rows when fused), then x_top = Linv^T y_top.
-// LDS: 3 NB nrhs entries.
+// one workgroup per front step, the front's steps from the last: task (step, 0); wave v owns
+// diagonal block v of the step (lane = column, t in registers, less the L21^T x partials).
+// Stage s (from the last block): every wave first loads 64 entries into registers (wave s: its
+// Linv column; an earlier wave v: its column of the tile L_sv, rows contiguous per lane), then
+// wave s forms x_s = Linv_ss^T t_s and publishes it in LDS, and every earlier wave v
+// subtracts L_sv^T x_s from its t.
+// LDS: (waves + 1) 64 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_front(DevSym S, const int32_t* __restrict__ fronts,
-                                                  const int64_t* __restrict__ poff, const T* __restrict__ factor,
-                                                  const T* __restrict__ DB, T* __restrict__ Y,
-                                                  const T* __restrict__ part, int nrhs) {
-    T* ts = reinterpret_cast<T*>(sweep_lds);
-    T* ys = ts + NB * nrhs;
-    T* Xs = ys + NB * nrhs;
-    const int s = fronts[blockIdx.x];
-    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
-    const int c0 = S.super_ptr[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
+__global__ void __launch_bounds__(512) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                 const T* __restrict__ DB, T* __restrict__ Y,
+                                                 const T* __restrict__ part, int nrhs) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    T* yx = reinterpret_cast<T*>(sweep_lds);  // (nt / 64) blocks x 64 x 8: the blocks' solutions
+    T* tb = yx + (nt >> 6) * 64 * 8;          // 64 x 8: the block being solved
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1;
+    const int f = S.fnrows[s];
+    const int cg = S.super_ptr[s];
     const T* P = factor + S.panel_off[s];
-    const int32_t* rows = S.rowval + S.rowptr[s];
-    T* y = Y + (int64_t)c0 * nrhs;
-    for (int q = tid; q < w * nrhs; q += nt) ts[q] = y[q];
-    __syncthreads();
-    if (m > 0) {
-        const int64_t po = poff[s];
-        if (po >= 0) {
-            const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
-            const T* pp = part + po * nrhs;
-            for (int q = tid; q < w * nrhs; q += nt) {
-                T v = ts[q];
-                for (int ch = 0; ch < nch; ++ch) v -= pp[(int64_t)ch * w * nrhs + q];
-                ts[q] = v;
-            }
-        } else {
-            for (int i0 = w; i0 < f; i0 += 64) {
-                const int ni = min(64, f - i0);
-                for (int q = tid; q < ni * nrhs; q += nt) {
-                    const int ii = q / nrhs, r = q - ii * nrhs;
-                    Xs[q] = Y[(int64_t)rows[i0 + ii] * nrhs + r];
-                }
-                __syncthreads();
-                for (int q = tid; q < w * nrhs; q += nt) {
-                    const int j = q / nrhs, r = q - j * nrhs;
-                    const T* pj = P + (int64_t)j * f + i0;
-                    T acc = T(0);
-                    for (int ii = 0; ii < ni; ++ii) acc += pj[ii] * Xs[ii * nrhs + r];
-                    ts[q] -= acc;
-                }
-                __syncthreads();
-            }
-        }
-        __syncthreads();
-    }
-    block_gemv<T>(DB + (int64_t)c0 * NB, ts, ys, w, nrhs, 1);
-    __syncthreads();
-    for (int q = tid; q < w * nrhs; q += nt) y[q] = ys[q];
-}
-
-// one wave per diagonal block of the wide fronts, the blocks of a front from the last: task
-// (s, k, flag base, 0). t = y_k - L21^T x partials (chunk order) - sum_{j > k, descending}
-// L_jk^T x_j (after block j's flag; L_jk staged through LDS), x_k = Linv_kk^T t, flag set.
-// LDS: NB nrhs + NB (NB + 1) + NB 8 entries.
-template <typename T>
-__global__ void __launch_bounds__(64) k_bwd_sf(DevSym S, const Task* __restrict__ tasks,
-                                              const int64_t* __restrict__ poff, const T* __restrict__ factor,
-                                              const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ part,
-                                              int32_t* __restrict__ flags, int32_t* __restrict__ err, int nrhs) {
-    T* ts = reinterpret_cast<T*>(sweep_lds);  // NB x nrhs
-    T(*Lt)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(ts + NB * nrhs);
-    T* xj = ts + NB * nrhs + NB * (NB + 1);  // NB x 8
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, k = t.a;
-    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
-    const int c0 = S.super_ptr[s];
-    const int nb = min(NB, w - k);
-    const int lane = threadIdx.x;
-    const T* P = factor + S.panel_off[s];
-    T* y = Y + (int64_t)c0 * nrhs;
-    const int64_t po = poff[s];
-    const int nch = m > 0 && po >= 0 ? (m + L21T_CHUNK - 1) / L21T_CHUNK : 0;
-    for (int q = lane; q < nb * nrhs; q += 64) {
-        T v = y[(int64_t)k * nrhs + q];
-        for (int ch = 0; ch < nch; ++ch) v -= part[(po + (int64_t)ch * w + k) * nrhs + q];
-        ts[q] = v;
-    }
-    const int cl = min(lane, nb - 1);  // this lane's column (clamped; lanes >= nb unused)
-    const int kl = ((w - 1) / NB) * NB;
+    T* y = Y + (int64_t)cg * nrhs;
+    const int nw = c1 - c0, nblk = (nw + NB - 1) / NB;
+    const T* pp = part + st.poff * nrhs;
+    const bool active = wv < nblk;
+    const int kb = c0 + NB * wv;
+    const int nbw = active ? min(NB, c1 - kb) : 1;
+    const int kbs = active ? kb : c0;  // a valid block for the loads of an idle wave
+    const int col = kb + min(lane, nbw - 1);  // this lane's column (clamped)
     for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
         T acc[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] = T(0);
-        for (int j = kl; j > k; j -= NB) {
-            const int njb = min(NB, w - j);
-            const int ir = j + min(lane, njb - 1);
-            __syncthreads();
-#pragma unroll 16
-            for (int c = 0; c < NB; ++c) Lt[c][lane] = lane < njb ? P[(int64_t)(k + min(c, nb - 1)) * f + ir] : T(0);
-            wait_flag(flags + t.b + j / NB, err);
+        for (int r = 0; r < 8; ++r) {
+            T v = T(0);
+            if (active && r < nr) {
+                v = y[(int64_t)col * nrhs + rb + r];
+                for (int ch = 0; ch < st.nch; ++ch) v -= pp[((int64_t)ch * nw + col - c0) * nrhs + rb + r];
+            }
+            acc[r] = v;
+        }
+        for (int sb = nblk - 1; sb >= 0; --sb) {
+            const int ks = c0 + NB * sb, nbs = min(NB, c1 - ks);
+            // this stage's registers: the tile column L(block s, col) or the Linv column
+            const bool tile = active && wv < sb;
+            const T* base = tile ? P + (int64_t)col * f + ks
+                                 : DB + (int64_t)(cg + kbs) * NB + (int64_t)min(lane, nbw - 1) * NB;
+            const int lim = tile ? nbs : nbw;
+            T l[NB];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) xj[lane * 8 + r] = rb + r < nrhs ? y[(int64_t)ir * nrhs + rb + r] : T(0);
-            __syncthreads();
-#pragma unroll 8
-            for (int ii = 0; ii < NB; ++ii) {
-                const T l = Lt[cl][ii];
+            for (int i = 0; i < NB; ++i) {
+                const T v = base[min(i, lim - 1)];
+                l[i] = i < lim ? v : T(0);
+            }
+            if (wv == sb)
 #pragma unroll
-                for (int r = 0; r < 8; ++r) acc[r] += l * xj[ii * 8 + r];
+                for (int r = 0; r < 8; ++r) tb[lane * 8 + r] = acc[r];
+            __syncthreads();
+            if (wv == sb) {
+                T xv[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) xv[r] = T(0);
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) xv[r] += l[i] * tb[i * 8 + r];  // Linv(i, lane): zero for i < lane
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const T v = lane < nbs ? xv[r] : T(0);
+                    yx[(sb * NB + lane) * 8 + r] = v;
+                }
+            }
+            __syncthreads();
+            if (tile) {
+                const T* xsb = yx + sb * NB * 8;
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) acc[r] -= l[i] * xsb[i * 8 + r];
             }
         }
+        // the step's solution to global memory (a store inside the stage loop makes the
+        // compiler spill the register tiles)
+        for (int q = tid; q < nw * 8; q += nt)
+            if ((q & 7) < nr) y[(int64_t)(c0 + (q >> 3)) * nrhs + rb + (q & 7)] = yx[q];
         __syncthreads();
-        if (lane < nb)
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) ts[lane * nrhs + rb + r] -= acc[r];
     }
-    // Linv^T: Lt[c][i] = Linv(i, c)
-    const T* Li = DB + (int64_t)(c0 + k) * NB;
-    __syncthreads();
-#pragma unroll 16
-    for (int c = 0; c < NB; ++c) Lt[c][lane] = c < nb ? Li[(int64_t)c * NB + lane] : T(0);
-    __syncthreads();
-    for (int rb = 0; rb < nrhs; rb += 8) {
-        T acc[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] = T(0);
-        for (int ii = 0; ii < nb; ++ii) {
-            const T l = Lt[cl][ii];  // zero above the diagonal
-#pragma unroll
-            for (int r = 0; r < 8; ++r) acc[r] += l * ts[ii * nrhs + min(rb + r, nrhs - 1)];
-        }
-        if (lane < nb)
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (rb + r < nrhs) y[(int64_t)(k + lane) * nrhs + rb + r] = acc[r];
-    }
-    set_flag(flags + t.b + k / NB);
 }
 
 // ---------------------------------------------------------------------------
+// wide solve steps (more than NB columns): the inverse X of the step's diagonal block
 
-// dynamic LDS beyond 64 KiB has to be allowed per kernel (once per process and kernel)
-template <typename K>
-void allow_lds(K kernel, size_t bytes) {
-    if (bytes > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+// X = L(step, step)^-1 for a wide step, one workgroup per (step, column block kk): X_kk = Linv_kk
+// (from DB), then for the blocks below X_ik = -Linv_ii sum_{j = kk}^{i - 1} L_ij X_jk, 64 x 64 x 64
+// products through LDS (thread: a 4 x 4 block). X column-major, ld nw, at xoff.
+// task: (step, kk)
+template <typename T>
+__global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep* __restrict__ steps,
+                                                     const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                     const T* __restrict__ DB, T* __restrict__ XB) {
+    __shared__ T A[NB][NB + 1];
+    __shared__ T B[NB][NB + 1];
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1, nw = c1 - c0, nblk = (nw + NB - 1) / NB;
+    const int f = S.fnrows[s], cg = S.super_ptr[s];
+    const T* P = factor + S.panel_off[s];
+    T* X = XB + st.xoff;
+    const int kc = tk.a * NB, nbk = min(NB, nw - kc);
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int c = e / NB, r = e % NB;
+        if (c < nbk && r < nbk) X[(int64_t)(kc + c) * nw + kc + r] = DB[(int64_t)(cg + c0 + kc + c) * NB + r];
+    }
+    __syncthreads();
+    for (int ib = tk.a + 1; ib < nblk; ++ib) {
+        const int ic = ib * NB, nbi = min(NB, nw - ic);
+        T acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+        for (int jb = tk.a; jb < ib; ++jb) {
+            const int jc = jb * NB;  // a full block: jb < ib
+            for (int e = tid; e < NB * NB; e += 256) {
+                const int c = e / NB, r = e % NB;
+                A[r][c] = r < nbi ? P[(int64_t)(c0 + jc + c) * f + c0 + ic + r] : T(0);
+                B[r][c] = c < nbk ? X[(int64_t)(kc + c) * nw + jc + r] : T(0);
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int q = 0; q < NB; ++q) {
+                T a4[4], b4[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) a4[a] = A[4 * tr + a][q];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) b4[b] = B[q][4 * tc + b];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] += a4[a] * b4[b];
+            }
+            __syncthreads();
+        }
+        // X_ik = -Linv_ii S
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int c = e / NB, r = e % NB;
+            A[r][c] = r < nbi && c < nbi ? DB[(int64_t)(cg + c0 + ic + c) * NB + r] : T(0);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) B[4 * tr + a][4 * tc + b] = acc[a][b];
+        __syncthreads();
+        T out[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) out[a][b] = T(0);
+#pragma unroll 8
+        for (int q = 0; q < NB; ++q) {
+            T a4[4], b4[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) a4[a] = A[4 * tr + a][q];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) b4[b] = B[q][4 * tc + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) out[a][b] += a4[a] * b4[b];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = 4 * tr + a, c = 4 * tc + b;
+                if (r < nbi && c < nbk) X[(int64_t)(kc + c) * nw + ic + r] = -out[a][b];
+            }
+        __syncthreads();
+    }
 }
+
+// forward, wide steps: y_i = sum_{k <= i} X_ik t_k for 16 rows [ri0, ri0 + 16) of a step, one
+// workgroup of 256 threads per (step, ri0): thread (row, column group g of 16) takes the
+// 16-column batches g, g + 16, ... (a wave reads four 16-row column segments per load); the 16
+// partial sums of a row are added in group order. t = the step's rows of y, less the children's
+// update entries at the first step of a front (applied here to an LDS copy). The result goes
+// to Z (the other workgroups still read y); k_copy_rows moves it to y. task: (step, ri0).
+// LDS: SOLVE_STEP 8 + 16 16 8 entries.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __restrict__ steps,
+                                                const Task* __restrict__ tasks, const T* __restrict__ XB,
+                                                const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ U,
+                                                int nrhs) {
+    T* t = reinterpret_cast<T*>(sweep_lds);  // step rows x 8
+    T* red = t + SOLVE_STEP * 8;              // 16 groups x 16 rows x 8
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1, nw = c1 - c0;
+    const int cg = S.super_ptr[s];
+    const T* X = XB + st.xoff;
+    const int ri0 = tk.a, nbi = min(16, nw - ri0), ncols = ri0 + nbi;
+    const int tid = threadIdx.x, rl = tid & 15, g = tid >> 4;
+    const int r = ri0 + min(rl, nbi - 1);
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
+        for (int q = tid; q < ncols * 8; q += 256) {
+            const int i = q >> 3, rr = q & 7;
+            t[q] = rr < nr ? Y[(int64_t)(cg + c0 + i) * nrhs + rb + rr] : T(0);
+        }
+        __syncthreads();
+        if (c0 == 0) {
+            for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+                const int c = S.child_list[kc];
+                if (S.uv_off[c] < 0) continue;
+                const int mc = S.fnrows[c] - S.fncols[c];
+                const T* uc = U + S.uv_off[c] * nrhs;
+                const int32_t* ri = S.relind + S.relind_ptr[c];
+                const int hi = lower_bound_i32(ri, mc, ncols);
+                for (int q = tid; q < hi * 8; q += 256) {
+                    const int ii = q >> 3, rr = q & 7;
+                    if (rr < nr) t[ri[ii] * 8 + rr] -= uc[(int64_t)ii * nrhs + rb + rr];
+                }
+                __syncthreads();
+            }
+        }
+        T acc[8];
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) acc[rr] = T(0);
+        const T* xr = X + r;
+        for (int cb = 16 * g; cb < ncols; cb += 256) {
+            T l[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) l[jj] = xr[(int64_t)min(cb + jj, ncols - 1) * nw];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const T lv = cb + jj < ncols ? l[jj] : T(0);
+                const T* tj = t + min(cb + jj, ncols - 1) * 8;
+#pragma unroll
+                for (int rr = 0; rr < 8; ++rr) acc[rr] += lv * tj[rr];
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) red[(g * 16 + rl) * 8 + rr] = acc[rr];
+        __syncthreads();
+        if (tid < 16 * 8) {
+            const int row = tid >> 3, rr = tid & 7;
+            if (row < nbi && rr < nr) {
+                T v = red[row * 8 + rr];
+#pragma unroll
+                for (int u = 1; u < 16; ++u) v += red[(u * 16 + row) * 8 + rr];
+                Z[(int64_t)(cg + c0 + ri0 + row) * nrhs + rb + rr] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// backward, wide steps: x_j = sum_{i >= j} X_ij t_i for 8 columns [col0, col0 + 8) of a step,
+// one wave per (step, col0); t = the step's rows of y (k_sub_partials has subtracted the
+// L21^T x partial sums), rows >= col0 staged in LDS. Lanes along the rows of X (coalesced); the 64 sums are
+// reduce-scattered across the lanes. The result goes to Z; k_copy_rows moves it to y.
+// task: (step, col0). LDS: SOLVE_STEP 8 entries.
+template <typename T>
+__global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __restrict__ steps,
+                                               const Task* __restrict__ tasks, const T* __restrict__ XB,
+                                               const T* __restrict__ Y, T* __restrict__ Z, int nrhs) {
+    T* t = reinterpret_cast<T*>(sweep_lds);  // rows [col0, nw) of the step x 8
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int s = st.s, c0 = st.c0, c1 = st.c1, nw = c1 - c0;
+    const int cg = S.super_ptr[s];
+    const T* X = XB + st.xoff;
+    const int col0 = tk.a, cend = min(nw, col0 + 8);
+    const int lane = threadIdx.x;
+    for (int rb = 0; rb < nrhs; rb += 8) {
+        const int nr = min(8, nrhs - rb);
+        for (int q = lane; q < (nw - col0) * 8; q += 64) {
+            const int i = col0 + (q >> 3), rr = q & 7;
+            t[q] = rr < nr ? Y[(int64_t)(cg + c0 + i) * nrhs + rb + rr] : T(0);
+        }
+        __syncthreads();
+        T a[64];
+#pragma unroll
+        for (int v = 0; v < 64; ++v) a[v] = T(0);
+#pragma unroll 2
+        for (int i = col0 + lane; i < nw; i += 64) {
+            const T* ti = t + (i - col0) * 8;
+            T x[8];
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr) x[rr] = ti[rr];
+            const T* pc = X + (int64_t)col0 * nw + i;  // column col0 + c; columns past cend read cend - 1
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const T l = *pc;
+                pc += col0 + c + 1 < cend ? nw : 0;
+#pragma unroll
+                for (int rr = 0; rr < 8; ++rr) a[c * 8 + rr] += l * x[rr];
+            }
+        }
+        const T v = wave_reduce_scatter64<T>(a);
+        const int col = col0 + (lane >> 3), rr = lane & 7;
+        if (col < cend && rr < nr) Z[(int64_t)(cg + c0 + col) * nrhs + rb + rr] = v;
+        __syncthreads();
+    }
+}
+
+// y rows [r0, r0 + nr) of a wide step less the L21^T x partial sums of those rows, in chunk
+// order. task: (step, r0, rows)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sub_partials(DevSym S, const SolveStep* __restrict__ steps,
+                                                     const Task* __restrict__ tasks, T* __restrict__ Y,
+                                                     const T* __restrict__ part, int nrhs) {
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    if (st.nch == 0) return;
+    const int nw = st.c1 - st.c0;
+    T* y = Y + (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
+    const T* pp = part + (st.poff + tk.a) * nrhs;
+    for (int q = threadIdx.x; q < tk.b * nrhs; q += 256) {
+        T v = y[q];
+        for (int ch = 0; ch < st.nch; ++ch) v -= pp[(int64_t)ch * nw * nrhs + q];
+        y[q] = v;
+    }
+}
+
+// y rows [r0, r0 + nr) of a wide step from Z (the result of k_fwd_inv / k_bwd_inv).
+// task: (step, r0, rows)
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_rows(DevSym S, const SolveStep* __restrict__ steps,
+                                                  const Task* __restrict__ tasks, T* __restrict__ Y,
+                                                  const T* __restrict__ Z, int nrhs) {
+    const Task tk = tasks[blockIdx.x];
+    const SolveStep st = steps[tk.s];
+    const int64_t o = (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
+    for (int q = threadIdx.x; q < tk.b * nrhs; q += 256) Y[o + q] = Z[o + q];
+}
+
+// ---------------------------------------------------------------------------
 
 template <typename T>
 void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB) {
@@ -524,68 +733,84 @@ void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, in
 }
 
 template <typename T>
-void launch_fwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const T* factor,
-                      const T* DB, T* Y, T* U, int nrhs) {
-    const size_t lds = 2 * NB * nrhs * sizeof(T);
-    allow_lds(k_fwd_front<T>, lds);
-    hipLaunchKernelGGL(k_fwd_front<T>, dim3(n), dim3(threads), lds, st, S, fronts, factor, DB, Y, U, nrhs);
+void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
+                     const T* factor, const T* DB, T* Y, T* U, int nrhs) {
+    const size_t lds = (threads / 64 + 1) * 64 * 8 * sizeof(T);
+    hipLaunchKernelGGL(k_fwd_diag<T>, dim3(n), dim3(threads), lds, st, S, steps, tasks, factor, DB, Y, U, nrhs);
 }
 
 template <typename T>
-void launch_fwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
-                   const T* U, int32_t* flags, int32_t* err, int nrhs) {
-    hipLaunchKernelGGL(k_fwd_sf<T>, dim3(n), dim3(64), (NB * nrhs + NB * 8) * sizeof(T), st, S, tasks, factor, DB, Y,
-                       U, flags, err, nrhs);
+void launch_fwd_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                     const T* factor, T* Y, T* U, int nrhs) {
+    const size_t lds = (SOLVE_STEP * 8 + 8 * 64 * 8) * sizeof(T);
+    hipLaunchKernelGGL(k_fwd_rows<T>, dim3(n), dim3(512), lds, st, S, steps, tasks, factor, Y, U, nrhs);
 }
 
 template <typename T>
-void launch_fwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* U,
-                    int nrhs) {
-    hipLaunchKernelGGL(k_fwd_l21<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, Y, U, nrhs);
+void launch_bwd_l21t(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                     const T* factor, const T* Y, T* part, int nrhs) {
+    hipLaunchKernelGGL(k_bwd_l21t<T>, dim3(n), dim3(64), 0, st, S, steps, tasks, factor, Y, part, nrhs);
 }
 
 template <typename T>
-void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
-                     const T* Y, T* part, int nrhs) {
-    if (nrhs >= 8)
-        hipLaunchKernelGGL((k_bwd_l21t<T, 8>), dim3(n * 8), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
-    else if (nrhs >= 4)
-        hipLaunchKernelGGL((k_bwd_l21t<T, 4>), dim3(n * 4), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
-    else if (nrhs >= 2)
-        hipLaunchKernelGGL((k_bwd_l21t<T, 2>), dim3(n * 2), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
-    else
-        hipLaunchKernelGGL((k_bwd_l21t<T, 1>), dim3(n), dim3(64), 0, st, S, tasks, poff, factor, Y, part, nrhs);
+void launch_bwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
+                     const T* factor, const T* DB, T* Y, const T* part, int nrhs) {
+    const size_t lds = (threads / 64 + 1) * 64 * 8 * sizeof(T);
+    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(n), dim3(threads), lds, st, S, steps, tasks, factor, DB, Y, part, nrhs);
 }
 
 template <typename T>
-void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const int64_t* poff,
-                      const T* factor, const T* DB, T* Y, const T* part, int nrhs) {
-    const size_t lds = 3 * NB * nrhs * sizeof(T);
-    allow_lds(k_bwd_front<T>, lds);
-    hipLaunchKernelGGL(k_bwd_front<T>, dim3(n), dim3(threads), lds, st, S, fronts, poff, factor, DB, Y, part, nrhs);
+void launch_step_inverse(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                         const T* factor, const T* DB, T* XB) {
+    hipLaunchKernelGGL(k_step_inverse<T>, dim3(n), dim3(256), 0, st, S, steps, tasks, factor, DB, XB);
 }
 
 template <typename T>
-void launch_bwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
-                   const T* DB, T* Y, const T* part, int32_t* flags, int32_t* err, int nrhs) {
-    const size_t lds = (NB * nrhs + NB * (NB + 1) + NB * 8) * sizeof(T);
-    allow_lds(k_bwd_sf<T>, lds);
-    hipLaunchKernelGGL(k_bwd_sf<T>, dim3(n), dim3(64), lds, st, S, tasks, poff, factor, DB, Y, part, flags, err, nrhs);
+void launch_fwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
+                    const T* Y, T* Z, const T* U, int nrhs) {
+    const size_t lds = (SOLVE_STEP * 8 + 16 * 16 * 8) * sizeof(T);
+    hipLaunchKernelGGL(k_fwd_inv<T>, dim3(n), dim3(256), lds, st, S, steps, tasks, XB, Y, Z, U, nrhs);
+}
+
+template <typename T>
+void launch_bwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
+                    const T* Y, T* Z, int nrhs) {
+    const size_t lds = SOLVE_STEP * 8 * sizeof(T);
+    hipLaunchKernelGGL(k_bwd_inv<T>, dim3(n), dim3(64), lds, st, S, steps, tasks, XB, Y, Z, nrhs);
+}
+
+template <typename T>
+void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
+                         const T* part, int nrhs) {
+    hipLaunchKernelGGL(k_sub_partials<T>, dim3(n), dim3(256), 0, st, S, steps, tasks, Y, part, nrhs);
+}
+
+template <typename T>
+void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
+                      const T* Z, int nrhs) {
+    hipLaunchKernelGGL(k_copy_rows<T>, dim3(n), dim3(256), 0, st, S, steps, tasks, Y, Z, nrhs);
 }
 
 #define INST(T)                                                                                                     \
     template void launch_block_inverse<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                   \
-    template void launch_fwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const T*, const T*, T*, \
-                                      T*, int);                                                                    \
-    template void launch_fwd_sf<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, const T*,  \
-                                   int32_t*, int32_t*, int);                                                       \
-    template void launch_fwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);    \
-    template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const Task*, int, const int64_t*, const T*,       \
+    template void launch_fwd_diag<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, int, const T*, \
+                                     const T*, T*, T*, int);                                                       \
+    template void launch_fwd_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*, T*, \
+                                     T*, int);                                                                     \
+    template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,     \
                                      const T*, T*, int);                                                           \
-    template void launch_bwd_front<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const int64_t*,        \
-                                      const T*, const T*, T*, const T*, int);                                      \
-    template void launch_bwd_sf<T>(hipStream_t, const DevSym&, const Task*, int, const int64_t*, const T*,         \
-                                   const T*, T*, const T*, int32_t*, int32_t*, int);
+    template void launch_bwd_diag<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, int, const T*, \
+                                     const T*, T*, const T*, int);                                                \
+    template void launch_step_inverse<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int,          \
+                                         const T*, const T*, T*);                                                  \
+    template void launch_fwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
+                                    const T*, T*, const T*, int);                                                            \
+    template void launch_bwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
+                                    const T*, T*, int);                                                            \
+    template void launch_sub_partials<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, T*,      \
+                                         const T*, int);                                                           \
+    template void launch_copy_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, T*,         \
+                                      const T*, int);
 INST(double)
 INST(float)
 #undef INST

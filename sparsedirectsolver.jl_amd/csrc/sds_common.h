@@ -44,11 +44,21 @@ constexpr int TILE = 64;   // Schur-update tile
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
-// Cholesky sweeps with inverted diagonal blocks: narrow fronts (at most NB columns) with at most
-// FUSE_ROWS rows below their columns are solved by one workgroup each; the others get separate
-// L21 launches, with L21^T partial sums per L21T_CHUNK rows
+// Cholesky sweeps with inverted diagonal blocks: a front's columns are solved in steps of at
+// most SOLVE_STEP columns by one workgroup each; narrow fronts (at most NB columns) with at most
+// FUSE_ROWS rows below their columns form their update vector in the same workgroup, the others
+// in separate row-tile launches; L21^T partial sums are formed per L21T_CHUNK rows
+constexpr int SOLVE_STEP = 512;
 constexpr int FUSE_ROWS = 256;
 constexpr int L21T_CHUNK = 512;
+
+// one solve step: columns [c0, c1) of front s; L(rows below c1, step)^T x partial sums at poff
+// (units of nrhs), nch chunks of L21T_CHUNK rows; a wide step (more than NB columns) keeps the
+// inverse of its diagonal block at xoff (column-major, ld c1 - c0; -1: none)
+struct SolveStep {
+    int32_t s, c0, c1, nch;
+    int64_t poff, xoff;
+};
 
 // value of `v` in lane `lane` (wave-uniform) of the calling wave, as a register broadcast
 __device__ inline double lane_bcast(double v, int lane) {

@@ -74,15 +74,15 @@ struct Level {
     double small_flops = 0, small_bytes = 0;
     double fwd_bytes = 0, fwd_flops = 0;  // factor bytes read by a sweep, flops per right-hand side
     double rhs_rows = 0;                  // rows of the level's fronts (right-hand-side entries touched)
-    // Cholesky sweeps with inverted diagonal blocks: the narrow fronts (one workgroup each, of
-    // narrow_threads threads), the diagonal blocks of the wide fronts (sf_fwd, sf_bwd), and the
-    // L21 row tiles (inv_fwd) and L21^T partial sums (inv_bwd) of the wide and tall fronts
-    std::vector<int32_t> narrow;
-    int64_t off_narrow = 0;
-    int narrow_threads = 256;
-    Launch narrow_fwd{S_FWD_STEP, 0, 0, 0, 0}, narrow_bwd{S_BWD_STEP, 0, 0, 0, 0};
-    Launch sf_fwd{S_FWD_STEP, 0, 0, 0, 0}, sf_bwd{S_BWD_STEP, 0, 0, 0, 0};
-    Launch inv_fwd{S_FWD_STEP, 0, 0, 0, 0}, inv_bwd{S_BWD_L21, 0, 0, 0, 0};
+    // Cholesky sweeps with inverted diagonal blocks, per solve step g of the level's fronts:
+    // forward diagonal (one workgroup per front step, `threads` each) and row-tile launches,
+    // backward partial-sum and diagonal launches
+    struct StepGroup {
+        Launch fdiag, frows, bl21t, bdiag;  // narrow steps (one block), rows below, partial sums
+        Launch fwide, bwide, copy;           // wide steps: the inverse products, the copies of their rows
+        int threads;
+    };
+    std::vector<StepGroup> groups;
 };
 
 // plan launch kind -> profiling kind of include/sds_hip.h
@@ -127,10 +127,10 @@ struct sds_handle {
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
     Launch inv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the diagonal blocks (Cholesky)
-    int64_t* d_poff = nullptr;               // per front: partial-sum offset of its L21^T (units of nrhs)
+    SolveStep* d_steps = nullptr;            // solve steps of the Cholesky sweeps
+    Launch sinv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the wide steps' diagonal blocks
+    void* xbuf = nullptr;                    // those inverses
     int64_t part_units = 0;                  // partial-sum buffer, units of nrhs
-    int64_t nflags = 0;                      // diagonal-block flags of the wide fronts' sweeps
-    int32_t* d_flags = nullptr;              // nflags + 1 words: the flags, then the spin-timeout word
     DevBuf part;
     std::vector<DevBuf> symbufs;  // device maps
     DevSym dsym{};
@@ -645,7 +645,6 @@ int build_schedule(sds_handle* h, int dtype) {
         push_solve(L.bwd_plan, S_BWD_FINISH, t, 0, 0);
     }
     // Cholesky: the diagonal-block inverses and the inverse-block sweep plans
-    std::vector<int64_t> poff(ns, -1);
     h->part_units = 0;
     if (h->method == M_CHOLESKY) {
         std::vector<Task> t;
@@ -659,63 +658,79 @@ int build_schedule(sds_handle* h, int dtype) {
             }
         h->inv_launch = {L_POTRF, (int64_t)tasks.size(), (int)t.size(), fl, by};
         tasks.insert(tasks.end(), t.begin(), t.end());
-        int64_t units = 0, nfl = 0;
+        std::vector<SolveStep> steps;
+        std::vector<Task> ts;  // (step, column block) of the wide steps' inverses
+        int64_t xlen = 0;
+        double sfl = 0, sby = 0;
         auto put = [&](std::vector<Task>& v, int kind, double fl2, double by2) {
             Launch ln{kind, (int64_t)tasks.size(), (int)v.size(), fl2, by2};
             tasks.insert(tasks.end(), v.begin(), v.end());
             return ln;
         };
         for (auto& L : h->levels) {
-            std::vector<Task> sff, sfb, tf, tb;
-            double nfl_ = 0, nby = 0, sfl = 0, sby = 0, ffl = 0, fby = 0;
-            int fmax = 0;
-            units = 0;
-            for (int32_t s : L.all) {
-                const int f = h->fnrows[s], w = h->fncols[s], m = f - w;
-                // factor entries a sweep reads: L11 lower triangle + L21, and the inverse blocks
-                const double e11 = (double)w * (w + 1) / 2 + (double)NB * w, e21 = (double)m * w;
-                const bool narrow = w <= NB;
-                const bool fused = narrow && m <= FUSE_ROWS;
-                if (narrow) {
-                    L.narrow.push_back(s);
-                    fmax = std::max(fmax, f);
-                    nfl_ += 2.0 * (e11 + (fused ? e21 : 0));
-                    nby += dbl_es * (e11 + (fused ? e21 : 0));
-                } else {
-                    for (int k = 0; k < w; k += NB) sff.push_back({s, k, (int32_t)nfl, 0});
-                    for (int k = ((w - 1) / NB) * NB; k >= 0; k -= NB) sfb.push_back({s, k, (int32_t)nfl, 0});
-                    nfl += (w + NB - 1) / NB;
-                    sfl += 2.0 * e11;
-                    sby += dbl_es * e11;
+            int wmax = 0;
+            for (int32_t s : L.all) wmax = std::max(wmax, h->fncols[s]);
+            for (int g = 0; g * SOLVE_STEP < wmax; ++g) {
+                std::vector<Task> td, tr, tt, tfw, tbw, tcp;
+                double e11s = 0, efused = 0, erows = 0;  // factor entries read (see below)
+                int threads = 64;  // narrow steps: one diagonal block (more threads for fused rows)
+                int64_t units = 0;
+                for (int32_t s : L.all) {
+                    const int f = h->fnrows[s], w = h->fncols[s], m = f - w;
+                    const int c0 = g * SOLVE_STEP, c1 = std::min(w, c0 + SOLVE_STEP);
+                    if (c0 >= w) continue;
+                    const int nw = c1 - c0;
+                    const bool fused = w <= NB && m <= FUSE_ROWS;
+                    const int mb = f - c1;  // rows below the step
+                    const int nch = mb > 0 ? (mb + L21T_CHUNK - 1) / L21T_CHUNK : 0;
+                    const int32_t id = (int32_t)steps.size();
+                    const bool wide = nw > NB;
+                    steps.push_back({s, c0, c1, nch, units, wide ? xlen : -1});
+                    units += (int64_t)nch * nw;
+                    if (wide) {
+                        for (int b = 0; b * NB < nw; ++b) {
+                            ts.push_back({id, b, 0, 0});
+                            tcp.push_back({id, b * NB, std::min(NB, nw - b * NB), 0});
+                        }
+                        for (int r = 0; r < nw; r += 16) tfw.push_back({id, r, 0, 0});
+                        for (int c = 0; c < nw; c += 8) tbw.push_back({id, c, 0, 0});
+                        xlen += (int64_t)nw * nw;
+                        const double w3 = (double)nw * nw * nw;
+                        sfl += w3 / 3.0;
+                        sby += dbl_es * ((double)nw * nw / 2 + (double)nw * nw);
+                    } else {
+                        td.push_back({id, fused ? 1 : 0, 0, 0});
+                        if (fused && m > 0) threads = 256;
+                    }
+                    // factor entries read: the step's triangle and inverse blocks, the rows below
+                    e11s += (double)nw * (nw + 1) / 2 + (double)NB * nw;
+                    (fused ? efused : erows) += (double)mb * nw;
+                    if (!fused) {
+                        for (int r0 = c1; r0 < w; r0 += 64) tr.push_back({id, r0, 0, 0});
+                        if (m > 0 && uv_off[s] >= 0)
+                            for (int r0 = w; r0 < f; r0 += 64) tr.push_back({id, r0, 0, 0});
+                    }
+                    for (int c = c0; c < c1 && nch; c += 8)
+                        for (int ch = 0; ch < nch; ++ch) tt.push_back({id, c, ch, 0});
                 }
-                if (!fused && m > 0 && uv_off[s] >= 0) {
-                    for (int r0 = w; r0 < f; r0 += 64) tf.push_back({s, r0, 0, 0});
-                    poff[s] = units;
-                    const int nch = (m + L21T_CHUNK - 1) / L21T_CHUNK;
-                    for (int c = 0; c < w; c += 64)
-                        for (int ch = 0; ch < nch; ++ch) tb.push_back({s, c, ch, 0});
-                    units += (int64_t)nch * w;
-                    ffl += 2.0 * e21;
-                    fby += dbl_es * e21;
-                }
+                h->part_units = std::max(h->part_units, units);
+                Level::StepGroup G{};
+                G.threads = threads;
+                // flops per right-hand side and factor bytes of each launch
+                G.fdiag = put(td, S_FWD_STEP, 2.0 * (e11s + efused), dbl_es * (e11s + efused));
+                G.frows = put(tr, S_FWD_STEP, 2.0 * erows, dbl_es * erows);
+                G.bl21t = put(tt, S_BWD_L21, 2.0 * (efused + erows), dbl_es * (efused + erows));
+                G.bdiag = put(td, S_BWD_STEP, 2.0 * e11s, dbl_es * e11s);
+                G.fwide = put(tfw, S_FWD_STEP, 0, 0);
+                G.bwide = put(tbw, S_BWD_STEP, 0, 0);
+                G.copy = put(tcp, S_FWD_FINISH, 0, 0);
+                L.groups.push_back(G);
             }
-            L.narrow_threads = fmax <= 64 ? 64 : 256;
-            L.narrow_fwd = {S_FWD_STEP, 0, (int)L.narrow.size(), nfl_, nby};
-            L.narrow_bwd = {S_BWD_STEP, 0, (int)L.narrow.size(), nfl_, nby};
-            h->part_units = std::max(h->part_units, units);
-            if (!sff.empty()) L.sf_fwd = put(sff, S_FWD_STEP, sfl, sby);
-            if (!sfb.empty()) L.sf_bwd = put(sfb, S_BWD_STEP, sfl, sby);
-            if (!tf.empty()) L.inv_fwd = put(tf, S_FWD_STEP, ffl, fby);
-            if (!tb.empty()) L.inv_bwd = put(tb, S_BWD_L21, ffl, fby);
         }
-        h->nflags = nfl;
-        for (auto& L : h->levels) {
-            L.off_narrow = (int64_t)flist.size();
-            flist.insert(flist.end(), L.narrow.begin(), L.narrow.end());
-        }
-        if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (nfl + 1), (void**)&h->d_flags))) return rc;
+        if ((rc = upload(h, h->numbufs, steps, &h->d_steps))) return rc;
+        h->sinv_launch = put(ts, L_POTRF, sfl, sby);
+        if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(xlen, 1), &h->xbuf))) return rc;
     }
-    if ((rc = upload(h, h->numbufs, poff, &h->d_poff))) return rc;
     if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
     if ((rc = upload(h, h->numbufs, tasks, &h->d_tasks))) return rc;
     if ((rc = upload(h, h->numbufs, cb_off, &h->d_cb_off))) return rc;
@@ -763,6 +778,11 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
         timed(h, SDS_K_INVERSE, h->inv_launch.flops, h->inv_launch.bytes, [&] {
             launch_block_inverse<T>(st, h->dsym, h->d_tasks + h->inv_launch.off, h->inv_launch.n, factor,
                                     static_cast<T*>(h->dbuf));
+        });
+    if (h->sinv_launch.n)
+        timed(h, SDS_K_INVERSE, h->sinv_launch.flops, h->sinv_launch.bytes, [&] {
+            launch_step_inverse<T>(st, h->dsym, h->d_steps, h->d_tasks + h->sinv_launch.off, h->sinv_launch.n, factor,
+                                   static_cast<const T*>(h->dbuf), static_cast<T*>(h->xbuf));
         });
     HIPCHK(h, hipGetLastError());
     return SDS_SUCCESS;
@@ -900,48 +920,54 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
         // inverted diagonal blocks (DB) and level launches over all fronts
         const T* Li = static_cast<const T*>(h->dbuf);
         T* part = static_cast<T*>(h->part.p);
-        int32_t* flags = h->d_flags;
-        int32_t* err = h->d_flags + h->nflags;
-        const double rb = es * nrhs;  // right-hand-side bytes per row
-        if (fwd) {
-            if (h->nflags) launch_fill_i32(st, flags, h->nflags + 1, 0);
-            for (auto& Lv : h->levels) {
-                if (!Lv.narrow.empty())
-                    timed(h, SDS_K_FWD, Lv.narrow_fwd.flops * nrhs, Lv.narrow_fwd.bytes, [&] {
-                        launch_fwd_front<T>(st, h->dsym, h->d_fronts + Lv.off_narrow, (int)Lv.narrow.size(),
-                                            Lv.narrow_threads, L, Li, Y, U, nrhs);
-                    });
-                if (Lv.sf_fwd.n)
-                    timed(h, SDS_K_FWD, Lv.sf_fwd.flops * nrhs, Lv.sf_fwd.bytes, [&] {
-                        launch_fwd_sf<T>(st, h->dsym, h->d_tasks + Lv.sf_fwd.off, Lv.sf_fwd.n, L, Li, Y, U, flags, err,
-                                         nrhs);
-                    });
-                if (Lv.inv_fwd.n)
-                    timed(h, SDS_K_FWD, Lv.inv_fwd.flops * nrhs, Lv.inv_fwd.bytes + rb * 64.0 * Lv.inv_fwd.n, [&] {
-                        launch_fwd_l21<T>(st, h->dsym, h->d_tasks + Lv.inv_fwd.off, Lv.inv_fwd.n, L, Y, U, nrhs);
-                    });
-            }
-        }
-        if (bwd) {
-            if (h->nflags) launch_fill_i32(st, flags, h->nflags + 1, 0);
-            for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it) {
-                if (it->inv_bwd.n)
-                    timed(h, SDS_K_BWD, it->inv_bwd.flops * nrhs, it->inv_bwd.bytes, [&] {
-                        launch_bwd_l21t<T>(st, h->dsym, h->d_tasks + it->inv_bwd.off, it->inv_bwd.n, h->d_poff, L, Y,
-                                           part, nrhs);
-                    });
-                if (!it->narrow.empty())
-                    timed(h, SDS_K_BWD, it->narrow_bwd.flops * nrhs, it->narrow_bwd.bytes, [&] {
-                        launch_bwd_front<T>(st, h->dsym, h->d_fronts + it->off_narrow, (int)it->narrow.size(),
-                                            it->narrow_threads, h->d_poff, L, Li, Y, part, nrhs);
-                    });
-                if (it->sf_bwd.n)
-                    timed(h, SDS_K_BWD, it->sf_bwd.flops * nrhs, it->sf_bwd.bytes, [&] {
-                        launch_bwd_sf<T>(st, h->dsym, h->d_tasks + it->sf_bwd.off, it->sf_bwd.n, h->d_poff, L, Li, Y,
-                                         part, flags, err, nrhs);
-                    });
-            }
-        }
+        const SolveStep* stp = h->d_steps;
+        const T* Xb = static_cast<const T*>(h->xbuf);
+        const Task* tk = h->d_tasks;
+        if (fwd)
+            for (auto& Lv : h->levels)
+                for (auto& G : Lv.groups) {
+                    if (G.fdiag.n)
+                        timed(h, SDS_K_FWD, G.fdiag.flops * nrhs, G.fdiag.bytes, [&] {
+                            launch_fwd_diag<T>(st, h->dsym, stp, tk + G.fdiag.off, G.fdiag.n, G.threads, L, Li, Y, U,
+                                               nrhs);
+                        });
+                    if (G.fwide.n) {
+                        timed(h, SDS_K_FWD, 0, 0, [&] {
+                            launch_fwd_inv<T>(st, h->dsym, stp, tk + G.fwide.off, G.fwide.n, Xb, Y, Z, U, nrhs);
+                        });
+                        timed(h, SDS_K_FWD, 0, 0, [&] {
+                            launch_copy_rows<T>(st, h->dsym, stp, tk + G.copy.off, G.copy.n, Y, Z, nrhs);
+                        });
+                    }
+                    if (G.frows.n)
+                        timed(h, SDS_K_FWD, G.frows.flops * nrhs, G.frows.bytes, [&] {
+                            launch_fwd_rows<T>(st, h->dsym, stp, tk + G.frows.off, G.frows.n, L, Y, U, nrhs);
+                        });
+                }
+        if (bwd)
+            for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it)
+                for (auto G = it->groups.rbegin(); G != it->groups.rend(); ++G) {
+                    if (G->bl21t.n)
+                        timed(h, SDS_K_BWD, G->bl21t.flops * nrhs, G->bl21t.bytes, [&] {
+                            launch_bwd_l21t<T>(st, h->dsym, stp, tk + G->bl21t.off, G->bl21t.n, L, Y, part, nrhs);
+                        });
+                    if (G->bdiag.n)
+                        timed(h, SDS_K_BWD, G->bdiag.flops * nrhs, G->bdiag.bytes, [&] {
+                            launch_bwd_diag<T>(st, h->dsym, stp, tk + G->bdiag.off, G->bdiag.n, 64, L, Li, Y, part,
+                                               nrhs);
+                        });
+                    if (G->bwide.n) {
+                        timed(h, SDS_K_BWD, 0, 0, [&] {
+                            launch_sub_partials<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, part, nrhs);
+                        });
+                        timed(h, SDS_K_BWD, 0, 0, [&] {
+                            launch_bwd_inv<T>(st, h->dsym, stp, tk + G->bwide.off, G->bwide.n, Xb, Y, Z, nrhs);
+                        });
+                        timed(h, SDS_K_BWD, 0, 0, [&] {
+                            launch_copy_rows<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, Z, nrhs);
+                        });
+                    }
+                }
         return;
     }
     auto run = [&](const Launch& ln, int kind) {

@@ -89,23 +89,32 @@ void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 template <typename T>
 void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB);
 template <typename T>
-void launch_fwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const T* factor,
-                      const T* DB, T* Y, T* U, int nrhs);
+void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
+                     const T* factor, const T* DB, T* Y, T* U, int nrhs);
 template <typename T>
-void launch_fwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* DB, T* Y,
-                   const T* U, int32_t* flags, int32_t* err, int nrhs);
+void launch_fwd_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                     const T* factor, T* Y, T* U, int nrhs);
 template <typename T>
-void launch_fwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* U,
-                    int nrhs);
+void launch_bwd_l21t(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                     const T* factor, const T* Y, T* part, int nrhs);
 template <typename T>
-void launch_bwd_l21t(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
-                     const T* Y, T* part, int nrhs);
+void launch_bwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
+                     const T* factor, const T* DB, T* Y, const T* part, int nrhs);
 template <typename T>
-void launch_bwd_front(hipStream_t st, const DevSym& S, const int32_t* fronts, int n, int threads, const int64_t* poff,
-                      const T* factor, const T* DB, T* Y, const T* part, int nrhs);
+void launch_step_inverse(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                         const T* factor, const T* DB, T* XB);
 template <typename T>
-void launch_bwd_sf(hipStream_t st, const DevSym& S, const Task* tasks, int n, const int64_t* poff, const T* factor,
-                   const T* DB, T* Y, const T* part, int32_t* flags, int32_t* err, int nrhs);
+void launch_fwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
+                    const T* Y, T* Z, const T* U, int nrhs);
+template <typename T>
+void launch_bwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
+                    const T* Y, T* Z, int nrhs);
+template <typename T>
+void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
+                         const T* part, int nrhs);
+template <typename T>
+void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
+                      const T* Z, int nrhs);
 void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
                             const int8_t* aflag, const double* lam, int64_t ldl, const double* X, int64_t ldx,
                             double* grad);
