@@ -76,6 +76,27 @@ def time_steps(s, X, B, steps, warmup, ir_steps):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def time_graph(s, X, B, steps, warmup):
+    """Refactorization + solve (with the solver's refinement steps) captured once into a HIP
+    graph and replayed: ms per replay, inputs resident (B is copied into the graph's buffer
+    before the timed replays, X out after them)."""
+    import torch
+    from sparsedirectsolver.solver import graph_capture, graph_replay
+    graph_capture(s, B.shape[1], "solve")
+    graph_replay(s, None, B)
+    for _ in range(warmup):
+        graph_replay(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph_replay(s)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    graph_replay(s, X)
+    torch.cuda.synchronize()
+    return ms
+
+
 def time_phase(s, steps, fn):
     import torch
     torch.cuda.synchronize()
@@ -235,10 +256,15 @@ def main():
     S = s64.symbolic
     t_step64 = time_steps(s64, X, B, args.steps, args.warmup, 0)
     berr64 = backward_error(A, X.cpu().numpy(), Bh)
-    # profiled pass over the same steps: per-kernel-class HIP events on the solver's stream
-    profile(s64, True)
+    t_graph64 = time_graph(s64, X, B, args.steps, args.warmup)
+    berr64g = backward_error(A, X.cpu().numpy(), Bh)
     t_refact64 = time_phase(s64, args.steps, lambda: sds.execute("refactorization", s64))
     t_solve64 = time_phase(s64, args.steps, lambda: sds.execute("solve", s64, X, B))
+    # profiled pass over the same steps: per-kernel-class HIP events on the solver's stream
+    # (the events serialize the launches, so these phase times run longer than the ones above)
+    profile(s64, True)
+    time_phase(s64, args.steps, lambda: sds.execute("refactorization", s64))
+    time_phase(s64, args.steps, lambda: sds.execute("solve", s64, X, B))
     prof = profile_read(s64)
     profile(s64, False)
     flops_true = S.flops_true
@@ -255,6 +281,7 @@ def main():
         "fp64": {"refactor_ms": round(t_refact64, 3), "solve_ms": round(t_solve64, 3),
                  "factor_tflops": round(flops_true / (t_refact64 * 1e-3) / 1e12, 3),
                  "backward_error": berr64},
+        "graph": {"fp64_ms_per_step": round(t_graph64, 3), "fp64_backward_error": berr64g},
     }
     # dominant kernel class of the profiled region and its roofline
     dom = max(prof, key=lambda k: prof[k][0])
@@ -285,6 +312,10 @@ def main():
             t_step32 = time_steps(s32, X, B, args.steps, args.warmup, steps_needed)
             berr32 = backward_error(A, X.cpu().numpy(), Bh)
             t_refact32 = time_phase(s32, args.steps, lambda: sds.execute("refactorization", s32))
+            t_graph32 = time_graph(s32, X, B, args.steps, args.warmup)
+            res["graph"].update({"fp32_ir_ms_per_step": round(t_graph32, 3),
+                                 "fp32_ir_backward_error": backward_error(A, X.cpu().numpy(), Bh),
+                                 "fp32_ir_speedup_vs_fp64": round(t_graph64 / t_graph32, 3)})
             res["fp32_ir"] = {"ms_per_step": round(t_step32, 3), "ir_steps": steps_needed,
                               "refactor_ms": round(t_refact32, 3),
                               "factor_tflops": round(flops_true / (t_refact32 * 1e-3) / 1e12, 3),

@@ -2,26 +2,28 @@
 //
 // After a Cholesky factorization one launch inverts every NB x NB diagonal block of every
 // front (Linv = L_bb^-1, lower triangular, kept in the DB buffer: NB entries per factor column,
-// column-major with leading dimension NB). The sweeps then need no serial substitution: a
-// front's fully summed part is a short sequence of NB-row GEMVs done by one workgroup, and the
-// tall L21 products of the big fronts are separate, fully parallel launches. This is the
-// partitioned-inverse form of the reference's solve_alg = "algo1" (src/solver.jl:834,
-// src/solve/fused.jl) applied per diagonal block.
+// column-major with leading dimension NB), and a second one forms the inverses of the wider
+// solve steps' diagonal blocks from them. The sweeps then need no serial substitution: every
+// triangular solve is a GEMV with an inverse, and every launch of a level is fully parallel
+// across rows or columns. This is the partitioned-inverse form of the reference's
+// solve_alg = "algo1" (src/solver.jl:834, src/solve/fused.jl). Every entry is produced by
+// one thread or one fixed reduction tree: no atomics, and the result is bitwise reproducible.
 //
 // A front's fully summed columns are cut into solve steps of at most SOLVE_STEP columns
-// [c0, c1) (one step unless the front is wider). Per tree level and step:
-//   forward : k_fwd_diag, one workgroup per front: (first step) pull the children's update
-//             vectors; then block by block y_k = Linv_kk t_k and the step's rows below the block
-//             get t_i -= L_ik y_k, one row per thread, the step's right-hand side in LDS. Narrow
-//             fronts with at most FUSE_ROWS rows below their columns form their update vector
-//             here too; for the others k_fwd_rows adds L(rows, step) y_step to the rows below
-//             the step (64 per wave; the update vector rows are zeroed and get the children's
-//             entries of their rows first).
+// [c0, c1) (one step unless the front is wider). A narrow step (at most NB columns) uses the
+// inverse Linv of its single diagonal block; a wide step uses X, the inverse of its whole
+// diagonal block, formed after the factorization by k_step_inverse. Per tree level and step:
+//   forward : pull the children's update vectors and form y_step = inverse * t_step:
+//             k_fwd_diag (narrow, one workgroup per front; a narrow front with at most FUSE_ROWS
+//             rows below its columns forms its update vector there too) or k_fwd_inv (wide,
+//             16 rows per workgroup, into Z, then k_copy_rows); then k_fwd_rows adds
+//             L(rows, step) y_step to the rows below the step, 64 per workgroup (the update
+//             vector rows are zeroed and get the children's entries of their rows first).
 //   backward: k_bwd_l21t, partial sums of L(rows below the step, step)^T x per chunk of
-//             L21T_CHUNK rows (one wave per group of columns, lanes along the rows, a
-//             reduce-scatter across the lanes at the end); k_bwd_diag, one workgroup per front:
-//             t = y_step - partials, then block by block from the last x_k = Linv_kk^T t_k and
-//             the step's columns left of the block get t_j -= L_kj^T x_k, one column per thread.
+//             L21T_CHUNK rows (one wave per 8 columns, lanes along the rows, a reduce-scatter
+//             across the lanes at the end); then x_step = inverse^T (y_step - partials):
+//             k_bwd_diag (narrow) or k_sub_partials + k_bwd_inv (wide, one wave per 8 columns,
+//             into Z) + k_copy_rows.
 #include "sds_common.h"
 #include "sds_kernels.h"
 
@@ -127,116 +129,68 @@ __device__ __attribute__((always_inline)) inline T wave_reduce_scatter64(T (&a)[
 // ---------------------------------------------------------------------------
 // forward
 
-// one workgroup per front step: task (step, fused); wave v owns diagonal block v of the step
-// (64 rows, lane = row, its right-hand side t in registers). Stage s: every wave first loads
-// 64 entries into registers (wave s: its Linv row; a later wave v: its row of the tile L_vs,
-// coalesced across the lanes), then wave s forms y_s = Linv_ss t_s and publishes it in LDS,
-// and every later wave v subtracts L_vs y_s from its t. A fused front (one block, at most
-// FUSE_ROWS rows below) also forms its update vector u_i = L(i, front) y, one row per thread.
-// LDS: (waves + 1) 64 8 entries.
+// one workgroup per narrow front (one solve step of at most NB columns): task (step, fused).
+// Pull the children's update vectors; y = Linv t (entry (j, r) per thread, Linv rows read
+// coalesced, t in LDS); a fused front (at most FUSE_ROWS rows below) also forms its update
+// vector u_i = L(i, front) y, one row per thread. LDS: 2 NB 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(512) k_fwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+__global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __restrict__ steps,
                                                  const Task* __restrict__ tasks, const T* __restrict__ factor,
                                                  const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U,
                                                  int nrhs) {
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    T* yx = reinterpret_cast<T*>(sweep_lds);  // (nt / 64) blocks x 64 rows x 8: the blocks' solutions
-    T* tb = yx + (nt >> 6) * 64 * 8;          // 64 x 8: the block being solved
+    T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8: t
+    T* yx = tb + NB * 8;                       // NB x 8: y
+    const int tid = threadIdx.x, nt = blockDim.x;
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
-    const int s = st.s, c0 = st.c0, c1 = st.c1;
+    const int s = st.s;
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int cg = S.super_ptr[s];
     const T* P = factor + S.panel_off[s];
-    const bool fused = tk.a != 0;  // single step and single block; rows below the front done here
+    const bool fused = tk.a != 0;
     T* us = fused && m > 0 && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
-    if (c0 == 0) {
-        if (us)
-            for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
-        __syncthreads();
-        // the children's update vectors, in child_list order (only the rows < w unless fused)
-        for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-            const int c = S.child_list[kc];
-            if (S.uv_off[c] < 0) continue;
-            const int mc = S.fnrows[c] - S.fncols[c];
-            const T* uc = U + S.uv_off[c] * nrhs;
-            const int32_t* ri = S.relind + S.relind_ptr[c];
-            const int hi = us ? mc : lower_bound_i32(ri, mc, w);
-            for (int q = tid; q < hi * nrhs; q += nt) {
-                const int ii = q / nrhs, r = q - ii * nrhs;
-                const int pos = ri[ii];
-                if (pos < w)
-                    Y[(int64_t)(cg + pos) * nrhs + r] -= uc[q];
-                else
-                    us[(pos - w) * nrhs + r] += uc[q];
-            }
-            __syncthreads();
+    if (us)
+        for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
+    __syncthreads();
+    // the children's update vectors, in child_list order (only the rows < w unless fused)
+    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+        const int c = S.child_list[kc];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        const int hi = us ? mc : lower_bound_i32(ri, mc, w);
+        for (int q = tid; q < hi * nrhs; q += nt) {
+            const int ii = q / nrhs, r = q - ii * nrhs;
+            const int pos = ri[ii];
+            if (pos < w)
+                Y[(int64_t)(cg + pos) * nrhs + r] -= uc[q];
+            else
+                us[(pos - w) * nrhs + r] += uc[q];
         }
+        __syncthreads();
     }
-    const int nw = c1 - c0, nblk = (nw + NB - 1) / NB;
-    const bool active = wv < nblk;
-    const int kb = c0 + NB * wv;  // this wave's block
-    const int nbw = active ? min(NB, c1 - kb) : 1;
-    const int kbs = active ? kb : c0;  // a valid block for the loads of an idle wave
-    const int row = kb + min(lane, nbw - 1);  // this lane's row (clamped)
     T* y = Y + (int64_t)cg * nrhs;
+    const T* Li = DB + (int64_t)cg * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
-        T acc[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r] = active && r < nr ? y[(int64_t)row * nrhs + rb + r] : T(0);
-        for (int sb = 0; sb < nblk; ++sb) {
-            const int ks = c0 + NB * sb, nbs = min(NB, c1 - ks);
-            // this stage's registers: the tile row L(row, block s) (unconditional loads with a
-            // chosen base: a conditional reload of a register array makes the compiler spill)
-            const bool tile = active && wv > sb;
-            const T* base = tile ? P + (int64_t)ks * f + row : DB + (int64_t)(cg + kbs) * NB + min(lane, nbw - 1);
-            const int64_t stride = tile ? f : NB;
-            const int lim = tile ? NB : nbw;
-            T l[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                const T v = base[min(c, lim - 1) * stride];
-                l[c] = c < lim ? v : T(0);
-            }
-            if (wv == sb)
-#pragma unroll
-                for (int r = 0; r < 8; ++r) tb[lane * 8 + r] = acc[r];
-            __syncthreads();
-            if (wv == sb) {
-                T yv[8];
-#pragma unroll
-                for (int r = 0; r < 8; ++r) yv[r] = T(0);
-#pragma unroll
-                for (int i = 0; i < NB; ++i)
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) yv[r] += l[i] * tb[i * 8 + r];  // Linv(lane, i): zero for i > lane
-#pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const T v = lane < nbs ? yv[r] : T(0);
-                    yx[(sb * NB + lane) * 8 + r] = v;
-                }
-            }
-            __syncthreads();
-            if (tile) {
-                const T* ysb = yx + sb * NB * 8;
-#pragma unroll
-                for (int c = 0; c < NB; ++c)
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) acc[r] -= l[c] * ysb[c * 8 + r];
-            }
+        for (int q = tid; q < w * 8; q += nt) tb[q] = (q & 7) < nr ? y[(q >> 3) * nrhs + rb + (q & 7)] : T(0);
+        __syncthreads();
+        for (int q = tid; q < w * 8; q += nt) {
+            const int j = q >> 3, r = q & 7;
+            T acc = T(0);
+#pragma unroll 8
+            for (int i = 0; i <= j; ++i) acc += Li[(int64_t)i * NB + j] * tb[i * 8 + r];
+            yx[q] = acc;
+            if (r < nr) y[j * nrhs + rb + r] = acc;
         }
-        // the step's solution to global memory (a store inside the stage loop makes the
-        // compiler spill the register tiles)
-        for (int q = tid; q < nw * 8; q += nt)
-            if ((q & 7) < nr) y[(int64_t)(c0 + (q >> 3)) * nrhs + rb + (q & 7)] = yx[q];
-        if (us) {
-            // update vector rows, one per thread: u_i += L(i, 0:w) y
+        __syncthreads();
+        if (us)
             for (int i = w + tid; i < f; i += nt) {
                 T a[8];
 #pragma unroll
                 for (int r = 0; r < 8; ++r) a[r] = T(0);
-#pragma unroll 16
+#pragma unroll 8
                 for (int j = 0; j < w; ++j) {
                     const T lv = P[(int64_t)j * f + i];
 #pragma unroll
@@ -246,7 +200,6 @@ __global__ void __launch_bounds__(512) k_fwd_diag(DevSym S, const SolveStep* __r
                 for (int r = 0; r < 8; ++r)
                     if (r < nr) us[(i - w) * nrhs + rb + r] += a[r];
             }
-        }
         __syncthreads();
     }
 }
@@ -387,91 +340,41 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
     }
 }
 
-// one workgroup per front step, the front's steps from the last: task (step, 0); wave v owns
-// diagonal block v of the step (lane = column, t in registers, less the L21^T x partials).
-// Stage s (from the last block): every wave first loads 64 entries into registers (wave s: its
-// Linv column; an earlier wave v: its column of the tile L_sv, rows contiguous per lane), then
-// wave s forms x_s = Linv_ss^T t_s and publishes it in LDS, and every earlier wave v
-// subtracts L_sv^T x_s from its t.
-// LDS: (waves + 1) 64 8 entries.
+// one workgroup per narrow front: task (step, 0). t = y - the L21^T x partial sums (chunk
+// order), x = Linv^T t (entry (j, r) per thread). LDS: NB 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(512) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+__global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
                                                  const Task* __restrict__ tasks, const T* __restrict__ factor,
                                                  const T* __restrict__ DB, T* __restrict__ Y,
                                                  const T* __restrict__ part, int nrhs) {
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    T* yx = reinterpret_cast<T*>(sweep_lds);  // (nt / 64) blocks x 64 x 8: the blocks' solutions
-    T* tb = yx + (nt >> 6) * 64 * 8;          // 64 x 8: the block being solved
+    T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8
+    const int tid = threadIdx.x, nt = blockDim.x;
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
-    const int s = st.s, c0 = st.c0, c1 = st.c1;
-    const int f = S.fnrows[s];
+    const int s = st.s, w = st.c1;
     const int cg = S.super_ptr[s];
-    const T* P = factor + S.panel_off[s];
     T* y = Y + (int64_t)cg * nrhs;
-    const int nw = c1 - c0, nblk = (nw + NB - 1) / NB;
     const T* pp = part + st.poff * nrhs;
-    const bool active = wv < nblk;
-    const int kb = c0 + NB * wv;
-    const int nbw = active ? min(NB, c1 - kb) : 1;
-    const int kbs = active ? kb : c0;  // a valid block for the loads of an idle wave
-    const int col = kb + min(lane, nbw - 1);  // this lane's column (clamped)
+    const T* Li = DB + (int64_t)cg * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
-        T acc[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int q = tid; q < w * 8; q += nt) {
+            const int j = q >> 3, r = q & 7;
             T v = T(0);
-            if (active && r < nr) {
-                v = y[(int64_t)col * nrhs + rb + r];
-                for (int ch = 0; ch < st.nch; ++ch) v -= pp[((int64_t)ch * nw + col - c0) * nrhs + rb + r];
+            if (r < nr) {
+                v = y[j * nrhs + rb + r];
+                for (int ch = 0; ch < st.nch; ++ch) v -= pp[((int64_t)ch * w + j) * nrhs + rb + r];
             }
-            acc[r] = v;
+            tb[q] = v;
         }
-        for (int sb = nblk - 1; sb >= 0; --sb) {
-            const int ks = c0 + NB * sb, nbs = min(NB, c1 - ks);
-            // this stage's registers: the tile column L(block s, col) or the Linv column
-            const bool tile = active && wv < sb;
-            const T* base = tile ? P + (int64_t)col * f + ks
-                                 : DB + (int64_t)(cg + kbs) * NB + (int64_t)min(lane, nbw - 1) * NB;
-            const int lim = tile ? nbs : nbw;
-            T l[NB];
-#pragma unroll
-            for (int i = 0; i < NB; ++i) {
-                const T v = base[min(i, lim - 1)];
-                l[i] = i < lim ? v : T(0);
-            }
-            if (wv == sb)
-#pragma unroll
-                for (int r = 0; r < 8; ++r) tb[lane * 8 + r] = acc[r];
-            __syncthreads();
-            if (wv == sb) {
-                T xv[8];
-#pragma unroll
-                for (int r = 0; r < 8; ++r) xv[r] = T(0);
-#pragma unroll
-                for (int i = 0; i < NB; ++i)
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) xv[r] += l[i] * tb[i * 8 + r];  // Linv(i, lane): zero for i < lane
-#pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const T v = lane < nbs ? xv[r] : T(0);
-                    yx[(sb * NB + lane) * 8 + r] = v;
-                }
-            }
-            __syncthreads();
-            if (tile) {
-                const T* xsb = yx + sb * NB * 8;
-#pragma unroll
-                for (int i = 0; i < NB; ++i)
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) acc[r] -= l[i] * xsb[i * 8 + r];
-            }
+        __syncthreads();
+        for (int q = tid; q < w * 8; q += nt) {
+            const int j = q >> 3, r = q & 7;
+            T acc = T(0);
+#pragma unroll 8
+            for (int i = j; i < w; ++i) acc += Li[(int64_t)j * NB + i] * tb[i * 8 + r];
+            if (r < nr) y[j * nrhs + rb + r] = acc;
         }
-        // the step's solution to global memory (a store inside the stage loop makes the
-        // compiler spill the register tiles)
-        for (int q = tid; q < nw * 8; q += nt)
-            if ((q & 7) < nr) y[(int64_t)(c0 + (q >> 3)) * nrhs + rb + (q & 7)] = yx[q];
         __syncthreads();
     }
 }
@@ -735,7 +638,7 @@ void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, in
 template <typename T>
 void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
                      const T* factor, const T* DB, T* Y, T* U, int nrhs) {
-    const size_t lds = (threads / 64 + 1) * 64 * 8 * sizeof(T);
+    const size_t lds = 2 * NB * 8 * sizeof(T);
     hipLaunchKernelGGL(k_fwd_diag<T>, dim3(n), dim3(threads), lds, st, S, steps, tasks, factor, DB, Y, U, nrhs);
 }
 
@@ -755,7 +658,7 @@ void launch_bwd_l21t(hipStream_t st, const DevSym& S, const SolveStep* steps, co
 template <typename T>
 void launch_bwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
                      const T* factor, const T* DB, T* Y, const T* part, int nrhs) {
-    const size_t lds = (threads / 64 + 1) * 64 * 8 * sizeof(T);
+    const size_t lds = NB * 8 * sizeof(T);
     hipLaunchKernelGGL(k_bwd_diag<T>, dim3(n), dim3(threads), lds, st, S, steps, tasks, factor, DB, Y, part, nrhs);
 }
 

@@ -37,7 +37,8 @@ enum LaunchKind {
     L_EXTEND_ADD,
     L_POTRF,
     L_TRSM,
-    L_UPDATE_PANEL,
+    L_UPDATE_PANEL,  // the step's update of the next step's panel columns (look-ahead)
+    L_UPDATE_REST,   // the step's update of the later panel columns (overlaps the next panel)
     L_UPDATE_CB,
     L_LDLT_PANEL,
     L_LDLT_CB,
@@ -93,6 +94,7 @@ int prof_kind(int kind) {
         case L_POTRF: return SDS_K_POTRF;
         case L_TRSM: return SDS_K_TRSM;
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
+        case L_UPDATE_REST: return SDS_K_UPDATE_PANEL;
         case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
         case L_LU_PANEL: return SDS_K_PIVOT_PANEL;
         case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
@@ -173,6 +175,9 @@ struct sds_handle {
     int fsmall = 0;
     // captured refactorization + solve (sds_graph_capture), on a private capture stream
     hipStream_t cstream = nullptr;
+    // the factorization's second stream (the trailing updates that overlap the next panel)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // profiling (HIP events around every launch on the handle's stream)
@@ -518,8 +523,8 @@ int build_schedule(sds_handle* h, int dtype) {
             wmax = 0;
         }
         for (int k = 0; k < wmax; k += NB) {
-            std::vector<Task> tt, tu;
-            double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0;
+            std::vector<Task> tt, tu, tr;
+            double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0, fr = 0, br = 0;
             for (int32_t s : L.big) {
                 int f = h->fnrows[s], w = h->fncols[s];
                 if (k >= w) continue;
@@ -538,14 +543,23 @@ int build_schedule(sds_handle* h, int dtype) {
                 if (base < w)
                     for (int I0 = base; I0 < f; I0 += TILE)
                         for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
-                            tu.push_back({s, I0, J0, k});
                             double e = tile_entries(I0, J0, f, base, w);
-                            fu += 2.0 * e * nb;
-                            bu += 2.0 * dbl_es * e;
+                            if (J0 == base) {
+                                tu.push_back({s, I0, J0, k});
+                                fu += 2.0 * e * nb;
+                                bu += 2.0 * dbl_es * e;
+                            } else {
+                                tr.push_back({s, I0, J0, k});
+                                fr += 2.0 * e * nb;
+                                br += 2.0 * dbl_es * e;
+                            }
                         }
             }
+            // the step: its panel, then the update of the next step's columns, then the rest
+            // (the rest overlaps the next panel on the second stream when not profiling)
             push_launch(L, L_TRSM, tt, ft + fp, bt + bp);
             push_launch(L, L_UPDATE_PANEL, tu, fu, bu);
+            push_launch(L, L_UPDATE_REST, tr, fr, br);
         }
         t.clear();
         fl = by = 0;
@@ -700,7 +714,7 @@ int build_schedule(sds_handle* h, int dtype) {
                         sby += dbl_es * ((double)nw * nw / 2 + (double)nw * nw);
                     } else {
                         td.push_back({id, fused ? 1 : 0, 0, 0});
-                        if (fused && m > 0) threads = 256;
+                        if (f > 64) threads = 256;
                     }
                     // factor entries read: the step's triangle and inverse blocks, the rows below
                     e11s += (double)nw * (nw + 1) / 2 + (double)NB * nw;
@@ -747,6 +761,11 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     T* factor = static_cast<T*>(h->factor);
     T* cb = static_cast<T*>(h->cbpool);
     launch_fill_i32(st, h->status, h->ns, 0);
+    // look-ahead: a step's update of the later panel columns runs on the second stream while
+    // the next step's panel runs here; every other launch waits for it first
+    const bool overlap = !h->prof && h->stream2;
+    bool pending = false;   // a forked launch not yet joined
+    bool may_skip = false;  // the launch right after the fork (the next panel) need not wait
     for (auto& L : h->levels) {
         if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
         if (!L.small.empty())
@@ -756,6 +775,19 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             });
         for (const Launch& ln : L.plan) {
             const Task* t = h->d_tasks + ln.off;
+            if (overlap && ln.kind == L_UPDATE_REST) {
+                HIPCHK(h, hipEventRecord(h->ev_fork, st));
+                HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+                launch_big_update<T>(h->stream2, h->dsym, t, ln.n, factor, cb, h->status, 0);
+                HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
+                pending = may_skip = true;
+                continue;
+            }
+            if (pending && !(may_skip && ln.kind == L_TRSM)) {
+                HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
+                pending = false;
+            }
+            may_skip = false;
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
@@ -764,6 +796,7 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                         launch_big_panel<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
                         break;
                     case L_UPDATE_PANEL:
+                    case L_UPDATE_REST:
                         launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
                         break;
                     case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
@@ -771,6 +804,7 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             });
         }
     }
+    if (pending) HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
     // the inverses of the diagonal blocks for the sweeps (a failed factorization gives
     // inverses of whatever the panels hold; its solves are not meaningful anyway)
@@ -1091,6 +1125,9 @@ int sds_destroy(sds_handle_t h) {
     else (void)hipDeviceSynchronize();
     graph_free(h);
     if (h->cstream) (void)hipStreamDestroy(h->cstream);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     free_numeric(h);
     free_all(h->symbufs);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
@@ -1275,6 +1312,11 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     free_numeric(h);
     int rc = build_schedule(h, dtype);
     if (rc) return rc;
+    if (!h->stream2) {
+        HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+        HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    }
     size_t es = dtype == SDS_R64 ? 8 : 4;
     void* p;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->factor))) return rc;
