@@ -166,125 +166,6 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     }
 }
 
-// Block step of a large front: Cholesky of the (updated) diagonal block [k, k + nb) and the
-// triangular solve of a tile of PANEL_ROWS panel rows below it, X <- X L_bb^-T. Every
-// workgroup of the step factors the 64 x 64 block in LDS itself (two barriers per column:
-// each lane recomputes the pivot's square root, no broadcast needed), so the step is one
-// launch; the task with c = 1 keeps the factored block in DB (NB entries per factor column)
-// and records a failed pivot. The panel's own diagonal block keeps its unfactored values
-// until the factorization's last launch copies DB into it: the blocks are read by nobody in
-// between, and no workgroup of the step writes what another one reads.
-// Each thread then solves its row in registers, right-looking (column jj scaled by the
-// pivot's reciprocal, then subtracted from the later columns: 63 independent updates); the
-// row is loaded and stored straight from the panel (consecutive threads, consecutive rows).
-// task: (s, k, row0, c)
-template <typename T>
-__global__ void __launch_bounds__(PANEL_ROWS) k_big_panel(DevSym S, const Task* __restrict__ tasks,
-                                                         T* __restrict__ factor, T* __restrict__ DB,
-                                                         int32_t* __restrict__ status, double eps) {
-    __shared__ T D[NB][NB + 1];  // D[j][i] = L(k + i, k + j)
-    __shared__ T rd[NB];         // reciprocals of the pivots
-    __shared__ int fail;
-    const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, row0 = t.b;
-    if (status[s] != 0) return;
-    const int nb = min(NB, w - k);
-    T* P = factor + S.panel_off[s];
-    const int tid = threadIdx.x;
-    const int i = row0 + tid;
-    // the diagonal block (lower part, identity past nb)
-#pragma unroll
-    for (int q = tid; q < NB * NB; q += PANEL_ROWS) {
-        const int j = q / NB, ii = q - j * NB;
-        D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? P[(int64_t)(k + j) * f + k + ii] : T(0)) : (ii == j ? T(1) : T(0));
-    }
-    if (tid == 0) fail = 0;
-    __syncthreads();
-    // blocked Cholesky of D: per 16-column panel, wave 0 factors the panel's rows in registers
-    // (lane i holds row kb + i of the panel; the pivot and the column entries move between
-    // lanes by v_readlane, no barrier inside the panel; 1/sqrt is the hardware estimate with
-    // two Newton steps), then all four waves apply the panel to the trailing block from LDS.
-    const int ii = tid & (NB - 1), part = tid / NB;
-    static_assert(PANEL_ROWS == 4 * NB, "four waves of NB rows");
-    for (int kb = 0; kb < nb; kb += 16) {
-        if (part == 0) {
-            const int row = kb + ii;
-            T p[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) p[c] = row < NB ? D[kb + c][row] : T(0);
-            int bad = 0;
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const double piv = (double)lane_bcast(p[c], c);
-                if (kb + c < nb && !chol_accept(piv, eps) && !bad) bad = kb + c + 1;
-                const T r = T(rsqrt_nr(fmax(piv, eps)));
-                const T d = T(fmax(piv, eps)) * r;
-                p[c] = ii > c ? p[c] * r : (ii == c ? d : p[c]);
-                if (ii == 0) rd[kb + c] = r;
-#pragma unroll
-                for (int cc = c + 1; cc < 16; ++cc) p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
-            }
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (row < NB) D[kb + c][row] = p[c];
-            if (ii == 0 && bad) fail = bad;
-        }
-        __syncthreads();
-        if (fail) break;  // uniform after the barrier
-        const int j0 = kb + 16;
-        if (ii >= j0) {
-            T lr[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) lr[c] = D[kb + c][ii];
-            for (int jj = j0 + part; jj <= ii; jj += 4) {
-                T acc = D[jj][ii];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) acc = fma(-lr[c], D[kb + c][jj], acc);
-                D[jj][ii] = acc;
-            }
-        }
-        __syncthreads();
-    }
-    __syncthreads();
-    if (fail) {
-        if (t.c && tid == 0) status[s] = k + fail;
-        return;
-    }
-    for (int j = nb + tid; j < NB; j += PANEL_ROWS) rd[j] = T(1);
-    if (t.c) {
-        T* db = DB + (int64_t)(S.super_ptr[s] + k) * NB;
-        for (int q = tid; q < nb * NB; q += PANEL_ROWS) db[q] = D[q / NB][q % NB];
-    }
-    __syncthreads();
-    if (i >= f) return;
-    // this thread's row. Per-column guards on the register row make the compiler spill it,
-    // so columns past nb load column nb - 1 (they solve to garbage that nothing reads: the
-    // padding of D keeps them out of the real columns) and the stores run from the last
-    // column down, so that column nb - 1 is written last with its real value.
-    T x[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = P[(int64_t)(k + min(c, nb - 1)) * f + i];
-#pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
-        x[jj] = x[jj] * rd[jj];
-        const T xj = x[jj];
-#pragma unroll
-        for (int kk = jj + 1; kk < NB; ++kk) x[kk] = fma(-xj, D[jj][kk], x[kk]);
-    }
-#pragma unroll
-    for (int c = NB - 1; c >= 0; --c) P[(int64_t)(k + min(c, nb - 1)) * f + i] = x[c];
-}
-
-// Schur update of one 64x64 tile on the matrix cores: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T
-// on the lower triangle of the front, A the panel columns k0..k1-1. mode 0: step update of
-// the panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
-// task: (s, I0, J0, k0). 256 threads = 4 waves, each wave a 32x32 quadrant of the tile as
-// 2x2 blocks of 16x16 accumulators; K in chunks of 64 staged through LDS (one chunk holds a
-// whole step), the next chunk prefetched into registers while the current one is multiplied,
-// and the tile's old values loaded at the start. MFMA 16x16x4 operand layout (checked on
-// gfx950): lane l gives A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator element q of lane l is D[row(l / 16, q)][l % 16] with
-// row = 4 (l / 16) + q for the f32 instruction and 4 q + l / 16 for the f64 one.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -307,16 +188,188 @@ struct Mfma<float> {
     static __device__ inline int row(int l4, int q) { return 4 * l4 + q; }
 };
 
+// Cholesky of the 64 x 64 block D in LDS (D[j][i] = L(i, j), lower part; padded with the
+// identity past nb), four waves: per 16-column panel, wave 0 factors the panel's rows in
+// registers (lane i holds row kb + i; the pivot and the column entries move between lanes
+// by v_readlane; 1/sqrt is the hardware estimate with two Newton steps), then all four waves
+// apply the panel to the trailing block. rd gets the pivots' reciprocals (1 past nb); a
+// failed pivot (1-based within the block) is left in fail. Ends with a barrier.
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
-                                                        T* __restrict__ factor, T* __restrict__ cbpool,
-                                                        const int32_t* __restrict__ status, int mode) {
-    constexpr int KC = 64;                 // a step's whole K (mode 0) in one chunk
+__device__ __attribute__((always_inline)) inline void block_potrf(T (*D)[NB + 1], T* rd, int* fail, int nb, double eps) {
+    const int tid = threadIdx.x, ii = tid & (NB - 1), part = tid / NB;
+    if (tid == 0) *fail = 0;
+    __syncthreads();
+    for (int kb = 0; kb < nb; kb += 16) {
+        if (part == 0) {
+            const int row = kb + ii;
+            T p[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) p[c] = row < NB ? D[kb + c][row] : T(0);
+            int bad = 0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const double piv = (double)lane_bcast(p[c], c);
+                if (kb + c < nb && !chol_accept(piv, eps) && !bad) bad = kb + c + 1;
+                const T r = T(rsqrt_nr(fmax(piv, eps)));
+                const T d = T(fmax(piv, eps)) * r;
+                p[c] = ii > c ? p[c] * r : (ii == c ? d : p[c]);
+                if (ii == 0) rd[kb + c] = r;
+#pragma unroll
+                for (int cc = c + 1; cc < 16; ++cc) p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (row < NB) D[kb + c][row] = p[c];
+            if (ii == 0 && bad) *fail = bad;
+        }
+        __syncthreads();
+        if (*fail) break;  // uniform after the barrier
+        const int j0 = kb + 16;
+        if (ii >= j0) {
+            T lr[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lr[c] = D[kb + c][ii];
+            for (int jj = j0 + part; jj <= ii; jj += 4) {
+                T acc = D[jj][ii];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) acc = fma(-lr[c], D[kb + c][jj], acc);
+                D[jj][ii] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = nb + tid; j < NB; j += blockDim.x) rd[j] = T(1);
+    __syncthreads();
+}
+
+// Linv = L^-1 of the factored 64 x 64 block in D (D[j][i] = L(i, j), identity past nb; rd the
+// pivots' reciprocals) on the matrix cores, into X (X[j][i] = Linv(i, j)), blocked by 16:
+// wave i inverts the diagonal block L_ii by forward substitution (lane c: column c, sixteen
+// steps), then wave j forms its column blocks X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, i = j+1..3,
+// with 16 x 16 x 16 MFMA products (T staged through LDS between the two products). Ends with
+// a barrier.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void block_inverse_mfma(const T (*D)[NB + 1], const T* rd,
+                                                                         T (*X)[NB + 1], T (*Ts)[16][17]) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    // zero the upper blocks, then the diagonal blocks' inverses
+    for (int e = tid; e < NB * NB; e += blockDim.x) {
+        const int j = e / NB, i = e - j * NB;
+        if (i / 16 < j / 16) X[j][i] = T(0);
+    }
+    {
+        const int b = 16 * wv, c = l16;
+        T x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = r == c ? T(1) : T(0);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            x[m] = x[m] * rd[b + m];
+            const T xm = x[m];
+#pragma unroll
+            for (int r = m + 1; r < 16; ++r) x[r] = fma(-D[b + m][b + r], xm, x[r]);
+        }
+        if (lane < 16)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) X[b + c][b + r] = x[r];
+    }
+    __syncthreads();
+    // off-diagonal blocks of column block j = wv (waves 0..2): block row i = j + it at round it
+    // (every wave runs the three rounds, so the barriers match)
+    const int j = wv;
+    for (int it = 1; it < 4; ++it) {
+        const int i = j + it;
+        const bool act = i < 4;
+        acc_t acc = acc_t{};
+        if (act) {
+            for (int k = j; k < i; ++k)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const T a = D[16 * k + 4 * s + l4][16 * i + l16];  // L(16 i + m, 16 k + 4 s + kk)
+                    const T b = X[16 * j + l16][16 * k + 4 * s + l4];  // X(16 k + 4 s + kk, 16 j + n)
+                    acc = Mfma<T>::mac(a, b, acc);
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ts[wv][Mfma<T>::row(l4, q)][l16] = acc[q];  // Ts[m][n] = T(m, n)
+        }
+        __syncthreads();
+        if (act) {
+            acc = acc_t{};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const T a = -X[16 * i + 4 * s + l4][16 * i + l16];  // -X(16 i + m, 16 i + 4 s + kk)
+                const T b = Ts[wv][4 * s + l4][l16];                 // T(4 s + kk, n)
+                acc = Mfma<T>::mac(a, b, acc);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) X[16 * j + l16][16 * i + Mfma<T>::row(l4, q)] = acc[q];
+        }
+        __syncthreads();
+    }
+}
+
+// after block_potrf and block_inverse_mfma: the factored block into the panel (lower part)
+// and Linv into DB (NB entries per factor column, zero past nb) for the later row solves and
+// the sweeps
+template <typename T>
+__device__ __attribute__((always_inline)) inline void block_store(const T (*D)[NB + 1], const T (*X)[NB + 1], T* P,
+                                                                  int f, int k, int nb, T* out) {
+    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
+        const int j = q / NB, i = q - j * NB;
+        if (j < nb && i >= j && i < nb) P[(int64_t)(k + j) * f + k + i] = D[j][i];
+        if (j < nb) out[q] = i < nb ? X[j][i] : T(0);
+    }
+}
+
+// Diagonal block of a large front's first step: its Cholesky and inverse, one workgroup per
+// front. The later steps' diagonal blocks are factored inside k_big_step. task: (s, k)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                 T* __restrict__ DB, int32_t* __restrict__ status, double eps) {
+    __shared__ T D[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];
+    __shared__ T Ts[4][16][17];
+    __shared__ T rd[NB];
+    __shared__ int fail;
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a;
+    if (status[s] != 0) return;
+    const int nb = min(NB, w - k);
+    T* P = factor + S.panel_off[s];
+    for (int q = threadIdx.x; q < NB * NB; q += 256) {
+        const int j = q / NB, ii = q - j * NB;
+        D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? P[(int64_t)(k + j) * f + k + ii] : T(0)) : (ii == j ? T(1) : T(0));
+    }
+    block_potrf<T>(D, rd, &fail, nb, eps);
+    if (fail) {
+        if (threadIdx.x == 0) status[s] = k + fail;
+        return;
+    }
+    block_inverse_mfma<T>(D, rd, X, Ts);
+    block_store<T>(D, X, P, f, k, nb, DB + (int64_t)(S.super_ptr[s] + k) * NB);
+}
+
+// Schur update of one 64x64 tile on the matrix cores: C(I, J) -= A(I, k0:k1) A(J, k0:k1)^T
+// on the lower triangle of the front, A the panel columns k0..k1-1. mode 0: step update of
+// the panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
+// task: (s, I0, J0, k0). 256 threads = 4 waves, each wave a 32x32 quadrant of the tile as
+// 2x2 blocks of 16x16 accumulators; K in chunks of 64 staged through LDS (one chunk holds a
+// whole step), the next chunk prefetched into registers while the current one is multiplied,
+// and the tile's old values loaded at the start. MFMA 16x16x4 operand layout (checked on
+// gfx950): lane l gives A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator element q of lane l is D[row(l / 16, q)][l % 16] with
+// row = 4 (l / 16) + q for the f32 instruction and 4 q + l / 16 for the f64 one.
+constexpr int UPD_KC = 64;  // a step's whole K (mode 0) in one chunk
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void update_tile(const DevSym& S, const Task t, T* __restrict__ factor,
+                                                                  T* __restrict__ cbpool,
+                                                                  const int32_t* __restrict__ status, int mode,
+                                                                  T (*As)[TILE], T (*Bs)[TILE]) {
+    constexpr int KC = UPD_KC;
     constexpr int PER = KC * TILE / 256;   // entries of A (and of B) per thread and chunk
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As[KC][TILE];
-    __shared__ T Bs[KC][TILE];
-    const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
     if (status[s] != 0) return;
     const int I0 = t.a, J0 = t.b;
@@ -395,6 +448,192 @@ __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* _
         }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
+                                                        T* __restrict__ factor, T* __restrict__ cbpool,
+                                                        const int32_t* __restrict__ status, int mode) {
+    __shared__ T As[UPD_KC][TILE];
+    __shared__ T Bs[UPD_KC][TILE];
+    update_tile<T>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
+}
+
+// the rows [k + 64, k + 128) of a step's columns, kept in LN by k_big_step, into the panel.
+// task: (s, k)
+template <typename T>
+__device__ inline void copy_ln(const DevSym& S, const Task t, T* __restrict__ factor, const T* __restrict__ LN,
+                               const int32_t* __restrict__ status) {
+    const int s = t.s, f = S.fnrows[s], k = t.a, J = k + NB;
+    if (status[s] != 0) return;
+    T* P = factor + S.panel_off[s];
+    const T* src = LN + (int64_t)(S.super_ptr[s] + k) * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int c = e / NB, r = e - c * NB;
+        if (J + r < f) P[(int64_t)(k + c) * f + J + r] = src[e];
+    }
+}
+
+// One step b (columns [k, k + 64)) of a large front for a tile of 64 rows [I0, I0 + 64) below
+// the step's diagonal block, on the matrix cores, one workgroup of 4 waves (16 rows each):
+//  1. row solve: L(I, b) = A(I, b) Linv_b^T (Linv_b in DB, staged in LDS; the A fragments are
+//     all loaded before the MFMAs, so the rows are replaced in place);
+//  2. look-ahead, when a next block b + 1 exists (J = k + 64): the tile's rows of the next
+//     block's columns get C(I, J) -= L(I, b) L(J, b)^T, and also -= L(I, b-1) L(J, b-1)^T
+//     when b > 0 (the steps before b-1 reach block b + 1 through the trailing-update launches,
+//     which cover the blocks from three past their step). L(J, b) is recomputed by every
+//     workgroup from A(J, b);
+//  3. the workgroup whose tile is the next diagonal block (I0 = J) factors it (Cholesky and
+//     inverse, as k_big_diag) for step b + 1.
+// The rows J of the step's columns are read (unsolved) by every workgroup, so the workgroup
+// that solves them keeps them in LN (NB entries per factor column) instead of the panel.
+// The next step's launch also carries this step's copies of those rows (copy_ln) and its
+// trailing update of the blocks three on (update_tile, mode 0): they touch no block that
+// step reads or writes, and they fill the machine while the step's chain runs.
+// tasks: nstep steps (s, k, I0), then ncopy copies (s, k), then update tiles (s, I0, J0, k)
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                 T* __restrict__ cbpool, T* __restrict__ DB, T* __restrict__ LN,
+                                                 int32_t* __restrict__ status, double eps, int nstep, int ncopy) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T Bs[NB][NB + 1];  // Bs[kk][c] = Linv_b(c, kk); later L(J + r, b - 1) as Bs[r][kk]; later D
+    __shared__ T Lc[NB][NB + 1];  // Lc[r][kk] = L(I0 + r, k + kk)
+    __shared__ T Ln[NB][NB + 1];  // Ln[r][kk] = L(J + r, k + kk)
+    __shared__ T rd[NB];
+    __shared__ int fail;
+    const Task t = tasks[blockIdx.x];
+    if ((int)blockIdx.x >= nstep + ncopy) {
+        update_tile<T>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Lc[0][0]),
+                       reinterpret_cast<T(*)[TILE]>(&Ln[0][0]));
+        return;
+    }
+    if ((int)blockIdx.x >= nstep) {
+        copy_ln<T>(S, t, factor, LN, status);
+        return;
+    }
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, I0 = t.b;
+    if (status[s] != 0) return;
+    const int nb = min(NB, w - k);
+    const bool la = k + NB < w;  // a next block exists (then nb = NB)
+    const int J = k + NB, nbn = la ? min(NB, w - J) : 0;
+    T* P = factor + S.panel_off[s];
+    const int cg = S.super_ptr[s];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const T* Li = DB + (int64_t)(cg + k) * NB;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int kk = e / NB, c = e - kk * NB;
+        Bs[kk][c] = kk < nb && c < nb ? Li[e] : T(0);
+    }
+    const int rI = I0 + 16 * wv + l16, rJ = J + 16 * wv + l16;
+    T aI[16], aJ[16];
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const int col = 4 * ks + l4, cc = k + min(col, nb - 1);
+        const T vi = P[(int64_t)cc * f + min(rI, f - 1)];
+        const T vj = P[(int64_t)cc * f + min(rJ, f - 1)];
+        aI[ks] = rI < f && col < nb ? vi : T(0);
+        aJ[ks] = la && rJ < f && col < nb ? vj : T(0);
+    }
+    __syncthreads();
+    acc_t accI[4], accJ[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) accI[nt] = accJ[nt] = acc_t{};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const T b = Bs[4 * ks + l4][16 * nt + l16];
+            accI[nt] = Mfma<T>::mac(aI[ks], b, accI[nt]);
+            if (la) accJ[nt] = Mfma<T>::mac(aJ[ks], b, accJ[nt]);
+        }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * wv + Mfma<T>::row(l4, q), c = 16 * nt + l16;
+            if (I0 + r < f && c < nb) {
+                if (la && I0 == J)
+                    LN[(int64_t)(cg + k + c) * NB + r] = accI[nt][q];
+                else
+                    P[(int64_t)(k + c) * f + I0 + r] = accI[nt][q];
+            }
+            if (la) {
+                Lc[r][c] = I0 + r < f ? accI[nt][q] : T(0);
+                Ln[r][c] = J + r < f ? accJ[nt][q] : T(0);
+            }
+        }
+    if (!la) return;
+    // the previous step's rows of the next block, for its part of the look-ahead
+    const bool prev = k >= NB;
+    __syncthreads();
+    if (prev)
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int kk = e / NB, r = e - kk * NB;
+            Bs[r][kk] = J + r < f ? P[(int64_t)(k - NB + kk) * f + J + r] : T(0);
+        }
+    // the tile's old values of the next block's columns
+    acc_t accC[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = I0 + 16 * wv + Mfma<T>::row(l4, q), j = J + 16 * nt + l16;
+            accC[nt][q] = i < f && j < J + nbn && i >= j ? P[(int64_t)j * f + i] : T(0);
+        }
+    T ap[16];
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const T v = P[(int64_t)(max(k - NB, 0) + 4 * ks + l4) * f + min(rI, f - 1)];
+        ap[ks] = prev && rI < f ? -v : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const T a = -Lc[16 * wv + l16][4 * ks + l4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(a, Ln[16 * nt + l16][4 * ks + l4], accC[nt]);
+    }
+    if (prev)
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(ap[ks], Bs[16 * nt + l16][4 * ks + l4], accC[nt]);
+    if (I0 != J) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = I0 + 16 * wv + Mfma<T>::row(l4, q), j = J + 16 * nt + l16;
+                if (i < f && j < J + nbn) P[(int64_t)j * f + i] = accC[nt][q];
+            }
+        return;
+    }
+    // the next diagonal block: factor and invert it for step b + 1 (a narrower last block
+    // leaves rows below it in this tile: those are stored as updated)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * wv + Mfma<T>::row(l4, q), j = 16 * nt + l16;
+            if (i >= nbn && J + i < f && j < nbn) P[(int64_t)(J + j) * f + J + i] = accC[nt][q];
+        }
+    __syncthreads();
+    T(*D)[NB + 1] = Bs;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * wv + Mfma<T>::row(l4, q), j = 16 * nt + l16;
+            D[j][i] = (i < nbn && j < nbn) ? (i >= j ? accC[nt][q] : T(0)) : (i == j ? T(1) : T(0));
+        }
+    block_potrf<T>(D, rd, &fail, nbn, eps);
+    if (fail) {
+        if (tid == 0) status[s] = J + fail;
+        return;
+    }
+    block_inverse_mfma<T>(D, rd, Lc, reinterpret_cast<T(*)[16][17]>(&Ln[0][0]));
+    block_store<T>(D, Lc, P, f, J, nbn, DB + (int64_t)(cg + J) * NB);
+}
+
+
 // per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and the smallest
 // failed factor column (1-based, 0 = none) into info[0]
 __global__ void __launch_bounds__(256) k_chol_stats(DevSym S, const int32_t* __restrict__ status,
@@ -446,9 +685,16 @@ void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, i
 }
 
 template <typename T>
-void launch_big_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
-                      double eps) {
-    hipLaunchKernelGGL(k_big_panel<T>, dim3(n), dim3(PANEL_ROWS), 0, st, S, tasks, factor, DB, status, eps);
+void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
+                     double eps) {
+    hipLaunchKernelGGL(k_big_diag<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, DB, status, eps);
+}
+
+template <typename T>
+void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps) {
+    hipLaunchKernelGGL(k_big_step<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps, nstep,
+                       ncopy);
 }
 
 template <typename T>
@@ -472,7 +718,9 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
                                              T*, T*, int32_t*, double);                                           \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
-    template void launch_big_panel<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);    \
+    template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \
+    template void launch_big_step<T>(hipStream_t, const DevSym&, const Task*, int, int, int, T*, T*, T*, T*,      \
+                                     int32_t*, double);                                                            \
     template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int);
 INST(double)
 INST(float)

@@ -33,13 +33,12 @@ namespace sds {
 #pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
-// Linv of the diagonal block [k, k + nb) of front s: one wave; lane c computes column c of
-// the inverse by forward substitution with the partial sums in registers (fully unrolled:
-// the block is padded with the identity past nb, so no guards touch the register array;
-// no scheduling barriers either: with them the compiler spills this kernel). The block is
-// read from the panel (small fronts) or, for the large fronts (c = 1), from DB, where the
-// block steps left it; it is then also written into its panel. Linv replaces it in DB.
-// task: (s, k, 0, large)
+// Linv of the diagonal block [k, k + nb) of a small front: one wave; lane c computes column
+// c of the inverse by forward substitution with the partial sums in registers (fully
+// unrolled: the block is padded with the identity past nb, so no guards touch the register
+// array; no scheduling barriers either: with them the compiler spills this kernel). The
+// large fronts' blocks are inverted by k_big_diag during the factorization. Linv goes to DB.
+// task: (s, k)
 
 template <typename T>
 __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __restrict__ tasks,

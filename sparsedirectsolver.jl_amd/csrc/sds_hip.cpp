@@ -37,8 +37,8 @@ enum LaunchKind {
     L_EXTEND_ADD,
     L_POTRF,
     L_TRSM,
-    L_UPDATE_PANEL,  // the step's update of the next step's panel columns (look-ahead)
-    L_UPDATE_REST,   // the step's update of the later panel columns (overlaps the next panel)
+    L_UPDATE_PANEL,
+    L_STEP,          // k_big_step: a step, with the previous step's copies and trailing update
     L_UPDATE_CB,
     L_LDLT_PANEL,
     L_LDLT_CB,
@@ -63,6 +63,7 @@ struct Launch {
     int64_t off;
     int n;
     double flops, bytes;  // algorithmic work of the launch
+    int nstep = 0, ncopy = 0;  // k_big_step launches: step tasks, then copies, then update tiles
 };
 
 struct Level {
@@ -94,7 +95,7 @@ int prof_kind(int kind) {
         case L_POTRF: return SDS_K_POTRF;
         case L_TRSM: return SDS_K_TRSM;
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
-        case L_UPDATE_REST: return SDS_K_UPDATE_PANEL;
+        case L_STEP: return SDS_K_TRSM;
         case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
         case L_LU_PANEL: return SDS_K_PIVOT_PANEL;
         case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
@@ -157,7 +158,8 @@ struct sds_handle {
     void* factor = nullptr;
     void* ufactor = nullptr;  // U^T panels of "G"
     void* cbpool = nullptr;
-    void* dbuf = nullptr;  // factored diagonal blocks of the large fronts (NB entries per column)
+    void* dbuf = nullptr;   // inverse diagonal blocks (NB entries per column)
+    void* lnbuf = nullptr;  // a large-front step's solved rows of the next block (NB entries per column)
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -175,9 +177,7 @@ struct sds_handle {
     int fsmall = 0;
     // captured refactorization + solve (sds_graph_capture), on a private capture stream
     hipStream_t cstream = nullptr;
-    // the factorization's second stream (the trailing updates that overlap the next panel)
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // profiling (HIP events around every launch on the handle's stream)
@@ -522,44 +522,63 @@ int build_schedule(sds_handle* h, int dtype) {
             push_launch(L, lu ? L_LU_PANEL : L_LDLT_PANEL, t, fl, by);
             wmax = 0;
         }
+        // per 64-column step b (k = 64 b): one k_big_step launch with the step's row tiles
+        // (row solve, look-ahead update of block b + 1, the next diagonal block), the previous
+        // step's copies and its trailing update of blocks b + 2 on (the first step's diagonal
+        // block by k_big_diag before; the last step's copies and update in a launch of their own)
+        std::vector<Task> pc, pr;  // the previous step's copies and update tiles
+        double pfr = 0, pbr = 0;
+        auto push_step = [&](std::vector<Task>& st, double fs, double bs) {
+            if (st.empty() && pc.empty() && pr.empty()) return;
+            std::vector<Task> all(st);
+            all.insert(all.end(), pc.begin(), pc.end());
+            all.insert(all.end(), pr.begin(), pr.end());
+            push_launch(L, L_STEP, all, fs + pfr, bs + pbr + dbl_es * 2.0 * NB * NB * (double)pc.size());
+            L.plan.back().nstep = (int)st.size();
+            L.plan.back().ncopy = (int)pc.size();
+        };
         for (int k = 0; k < wmax; k += NB) {
-            std::vector<Task> tt, tu, tr;
-            double fp = 0, ft = 0, fu = 0, bp = 0, bt = 0, bu = 0, fr = 0, br = 0;
+            std::vector<Task> td, tt, tr, tc;
+            double fp = 0, ft = 0, fr = 0, bp = 0, bt = 0, br = 0;
             for (int32_t s : L.big) {
                 int f = h->fnrows[s], w = h->fncols[s];
                 if (k >= w) continue;
                 int nb = std::min(NB, w - k);
-                fp += (double)nb * nb * nb / 3.0;
-                bp += dbl_es * (double)nb * (nb + 1);
-                int first = 1;
-                for (int r0 = k + nb; r0 < f; r0 += PANEL_ROWS) {
-                    tt.push_back({s, k, r0, first});
-                    first = 0;
+                if (k == 0) td.push_back({s, 0, 0, 0});
+                // the diagonal block's factorization and inverse
+                fp += (double)nb * nb * nb / 3.0 * 2.0;
+                bp += dbl_es * ((double)nb * (nb + 1) + (double)NB * nb);
+                for (int r0 = k + nb; r0 < f; r0 += 64) tt.push_back({s, k, r0, 0});
+                ft += 2.0 * (double)(f - k - nb) * nb * nb;
+                bt += dbl_es * 2.0 * (double)(f - k - nb) * nb;
+                if (k + NB < w) {  // the look-ahead update of block b + 1 (and b's predecessor's part)
+                    const int J = k + NB;
+                    tc.push_back({s, k, 0, 0});
+                    for (int I0 = J; I0 < f; I0 += TILE) {
+                        double e = tile_entries(I0, J, f, J, w);
+                        ft += 2.0 * e * (k >= NB ? 2 * NB : NB);
+                        bt += 2.0 * dbl_es * e;
+                    }
                 }
-                if (first) tt.push_back({s, k, f, 1});
-                ft += (double)(f - k - nb) * nb * nb;
-                bt += dbl_es * ((double)nb * (nb + 1) / 2 + 2.0 * (double)(f - k - nb) * nb);
-                int base = k + nb;
-                if (base < w)
-                    for (int I0 = base; I0 < f; I0 += TILE)
-                        for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
-                            double e = tile_entries(I0, J0, f, base, w);
-                            if (J0 == base) {
-                                tu.push_back({s, I0, J0, k});
-                                fu += 2.0 * e * nb;
-                                bu += 2.0 * dbl_es * e;
-                            } else {
-                                tr.push_back({s, I0, J0, k});
-                                fr += 2.0 * e * nb;
-                                br += 2.0 * dbl_es * e;
-                            }
-                        }
+                const int base = k + 3 * NB;
+                for (int I0 = base; I0 < f; I0 += TILE)
+                    for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
+                        tr.push_back({s, I0, J0, k});
+                        double e = tile_entries(I0, J0, f, base, w);
+                        fr += 2.0 * e * nb;
+                        br += 2.0 * dbl_es * e;
+                    }
             }
-            // the step: its panel, then the update of the next step's columns, then the rest
-            // (the rest overlaps the next panel on the second stream when not profiling)
-            push_launch(L, L_TRSM, tt, ft + fp, bt + bp);
-            push_launch(L, L_UPDATE_PANEL, tu, fu, bu);
-            push_launch(L, L_UPDATE_REST, tr, fr, br);
+            push_launch(L, L_POTRF, td, fp, bp);
+            push_step(tt, ft, bt);
+            pc = tc;
+            pr = tr;
+            pfr = fr;
+            pbr = br;
+        }
+        {
+            std::vector<Task> none;
+            push_step(none, 0, 0);
         }
         t.clear();
         fl = by = 0;
@@ -664,8 +683,8 @@ int build_schedule(sds_handle* h, int dtype) {
         std::vector<Task> t;
         double fl = 0, by = 0;
         for (int64_t s = 0; s < ns; ++s)
-            for (int k = 0; k < h->fncols[s]; k += NB) {
-                t.push_back({(int32_t)s, k, 0, big[s] ? 1 : 0});
+            for (int k = 0; k < h->fncols[s] && !big[s]; k += NB) {  // big fronts: done by k_big_diag
+                t.push_back({(int32_t)s, k, 0, 0});
                 const double nb = std::min(NB, h->fncols[s] - k);
                 fl += nb * nb * nb / 3.0;
                 by += dbl_es * (nb * (nb + 1) / 2 + (double)NB * nb);
@@ -761,11 +780,6 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     T* factor = static_cast<T*>(h->factor);
     T* cb = static_cast<T*>(h->cbpool);
     launch_fill_i32(st, h->status, h->ns, 0);
-    // look-ahead: a step's update of the later panel columns runs on the second stream while
-    // the next step's panel runs here; every other launch waits for it first
-    const bool overlap = !h->prof && h->stream2;
-    bool pending = false;   // a forked launch not yet joined
-    bool may_skip = false;  // the launch right after the fork (the next panel) need not wait
     for (auto& L : h->levels) {
         if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
         if (!L.small.empty())
@@ -775,28 +789,18 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             });
         for (const Launch& ln : L.plan) {
             const Task* t = h->d_tasks + ln.off;
-            if (overlap && ln.kind == L_UPDATE_REST) {
-                HIPCHK(h, hipEventRecord(h->ev_fork, st));
-                HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
-                launch_big_update<T>(h->stream2, h->dsym, t, ln.n, factor, cb, h->status, 0);
-                HIPCHK(h, hipEventRecord(h->ev_join, h->stream2));
-                pending = may_skip = true;
-                continue;
-            }
-            if (pending && !(may_skip && ln.kind == L_TRSM)) {
-                HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
-                pending = false;
-            }
-            may_skip = false;
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
                     case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
-                    case L_TRSM:
-                        launch_big_panel<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
+                    case L_POTRF:
+                        launch_big_diag<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
+                        break;
+                    case L_STEP:
+                        launch_big_step<T>(st, h->dsym, t, ln.n, ln.nstep, ln.ncopy, factor, cb,
+                                           static_cast<T*>(h->dbuf), static_cast<T*>(h->lnbuf), h->status, eps);
                         break;
                     case L_UPDATE_PANEL:
-                    case L_UPDATE_REST:
                         launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
                         break;
                     case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
@@ -804,7 +808,6 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             });
         }
     }
-    if (pending) HIPCHK(h, hipStreamWaitEvent(st, h->ev_join, 0));
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
     // the inverses of the diagonal blocks for the sweeps (a failed factorization gives
     // inverses of whatever the panels hold; its solves are not meaningful anyway)
@@ -1125,9 +1128,7 @@ int sds_destroy(sds_handle_t h) {
     else (void)hipDeviceSynchronize();
     graph_free(h);
     if (h->cstream) (void)hipStreamDestroy(h->cstream);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
-    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+
     free_numeric(h);
     free_all(h->symbufs);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
@@ -1312,11 +1313,7 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     free_numeric(h);
     int rc = build_schedule(h, dtype);
     if (rc) return rc;
-    if (!h->stream2) {
-        HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
-        HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-        HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
-    }
+
     size_t es = dtype == SDS_R64 ? 8 : 4;
     void* p;
     if ((rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->factor))) return rc;
@@ -1326,6 +1323,7 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
         (rc = dev_alloc(h, h->numbufs, es * std::max<int64_t>(h->factor_len, 1), &h->ufactor)))
         return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->dbuf))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->lnbuf))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
     h->status = static_cast<int32_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, es * 2 * std::max<int64_t>(h->n, 1), &h->dvec))) return rc;

@@ -13,8 +13,11 @@ void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks,
 template <typename T>
 void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool);
 template <typename T>
-void launch_big_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
-                      double eps);
+void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
+                     double eps);
+template <typename T>
+void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode);
