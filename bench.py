@@ -220,6 +220,122 @@ def bench_batch(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_adjoint(args):
+    """Differentiation (C5): the loss 0.5 ||x - x*||^2 with x = A^-1 b, b and x* ~ N(0, 1), on a
+    48^3 variable-coefficient 7-point operator (log-normal conductivities, SPD Cholesky) and on
+    the nonsymmetric upwind convection-diffusion variant (Peclet 10, LU with threshold partial
+    pivoting), FP64 and FP32 factor + FP64 refinement. Gradients with respect to b and to the
+    stored values of A come from the transposed solve on the same factor and the HIP scatter of
+    -lambda x^T; they are checked against central finite differences on random stored entries
+    (each difference is a refactorization + solve) and entries of b. Timed: refactorization +
+    solve + transposed solve + scatter."""
+    import torch
+    import sparsedirectsolver as sds
+    from sparsedirectsolver import matrices as M
+    from sparsedirectsolver.adjoint import rrule_solve
+
+    k0 = args.grid if args.grid != 64 else 48
+    out = {"metric": "refactor+solve+adjoint ms, gradient vs central finite differences",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "data": "synthetic", "cases": []}
+    rng = np.random.default_rng(11)
+    for kind in ("spd", "lu"):
+        k = k0 if kind == "spd" else args.lu_grid
+        nfd = args.fd_entries if kind == "spd" else max(8, args.fd_entries // 4)
+        if kind == "spd":
+            A = M.variable_laplacian((k, k, k), seed=5)
+            Ls = M.lower_csr(A)
+            structure, view = "SPD", "L"
+        else:
+            A = M.convection_diffusion3d(k, peclet=10.0).tocsr()
+            A.sort_indices()
+            Ls = A
+            structure, view = "G", "F"
+        rp, cv, nz = M.csr_arrays(Ls)
+        n = A.shape[0]
+        bh, xstar = rng.standard_normal(n), rng.standard_normal(n)
+        b = torch.tensor(bh, dtype=torch.float64, device="cuda")
+        xs_t = torch.tensor(xstar, dtype=torch.float64, device="cuda")
+        for fp in ("Float64", "Float32"):
+            s = sds.DirectSolver((rp, cv, nz, n), structure, view)
+            sds.setparam(s, "user_perm", sds.grid_nd_perm(k, k, k))
+            sds.setparam(s, "amalgamation", (512, 0.25, 8))
+            if fp == "Float32":
+                sds.setparam(s, "factor_precision", "Float32")
+                sds.setparam(s, "ir_n_steps", 2)
+            sds.execute("analysis", s)
+            t0 = time.perf_counter()
+            sds.execute("factorization", s)
+            torch.cuda.synchronize()
+            print(f"{kind} {k}^3 {fp}: analysis + first factorization done ({time.perf_counter() - t0:.2f} s)",
+                  file=sys.stderr, flush=True)
+
+            def step():
+                sds.execute("refactorization", s)
+                x, pullback = rrule_solve(s, b)
+                return x, pullback(x - xs_t)
+
+            for _ in range(args.warmup):
+                step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                x, (bbar, abar) = step()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            xh, bbar, abar = x.cpu().numpy(), bbar.cpu().numpy(), abar.cpu().numpy()
+            Ah = A.tocsr()
+            berr = float(np.abs(Ah @ xh - bh).max() / (np.abs(Ah).max() * np.abs(xh).max() + np.abs(bh).max()))
+            print(f"  timed {ms:.2f} ms per step, backward error {berr:.2e}", file=sys.stderr, flush=True)
+
+            def loss(zv, bv):
+                sds.update(s, torch.tensor(zv, dtype=torch.float64, device="cuda"))
+                sds.execute("refactorization", s)
+                xx = torch.zeros_like(b)
+                sds.execute("solve", s, xx, torch.tensor(bv, dtype=torch.float64, device="cuda"))
+                return 0.5 * float(np.sum((xx.cpu().numpy() - xstar) ** 2))
+
+            # central differences: truncation O(h^2), cancellation O(eps |loss| / h); the LU case's loss
+            # is large, so it takes the larger step
+            hrel = 1e-6 if fp == "Float64" and kind == "spd" else 1e-4
+            picks = rng.choice(nz.shape[0], nfd, replace=False)
+            gmax = max(np.abs(abar).max(), 1e-300)
+            errs = []
+            for p in picks:
+                h = hrel * max(1.0, abs(nz[p]))
+                zp, zm = nz.copy(), nz.copy()
+                zp[p] += h
+                zm[p] -= h
+                errs.append(abs((loss(zp, bh) - loss(zm, bh)) / (2 * h) - abar[p]) / gmax)
+                if len(errs) % 8 == 0:
+                    print(f"  {len(errs)} finite differences, max rel err {max(errs):.2e}", file=sys.stderr, flush=True)
+            berrs = []
+            bmax = max(np.abs(bbar).max(), 1e-300)
+            for i in rng.choice(n, 8, replace=False):
+                h = hrel
+                bp, bm = bh.copy(), bh.copy()
+                bp[i] += h
+                bm[i] -= h
+                berrs.append(abs((loss(nz, bp) - loss(nz, bm)) / (2 * h) - bbar[i]) / bmax)
+            loss(nz, bh)
+            out["cases"].append({
+                "matrix": (f"3-D variable-coefficient 7-point {k}^3, SPD Cholesky" if kind == "spd" else
+                           f"3-D upwind convection-diffusion {k}^3 (Peclet 10), LU with threshold pivoting"),
+                "n": n, "factor": fp, "ms_per_step": round(ms, 3), "backward_error": berr,
+                "fd_entries": int(nfd), "fd_max_rel_err_A": float(max(errs)),
+                "fd_max_rel_err_b": float(max(berrs)), "fd_step_rel": hrel})
+            print(json.dumps(out["cases"][-1]), file=sys.stderr, flush=True)
+            del s
+    out["value"] = out["cases"][0]["ms_per_step"]
+    out["unit"] = "ms"
+    out["ms_per_step"] = out["value"]
+    out["dtype"] = "f64"
+    out["config"] = {"workload": f"differentiation of 0.5||x - x*||^2 through x = A^-1 b, {k0}^3 SPD and "
+                                 f"{args.lu_grid}^3 LU",
+                     "ordering": "geometric nested dissection", "amalgamation": [512, 0.25, 8]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,7 +346,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=5e10, help="flops of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--workload", choices=("laplacian3d", "batch"), default="laplacian3d")
+    ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint"), default="laplacian3d")
+    ap.add_argument("--fd-entries", type=int, default=64, help="adjoint workload: finite-difference checks")
+    ap.add_argument("--lu-grid", type=int, default=48, help="adjoint workload: grid side of the LU case")
     ap.add_argument("--members", type=int, default=1024)
     ap.add_argument("--ir-steps", type=int, default=2)
     args = ap.parse_args()
@@ -240,6 +358,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
     import torch
     import sparsedirectsolver as sds
