@@ -18,6 +18,11 @@
 #include "sds_common.h"
 #include "sds_kernels.h"
 
+// phase marks of the large-front step chain; empty unless a timing build defines them
+#ifndef SDS_STEP_MARK
+#define SDS_STEP_MARK(i)
+#endif
+
 namespace sds {
 
 // Cholesky pivot acceptance on the FP64 value of the pivot (src/numeric/front.jl:46-50):
@@ -191,50 +196,77 @@ struct Mfma<float> {
 // Cholesky of the 64 x 64 block D in LDS (D[j][i] = L(i, j), lower part; padded with the
 // identity past nb), four waves: per 16-column panel, wave 0 factors the panel's rows in
 // registers (lane i holds row kb + i; the pivot and the column entries move between lanes
-// by v_readlane; 1/sqrt is the hardware estimate with two Newton steps), then all four waves
-// apply the panel to the trailing block. rd gets the pivots' reciprocals (1 past nb); a
-// failed pivot (1-based within the block) is left in fail. Ends with a barrier.
+// by v_readlane; 1/sqrt is the hardware estimate with two Newton steps), then the four waves
+// apply the panel to the trailing lower 16 x 16 tiles on the matrix cores (K = 16). The upper
+// triangle of D holds scratch values afterwards. rd gets the pivots' reciprocals (1 past nb);
+// a failed pivot (1-based within the block) is left in fail. Ends with a barrier.
 template <typename T>
 __device__ __attribute__((always_inline)) inline void block_potrf(T (*D)[NB + 1], T* rd, int* fail, int nb, double eps) {
-    const int tid = threadIdx.x, ii = tid & (NB - 1), part = tid / NB;
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int tid = threadIdx.x, wv = tid >> 6;
     if (tid == 0) *fail = 0;
     __syncthreads();
     for (int kb = 0; kb < nb; kb += 16) {
-        if (part == 0) {
-            const int row = kb + ii;
+        // the lane index is re-read each panel so that the per-column lane masks are formed
+        // where they are used instead of being kept (and spilled) across the loop
+        int ii = tid & (NB - 1);
+        asm volatile("" : "+v"(ii));
+        if (wv == 0) {
+            const int row = kb + ii, rowc = min(row, NB - 1);
             T p[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) p[c] = row < NB ? D[kb + c][row] : T(0);
-            int bad = 0;
+            for (int c = 0; c < 16; ++c) {
+                const T v = D[kb + c][rowc];
+                p[c] = row < NB ? v : T(0);
+            }
+            T myr = T(1);         // lane c keeps the reciprocal of pivot c
+            double mypiv = 1.0;   // and pivot c, for the acceptance test after the panel
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 const double piv = (double)lane_bcast(p[c], c);
-                if (kb + c < nb && !chol_accept(piv, eps) && !bad) bad = kb + c + 1;
-                const T r = T(rsqrt_nr(fmax(piv, eps)));
-                const T d = T(fmax(piv, eps)) * r;
-                p[c] = ii > c ? p[c] * r : (ii == c ? d : p[c]);
-                if (ii == 0) rd[kb + c] = r;
+                const double pe = fmax(piv, eps);
+                const T r = T(rsqrt_nr(pe));
+                // rows below the pivot get L(i, c) = a(i, c) r, the pivot row sqrt(max(a, eps)); the
+                // rows above (the upper triangle) only hold scratch values
+                const bool isc = ii == c;
+                p[c] = (isc ? T(pe) : p[c]) * r;
+                myr = isc ? r : myr;
+                mypiv = isc ? piv : mypiv;
+                // each broadcast is consumed before the next is read, which keeps the scalar
+                // registers they pass through few (the scheduler would otherwise read them all
+                // ahead and spill them)
 #pragma unroll
-                for (int cc = c + 1; cc < 16; ++cc) p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
+                for (int cc = c + 1; cc < 16; ++cc) {
+                    p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
+            if (row < NB) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (row < NB) D[kb + c][row] = p[c];
-            if (ii == 0 && bad) *fail = bad;
+                for (int c = 0; c < 16; ++c) D[kb + c][row] = p[c];
+            }
+            if (ii < 16) rd[kb + ii] = myr;
+            // the first pivot of the panel that fails the acceptance test (1-based), if any
+            const unsigned long long bad = __ballot(ii < 16 && kb + ii < nb && !chol_accept(mypiv, eps));
+            if (ii == 0 && bad) *fail = kb + __ffsll((long long)bad);
         }
         __syncthreads();
         if (*fail) break;  // uniform after the barrier
-        const int j0 = kb + 16;
-        if (ii >= j0) {
-            T lr[16];
+        // trailing tiles (I, J), kb + 16 <= J <= I < 64: A(I, J) -= L(I, panel) L(J, panel)^T
+        const int nr = (NB - kb - 16) / 16, ntile = nr * (nr + 1) / 2;
+        const int lane = tid & 63, l16 = lane & 15, l4 = lane >> 4;
+        for (int q0 = wv; q0 < ntile; q0 += 4) {
+            int bi = 0, rem = q0;
+            while (rem > bi) rem -= ++bi;  // q0 = bi (bi + 1) / 2 + bj, bj <= bi
+            const int I = kb + 16 + 16 * bi, J = kb + 16 + 16 * rem;
+            acc_t acc;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) lr[c] = D[kb + c][ii];
-            for (int jj = j0 + part; jj <= ii; jj += 4) {
-                T acc = D[jj][ii];
+            for (int q = 0; q < 4; ++q) acc[q] = D[J + l16][I + Mfma<T>::row(l4, q)];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) acc = fma(-lr[c], D[kb + c][jj], acc);
-                D[jj][ii] = acc;
-            }
+            for (int ks = 0; ks < 4; ++ks)
+                acc = Mfma<T>::mac(-D[kb + 4 * ks + l4][I + l16], D[kb + 4 * ks + l4][J + l16], acc);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) D[J + l16][I + Mfma<T>::row(l4, q)] = acc[q];
         }
         __syncthreads();
     }
@@ -382,7 +414,10 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int l16 = lane & 15, l4 = lane >> 4;
-    // the destination values, read now so that their latency hides under the products
+    // the destination values, read now so that their latency hides under the products. The
+    // accumulators hold the transposed tile (element q of lane l of block (a, b) is entry
+    // (I0 + 32 wm + 16 a + l % 16, J0 + 32 wn + 16 b + row(l / 16, q))), so that each register
+    // covers 16 consecutive rows of one column: coalesced loads and stores
     T old[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -390,8 +425,8 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int j = J0 + wn * 32 + b * 16 + l16;
-                const int i = I0 + wm * 32 + a * 16 + Mfma<T>::row(l4, q);
+                const int i = I0 + wm * 32 + a * 16 + l16;
+                const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
                 const bool in = j < jlim && j >= jmin && i < f && i >= j;
                 old[a][b][q] = in ? *front_at(P, C, f, w, i, j) : T(0);
             }
@@ -426,26 +461,23 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
         for (int k4 = 0; k4 < kn; k4 += 4) {
             const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
             const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
-            acc[0][0] = Mfma<T>::mac(a0, b0, acc[0][0]);
-            acc[0][1] = Mfma<T>::mac(a0, b1, acc[0][1]);
-            acc[1][0] = Mfma<T>::mac(a1, b0, acc[1][0]);
-            acc[1][1] = Mfma<T>::mac(a1, b1, acc[1][1]);
+            acc[0][0] = Mfma<T>::mac(b0, a0, acc[0][0]);
+            acc[0][1] = Mfma<T>::mac(b1, a0, acc[0][1]);
+            acc[1][0] = Mfma<T>::mac(b0, a1, acc[1][0]);
+            acc[1][1] = Mfma<T>::mac(b1, a1, acc[1][1]);
         }
         __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int j = J0 + wn * 32 + b * 16 + l16;
-            if (j >= jlim || j < jmin) continue;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int i = I0 + wm * 32 + a * 16 + Mfma<T>::row(l4, q);
-                if (i >= f || i < j) continue;
-                *front_at(P, C, f, w, i, j) = old[a][b][q] - acc[a][b][q];
+                const int i = I0 + wm * 32 + a * 16 + l16;
+                const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
+                if (j < jlim && j >= jmin && i < f && i >= j) *front_at(P, C, f, w, i, j) = old[a][b][q] - acc[a][b][q];
             }
-        }
 }
 
 template <typename T>
@@ -511,6 +543,7 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
     }
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a, I0 = t.b;
     if (status[s] != 0) return;
+    SDS_STEP_MARK(0);
     const int nb = min(NB, w - k);
     const bool la = k + NB < w;  // a next block exists (then nb = NB)
     const int J = k + NB, nbn = la ? min(NB, w - J) : 0;
@@ -523,32 +556,83 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
         Bs[kk][c] = kk < nb && c < nb ? Li[e] : T(0);
     }
     const int rI = I0 + 16 * wv + l16, rJ = J + 16 * wv + l16;
+    // the diagonal workgroup's rows are the rows J: it reads them once
+    const bool two = la && I0 != J;
     T aI[16], aJ[16];
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) {
         const int col = 4 * ks + l4, cc = k + min(col, nb - 1);
         const T vi = P[(int64_t)cc * f + min(rI, f - 1)];
-        const T vj = P[(int64_t)cc * f + min(rJ, f - 1)];
         aI[ks] = rI < f && col < nb ? vi : T(0);
-        aJ[ks] = la && rJ < f && col < nb ? vj : T(0);
     }
-    __syncthreads();
-    acc_t accI[4], accJ[4];
+    if (two)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) accI[nt] = accJ[nt] = acc_t{};
+        for (int ks = 0; ks < 16; ++ks) {
+            const int col = 4 * ks + l4, cc = k + min(col, nb - 1);
+            const T vj = P[(int64_t)cc * f + min(rJ, f - 1)];
+            aJ[ks] = rJ < f && col < nb ? vj : T(0);
+        }
+    // the look-ahead's operands that do not depend on this step, read now so that their
+    // latency hides under the row solve: the previous step's rows J and I0, and the tile's old
+    // values of the next block's columns (all final before this launch: the trailing-update
+    // tiles of this launch touch neither)
+    const bool prev = k >= NB;
+    const int kp = max(k - NB, 0);
+    T pj[16], ap[16];
+    acc_t accC[4];  // transposed tile: element q of lane l is C(I0 + 16 wv + l % 16, J + 16 nt + row(l / 16, q))
+    if (la)  // then kp + 63 < w: the columns read lie in the panel
 #pragma unroll
-    for (int ks = 0; ks < 16; ++ks)
+        for (int u = 0; u < 16; ++u) {
+            const int e = tid + 256 * u, kk = e / NB, r = e - kk * NB;
+            const T v = P[(int64_t)(kp + kk) * f + min(J + r, f - 1)];
+            pj[u] = prev && J + r < f ? v : T(0);
+        }
+    if (two)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const T b = Bs[4 * ks + l4][16 * nt + l16];
-            accI[nt] = Mfma<T>::mac(aI[ks], b, accI[nt]);
-            if (la) accJ[nt] = Mfma<T>::mac(aJ[ks], b, accJ[nt]);
+        for (int ks = 0; ks < 16; ++ks) {
+            const T v = P[(int64_t)(kp + 4 * ks + l4) * f + min(rI, f - 1)];
+            ap[ks] = prev && rI < f ? v : T(0);
         }
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int r = 16 * wv + Mfma<T>::row(l4, q), c = 16 * nt + l16;
+            const int i = rI, j = J + 16 * nt + Mfma<T>::row(l4, q);
+            const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
+            accC[nt][q] = i < f && j < J + nbn && i >= j ? v : T(0);
+        }
+    __syncthreads();
+    SDS_STEP_MARK(1);
+    // the row solve as L(I0, b)^T = Linv_b A(I0, b)^T: transposed products, so that each
+    // accumulator register holds 16 consecutive rows of one column (coalesced stores); element
+    // q of lane l is L(I0 + 16 wv + l % 16, k + 16 nt + row(l / 16, q))
+    acc_t accI[4], accJ[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) accI[nt] = accJ[nt] = acc_t{};
+    if (two) {
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const T a = Bs[4 * ks + l4][16 * nt + l16];
+                accI[nt] = Mfma<T>::mac(a, aI[ks], accI[nt]);
+                accJ[nt] = Mfma<T>::mac(a, aJ[ks], accJ[nt]);
+            }
+    } else {
+        // the diagonal workgroup's rows are the rows J (or there is no next block): one product
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) accI[nt] = Mfma<T>::mac(Bs[4 * ks + l4][16 * nt + l16], aI[ks], accI[nt]);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) accJ[nt] = accI[nt];
+    }
+    const int r = 16 * wv + l16;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = 16 * nt + Mfma<T>::row(l4, q);
             if (I0 + r < f && c < nb) {
                 if (la && I0 == J)
                     LN[(int64_t)(cg + k + c) * NB + r] = accI[nt][q];
@@ -560,48 +644,40 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
                 Ln[r][c] = J + r < f ? accJ[nt][q] : T(0);
             }
         }
+    SDS_STEP_MARK(2);
     if (!la) return;
-    // the previous step's rows of the next block, for its part of the look-ahead
-    const bool prev = k >= NB;
+    // the previous step's rows of the next block into Bs, for its part of the look-ahead
     __syncthreads();
     if (prev)
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int kk = e / NB, r = e - kk * NB;
-            Bs[r][kk] = J + r < f ? P[(int64_t)(k - NB + kk) * f + J + r] : T(0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = tid + 256 * u, kk = e / NB, rr = e - kk * NB;
+            Bs[rr][kk] = pj[u];
         }
-    // the tile's old values of the next block's columns
-    acc_t accC[4];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = I0 + 16 * wv + Mfma<T>::row(l4, q), j = J + 16 * nt + l16;
-            accC[nt][q] = i < f && j < J + nbn && i >= j ? P[(int64_t)j * f + i] : T(0);
-        }
-    T ap[16];
-#pragma unroll
-    for (int ks = 0; ks < 16; ++ks) {
-        const T v = P[(int64_t)(max(k - NB, 0) + 4 * ks + l4) * f + min(rI, f - 1)];
-        ap[ks] = prev && rI < f ? -v : T(0);
-    }
     __syncthreads();
+    if (!two)  // the diagonal workgroup: L(I0, b - 1) = L(J, b - 1) is in Bs
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) ap[ks] = Bs[r][4 * ks + l4];
+    SDS_STEP_MARK(3);
+    // C(I0.., J..)^T -= L(J, b) L(I0, b)^T + L(J, b - 1) L(I0, b - 1)^T
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) {
-        const T a = -Lc[16 * wv + l16][4 * ks + l4];
+        const T bb = Lc[r][4 * ks + l4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(a, Ln[16 * nt + l16][4 * ks + l4], accC[nt]);
+        for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(-Ln[16 * nt + l16][4 * ks + l4], bb, accC[nt]);
     }
     if (prev)
 #pragma unroll
         for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(ap[ks], Bs[16 * nt + l16][4 * ks + l4], accC[nt]);
+            for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(-Bs[16 * nt + l16][4 * ks + l4], ap[ks], accC[nt]);
+    SDS_STEP_MARK(4);
     if (I0 != J) {
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int i = I0 + 16 * wv + Mfma<T>::row(l4, q), j = J + 16 * nt + l16;
+                const int i = rI, j = J + 16 * nt + Mfma<T>::row(l4, q);
                 if (i < f && j < J + nbn) P[(int64_t)j * f + i] = accC[nt][q];
             }
         return;
@@ -612,7 +688,7 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 16 * wv + Mfma<T>::row(l4, q), j = 16 * nt + l16;
+            const int i = r, j = 16 * nt + Mfma<T>::row(l4, q);
             if (i >= nbn && J + i < f && j < nbn) P[(int64_t)(J + j) * f + J + i] = accC[nt][q];
         }
     __syncthreads();
@@ -621,16 +697,20 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 16 * wv + Mfma<T>::row(l4, q), j = 16 * nt + l16;
+            const int i = r, j = 16 * nt + Mfma<T>::row(l4, q);
             D[j][i] = (i < nbn && j < nbn) ? (i >= j ? accC[nt][q] : T(0)) : (i == j ? T(1) : T(0));
         }
+    SDS_STEP_MARK(5);
     block_potrf<T>(D, rd, &fail, nbn, eps);
+    SDS_STEP_MARK(6);
     if (fail) {
         if (tid == 0) status[s] = J + fail;
         return;
     }
     block_inverse_mfma<T>(D, rd, Lc, reinterpret_cast<T(*)[16][17]>(&Ln[0][0]));
+    SDS_STEP_MARK(7);
     block_store<T>(D, Lc, P, f, J, nbn, DB + (int64_t)(cg + J) * NB);
+    SDS_STEP_MARK(8);
 }
 
 
