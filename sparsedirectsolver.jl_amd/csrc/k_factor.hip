@@ -386,21 +386,24 @@ __global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restri
 // on the lower triangle of the front, A the panel columns k0..k1-1. mode 0: step update of
 // the panel columns (k1 = min(k0 + NB, w)); mode 1: contribution block (k0 = 0, k1 = w).
 // task: (s, I0, J0, k0). 256 threads = 4 waves, each wave a 32x32 quadrant of the tile as
-// 2x2 blocks of 16x16 accumulators; K in chunks of 64 staged through LDS (one chunk holds a
-// whole step), the next chunk prefetched into registers while the current one is multiplied,
-// and the tile's old values loaded at the start. MFMA 16x16x4 operand layout (checked on
-// gfx950): lane l gives A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator element q of lane l is D[row(l / 16, q)][l % 16] with
-// row = 4 (l / 16) + q for the f32 instruction and 4 q + l / 16 for the f64 one.
-constexpr int UPD_KC = 64;  // a step's whole K (mode 0) in one chunk
+// 2x2 blocks of 16x16 accumulators; K in chunks of KC staged through LDS (wave v stages rows
+// v, v + 4, ... of a chunk), the next chunk's loads issued between the current chunk's
+// products. EARLY: the tile's old values are read at the start (one workgroup per CU: their
+// latency hides under the products) rather than at the end (several workgroups per CU, fewer
+// live registers). The accumulators hold the transposed tile, so that each register covers 16
+// consecutive rows of one column (coalesced loads and stores). MFMA 16x16x4 operand layout
+// (checked on gfx950): lane l gives A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator
+// element q of lane l is D[row(l / 16, q)][l % 16] with row = 4 (l / 16) + q for the f32
+// instruction and 4 q + l / 16 for the f64 one.
+constexpr int UPD_KC = 32;  // chunk of the contribution-block update (four workgroups per CU)
 
-template <typename T>
+template <typename T, int KC, bool EARLY>
 __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& S, const Task t, T* __restrict__ factor,
                                                                   T* __restrict__ cbpool,
                                                                   const int32_t* __restrict__ status, int mode,
                                                                   T (*As)[TILE], T (*Bs)[TILE]) {
-    constexpr int KC = UPD_KC;
-    constexpr int PER = KC * TILE / 256;   // entries of A (and of B) per thread and chunk
+    constexpr int PER = KC * TILE / 256;  // chunk rows per wave
+    constexpr int EVERY = KC / 4 / PER;   // products per issued pair of next-chunk loads
     typedef typename Mfma<T>::acc_t acc_t;
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
     if (status[s] != 0) return;
@@ -414,57 +417,73 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
     const int l16 = lane & 15, l4 = lane >> 4;
-    // the destination values, read now so that their latency hides under the products. The
-    // accumulators hold the transposed tile (element q of lane l of block (a, b) is entry
-    // (I0 + 32 wm + 16 a + l % 16, J0 + 32 wn + 16 b + row(l / 16, q))), so that each register
-    // covers 16 consecutive rows of one column: coalesced loads and stores
+    // element q of lane l of block (a, b) is entry (I0 + 32 wm + 16 a + l % 16, J0 + 32 wn + 16 b + row(l / 16, q))
     T old[2][2][4];
+    if (EARLY)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = I0 + wm * 32 + a * 16 + l16;
-                const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
-                const bool in = j < jlim && j >= jmin && i < f && i >= j;
-                old[a][b][q] = in ? *front_at(P, C, f, w, i, j) : T(0);
-            }
+                for (int q = 0; q < 4; ++q) {
+                    const int i = I0 + wm * 32 + a * 16 + l16;
+                    const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
+                    const bool in = j < jlim && j >= jmin && i < f && i >= j;
+                    old[a][b][q] = in ? *front_at(P, C, f, w, i, j) : T(0);
+                }
     acc_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-    // this thread's share of a chunk: PER entries of A and of B, (kk, r) = (q / 64, q % 64)
+    // wave wv stages chunk rows kk = wv + 4 u (panel column kc + kk), lanes along the tile rows
+    const bool rowA = I0 + lane < f, rowB = J0 + lane < f;
+    const int64_t s4 = 4 * (int64_t)f;
+    const T* pA = P + (int64_t)(k0 + wv) * f + min(I0 + lane, f - 1);
+    const T* pB = P + (int64_t)(k0 + wv) * f + min(J0 + lane, f - 1);
     T pa[PER], pb[PER];
-    auto fetch = [&](int kc) {
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
-            const int kcol = kc + kk;
-            const bool kin = kcol < k1;
-            pa[u] = (kin && I0 + r < f) ? P[(int64_t)kcol * f + I0 + r] : T(0);
-            pb[u] = (kin && J0 + r < f) ? P[(int64_t)kcol * f + J0 + r] : T(0);
-        }
-    };
-    fetch(k0);
+    for (int u = 0; u < PER; ++u) {
+        const bool kin = k0 + wv + 4 * u < k1;
+        pa[u] = kin && rowA ? pA[u * s4] : T(0);
+        pb[u] = kin && rowB ? pB[u * s4] : T(0);
+    }
     for (int kc = k0; kc < k1; kc += KC) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int q = tid + 256 * u, kk = q >> 6, r = q & 63;
-            As[kk][r] = pa[u];
-            Bs[kk][r] = pb[u];
+            As[wv + 4 * u][lane] = pa[u];
+            Bs[wv + 4 * u][lane] = pb[u];
         }
         __syncthreads();
-        if (kc + KC < k1) fetch(kc + KC);
+        pA += KC * (int64_t)f;
+        pB += KC * (int64_t)f;
         const int kn = min(KC, k1 - kc);
-        for (int k4 = 0; k4 < kn; k4 += 4) {
-            const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
-            const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
-            acc[0][0] = Mfma<T>::mac(b0, a0, acc[0][0]);
-            acc[0][1] = Mfma<T>::mac(b1, a0, acc[0][1]);
-            acc[1][0] = Mfma<T>::mac(b0, a1, acc[1][0]);
-            acc[1][1] = Mfma<T>::mac(b1, a1, acc[1][1]);
+        const int kcn = kc + KC;
+        if (kn == KC) {
+#pragma unroll
+            for (int k4 = 0; k4 < KC; k4 += 4) {
+                if ((k4 / 4) % EVERY == 0) {
+                    const int u = k4 / 4 / EVERY;
+                    const bool kin = kcn + wv + 4 * u < k1;
+                    pa[u] = kin && rowA ? pA[u * s4] : T(0);
+                    pb[u] = kin && rowB ? pB[u * s4] : T(0);
+                }
+                const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
+                const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
+                acc[0][0] = Mfma<T>::mac(b0, a0, acc[0][0]);
+                acc[0][1] = Mfma<T>::mac(b1, a0, acc[0][1]);
+                acc[1][0] = Mfma<T>::mac(b0, a1, acc[1][0]);
+                acc[1][1] = Mfma<T>::mac(b1, a1, acc[1][1]);
+            }
+        } else {  // the last, partial chunk (nothing left to load)
+            for (int k4 = 0; k4 < kn; k4 += 4) {
+                const T a0 = As[k4 + l4][wm * 32 + l16], a1 = As[k4 + l4][wm * 32 + 16 + l16];
+                const T b0 = Bs[k4 + l4][wn * 32 + l16], b1 = Bs[k4 + l4][wn * 32 + 16 + l16];
+                acc[0][0] = Mfma<T>::mac(b0, a0, acc[0][0]);
+                acc[0][1] = Mfma<T>::mac(b1, a0, acc[0][1]);
+                acc[1][0] = Mfma<T>::mac(b0, a1, acc[1][0]);
+                acc[1][1] = Mfma<T>::mac(b1, a1, acc[1][1]);
+            }
         }
         __syncthreads();
     }
@@ -476,7 +495,10 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
             for (int q = 0; q < 4; ++q) {
                 const int i = I0 + wm * 32 + a * 16 + l16;
                 const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
-                if (j < jlim && j >= jmin && i < f && i >= j) *front_at(P, C, f, w, i, j) = old[a][b][q] - acc[a][b][q];
+                if (j < jlim && j >= jmin && i < f && i >= j) {
+                    T* d = front_at(P, C, f, w, i, j);
+                    *d = (EARLY ? old[a][b][q] : *d) - acc[a][b][q];
+                }
             }
 }
 
@@ -486,7 +508,7 @@ __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* _
                                                         const int32_t* __restrict__ status, int mode) {
     __shared__ T As[UPD_KC][TILE];
     __shared__ T Bs[UPD_KC][TILE];
-    update_tile<T>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
+    update_tile<T, UPD_KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
 }
 
 // the rows [k + 64, k + 128) of a step's columns, kept in LN by k_big_step, into the panel.
@@ -533,8 +555,8 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
     __shared__ int fail;
     const Task t = tasks[blockIdx.x];
     if ((int)blockIdx.x >= nstep + ncopy) {
-        update_tile<T>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Lc[0][0]),
-                       reinterpret_cast<T(*)[TILE]>(&Ln[0][0]));
+        update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Lc[0][0]),
+                                 reinterpret_cast<T(*)[TILE]>(&Ln[0][0]));
         return;
     }
     if ((int)blockIdx.x >= nstep) {
