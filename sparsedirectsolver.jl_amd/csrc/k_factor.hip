@@ -118,8 +118,8 @@ __global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* 
     for (int j = t.a; j < j1; ++j) {
         if (j < w) {
             for (int i = tid; i < f; i += nt) P[(int64_t)j * f + i] = T(0);
-        } else {
-            for (int i = tid; i < m; i += nt) C[(int64_t)(j - w) * m + i] = T(0);
+        } else {  // the lower triangle: the only part of a contribution block that is read
+            for (int i = (j - w) + tid; i < m; i += nt) C[(int64_t)(j - w) * m + i] = T(0);
         }
     }
     __syncthreads();
@@ -496,8 +496,10 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
                 const int i = I0 + wm * 32 + a * 16 + l16;
                 const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
                 if (j < jlim && j >= jmin && i < f && i >= j) {
+                    // mode 1 writes the contribution block whole (the children's entries are
+                    // added to it afterwards)
                     T* d = front_at(P, C, f, w, i, j);
-                    *d = (EARLY ? old[a][b][q] : *d) - acc[a][b][q];
+                    *d = (EARLY ? old[a][b][q] : mode == 1 ? T(0) : *d) - acc[a][b][q];
                 }
             }
 }

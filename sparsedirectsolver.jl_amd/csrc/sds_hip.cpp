@@ -129,6 +129,7 @@ struct sds_handle {
     int method = M_CHOLESKY;
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
+    std::vector<int32_t> cb_split;  // per child: its contribution-block columns that land in the parent's panel
     Launch inv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the diagonal blocks (Cholesky)
     SolveStep* d_steps = nullptr;            // solve steps of the Cholesky sweeps
     Launch sinv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the wide steps' diagonal blocks
@@ -472,14 +473,40 @@ int build_schedule(sds_handle* h, int dtype) {
         L.plan.push_back({kind, (int64_t)tasks.size(), (int)t.size(), flops, bytes});
         tasks.insert(tasks.end(), t.begin(), t.end());
     };
+    // Cholesky big fronts: the contribution block is written whole by its update (no zeroing),
+    // so the children's entries are added in two parts: the columns that land in the panel
+    // before the factorization, the contribution-block columns after the update
+    const bool chol = h->method == M_CHOLESKY;
+    auto push_extend = [&](Level& L, bool cb_part, int kind) {
+        int maxch = 0;
+        for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
+        for (int r = 0; r < maxch; ++r) {  // one launch per child rank: each parent entry has one writer
+            std::vector<Task> t;
+            double by = 0;
+            for (int32_t s : L.big) {
+                if (h->child_ptr[s] + r >= h->child_ptr[s + 1]) continue;
+                int32_t c = h->child_list[h->child_ptr[s] + r];
+                if (cb_off[c] < 0) continue;
+                const int mc = h->fnrows[c] - h->fncols[c];
+                const int j0 = chol && cb_part ? h->cb_split[c] : 0, j1 = chol && !cb_part ? h->cb_split[c] : mc;
+                for (int jj = j0; jj < j1; jj += 4) t.push_back({s, c, jj, std::min(4, j1 - jj)});
+                // read the child's block, read + write the parent
+                const double e = kind == L_LU_EXTEND ? (double)mc * mc
+                                                     : ((double)(mc - j0) * (mc - j0 + 1) - (double)(mc - j1) * (mc - j1 + 1)) / 2;
+                by += 3.0 * dbl_es * e;
+            }
+            push_launch(L, kind, t, 0, by);
+        }
+    };
     for (auto& L : h->levels) {
         if (L.big.empty()) continue;
         std::vector<Task> t;
         double fl = 0, by = 0;
         for (int32_t s : L.big) {
             double f = h->fnrows[s], w = h->fncols[s], m = f - w;
-            for (int c = 0; c < h->fnrows[s]; c += 8) t.push_back({s, c, 8, 0});
-            by += dbl_es * (f * w + m * m) + 16.0 * (double)h->asm_count[s];
+            const int cols = chol ? h->fncols[s] : h->fnrows[s];
+            for (int c = 0; c < cols; c += 8) t.push_back({s, c, 8, 0});
+            by += dbl_es * (f * w + (chol ? 0.0 : m * (m + 1) / 2)) + 16.0 * (double)h->asm_count[s];
         }
         const bool lu = h->method == M_LU;
         if (lu) {
@@ -491,22 +518,7 @@ int build_schedule(sds_handle* h, int dtype) {
         } else {
             push_launch(L, L_ZERO_SCATTER, t, 0, by);
         }
-        int maxch = 0;
-        for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
-        for (int r = 0; r < maxch; ++r) {
-            t.clear();
-            by = 0;
-            for (int32_t s : L.big) {
-                if (h->child_ptr[s] + r >= h->child_ptr[s + 1]) continue;
-                int32_t c = h->child_list[h->child_ptr[s] + r];
-                if (cb_off[c] < 0) continue;
-                int mc = h->fnrows[c] - h->fncols[c];
-                for (int jj = 0; jj < mc; jj += 4) t.push_back({s, c, jj, 4});
-                // read the child's block, read + write the parent
-                by += 3.0 * dbl_es * (lu ? (double)mc * mc : (double)mc * (mc + 1) / 2);
-            }
-            push_launch(L, lu ? L_LU_EXTEND : L_EXTEND_ADD, t, 0, by);
-        }
+        push_extend(L, false, lu ? L_LU_EXTEND : L_EXTEND_ADD);
         int wmax = 0;
         for (int32_t s : L.big) wmax = std::max(wmax, h->fncols[s]);
         if (h->method != M_CHOLESKY) {
@@ -596,6 +608,7 @@ int build_schedule(sds_handle* h, int dtype) {
             by += dbl_es * (double)(f - w) * w;  // L21 read once
         }
         push_launch(L, h->method == M_CHOLESKY ? L_UPDATE_CB : lu ? L_LU_CB : L_LDLT_CB, t, fl, by);
+        if (chol) push_extend(L, true, L_EXTEND_ADD);
     }
     void* p;
     int rc;
@@ -1194,6 +1207,16 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     for (int64_t s = 0; s <= ns; ++s) relind_ptr[s] = riptr[s] - 1;
     std::vector<int32_t> relind(relind_ptr[ns]);
     for (int64_t q = 0; q < relind_ptr[ns]; ++q) relind[q] = (int32_t)(rind[q] - 1);
+    // relind is increasing: a child's first cb_split columns land in the parent's panel
+    // columns, the rest in its contribution block
+    h->cb_split.assign(ns, 0);
+    for (int64_t c = 0; c < ns; ++c) {
+        const int32_t p = h->snparent[c];
+        if (p < 0) continue;
+        int64_t q = relind_ptr[c];
+        while (q < relind_ptr[c + 1] && relind[q] < h->fncols[p]) ++q;
+        h->cb_split[c] = (int32_t)(q - relind_ptr[c]);
+    }
     // assembly entries grouped by front (owner-pull), with local panel offsets
     std::vector<int64_t> asm_ptr(ns + 1), asm_src, asm_dst, acol_ptr(n + 1, 0);
     for (int64_t s = 0; s <= ns; ++s) asm_ptr[s] = aptr[s] - 1;
