@@ -138,36 +138,40 @@ __global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* 
 }
 
 // extend-add of one child per front (the launch covers child rank r of every front);
-// task: (s, child, jj0, njj)
+// task: (s, child, jj0, njj), njj <= EA_COLS child columns. A thread takes one child row at a
+// time (its parent row read once) and all the task's columns of that row: the child reads and
+// the parent read-modify-writes are coalesced along the rows, several in flight per thread.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __restrict__ tasks,
                                                        T* __restrict__ factor, T* __restrict__ cbpool) {
     const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int c = t.a, mc = S.fnrows[c] - S.fncols[c];
     T* P = factor + S.panel_off[s];
     T* Cs = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
     const T* Cc = cbpool + S.cb_off[c];
     const int32_t* ri = S.relind + S.relind_ptr[c];
-    const int jend = min(t.b + t.c, mc);
-    constexpr int U = 8;  // entries per thread in flight: all loads issued before the stores
-    for (int jj = t.b; jj < jend; ++jj) {
-        const int pj = ri[jj];
-        for (int ii0 = jj + threadIdx.x; ii0 < mc; ii0 += 256 * U) {
-            T v[U], old[U];
-            T* dst[U];
+    const int jj0 = t.b, nj = min(t.c, mc - jj0);
+    int pj[EA_COLS];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int ii = ii0 + 256 * u;
-                dst[u] = ii < mc ? front_at(P, Cs, f, w, ri[ii], pj) : nullptr;
-                v[u] = ii < mc ? Cc[(int64_t)jj * mc + ii] : T(0);
-            }
+    for (int q = 0; q < EA_COLS; ++q) pj[q] = ri[jj0 + min(q, nj - 1)];
+    for (int ii = jj0 + threadIdx.x; ii < mc; ii += 256) {
+        const int pi = ri[ii];
+        T v[EA_COLS], old[EA_COLS];
+        T* dst[EA_COLS];
 #pragma unroll
-            for (int u = 0; u < U; ++u) old[u] = dst[u] ? *dst[u] : T(0);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (dst[u]) *dst[u] = old[u] + v[u];
+        for (int q = 0; q < EA_COLS; ++q) {
+            const int jj = jj0 + q;
+            const bool in = q < nj && ii >= jj;
+            // parent entry (pi, pj): panel column pj < w, else the contribution block
+            dst[q] = !in ? nullptr : pj[q] < w ? P + (int64_t)pj[q] * f + pi : Cs + (int64_t)(pj[q] - w) * m + (pi - w);
+            v[q] = in ? Cc[(int64_t)jj * mc + ii] : T(0);
         }
+#pragma unroll
+        for (int q = 0; q < EA_COLS; ++q) old[q] = dst[q] ? *dst[q] : T(0);
+#pragma unroll
+        for (int q = 0; q < EA_COLS; ++q)
+            if (dst[q]) *dst[q] = old[q] + v[q];
     }
 }
 

@@ -42,6 +42,7 @@ enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4
 constexpr int NB = 64;     // pivot block of the blocked front factorization
 constexpr int TILE = 64;   // Schur-update tile
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
+constexpr int EA_COLS = 8;      // child columns per extend-add task
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
 // Cholesky sweeps with inverted diagonal blocks: a front's columns are solved in steps of at

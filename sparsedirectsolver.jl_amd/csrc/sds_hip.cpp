@@ -489,7 +489,8 @@ int build_schedule(sds_handle* h, int dtype) {
                 if (cb_off[c] < 0) continue;
                 const int mc = h->fnrows[c] - h->fncols[c];
                 const int j0 = chol && cb_part ? h->cb_split[c] : 0, j1 = chol && !cb_part ? h->cb_split[c] : mc;
-                for (int jj = j0; jj < j1; jj += 4) t.push_back({s, c, jj, std::min(4, j1 - jj)});
+                const int cols = kind == L_LU_EXTEND ? 4 : EA_COLS;
+                for (int jj = j0; jj < j1; jj += cols) t.push_back({s, c, jj, std::min(cols, j1 - jj)});
                 // read the child's block, read + write the parent
                 const double e = kind == L_LU_EXTEND ? (double)mc * mc
                                                      : ((double)(mc - j0) * (mc - j0 + 1) - (double)(mc - j1) * (mc - j1 + 1)) / 2;
