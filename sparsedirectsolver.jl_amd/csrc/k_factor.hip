@@ -138,9 +138,10 @@ __global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* 
 }
 
 // extend-add of one child per front (the launch covers child rank r of every front);
-// task: (s, child, jj0, njj), njj <= EA_COLS child columns. A thread takes one child row at a
-// time (its parent row read once) and all the task's columns of that row: the child reads and
-// the parent read-modify-writes are coalesced along the rows, several in flight per thread.
+// task: (s, child, jj0, njj + 16 r0): child columns jj0 .. jj0 + njj - 1 (njj <= EA_COLS),
+// child rows r0 .. r0 + EA_ROWS - 1 (and below the diagonal). A thread takes one child row at
+// a time (its parent row read once) and all the task's columns of that row: the child reads
+// and the parent read-modify-writes are coalesced along the rows, several in flight per thread.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __restrict__ tasks,
                                                        T* __restrict__ factor, T* __restrict__ cbpool) {
@@ -151,11 +152,12 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     T* Cs = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
     const T* Cc = cbpool + S.cb_off[c];
     const int32_t* ri = S.relind + S.relind_ptr[c];
-    const int jj0 = t.b, nj = min(t.c, mc - jj0);
+    const int jj0 = t.b, nj = min(t.c & 15, mc - jj0), r0 = t.c >> 4;
+    const int iend = min(mc, r0 + EA_ROWS);
     int pj[EA_COLS];
 #pragma unroll
     for (int q = 0; q < EA_COLS; ++q) pj[q] = ri[jj0 + min(q, nj - 1)];
-    for (int ii = jj0 + threadIdx.x; ii < mc; ii += 256) {
+    for (int ii = max(jj0, r0) + threadIdx.x; ii < iend; ii += 256) {
         const int pi = ri[ii];
         T v[EA_COLS], old[EA_COLS];
         T* dst[EA_COLS];

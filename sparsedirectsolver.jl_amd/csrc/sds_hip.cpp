@@ -489,8 +489,13 @@ int build_schedule(sds_handle* h, int dtype) {
                 if (cb_off[c] < 0) continue;
                 const int mc = h->fnrows[c] - h->fncols[c];
                 const int j0 = chol && cb_part ? h->cb_split[c] : 0, j1 = chol && !cb_part ? h->cb_split[c] : mc;
-                const int cols = kind == L_LU_EXTEND ? 4 : EA_COLS;
-                for (int jj = j0; jj < j1; jj += cols) t.push_back({s, c, jj, std::min(cols, j1 - jj)});
+                if (kind == L_LU_EXTEND) {
+                    for (int jj = j0; jj < j1; jj += 4) t.push_back({s, c, jj, std::min(4, j1 - jj)});
+                } else {
+                    for (int jj = j0; jj < j1; jj += EA_COLS)
+                        for (int r0 = jj - jj % EA_ROWS; r0 < mc; r0 += EA_ROWS)
+                            t.push_back({s, c, jj, std::min(EA_COLS, j1 - jj) + 16 * r0});
+                }
                 // read the child's block, read + write the parent
                 const double e = kind == L_LU_EXTEND ? (double)mc * mc
                                                      : ((double)(mc - j0) * (mc - j0 + 1) - (double)(mc - j1) * (mc - j1 + 1)) / 2;
