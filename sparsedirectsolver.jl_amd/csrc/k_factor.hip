@@ -540,11 +540,14 @@ __device__ inline void copy_ln(const DevSym& S, const Task t, T* __restrict__ fa
 //     all loaded before the MFMAs, so the rows are replaced in place);
 //  2. look-ahead, when a next block b + 1 exists (J = k + 64): the tile's rows of the next
 //     block's columns get C(I, J) -= L(I, b) L(J, b)^T, and also -= L(I, b-1) L(J, b-1)^T
-//     when b > 0 (the steps before b-1 reach block b + 1 through the trailing-update launches,
-//     which cover the blocks from three past their step). L(J, b) is recomputed by every
-//     workgroup from A(J, b);
+//     when b > 0 and I0 > J (the steps before b-1 reach block b + 1 through the trailing-update
+//     tiles, which cover the blocks from three past their step). L(J, b) is recomputed by
+//     every workgroup from A(J, b);
 //  3. the workgroup whose tile is the next diagonal block (I0 = J) factors it (Cholesky and
-//     inverse, as k_big_diag) for step b + 1.
+//     inverse, as k_big_diag) for step b + 1. Its tile got the step b-1 part one launch
+//     earlier (4.), so its chain holds the step's own product only;
+//  4. the workgroup of the rows after (I0 = J + 64) also updates its own diagonal tile, block
+//     b + 2, with steps b and b-1; the trailing-update tiles of step b-1 leave that tile out.
 // The rows J of the step's columns are read (unsolved) by every workgroup, so the workgroup
 // that solves them keeps them in LN (NB entries per factor column) instead of the panel.
 // The next step's launch also carries this step's copies of those rows (copy_ln) and its
@@ -602,15 +605,18 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
             const T vj = P[(int64_t)cc * f + min(rJ, f - 1)];
             aJ[ks] = rJ < f && col < nb ? vj : T(0);
         }
-    // the look-ahead's operands that do not depend on this step, read now so that their
-    // latency hides under the row solve: the previous step's rows J and I0, and the tile's old
-    // values of the next block's columns (all final before this launch: the trailing-update
-    // tiles of this launch touch neither)
+    __syncthreads();
+    SDS_STEP_MARK(1);
+    // the look-ahead's operands that do not depend on this step, read after the barrier (which
+    // waits only for Linv_b and the step's rows) so that their latency hides under the row
+    // solve: the previous step's rows J and I0, and the tile's old values of the next block's
+    // columns (all final before this launch: the trailing-update tiles of this launch touch
+    // neither)
     const bool prev = k >= NB;
     const int kp = max(k - NB, 0);
     T pj[16], ap[16];
     acc_t accC[4];  // transposed tile: element q of lane l is C(I0 + 16 wv + l % 16, J + 16 nt + row(l / 16, q))
-    if (la)  // then kp + 63 < w: the columns read lie in the panel
+    if (two)  // then kp + 63 < w: the columns read lie in the panel
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int e = tid + 256 * u, kk = e / NB, r = e - kk * NB;
@@ -631,8 +637,6 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
             const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
             accC[nt][q] = i < f && j < J + nbn && i >= j ? v : T(0);
         }
-    __syncthreads();
-    SDS_STEP_MARK(1);
     // the row solve as L(I0, b)^T = Linv_b A(I0, b)^T: transposed products, so that each
     // accumulator register holds 16 consecutive rows of one column (coalesced stores); element
     // q of lane l is L(I0 + 16 wv + l % 16, k + 16 nt + row(l / 16, q))
@@ -676,27 +680,27 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
         }
     SDS_STEP_MARK(2);
     if (!la) return;
-    // the previous step's rows of the next block into Bs, for its part of the look-ahead
+    // the previous step's rows of the next block into Bs, for its part of the look-ahead (the
+    // diagonal tile has that part already: the step before applied it, below)
     __syncthreads();
-    if (prev)
+    const bool pv = prev && two;
+    if (pv) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int e = tid + 256 * u, kk = e / NB, rr = e - kk * NB;
             Bs[rr][kk] = pj[u];
         }
-    __syncthreads();
-    if (!two)  // the diagonal workgroup: L(I0, b - 1) = L(J, b - 1) is in Bs
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks) ap[ks] = Bs[r][4 * ks + l4];
+        __syncthreads();
+    }
     SDS_STEP_MARK(3);
-    // C(I0.., J..)^T -= L(J, b) L(I0, b)^T + L(J, b - 1) L(I0, b - 1)^T
+    // C(I0.., J..)^T -= L(J, b) L(I0, b)^T (+ L(J, b - 1) L(I0, b - 1)^T off the diagonal tile)
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) {
         const T bb = Lc[r][4 * ks + l4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(-Ln[16 * nt + l16][4 * ks + l4], bb, accC[nt]);
     }
-    if (prev)
+    if (pv)
 #pragma unroll
         for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
@@ -709,6 +713,48 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
             for (int q = 0; q < 4; ++q) {
                 const int i = rI, j = J + 16 * nt + Mfma<T>::row(l4, q);
                 if (i < f && j < J + nbn) P[(int64_t)j * f + i] = accC[nt][q];
+            }
+        const int I = J + NB;  // the block after next
+        if (I0 != I || I >= w) return;
+        // this workgroup's rows are the block after next: its diagonal tile gets steps b and
+        // b - 1 here (the trailing-update tiles of step b - 1 in this launch leave it out), so
+        // that the next step's diagonal workgroup only applies its own step
+        const int nb3 = min(NB, w - I);
+        acc_t accD[4];  // element q of lane l is C(I + 16 wv + l % 16, I + 16 nt + row(l / 16, q))
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = rI, j = I + 16 * nt + Mfma<T>::row(l4, q);
+                const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
+                accD[nt][q] = i < f && j < I + nb3 && i >= j ? v : T(0);
+            }
+        __syncthreads();  // Ln is free: L(I, b - 1) into it as rows
+        if (prev)
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) Ln[r][4 * ks + l4] = ap[ks];
+        __syncthreads();
+        // step b: the K index runs over accI's registers (element q of accI[nt'] is
+        // L(I + r, k + 16 nt' + row(l / 16, q))), the rows of L(I, b) from Lc
+#pragma unroll
+        for (int ntp = 0; ntp < 4; ++ntp)
+#pragma unroll
+            for (int qp = 0; qp < 4; ++qp) {
+                const int kx = 16 * ntp + Mfma<T>::row(l4, qp);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) accD[nt] = Mfma<T>::mac(-Lc[16 * nt + l16][kx], accI[ntp][qp], accD[nt]);
+            }
+        if (prev)
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) accD[nt] = Mfma<T>::mac(-Ln[16 * nt + l16][4 * ks + l4], ap[ks], accD[nt]);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = rI, j = I + 16 * nt + Mfma<T>::row(l4, q);
+                if (i < f && j < I + nb3 && i >= j) P[(int64_t)j * f + i] = accD[nt][q];
             }
         return;
     }

@@ -577,10 +577,18 @@ int build_schedule(sds_handle* h, int dtype) {
                         ft += 2.0 * e * (k >= NB ? 2 * NB : NB);
                         bt += 2.0 * dbl_es * e;
                     }
+                    if (J + NB < w) {  // the diagonal tile after next, by the row tile of its rows
+                        double e = tile_entries(J + NB, J + NB, f, J + NB, w);
+                        ft += 2.0 * e * (k >= NB ? 2 * NB : NB);
+                        bt += 2.0 * dbl_es * e;
+                    }
                 }
                 const int base = k + 3 * NB;
                 for (int I0 = base; I0 < f; I0 += TILE)
                     for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
+                        // the first diagonal tile gets this step's update from the next step's
+                        // row tile of the same rows (with that step's own update)
+                        if (I0 == base && J0 == base) continue;
                         tr.push_back({s, I0, J0, k});
                         double e = tile_entries(I0, J0, f, base, w);
                         fr += 2.0 * e * nb;
