@@ -22,6 +22,9 @@
 #ifndef SDS_STEP_MARK
 #define SDS_STEP_MARK(i)
 #endif
+#ifndef SDS_POTRF_MARK
+#define SDS_POTRF_MARK(i)
+#endif
 
 namespace sds {
 
@@ -199,43 +202,95 @@ struct Mfma<float> {
     static __device__ inline int row(int l4, int q) { return 4 * l4 + q; }
 };
 
-// Cholesky of the 64 x 64 block D in LDS (D[j][i] = L(i, j), lower part; padded with the
-// identity past nb), four waves: per 16-column panel, wave 0 factors the panel's rows in
-// registers (lane i holds row kb + i; the pivot and the column entries move between lanes
-// by v_readlane; 1/sqrt is the hardware estimate with two Newton steps), then the four waves
-// apply the panel to the trailing lower 16 x 16 tiles on the matrix cores (K = 16). The upper
-// triangle of D holds scratch values afterwards. rd gets the pivots' reciprocals (1 past nb);
-// a failed pivot (1-based within the block) is left in fail. Ends with a barrier.
+// One block row p of X = L^-1, blocked by 16, for the 64 x 64 factor in D (D[j][i] = L(i, j);
+// rd the pivots' reciprocals), into X (X[j][i] = X(i, j)), by one wave: the diagonal block
+// X_pp by forward substitution (lane c: column c, sixteen steps), written when this wave also
+// forms a piece or diag is set; piece j < p: X_pj = -X_pp sum_{k=j}^{p-1} L_pk X_kj with 16 x 16
+// MFMA products (the second product's K runs through the first's accumulator registers, so T
+// needs no staging). X_kj, k < p, must be in X already.
 template <typename T>
-__device__ __attribute__((always_inline)) inline void block_potrf(T (*D)[NB + 1], T* rd, int* fail, int nb, double eps) {
+__device__ __attribute__((always_inline)) inline void inverse_row(const T (*D)[NB + 1], const T* rd, T (*X)[NB + 1],
+                                                                  int p, int j, bool diag) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, l16 = lane & 15, l4 = lane >> 4;
+    const int b = 16 * p;
+    {
+        const int c = l16;
+        T x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = r == c ? T(1) : T(0);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            x[m] = x[m] * rd[b + m];
+            const T xm = x[m];
+#pragma unroll
+            for (int r = m + 1; r < 16; ++r) x[r] = fma(-D[b + m][b + r], xm, x[r]);
+        }
+        if (lane < 16 && (diag || j < p))
+#pragma unroll
+            for (int r = 0; r < 16; ++r) X[b + c][b + r] = x[r];
+    }
+    if (j >= p) return;
+    acc_t t = acc_t{};
+    for (int k = j; k < p; ++k)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+            t = Mfma<T>::mac(D[16 * k + 4 * s4 + l4][b + l16], X[16 * j + l16][16 * k + 4 * s4 + l4], t);
+    acc_t x2 = acc_t{};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int kx = Mfma<T>::row(l4, q);  // t's element q of lane l is T(kx, l % 16)
+        x2 = Mfma<T>::mac(-X[b + kx][b + l16], t[q], x2);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X[16 * j + l16][b + Mfma<T>::row(l4, q)] = x2[q];
+}
+
+// Cholesky of the 64 x 64 block D in LDS (D[j][i] = L(i, j), lower part; padded with the
+// identity past nb) and its inverse X = L^-1 (X[j][i] = X(i, j)), four waves: per 16-column
+// panel, wave 0 factors the panel's rows in registers (lane i holds row kb + i; the pivot and
+// the column entries move between lanes by v_readlane; 1/sqrt is the hardware estimate with
+// two Newton steps) while waves 1-3 form the previous panel's block row of X; then the four
+// waves apply the panel to the trailing lower 16 x 16 tiles on the matrix cores (K = 16). The
+// last block row of X follows the loop. The upper triangle of D holds scratch values
+// afterwards. rd gets the pivots' reciprocals (1 past nb); a failed pivot (1-based within the
+// block) is left in fail, and X is then incomplete. Ends with a barrier.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB + 1], T* rd, int* fail, int nb,
+                                                                      double eps, T (*X)[NB + 1]) {
     typedef typename Mfma<T>::acc_t acc_t;
     const int tid = threadIdx.x, wv = tid >> 6;
     if (tid == 0) *fail = 0;
+    for (int e = tid; e < NB * NB; e += blockDim.x) {
+        const int j = e / NB, i = e - j * NB;
+        if (i / 16 < j / 16) X[j][i] = T(0);
+    }
     __syncthreads();
-    for (int kb = 0; kb < nb; kb += 16) {
+    int p = 0;  // the panel of this round
+    for (int kb = 0; kb < nb; kb += 16, ++p) {
         // the lane index is re-read each panel so that the per-column lane masks are formed
         // where they are used instead of being kept (and spilled) across the loop
         int ii = tid & (NB - 1);
         asm volatile("" : "+v"(ii));
         if (wv == 0) {
             const int row = kb + ii, rowc = min(row, NB - 1);
-            T p[16];
+            T p16[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 const T v = D[kb + c][rowc];
-                p[c] = row < NB ? v : T(0);
+                p16[c] = row < NB ? v : T(0);
             }
             T myr = T(1);         // lane c keeps the reciprocal of pivot c
             double mypiv = 1.0;   // and pivot c, for the acceptance test after the panel
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
-                const double piv = (double)lane_bcast(p[c], c);
+                const double piv = (double)lane_bcast(p16[c], c);
                 const double pe = fmax(piv, eps);
                 const T r = T(rsqrt_nr(pe));
                 // rows below the pivot get L(i, c) = a(i, c) r, the pivot row sqrt(max(a, eps)); the
                 // rows above (the upper triangle) only hold scratch values
                 const bool isc = ii == c;
-                p[c] = (isc ? T(pe) : p[c]) * r;
+                p16[c] = (isc ? T(pe) : p16[c]) * r;
                 myr = isc ? r : myr;
                 mypiv = isc ? piv : mypiv;
                 // each broadcast is consumed before the next is read, which keeps the scalar
@@ -243,20 +298,26 @@ __device__ __attribute__((always_inline)) inline void block_potrf(T (*D)[NB + 1]
                 // ahead and spill them)
 #pragma unroll
                 for (int cc = c + 1; cc < 16; ++cc) {
-                    p[cc] = fma(-p[c], lane_bcast(p[c], cc), p[cc]);
+                    p16[cc] = fma(-p16[c], lane_bcast(p16[c], cc), p16[cc]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
             if (row < NB) {
 #pragma unroll
-                for (int c = 0; c < 16; ++c) D[kb + c][row] = p[c];
+                for (int c = 0; c < 16; ++c) D[kb + c][row] = p16[c];
             }
             if (ii < 16) rd[kb + ii] = myr;
             // the first pivot of the panel that fails the acceptance test (1-based), if any
             const unsigned long long bad = __ballot(ii < 16 && kb + ii < nb && !chol_accept(mypiv, eps));
             if (ii == 0 && bad) *fail = kb + __ffsll((long long)bad);
+        } else if (p >= 1) {
+            // block row p - 1 of X: wave w forms piece j = w - 1 (wave 1 writes the diagonal block)
+            const int pr = p - 1, j = wv - 1;
+            if (j < pr || wv == 1) inverse_row<T>(D, rd, X, pr, min(j, pr), wv == 1);
         }
+        SDS_POTRF_MARK(0);
         __syncthreads();
+        SDS_POTRF_MARK(1);
         if (*fail) break;  // uniform after the barrier
         // trailing tiles (I, J), kb + 16 <= J <= I < 64: A(I, J) -= L(I, panel) L(J, panel)^T
         const int nr = (NB - kb - 16) / 16, ntile = nr * (nr + 1) / 2;
@@ -275,79 +336,24 @@ __device__ __attribute__((always_inline)) inline void block_potrf(T (*D)[NB + 1]
             for (int q = 0; q < 4; ++q) D[J + l16][I + Mfma<T>::row(l4, q)] = acc[q];
         }
         __syncthreads();
+        SDS_POTRF_MARK(2);
+    }
+    if (!*fail) {
+        // the last factored block row P (wave w: piece w, the diagonal block alone for w = P),
+        // and the identity for the block rows past it (pure padding)
+        const int P = p - 1;
+        if (wv <= P) inverse_row<T>(D, rd, X, P, wv, wv == P);
+        for (int e = tid; e < NB * NB; e += blockDim.x) {
+            const int j = e / NB, i = e - j * NB;
+            if (i / 16 > P && j / 16 <= i / 16) X[j][i] = i == j ? T(1) : T(0);
+        }
     }
     for (int j = nb + tid; j < NB; j += blockDim.x) rd[j] = T(1);
     __syncthreads();
+    SDS_POTRF_MARK(3);
 }
 
-// Linv = L^-1 of the factored 64 x 64 block in D (D[j][i] = L(i, j), identity past nb; rd the
-// pivots' reciprocals) on the matrix cores, into X (X[j][i] = Linv(i, j)), blocked by 16:
-// wave i inverts the diagonal block L_ii by forward substitution (lane c: column c, sixteen
-// steps), then wave j forms its column blocks X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, i = j+1..3,
-// with 16 x 16 x 16 MFMA products (T staged through LDS between the two products). Ends with
-// a barrier.
-template <typename T>
-__device__ __attribute__((always_inline)) inline void block_inverse_mfma(const T (*D)[NB + 1], const T* rd,
-                                                                         T (*X)[NB + 1], T (*Ts)[16][17]) {
-    typedef typename Mfma<T>::acc_t acc_t;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
-    // zero the upper blocks, then the diagonal blocks' inverses
-    for (int e = tid; e < NB * NB; e += blockDim.x) {
-        const int j = e / NB, i = e - j * NB;
-        if (i / 16 < j / 16) X[j][i] = T(0);
-    }
-    {
-        const int b = 16 * wv, c = l16;
-        T x[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) x[r] = r == c ? T(1) : T(0);
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            x[m] = x[m] * rd[b + m];
-            const T xm = x[m];
-#pragma unroll
-            for (int r = m + 1; r < 16; ++r) x[r] = fma(-D[b + m][b + r], xm, x[r]);
-        }
-        if (lane < 16)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) X[b + c][b + r] = x[r];
-    }
-    __syncthreads();
-    // off-diagonal blocks of column block j = wv (waves 0..2): block row i = j + it at round it
-    // (every wave runs the three rounds, so the barriers match)
-    const int j = wv;
-    for (int it = 1; it < 4; ++it) {
-        const int i = j + it;
-        const bool act = i < 4;
-        acc_t acc = acc_t{};
-        if (act) {
-            for (int k = j; k < i; ++k)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const T a = D[16 * k + 4 * s + l4][16 * i + l16];  // L(16 i + m, 16 k + 4 s + kk)
-                    const T b = X[16 * j + l16][16 * k + 4 * s + l4];  // X(16 k + 4 s + kk, 16 j + n)
-                    acc = Mfma<T>::mac(a, b, acc);
-                }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Ts[wv][Mfma<T>::row(l4, q)][l16] = acc[q];  // Ts[m][n] = T(m, n)
-        }
-        __syncthreads();
-        if (act) {
-            acc = acc_t{};
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const T a = -X[16 * i + 4 * s + l4][16 * i + l16];  // -X(16 i + m, 16 i + 4 s + kk)
-                const T b = Ts[wv][4 * s + l4][l16];                 // T(4 s + kk, n)
-                acc = Mfma<T>::mac(a, b, acc);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) X[16 * j + l16][16 * i + Mfma<T>::row(l4, q)] = acc[q];
-        }
-        __syncthreads();
-    }
-}
-
-// after block_potrf and block_inverse_mfma: the factored block into the panel (lower part)
+// after block_potrf_inv: the factored block into the panel (lower part)
 // and Linv into DB (NB entries per factor column, zero past nb) for the later row solves and
 // the sweeps
 template <typename T>
@@ -367,7 +373,6 @@ __global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restri
                                                  T* __restrict__ DB, int32_t* __restrict__ status, double eps) {
     __shared__ T D[NB][NB + 1];
     __shared__ T X[NB][NB + 1];
-    __shared__ T Ts[4][16][17];
     __shared__ T rd[NB];
     __shared__ int fail;
     const Task t = tasks[blockIdx.x];
@@ -379,12 +384,11 @@ __global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restri
         const int j = q / NB, ii = q - j * NB;
         D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? P[(int64_t)(k + j) * f + k + ii] : T(0)) : (ii == j ? T(1) : T(0));
     }
-    block_potrf<T>(D, rd, &fail, nb, eps);
+    block_potrf_inv<T>(D, rd, &fail, nb, eps, X);
     if (fail) {
         if (threadIdx.x == 0) status[s] = k + fail;
         return;
     }
-    block_inverse_mfma<T>(D, rd, X, Ts);
     block_store<T>(D, X, P, f, k, nb, DB + (int64_t)(S.super_ptr[s] + k) * NB);
 }
 
@@ -777,13 +781,12 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
             D[j][i] = (i < nbn && j < nbn) ? (i >= j ? accC[nt][q] : T(0)) : (i == j ? T(1) : T(0));
         }
     SDS_STEP_MARK(5);
-    block_potrf<T>(D, rd, &fail, nbn, eps);
+    block_potrf_inv<T>(D, rd, &fail, nbn, eps, Lc);
     SDS_STEP_MARK(6);
     if (fail) {
         if (tid == 0) status[s] = J + fail;
         return;
     }
-    block_inverse_mfma<T>(D, rd, Lc, reinterpret_cast<T(*)[16][17]>(&Ln[0][0]));
     SDS_STEP_MARK(7);
     block_store<T>(D, Lc, P, f, J, nbn, DB + (int64_t)(cg + J) * NB);
     SDS_STEP_MARK(8);
