@@ -336,6 +336,91 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_kkt(args):
+    """The KKT system [H J^T; J -delta I] (delta = 1e-8; H random sparse symmetric, about 10 entries
+    per row, diagonal = row-abs-sum + U[0,1]; J with 4 entries per row at uniform random columns),
+    LDL^T "S" with Bunch-Kaufman pivoting, threshold 0.01, AMD ordering, one right-hand side
+    b ~ N(0, 1). Timed: refactorization + solve, FP64 and FP32 factor + FP64 refinement. Checked:
+    inertia (nh, nj, 0), and the FP64 pivot order, pivot kinds and D against the CPU oracle's
+    LDL^T on the same matrix and analysis (also the CPU baseline). The random structure fills in
+    like a dense matrix (nnz(L) grows about as n^2), so the default size is nh = 8000, nj = 4000."""
+    import torch
+    import sparsedirectsolver as sds
+    from sparsedirectsolver import matrices as M
+    from sparsedirectsolver.solver import numeric_pivots, numeric_diag
+
+    nh = args.kkt_nh
+    nj = nh // 2
+    K = M.kkt_matrix(nh, nj)
+    rp, cv, nz = M.csr_arrays(M.lower_csr(K))
+    n = K.shape[0]
+    bh = np.random.default_rng(666).standard_normal(n)
+    b = torch.tensor(bh, dtype=torch.float64, device="cuda")
+    amal = (512, 0.25, 8)
+    cases = []
+    s64 = None
+    for fp in ("Float64", "Float32"):
+        s = sds.DirectSolver((rp, cv, nz, n), "S", "L")
+        sds.setparam(s, "amalgamation", amal)
+        sds.setparam(s, "pivot_type", "A")
+        sds.setparam(s, "pivot_threshold", 0.01)
+        if fp == "Float32":
+            sds.setparam(s, "factor_precision", "Float32")
+            sds.setparam(s, "ir_n_steps", args.ir_steps)
+        sds.execute("analysis", s)
+        t0 = time.perf_counter()
+        sds.execute("factorization", s)
+        torch.cuda.synchronize()
+        print(f"kkt n={n} {fp}: analysis + first factorization done ({time.perf_counter() - t0:.2f} s)",
+              file=sys.stderr, flush=True)
+        x = torch.zeros_like(b)
+        for _ in range(args.warmup):
+            sds.execute("refactorization", s)
+            sds.execute("solve", s, x, b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sds.execute("refactorization", s)
+            sds.execute("solve", s, x, b)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        t_ref = time_phase(s, args.steps, lambda: sds.execute("refactorization", s))
+        xh = x.cpu().numpy()
+        berr = backward_error(K, xh[:, None], bh[:, None])
+        npos, nneg = sds.getparam(s, "inertia")
+        cases.append({"factor": fp, "ms_per_step": round(ms, 3), "refactor_ms": round(t_ref, 3),
+                      "backward_error": berr, "inertia": [int(npos), int(nneg), int(n - npos - nneg)],
+                      "inertia_expected": [nh, nj, 0], "npivots": int(sds.getparam(s, "npivots")),
+                      "pivots_2x2": sds.getparam(s, "pivot_stats")["n2x2"]})
+        print(json.dumps(cases[-1]), file=sys.stderr, flush=True)
+        if fp == "Float64":
+            s64 = s
+    S = s64.symbolic
+    res = {"metric": "KKT LDL^T (Bunch-Kaufman) refactor+solve ms, inertia, FP64 pivots vs the CPU oracle",
+           "value": cases[0]["ms_per_step"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": cases[0]["ms_per_step"], "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"KKT [H J^T; J -1e-8 I], H {nh} x {nh} (~10 nnz/row), J {nj} x {nh} (4 nnz/row), "
+                                  f"n = {n}, LDL^T Bunch-Kaufman, threshold 0.01, one right-hand side",
+                      "ordering": "AMD", "amalgamation": list(amal), "nnz_L": S.nnz_L, "nnz_stored": S.nnz_stored,
+                      "supernodes": S.ns, "factor_flops": S.flops_true, "parallelism": "single GPU"},
+           "cases": cases}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        print("cpu baseline: oracle LDL^T on the same analysis ...", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        F = oracle.ldlt(S, nz, pivot_type="A", pivot_threshold=0.01, eps=1e-13)
+        t_cpu = (time.perf_counter() - t0) * 1e3
+        piv, kind = numeric_pivots(s64)
+        res["pivots_bitwise_vs_oracle"] = bool(np.array_equal(piv, F["piv"]) and np.array_equal(kind, F["kind"]))
+        res["d_bitwise_vs_oracle"] = bool(np.array_equal(numeric_diag(s64), F["d"]))
+        res["cpu_baseline"] = {"value": round(t_cpu, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the whole oracle LDL^T factorization of the same matrix and analysis "
+                                         "(single thread)"}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,7 +431,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=5e10, help="flops of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint"), default="laplacian3d")
+    ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint", "kkt"), default="laplacian3d")
+    ap.add_argument("--kkt-nh", type=int, default=8000, help="kkt workload: order of H (J has half as many rows)")
     ap.add_argument("--fd-entries", type=int, default=64, help="adjoint workload: finite-difference checks")
     ap.add_argument("--lu-grid", type=int, default=48, help="adjoint workload: grid side of the LU case")
     ap.add_argument("--members", type=int, default=1024)
@@ -361,6 +447,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "kkt":
+        bench_kkt(args)
         return
     import torch
     import sparsedirectsolver as sds

@@ -303,40 +303,180 @@ __global__ void __launch_bounds__(256) k_big_lu_extend_add(DevSym S, const Task*
     }
 }
 
-// the pivot loop of a large front in place, one workgroup of 1024 threads. task: (s)
+// ---------------------------------------------------------------------------
+// large fronts, one pivot per pair of launches: k_lu_pivot (one workgroup per front) takes
+// the previous pivot's deferred scaling, searches column k, swaps the rows and sets d_k;
+// k_lu_update (64 x 64 tiles over every front of the level) applies the rank-1 update of
+// step k to the fully summed columns (all rows) and to U12 (the fully summed rows). Each
+// entry goes through the same operations, in the same order, as in lu_front above; the
+// scaling of column k and row k by d_k moves to the next pivot launch because the update
+// tiles read them unscaled.
+
+// the pivot of step k of every front with more than k columns. task: (s)
 template <typename T>
-__global__ void __launch_bounds__(1024) k_big_lu_panel(DevSym S, const Task* __restrict__ tasks,
-                                                      T* __restrict__ factor, T* __restrict__ ufactor,
-                                                      T* __restrict__ dvec, int32_t* __restrict__ piv,
-                                                      int8_t* __restrict__ pkind, int64_t* __restrict__ stats,
-                                                      int pivoting, double u, double eps, int scaled,
-                                                      const double* __restrict__ aux) {
+__global__ void __launch_bounds__(1024) k_lu_pivot(DevSym S, const Task* __restrict__ tasks, int k,
+                                                  T* __restrict__ factor, T* __restrict__ ufactor,
+                                                  T* __restrict__ dvec, int32_t* __restrict__ piv,
+                                                  int8_t* __restrict__ pkind, int64_t* __restrict__ stats,
+                                                  int pivoting, double u, double eps, int scaled,
+                                                  const double* __restrict__ aux) {
     constexpr int NT = 1024;
     __shared__ LuRed<NT> rb;
-    __shared__ int st[NF];
-    __shared__ double sh[2];
     const int s = tasks[blockIdx.x].s;
     const int f = S.fnrows[s], w = S.fncols[s];
+    if (k >= w) return;
     const int c0 = S.super_ptr[s];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (scaled && aux[0] > 0) eps = eps * aux[0];
-    T* P = factor + S.panel_off[s];
-    T* Q = ufactor + S.panel_off[s];
+    LuFront<T> Fr{factor + S.panel_off[s], ufactor + S.panel_off[s], f, w};
     int* lp = piv + c0;
     int8_t* kind = pkind + c0;
     T* dd = dvec + c0;
-    for (int k = tid; k < w; k += NT) {
-        lp[k] = k;
-        kind[k] = KIND_1X1;
-        dd[k] = T(0);
-        dvec[S.n + c0 + k] = T(0);
+    int64_t* st = stats + (int64_t)s * NF;
+    if (k == 0) {
+        for (int q = tid; q < w; q += NT) {
+            lp[q] = q;
+            kind[q] = KIND_1X1;
+            dd[q] = T(0);
+            dvec[S.n + c0 + q] = T(0);
+        }
+        if (tid < NF) st[tid] = 0;
+    } else {
+        // step k - 1's column and row, divided by its pivot
+        const T dp = dd[k - 1];
+        for (int i = k + tid; i < f; i += NT) Fr.l(i, k - 1) = Fr.l(i, k - 1) / dp;
+        for (int j = k + tid; j < f; j += NT) {
+            T& x = Fr.at(k - 1, j);
+            x = x / dp;
+        }
     }
-    if (tid < NF) st[tid] = 0;
     __syncthreads();
+    int r = k;
+    if (pivoting) {
+        double g = 0.0;
+        for (int i = k + tid; i < f; i += NT) g = lu_jmax(g, fabs((double)Fr.l(i, k)));
+        for (int o = 32; o > 0; o >>= 1) g = lu_jmax(g, __shfl_xor(g, o));
+        if (lane == 0) rb.v[wv] = g;
+        __syncthreads();
+        g = rb.v[0];
+        for (int q = 1; q < NT / 64; ++q) g = lu_jmax(g, rb.v[q]);
+        __syncthreads();
+        const double akk = fabs((double)Fr.l(k, k));
+        if (!(akk >= u * g)) {
+            double bv = akk;
+            int bi = k;
+            for (int i = k + 1 + tid; i < w; i += NT) {
+                const double a = fabs((double)Fr.l(i, k));
+                if (a > bv) {
+                    bv = a;
+                    bi = i;
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(bv, o);
+                const int oi = __shfl_xor(bi, o);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                rb.v[wv] = bv;
+                rb.i[wv] = bi;
+            }
+            __syncthreads();
+            bv = rb.v[0];
+            bi = rb.i[0];
+            for (int q = 1; q < NT / 64; ++q)
+                if (rb.v[q] > bv || (rb.v[q] == bv && rb.i[q] < bi)) {
+                    bv = rb.v[q];
+                    bi = rb.i[q];
+                }
+            r = bi;
+        }
+        __syncthreads();
+    }
+    if (r != k) {
+        for (int j = tid; j < f; j += NT) {
+            T& a = Fr.at(k, j);
+            T& b = Fr.at(r, j);
+            const T t = a;
+            a = b;
+            b = t;
+        }
+        if (tid == 0) {
+            const int t = lp[k];
+            lp[k] = lp[r];
+            lp[r] = t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double dk = (double)Fr.l(k, k);
+        int8_t kd = KIND_1X1;
+        if (!(fabs(dk) >= eps) || dk == 0.0) {
+            if (dk == 0.0) st[LST_NZERO]++;
+            dk = (dk < 0.0 ? -1.0 : 1.0) * eps;
+            kd = KIND_PERTURBED;
+            st[LST_NPERT]++;
+            if (dk == 0.0 && st[LST_INFO] == 0) st[LST_INFO] = k + 1;
+        }
+        Fr.l(k, k) = T(dk);
+        dd[k] = T(dk);
+        kind[k] = kd;
+    }
+}
+
+// the rank-1 update of step k: tile (s, I0, J0) of 64 x 64 entries, J0 < w: fully summed
+// columns (rows I0.. of every row > k), J0 >= w: U12 (rows I0.. < w, stored transposed)
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_update(DevSym S, const Task* __restrict__ tasks, int k,
+                                                   T* __restrict__ factor, T* __restrict__ ufactor,
+                                                   const T* __restrict__ dvec) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    if (k >= w) return;
+    const int I0 = t.a, J0 = t.b;
+    LuFront<T> Fr{factor + S.panel_off[s], ufactor + S.panel_off[s], f, w};
+    const T dk = dvec[S.super_ptr[s] + k];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (J0 < w) {
+        // lanes along the rows: column-major panel
+        const int i = I0 + tx;
+        if (i <= k || i >= f) return;
+        const T lk = Fr.l(i, k) / dk;
+        for (int j = max(J0, k + 1) + ty; j < min(J0 + 64, w); j += 4) Fr.l(i, j) = Fr.l(i, j) - lk * Fr.l(k, j);
+    } else {
+        // lanes along the columns: U12 is stored transposed
+        const int j = J0 + tx;
+        if (j >= f) return;
+        const T rk = Fr.u(k, j);
+        for (int i = max(I0, k + 1) + ty; i < min(I0 + 64, w); i += 4) {
+            const T lk = Fr.l(i, k) / dk;
+            Fr.u(i, j) = Fr.u(i, j) - lk * rk;
+        }
+    }
+}
+
+// after the last pivot of a front: its deferred scaling, then U^T (the upper triangle of the
+// fully summed square moves to Q) and the unit diagonals. task: (s)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_lu_finish(DevSym S, const Task* __restrict__ tasks,
+                                                   T* __restrict__ factor, T* __restrict__ ufactor,
+                                                   const T* __restrict__ dvec, int32_t* __restrict__ piv) {
+    constexpr int NT = 1024;
+    const int s = tasks[blockIdx.x].s;
+    const int f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s], tid = threadIdx.x;
+    T* P = factor + S.panel_off[s];
+    T* Q = ufactor + S.panel_off[s];
     LuFront<T> Fr{P, Q, f, w};
-    lu_front<T, NT>(Fr, pivoting, u, eps, lp, kind, dd, st, rb, sh);
-    // U^T: the upper triangle of the fully summed square moves to Q, then both panels get a
-    // unit diagonal and an empty upper triangle
+    const T dp = dvec[c0 + w - 1];
+    for (int i = w + tid; i < f; i += NT) Fr.l(i, w - 1) = Fr.l(i, w - 1) / dp;
+    for (int j = w + tid; j < f; j += NT) {
+        T& x = Fr.at(w - 1, j);
+        x = x / dp;
+    }
+    __syncthreads();
     for (int q = tid; q < w * w; q += NT) {
         const int j = q / w, i = q - j * w;
         if (i > j) Q[(int64_t)j * f + i] = P[(int64_t)i * f + j];
@@ -349,8 +489,7 @@ __global__ void __launch_bounds__(1024) k_big_lu_panel(DevSym S, const Task* __r
             Q[(int64_t)j * f + i] = i == j ? T(1) : T(0);
         }
     }
-    for (int k = tid; k < w; k += NT) lp[k] = c0 + lp[k];
-    if (tid < NF) stats[(int64_t)s * NF + tid] = st[tid];
+    for (int q = tid; q < w; q += NT) piv[c0 + q] = c0 + piv[c0 + q];
 }
 
 // F22 <- F22 - (L21 D) U12 by 64 x 64 tiles (I0, J0 front rows/columns >= w), each entry's
@@ -439,11 +578,23 @@ void launch_big_lu_extend_add(hipStream_t st, const DevSym& S, const Task* tasks
 }
 
 template <typename T>
-void launch_big_lu_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor, T* dvec,
-                         int32_t* piv, int8_t* kind, int64_t* stats, int pivoting, double u, double eps, int scaled,
-                         const double* aux) {
-    hipLaunchKernelGGL(k_big_lu_panel<T>, dim3(n), dim3(1024), 0, st, S, tasks, factor, ufactor, dvec, piv, kind,
+void launch_lu_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n, int k, T* factor, T* ufactor,
+                     T* dvec, int32_t* piv, int8_t* kind, int64_t* stats, int pivoting, double u, double eps,
+                     int scaled, const double* aux) {
+    hipLaunchKernelGGL(k_lu_pivot<T>, dim3(n), dim3(1024), 0, st, S, tasks, k, factor, ufactor, dvec, piv, kind,
                        stats, pivoting, u, eps, scaled, aux);
+}
+
+template <typename T>
+void launch_lu_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, int k, T* factor, T* ufactor,
+                      const T* dvec) {
+    hipLaunchKernelGGL(k_lu_update<T>, dim3(n), dim3(256), 0, st, S, tasks, k, factor, ufactor, dvec);
+}
+
+template <typename T>
+void launch_lu_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor,
+                      const T* dvec, int32_t* piv) {
+    hipLaunchKernelGGL(k_lu_finish<T>, dim3(n), dim3(1024), 0, st, S, tasks, factor, ufactor, dvec, piv);
 }
 
 template <typename T>
@@ -459,9 +610,11 @@ void launch_lu_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, con
     template void launch_big_lu_zero<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, T*);                  \
     template void launch_big_lu_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*);    \
     template void launch_big_lu_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, T*);            \
-    template void launch_big_lu_panel<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, T*, int32_t*,        \
-                                         int8_t*, int64_t*, int, double, double, int, const double*);               \
-    template void launch_lu_cb<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, const T*);
+    template void launch_lu_cb<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, const T*); \
+    template void launch_lu_pivot<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, T*, int32_t*,      \
+                                     int8_t*, int64_t*, int, double, double, int, const double*);                   \
+    template void launch_lu_update<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, const T*);         \
+    template void launch_lu_finish<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const T*, int32_t*);
 INST(double)
 INST(float)
 #undef INST

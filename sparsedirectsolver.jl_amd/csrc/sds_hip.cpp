@@ -45,8 +45,10 @@ enum LaunchKind {
     L_LU_ZERO,
     L_LU_SCATTER,
     L_LU_EXTEND,
-    L_LU_PANEL,
     L_LU_CB,
+    L_LU_PIVOT,      // k_lu_pivot: step k's pivot of every large front of the level
+    L_LU_UPDATE,     // k_lu_update: step k's rank-1 update tiles
+    L_LU_FINISH,     // k_lu_finish: the last deferred scaling, U^T and unit diagonals
     S_FWD_PULL,
     S_FWD_STEP,
     S_FWD_FINISH,
@@ -64,6 +66,7 @@ struct Launch {
     int n;
     double flops, bytes;  // algorithmic work of the launch
     int nstep = 0, ncopy = 0;  // k_big_step launches: step tasks, then copies, then update tiles
+    int k = 0;                 // pivot step of the k_lu_pivot / k_lu_update launches
 };
 
 struct Level {
@@ -97,7 +100,9 @@ int prof_kind(int kind) {
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
         case L_STEP: return SDS_K_TRSM;
         case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
-        case L_LU_PANEL: return SDS_K_PIVOT_PANEL;
+        case L_LU_PIVOT: return SDS_K_PIVOT_PANEL;
+        case L_LU_UPDATE: return SDS_K_PIVOT_PANEL;
+        case L_LU_FINISH: return SDS_K_PIVOT_PANEL;
         case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
         case L_LU_SCATTER: return SDS_K_ZERO_SCATTER;
         case L_LU_EXTEND: return SDS_K_EXTEND_ADD;
@@ -527,7 +532,44 @@ int build_schedule(sds_handle* h, int dtype) {
         push_extend(L, false, lu ? L_LU_EXTEND : L_EXTEND_ADD);
         int wmax = 0;
         for (int32_t s : L.big) wmax = std::max(wmax, h->fncols[s]);
-        if (h->method != M_CHOLESKY) {
+        if (lu) {
+            // pivoted LU of the large fronts, one step at a time over all of them: the pivot
+            // (one workgroup per front) and the rank-1 update tiles (every front's remaining
+            // fully summed columns and U12, 64 x 64 tiles; one tile list per 64 steps)
+            std::vector<Task> tp;
+            for (int32_t s : L.big) tp.push_back({s, 0, 0, 0});
+            const int64_t off_p = (int64_t)tasks.size();
+            tasks.insert(tasks.end(), tp.begin(), tp.end());
+            for (int e = 0; e * 64 < wmax; ++e) {
+                std::vector<Task> tu;
+                for (int32_t s : L.big) {
+                    const int f = h->fnrows[s], w = h->fncols[s];
+                    if (64 * e >= w) continue;
+                    for (int J0 = 64 * e; J0 < w; J0 += 64)
+                        for (int I0 = 64 * e; I0 < f; I0 += 64) tu.push_back({s, I0, J0, 0});
+                    for (int J0 = w; J0 < f; J0 += 64)
+                        for (int I0 = 64 * e; I0 < w; I0 += 64) tu.push_back({s, I0, J0, 0});
+                }
+                const int64_t off_u = (int64_t)tasks.size();
+                tasks.insert(tasks.end(), tu.begin(), tu.end());
+                for (int k = 64 * e; k < std::min(wmax, 64 * e + 64); ++k) {
+                    double fl1 = 0, by1 = 0;
+                    for (int32_t s : L.big) {
+                        const double f = h->fnrows[s], w = h->fncols[s];
+                        if (k >= w) continue;
+                        const double ent = (f - k - 1) * (w - k - 1) + (w - k - 1) * (f - w);
+                        fl1 += 3.0 * ent;
+                        by1 += 2.0 * dbl_es * ent;
+                    }
+                    L.plan.push_back({L_LU_PIVOT, off_p, (int)tp.size(), 0.0, 0.0});
+                    L.plan.back().k = k;
+                    L.plan.push_back({L_LU_UPDATE, off_u, (int)tu.size(), fl1, by1});
+                    L.plan.back().k = k;
+                }
+            }
+            L.plan.push_back({L_LU_FINISH, off_p, (int)tp.size(), 0.0, 0.0});
+            wmax = 0;
+        } else if (h->method != M_CHOLESKY) {
             // pivoted fronts: the whole panel by one workgroup per front
             t.clear();
             fl = by = 0;
@@ -537,7 +579,7 @@ int build_schedule(sds_handle* h, int dtype) {
                 for (int k = 0; k < (int)w; ++k) fl += 2.0 * (f - k) * (w - k);
                 by += 2.0 * dbl_es * f * w;
             }
-            push_launch(L, lu ? L_LU_PANEL : L_LDLT_PANEL, t, fl, by);
+            push_launch(L, L_LDLT_PANEL, t, fl, by);
             wmax = 0;
         }
         // per 64-column step b (k = 64 b): one k_big_step launch with the step's row tiles
@@ -921,10 +963,12 @@ int factorize_lu(sds_handle* h, const double* nz, int pivoting, double u, double
                     case L_LU_ZERO: launch_big_lu_zero<T>(st, h->dsym, t, ln.n, factor, uf, cb); break;
                     case L_LU_SCATTER: launch_big_lu_scatter<T>(st, h->dsym, t, ln.n, nz, factor, uf); break;
                     case L_LU_EXTEND: launch_big_lu_extend_add<T>(st, h->dsym, t, ln.n, factor, uf, cb); break;
-                    case L_LU_PANEL:
-                        launch_big_lu_panel<T>(st, h->dsym, t, ln.n, factor, uf, dv, h->piv, h->pkind, h->stats,
-                                               pivoting, u, eps, scaled, h->aux);
+                    case L_LU_PIVOT:
+                        launch_lu_pivot<T>(st, h->dsym, t, ln.n, ln.k, factor, uf, dv, h->piv, h->pkind, h->stats,
+                                           pivoting, u, eps, scaled, h->aux);
                         break;
+                    case L_LU_UPDATE: launch_lu_update<T>(st, h->dsym, t, ln.n, ln.k, factor, uf, dv); break;
+                    case L_LU_FINISH: launch_lu_finish<T>(st, h->dsym, t, ln.n, factor, uf, dv, h->piv); break;
                     case L_LU_CB: launch_lu_cb<T>(st, h->dsym, t, ln.n, factor, uf, cb, dv); break;
                 }
             });

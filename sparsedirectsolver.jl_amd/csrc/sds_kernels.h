@@ -53,12 +53,18 @@ template <typename T>
 void launch_big_lu_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor,
                               T* cbpool);
 template <typename T>
-void launch_big_lu_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor, T* dvec,
-                         int32_t* piv, int8_t* kind, int64_t* stats, int pivoting, double u, double eps, int scaled,
-                         const double* aux);
-template <typename T>
 void launch_lu_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* ufactor,
                   T* cbpool, const T* dvec);
+template <typename T>
+void launch_lu_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n, int k, T* factor, T* ufactor,
+                     T* dvec, int32_t* piv, int8_t* kind, int64_t* stats, int pivoting, double u, double eps,
+                     int scaled, const double* aux);
+template <typename T>
+void launch_lu_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, int k, T* factor, T* ufactor,
+                      const T* dvec);
+template <typename T>
+void launch_lu_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* ufactor,
+                      const T* dvec, int32_t* piv);
 void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stats, int64_t* info);
 
 template <typename T>

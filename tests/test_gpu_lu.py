@@ -60,6 +60,8 @@ CASES = [
     ("convdiff_10_big", lambda: _small_diag(M.convection_diffusion3d(10, peclet=10.0), 5, 1e-2),
      (128, 0.5, 8), "A", 0.5),
     ("unsym_300_big", lambda: random_unsym(300, 0.03, _rng(3), 0.1), (64, 0.5, 8), "A", 0.1),
+    ("convdiff_14_wide", lambda: _small_diag(M.convection_diffusion3d(14, peclet=10.0), 9, 1e-2),
+     (256, 0.5, 8), "A", 0.5),
     ("unsym_200_nopivot", lambda: random_unsym(200, 0.03, _rng(4), 5.0), (32, 0.25, 8), "N", 0.01),
     ("unsym_150_one_front", lambda: random_unsym(150, 0.05, _rng(5), 0.5), (100000, 1.0, 100000), "B", 0.01),
 ]
