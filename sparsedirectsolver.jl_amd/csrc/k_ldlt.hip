@@ -11,11 +11,12 @@
 //
 // Small fronts (f x f fits the LDS) are assembled, factored and their contribution
 // block formed in one workgroup; large fronts are assembled in global memory by the
-// shared column-chunk and extend-add kernels, factored by one workgroup of 1024 threads
-// on the panel in place (the full w x w fully summed block lives in the panel's top
-// rows, the upper triangle being the mirror the reference keeps), then their
-// contribution block is updated by 64 x 64 tiles, each entry subtracting the terms
-// W[i, k] L[j, k] in the order k = 0, 1, ... of the reference.
+// shared column-chunk and extend-add kernels and factored on the panel in place (the full
+// w x w fully summed block lives in the panel's top rows, the upper triangle being the
+// mirror the reference keeps), one pivot per pair of launches: the pivot choice by one
+// workgroup per front, its update by 64 x 64 tiles over the GPU; then their contribution
+// block is updated by 64 x 64 tiles, each entry subtracting the terms W[i, k] L[j, k] in
+// the order k = 0, 1, ... of the reference.
 //
 // Reference counterparts: ref_ldlt! and _ldlt_front! (src/reference/ldlt.jl:121-202,
 // :384-454), _choose_pivot (:280-325), _best_1x1 (:264-276), the threshold test
@@ -475,19 +476,26 @@ __global__ void __launch_bounds__(256) k_front_small_ldlt(DevSym S, const int32_
 }
 
 // ---------------------------------------------------------------------------
-// large fronts: the panel in place, one workgroup of 1024 threads per front.
-// task: (s). The local pivot order is kept in piv (c0 + k) during the loop.
+// large fronts, one pivot per pair of launches over the whole GPU: k_ldlt_pivot (one
+// workgroup per front) chooses the pivot of the front's next column, swaps, sets D and
+// scales the pivot column(s); k_ldlt_update (64 x 64 tiles over every front of the level)
+// applies that pivot's rank-1 or rank-2 update to the remaining fully summed columns (all
+// rows). Each entry goes through the operations of ldlt_front above, in the same order. A
+// 2x2 pivot takes two columns in one pair, so the number of pairs a front needs is only
+// known on the device: the launch pairs run to the level's widest front and a front whose
+// columns are done skips them. lstep[3 s ..]: the step's first column, its size (1, 2, or 0
+// when the front is done) and the next column.
 
+// launch pair t: the next pivot of every large front. task: (s)
 template <typename T>
-__global__ void __launch_bounds__(1024) k_big_ldlt_panel(DevSym S, const Task* __restrict__ tasks,
-                                                        T* __restrict__ factor, T* __restrict__ dvec,
-                                                        int32_t* __restrict__ piv, int8_t* __restrict__ pkind,
-                                                        int64_t* __restrict__ stats, LdltPrm prm, int scaled,
-                                                        const double* __restrict__ aux,
-                                                        const int8_t* __restrict__ psign) {
+__global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __restrict__ tasks, int t,
+                                                    T* __restrict__ factor, T* __restrict__ dvec,
+                                                    int32_t* __restrict__ piv, int8_t* __restrict__ pkind,
+                                                    int64_t* __restrict__ stats, int32_t* __restrict__ lstep,
+                                                    LdltPrm prm, int scaled, const double* __restrict__ aux,
+                                                    const int8_t* __restrict__ psign) {
     constexpr int NT = 1024;
     __shared__ RedBuf<NT> rb;
-    __shared__ int st[NF];
     __shared__ double sh[4];
     const int s = tasks[blockIdx.x].s;
     const int f = S.fnrows[s], w = S.fncols[s];
@@ -499,26 +507,152 @@ __global__ void __launch_bounds__(1024) k_big_ldlt_panel(DevSym S, const Task* _
     int8_t* kind = pkind + c0;
     T* dd = dvec + c0;
     T* ee = dvec + S.n + c0;
-    for (int k = tid; k < w; k += NT) {
-        lp[k] = k;
-        kind[k] = 0;
-        dd[k] = ee[k] = T(0);
-    }
-    if (tid < NF) st[tid] = 0;
-    for (int q = tid; q < w * w; q += NT) {
-        const int j = q / w, i = q - j * w;
-        if (i > j) P[(int64_t)i * f + j] = P[(int64_t)j * f + i];
+    int64_t* st = stats + (int64_t)s * NF;
+    int32_t* ls = lstep + 3 * s;
+    int k;
+    if (t == 0) {
+        for (int q = tid; q < w; q += NT) {
+            lp[q] = q;
+            kind[q] = 0;
+            dd[q] = ee[q] = T(0);
+        }
+        if (tid < NF) st[tid] = 0;
+        // the upper triangle of the fully summed square mirrors the lower (the reference's
+        // full square)
+        for (int q = tid; q < w * w; q += NT) {
+            const int j = q / w, i = q - j * w;
+            if (i > j) P[(int64_t)i * f + j] = P[(int64_t)j * f + i];
+        }
+        k = 0;
+    } else {
+        k = ls[2];
     }
     __syncthreads();
+    if (k >= w) {
+        if (tid == 0) ls[1] = 0;
+        return;
+    }
     Front<T, NT> Fr{P, f, w, lp, &rb};
-    ldlt_front<T, NT>(Fr, prm, psign, S.perm + c0, PivOut<T>{kind, dd, ee, st}, sh);
-    // unit diagonal, empty upper triangle (the contribution-block kernel reads L21 only)
+    int c, r;
+    Fr.choose(k, prm, c, r);
+    const int8_t* sreq_row = psign;
+    const int32_t* perm_c0 = S.perm + c0;
+    if (r < 0) {
+        Fr.swap(k, c);
+        if (tid == 0) {
+            double d = Fr.at(k, k);
+            int8_t kd = KIND_1X1;
+            if (!(fabs(d) >= prm.eps) || d == 0.0) {
+                if (d == 0.0) st[ST_NZERO]++;
+                const int req = sreq_row ? (int)sreq_row[perm_c0[lp[k]]] : 0;
+                const double sg = req != 0 ? (double)req : (d < 0.0 ? -1.0 : 1.0);
+                d = sg * prm.eps;
+                kd = KIND_PERTURBED;
+                st[ST_NPERT]++;
+                if (d == 0.0 && st[ST_INFO] == 0) st[ST_INFO] = lp[k] + 1;
+            }
+            Fr.ref(k, k) = T(d);
+            kind[k] = kd;
+            dd[k] = T(d);
+            ee[k] = T(0);
+            if (d > 0.0)
+                st[ST_NPOS]++;
+            else if (d < 0.0)
+                st[ST_NNEG]++;
+            sh[0] = d;
+            ls[0] = k;
+            ls[1] = 1;
+            ls[2] = k + 1;
+        }
+        __syncthreads();
+        const T d = T(sh[0]);
+        for (int i = k + 1 + tid; i < f; i += NT) Fr.ref(i, k) = Fr.ref(i, k) / d;
+    } else {
+        Fr.swap(k + 1, r);
+        if (tid == 0) {
+            const T a = Fr.ref(k, k), b = Fr.ref(k + 1, k), c2 = Fr.ref(k + 1, k + 1);
+            const T up = b;
+            Fr.ref(k, k + 1) = up;
+            const T det = a * c2 - up * b;
+            sh[0] = (double)(c2 / det);
+            sh[1] = (double)(-up / det);
+            sh[2] = (double)(-b / det);
+            sh[3] = (double)(a / det);
+            kind[k] = KIND_2X2_FIRST;
+            kind[k + 1] = KIND_2X2_SECOND;
+            dd[k] = a;
+            dd[k + 1] = c2;
+            ee[k] = b;
+            ee[k + 1] = T(0);
+            st[ST_N2X2]++;
+            if (det < T(0)) {
+                st[ST_NPOS]++;
+                st[ST_NNEG]++;
+            } else if (a > T(0)) {
+                st[ST_NPOS] += 2;
+            } else {
+                st[ST_NNEG] += 2;
+            }
+            Fr.ref(k + 1, k) = T(0);
+            ls[0] = k;
+            ls[1] = 2;
+            ls[2] = k + 2;
+        }
+        __syncthreads();
+        const T e11 = T(sh[0]), e12 = T(sh[1]), e21 = T(sh[2]), e22 = T(sh[3]);
+        for (int i = k + 2 + tid; i < f; i += NT) {
+            const T x1 = Fr.ref(i, k), x2 = Fr.ref(i, k + 1);
+            Fr.ref(i, k) = x1 * e11 + x2 * e21;
+            Fr.ref(i, k + 1) = x1 * e12 + x2 * e22;
+        }
+    }
+}
+
+// launch pair t: the update of each front's pivot of that pair on the fully summed columns
+// after it (every row below). task: (s, I0, J0), a 64 x 64 tile, lanes along the rows
+template <typename T>
+__global__ void __launch_bounds__(256) k_ldlt_update(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                     const int32_t* __restrict__ lstep) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    const int32_t* ls = lstep + 3 * s;
+    const int sz = ls[1];
+    if (sz == 0) return;
+    const int k = ls[0], k2 = k + sz;
+    const int i = t.a + (threadIdx.x & 63);
+    if (i < k2 || i >= f) return;
+    T* P = factor + S.panel_off[s];
+    const T* ck = P + (int64_t)k * f;
+    const int j1 = min(t.b + 64, w);
+    if (sz == 1) {
+        const T lk = ck[i];
+        for (int j = max(t.b, k2) + (threadIdx.x >> 6); j < j1; j += 4) {
+            T& a = P[(int64_t)j * f + i];
+            a = a - lk * P[(int64_t)j * f + k];
+        }
+    } else {
+        const T l1 = ck[i], l2 = ck[f + i];
+        for (int j = max(t.b, k2) + (threadIdx.x >> 6); j < j1; j += 4) {
+            T& a = P[(int64_t)j * f + i];
+            a = a - (l1 * P[(int64_t)j * f + k] + l2 * P[(int64_t)j * f + k + 1]);
+        }
+    }
+}
+
+// after the last pair: unit diagonal, empty upper triangle and the global pivot order.
+// task: (s)
+template <typename T>
+__global__ void __launch_bounds__(1024) k_ldlt_finish(DevSym S, const Task* __restrict__ tasks,
+                                                     T* __restrict__ factor, int32_t* __restrict__ piv) {
+    constexpr int NT = 1024;
+    const int s = tasks[blockIdx.x].s;
+    const int f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s], tid = threadIdx.x;
+    T* P = factor + S.panel_off[s];
     for (int q = tid; q < w * w; q += NT) {
         const int j = q / w, i = q - j * w;
         if (i <= j) P[(int64_t)j * f + i] = i == j ? T(1) : T(0);
     }
-    for (int k = tid; k < w; k += NT) lp[k] = c0 + lp[k];
-    if (tid < NF) stats[(int64_t)s * NF + tid] = st[tid];
+    for (int q = tid; q < w; q += NT) piv[c0 + q] = c0 + piv[c0 + q];
 }
 
 // Contribution block of a large front: the 64 x 64 tile (I0, J0) (front rows, I0 >= J0 >= w)
@@ -630,12 +764,22 @@ void launch_front_small_ldlt(hipStream_t st, const DevSym& S, const int32_t* fro
 }
 
 template <typename T>
-void launch_big_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* dvec,
-                           int32_t* piv, int8_t* kind, int64_t* stats, int pivot, double u, double eps, int scaled,
-                           const double* aux, const int8_t* psign) {
+void launch_ldlt_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n, int t, T* factor, T* dvec,
+                       int32_t* piv, int8_t* kind, int64_t* stats, int32_t* lstep, int pivot, double u, double eps,
+                       int scaled, const double* aux, const int8_t* psign) {
     LdltPrm p{pivot, u, eps};
-    hipLaunchKernelGGL(k_big_ldlt_panel<T>, dim3(n), dim3(1024), 0, st, S, tasks, factor, dvec, piv, kind, stats, p,
-                       scaled, aux, psign);
+    hipLaunchKernelGGL(k_ldlt_pivot<T>, dim3(n), dim3(1024), 0, st, S, tasks, t, factor, dvec, piv, kind, stats,
+                       lstep, p, scaled, aux, psign);
+}
+
+template <typename T>
+void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep) {
+    hipLaunchKernelGGL(k_ldlt_update<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, lstep);
+}
+
+template <typename T>
+void launch_ldlt_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* piv) {
+    hipLaunchKernelGGL(k_ldlt_finish<T>, dim3(n), dim3(1024), 0, st, S, tasks, factor, piv);
 }
 
 template <typename T>
@@ -657,11 +801,13 @@ void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stat
     template void launch_front_small_ldlt<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
                                              T*, T*, T*, int32_t*, int8_t*, int64_t*, int, double, double, int,   \
                                              const double*, const int8_t*);                                        \
-    template void launch_big_ldlt_panel<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*,        \
-                                           int8_t*, int64_t*, int, double, double, int, const double*,             \
-                                           const int8_t*);                                                         \
     template void launch_ldlt_cb<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, const T*,         \
-                                    const int8_t*);
+                                    const int8_t*);                                                                \
+    template void launch_ldlt_pivot<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, int32_t*,        \
+                                       int8_t*, int64_t*, int32_t*, int, double, double, int, const double*,       \
+                                       const int8_t*);                                                             \
+    template void launch_ldlt_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int32_t*);        \
+    template void launch_ldlt_finish<T>(hipStream_t, const DevSym&, const Task*, int, T*, int32_t*);
 INST(double)
 INST(float)
 #undef INST

@@ -40,12 +40,14 @@ enum LaunchKind {
     L_UPDATE_PANEL,
     L_STEP,          // k_big_step: a step, with the previous step's copies and trailing update
     L_UPDATE_CB,
-    L_LDLT_PANEL,
     L_LDLT_CB,
     L_LU_ZERO,
     L_LU_SCATTER,
     L_LU_EXTEND,
     L_LU_CB,
+    L_LDLT_PIVOT,    // k_ldlt_pivot: launch pair k's pivot of every large front of the level
+    L_LDLT_UPDATE,   // k_ldlt_update: that pivot's update tiles
+    L_LDLT_FINISH,   // k_ldlt_finish: unit diagonals and the pivot order
     L_LU_PIVOT,      // k_lu_pivot: step k's pivot of every large front of the level
     L_LU_UPDATE,     // k_lu_update: step k's rank-1 update tiles
     L_LU_FINISH,     // k_lu_finish: the last deferred scaling, U^T and unit diagonals
@@ -99,8 +101,10 @@ int prof_kind(int kind) {
         case L_TRSM: return SDS_K_TRSM;
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
         case L_STEP: return SDS_K_TRSM;
-        case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
         case L_LU_PIVOT: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_PIVOT: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_UPDATE: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_FINISH: return SDS_K_PIVOT_PANEL;
         case L_LU_UPDATE: return SDS_K_PIVOT_PANEL;
         case L_LU_FINISH: return SDS_K_PIVOT_PANEL;
         case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
@@ -166,6 +170,7 @@ struct sds_handle {
     void* cbpool = nullptr;
     void* dbuf = nullptr;   // inverse diagonal blocks (NB entries per column)
     void* lnbuf = nullptr;  // a large-front step's solved rows of the next block (NB entries per column)
+    int32_t* lstep = nullptr;  // LDL^T large fronts: per front the current step (column, size, next column)
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -570,16 +575,39 @@ int build_schedule(sds_handle* h, int dtype) {
             L.plan.push_back({L_LU_FINISH, off_p, (int)tp.size(), 0.0, 0.0});
             wmax = 0;
         } else if (h->method != M_CHOLESKY) {
-            // pivoted fronts: the whole panel by one workgroup per front
-            t.clear();
-            fl = by = 0;
-            for (int32_t s : L.big) {
-                double f = h->fnrows[s], w = h->fncols[s];
-                t.push_back({s, 0, 0, 0});
-                for (int k = 0; k < (int)w; ++k) fl += 2.0 * (f - k) * (w - k);
-                by += 2.0 * dbl_es * f * w;
+            // LDL^T of the large fronts, one pivot (1x1 or 2x2) per launch pair over all of them:
+            // the pivot (one workgroup per front) and its update tiles (the remaining fully summed
+            // columns, all rows, 64 x 64 tiles; one tile list per 64 pairs; a pair takes at least
+            // one column, so pair t works on columns >= t)
+            std::vector<Task> tp;
+            for (int32_t s : L.big) tp.push_back({s, 0, 0, 0});
+            const int64_t off_p = (int64_t)tasks.size();
+            tasks.insert(tasks.end(), tp.begin(), tp.end());
+            for (int e = 0; e * 64 < wmax; ++e) {
+                std::vector<Task> tu;
+                for (int32_t s : L.big) {
+                    const int f = h->fnrows[s], w = h->fncols[s];
+                    if (64 * e >= w) continue;
+                    for (int J0 = 64 * e; J0 < w; J0 += 64)
+                        for (int I0 = 64 * e; I0 < f; I0 += 64) tu.push_back({s, I0, J0, 0});
+                }
+                const int64_t off_u = (int64_t)tasks.size();
+                tasks.insert(tasks.end(), tu.begin(), tu.end());
+                for (int k = 64 * e; k < std::min(wmax, 64 * e + 64); ++k) {
+                    double fl1 = 0, by1 = 0;
+                    for (int32_t s : L.big) {
+                        const double f = h->fnrows[s], w = h->fncols[s];
+                        if (k >= w) continue;
+                        const double ent = (f - k - 1) * (w - k - 1);
+                        fl1 += 2.0 * ent;
+                        by1 += 2.0 * dbl_es * ent;
+                    }
+                    L.plan.push_back({L_LDLT_PIVOT, off_p, (int)tp.size(), 0.0, 0.0});
+                    L.plan.back().k = k;
+                    L.plan.push_back({L_LDLT_UPDATE, off_u, (int)tu.size(), fl1, by1});
+                }
             }
-            push_launch(L, L_LDLT_PANEL, t, fl, by);
+            L.plan.push_back({L_LDLT_FINISH, off_p, (int)tp.size(), 0.0, 0.0});
             wmax = 0;
         }
         // per 64-column step b (k = 64 b): one k_big_step launch with the step's row tiles
@@ -921,10 +949,12 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
                     case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
-                    case L_LDLT_PANEL:
-                        launch_big_ldlt_panel<T>(st, h->dsym, t, ln.n, factor, dv, h->piv, h->pkind, h->stats, pivot,
-                                                 u, eps, scaled, h->aux, psign);
+                    case L_LDLT_PIVOT:
+                        launch_ldlt_pivot<T>(st, h->dsym, t, ln.n, ln.k, factor, dv, h->piv, h->pkind, h->stats,
+                                             h->lstep, pivot, u, eps, scaled, h->aux, psign);
                         break;
+                    case L_LDLT_UPDATE: launch_ldlt_update<T>(st, h->dsym, t, ln.n, factor, h->lstep); break;
+                    case L_LDLT_FINISH: launch_ldlt_finish<T>(st, h->dsym, t, ln.n, factor, h->piv); break;
                     case L_LDLT_CB: launch_ldlt_cb<T>(st, h->dsym, t, ln.n, factor, cb, dv, h->pkind); break;
                 }
             });
@@ -1405,6 +1435,8 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
         return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->dbuf))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->lnbuf))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * 3 * (h->ns + 1), &p))) return rc;
+    h->lstep = static_cast<int32_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
     h->status = static_cast<int32_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, es * 2 * std::max<int64_t>(h->n, 1), &h->dvec))) return rc;

@@ -31,9 +31,13 @@ void launch_front_small_ldlt(hipStream_t st, const DevSym& S, const int32_t* fro
                              int64_t* stats, int pivot, double u, double eps, int scaled, const double* aux,
                              const int8_t* psign);
 template <typename T>
-void launch_big_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* dvec,
-                           int32_t* piv, int8_t* kind, int64_t* stats, int pivot, double u, double eps, int scaled,
-                           const double* aux, const int8_t* psign);
+void launch_ldlt_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n, int t, T* factor, T* dvec,
+                       int32_t* piv, int8_t* kind, int64_t* stats, int32_t* lstep, int pivot, double u, double eps,
+                       int scaled, const double* aux, const int8_t* psign);
+template <typename T>
+void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep);
+template <typename T>
+void launch_ldlt_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* piv);
 template <typename T>
 void launch_ldlt_cb(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* cbpool,
                     const T* dvec, const int8_t* kind);
