@@ -366,7 +366,8 @@ def bench_kkt(args):
         sds.setparam(s, "pivot_threshold", 0.01)
         if fp == "Float32":
             sds.setparam(s, "factor_precision", "Float32")
-            sds.setparam(s, "ir_n_steps", args.ir_steps)
+        # delta = 1e-8 makes the system ill-conditioned: both forms take refinement steps
+        sds.setparam(s, "ir_n_steps", args.ir_steps)
         sds.execute("analysis", s)
         t0 = time.perf_counter()
         sds.execute("factorization", s)
@@ -388,7 +389,7 @@ def bench_kkt(args):
         xh = x.cpu().numpy()
         berr = backward_error(K, xh[:, None], bh[:, None])
         npos, nneg = sds.getparam(s, "inertia")
-        cases.append({"factor": fp, "ms_per_step": round(ms, 3), "refactor_ms": round(t_ref, 3),
+        cases.append({"factor": fp, "ir_steps": args.ir_steps, "ms_per_step": round(ms, 3), "refactor_ms": round(t_ref, 3),
                       "backward_error": berr, "inertia": [int(npos), int(nneg), int(n - npos - nneg)],
                       "inertia_expected": [nh, nj, 0], "npivots": int(sds.getparam(s, "npivots")),
                       "pivots_2x2": sds.getparam(s, "pivot_stats")["n2x2"]})
