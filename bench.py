@@ -28,6 +28,24 @@ PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 MFMA_KINDS = ("update_panel", "update_cb", "front_small", "potrf", "trsm", "pivot_panel", "inverse")
 
 
+# launch class of the library's profile -> the kernel it times
+KIND_KERNEL = {"trsm": "k_big_step", "update_cb": "k_big_update_mfma", "extend_add": "k_big_extend_add"}
+
+
+def measured_traffic(kind):
+    """HBM bytes per launch of the class's kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None when it holds no figure for that kernel."""
+    name = KIND_KERNEL.get(kind)
+    path = os.path.join(ROOT, "profiles", "k_big_step_laplacian3d_64_hbm_traffic.txt")
+    if name is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == name and f[1] == "traffic_bytes_per_launch":
+            return float(f[2])
+    return None
+
+
 def backward_error(A, X, B):
     """max over right-hand sides of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
     R = B - A @ X
@@ -500,12 +518,14 @@ def main():
     if dom in MFMA_KINDS and fl > 0:
         ach = fl / (ms * 1e-3) / 1e12
         res["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK["fp64_tflops"],
-                           "unit": "TFLOP/s", "frac": round(ach / PEAK["fp64_tflops"], 4), "traffic": None,
+                           "unit": "TFLOP/s", "frac": round(ach / PEAK["fp64_tflops"], 4),
+                           "traffic": measured_traffic(dom), "algorithmic_bytes_per_launch": round(by / nl),
                            "launches": nl, "avg_launch_ms": round(ms / nl, 4)}
     else:
         ach = by / (ms * 1e-3) / 1e9
         res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK["hbm_gbs"],
-                           "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4), "traffic": None,
+                           "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4),
+                           "traffic": measured_traffic(dom), "algorithmic_bytes_per_launch": round(by / nl),
                            "launches": nl, "avg_launch_ms": round(ms / nl, 4)}
     res["profile_ms"] = {k: round(v[0] / args.steps, 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
 
