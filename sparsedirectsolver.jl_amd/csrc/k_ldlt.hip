@@ -113,6 +113,9 @@ struct Front {
     int f, w;
     int* lp;
     RedBuf<NT>* rb;
+    double* skey = nullptr;  // LDS scratch of best_1x1_sorted (scap entries), or none
+    int* sidx = nullptr;
+    int scap = 0;
     __device__ double at(int i, int j) const { return (double)F[(int64_t)j * f + i]; }
     __device__ T& ref(int i, int j) const { return F[(int64_t)j * f + i]; }
 
@@ -178,6 +181,68 @@ struct Front {
         __syncthreads();
         return i;
     }
+    // best_1x1 for a large front: the candidates sorted by |a_jj| (largest first, the smaller
+    // index first on ties; those below eps last), then tested sixteen at a time in that order
+    // (one wave per column, the column maximum over all rows below): the first acceptable one
+    // in the order is the one the sequential scan of the reference picks. Usually the first
+    // round decides, where the scan of every column reads (w - k) f entries.
+    __device__ int best_1x1_sorted(int k, const LdltPrm& p) const {
+        const int n = w - k, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        int np = 1;
+        while (np < n) np <<= 1;
+        for (int q = tid; q < np; q += NT) {
+            double key = -1.0;
+            if (q < n) {
+                const double a = fabs(at(k + q, k + q));
+                key = a >= p.eps ? a : -1.0;
+            }
+            skey[q] = key;
+            sidx[q] = k + q;
+        }
+        __syncthreads();
+        // bitonic sort into the order "before(x, y)": key larger, or equal key and smaller index
+        for (int size = 2; size <= np; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < np; i += NT) {
+                    const int l = i ^ stride;
+                    if (l <= i) continue;
+                    const bool lbefore = skey[l] > skey[i] || (skey[l] == skey[i] && sidx[l] < sidx[i]);
+                    const bool ibefore = skey[i] > skey[l] || (skey[i] == skey[l] && sidx[i] < sidx[l]);
+                    if ((i & size) == 0 ? lbefore : ibefore) {
+                        const double tk = skey[i];
+                        skey[i] = skey[l];
+                        skey[l] = tk;
+                        const int ti = sidx[i];
+                        sidx[i] = sidx[l];
+                        sidx[l] = ti;
+                    }
+                }
+                __syncthreads();
+            }
+        for (int base = 0; base < n && skey[base] >= 0.0; base += NT / 64) {
+            const int q = base + wv;
+            int found = 0x7fffffff;
+            if (q < n && skey[q] >= 0.0) {
+                const int j = sidx[q];
+                const double a = skey[q];
+                double m = 0.0;
+                for (int i = k + lane; i < f; i += 64)
+                    if (i != j) m = jmax(m, fabs(at(i, j)));
+                m = wave_jmax(m);
+                if (a >= p.u * m) found = q;
+            }
+            if (lane == 0) rb->i[wv] = found;
+            __syncthreads();
+            int best = rb->i[0];
+            for (int w2 = 1; w2 < NT / 64; ++w2) best = min(best, rb->i[w2]);
+            __syncthreads();
+            if (best != 0x7fffffff) return sidx[best];
+        }
+        return -1;
+    }
+    __device__ int best_1x1_any(int k, const LdltPrm& p) const {
+        return skey && w - k <= scap ? best_1x1_sorted(k, p) : best_1x1(k, p);
+    }
     // pivot choice at column k (ldlt.jl:280-325): c = pivot column, r = 2x2 partner or -1
     __device__ void choose(int k, const LdltPrm& p, int& c, int& r) const {
         r = -1;
@@ -190,7 +255,7 @@ struct Front {
                 c = k;
                 return;
             }
-            const int best = best_1x1(k, p);
+            const int best = best_1x1_any(k, p);
             if (best >= 0) {
                 c = best;
                 return;
@@ -245,7 +310,7 @@ struct Front {
             r = br;
             return;
         }
-        const int best = best_1x1(k, p);
+        const int best = best_1x1_any(k, p);
         if (best >= 0) {
             c = best;
             return;
@@ -495,8 +560,11 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
                                                     LdltPrm prm, int scaled, const double* __restrict__ aux,
                                                     const int8_t* __restrict__ psign) {
     constexpr int NT = 1024;
+    constexpr int SCAP = 2048;  // candidates of the sorted best-1x1 scan
     __shared__ RedBuf<NT> rb;
     __shared__ double sh[4];
+    __shared__ double skey[SCAP];
+    __shared__ int sidx[SCAP];
     const int s = tasks[blockIdx.x].s;
     const int f = S.fnrows[s], w = S.fncols[s];
     const int c0 = S.super_ptr[s];
@@ -532,7 +600,7 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
         if (tid == 0) ls[1] = 0;
         return;
     }
-    Front<T, NT> Fr{P, f, w, lp, &rb};
+    Front<T, NT> Fr{P, f, w, lp, &rb, skey, sidx, SCAP};
     int c, r;
     Fr.choose(k, prm, c, r);
     const int8_t* sreq_row = psign;
