@@ -241,7 +241,22 @@ struct Front {
         return -1;
     }
     __device__ int best_1x1_any(int k, const LdltPrm& p) const {
-        return skey && w - k <= scap ? best_1x1_sorted(k, p) : best_1x1(k, p);
+        if (!skey || w - k > scap) return best_1x1(k, p);
+        // the first candidate in the sorted order (largest |a_jj| >= eps, the smaller index on
+        // ties) tested by the whole block first: it usually is the answer
+        double v = -1.0;
+        int bi = -1;
+        for (int j = k + threadIdx.x; j < w; j += NT) {
+            const double a = fabs(at(j, j));
+            if (a >= p.eps && vi_better(a, j, v, bi)) {
+                v = a;
+                bi = j;
+            }
+        }
+        block_argmax<NT>(v, bi, *rb);
+        if (bi < 0) return -1;
+        if (v >= p.u * colmax(bi, k, bi, -1)) return bi;
+        return best_1x1_sorted(k, p);
     }
     // pivot choice at column k (ldlt.jl:280-325): c = pivot column, r = 2x2 partner or -1
     __device__ void choose(int k, const LdltPrm& p, int& c, int& r) const {
