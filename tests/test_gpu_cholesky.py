@@ -38,6 +38,8 @@ CASES = [
     ("varlap3d_14_nd", lambda: (M.variable_laplacian((14, 13, 12), seed=3), sds.grid_nd_perm(14, 13, 12),
                                 (128, 0.25, 8))),
     ("lap3d_20_natural", lambda: (M.laplacian3d(20, 20, 20), None, (64, 0.25, 8))),
+    # the reference's CPU-runnable configuration: 2-D 100 x 100, the build's AMD ordering
+    ("lap2d_100x100_amd", lambda: (M.laplacian2d(100, 100), None, (32, 0.25, 8))),
 ]
 
 
@@ -116,3 +118,28 @@ def test_refinement_fp64_and_fp32():
         berr = np.abs(A @ Xh - Bh).max(axis=0) / (abs(A).sum(axis=1).max() * np.abs(Xh).max(axis=0) +
                                                   np.abs(Bh).max(axis=0))
         assert berr.max() <= tol, (fp, berr.max())
+
+
+def test_laplacian3d_64_full_size_backward_error():
+    """The benchmark's configuration at full size (3-D 64^3, nested dissection, 8 right-hand
+    sides): size-independent checks, as the oracle would take minutes here. FP64 backward
+    error <= 1e-14, inertia-free stats (every pivot positive), bitwise-reproducible
+    refactorization and solve."""
+    import torch
+    k = 64
+    A = M.laplacian3d(k, k, k)
+    s, nz = _solver(A, sds.grid_nd_perm(k, k, k), (512, 0.25, 8))
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 0
+    n = A.shape[0]
+    B = torch.tensor(np.random.default_rng(666).standard_normal((n, 8)), dtype=torch.float64, device="cuda")
+    X = torch.zeros_like(B)
+    sds.execute("solve", s, X, B)
+    Xh, Bh = X.cpu().numpy(), B.cpu().numpy()
+    an = abs(A).sum(axis=1).max()
+    berr = np.max(np.abs(Bh - A @ Xh).max(axis=0) / (an * np.abs(Xh).max(axis=0) + np.abs(Bh).max(axis=0)))
+    assert berr <= 1e-14, f"backward error {berr:.2e}"
+    sds.execute("refactorization", s)
+    X2 = torch.zeros_like(B)
+    sds.execute("solve", s, X2, B)
+    assert torch.equal(X, X2)

@@ -56,6 +56,9 @@ CASES = [
     ("kkt_240_120_interleaved", lambda: kkt_matrix(240, 120, 1e-8, _rng(6)), kkt_interleaved_perm(240, 120),
      (128, 0.5, 8), "B", 0.01),
     ("kkt_400_200_delta0_amd", lambda: kkt_matrix(400, 200, 0.0, _rng(7)), None, (96, 0.4, 8), "B", 0.5),
+    # the configuration's KKT generator (random H, ~10 nnz per row): wide large fronts, many
+    # launch pairs per level, 2x2 pivots and the best-1x1 fallback
+    ("kkt_config_1500_750_amd", lambda: M.kkt_matrix(1500, 750), None, (512, 0.25, 8), "A", 0.01),
     ("symrand_150_zero_diag", lambda: random_symmetric(150, 0.05, _rng(5), 0.0), None, (32, 0.25, 8), "A", 0.01),
     ("symrand_400_bk_big", lambda: random_symmetric(400, 0.02, _rng(5), 0.01), None, (128, 0.5, 8), "A", 0.01),
     ("symrand_250_diag_big", lambda: random_symmetric(250, 0.04, _rng(5), 0.1), None, (32, 0.25, 8), "D", 0.01),
