@@ -25,6 +25,10 @@
 #ifndef SDS_POTRF_MARK
 #define SDS_POTRF_MARK(i)
 #endif
+// broadcasts of a panel column read ahead together before a scheduling barrier
+#ifndef SDS_BCAST_GROUP
+#define SDS_BCAST_GROUP 8
+#endif
 
 namespace sds {
 
@@ -293,13 +297,13 @@ __device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB 
                 p16[c] = (isc ? T(pe) : p16[c]) * r;
                 myr = isc ? r : myr;
                 mypiv = isc ? piv : mypiv;
-                // each broadcast is consumed before the next is read, which keeps the scalar
-                // registers they pass through few (the scheduler would otherwise read them all
-                // ahead and spill them)
+                // the broadcasts are read ahead in groups of SDS_BCAST_GROUP, which keeps the
+                // scalar registers they pass through few (the scheduler would otherwise read
+                // them all ahead and spill them) while their latencies overlap
 #pragma unroll
                 for (int cc = c + 1; cc < 16; ++cc) {
                     p16[cc] = fma(-p16[c], lane_bcast(p16[c], cc), p16[cc]);
-                    __builtin_amdgcn_sched_barrier(0);
+                    if ((cc - c) % SDS_BCAST_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
                 }
             }
             if (row < NB) {
