@@ -65,6 +65,7 @@ def make_workload(k, nrhs, seed=666):
 
 
 def make_solver(csr, n, perm, amalgamation, factor_precision=None):
+    """Analysis, the tile-shape autotuner on these values, then the first factorization."""
     import sparsedirectsolver as sds
     rp, cv, nz = csr
     s = sds.DirectSolver((rp, cv, nz, n), "SPD", "L")
@@ -73,6 +74,7 @@ def make_solver(csr, n, perm, amalgamation, factor_precision=None):
     if factor_precision:
         sds.setparam(s, "factor_precision", factor_precision)
     sds.execute("analysis", s)
+    s.tiles = sds.autotune(s, reps=2)
     sds.execute("factorization", s)
     return s
 
@@ -184,6 +186,7 @@ def bench_batch(args):
     n = batch.n
     B = torch.tensor(rng.standard_normal((n, 1)), dtype=torch.float64, device="cuda")
     X = torch.zeros_like(B)
+    tiles = sds.autotune(s, reps=2)
     sds.execute("factorization", s)
     graph_capture(s, nrhs=1)
     for _ in range(args.warmup):
@@ -231,7 +234,7 @@ def bench_batch(args):
                                f"[24, 140] (sum n = {n}), 1 right-hand side, {args.ir_steps} refinement steps",
                    "ordering": "geometric nested dissection per member", "amalgamation": list(amal),
                    "nnz_stored": S.nnz_stored, "supernodes": S.ns, "factor_flops": S.flops_true,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "autotuned_tiles": tiles},
         "backward_error": berr, "loop_ms": round(t_loop, 3), "speedup_vs_loop": round(t_loop / t_batch, 2),
         "analysis_s": round(t_analysis, 2),
     }
@@ -506,7 +509,7 @@ def main():
                                f"{args.nrhs} right-hand sides, refactorization + solve",
                    "ordering": "geometric nested dissection", "amalgamation": list(amal), "nrhs": args.nrhs,
                    "nnz_L": S.nnz_L, "nnz_stored": S.nnz_stored, "supernodes": S.ns,
-                   "factor_flops": flops_true, "parallelism": "single GPU"},
+                   "factor_flops": flops_true, "parallelism": "single GPU", "autotuned_tiles": s64.tiles},
         "fp64": {"refactor_ms": round(t_refact64, 3), "solve_ms": round(t_solve64, 3),
                  "factor_tflops": round(flops_true / (t_refact64 * 1e-3) / 1e12, 3),
                  "backward_error": berr64},

@@ -148,6 +148,19 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * Tile-shape autotuner (no counterpart in the reference, whose kernels have fixed shapes;
+ * the chosen form of level and batch launches whose tile shapes come from an autotuner):
+ * times `reps` factorizations of the values nzval for each candidate shape of the large-front
+ * kernels (child rows per extend-add task: 512, 1024, 2048, 4096; for Cholesky also the K
+ * chunk of the contribution-block update: 32 or 64), keeps the fastest, and leaves the
+ * numeric storage reallocated for it (factor contents are not kept: factorize again).
+ * tiles (may be NULL) receives {extend-add rows, K chunk}. Synchronous; needs
+ * sds_alloc_numeric first.
+ */
+int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
+                 int32_t* tiles);
+
+/*
  * HIP graph of one refactorization followed by one solve (no counterpart in the
  * reference, which replays nothing; it is this library's form of the
  * "refactorize + solve captured in a graph" of uneven batches and serving loops).

@@ -146,12 +146,12 @@ __global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* 
 
 // extend-add of one child per front (the launch covers child rank r of every front);
 // task: (s, child, jj0, njj + 16 r0): child columns jj0 .. jj0 + njj - 1 (njj <= EA_COLS),
-// child rows r0 .. r0 + EA_ROWS - 1 (and below the diagonal). A thread takes one child row at
+// child rows r0 .. r0 + ea_rows - 1 (and below the diagonal). A thread takes one child row at
 // a time (its parent row read once) and all the task's columns of that row: the child reads
 // and the parent read-modify-writes are coalesced along the rows, several in flight per thread.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __restrict__ tasks,
-                                                       T* __restrict__ factor, T* __restrict__ cbpool) {
+                                                       T* __restrict__ factor, T* __restrict__ cbpool, int ea_rows) {
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int c = t.a, mc = S.fnrows[c] - S.fncols[c];
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     const T* Cc = cbpool + S.cb_off[c];
     const int32_t* ri = S.relind + S.relind_ptr[c];
     const int jj0 = t.b, nj = min(t.c & 15, mc - jj0), r0 = t.c >> 4;
-    const int iend = min(mc, r0 + EA_ROWS);
+    const int iend = min(mc, r0 + ea_rows);
     int pj[EA_COLS];
 #pragma unroll
     for (int q = 0; q < EA_COLS; ++q) pj[q] = ri[jj0 + min(q, nj - 1)];
@@ -409,7 +409,6 @@ __global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restri
 // (checked on gfx950): lane l gives A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator
 // element q of lane l is D[row(l / 16, q)][l % 16] with row = 4 (l / 16) + q for the f32
 // instruction and 4 q + l / 16 for the f64 one.
-constexpr int UPD_KC = 32;  // chunk of the contribution-block update (four workgroups per CU)
 
 template <typename T, int KC, bool EARLY>
 __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& S, const Task t, T* __restrict__ factor,
@@ -518,13 +517,13 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
             }
 }
 
-template <typename T>
+template <typename T, int KC>
 __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
                                                         T* __restrict__ factor, T* __restrict__ cbpool,
                                                         const int32_t* __restrict__ status, int mode) {
-    __shared__ T As[UPD_KC][TILE];
-    __shared__ T Bs[UPD_KC][TILE];
-    update_tile<T, UPD_KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
+    __shared__ T As[KC][TILE];
+    __shared__ T Bs[KC][TILE];
+    update_tile<T, KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
 }
 
 // the rows [k + 64, k + 128) of a step's columns, kept in LN by k_big_step, into the panel.
@@ -843,8 +842,9 @@ void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks,
 }
 
 template <typename T>
-void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool) {
-    hipLaunchKernelGGL(k_big_extend_add<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool);
+void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
+                           int ea_rows) {
+    hipLaunchKernelGGL(k_big_extend_add<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, ea_rows);
 }
 
 template <typename T>
@@ -862,8 +862,11 @@ void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
-                       const int32_t* status, int mode) {
-    hipLaunchKernelGGL(k_big_update_mfma<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+                       const int32_t* status, int mode, int kc) {
+    if (kc == 64)
+        hipLaunchKernelGGL((k_big_update_mfma<T, 64>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+    else
+        hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
 }
 
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info) {
@@ -880,11 +883,12 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
     template void launch_front_small_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
                                              T*, T*, int32_t*, double);                                           \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
-    template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
+    template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);            \
     template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \
     template void launch_big_step<T>(hipStream_t, const DevSym&, const Task*, int, int, int, T*, T*, T*, T*,      \
                                      int32_t*, double);                                                            \
-    template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int);
+    template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int, \
+                                       int);
 INST(double)
 INST(float)
 #undef INST

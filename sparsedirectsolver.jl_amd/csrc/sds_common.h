@@ -43,7 +43,7 @@ constexpr int NB = 64;     // pivot block of the blocked front factorization
 constexpr int TILE = 64;   // Schur-update tile
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int EA_COLS = 8;      // child columns per extend-add task
-constexpr int EA_ROWS = 1024;   // child rows per extend-add task
+constexpr int EA_ROWS = 1024;   // child rows per extend-add task (default; the autotuner may pick another)
 constexpr int NF = 6;      // per-front statistics fields
 constexpr int MAX_NRHS = 64;  // right-hand sides per solve call
 // Cholesky sweeps with inverted diagonal blocks: a front's columns are solved in steps of at

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <chrono>
 #include <vector>
 
 #include "../../include/sds_hip.h"
@@ -171,6 +172,9 @@ struct sds_handle {
     void* dbuf = nullptr;   // inverse diagonal blocks (NB entries per column)
     void* lnbuf = nullptr;  // a large-front step's solved rows of the next block (NB entries per column)
     int32_t* lstep = nullptr;  // LDL^T large fronts: per front the current step (column, size, next column)
+    // tile shapes chosen by sds_autotune: child rows per extend-add task, K chunk of the
+    // contribution-block update (32: four workgroups per CU, 64: two)
+    int ea_rows = EA_ROWS, upd_kc = 32;
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -503,7 +507,7 @@ int build_schedule(sds_handle* h, int dtype) {
                     for (int jj = j0; jj < j1; jj += 4) t.push_back({s, c, jj, std::min(4, j1 - jj)});
                 } else {
                     for (int jj = j0; jj < j1; jj += EA_COLS)
-                        for (int r0 = jj - jj % EA_ROWS; r0 < mc; r0 += EA_ROWS)
+                        for (int r0 = jj - jj % h->ea_rows; r0 < mc; r0 += h->ea_rows)
                             t.push_back({s, c, jj, std::min(EA_COLS, j1 - jj) + 16 * r0});
                 }
                 // read the child's block, read + write the parent
@@ -889,7 +893,7 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
-                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb, h->ea_rows); break;
                     case L_POTRF:
                         launch_big_diag<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
                         break;
@@ -898,9 +902,11 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                                            static_cast<T*>(h->dbuf), static_cast<T*>(h->lnbuf), h->status, eps);
                         break;
                     case L_UPDATE_PANEL:
-                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0);
+                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0, h->upd_kc);
                         break;
-                    case L_UPDATE_CB: launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1); break;
+                    case L_UPDATE_CB:
+                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1, h->upd_kc);
+                        break;
                 }
             });
         }
@@ -948,7 +954,7 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
-                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb, h->ea_rows); break;
                     case L_LDLT_PIVOT:
                         launch_ldlt_pivot<T>(st, h->dsym, t, ln.n, ln.k, factor, dv, h->piv, h->pkind, h->stats,
                                              h->lstep, pivot, u, eps, scaled, h->aux, psign);
@@ -1549,6 +1555,45 @@ int sds_factorize(sds_handle_t h, const double* nzval, const sds_factor_params* 
     if (rc) return rc;
     if ((rc = read_info(h, info))) return rc;
     h->factorized = true;
+    return SDS_SUCCESS;
+}
+
+int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int32_t reps,
+                 int32_t* tiles) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (h->dtype < 0) return fail(h, SDS_FACTORIZATION_ERROR, "numeric storage not allocated");
+    if (reps < 1) return fail(h, SDS_INVALID_VALUE, "reps must be positive");
+    const int dtype = h->dtype;
+    // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
+    std::vector<std::pair<int, int>> cand;
+    for (int ea : {512, 1024, 2048, 4096})
+        for (int kc : {32, 64})
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc});
+    double best = 1e300;
+    std::pair<int, int> pick{h->ea_rows, h->upd_kc};
+    int64_t info = 0;
+    int rc;
+    for (const auto& c : cand) {
+        h->ea_rows = c.first;
+        h->upd_kc = c.second;
+        if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
+        if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; ++r)
+            if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (t < best) {
+            best = t;
+            pick = c;
+        }
+    }
+    h->ea_rows = pick.first;
+    h->upd_kc = pick.second;
+    if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
+    if (tiles) {
+        tiles[0] = pick.first;
+        tiles[1] = pick.second;
+    }
     return SDS_SUCCESS;
 }
 

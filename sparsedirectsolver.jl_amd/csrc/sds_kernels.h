@@ -11,7 +11,8 @@ template <typename T>
 void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks, int n, const double* nz, T* factor,
                              T* cbpool);
 template <typename T>
-void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool);
+void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
+                           int ea_rows);
 template <typename T>
 void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                      double eps);
@@ -20,7 +21,7 @@ void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
                      T* cbpool, T* DB, T* LN, int32_t* status, double eps);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
-                       const int32_t* status, int mode);
+                       const int32_t* status, int mode, int kc);
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info);
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 
