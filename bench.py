@@ -186,7 +186,9 @@ def bench_batch(args):
     n = batch.n
     B = torch.tensor(rng.standard_normal((n, 1)), dtype=torch.float64, device="cuda")
     X = torch.zeros_like(B)
-    tiles = sds.autotune(s, reps=2)
+    print("batch: autotuning the tile shapes ...", file=sys.stderr, flush=True)
+    tiles = sds.autotune(s, reps=1)
+    print(f"batch: tiles {tiles}", file=sys.stderr, flush=True)
     sds.execute("factorization", s)
     graph_capture(s, nrhs=1)
     for _ in range(args.warmup):

@@ -175,6 +175,8 @@ struct sds_handle {
     // tile shapes chosen by sds_autotune: child rows per extend-add task, K chunk of the
     // contribution-block update (32: four workgroups per CU, 64: two)
     int ea_rows = EA_ROWS, upd_kc = 32;
+    int rest_split = 0;  // trailing tiles of a step from which they get a launch of their own (0: never)
+    int max_rest = 0;    // most trailing tiles of one step in the current plan
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -397,6 +399,7 @@ int build_schedule(sds_handle* h, int dtype) {
     int nlev = 0;
     for (int64_t s = 0; s < ns; ++s) nlev = std::max(nlev, lev[s] + 1);
     h->levels.assign(nlev, Level{});
+    h->max_rest = 0;
     std::vector<char> big(ns, 0);
     for (int64_t s = 0; s < ns; ++s) {
         Level& L = h->levels[lev[s]];
@@ -620,14 +623,21 @@ int build_schedule(sds_handle* h, int dtype) {
         // block by k_big_diag before; the last step's copies and update in a launch of their own)
         std::vector<Task> pc, pr;  // the previous step's copies and update tiles
         double pfr = 0, pbr = 0;
+        // rest_split: a step whose trailing tiles number at least rest_split sends them to a
+        // launch of their own right after it (k_big_update_mfma, several workgroups per CU)
+        // instead of riding in the step launch (one workgroup per CU beside the chain)
         auto push_step = [&](std::vector<Task>& st, double fs, double bs) {
             if (st.empty() && pc.empty() && pr.empty()) return;
+            h->max_rest = std::max(h->max_rest, (int)pr.size());
+            const bool split = h->rest_split > 0 && (int)pr.size() >= h->rest_split;
             std::vector<Task> all(st);
             all.insert(all.end(), pc.begin(), pc.end());
-            all.insert(all.end(), pr.begin(), pr.end());
-            push_launch(L, L_STEP, all, fs + pfr, bs + pbr + dbl_es * 2.0 * NB * NB * (double)pc.size());
+            if (!split) all.insert(all.end(), pr.begin(), pr.end());
+            push_launch(L, L_STEP, all, fs + (split ? 0.0 : pfr),
+                        bs + (split ? 0.0 : pbr) + dbl_es * 2.0 * NB * NB * (double)pc.size());
             L.plan.back().nstep = (int)st.size();
             L.plan.back().ncopy = (int)pc.size();
+            if (split) push_launch(L, L_UPDATE_PANEL, pr, pfr, pbr);
         };
         for (int k = 0; k < wmax; k += NB) {
             std::vector<Task> td, tt, tr, tc;
@@ -1565,17 +1575,28 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     if (reps < 1) return fail(h, SDS_INVALID_VALUE, "reps must be positive");
     const int dtype = h->dtype;
     // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
-    std::vector<std::pair<int, int>> cand;
+    struct Cand {
+        int ea, kc, rs;
+    };
+    std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
         for (int kc : {32, 64})
-            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc});
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0});
+    if (h->method == M_CHOLESKY)
+        for (int rs : {256, 512, 1024})
+            if (rs <= h->max_rest) cand.push_back({-1, -1, rs});  // with the best of the above
     double best = 1e300;
-    std::pair<int, int> pick{h->ea_rows, h->upd_kc};
+    Cand pick{h->ea_rows, h->upd_kc, h->rest_split};
     int64_t info = 0;
     int rc;
-    for (const auto& c : cand) {
-        h->ea_rows = c.first;
-        h->upd_kc = c.second;
+    for (Cand c : cand) {
+        if (c.ea < 0) {
+            c.ea = pick.ea;
+            c.kc = pick.kc;
+        }
+        h->ea_rows = c.ea;
+        h->upd_kc = c.kc;
+        h->rest_split = c.rs;
         if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
         if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
@@ -1587,12 +1608,14 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
             pick = c;
         }
     }
-    h->ea_rows = pick.first;
-    h->upd_kc = pick.second;
+    h->ea_rows = pick.ea;
+    h->upd_kc = pick.kc;
+    h->rest_split = pick.rs;
     if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
     if (tiles) {
-        tiles[0] = pick.first;
-        tiles[1] = pick.second;
+        tiles[0] = pick.ea;
+        tiles[1] = pick.kc;
+        tiles[2] = pick.rs;
     }
     return SDS_SUCCESS;
 }

@@ -148,12 +148,13 @@ def test_laplacian3d_64_full_size_backward_error():
 def test_autotune_keeps_the_factor():
     """The tile-shape autotuner picks among shapes that apply the same operations in the same
     order to every entry: the factor after tuning is bitwise the one before."""
-    A = M.laplacian3d(18, 17, 16)
-    s, nz = _solver(A, sds.grid_nd_perm(18, 17, 16), (256, 0.25, 8))
+    A = M.laplacian3d(24, 23, 22)
+    s, nz = _solver(A, sds.grid_nd_perm(24, 23, 22), (512, 0.25, 8))
     sds.execute("factorization", s)
     before = numeric_factor(s)
     tiles = sds.autotune(s, reps=1)
     assert tiles["extend_add_rows"] in (512, 1024, 2048, 4096) and tiles["cb_k_chunk"] in (32, 64)
+    assert tiles["trailing_split"] in (0, 256, 512, 1024)
     sds.execute("factorization", s)
     assert sds.getparam(s, "info") == 0
     assert np.array_equal(numeric_factor(s), before)
