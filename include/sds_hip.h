@@ -152,11 +152,13 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
  * the chosen form of level and batch launches whose tile shapes come from an autotuner):
  * times `reps` factorizations of the values nzval for each candidate shape of the large-front
  * kernels (child rows per extend-add task: 512, 1024, 2048, 4096; for Cholesky also the K
- * chunk of the contribution-block update: 32 or 64, and the number of trailing-update tiles
- * of a step from which they get a launch of their own: never, 256, 512, 1024), keeps the
- * fastest, and leaves the numeric storage reallocated for it (factor contents are not kept:
- * factorize again). tiles (may be NULL) receives {extend-add rows, K chunk, trailing split}.
- * Synchronous; needs sds_alloc_numeric first.
+ * chunk of the contribution-block update: 32 or 64, the number of trailing-update tiles of a
+ * step from which they get a launch of their own: never, 256, 512, 1024, and the number of
+ * workgroups from which a step launch uses the build for two workgroups per CU: never, 256,
+ * 512), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
+ * are not kept: factorize again). tiles (may be NULL, else 4 entries) receives {extend-add
+ * rows, K chunk, trailing split, two-per-CU steps}. Synchronous; needs sds_alloc_numeric
+ * first.
  */
 int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
                  int32_t* tiles);

@@ -561,19 +561,24 @@ __device__ inline void copy_ln(const DevSym& S, const Task t, T* __restrict__ fa
 // trailing update of the blocks three on (update_tile, mode 0): they touch no block that
 // step reads or writes, and they fill the machine while the step's chain runs.
 // tasks: nstep steps (s, k, I0), then ncopy copies (s, k), then update tiles (s, I0, J0, k)
-template <typename T>
-__global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
-                                                 T* __restrict__ cbpool, T* __restrict__ DB, T* __restrict__ LN,
-                                                 int32_t* __restrict__ status, double eps, int nstep, int ncopy) {
+// OCC: workgroups per CU the kernel is built for (1: registers unbounded; 2: at most 256 VGPRs
+// per lane, for the launches with many workgroups). The two LDS tiles (67 KiB in FP64) allow two.
+template <typename T, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __restrict__ tasks,
+                                                      T* __restrict__ factor, T* __restrict__ cbpool,
+                                                      T* __restrict__ DB, T* __restrict__ LN,
+                                                      int32_t* __restrict__ status, double eps, int nstep,
+                                                      int ncopy) {
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T Bs[NB][NB + 1];  // Bs[kk][c] = Linv_b(c, kk); later L(J + r, b - 1) as Bs[r][kk]; later D
-    __shared__ T Lc[NB][NB + 1];  // Lc[r][kk] = L(I0 + r, k + kk)
-    __shared__ T Ln[NB][NB + 1];  // Ln[r][kk] = L(J + r, k + kk)
+    // Bs[kk][c] = Linv_b(c, kk); later L(J + r, b - 1) as Bs[r][kk]; later L(I0 + r, k + kk) (the
+    // rows after next) or D
+    __shared__ T Bs[NB][NB + 1];
+    __shared__ T Ln[NB][NB + 1];  // Ln[r][kk] = L(J + r, k + kk); later L(I0 + r, k - NB + kk) or Linv_{b+1}
     __shared__ T rd[NB];
     __shared__ int fail;
     const Task t = tasks[blockIdx.x];
     if ((int)blockIdx.x >= nstep + ncopy) {
-        update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Lc[0][0]),
+        update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Bs[0][0]),
                                  reinterpret_cast<T(*)[TILE]>(&Ln[0][0]));
         return;
     }
@@ -680,10 +685,7 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
                 else
                     P[(int64_t)(k + c) * f + I0 + r] = accI[nt][q];
             }
-            if (la) {
-                Lc[r][c] = I0 + r < f ? accI[nt][q] : T(0);
-                Ln[r][c] = J + r < f ? accJ[nt][q] : T(0);
-            }
+            if (la) Ln[r][c] = J + r < f ? accJ[nt][q] : T(0);
         }
     SDS_STEP_MARK(2);
     if (!la) return;
@@ -700,13 +702,17 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
         __syncthreads();
     }
     SDS_STEP_MARK(3);
-    // C(I0.., J..)^T -= L(J, b) L(I0, b)^T (+ L(J, b - 1) L(I0, b - 1)^T off the diagonal tile)
+    // C(I0.., J..)^T -= L(J, b) L(I0, b)^T (+ L(J, b - 1) L(I0, b - 1)^T off the diagonal tile);
+    // the K index of the first product runs over accI's registers (element q of accI[nt'] is
+    // L(I0 + r, k + 16 nt' + row(l / 16, q)), the B operand of this wave's rows as it stands)
 #pragma unroll
-    for (int ks = 0; ks < 16; ++ks) {
-        const T bb = Lc[r][4 * ks + l4];
+    for (int ntp = 0; ntp < 4; ++ntp)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(-Ln[16 * nt + l16][4 * ks + l4], bb, accC[nt]);
-    }
+        for (int qp = 0; qp < 4; ++qp) {
+            const int kx = 16 * ntp + Mfma<T>::row(l4, qp);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) accC[nt] = Mfma<T>::mac(-Ln[16 * nt + l16][kx], accI[ntp][qp], accC[nt]);
+        }
     if (pv)
 #pragma unroll
         for (int ks = 0; ks < 16; ++ks)
@@ -736,20 +742,23 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
                 const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
                 accD[nt][q] = i < f && j < I + nb3 && i >= j ? v : T(0);
             }
-        __syncthreads();  // Ln is free: L(I, b - 1) into it as rows
+        __syncthreads();  // Bs and Ln are free: L(I, b) and L(I, b - 1) into them as rows
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[r][16 * nt + Mfma<T>::row(l4, q)] = accI[nt][q];
         if (prev)
 #pragma unroll
             for (int ks = 0; ks < 16; ++ks) Ln[r][4 * ks + l4] = ap[ks];
         __syncthreads();
-        // step b: the K index runs over accI's registers (element q of accI[nt'] is
-        // L(I + r, k + 16 nt' + row(l / 16, q))), the rows of L(I, b) from Lc
+        // step b: the K index runs over accI's registers, the rows of L(I, b) from Bs
 #pragma unroll
         for (int ntp = 0; ntp < 4; ++ntp)
 #pragma unroll
             for (int qp = 0; qp < 4; ++qp) {
                 const int kx = 16 * ntp + Mfma<T>::row(l4, qp);
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) accD[nt] = Mfma<T>::mac(-Lc[16 * nt + l16][kx], accI[ntp][qp], accD[nt]);
+                for (int nt = 0; nt < 4; ++nt) accD[nt] = Mfma<T>::mac(-Bs[16 * nt + l16][kx], accI[ntp][qp], accD[nt]);
             }
         if (prev)
 #pragma unroll
@@ -784,14 +793,14 @@ __global__ void __launch_bounds__(256) k_big_step(DevSym S, const Task* __restri
             D[j][i] = (i < nbn && j < nbn) ? (i >= j ? accC[nt][q] : T(0)) : (i == j ? T(1) : T(0));
         }
     SDS_STEP_MARK(5);
-    block_potrf_inv<T>(D, rd, &fail, nbn, eps, Lc);
+    block_potrf_inv<T>(D, rd, &fail, nbn, eps, Ln);
     SDS_STEP_MARK(6);
     if (fail) {
         if (tid == 0) status[s] = J + fail;
         return;
     }
     SDS_STEP_MARK(7);
-    block_store<T>(D, Lc, P, f, J, nbn, DB + (int64_t)(cg + J) * NB);
+    block_store<T>(D, Ln, P, f, J, nbn, DB + (int64_t)(cg + J) * NB);
     SDS_STEP_MARK(8);
 }
 
@@ -855,9 +864,13 @@ void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 
 template <typename T>
 void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
-                     T* cbpool, T* DB, T* LN, int32_t* status, double eps) {
-    hipLaunchKernelGGL(k_big_step<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps, nstep,
-                       ncopy);
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ) {
+    if (occ == 2)
+        hipLaunchKernelGGL((k_big_step<T, 2>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps,
+                           nstep, ncopy);
+    else
+        hipLaunchKernelGGL((k_big_step<T, 1>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps,
+                           nstep, ncopy);
 }
 
 template <typename T>
@@ -886,7 +899,7 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);            \
     template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \
     template void launch_big_step<T>(hipStream_t, const DevSym&, const Task*, int, int, int, T*, T*, T*, T*,      \
-                                     int32_t*, double);                                                            \
+                                     int32_t*, double, int);                                                       \
     template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int, \
                                        int);
 INST(double)

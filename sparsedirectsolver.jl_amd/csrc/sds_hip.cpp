@@ -177,6 +177,7 @@ struct sds_handle {
     int ea_rows = EA_ROWS, upd_kc = 32;
     int rest_split = 0;  // trailing tiles of a step from which they get a launch of their own (0: never)
     int max_rest = 0;    // most trailing tiles of one step in the current plan
+    int step_occ2 = 0;   // step launches of at least this many workgroups use the two-per-CU build (0: none)
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -909,7 +910,8 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                         break;
                     case L_STEP:
                         launch_big_step<T>(st, h->dsym, t, ln.n, ln.nstep, ln.ncopy, factor, cb,
-                                           static_cast<T*>(h->dbuf), static_cast<T*>(h->lnbuf), h->status, eps);
+                                           static_cast<T*>(h->dbuf), static_cast<T*>(h->lnbuf), h->status, eps,
+                                           h->step_occ2 > 0 && ln.n >= h->step_occ2 ? 2 : 1);
                         break;
                     case L_UPDATE_PANEL:
                         launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0, h->upd_kc);
@@ -1576,17 +1578,19 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     const int dtype = h->dtype;
     // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
     struct Cand {
-        int ea, kc, rs;
+        int ea, kc, rs, oc;
     };
     std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
         for (int kc : {32, 64})
-            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0});
-    if (h->method == M_CHOLESKY)
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0});
+    if (h->method == M_CHOLESKY) {  // with the best of the above
         for (int rs : {256, 512, 1024})
-            if (rs <= h->max_rest) cand.push_back({-1, -1, rs});  // with the best of the above
+            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0});
+        for (int oc : {256, 512}) cand.push_back({-1, -1, -1, oc});
+    }
     double best = 1e300;
-    Cand pick{h->ea_rows, h->upd_kc, h->rest_split};
+    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2};
     int64_t info = 0;
     int rc;
     for (Cand c : cand) {
@@ -1594,9 +1598,11 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
             c.ea = pick.ea;
             c.kc = pick.kc;
         }
+        if (c.rs < 0) c.rs = pick.rs;
         h->ea_rows = c.ea;
         h->upd_kc = c.kc;
         h->rest_split = c.rs;
+        h->step_occ2 = c.oc;
         if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
         if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
@@ -1611,11 +1617,13 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     h->ea_rows = pick.ea;
     h->upd_kc = pick.kc;
     h->rest_split = pick.rs;
+    h->step_occ2 = pick.oc;
     if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
     if (tiles) {
         tiles[0] = pick.ea;
         tiles[1] = pick.kc;
         tiles[2] = pick.rs;
+        tiles[3] = pick.oc;
     }
     return SDS_SUCCESS;
 }

@@ -18,7 +18,7 @@ void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
                      double eps);
 template <typename T>
 void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
-                     T* cbpool, T* DB, T* LN, int32_t* status, double eps);
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode, int kc);

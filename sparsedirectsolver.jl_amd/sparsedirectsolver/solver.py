@@ -320,16 +320,17 @@ def autotune(solver, reps=2):
     """Choose the large-front kernels' tile shapes for this analysis by timing `reps`
     factorizations of the current values per candidate (libsds_hip's sds_autotune; no
     counterpart in the reference). Returns {"extend_add_rows": r, "cb_k_chunk": kc,
-    "trailing_split": t}; the solver then needs a "factorization" again."""
+    "trailing_split": t, "step_two_per_cu": o}; the solver then needs a "factorization" again."""
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, 'autotune needs "analysis"')
     p, keep = solver._factor_params()
-    tiles = (ctypes.c_int32 * 3)()
+    tiles = (ctypes.c_int32 * 4)()
     _check(solver._lib, solver._h, solver._lib.sds_autotune(solver._h, ctypes.c_void_p(solver.nzval.data_ptr()),
                                                              ctypes.byref(p), int(reps), tiles))
     del keep
     solver.stage = STAGE_ANALYZED
-    return {"extend_add_rows": int(tiles[0]), "cb_k_chunk": int(tiles[1]), "trailing_split": int(tiles[2])}
+    return {"extend_add_rows": int(tiles[0]), "cb_k_chunk": int(tiles[1]), "trailing_split": int(tiles[2]),
+            "step_two_per_cu": int(tiles[3])}
 
 
 def setparam(solver, name, value):
