@@ -33,51 +33,60 @@ namespace sds {
 #pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
-// Linv of the diagonal block [k, k + nb) of a small front: one wave; lane c computes column
-// c of the inverse by forward substitution with the partial sums in registers (fully
-// unrolled: the block is padded with the identity past nb, so no guards touch the register
-// array; no scheduling barriers either: with them the compiler spills this kernel). The
-// large fronts' blocks are inverted by k_big_diag during the factorization. Linv goes to DB.
-// task: (s, k)
+// Linv of the diagonal block [k, k + nb) of a small front: BS lanes per block, 64 / BS blocks
+// per wave (nb <= BS); lane c computes column c of the inverse by forward substitution with the
+// partial sums in registers (fully unrolled: the block is padded with the identity past nb, so
+// no guards touch the register array; no scheduling barriers either: with them the compiler
+// spills this kernel). The padding leaves the entries of the block itself unchanged, so every
+// BS gives the same Linv bit for bit; the host sends each block to the smallest BS that holds
+// it (most small fronts are under 16 columns wide). The large fronts' blocks are inverted by
+// k_big_diag during the factorization. Linv goes to DB (NB rows per column, zero past nb).
+// task: (s, k); a task with c = 1 takes L from DB and writes it back to the panel.
 
-template <typename T>
-__global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __restrict__ tasks,
+template <typename T, int BS>
+__global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __restrict__ tasks, int n,
                                                      T* __restrict__ factor, T* __restrict__ DB) {
-    __shared__ T Lb[NB][NB + 1];  // Lb[m][r] = L(k + r, k + m)
-    __shared__ T rinv[NB];
-    const Task t = tasks[blockIdx.x];
+    constexpr int G = 64 / BS;
+    __shared__ T Lb[G][BS][BS + 1];  // Lb[g][m][r] = L(k + r, k + m) of group g's block
+    __shared__ T rinv[G][BS];
+    const int g = threadIdx.x / BS, c = threadIdx.x % BS;
+    const int id = blockIdx.x * G + g;
+    const bool live = id < n;
+    const Task t = tasks[live ? id : n - 1];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.a;
-    const int nb = min(NB, w - k);
+    const int nb = min(BS, w - k);
     T* P = factor + S.panel_off[s];
     T* blk = DB + (int64_t)(S.super_ptr[s] + k) * NB;
-    const int c = threadIdx.x;
 #pragma unroll
-    for (int m = 0; m < NB; ++m) {
+    for (int m = 0; m < BS; ++m) {
         const int r = c;
         T v = r == m ? T(1) : T(0);
         if (m < nb && r < nb) v = r >= m ? (t.c ? blk[(int64_t)m * NB + r] : P[(int64_t)(k + m) * f + k + r]) : T(0);
-        Lb[m][r] = v;
+        Lb[g][m][r] = v;
     }
     __syncthreads();
-    if (t.c && c < nb)
-        for (int m = 0; m <= c; ++m) P[(int64_t)(k + m) * f + k + c] = Lb[m][c];
-    rinv[c] = T(1) / Lb[c][c];
+    if (live && t.c && c < nb)
+        for (int m = 0; m <= c; ++m) P[(int64_t)(k + m) * f + k + c] = Lb[g][m][c];
+    rinv[g][c] = T(1) / Lb[g][c][c];
     __syncthreads();
-    T x[NB];
+    T x[BS];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) x[r] = r == c ? T(1) : T(0);
+    for (int r = 0; r < BS; ++r) x[r] = r == c ? T(1) : T(0);
 #pragma unroll
-    for (int m = 0; m < NB; ++m) {
-        x[m] = x[m] * rinv[m];
+    for (int m = 0; m < BS; ++m) {
+        x[m] = x[m] * rinv[g][m];
         const T xm = x[m];
 #pragma unroll
-        for (int r = m + 1; r < NB; ++r) x[r] = fma(-Lb[m][r], xm, x[r]);
+        for (int r = m + 1; r < BS; ++r) x[r] = fma(-Lb[g][m][r], xm, x[r]);
     }
     // column c of the block (only the front's own columns: the last block may be narrow)
-    T* out = blk;
-    if (c < nb)
+    if (live && c < nb) {
+        T* out = blk + (int64_t)c * NB;
 #pragma unroll
-        for (int r = 0; r < NB; ++r) out[(int64_t)c * NB + r] = r < nb ? x[r] : T(0);
+        for (int r = 0; r < BS; ++r) out[r] = r < nb ? x[r] : T(0);
+#pragma unroll
+        for (int r = BS; r < NB; ++r) out[r] = T(0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -383,12 +392,16 @@ __global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __r
 
 // X = L(step, step)^-1 for a wide step, one workgroup per (step, column block kk): X_kk = Linv_kk
 // (from DB), then for the blocks below X_ik = -Linv_ii sum_{j = kk}^{i - 1} L_ij X_jk, 64 x 64 x 64
-// products through LDS (thread: a 4 x 4 block). X column-major, ld nw, at xoff.
+// products through LDS (thread: a 4 x 4 block). X column-major, ld nw, at xoff. The operand tiles
+// are loaded into registers in one unrolled batch (16 loads in flight per thread), and the next
+// product's tiles are loaded while the current one is computed: the chain of kk = 0 in a 512-wide
+// step is 35 products long, so each product's load latency is on the critical path.
 // task: (step, kk)
 template <typename T>
 __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep* __restrict__ steps,
                                                      const Task* __restrict__ tasks, const T* __restrict__ factor,
                                                      const T* __restrict__ DB, T* __restrict__ XB) {
+    constexpr int PER = NB * NB / 256;  // tile entries per thread: row tid % NB, columns tid / NB + 4 i
     __shared__ T A[NB][NB + 1];
     __shared__ T B[NB][NB + 1];
     const Task tk = tasks[blockIdx.x];
@@ -399,11 +412,25 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
     T* X = XB + st.xoff;
     const int kc = tk.a * NB, nbk = min(NB, nw - kc);
     const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int c = e / NB, r = e % NB;
-        if (c < nbk && r < nbk) X[(int64_t)(kc + c) * nw + kc + r] = DB[(int64_t)(cg + c0 + kc + c) * NB + r];
+    const int lr = tid % NB, lc = tid / NB;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int c = lc + 4 * i;
+        if (c < nbk && lr < nbk) X[(int64_t)(kc + c) * nw + kc + lr] = DB[(int64_t)(cg + c0 + kc + c) * NB + lr];
     }
-    __syncthreads();
+    // the operands of product (ib, jb): A = L(ic + r, jc + c), B = X(jc + r, kc + c)
+    T ra[PER], rb[PER];
+    auto load = [&](int ib, int jb) {
+        const int ic = ib * NB, jc = jb * NB, nbi = min(NB, nw - ic);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int c = lc + 4 * i;
+            ra[i] = lr < nbi ? P[(int64_t)(c0 + jc + c) * f + c0 + ic + lr] : T(0);
+            rb[i] = c < nbk ? X[(int64_t)(kc + c) * nw + jc + lr] : T(0);
+        }
+    };
+    __syncthreads();  // X_kk is read back as an operand below
+    if (tk.a + 1 < nblk) load(tk.a + 1, tk.a);
     for (int ib = tk.a + 1; ib < nblk; ++ib) {
         const int ic = ib * NB, nbi = min(NB, nw - ic);
         T acc[4][4];
@@ -412,13 +439,14 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
         for (int jb = tk.a; jb < ib; ++jb) {
-            const int jc = jb * NB;  // a full block: jb < ib
-            for (int e = tid; e < NB * NB; e += 256) {
-                const int c = e / NB, r = e % NB;
-                A[r][c] = r < nbi ? P[(int64_t)(c0 + jc + c) * f + c0 + ic + r] : T(0);
-                B[r][c] = c < nbk ? X[(int64_t)(kc + c) * nw + jc + r] : T(0);
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                A[lr][lc + 4 * i] = ra[i];
+                B[lr][lc + 4 * i] = rb[i];
             }
             __syncthreads();
+            // the next product of this block row; X(jb + 1, kk) was written before this row
+            if (jb + 1 < ib) load(ib, jb + 1);
 #pragma unroll 8
             for (int q = 0; q < NB; ++q) {
                 T a4[4], b4[4];
@@ -434,10 +462,13 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
             __syncthreads();
         }
         // X_ik = -Linv_ii S
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int c = e / NB, r = e % NB;
-            A[r][c] = r < nbi && c < nbi ? DB[(int64_t)(cg + c0 + ic + c) * NB + r] : T(0);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int c = lc + 4 * i;
+            ra[i] = lr < nbi && c < nbi ? DB[(int64_t)(cg + c0 + ic + c) * NB + lr] : T(0);
         }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) A[lr][lc + 4 * i] = ra[i];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -467,7 +498,8 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
                 const int r = 4 * tr + a, c = 4 * tc + b;
                 if (r < nbi && c < nbk) X[(int64_t)(kc + c) * nw + ic + r] = -out[a][b];
             }
-        __syncthreads();
+        __syncthreads();  // X_ik is visible to the next block row's loads; the LDS tiles are free
+        if (ib + 1 < nblk) load(ib + 1, tk.a);
     }
 }
 
@@ -630,8 +662,13 @@ __global__ void __launch_bounds__(256) k_copy_rows(DevSym S, const SolveStep* __
 // ---------------------------------------------------------------------------
 
 template <typename T>
-void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB) {
-    hipLaunchKernelGGL(k_block_inverse<T>, dim3(n), dim3(64), 0, st, S, tasks, factor, DB);
+void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int bs) {
+    if (bs == 16)
+        hipLaunchKernelGGL((k_block_inverse<T, 16>), dim3((n + 3) / 4), dim3(64), 0, st, S, tasks, n, factor, DB);
+    else if (bs == 32)
+        hipLaunchKernelGGL((k_block_inverse<T, 32>), dim3((n + 1) / 2), dim3(64), 0, st, S, tasks, n, factor, DB);
+    else
+        hipLaunchKernelGGL((k_block_inverse<T, NB>), dim3(n), dim3(64), 0, st, S, tasks, n, factor, DB);
 }
 
 template <typename T>
@@ -694,7 +731,7 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
 }
 
 #define INST(T)                                                                                                     \
-    template void launch_block_inverse<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                   \
+    template void launch_block_inverse<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);                 \
     template void launch_fwd_diag<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, int, const T*, \
                                      const T*, T*, T*, int);                                                       \
     template void launch_fwd_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*, T*, \

@@ -140,7 +140,8 @@ struct sds_handle {
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
     std::vector<int32_t> cb_split;  // per child: its contribution-block columns that land in the parent's panel
-    Launch inv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the diagonal blocks (Cholesky)
+    // inverses of the diagonal blocks (Cholesky): blocks of at most 16, 32 and NB columns
+    Launch inv_launch[3]{{L_POTRF, 0, 0, 0, 0}, {L_POTRF, 0, 0, 0, 0}, {L_POTRF, 0, 0, 0, 0}};
     SolveStep* d_steps = nullptr;            // solve steps of the Cholesky sweeps
     Launch sinv_launch{L_POTRF, 0, 0, 0, 0};  // inverses of the wide steps' diagonal blocks
     void* xbuf = nullptr;                    // those inverses
@@ -792,17 +793,20 @@ int build_schedule(sds_handle* h, int dtype) {
     // Cholesky: the diagonal-block inverses and the inverse-block sweep plans
     h->part_units = 0;
     if (h->method == M_CHOLESKY) {
-        std::vector<Task> t;
-        double fl = 0, by = 0;
+        std::vector<Task> t[3];
+        double fl[3] = {0, 0, 0}, by[3] = {0, 0, 0};
         for (int64_t s = 0; s < ns; ++s)
             for (int k = 0; k < h->fncols[s] && !big[s]; k += NB) {  // big fronts: done by k_big_diag
-                t.push_back({(int32_t)s, k, 0, 0});
-                const double nb = std::min(NB, h->fncols[s] - k);
-                fl += nb * nb * nb / 3.0;
-                by += dbl_es * (nb * (nb + 1) / 2 + (double)NB * nb);
+                const int nbi = std::min(NB, h->fncols[s] - k), cls = nbi <= 16 ? 0 : nbi <= 32 ? 1 : 2;
+                t[cls].push_back({(int32_t)s, k, 0, 0});
+                const double nb = nbi;
+                fl[cls] += nb * nb * nb / 3.0;
+                by[cls] += dbl_es * (nb * (nb + 1) / 2 + (double)NB * nb);
             }
-        h->inv_launch = {L_POTRF, (int64_t)tasks.size(), (int)t.size(), fl, by};
-        tasks.insert(tasks.end(), t.begin(), t.end());
+        for (int cls = 0; cls < 3; ++cls) {
+            h->inv_launch[cls] = {L_POTRF, (int64_t)tasks.size(), (int)t[cls].size(), fl[cls], by[cls]};
+            tasks.insert(tasks.end(), t[cls].begin(), t[cls].end());
+        }
         std::vector<SolveStep> steps;
         std::vector<Task> ts;  // (step, column block) of the wide steps' inverses
         int64_t xlen = 0;
@@ -926,11 +930,14 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
     // the inverses of the diagonal blocks for the sweeps (a failed factorization gives
     // inverses of whatever the panels hold; its solves are not meaningful anyway)
-    if (h->inv_launch.n)
-        timed(h, SDS_K_INVERSE, h->inv_launch.flops, h->inv_launch.bytes, [&] {
-            launch_block_inverse<T>(st, h->dsym, h->d_tasks + h->inv_launch.off, h->inv_launch.n, factor,
-                                    static_cast<T*>(h->dbuf));
-        });
+    for (int cls = 0; cls < 3; ++cls) {
+        const Launch& il = h->inv_launch[cls];
+        if (il.n)
+            timed(h, SDS_K_INVERSE, il.flops, il.bytes, [&] {
+                launch_block_inverse<T>(st, h->dsym, h->d_tasks + il.off, il.n, factor, static_cast<T*>(h->dbuf),
+                                        cls == 0 ? 16 : cls == 1 ? 32 : NB);
+            });
+    }
     if (h->sinv_launch.n)
         timed(h, SDS_K_INVERSE, h->sinv_launch.flops, h->sinv_launch.bytes, [&] {
             launch_step_inverse<T>(st, h->dsym, h->d_steps, h->d_tasks + h->sinv_launch.off, h->sinv_launch.n, factor,

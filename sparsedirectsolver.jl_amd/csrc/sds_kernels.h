@@ -101,7 +101,7 @@ void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
                      int unit);
 // Cholesky sweeps with inverted diagonal blocks (k_solve_inv.hip)
 template <typename T>
-void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB);
+void launch_block_inverse(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int bs);
 template <typename T>
 void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, int threads,
                      const T* factor, const T* DB, T* Y, T* U, int nrhs);
