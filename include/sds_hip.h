@@ -148,6 +148,21 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * Refinement mode of sds_solve: SDS_IR_PLAIN (0, the steps above) or SDS_IR_FGMRES (1):
+ * `ir_mode = "ir" | "fgmres"` (src/types.jl:185, :306; src/solver.jl:385-392, :907).
+ * With FGMRES the refinement phase replaces `fgmres_refine!` (src/solve/refinement.jl:554-620):
+ * R0 = B - op(A) X; when ir_tol > 0 and every ||R0_k||_2 <= ir_tol ||B_k||_2 nothing more is
+ * done; else FGMRES on op(A) D = R0 over the n nrhs stacked columns, right-preconditioned by the
+ * factorization, for at most ir_n_steps iterations (one SpMV and one solve each), stopping once
+ * ||R0 - op(A) D||_2 <= ir_tol min_k ||B_k||_2 (ir_tol = 0: all iterations), then X += D;
+ * *ir_steps gets the iterations. It synchronizes with the host once per iteration, so
+ * sds_graph_capture refuses it (SDS_NOT_SUPPORTED) when refinement is captured.
+ */
+#define SDS_IR_PLAIN 0
+#define SDS_IR_FGMRES 1
+int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
+
+/*
  * Tile-shape autotuner (no counterpart in the reference, whose kernels have fixed shapes;
  * the chosen form of level and batch launches whose tile shapes come from an autotuner):
  * times `reps` factorizations of the values nzval for each candidate shape of the large-front

@@ -421,7 +421,8 @@ __global__ void k_diag(int n, int nrhs, const T* __restrict__ d, const int8_t* _
 }
 
 // R = B - M X over the full-pattern map (rowptr/colval/src 0-based; src < 0: -(p + 1), conjugate),
-// FP64; one thread per (row, rhs); `roff` selects the rows of M^T of a "G" map
+// FP64; one thread per (row, rhs); `roff` selects the rows of M^T of a "G" map. B = nullptr gives
+// the product R = M X (the operator of FGMRES refinement).
 __global__ void k_residual(int n, int nrhs, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
                            const int64_t* __restrict__ src, const double* __restrict__ nz, const double* __restrict__ X,
                            int64_t ldx, const double* __restrict__ B, int64_t ldb, double* __restrict__ R) {
@@ -434,7 +435,66 @@ __global__ void k_residual(int n, int nrhs, const int64_t* __restrict__ rowptr, 
             const double v = nz[p >= 0 ? p : -(p + 1)];
             acc += v * X[colval[e] + r * ldx];
         }
-        R[q] = B[i + r * ldb] - acc;
+        R[q] = B ? B[i + r * ldb] - acc : acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FGMRES refinement: vector kernels over the n nrhs stacked columns, FP64. Reductions run in a
+// fixed order (DOT_BLOCKS grid-stride partial sums, then one workgroup's tree), so every run
+// gives the same bits.
+
+__device__ inline double block_sum256(double v, double* red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// part[block] = partial sum of x_i y_i
+__global__ void __launch_bounds__(256) k_dot_partial(int64_t len, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256)
+        acc = fma(x[i], y[i], acc);
+    const double v = block_sum256(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// out[0] = sum of part[0, np), np <= 256
+__global__ void __launch_bounds__(256) k_dot_final(int np, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double red[256];
+    const double v = block_sum256(threadIdx.x < np ? part[threadIdx.x] : 0.0, red);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// y -= a[0] x (the Gram-Schmidt coefficient stays on the device)
+__global__ void k_axpy_neg(int64_t len, double* __restrict__ y, const double* __restrict__ x,
+                           const double* __restrict__ a) {
+    const double s = -a[0];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = fma(s, x[i], y[i]);
+}
+
+__global__ void k_scale(int64_t len, double* __restrict__ y, double s) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] *= s;
+}
+
+// X(:, r) += sum_j c_j Z_j(:, r), the Z_j stacked n nrhs vectors at stride zs
+__global__ void k_add_combination(int n, int nrhs, double* __restrict__ X, int64_t ldx, const double* __restrict__ Z,
+                                  int64_t zs, int k, const double* __restrict__ c) {
+    const int64_t total = (int64_t)n * nrhs;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(q / n), i = (int)(q - (int64_t)r * n);
+        double acc = 0.0;
+        for (int j = 0; j < k; ++j) acc = fma(c[j], Z[j * zs + q], acc);
+        X[i + r * ldx] += acc;
     }
 }
 
@@ -527,6 +587,25 @@ void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t
                             double* grad) {
     hipLaunchKernelGGL(k_adjoint_scatter, dim3(grid_for(nnz)), dim3(256), 0, st, nnz, nrhs, ai, aj, aflag, lam, ldl, X,
                        ldx, grad);
+}
+
+void launch_dot(hipStream_t st, int64_t len, const double* x, const double* y, double* part, double* out) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(DOT_BLOCKS), dim3(256), 0, st, len, x, y, part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, DOT_BLOCKS, part, out);
+}
+
+void launch_axpy_neg(hipStream_t st, int64_t len, double* y, const double* x, const double* a) {
+    hipLaunchKernelGGL(k_axpy_neg, dim3(grid_for(len)), dim3(256), 0, st, len, y, x, a);
+}
+
+void launch_scale(hipStream_t st, int64_t len, double* y, double s) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(len)), dim3(256), 0, st, len, y, s);
+}
+
+void launch_add_combination(hipStream_t st, int n, int nrhs, double* X, int64_t ldx, const double* Z, int64_t zs, int k,
+                            const double* c) {
+    hipLaunchKernelGGL(k_add_combination, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, X, ldx, Z, zs,
+                       k, c);
 }
 
 void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,

@@ -193,6 +193,11 @@ struct sds_handle {
     int cap_nrhs = 0;
     DevBuf Y, Z, U, R, Bc, norms;
     double* h_norms = nullptr;  // pinned
+    // refinement: 0 plain steps, 1 FGMRES (sds_set_ir_mode); the FGMRES Krylov vectors
+    int ir_mode = 0;
+    DevBuf kry;
+    double* h_kry = nullptr;  // pinned, h_kry_cap entries
+    size_t h_kry_cap = 0;
     int fsmall = 0;
     // captured refactorization + solve (sds_graph_capture), on a private capture stream
     hipStream_t cstream = nullptr;
@@ -376,7 +381,10 @@ void free_numeric(sds_handle* h) {
     if (h->h_norms) (void)hipHostFree(h->h_norms);
     h->h_info = nullptr;
     h->h_norms = nullptr;
-    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms, &h->part}) {
+    if (h->h_kry) (void)hipHostFree(h->h_kry);
+    h->h_kry = nullptr;
+    h->h_kry_cap = 0;
+    for (DevBuf* b : {&h->Y, &h->Z, &h->U, &h->R, &h->Bc, &h->norms, &h->part, &h->kry}) {
         if (b->p) (void)hipFree(b->p);
         *b = DevBuf{};
     }
@@ -1170,6 +1178,136 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
         }
 }
 
+// FGMRES refinement (ir_mode "fgmres", src/solve/refinement.jl:554-620 with the Krylov.jl FGMRES
+// it drives): R0 = B - op(A) X; with tol > 0, nothing to do when every column has
+// ||R0_k|| <= tol ||B_k||; else FGMRES on op(A) D = R0 over the n nrhs stacked columns, right
+// preconditioned by the factorization (one full solve per iteration), at most nsteps iterations,
+// stopping once ||R0 - op(A) D|| <= tol min_k ||B_k|| (tol = 0: all nsteps, or an exact
+// residual); then X += D. Arnoldi by modified Gram-Schmidt with the coefficients on the device;
+// the Hessenberg least-squares problem by Givens rotations on the host, one synchronization per
+// iteration (the reference's dot products synchronize too). Returns the iterations performed.
+template <typename T>
+int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, double* X, int64_t ldx, int nrhs,
+             const double* nz, int m, double tol, int* iters) {
+    hipStream_t st = h->stream;
+    const int n = (int)h->n;
+    const int64_t N = (int64_t)n * nrhs;
+    *iters = 0;
+    // V_0..V_m, Z_0..Z_{m-1}, the Hessenberg columns (m + 2 per column), DOT_BLOCKS partials, then
+    // ns scalars: the 2 nrhs column norms, later the m solution coefficients
+    const int hc = m + 2, ns = std::max(2 * nrhs, m) + 8;
+    const size_t need = sizeof(double) * ((size_t)(2 * m + 1) * N + (size_t)hc * (m + 1) + DOT_BLOCKS + ns);
+    if (h->kry.bytes < need) {
+        if (h->kry.p) (void)hipFree(h->kry.p);
+        h->kry = DevBuf{};
+        hipError_t e = hipMalloc(&h->kry.p, need);
+        if (e != hipSuccess) return fail(h, SDS_DEVICE_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+        h->kry.bytes = need;
+    }
+    const size_t hcap = (size_t)std::max(ns, hc) + 8;
+    if (h->h_kry_cap < hcap) {
+        if (h->h_kry) (void)hipHostFree(h->h_kry);
+        h->h_kry = nullptr;
+        h->h_kry_cap = 0;
+        HIPCHK(h, hipHostMalloc((void**)&h->h_kry, sizeof(double) * hcap, hipHostMallocDefault));
+        h->h_kry_cap = hcap;
+    }
+    double* V = static_cast<double*>(h->kry.p);
+    double* Zk = V + (int64_t)(m + 1) * N;
+    double* H = Zk + (int64_t)m * N;  // column j at H + j hc
+    double* part = H + (int64_t)hc * (m + 1);
+    double* cn = part + DOT_BLOCKS;   // ns scalars
+    const int64_t* rp = h->full_rowptr + (transposed && h->method == M_LU ? h->n : 0);
+    T* Y = static_cast<T*>(h->Y.p);
+    const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
+    auto spmv = [&](const double* x, int64_t ldxx, const double* b, int64_t ldb, double* out) {
+        timed(h, SDS_K_RESIDUAL, 2.0 * h->nnz_full * nrhs, (double)h->nnz_full * (8 + 4 + 8) + 3.0 * 8 * n * nrhs,
+              [&] { launch_residual(st, n, nrhs, rp, h->full_colval, h->full_src, nz, x, ldxx, b, ldb, out); });
+    };
+    auto read = [&](const double* d, int cnt, double* hst) -> int {
+        HIPCHK(h, hipMemcpyAsync(h->h_kry, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        std::copy(h->h_kry, h->h_kry + cnt, hst);
+        return SDS_SUCCESS;
+    };
+    int rc;
+    spmv(X, ldx, Bsrc, ldbs, V);  // R0
+    double atol = 0.0;
+    if (tol > 0) {
+        for (int r = 0; r < nrhs; ++r) {
+            launch_dot(st, n, V + (int64_t)r * n, V + (int64_t)r * n, part, cn + r);
+            launch_dot(st, n, Bsrc + r * ldbs, Bsrc + r * ldbs, part, cn + nrhs + r);
+        }
+        std::vector<double> nr(2 * nrhs);
+        if ((rc = read(cn, 2 * nrhs, nr.data()))) return rc;
+        bool ok = true;
+        double bmin = INFINITY;
+        for (int r = 0; r < nrhs; ++r) {
+            if (std::sqrt(nr[r]) > tol * std::sqrt(nr[nrhs + r])) ok = false;
+            bmin = std::min(bmin, std::sqrt(nr[nrhs + r]));
+        }
+        if (ok) return SDS_SUCCESS;
+        atol = tol * bmin;
+    }
+    launch_dot(st, N, V, V, part, cn);
+    double beta;
+    if ((rc = read(cn, 1, &beta))) return rc;
+    beta = std::sqrt(beta);
+    if (!(beta > atol) || beta == 0.0) return SDS_SUCCESS;
+    launch_scale(st, N, V, 1.0 / beta);
+    // the rotated Hessenberg matrix (column-major, m + 1 rows), the rotations and g = Q^T beta e1
+    std::vector<double> Rm((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), col(hc);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+        double* Vj = V + (int64_t)j * N;
+        double* Vn = V + (int64_t)(j + 1) * N;
+        double* Zj = Zk + (int64_t)j * N;
+        // Z_j = op(A)^-1 V_j by the factorization, then V_{j+1} = op(A) Z_j, orthogonalized
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, Vj, n, Y); });
+        sweeps<T>(h, nrhs, 1, 1, 1, transposed);
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, Zj, n, 0); });
+        spmv(Zj, n, nullptr, 0, Vn);
+        for (int i = 0; i <= j; ++i) {
+            launch_dot(st, N, Vn, V + (int64_t)i * N, part, H + (int64_t)j * hc + i);
+            launch_axpy_neg(st, N, Vn, V + (int64_t)i * N, H + (int64_t)j * hc + i);
+        }
+        launch_dot(st, N, Vn, Vn, part, H + (int64_t)j * hc + j + 1);
+        if ((rc = read(H + (int64_t)j * hc, j + 2, col.data()))) return rc;
+        const double hn = std::sqrt(col[j + 1]);
+        if (hn > 0) launch_scale(st, N, Vn, 1.0 / hn);
+        // apply the earlier rotations to the new column, then the one that zeroes hn
+        double* rj = Rm.data() + (size_t)j * (m + 1);
+        for (int i = 0; i <= j; ++i) rj[i] = col[i];
+        for (int i = 0; i < j; ++i) {
+            const double a = cs[i] * rj[i] + sn[i] * rj[i + 1];
+            rj[i + 1] = -sn[i] * rj[i] + cs[i] * rj[i + 1];
+            rj[i] = a;
+        }
+        const double rr = std::hypot(rj[j], hn);
+        cs[j] = rr > 0 ? rj[j] / rr : 1.0;
+        sn[j] = rr > 0 ? hn / rr : 0.0;
+        rj[j] = rr;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        k = j + 1;
+        if (std::fabs(g[j + 1]) <= atol || hn == 0.0) break;
+    }
+    // y = R^-1 g, then X += sum_j y_j Z_j
+    std::vector<double> y(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+        double v = g[i];
+        for (int l = i + 1; l < k; ++l) v -= Rm[(size_t)l * (m + 1) + i] * y[l];
+        y[i] = Rm[(size_t)i * (m + 1) + i] != 0 ? v / Rm[(size_t)i * (m + 1) + i] : 0.0;
+    }
+    std::copy(y.begin(), y.end(), h->h_kry);
+    HIPCHK(h, hipMemcpyAsync(cn, h->h_kry, sizeof(double) * k, hipMemcpyHostToDevice, st));
+    launch_add_combination(st, n, nrhs, X, ldx, Zk, N, k, cn);
+    HIPCHK(h, hipStreamSynchronize(st));  // h_kry is read by the copy above
+    *iters = k;
+    return SDS_SUCCESS;
+}
+
 template <typename T>
 int solve_t(sds_handle* h, int32_t phase, int transposed, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
             const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
@@ -1196,6 +1334,14 @@ int solve_t(sds_handle* h, int32_t phase, int transposed, const double* B, int64
     if (refine) {
         if (!h->full_rowptr) return fail(h, SDS_INVALID_VALUE, "refinement needs the full-pattern map at upload");
         if (!nz) return fail(h, SDS_INVALID_VALUE, "refinement needs the matrix values");
+        if (h->ir_mode == SDS_IR_FGMRES) {
+            if ((rc = fgmres_t<T>(h, transposed, Bsrc, Bsrc == B ? ldb : n, X, ldx, nrhs, nz, ir_n_steps, ir_tol,
+                                  &done)))
+                return rc;
+            if (ir_steps) *ir_steps = done;
+            HIPCHK(h, hipGetLastError());
+            return SDS_SUCCESS;
+        }
         double* R = static_cast<double*>(h->R.p);
         for (int it = 0; it < ir_n_steps; ++it) {
             // one SpMV: values + column + source index per entry, X gathered, B read, R written
@@ -1267,6 +1413,13 @@ const char* sds_last_error(sds_handle_t h) { return h ? h->err.c_str() : "null h
 int sds_set_stream(sds_handle_t h, void* stream) {
     if (!h) return SDS_INVALID_VALUE;
     h->stream = static_cast<hipStream_t>(stream);
+    return SDS_SUCCESS;
+}
+
+int sds_set_ir_mode(sds_handle_t h, int32_t mode) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (mode != SDS_IR_PLAIN && mode != SDS_IR_FGMRES) return fail(h, SDS_INVALID_VALUE, "ir_mode must be 0 or 1");
+    h->ir_mode = mode;
     return SDS_SUCCESS;
 }
 
@@ -1643,6 +1796,8 @@ int sds_graph_capture(sds_handle_t h, const double* nzval, const sds_factor_para
     int rc = check_solve_args(h, phase, solve_mode, ldb, ldx, nrhs);
     if (rc) return rc;
     if (!B || !X) return fail(h, SDS_INVALID_VALUE, "B and X are needed");
+    if (h->ir_mode == SDS_IR_FGMRES && (phase & SDS_PHASE_SOLVE_REFINEMENT) && ir_n_steps > 0)
+        return fail(h, SDS_NOT_SUPPORTED, "FGMRES refinement synchronizes with the host: it cannot be captured");
     // every allocation happens before the capture
     if ((rc = ensure_solve_workspace(h, nrhs))) return rc;
     graph_free(h);

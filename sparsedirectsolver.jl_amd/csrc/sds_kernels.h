@@ -132,6 +132,14 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
 void launch_adjoint_scatter(hipStream_t st, int64_t nnz, int nrhs, const int32_t* ai, const int32_t* aj,
                             const int8_t* aflag, const double* lam, int64_t ldl, const double* X, int64_t ldx,
                             double* grad);
+// FGMRES refinement vectors (k_solve.hip): out[0] = x . y (part: DOT_BLOCKS partial sums), y -= a[0] x,
+// y *= s, X(:, r) += sum_j c_j Z_j(:, r)
+constexpr int DOT_BLOCKS = 256;
+void launch_dot(hipStream_t st, int64_t len, const double* x, const double* y, double* part, double* out);
+void launch_axpy_neg(hipStream_t st, int64_t len, double* y, const double* x, const double* a);
+void launch_scale(hipStream_t st, int64_t len, double* y, double s);
+void launch_add_combination(hipStream_t st, int n, int nrhs, double* X, int64_t ldx, const double* Z, int64_t zs, int k,
+                            const double* c);
 void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, const int32_t* colval, const int64_t* src,
                      const double* nz, const double* X, int64_t ldx, const double* B, int64_t ldb, double* R);
 

@@ -26,6 +26,8 @@ CONFIG_PARAMETERS = ("reordering_alg", "factorization_alg", "solve_alg", "matchi
                      "hybrid_execute_mode", "pivot_epsilon_alg", "nd_nlevels", "ubatch_size", "ubatch_index",
                      "use_superpanels", "device_count", "device_indices", "schur_mode", "deterministic_mode",
                      "nd_ubfactor", "nd_nseps", "nd_seed")
+# refinement modes (src/types.jl:306): plain steps, or FGMRES preconditioned by the factorization
+IR_MODES = {"ir": 0, "fgmres": 1}
 EXTRA_CONFIG = ("ir_mode", "factor_precision", "amalgamation", "schedule", "pivot_pairs", "pivot_pair_tolerance")
 USER_DATA = ("user_perm", "user_host_interrupt", "pivot_sign", "user_schur_indices", "user_nd_partition_tree",
              "ubatch_mask")
@@ -50,7 +52,7 @@ def _defaults():
         "host_nthreads": 0, "hybrid_execute_mode": 0, "nd_nlevels": 10, "ubatch_size": 0, "ubatch_index": -1,
         "use_superpanels": 1, "device_count": 1, "device_indices": None, "schur_mode": 0,
         "deterministic_mode": 0, "nd_ubfactor": -1, "nd_nseps": -1, "nd_seed": -1,
-        "ir_mode": "plain", "factor_precision": None, "amalgamation": (32, 0.25, 8), "schedule": "auto",
+        "ir_mode": "ir", "factor_precision": None, "amalgamation": (32, 0.25, 8), "schedule": "auto",
         "pivot_pairs": "default", "pivot_pair_tolerance": 1e-6,
         "user_perm": None, "user_host_interrupt": None, "pivot_sign": None, "user_schur_indices": None,
         "user_nd_partition_tree": None, "ubatch_mask": None,
@@ -242,6 +244,7 @@ class DirectSolver:
         o = self.options
         steps = ctypes.c_int32(0)
         nsteps = int(o["ir_n_steps"]) if bits & PHASES["solve_refinement"] else 0
+        _check(self._lib, self._h, self._lib.sds_set_ir_mode(self._h, IR_MODES[o["ir_mode"]]))
         _check(self._lib, self._h, self._lib.sds_solve(
             self._h, bits, int(o["solve_mode"]), ctypes.c_void_p(Bc.data_ptr()), self.n,
             ctypes.c_void_p(Xc.data_ptr()), self.n, nrhs, ctypes.c_void_p(self.nzval.data_ptr()), nsteps,
@@ -286,6 +289,7 @@ def graph_capture(solver, nrhs=1, phase="solve"):
     p, keep = solver._factor_params()
     o = solver.options
     nsteps = int(o["ir_n_steps"]) if PHASES[phase] & PHASES["solve_refinement"] else 0
+    _check(solver._lib, solver._h, solver._lib.sds_set_ir_mode(solver._h, IR_MODES[o["ir_mode"]]))
     _check(solver._lib, solver._h, solver._lib.sds_graph_capture(
         solver._h, ctypes.c_void_p(solver.nzval.data_ptr()), ctypes.byref(p), PHASES[phase], int(o["solve_mode"]),
         ctypes.c_void_p(solver._gB.data_ptr()), n, ctypes.c_void_p(solver._gX.data_ptr()), n, nrhs, nsteps))
@@ -347,6 +351,8 @@ def setparam(solver, name, value):
             raise NotSupportedError("global pivoting ('C', 'R') is not supported")
     elif name in ("ir_n_steps",) and (not isinstance(value, (int, np.integer)) or value < 0):
         raise InvalidValueError("ir_n_steps must be a nonnegative integer")
+    elif name == "ir_mode" and value not in IR_MODES:
+        raise InvalidValueError(f"ir_mode must be one of {', '.join(IR_MODES)}, got {value!r}")
     elif name in ("ir_tol", "pivot_threshold") and not (float(value) >= 0):
         raise InvalidValueError(f"{name} must be nonnegative")
     elif name == "pivot_epsilon" and value is not None and not (float(value) >= 0):
