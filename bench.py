@@ -363,7 +363,7 @@ def bench_kkt(args):
     """The KKT system [H J^T; J -delta I] (delta = 1e-8; H random sparse symmetric, about 10 entries
     per row, diagonal = row-abs-sum + U[0,1]; J with 4 entries per row at uniform random columns),
     LDL^T "S" with Bunch-Kaufman pivoting, threshold 0.01, AMD ordering, one right-hand side
-    b ~ N(0, 1). Timed: refactorization + solve, FP64 and FP32 factor + FP64 refinement. Checked:
+    b ~ N(0, 1). Timed: refactorization + solve, FP64, and FP32 factor + FP64 refinement (plain and FGMRES). Checked:
     inertia (nh, nj, 0), and the FP64 pivot order, pivot kinds and D against the CPU oracle's
     LDL^T on the same matrix and analysis (also the CPU baseline). The random structure fills in
     like a dense matrix (nnz(L) grows about as n^2), so the default size is nh = 8000, nj = 4000."""
@@ -382,20 +382,25 @@ def bench_kkt(args):
     amal = (512, 0.25, 8)
     cases = []
     s64 = None
-    for fp in ("Float64", "Float32"):
+    # the FP32 factor perturbs many tiny pivots: plain refinement stalls there, FGMRES (preconditioned
+    # by the same factor, stopping at a relative residual of 1e-14) is the form meant for it
+    for fp, mode in (("Float64", "ir"), ("Float32", "ir"), ("Float32", "fgmres")):
         s = sds.DirectSolver((rp, cv, nz, n), "S", "L")
         sds.setparam(s, "amalgamation", amal)
         sds.setparam(s, "pivot_type", "A")
         sds.setparam(s, "pivot_threshold", 0.01)
         if fp == "Float32":
             sds.setparam(s, "factor_precision", "Float32")
-        # delta = 1e-8 makes the system ill-conditioned: both forms take refinement steps
-        sds.setparam(s, "ir_n_steps", args.ir_steps)
+        # delta = 1e-8 makes the system ill-conditioned: every form takes refinement steps
+        sds.setparam(s, "ir_mode", mode)
+        sds.setparam(s, "ir_n_steps", args.ir_steps if mode == "ir" else 20)
+        if mode == "fgmres":
+            sds.setparam(s, "ir_tol", 1e-14)
         sds.execute("analysis", s)
         t0 = time.perf_counter()
         sds.execute("factorization", s)
         torch.cuda.synchronize()
-        print(f"kkt n={n} {fp}: analysis + first factorization done ({time.perf_counter() - t0:.2f} s)",
+        print(f"kkt n={n} {fp} {mode}: analysis + first factorization done ({time.perf_counter() - t0:.2f} s)",
               file=sys.stderr, flush=True)
         x = torch.zeros_like(b)
         for _ in range(args.warmup):
@@ -412,13 +417,16 @@ def bench_kkt(args):
         xh = x.cpu().numpy()
         berr = backward_error(K, xh[:, None], bh[:, None])
         npos, nneg = sds.getparam(s, "inertia")
-        cases.append({"factor": fp, "ir_steps": args.ir_steps, "ms_per_step": round(ms, 3), "refactor_ms": round(t_ref, 3),
+        cases.append({"factor": fp, "ir_mode": mode, "ir_steps": int(sds.getparam(s, "ir_n_steps")),
+                      "ms_per_step": round(ms, 3), "refactor_ms": round(t_ref, 3),
                       "backward_error": berr, "inertia": [int(npos), int(nneg), int(n - npos - nneg)],
                       "inertia_expected": [nh, nj, 0], "npivots": int(sds.getparam(s, "npivots")),
                       "pivots_2x2": sds.getparam(s, "pivot_stats")["n2x2"]})
         print(json.dumps(cases[-1]), file=sys.stderr, flush=True)
         if fp == "Float64":
             s64 = s
+        else:
+            del s
     S = s64.symbolic
     res = {"metric": "KKT LDL^T (Bunch-Kaufman) refactor+solve ms, inertia, FP64 pivots vs the CPU oracle",
            "value": cases[0]["ms_per_step"], "unit": "ms", "n_gpus": 1, "steps": args.steps,

@@ -123,6 +123,9 @@ end
 function solve!(h::Handle, phase::Phase, X::ROCMatrix{Float64}, B::ROCMatrix{Float64},
                 nzval::ROCVector{Float64}, opts)
     steps = Ref{Int32}(0)
+    # ir_mode "ir" / "fgmres" (src/types.jl:306): IR_PLAIN = 0, IR_FGMRES = 1 in the C ABI
+    _check(h, ccall((:sds_set_ir_mode, libsds_hip), Cint, (Ptr{Cvoid}, Cint), h.ptr,
+                    Cint(opts.ir_mode == IR_FGMRES ? 1 : 0)))
     _check(h, ccall((:sds_solve, libsds_hip), Cint,
                     (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Cint, Ptr{Float64}, Cint,
                      Float64, Ref{Int32}),
