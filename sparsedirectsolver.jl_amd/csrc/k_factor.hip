@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
             const bool in = q < nj && ii >= jj;
             // parent entry (pi, pj): panel column pj < w, else the contribution block
             dst[q] = !in ? nullptr : pj[q] < w ? P + (int64_t)pj[q] * f + pi : Cs + (int64_t)(pj[q] - w) * m + (pi - w);
-            v[q] = in ? Cc[(int64_t)jj * mc + ii] : T(0);
+            // the child's block is read once: keep it out of the caches the parent's entries use
+            v[q] = in ? __builtin_nontemporal_load(Cc + (int64_t)jj * mc + ii) : T(0);
         }
 #pragma unroll
         for (int q = 0; q < EA_COLS; ++q) old[q] = dst[q] ? *dst[q] : T(0);
