@@ -273,7 +273,8 @@ __global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __r
         for (int jb = 16 * wv; jb < nw; jb += 16 * NWV) {
             T l[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) l[jj] = pi[(int64_t)(c0 + min(jb + jj, nw - 1)) * f];
+            for (int jj = 0; jj < 16; ++jj)  // the factor streams through once per sweep
+                l[jj] = __builtin_nontemporal_load(pi + (int64_t)(c0 + min(jb + jj, nw - 1)) * f);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const T lv = jb + jj < nw ? l[jj] : T(0);
@@ -336,7 +337,7 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
             const T* pc = P + (int64_t)col0 * f + i;  // column col0 + c; columns past c1 read c1 - 1
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const T l = *pc;
+                const T l = __builtin_nontemporal_load(pc);  // the factor streams through once per sweep
                 pc += col0 + c + 1 < c1 ? f : 0;
 #pragma unroll
                 for (int r = 0; r < 8; ++r) a[c * 8 + r] += l * x[r];
