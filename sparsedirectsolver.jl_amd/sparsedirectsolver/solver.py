@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _abi
 from ._native import hip_lib
-from .analysis import symbolic_analysis
+from .analysis import merge_symbolic, symbolic_analysis
 from .enums import PHASES, PIVOT_TYPES, STRUCTURES, VIEWS
 from .errors import FactorizationError, InterruptedError_, InvalidValueError, NotSupportedError
 
@@ -106,6 +106,27 @@ class DirectSolver:
         self.rowptr = np.asarray(rowptr, dtype=np.int64)
         self.colval = np.asarray(colval, dtype=np.int64)
         self.index = index
+        # uniform batch (src/solver.jl:135-150): nbatch matrices with this pattern, values member
+        # after member (a vector of nbatch nnz entries or an nnz x nbatch matrix); the members are
+        # factored as the blocks of one block-diagonal matrix whose elimination forest holds
+        # nbatch copies of the member's analysis, so each level is one set of launches for all
+        nnz = self.colval.shape[0]
+        if getattr(nzval, "ndim", 1) == 2:
+            if nzval.shape[0] != nnz:
+                raise InvalidValueError(f"nzval is {tuple(nzval.shape)}, the pattern has {nnz} entries")
+            nzval = nzval.t().reshape(-1) if isinstance(nzval, torch.Tensor) else np.asarray(nzval).T.reshape(-1)
+        count = int(nzval.numel() if isinstance(nzval, torch.Tensor) else np.size(nzval))
+        if nnz == 0 or count % nnz != 0 or count == 0:
+            raise InvalidValueError(f"nzval has {count} entries, the pattern has {nnz}")
+        self.nbatch = count // nnz
+        self.n_member = self.n
+        if self.nbatch > 1:
+            nb, b = self.nbatch, (1 if index == "O" else 0)
+            rp0 = self.rowptr - b
+            self.rowptr = np.concatenate([rp0[:-1] + k * nnz for k in range(nb)] + [[nb * nnz]]) + b
+            self.colval = np.concatenate([self.colval + k * self.n for k in range(nb)])
+            self.n = self.n * nb
+            self._member_pattern = (rp0 + b, self.colval[:nnz])
         self._base = base
         self.structure = structure
         self.view = view
@@ -122,6 +143,8 @@ class DirectSolver:
 
     def _set_values(self, nzval):
         torch = _torch()
+        if getattr(nzval, "ndim", 1) == 2:  # nnz x nbatch: member after member
+            nzval = nzval.t().reshape(-1) if isinstance(nzval, torch.Tensor) else np.asarray(nzval).T.reshape(-1)
         if isinstance(nzval, torch.Tensor):
             if nzval.dtype != torch.float64:
                 raise NotSupportedError("matrix values must be Float64 (factor_precision selects an FP32 factor)")
@@ -158,12 +181,23 @@ class DirectSolver:
             self.symbolic = self._analysis(o)
             self.stage = STAGE_REORDERED
             return
+        if o["ubatch_size"] not in (0, self.nbatch):
+            raise InvalidValueError(f"ubatch_size = {o['ubatch_size']}, but the matrix values hold {self.nbatch} "
+                                    "batch members")
         ordering = {"default": "default", "algo3": "algo3", "algo5": "algo5"}.get(o["reordering_alg"])
         if ordering is None and o["user_perm"] is None:
             raise NotSupportedError(f"reordering_alg = {o['reordering_alg']!r} is not available in this build")
-        self.symbolic = symbolic_analysis(self.rowptr, self.colval, self.n, self.structure, self.view,
-                                          self.index, perm=o["user_perm"], ordering=ordering or "default",
-                                          amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"])
+        if self.nbatch > 1:
+            rp, cv = self._member_pattern
+            S1 = symbolic_analysis(rp, cv, self.n_member, self.structure, self.view, self.index, perm=o["user_perm"],
+                                   ordering=ordering or "default", amalgamation=o["amalgamation"],
+                                   use_superpanels=o["use_superpanels"])
+            self.symbolic = merge_symbolic([S1] * self.nbatch)
+            self._ns_member = S1.ns
+        else:
+            self.symbolic = symbolic_analysis(self.rowptr, self.colval, self.n, self.structure, self.view,
+                                              self.index, perm=o["user_perm"], ordering=ordering or "default",
+                                              amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"])
         self.stage = STAGE_REORDERED
 
     def _symbolic_factorization(self):
@@ -203,13 +237,68 @@ class DirectSolver:
             raise FactorizationError(0, 'phase "refactorization": needs a previous "factorization"')
         p, keep = self._factor_params()
         info = ctypes.c_int64(0)
-        self.info = 0
+        vals = self.nzval
+        members = self._members()
+        if self.nbatch > 1 and len(members) < self.nbatch:
+            # the inactive members keep their factors: they are factored again from the values of
+            # their last factorization, which gives the same factor bit for bit
+            if getattr(self, "_fact_vals", None) is None:
+                raise InvalidValueError("ubatch_index / ubatch_mask need a factorization of every member first")
+            nnz = self.nzval.numel() // self.nbatch
+            vals = self._fact_vals.clone()
+            for k in members:
+                vals[k * nnz:(k + 1) * nnz] = self.nzval[k * nnz:(k + 1) * nnz]
         self.stage = STAGE_ANALYZED
-        _check(self._lib, self._h, self._lib.sds_factorize(self._h, ctypes.c_void_p(self.nzval.data_ptr()),
+        _check(self._lib, self._h, self._lib.sds_factorize(self._h, ctypes.c_void_p(vals.data_ptr()),
                                                             ctypes.byref(p), ctypes.byref(info)))
         del keep
         self.info = int(info.value)
+        if self.nbatch > 1:
+            self._fact_vals = vals.clone()
         self.stage = STAGE_FACTORIZED
+        if self.nbatch > 1:
+            self.info = self._member_info()
+
+    def _member_stats(self):
+        """Per-member sums of the fronts' (npos, nneg, nzero, nperturbed, n2x2) and each member's
+        info: 0 or its original column (1-based) of its first failed pivot."""
+        S = self.symbolic
+        st = np.zeros(6 * S.ns, dtype=np.int64)
+        _check(self._lib, self._h, self._lib.sds_get_stats(self._h, st.ctypes.data, None))
+        st = st.reshape(S.ns, 6)
+        ns1, out = self._ns_member, []
+        for k in range(self.nbatch):
+            part = st[k * ns1:(k + 1) * ns1]
+            fi = part[:, 5]
+            bad = np.nonzero(fi > 0)[0]
+            info = 0
+            if bad.size:
+                col = int(np.min(S.super_ptr[k * ns1 + bad] - 1 + fi[bad]))  # 1-based factor column
+                info = int(S.perm[col - 1]) - k * self.n_member
+            out.append((part[:, :5].sum(axis=0), info))
+        return out
+
+    def _member_info(self):
+        return [info for _, info in self._member_stats()]
+
+    def _members(self):
+        """The batch members the next numeric or solve phase processes (ubatch_index, ubatch_mask;
+        src/solver.jl:694-707), 0-based."""
+        o = self.options
+        if self.nbatch == 1:
+            return [0]
+        idx, mask = int(o["ubatch_index"]), o["ubatch_mask"]
+        if idx < -1 or idx >= self.nbatch:
+            raise InvalidValueError(f"ubatch_index must be -1 or a member index below {self.nbatch}")
+        members = list(range(self.nbatch)) if idx == -1 else [idx]
+        if mask is not None:
+            mask = np.asarray(mask)
+            if mask.shape != (self.nbatch,) or np.any((mask != 0) & (mask != 1)):
+                raise InvalidValueError(f"ubatch_mask must hold {self.nbatch} flags in {{0, 1}}")
+            members = [k for k in members if mask[k]]
+        if not members:
+            raise InvalidValueError("ubatch_index and ubatch_mask select no batch member")
+        return members
 
     def _factor_params(self):
         """(sds_factor_params, objects to keep alive during the call)"""
@@ -224,15 +313,53 @@ class DirectSolver:
         sign = o["pivot_sign"]
         if sign is not None:
             sign = np.ascontiguousarray(sign, dtype=np.int8)
-            if sign.shape != (self.n,) or np.any(np.abs(sign) > 1):
+            if sign.shape != (self.n_member,) or np.any(np.abs(sign) > 1):
                 raise InvalidValueError("pivot_sign must hold n entries in {-1, 0, 1}")
+            sign = np.ascontiguousarray(np.tile(sign, self.nbatch))  # the same signs for every member
             p.pivot_sign = sign.ctypes.data
         return p, (sign, flag)
 
     def _solve(self, phase, X, B):
-        torch = _torch()
         if self.stage < STAGE_FACTORIZED:
             raise FactorizationError(0, f'phase "{phase}": needs a "factorization"')
+        if self.nbatch > 1:
+            return self._solve_batch(phase, X, B)
+        return self._solve_one(phase, X, B)
+
+    def _solve_batch(self, phase, X, B):
+        """Uniform batch: right-hand sides and solutions hold nrhs columns per member, member after
+        member (an n x (nrhs nbatch) matrix, an n nrhs nbatch vector or an n x nrhs x nbatch array,
+        src/solver.jl:142-147); the inactive members' solutions are left untouched."""
+        n, nb = self.n_member, self.nbatch
+
+        def merged(T, what):
+            if T.dim() == 1 and T.numel() % (n * nb) == 0:
+                r = T.numel() // (n * nb)
+                return T.reshape(nb, r, n).permute(0, 2, 1).reshape(nb * n, r), r
+            if T.dim() == 2 and T.shape[0] == n and T.shape[1] % nb == 0:
+                r = T.shape[1] // nb
+                return T.reshape(n, nb, r).permute(1, 0, 2).reshape(nb * n, r), r
+            if T.dim() == 3 and T.shape[0] == n and T.shape[2] == nb:
+                return T.permute(2, 0, 1).reshape(nb * n, T.shape[1]), T.shape[1]
+            raise InvalidValueError(f"{what} is {tuple(T.shape)}: a batch of {nb} members of order {n} needs an "
+                                    "n x (nrhs nbatch) matrix, an n nrhs nbatch vector or an n x nrhs x nbatch array")
+        Bm, r = merged(B, "B")
+        Xm, rx = merged(X, "X")
+        if rx != r:
+            raise InvalidValueError(f"X holds {rx} columns per member, B {r}")
+        Xm = Xm.contiguous().clone()
+        self._solve_one(phase, Xm, Bm.contiguous())
+        for k in self._members():
+            xk = Xm[k * n:(k + 1) * n]
+            if X.dim() == 1:
+                X.view(nb, r, n)[k].copy_(xk.t())
+            elif X.dim() == 2:
+                X.view(n, nb, r)[:, k].copy_(xk)
+            else:
+                X[:, :, k].copy_(xk)
+
+    def _solve_one(self, phase, X, B):
+        torch = _torch()
         bits = PHASES[phase]
         vec = B.dim() == 1
         nrhs = 1 if vec else B.shape[1]
@@ -363,7 +490,7 @@ def setparam(solver, name, value):
         raise InvalidValueError("solve_mode must be 0, 1 or 2")
     elif name == "user_perm" and value is not None:
         v = np.asarray(value, dtype=np.int64)
-        if v.shape != (solver.n,) or not np.array_equal(np.sort(v), np.arange(1, solver.n + 1)):
+        if v.shape != (solver.n_member,) or not np.array_equal(np.sort(v), np.arange(1, solver.n_member + 1)):
             raise InvalidValueError("user_perm must be a permutation of 1:n")
         value = v
     elif name == "amalgamation":
@@ -406,6 +533,13 @@ def getparam(solver, name):
         return est
     if solver.stage < STAGE_FACTORIZED:
         raise FactorizationError(0, f'"{name}" needs a "factorization"')
+    if name in ("inertia", "npivots", "pivot_stats") and solver.nbatch > 1:
+        # uniform batch: one entry per member (src/solver.jl:1070-1074)
+        out = []
+        for t, _ in solver._member_stats():
+            out.append((int(t[0]), int(t[1])) if name == "inertia" else int(t[3]) if name == "npivots" else
+                       dict(npos=int(t[0]), nneg=int(t[1]), nzero=int(t[2]), nperturbed=int(t[3]), n2x2=int(t[4])))
+        return out
     if name in ("inertia", "npivots", "pivot_stats"):
         tot = np.zeros(6, dtype=np.int64)
         _check(lib, h, lib.sds_get_stats(h, None, tot.ctypes.data))
