@@ -185,28 +185,6 @@ __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __
     }
 }
 
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-    typedef f64x4 acc_t;
-    static __device__ inline acc_t mac(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int l4, int q) { return 4 * q + l4; }
-};
-template <>
-struct Mfma<float> {
-    typedef f32x4 acc_t;
-    static __device__ inline acc_t mac(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int l4, int q) { return 4 * l4 + q; }
-};
-
 // One block row p of X = L^-1, blocked by 16, for the 64 x 64 factor in D (D[j][i] = L(i, j);
 // rd the pivots' reciprocals), into X (X[j][i] = X(i, j)), by one wave: the diagonal block
 // X_pp by forward substitution (lane c: column c, sixteen steps), written when this wave also

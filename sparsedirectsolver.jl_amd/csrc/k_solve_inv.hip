@@ -393,15 +393,35 @@ __global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __r
 
 // X = L(step, step)^-1 for a wide step, one workgroup per (step, column block kk): X_kk = Linv_kk
 // (from DB), then for the blocks below X_ik = -Linv_ii sum_{j = kk}^{i - 1} L_ij X_jk, 64 x 64 x 64
-// products through LDS (thread: a 4 x 4 block). X column-major, ld nw, at xoff. The operand tiles
+// products through LDS on the matrix cores (wave w: the 32 x 32 quadrant (w / 2, w % 2) of the
+// 64 x 64 result, four 16 x 16 accumulators). X column-major, ld nw, at xoff. The operand tiles
 // are loaded into registers in one unrolled batch (16 loads in flight per thread), and the next
 // product's tiles are loaded while the current one is computed: the chain of kk = 0 in a 512-wide
 // step is 35 products long, so each product's load latency is on the critical path.
 // task: (step, kk)
 template <typename T>
+__device__ __attribute__((always_inline)) inline void tile_product_mfma(const T (*A)[NB + 1], const T (*B)[NB + 1],
+                                                                        typename Mfma<T>::acc_t (&acc)[2][2],
+                                                                        int row0, int col0, int l16, int l4) {
+#pragma unroll 4
+    for (int kk = 0; kk < NB / 4; ++kk) {
+        T av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = A[row0 + 16 * a + l16][4 * kk + l4];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = B[4 * kk + l4][col0 + 16 * b + l16];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::mac(av[a], bv[b], acc[a][b]);
+    }
+}
+
+template <typename T>
 __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep* __restrict__ steps,
                                                      const Task* __restrict__ tasks, const T* __restrict__ factor,
                                                      const T* __restrict__ DB, T* __restrict__ XB) {
+    typedef typename Mfma<T>::acc_t acc_t;
     constexpr int PER = NB * NB / 256;  // tile entries per thread: row tid % NB, columns tid / NB + 4 i
     __shared__ T A[NB][NB + 1];
     __shared__ T B[NB][NB + 1];
@@ -412,7 +432,8 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
     const T* P = factor + S.panel_off[s];
     T* X = XB + st.xoff;
     const int kc = tk.a * NB, nbk = min(NB, nw - kc);
-    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const int row0 = 32 * (wv >> 1), col0 = 32 * (wv & 1);
     const int lr = tid % NB, lc = tid / NB;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -434,11 +455,11 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
     if (tk.a + 1 < nblk) load(tk.a + 1, tk.a);
     for (int ib = tk.a + 1; ib < nblk; ++ib) {
         const int ic = ib * NB, nbi = min(NB, nw - ic);
-        T acc[4][4];
+        acc_t acc[2][2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+            for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{};
         for (int jb = tk.a; jb < ib; ++jb) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
@@ -448,21 +469,10 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
             __syncthreads();
             // the next product of this block row; X(jb + 1, kk) was written before this row
             if (jb + 1 < ib) load(ib, jb + 1);
-#pragma unroll 8
-            for (int q = 0; q < NB; ++q) {
-                T a4[4], b4[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) a4[a] = A[4 * tr + a][q];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) b4[b] = B[q][4 * tc + b];
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] += a4[a] * b4[b];
-            }
+            tile_product_mfma<T>(A, B, acc, row0, col0, l16, l4);
             __syncthreads();
         }
-        // X_ik = -Linv_ii S
+        // X_ik = -Linv_ii S: Linv_ii into A, the sum S into B
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int c = lc + 4 * i;
@@ -471,34 +481,28 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
 #pragma unroll
         for (int i = 0; i < PER; ++i) A[lr][lc + 4 * i] = ra[i];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) B[4 * tr + a][4 * tc + b] = acc[a][b];
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    B[row0 + 16 * a + Mfma<T>::row(l4, q)][col0 + 16 * b + l16] = acc[a][b][q];
         __syncthreads();
-        T out[4][4];
+        acc_t out[2][2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) out[a][b] = T(0);
-#pragma unroll 8
-        for (int q = 0; q < NB; ++q) {
-            T a4[4], b4[4];
+            for (int b = 0; b < 2; ++b) out[a][b] = acc_t{};
+        tile_product_mfma<T>(A, B, out, row0, col0, l16, l4);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) a4[a] = A[4 * tr + a][q];
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) b4[b] = B[q][4 * tc + b];
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) out[a][b] += a4[a] * b4[b];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = 4 * tr + a, c = 4 * tc + b;
-                if (r < nbi && c < nbk) X[(int64_t)(kc + c) * nw + ic + r] = -out[a][b];
-            }
+                for (int q = 0; q < 4; ++q) {
+                    const int r = row0 + 16 * a + Mfma<T>::row(l4, q), c = col0 + 16 * b + l16;
+                    if (r < nbi && c < nbk) X[(int64_t)(kc + c) * nw + ic + r] = -out[a][b][q];
+                }
         __syncthreads();  // X_ik is visible to the next block row's loads; the LDS tiles are free
         if (ib + 1 < nblk) load(ib + 1, tk.a);
     }

@@ -81,4 +81,29 @@ __device__ inline float lane_bcast(float v, int lane) {
 
 int hip_fail(hipError_t e, const char* what);
 
+// the 16 x 16 x 4 matrix-core products (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32): lane l
+// gives A(l % 16, l / 16) and B(l / 16, l % 16); element q of its accumulator is
+// C(row(l / 16, q), l % 16)
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ inline acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l4, int q) { return 4 * q + l4; }
+};
+template <>
+struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ inline acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l4, int q) { return 4 * l4 + q; }
+};
+
 }  // namespace sds
