@@ -135,3 +135,30 @@ def test_fgmres_repeatable_and_guarded():
         graph_capture(s, 2, "solve")
     sds.setparam(s, "ir_mode", "ir")
     torch.cuda.synchronize()
+
+
+def test_fgmres_kkt_ldlt_fp32_factor():
+    """The KKT generator of the benchmark ([H J^T; J -1e-8 I]) with an FP32 Bunch-Kaufman
+    factor, whose tiny pivots are perturbed: FGMRES preconditioned by that factor reaches FP64
+    accuracy, at least as well as plain refinement with the same number of solves."""
+    import torch
+    nh, nj = 1500, 750
+    K = M.kkt_matrix(nh, nj)
+    rp, cv, nz = M.csr_arrays(M.lower_csr(K))
+    n = K.shape[0]
+    s = sds.DirectSolver((rp, cv, nz, n), "S", "L")
+    sds.setparam(s, "amalgamation", (512, 0.25, 8))
+    sds.setparam(s, "pivot_type", "A")
+    sds.setparam(s, "pivot_threshold", 0.01)
+    sds.setparam(s, "factor_precision", "Float32")
+    sds.execute("analysis", s)
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "inertia") == (nh, nj)
+    B = _rhs(n, 1, 5)
+    Bh = B.cpu().numpy()
+    Xg, its = _solve(s, B, "fgmres", 20, tol=1e-14)
+    Xi, _ = _solve(s, B, "ir", its)
+    bg, bi = _berr(K, Xg, Bh), _berr(K, Xi, Bh)
+    print(f"KKT n={n} FP32 factor: fgmres {its} iterations berr {bg:.2e}, plain refinement {bi:.2e}")
+    assert bg <= 1e-14
+    assert bg <= 1.05 * bi + 1e-16
