@@ -169,8 +169,8 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  * kernels (child rows per extend-add task: 512, 1024, 2048, 4096; for Cholesky also the K
  * chunk of the contribution-block update: 32 or 64, the number of trailing-update tiles of a
  * step from which they get a launch of their own: never, 256, 512, 1024, and the number of
- * workgroups from which a step launch uses the build for two workgroups per CU: never, 256,
- * 512), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
+ * workgroups from which a step launch uses the build for two workgroups per CU: never, 1, 64,
+ * 128, 256, 512), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
  * are not kept: factorize again). tiles (may be NULL, else 4 entries) receives {extend-add
  * rows, K chunk, trailing split, two-per-CU steps}. Synchronous; needs sds_alloc_numeric
  * first.

@@ -1747,7 +1747,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     if (h->method == M_CHOLESKY) {  // with the best of the above
         for (int rs : {256, 512, 1024})
             if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0});
-        for (int oc : {256, 512}) cand.push_back({-1, -1, -1, oc});
+        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc});
     }
     double best = 1e300;
     Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2};
