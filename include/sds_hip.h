@@ -105,7 +105,9 @@ int sds_upload_symbolic(sds_handle_t handle, const sds_symbolic_desc* desc);
 /*
  * Allocate the numeric storage for factors of element type `dtype`
  * (SDS_R64, or SDS_R32 = FP32 factor with FP64 refinement) and `nbatch`
- * members (1). Replaces `allocate_numeric` (src/numeric/storage.jl:205-256).
+ * members (must be 1: a uniform batch is uploaded as one block-diagonal forest of
+ * copies of the member's analysis, whose members are the blocks). Replaces
+ * `allocate_numeric` (src/numeric/storage.jl:205-256).
  */
 int sds_alloc_numeric(sds_handle_t handle, int32_t dtype, int32_t nbatch);
 

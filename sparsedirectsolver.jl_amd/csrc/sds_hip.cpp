@@ -1598,7 +1598,9 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     if (!h) return SDS_INVALID_VALUE;
     if (!h->have_sym) return fail(h, SDS_FACTORIZATION_ERROR, "no analysis uploaded");
     if (dtype != SDS_R64 && dtype != SDS_R32) return fail(h, SDS_INVALID_VALUE, "dtype must be SDS_R64 or SDS_R32");
-    if (nbatch != 1) return fail(h, SDS_NOT_SUPPORTED, "uniform batches are not available in this build");
+    if (nbatch != 1)
+        return fail(h, SDS_NOT_SUPPORTED,
+                    "nbatch must be 1: upload a uniform batch as the block-diagonal forest of its members");
     free_numeric(h);
     int rc = build_schedule(h, dtype);
     if (rc) return rc;
