@@ -169,7 +169,7 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  * the chosen form of level and batch launches whose tile shapes come from an autotuner):
  * times `reps` factorizations of the values nzval for each candidate shape of the large-front
  * kernels (child rows per extend-add task: 512, 1024, 2048, 4096; for Cholesky also the K
- * chunk of the contribution-block update: 32 or 64, the number of trailing-update tiles of a
+ * chunk of the contribution-block update: 16, 32 or 64, the number of trailing-update tiles of a
  * step from which they get a launch of their own: never, 256, 512, 1024, and the number of
  * workgroups from which a step launch uses the build for two workgroups per CU: never, 1, 64,
  * 128, 256, 512), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents

@@ -857,6 +857,8 @@ void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n
                        const int32_t* status, int mode, int kc) {
     if (kc == 64)
         hipLaunchKernelGGL((k_big_update_mfma<T, 64>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+    else if (kc == 16)
+        hipLaunchKernelGGL((k_big_update_mfma<T, 16>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
     else
         hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
 }

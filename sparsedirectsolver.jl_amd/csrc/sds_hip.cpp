@@ -1744,7 +1744,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     };
     std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
-        for (int kc : {32, 64})
+        for (int kc : {16, 32, 64})
             if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0});
     if (h->method == M_CHOLESKY) {  // with the best of the above
         for (int rs : {256, 512, 1024})
