@@ -153,8 +153,9 @@ def test_autotune_keeps_the_factor():
     sds.execute("factorization", s)
     before = numeric_factor(s)
     tiles = sds.autotune(s, reps=1)
-    assert tiles["extend_add_rows"] in (512, 1024, 2048, 4096) and tiles["cb_k_chunk"] in (32, 64)
-    assert tiles["trailing_split"] in (0, 256, 512, 1024) and tiles["step_two_per_cu"] in (0, 256, 512)
+    assert tiles["extend_add_rows"] in (512, 1024, 2048, 4096) and tiles["cb_k_chunk"] in (16, 32, 64)
+    assert tiles["trailing_split"] in (0, 256, 512, 1024)
+    assert tiles["step_two_per_cu"] in (0, 1, 64, 128, 256, 512)
     sds.execute("factorization", s)
     assert sds.getparam(s, "info") == 0
     assert np.array_equal(numeric_factor(s), before)
