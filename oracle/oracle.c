@@ -662,6 +662,49 @@ static int64_t lu_choose_row(const double *F, int64_t f, int64_t k, int64_t w, d
     return r;
 }
 
+/* dense L D U of the fully summed columns 0..w-1 of the full f x f front F with row
+ * interchanges among the rows 0..w-1: _lu_front! (lu.jl:163-214). On return the strict lower
+ * part of F[:, 0:w] is L and the strict upper part of F[0:w, :] is U (both unit), F[k,k] = D[k];
+ * lp[k] = the assembled row eliminated at step k (0-based). The contribution block F22 is left
+ * to the caller. */
+static void lu_front(double *F, int64_t f, int64_t w, int pivoting, double u, double eps, int64_t *lp, int8_t *kind,
+                     double *dd, int64_t *st) {
+    for (int64_t k = 0; k < w; ++k) {
+        lp[k] = k;
+        kind[k] = K_1X1;
+    }
+    for (int q = 0; q < NF; ++q) st[q] = 0;
+    for (int64_t k = 0; k < w; ++k) {
+        int64_t r = pivoting ? lu_choose_row(F, f, k, w, u) : k;
+        if (r != k) {
+            for (int64_t j = 0; j < f; ++j) {
+                double t = F_(k, j);
+                F_(k, j) = F_(r, j);
+                F_(r, j) = t;
+            }
+            int64_t t = lp[k];
+            lp[k] = lp[r];
+            lp[r] = t;
+        }
+        double dk = F_(k, k);
+        if (!(fabs(dk) >= eps) || dk == 0.0) {
+            if (dk == 0.0) st[ST_NZERO]++;
+            dk = (dk < 0.0 ? -1.0 : 1.0) * eps;
+            kind[k] = K_PERTURBED;
+            st[ST_NPERT]++;
+            if (dk == 0.0 && st[ST_INFO] == 0) st[ST_INFO] = k + 1;
+        }
+        F_(k, k) = dk;
+        dd[k] = dk;
+        for (int64_t j = k + 1; j < w; ++j)
+            for (int64_t i = k + 1; i < f; ++i) F_(i, j) -= F_(i, k) / dk * F_(k, j);
+        for (int64_t j = w; j < f; ++j)
+            for (int64_t i = k + 1; i < w; ++i) F_(i, j) -= F_(i, k) / dk * F_(k, j);
+        for (int64_t i = k + 1; i < f; ++i) F_(i, k) /= dk;
+        for (int64_t j = k + 1; j < f; ++j) F_(k, j) /= dk;
+    }
+}
+
 int64_t oracle_lu(const oracle_sym *S, const double *nz, int pivoting, double u, double eps, double *factor,
                   double *ufactor, double *d, int32_t *piv, int8_t *pkind, int64_t *stats) {
     int64_t ns = S->ns, n = S->n;
@@ -687,39 +730,7 @@ int64_t oracle_lu(const oracle_sym *S, const double *nz, int pivoting, double u,
         int64_t c0 = S->super_ptr[s] - 1, base = S->front_ptr[s] - 1;
         assemble(S, s, nz, F, &cbs, 1, 1);
         int64_t *st = stats + s * NF;
-        for (int64_t k = 0; k < w; ++k) {
-            lp[k] = k;
-            pkind[c0 + k] = K_1X1;
-        }
-        for (int64_t k = 0; k < w; ++k) {
-            int64_t r = pivoting ? lu_choose_row(F, f, k, w, u) : k;
-            if (r != k) {
-                for (int64_t j = 0; j < f; ++j) {
-                    double t = F_(k, j);
-                    F_(k, j) = F_(r, j);
-                    F_(r, j) = t;
-                }
-                int64_t t = lp[k];
-                lp[k] = lp[r];
-                lp[r] = t;
-            }
-            double dk = F_(k, k);
-            if (!(fabs(dk) >= eps) || dk == 0.0) {
-                if (dk == 0.0) st[ST_NZERO]++;
-                dk = (dk < 0.0 ? -1.0 : 1.0) * eps;
-                pkind[c0 + k] = K_PERTURBED;
-                st[ST_NPERT]++;
-                if (dk == 0.0 && st[ST_INFO] == 0) st[ST_INFO] = k + 1;
-            }
-            F_(k, k) = dk;
-            dd[k] = dk;
-            for (int64_t j = k + 1; j < w; ++j)
-                for (int64_t i = k + 1; i < f; ++i) F_(i, j) -= F_(i, k) / dk * F_(k, j);
-            for (int64_t j = w; j < f; ++j)
-                for (int64_t i = k + 1; i < w; ++i) F_(i, j) -= F_(i, k) / dk * F_(k, j);
-            for (int64_t i = k + 1; i < f; ++i) F_(i, k) /= dk;
-            for (int64_t j = k + 1; j < f; ++j) F_(k, j) /= dk;
-        }
+        lu_front(F, f, w, pivoting, u, eps, lp, pkind + c0, dd, st);
         for (int64_t k = 0; k < w; ++k) {
             piv[c0 + k] = (int32_t)(c0 + lp[k] + 1);
             d[c0 + k] = dd[k];
@@ -756,4 +767,21 @@ int64_t oracle_lu(const oracle_sym *S, const double *nz, int pivoting, double u,
         }
     }
     return best == 0 ? 0 : S->perm[best - 1];
+}
+
+/* ------------------------------------------------------------------------- */
+/* The dense in-front kernels on their own, for pinning the pivot rules on fronts
+ * with contribution-block rows (f > w). F is f x f column-major; for LDL^T both
+ * triangles of the leading w x w block and rows w..f-1 of columns 0..w-1 are
+ * read. The per-column outputs have w entries; st has 6. */
+
+void oracle_ldlt_front(double *F, int64_t f, int64_t w, int pivot, double u, double eps, const int8_t *sign_req,
+                       int64_t *lp, int8_t *kind, double *dd, double *ee, int64_t *st) {
+    ldlt_params prm = {pivot, u, eps};
+    ldlt_front(F, f, w, &prm, sign_req, lp, kind, dd, ee, st);
+}
+
+void oracle_lu_front(double *F, int64_t f, int64_t w, int pivoting, double u, double eps, int64_t *lp, int8_t *kind,
+                     double *dd, int64_t *st) {
+    lu_front(F, f, w, pivoting, u, eps, lp, kind, dd, st);
 }

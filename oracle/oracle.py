@@ -11,6 +11,11 @@ Each function restates one function of SparseDirectSolver.jl's CPU reference
   solve_ldlt     ref_solve_ldlt!  src/reference/ldlt.jl:467-539
   lu             ref_lu!          src/reference/lu.jl:84-155, _lu_front! :163-214
   solve_lu       ref_solve_lu!    src/reference/lu.jl:283-294
+  ldlt_front     _ldlt_front!     src/reference/ldlt.jl:384-454 (one dense front on its own)
+  lu_front       _lu_front!       src/reference/lu.jl:163-214 (one dense front on its own)
+
+SDS_ORACLE_LIB, when set, names another build of oracle.c to load (the mutation tests
+build deliberately broken copies in a temporary folder and check that the pins fail).
 """
 import ctypes
 import os
@@ -18,7 +23,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+LIB = os.environ.get("SDS_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _I64P = ctypes.POINTER(ctypes.c_int64)
 
@@ -52,6 +57,9 @@ def lib():
         L.oracle_lu.restype = ctypes.c_int64
         L.oracle_lu.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp, vp, vp]
         L.oracle_solve_lu.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
+        i64, dbl = ctypes.c_int64, ctypes.c_double
+        L.oracle_ldlt_front.argtypes = [vp, i64, i64, ctypes.c_int, dbl, dbl, vp, vp, vp, vp, vp, vp]
+        L.oracle_lu_front.argtypes = [vp, i64, i64, ctypes.c_int, dbl, dbl, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -177,3 +185,34 @@ def extract_L(S, factor):
         for j in range(w):
             L[rows[j:], c0 + j] = P[j:, j]
     return L
+
+
+def ldlt_front(F, w, pivot_type="A", pivot_threshold=0.01, eps=1e-13, sign_req=None):
+    """The dense in-front LDL^T of one assembled f x f front (both triangles of the leading w x w
+    block filled). Returns dict(F (factored in place, a copy), lp (0-based local pivot order),
+    kind, d, e, stats). The per-column outputs are allocated f long and cut to w."""
+    F = np.array(F, dtype=np.float64, order="F", copy=True)
+    f = F.shape[0]
+    lp = np.zeros(f, dtype=np.int64)
+    kind = np.zeros(f, dtype=np.int8)
+    d = np.zeros(f)
+    e = np.zeros(f)
+    st = np.zeros(6, dtype=np.int64)
+    sr = None if sign_req is None else np.ascontiguousarray(sign_req, dtype=np.int8)
+    lib().oracle_ldlt_front(_p(F), f, int(w), PIVOTS[pivot_type], float(pivot_threshold), float(eps), _p(sr),
+                            _p(lp), _p(kind), _p(d), _p(e), _p(st))
+    return dict(F=F, lp=lp[:w], kind=kind[:w], d=d[:w], e=e[:w], stats=st)
+
+
+def lu_front(F, w, pivoting=True, pivot_threshold=0.01, eps=1e-13):
+    """The dense in-front L D U of one assembled f x f front. Returns dict(F (a factored copy),
+    lp (0-based local row order), kind, d, stats)."""
+    F = np.array(F, dtype=np.float64, order="F", copy=True)
+    f = F.shape[0]
+    lp = np.zeros(f, dtype=np.int64)
+    kind = np.zeros(f, dtype=np.int8)
+    d = np.zeros(f)
+    st = np.zeros(6, dtype=np.int64)
+    lib().oracle_lu_front(_p(F), f, int(w), int(pivoting), float(pivot_threshold), float(eps), _p(lp), _p(kind),
+                          _p(d), _p(st))
+    return dict(F=F, lp=lp[:w], kind=kind[:w], d=d[:w], stats=st)
