@@ -118,7 +118,9 @@ typedef struct {
     double pivot_threshold;
     double pivot_epsilon;      /* < 0: the default for the factor precision (src/options.jl:57-61) */
     const int8_t* pivot_sign;  /* host, n entries (original rows), or NULL */
-    const volatile int32_t* interrupt; /* host flag polled between launch groups, or NULL */
+    /* host flag polled between launch groups, or NULL: one byte, nonzero = interrupt (the layout of
+     * the reference's `user_host_interrupt`, a Threads.Atomic{Bool}: src/options.jl:201) */
+    const volatile uint8_t* interrupt;
 } sds_factor_params;
 
 /*
@@ -141,8 +143,8 @@ int sds_factorize(sds_handle_t handle, const double* nzval, const sds_factor_par
  * src/solve/refinement.jl:359-404) runs `ir_n_steps` steps of
  * X += A^-1 (B - A X) with the residual in Float64 on the values `nzval`
  * (device; may be NULL without refinement), stopping early when
- * ir_tol > 0 and every ||R_k||_2 <= ir_tol ||B_k||_2; the steps done go to
- * *ir_steps (may be NULL). Replaces permute_rhs!, forward_sweep!,
+ * ir_tol > 0 and max_k ||R_k||_2 / ||B_k||_2 <= ir_tol (||R_k||_2 for a zero B_k,
+ * src/solve/refinement.jl:284-291); the steps done go to *ir_steps (may be NULL). Replaces permute_rhs!, forward_sweep!,
  * diagonal_sweep!, backward_sweep!, unpermute_solution! and refine!
  * (src/solve/permute.jl:117-150, src/solve/sweeps.jl:908-1039).
  */
@@ -153,10 +155,11 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
  * Refinement mode of sds_solve: SDS_IR_PLAIN (0, the steps above) or SDS_IR_FGMRES (1):
  * `ir_mode = "ir" | "fgmres"` (src/types.jl:185, :306; src/solver.jl:385-392, :907).
  * With FGMRES the refinement phase replaces `fgmres_refine!` (src/solve/refinement.jl:554-620):
- * R0 = B - op(A) X; when ir_tol > 0 and every ||R0_k||_2 <= ir_tol ||B_k||_2 nothing more is
+ * R0 = B - op(A) X; when ir_tol > 0 and max_k ||R0_k||_2 / ||B_k||_2 <= ir_tol nothing more is
  * done; else FGMRES on op(A) D = R0 over the n nrhs stacked columns, right-preconditioned by the
  * factorization, for at most ir_n_steps iterations (one SpMV and one solve each), stopping once
- * ||R0 - op(A) D||_2 <= ir_tol min_k ||B_k||_2 (ir_tol = 0: all iterations), then X += D;
+ * ||R0 - op(A) D||_2 <= ir_tol min_k ||B_k||_2 (1 for a zero B_k; ir_tol = 0: all iterations),
+ * then X += D;
  * *ir_steps gets the iterations. It synchronizes with the host once per iteration, so
  * sds_graph_capture refuses it (SDS_NOT_SUPPORTED) when refinement is captured.
  */
@@ -179,6 +182,15 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  */
 int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
                  int32_t* tiles);
+
+/*
+ * Set the tile shapes of the large-front kernels directly, as sds_autotune reports them: tiles =
+ * {extend-add rows (512, 1024, 2048 or 4096), K chunk (16, 32 or 64), trailing split (0 = never,
+ * else >= 1), two-per-CU steps (0 = never, else >= 1)}. No counterpart in the reference (whose
+ * kernels have fixed shapes); it restores a saved tuning without timing again. Takes effect at
+ * the next sds_alloc_numeric (the launch plan depends on it).
+ */
+int sds_set_tiles(sds_handle_t handle, const int32_t* tiles);
 
 /*
  * HIP graph of one refactorization followed by one solve (no counterpart in the

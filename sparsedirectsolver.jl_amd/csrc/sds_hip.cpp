@@ -123,6 +123,16 @@ struct ProfRec {
     double flops, bytes;
 };
 
+// The solve workspace of one call: Y and Z (n x nrhs of the factor type, row-interleaved), the
+// update vectors U, the backward sweep's partial sums, the residual R and the copy Bc of B
+// (Float64), 2 MAX_NRHS slots for the column norms followed by DOT_BLOCKS partial sums of the
+// dot products, and the host copy of the norms: the handle's buffers (sds_solve).
+struct SolveWork {
+    void *Y = nullptr, *Z = nullptr, *U = nullptr, *part = nullptr;
+    double *R = nullptr, *Bc = nullptr, *norms = nullptr;
+    double* hnorms = nullptr;
+};
+
 }  // namespace
 }  // namespace sds
 
@@ -899,7 +909,7 @@ int build_schedule(sds_handle* h, int dtype) {
 }
 
 template <typename T>
-int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volatile int32_t* interrupt) {
+int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volatile uint8_t* interrupt) {
     hipStream_t st = h->stream;
     T* factor = static_cast<T*>(h->factor);
     T* cb = static_cast<T*>(h->cbpool);
@@ -959,7 +969,7 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
 // 2 Bunch-Kaufman; `scaled`: eps times max |a_ij| (pivot_epsilon_alg = "algo1")
 template <typename T>
 int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double eps, int scaled,
-                   const int8_t* psign, const volatile int32_t* interrupt) {
+                   const int8_t* psign, const volatile uint8_t* interrupt) {
     hipStream_t st = h->stream;
     T* factor = static_cast<T*>(h->factor);
     T* cb = static_cast<T*>(h->cbpool);
@@ -1001,7 +1011,7 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
 // LU of "G" (src/numeric/lu.jl:543-555 drives the same steps)
 template <typename T>
 int factorize_lu(sds_handle* h, const double* nz, int pivoting, double u, double eps, int scaled,
-                 const volatile int32_t* interrupt) {
+                 const volatile uint8_t* interrupt) {
     hipStream_t st = h->stream;
     T* factor = static_cast<T*>(h->factor);
     T* uf = static_cast<T*>(h->ufactor);
@@ -1062,23 +1072,36 @@ int ensure_solve_workspace(sds_handle* h, int nrhs) {
     if ((rc = get(h->U, es * std::max<int64_t>(h->uv_len, 1) * nrhs))) return rc;
     if ((rc = get(h->R, 8 * h->n * nrhs))) return rc;
     if ((rc = get(h->Bc, 8 * h->n * nrhs))) return rc;
-    if ((rc = get(h->norms, 16 * nrhs))) return rc;
+    if ((rc = get(h->norms, 8 * (2 * MAX_NRHS + DOT_BLOCKS)))) return rc;
     if ((rc = get(h->part, es * std::max<int64_t>(h->part_units, 1) * nrhs))) return rc;
     h->cap_nrhs = nrhs;
     return SDS_SUCCESS;
+}
+
+SolveWork handle_work(sds_handle* h) {
+    SolveWork W;
+    W.Y = h->Y.p;
+    W.Z = h->Z.p;
+    W.U = h->U.p;
+    W.part = h->part.p;
+    W.R = static_cast<double*>(h->R.p);
+    W.Bc = static_cast<double*>(h->Bc.p);
+    W.norms = static_cast<double*>(h->norms.p);
+    W.hnorms = h->h_norms;
+    return W;
 }
 
 // The forward sweep runs with the lower panels `ffac` (the local row order `fpiv` applied
 // first), the backward sweep with the transposes of `bfac` (`bpiv` undone last):
 // Cholesky L / L; LDL^T L + piv / L + piv; LU L + piv / U^T; LU transposed U^T / L + piv.
 template <typename T>
-void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed) {
+void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int bwd, int transposed) {
     hipStream_t st = h->stream;
     const T* L = static_cast<const T*>(h->factor);
     const T* Ut = h->ufactor ? static_cast<const T*>(h->ufactor) : L;
-    T* Y = static_cast<T*>(h->Y.p);
-    T* Z = static_cast<T*>(h->Z.p);
-    T* U = static_cast<T*>(h->U.p);
+    T* Y = static_cast<T*>(W.Y);
+    T* Z = static_cast<T*>(W.Z);
+    T* U = static_cast<T*>(W.U);
     const int unit = h->method == M_CHOLESKY ? 0 : 1;
     const int32_t* pv = h->method == M_CHOLESKY ? nullptr : h->piv;
     const bool lu = h->method == M_LU;
@@ -1090,7 +1113,7 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
     if (h->method == M_CHOLESKY) {
         // inverted diagonal blocks (DB) and level launches over all fronts
         const T* Li = static_cast<const T*>(h->dbuf);
-        T* part = static_cast<T*>(h->part.p);
+        T* part = static_cast<T*>(W.part);
         const SolveStep* stp = h->d_steps;
         const T* Xb = static_cast<const T*>(h->xbuf);
         const Task* tk = h->d_tasks;
@@ -1187,8 +1210,8 @@ void sweeps(sds_handle* h, int nrhs, int fwd, int diag, int bwd, int transposed)
 // the Hessenberg least-squares problem by Givens rotations on the host, one synchronization per
 // iteration (the reference's dot products synchronize too). Returns the iterations performed.
 template <typename T>
-int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, double* X, int64_t ldx, int nrhs,
-             const double* nz, int m, double tol, int* iters) {
+int fgmres_t(sds_handle* h, const SolveWork& W, int transposed, const double* Bsrc, int64_t ldbs, double* X,
+             int64_t ldx, int nrhs, const double* nz, int m, double tol, int* iters) {
     hipStream_t st = h->stream;
     const int n = (int)h->n;
     const int64_t N = (int64_t)n * nrhs;
@@ -1218,7 +1241,7 @@ int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, do
     double* part = H + (int64_t)hc * (m + 1);
     double* cn = part + DOT_BLOCKS;   // ns scalars
     const int64_t* rp = h->full_rowptr + (transposed && h->method == M_LU ? h->n : 0);
-    T* Y = static_cast<T*>(h->Y.p);
+    T* Y = static_cast<T*>(W.Y);
     const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
     auto spmv = [&](const double* x, int64_t ldxx, const double* b, int64_t ldb, double* out) {
         timed(h, SDS_K_RESIDUAL, 2.0 * h->nnz_full * nrhs, (double)h->nnz_full * (8 + 4 + 8) + 3.0 * 8 * n * nrhs,
@@ -1240,13 +1263,15 @@ int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, do
         }
         std::vector<double> nr(2 * nrhs);
         if ((rc = read(cn, 2 * nrhs, nr.data()))) return rc;
-        bool ok = true;
-        double bmin = INFINITY;
+        // the largest relative residual (||R_k|| for B_k = 0) and the smallest ||B_k|| (1 for
+        // B_k = 0): _max_relative_residual and _min_rhs_norm, src/solve/refinement.jl:284, :544
+        double rel = 0.0, bmin = INFINITY;
         for (int r = 0; r < nrhs; ++r) {
-            if (std::sqrt(nr[r]) > tol * std::sqrt(nr[nrhs + r])) ok = false;
-            bmin = std::min(bmin, std::sqrt(nr[nrhs + r]));
+            const double rk = std::sqrt(nr[r]), bk = std::sqrt(nr[nrhs + r]);
+            rel = std::max(rel, bk > 0 ? rk / bk : rk);
+            bmin = std::min(bmin, bk > 0 ? bk : 1.0);
         }
-        if (ok) return SDS_SUCCESS;
+        if (rel <= tol) return SDS_SUCCESS;
         atol = tol * bmin;
     }
     launch_dot(st, N, V, V, part, cn);
@@ -1265,7 +1290,7 @@ int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, do
         double* Zj = Zk + (int64_t)j * N;
         // Z_j = op(A)^-1 V_j by the factorization, then V_{j+1} = op(A) Z_j, orthogonalized
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, Vj, n, Y); });
-        sweeps<T>(h, nrhs, 1, 1, 1, transposed);
+        sweeps<T>(h, W, nrhs, 1, 1, 1, transposed);
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, Zj, n, 0); });
         spmv(Zj, n, nullptr, 0, Vn);
         for (int i = 0; i <= j; ++i) {
@@ -1308,25 +1333,35 @@ int fgmres_t(sds_handle* h, int transposed, const double* Bsrc, int64_t ldbs, do
     return SDS_SUCCESS;
 }
 
+// Largest relative residual max_k ||R_k|| / ||B_k|| (||R_k|| for B_k = 0) from the 2 nrhs squared
+// column norms (R's, then B's): _max_relative_residual, src/solve/refinement.jl:284-291.
+double max_relative_residual(const double* nr, int nrhs) {
+    double rel = 0.0;
+    for (int r = 0; r < nrhs; ++r) {
+        const double rk = std::sqrt(nr[r]), bk = std::sqrt(nr[nrhs + r]);
+        rel = std::max(rel, bk > 0 ? rk / bk : rk);
+    }
+    return rel;
+}
+
 template <typename T>
-int solve_t(sds_handle* h, int32_t phase, int transposed, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
-            const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
+int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, const double* B, int64_t ldb, double* X,
+            int64_t ldx, int nrhs, const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
     hipStream_t st = h->stream;
-    int rc = ensure_solve_workspace(h, nrhs);
-    if (rc) return rc;
-    T* Y = static_cast<T*>(h->Y.p);
+    T* Y = static_cast<T*>(W.Y);
     const int n = (int)h->n;
     const bool refine = (phase & SDS_PHASE_SOLVE_REFINEMENT) && ir_n_steps > 0;
     const double* Bsrc = B;
     if (refine && X == B) {
-        HIPCHK(h, hipMemcpy2DAsync(h->Bc.p, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+        HIPCHK(h, hipMemcpy2DAsync(W.Bc, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                                    hipMemcpyDeviceToDevice, st));
-        Bsrc = static_cast<const double*>(h->Bc.p);
+        Bsrc = W.Bc;
     }
+    const int64_t ldbs = Bsrc == B ? ldb : n;
     const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
     if (phase & SDS_PHASE_SOLVE_FWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y); });
-    sweeps<T>(h, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD,
+    sweeps<T>(h, W, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD,
               transposed);
     if (phase & SDS_PHASE_SOLVE_BWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0); });
@@ -1335,42 +1370,37 @@ int solve_t(sds_handle* h, int32_t phase, int transposed, const double* B, int64
         if (!h->full_rowptr) return fail(h, SDS_INVALID_VALUE, "refinement needs the full-pattern map at upload");
         if (!nz) return fail(h, SDS_INVALID_VALUE, "refinement needs the matrix values");
         if (h->ir_mode == SDS_IR_FGMRES) {
-            if ((rc = fgmres_t<T>(h, transposed, Bsrc, Bsrc == B ? ldb : n, X, ldx, nrhs, nz, ir_n_steps, ir_tol,
-                                  &done)))
+            int rc;
+            if ((rc = fgmres_t<T>(h, W, transposed, Bsrc, ldbs, X, ldx, nrhs, nz, ir_n_steps, ir_tol, &done)))
                 return rc;
             if (ir_steps) *ir_steps = done;
             HIPCHK(h, hipGetLastError());
             return SDS_SUCCESS;
         }
-        double* R = static_cast<double*>(h->R.p);
+        double* R = W.R;
+        double* dotp = W.norms + 2 * MAX_NRHS;  // DOT_BLOCKS partial sums
+        // with ir_tol > 0: B's squared column norms once, on the device (they do not change)
+        if (ir_tol > 0)
+            for (int r = 0; r < nrhs; ++r)
+                launch_dot(st, n, Bsrc + r * ldbs, Bsrc + r * ldbs, dotp, W.norms + nrhs + r);
         for (int it = 0; it < ir_n_steps; ++it) {
             // one SpMV: values + column + source index per entry, X gathered, B read, R written
             timed(h, SDS_K_RESIDUAL, 2.0 * h->nnz_full * nrhs,
                   (double)h->nnz_full * (8 + 4 + 8) + 3.0 * 8 * n * nrhs, [&] {
                       // the rows n..2n-1 of a "G" map are those of M^T
                       launch_residual(st, n, nrhs, h->full_rowptr + (transposed && h->method == M_LU ? n : 0),
-                                      h->full_colval, h->full_src, nz, X, ldx, Bsrc, Bsrc == B ? ldb : n, R);
+                                      h->full_colval, h->full_src, nz, X, ldx, Bsrc, ldbs, R);
                   });
             if (ir_tol > 0) {
-                // host copies of the residual and rhs norms (one synchronization per step)
-                std::vector<double> hr((size_t)n * nrhs), hb((size_t)n * nrhs);
-                HIPCHK(h, hipMemcpyAsync(hr.data(), R, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost, st));
-                HIPCHK(h, hipMemcpy2DAsync(hb.data(), sizeof(double) * n, Bsrc, sizeof(double) * (Bsrc == B ? ldb : n),
-                                           sizeof(double) * n, nrhs, hipMemcpyDeviceToHost, st));
+                // the residual's squared column norms on the device; 2 nrhs scalars come back
+                // (one synchronization per step, as residual_norms!, src/solve/refinement.jl:378-383)
+                for (int r = 0; r < nrhs; ++r) launch_dot(st, n, R + (int64_t)r * n, R + (int64_t)r * n, dotp, W.norms + r);
+                HIPCHK(h, hipMemcpyAsync(W.hnorms, W.norms, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, st));
                 HIPCHK(h, hipStreamSynchronize(st));
-                bool ok = true;
-                for (int r = 0; r < nrhs; ++r) {
-                    double a = 0, b = 0;
-                    for (int i = 0; i < n; ++i) {
-                        a += hr[(size_t)r * n + i] * hr[(size_t)r * n + i];
-                        b += hb[(size_t)r * n + i] * hb[(size_t)r * n + i];
-                    }
-                    if (std::sqrt(a) > ir_tol * std::sqrt(b)) ok = false;
-                }
-                if (ok) break;
+                if (max_relative_residual(W.hnorms, nrhs) <= ir_tol) break;
             }
             timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y); });
-            sweeps<T>(h, nrhs, 1, 1, 1, transposed);
+            sweeps<T>(h, W, nrhs, 1, 1, 1, transposed);
             timed(h, SDS_K_PERMUTE, 0, pbytes + 8.0 * n * nrhs,
                   [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1); });
             ++done;
@@ -1633,6 +1663,7 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * 8, &p))) return rc;
     h->d_info = static_cast<int64_t*>(p);
     HIPCHK(h, hipHostMalloc((void**)&h->h_info, sizeof(int64_t) * 8, hipHostMallocDefault));
+    HIPCHK(h, hipHostMalloc((void**)&h->h_norms, sizeof(double) * 2 * MAX_NRHS, hipHostMallocDefault));
     h->dtype = dtype;
     return SDS_SUCCESS;
 }
@@ -1642,13 +1673,13 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
 namespace {
 
 // the launches of one factorization on the handle's stream, without the host read of info.
-// sign_sync: upload pivot_sign with a blocking copy (before a graph capture) instead of an
-// asynchronous one on the stream.
-int factorize_enqueue(sds_handle* h, const double* nzval, const sds_factor_params* prm, bool sign_sync) {
+// sign_ready: pivot_sign is already in h->psign (sds_graph_capture copies it before the capture
+// begins: a blocking copy cannot be recorded); else it is copied asynchronously on the stream.
+int factorize_enqueue(sds_handle* h, const double* nzval, const sds_factor_params* prm, bool sign_ready) {
     if (h->dtype < 0) return fail(h, SDS_FACTORIZATION_ERROR, "numeric storage not allocated");
     if (!nzval && h->nnz > 0) return fail(h, SDS_INVALID_VALUE, "nzval is NULL");
     double eps = h->dtype == SDS_R64 ? 1e-13 : 1e-5;  // default_pivot_epsilon (src/options.jl:57-61)
-    const volatile int32_t* intr = nullptr;
+    const volatile uint8_t* intr = nullptr;
     int ptype = 0, scaled = 0;
     double u = 0.01;
     const int8_t* psign = nullptr;
@@ -1662,9 +1693,7 @@ int factorize_enqueue(sds_handle* h, const double* nzval, const sds_factor_param
             return fail(h, SDS_INVALID_VALUE, "pivot_epsilon_alg must be 0, 1 or 2");
         if (!(u >= 0.0 && u <= 1.0)) return fail(h, SDS_INVALID_VALUE, "pivot_threshold must be in [0, 1]");
         if (prm->pivot_sign) {
-            if (sign_sync)
-                HIPCHK(h, hipMemcpy(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice));
-            else
+            if (!sign_ready)
                 HIPCHK(h, hipMemcpyAsync(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice, h->stream));
             psign = h->psign;
         }
@@ -1790,6 +1819,21 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     return SDS_SUCCESS;
 }
 
+int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!tiles) return fail(h, SDS_INVALID_VALUE, "tiles is NULL");
+    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3];
+    if (ea != 512 && ea != 1024 && ea != 2048 && ea != 4096)
+        return fail(h, SDS_INVALID_VALUE, "extend-add rows must be 512, 1024, 2048 or 4096");
+    if (kc != 16 && kc != 32 && kc != 64) return fail(h, SDS_INVALID_VALUE, "the K chunk must be 16, 32 or 64");
+    if (rs < 0 || oc < 0) return fail(h, SDS_INVALID_VALUE, "the trailing split and two-per-CU steps must be >= 0");
+    h->ea_rows = ea;
+    h->upd_kc = kc;
+    h->rest_split = rs;
+    h->step_occ2 = oc;
+    return SDS_SUCCESS;
+}
+
 int sds_graph_capture(sds_handle_t h, const double* nzval, const sds_factor_params* prm, int32_t phase,
                       int32_t solve_mode, const double* B, int64_t ldb, double* X, int64_t ldx, int32_t nrhs,
                       int32_t ir_n_steps) {
@@ -1805,18 +1849,21 @@ int sds_graph_capture(sds_handle_t h, const double* nzval, const sds_factor_para
     graph_free(h);
     if (!h->cstream) HIPCHK(h, hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    // pivot_sign goes to the device now: the graph reads it from there at every replay
+    if (prm && prm->pivot_sign) HIPCHK(h, hipMemcpy(h->psign, prm->pivot_sign, h->n, hipMemcpyHostToDevice));
     hipStream_t user = h->stream;
     const bool prof = h->prof;
     h->prof = false;
     h->stream = h->cstream;
+    const SolveWork W = handle_work(h);
     hipError_t e = hipStreamBeginCapture(h->cstream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         rc = factorize_enqueue(h, nzval, prm, true);
         const int tr = h->method == M_LU && solve_mode != 0;
         if (!rc)
-            rc = h->dtype == SDS_R64 ? solve_t<double>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
+            rc = h->dtype == SDS_R64 ? solve_t<double>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
                                                        nullptr)
-                                     : solve_t<float>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
+                                     : solve_t<float>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, 0.0,
                                                       nullptr);
         hipError_t e2 = hipStreamEndCapture(h->cstream, &h->graph);
         if (!rc && e2 != hipSuccess) rc = fail(h, SDS_DEVICE_ERROR, std::string("hipStreamEndCapture: ") +
@@ -1859,11 +1906,13 @@ int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B
     if (!h->factorized) return fail(h, SDS_FACTORIZATION_ERROR, "solve before a factorization");
     int rc = check_solve_args(h, phase, solve_mode, ldb, ldx, nrhs);
     if (rc) return rc;
+    if ((rc = ensure_solve_workspace(h, nrhs))) return rc;
+    const SolveWork W = handle_work(h);
     // symmetric structures: A^T = A; real values: A^H = A^T (src/solve/refinement.jl:10-15)
     const int tr = h->method == M_LU && solve_mode != 0;
     return h->dtype == SDS_R64
-               ? solve_t<double>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
-               : solve_t<float>(h, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+               ? solve_t<double>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
+               : solve_t<float>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
 }
 
 int sds_adjoint_scatter(sds_handle_t h, const double* lambda, int64_t ldl, const double* X, int64_t ldx, int32_t nrhs,

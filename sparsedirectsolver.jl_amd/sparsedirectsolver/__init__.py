@@ -5,7 +5,8 @@ update); the numeric and solve phases run in libsds_hip.so (HIP, gfx950).
 """
 from .analysis import Symbolic, grid_nd_perm, symbolic_analysis
 from .errors import FactorizationError, InterruptedError_, InvalidValueError, NotSupportedError
-from .solver import DirectSolver, autotune, execute, getparam, setparam, update
+from .solver import DirectSolver, autotune, execute, getparam, set_tiles, setparam, update
 
-__all__ = ["DirectSolver", "execute", "setparam", "getparam", "update", "autotune", "Symbolic", "symbolic_analysis",
+__all__ = ["DirectSolver", "execute", "setparam", "getparam", "update", "autotune", "set_tiles", "Symbolic",
+           "symbolic_analysis",
            "grid_nd_perm", "InvalidValueError", "NotSupportedError", "FactorizationError", "InterruptedError_"]

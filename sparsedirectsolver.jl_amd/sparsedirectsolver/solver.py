@@ -309,6 +309,8 @@ class DirectSolver:
         p.pivot_threshold = float(o["pivot_threshold"])
         p.pivot_epsilon = -1.0 if o["pivot_epsilon"] is None else float(o["pivot_epsilon"])
         flag = o["user_host_interrupt"]
+        if flag is not None and not isinstance(flag, (ctypes.c_bool, ctypes.c_uint8)):
+            raise InvalidValueError("user_host_interrupt must be a one-byte flag (ctypes.c_bool or c_uint8) or None")
         p.interrupt = ctypes.addressof(flag) if flag is not None else None
         sign = o["pivot_sign"]
         if sign is not None:
@@ -462,6 +464,22 @@ def autotune(solver, reps=2):
     solver.stage = STAGE_ANALYZED
     return {"extend_add_rows": int(tiles[0]), "cb_k_chunk": int(tiles[1]), "trailing_split": int(tiles[2]),
             "step_two_per_cu": int(tiles[3])}
+
+
+def set_tiles(solver, tiles):
+    """Set the large-front kernels' tile shapes as `autotune` returns them (a dict with the keys
+    "extend_add_rows", "cb_k_chunk", "trailing_split", "step_two_per_cu"; libsds_hip's
+    sds_set_tiles) and reallocate the numeric storage for them: the solver then needs a
+    "factorization" again."""
+    if solver.stage < STAGE_ANALYZED:
+        raise FactorizationError(0, 'set_tiles needs "analysis"')
+    t = (ctypes.c_int32 * 4)(int(tiles["extend_add_rows"]), int(tiles["cb_k_chunk"]), int(tiles["trailing_split"]),
+                             int(tiles["step_two_per_cu"]))
+    _check(solver._lib, solver._h, solver._lib.sds_set_tiles(solver._h, t))
+    fp = solver.options["factor_precision"]
+    dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64
+    _check(solver._lib, solver._h, solver._lib.sds_alloc_numeric(solver._h, dtype, 1))
+    solver.stage = STAGE_ANALYZED
 
 
 def setparam(solver, name, value):

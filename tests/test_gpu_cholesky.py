@@ -145,6 +145,31 @@ def test_laplacian3d_64_full_size_backward_error():
     assert torch.equal(X, X2)
 
 
+def test_every_tile_candidate_gives_the_same_factor():
+    """Every tile shape the autotuner can pick (sds_autotune's candidates, forced one at a time
+    through sds_set_tiles) gives bitwise the factor of the default shape. The 56 x 56 x 40
+    Laplacian in nested dissection with (512, 0.25, 8) has a step with 1,282 trailing tiles over
+    its level and step launches of 1,564 workgroups, so every trailing-split threshold (256, 512,
+    1024) and every two-per-CU threshold (1 to 512) switches some launch over."""
+    A = M.laplacian3d(56, 56, 40)
+    s, nz = _solver(A, sds.grid_nd_perm(56, 56, 40), (512, 0.25, 8))
+    base = dict(extend_add_rows=1024, cb_k_chunk=32, trailing_split=0, step_two_per_cu=0)
+    sds.set_tiles(s, base)
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 0
+    ref = numeric_factor(s)
+    cands = [dict(base, extend_add_rows=ea, cb_k_chunk=kc) for ea in (512, 1024, 2048, 4096) for kc in (16, 32, 64)]
+    cands += [dict(base, trailing_split=rs) for rs in (256, 512, 1024)]
+    cands += [dict(base, step_two_per_cu=oc) for oc in (1, 64, 128, 256, 512)]
+    cands += [dict(base, trailing_split=256, step_two_per_cu=1)]
+    for c in cands:
+        print("tiles", c, flush=True)
+        sds.set_tiles(s, c)
+        sds.execute("factorization", s)
+        assert sds.getparam(s, "info") == 0
+        assert np.array_equal(numeric_factor(s), ref), f"tile shape {c} changes the factor"
+
+
 def test_autotune_keeps_the_factor():
     """The tile-shape autotuner picks among shapes that apply the same operations in the same
     order to every entry: the factor after tuning is bitwise the one before."""
@@ -159,3 +184,22 @@ def test_autotune_keeps_the_factor():
     sds.execute("factorization", s)
     assert sds.getparam(s, "info") == 0
     assert np.array_equal(numeric_factor(s), before)
+
+
+def test_interrupt_flag_stops_the_factorization():
+    """user_host_interrupt (a one-byte flag) set before the phase: the library's level loop sees
+    it before the first launch group and the phase raises InterruptedError
+    (src/numeric/factorize.jl:365-379, src/logging.jl:59-63); cleared, the same solver factors."""
+    import ctypes
+    A = M.laplacian3d(10, 10, 10)
+    s, nz = _solver(A, sds.grid_nd_perm(10, 10, 10))
+    flag = ctypes.c_bool(True)
+    sds.setparam(s, "user_host_interrupt", flag)
+    with pytest.raises(sds.InterruptedError_):
+        sds.execute("factorization", s)
+    flag.value = False
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 0
+    with pytest.raises(sds.InvalidValueError):
+        sds.setparam(s, "user_host_interrupt", ctypes.c_int32(0))
+        sds.execute("factorization", s)

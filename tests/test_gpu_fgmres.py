@@ -162,3 +162,19 @@ def test_fgmres_kkt_ldlt_fp32_factor():
     print(f"KKT n={n} FP32 factor: fgmres {its} iterations berr {bg:.2e}, plain refinement {bi:.2e}")
     assert bg <= 1e-14
     assert bg <= 1.05 * bi + 1e-16
+
+
+def test_early_exit_with_a_zero_rhs_column():
+    """A zero right-hand-side column counts with its residual norm itself, and with 1 in the
+    smallest ||B_k|| (_max_relative_residual and _min_rhs_norm, src/solve/refinement.jl:284,
+    :544): both refinement modes stop early, and the zero column's solution stays zero."""
+    import torch
+    A = M.laplacian3d(14, 13, 12)
+    s = _spd_solver(A, sds.grid_nd_perm(14, 13, 12))
+    B = _rhs(A.shape[0], 3, 21)
+    B[:, 1] = 0.0
+    for mode in ("ir", "fgmres"):
+        X, it = _solve(s, B, mode, 20, tol=1e-13)
+        assert 0 < it < 20, f"{mode}: {it} iterations"
+        assert np.all(X[:, 1] == 0.0)
+        assert _berr(A, X, B.cpu().numpy()) <= 1e-14

@@ -64,3 +64,38 @@ def test_graph_replays_match_eager(kind):
         sds.execute("solve", s, Xe, B)
         torch.cuda.synchronize()
         assert torch.equal(Xg, Xe), f"replay {rep}: max diff {(Xg - Xe).abs().max().item():.3e}"
+
+
+def test_graph_ldlt_with_pivot_sign():
+    """LDL^T with requested perturbation signs (pivot_sign): the signs go to the device before the
+    capture begins, and every replay gives bitwise the eager refactorization + solve. A large
+    pivot_epsilon perturbs many pivots, so the requested signs decide the inertia."""
+    import torch
+    A = kkt_matrix(200, 80, 1e-8, np.random.default_rng(3))
+    L = M.lower_csr(A)
+    rp, cv, nz = M.csr_arrays(L)
+    n = A.shape[0]
+    s = sds.DirectSolver((rp, cv, nz, n), "S", "L")
+    sds.setparam(s, "amalgamation", (64, 0.4, 8))
+    sds.setparam(s, "pivot_epsilon", 50.0)
+    sds.execute("analysis", s)
+    inertia = {}
+    for sign in (1, -1):
+        sds.setparam(s, "pivot_sign", np.full(n, sign, dtype=np.int8))
+        sds.execute("factorization", s)
+        inertia[sign] = sds.getparam(s, "inertia")
+    assert sds.getparam(s, "npivots") > 0 and inertia[1] != inertia[-1]
+    sds.setparam(s, "pivot_sign", np.where(np.arange(n) % 3 == 0, -1, 1).astype(np.int8))
+    rng = np.random.default_rng(9)
+    sds.execute("factorization", s)
+    graph_capture(s, nrhs=2)
+    for rep in range(3):
+        B = torch.tensor(rng.standard_normal((n, 2)), dtype=torch.float64, device="cuda")
+        Xg = torch.zeros_like(B)
+        assert graph_replay(s, Xg, B, check_info=True) == 0
+        torch.cuda.synchronize()
+        Xe = torch.zeros_like(B)
+        sds.execute("refactorization", s)
+        sds.execute("solve", s, Xe, B)
+        torch.cuda.synchronize()
+        assert torch.equal(Xg, Xe), f"replay {rep}: max diff {(Xg - Xe).abs().max().item():.3e}"
