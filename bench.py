@@ -10,8 +10,9 @@ the number of steps that reaches a normwise backward error <= 1e-10.
 `value` is the FP64 form's refactorize+solve milliseconds per step (lower is
 better); the FP32+IR form, factor TFLOP/s, backward errors, the dominant
 kernel's roofline (HIP events inside the library, on the solver's stream) and
-a CPU baseline (the C oracle on a bounded sample, see `cpu_baseline.sample`)
-are on the same JSON line.
+a CPU baseline (the reference's CPU path with LAPACK/BLAS per front on the whole
+workload, see `cpu_baseline.sample`) are on the same JSON line, with the GPU
+factors' largest element-wise errors against that CPU path's FP64 factor.
 """
 import argparse
 import json
@@ -127,36 +128,54 @@ def time_phase(s, steps, fn):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def cpu_baseline(S, nz, factor_gpu, B, budget_flops):
-    """The C oracle (a single-threaded restatement of src/reference/cholesky.jl) on the host:
-    the factorization of the postorder prefix whose flops stay under `budget_flops` (a union
-    of whole subtrees), scaled to the whole tree by flops; plus the full oracle solve with all
-    right-hand sides on the same factor."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(S, nz, A, B):
+    """The reference's CPU path (oracle/reference_path.py: ref_factorize! and ref_solve! of
+    src/reference/cholesky.jl with LAPACK potrf and BLAS trsm / syrk / gemm per front) on the whole
+    workload, timed on the host with one BLAS thread and with all the threads this process may
+    use. `value` is the faster of the two (factorization + solve). Returns the result and the
+    CPU factor (FP64), against which the GPU factors' errors are stated."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    f = S.front_nrows.astype(np.float64)
-    w = S.front_ncols.astype(np.float64)
-    fl = np.array([np.sum((fi - np.arange(wi)) ** 2) for fi, wi in zip(f, w)])
-    post = S.snpost - 1
-    cum = np.cumsum(fl[post])
-    q1 = int(np.searchsorted(cum, budget_flops, side="right"))
-    q1 = max(q1, 1)
-    t0 = time.perf_counter()
-    oracle.cholesky_prefix(S, nz, q1)
-    t_sample = time.perf_counter() - t0
-    sample_flops = float(cum[q1 - 1])
-    t_factor = t_sample * fl.sum() / sample_flops
-    t0 = time.perf_counter()
-    oracle.solve_cholesky(S, factor_gpu, B)
-    t_solve = time.perf_counter() - t0
+    import reference_path
+    from threadpoolctl import threadpool_limits
+    cores = len(os.sched_getaffinity(0))
+    runs = {}
+    fac = None
+    for th in (1, cores):
+        with threadpool_limits(limits=th):
+            t0 = time.perf_counter()
+            info, f = reference_path.cholesky(S, nz)
+            tf = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            X = reference_path.solve(S, f, B)
+            ts = time.perf_counter() - t0
+        assert info == 0
+        runs[th] = (tf, ts, backward_error(A, X, B))
+        fac = f
+        print(f"cpu baseline, {th} BLAS thread(s): factorization {tf:.2f} s, solve {ts:.2f} s", file=sys.stderr,
+              flush=True)
+    best = min(runs, key=lambda t: sum(runs[t][:2]))
+    tf, ts, berr = runs[best]
     return {
-        "value": round((t_factor + t_solve) * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
-        "sample": (f"oracle factorization of the first {q1} of {S.ns} supernodes in postorder "
-                   f"({sample_flops:.3e} of {fl.sum():.3e} flops, {t_sample:.2f} s, "
-                   f"{sample_flops / t_sample / 1e9:.2f} GFLOP/s) scaled by flops to the whole tree, "
-                   f"plus the full oracle solve with {B.shape[1]} right-hand sides ({t_solve:.2f} s)"),
-        "factor_ms_est": round(t_factor * 1e3, 1), "solve_ms": round(t_solve * 1e3, 1),
-    }
+        "value": round((tf + ts) * 1e3, 1), "unit": "ms", "cores": best, "kind": "port",
+        "sample": (f"the whole workload: ref_factorize! + ref_solve! restated with LAPACK potrf / BLAS trsm, syrk "
+                   f"per front (scipy's OpenBLAS) over the same analysis, {B.shape[1]} right-hand sides; the faster "
+                   f"of 1 and {cores} BLAS threads"),
+        "factor_ms": round(tf * 1e3, 1), "solve_ms": round(ts * 1e3, 1), "backward_error": berr,
+        "factor_gflops": round(S.flops_true / tf / 1e9, 2),
+        "by_threads": {str(t): {"factor_ms": round(v[0] * 1e3, 1), "solve_ms": round(v[1] * 1e3, 1)}
+                       for t, v in runs.items()},
+        "host": {"cpu": cpu_model(), "threads_available": cores},
+    }, fac
 
 
 def bench_batch(args):
@@ -460,7 +479,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--nrhs", type=int, default=8)
-    ap.add_argument("--cpu-budget", type=float, default=5e10, help="flops of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint", "kkt"), default="laplacian3d")
@@ -543,6 +561,7 @@ def main():
     res["profile_ms"] = {k: round(v[0] / args.steps, 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
 
     # -- FP32 factor + FP64 refinement
+    f32_factor = None
     if not args.no_fp32:
         s32 = make_solver(csr, n, perm, amal, "Float32")
         steps_needed = None
@@ -566,11 +585,17 @@ def main():
                               "backward_error": berr32, "speedup_vs_fp64": round(t_step64 / t_step32, 3)}
         else:
             res["fp32_ir"] = {"ms_per_step": None, "note": "backward error 1e-10 not reached in 10 refinement steps"}
+        f32_factor = numeric_factor(s32).astype(np.float64)
         del s32
 
-    # -- CPU baseline (rank 0, N = 1)
+    # -- CPU baseline (rank 0, N = 1) and the GPU factors' errors against its FP64 factor
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(S, csr[2], numeric_factor(s64), Bh, args.cpu_budget)
+        res["cpu_baseline"], fac_cpu = cpu_baseline(S, csr[2], A, Bh)
+        scale = np.abs(fac_cpu).max()
+        res["fp64"]["factor_rel_err_vs_cpu"] = float(np.abs(numeric_factor(s64) - fac_cpu).max() / scale)
+        if f32_factor is not None:
+            res["fp32_ir"]["factor_rel_err_vs_cpu_fp64"] = float(np.abs(f32_factor - fac_cpu).max() / scale)
+        res["cpu_baseline"]["gpu_speedup"] = round(res["cpu_baseline"]["value"] / t_step64, 1)
     print(json.dumps(res), flush=True)
 
 

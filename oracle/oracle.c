@@ -785,3 +785,47 @@ void oracle_lu_front(double *F, int64_t f, int64_t w, int pivoting, double u, do
                      double *dd, int64_t *st) {
     lu_front(F, f, w, pivoting, u, eps, lp, kind, dd, st);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Front-by-front pieces for a driver that does the dense work of each front with
+ * LAPACK/BLAS, as ref_factorize! does (src/reference/cholesky.jl:44-111: potrf, trsm,
+ * syrk per front): the context keeps the children's contribution blocks between the
+ * calls. assemble: zero the f x f front F (column-major) and add A and the children's
+ * blocks (cholesky.jl:73-92); keep: store the contribution block F22 + lower(C) of
+ * front s (C = -F21 F21^T, m x m column-major) for its parent. */
+
+typedef struct {
+    oracle_sym S;
+    cbstore cbs;
+} oracle_front_ctx;
+
+void *oracle_front_begin(const oracle_sym *S) {
+    oracle_front_ctx *c = (oracle_front_ctx *)malloc(sizeof(oracle_front_ctx));
+    c->S = *S;
+    cb_init(&c->cbs, S->ns);
+    return c;
+}
+
+void oracle_front_assemble(void *ctx, int64_t s, const double *nz, double *F) {
+    oracle_front_ctx *c = (oracle_front_ctx *)ctx;
+    assemble(&c->S, s, nz, F, &c->cbs, 0, 0);
+}
+
+void oracle_front_keep(void *ctx, int64_t s, const double *F, const double *C) {
+    oracle_front_ctx *c = (oracle_front_ctx *)ctx;
+    const oracle_sym *S = &c->S;
+    int64_t f = S->front_nrows[s], w = S->front_ncols[s], m = f - w;
+    if (m <= 0 || S->snparent[s] == 0) return;
+    double *B = (double *)malloc(sizeof(double) * (size_t)(m * m));
+    for (int64_t jj = 0; jj < m; ++jj)
+        for (int64_t ii = 0; ii < m; ++ii)
+            B[jj * m + ii] = ii >= jj ? F_(w + ii, w + jj) + C[jj * m + ii] : 0.0;
+    c->cbs.cb[s] = B;
+    c->cbs.m[s] = m;
+}
+
+void oracle_front_end(void *ctx) {
+    oracle_front_ctx *c = (oracle_front_ctx *)ctx;
+    cb_free(&c->cbs, c->S.ns);
+    free(c);
+}

@@ -60,6 +60,11 @@ def lib():
         i64, dbl = ctypes.c_int64, ctypes.c_double
         L.oracle_ldlt_front.argtypes = [vp, i64, i64, ctypes.c_int, dbl, dbl, vp, vp, vp, vp, vp, vp]
         L.oracle_lu_front.argtypes = [vp, i64, i64, ctypes.c_int, dbl, dbl, vp, vp, vp, vp]
+        L.oracle_front_begin.restype = vp
+        L.oracle_front_begin.argtypes = [vp]
+        L.oracle_front_assemble.argtypes = [vp, i64, vp, vp]
+        L.oracle_front_keep.argtypes = [vp, i64, vp, vp]
+        L.oracle_front_end.argtypes = [vp]
         _lib = L
     return _lib
 
