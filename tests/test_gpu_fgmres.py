@@ -177,4 +177,5 @@ def test_early_exit_with_a_zero_rhs_column():
         X, it = _solve(s, B, mode, 20, tol=1e-13)
         assert 0 < it < 20, f"{mode}: {it} iterations"
         assert np.all(X[:, 1] == 0.0)
-        assert _berr(A, X, B.cpu().numpy()) <= 1e-14
+        # the zero column's backward error is 0/0; its exact zero solution is checked above
+        assert _berr(A, X[:, [0, 2]], B.cpu().numpy()[:, [0, 2]]) <= 1e-14
