@@ -152,6 +152,25 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * sds_solve without device allocation. No counterpart in the reference: it is for callers that
+ * cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). Same arguments, kernels and results as sds_solve (bitwise), but the solve
+ * workspace (permuted right-hand sides, update vectors, partial sums, residual, a copy of B,
+ * column norms) lives in a fixed array of SDS_NOALLOC_SCRATCH_MIB MiB per GPU inside the library,
+ * reserved on a device at the first sds_solve_noalloc call there and held until the process
+ * ends; the call itself allocates nothing. Needs about (2 s + 16) n nrhs bytes plus the update
+ * vectors (s = 8 for an SDS_R64 factor, 4 for SDS_R32); a call that needs more returns
+ * SDS_INVALID_VALUE, and FGMRES refinement (whose Krylov basis is allocated) returns
+ * SDS_NOT_SUPPORTED. Every call on one GPU shares that GPU's array and leaves its contents
+ * behind: calls that overlap on one GPU (several handles, several streams or threads) race and
+ * must be serialized by the caller. Not profiled by sds_profile.
+ */
+#define SDS_NOALLOC_SCRATCH_MIB 128
+int sds_solve_noalloc(sds_handle_t handle, int32_t phase, int32_t solve_mode, const double* B, int64_t ldb, double* X,
+                      int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol,
+                      int32_t* ir_steps);
+
+/*
  * Refinement mode of sds_solve: SDS_IR_PLAIN (0, the steps above) or SDS_IR_FGMRES (1):
  * `ir_mode = "ir" | "fgmres"` (src/types.jl:185, :306; src/solver.jl:385-392, :907).
  * With FGMRES the refinement phase replaces `fgmres_refine!` (src/solve/refinement.jl:554-620):

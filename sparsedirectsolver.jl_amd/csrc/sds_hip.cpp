@@ -11,11 +11,19 @@
 #include <cstring>
 #include <string>
 #include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "../../include/sds_hip.h"
 #include "sds_common.h"
 #include "sds_kernels.h"
+
+// The solve workspace of sds_solve_noalloc: one array per GPU (each device holds its own
+// copy, at its own address, until the process ends), shared by every call on that GPU.
+// Host code never uses the name `scratch` as an address: it is a host-side stand-in; the
+// device address comes from hipGetSymbolAddress, once per device (scratch_base).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(SDS_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace sds {
 
@@ -1078,6 +1086,53 @@ int ensure_solve_workspace(sds_handle* h, int nrhs) {
     return SDS_SUCCESS;
 }
 
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the handle's GPU's scratch array, looked up with that GPU current on the
+// first call for the device.
+int scratch_base(sds_handle* h, void** out) {
+    if (h->device < 0 || h->device >= MAX_DEVICES) return fail(h, SDS_NOT_SUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[h->device]) {
+        int cur = -1;
+        HIPCHK(h, hipGetDevice(&cur));
+        if (cur != h->device) HIPCHK(h, hipSetDevice(h->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != h->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(h, SDS_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[h->device] = p;
+    }
+    *out = scratch_ptr[h->device];
+    return SDS_SUCCESS;
+}
+
+// The solve workspace for nrhs right-hand sides laid out from `base` (256-byte aligned
+// pieces, the sizes of ensure_solve_workspace); returns the bytes it spans.
+size_t noalloc_layout(const sds_handle* h, int nrhs, char* base, SolveWork* W) {
+    const size_t es = h->dtype == SDS_R64 ? 8 : 4;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += (std::max<size_t>(bytes, 8) + 255) & ~size_t(255);
+        return static_cast<void*>(p);
+    };
+    SolveWork w;
+    w.Y = take(es * h->n * nrhs);
+    w.Z = take(es * h->n * nrhs);
+    w.U = take(es * std::max<int64_t>(h->uv_len, 1) * nrhs);
+    w.R = static_cast<double*>(take(8 * h->n * nrhs));
+    w.Bc = static_cast<double*>(take(8 * h->n * nrhs));
+    w.norms = static_cast<double*>(take(8 * (2 * MAX_NRHS + DOT_BLOCKS)));
+    w.part = take(es * std::max<int64_t>(h->part_units, 1) * nrhs);
+    w.hnorms = h->h_norms;
+    if (W) *W = w;
+    return off;
+}
+
 SolveWork handle_work(sds_handle* h) {
     SolveWork W;
     W.Y = h->Y.p;
@@ -1913,6 +1968,36 @@ int sds_solve(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B
     return h->dtype == SDS_R64
                ? solve_t<double>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
                : solve_t<float>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+}
+
+int sds_solve_noalloc(sds_handle_t h, int32_t phase, int32_t solve_mode, const double* B, int64_t ldb, double* X,
+                      int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol,
+                      int32_t* ir_steps) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->factorized) return fail(h, SDS_FACTORIZATION_ERROR, "solve before a factorization");
+    int rc = check_solve_args(h, phase, solve_mode, ldb, ldx, nrhs);
+    if (rc) return rc;
+    if (h->ir_mode == SDS_IR_FGMRES && (phase & SDS_PHASE_SOLVE_REFINEMENT) && ir_n_steps > 0)
+        return fail(h, SDS_NOT_SUPPORTED, "sds_solve_noalloc: FGMRES refinement needs a Krylov basis; use sds_solve");
+    const size_t need = noalloc_layout(h, nrhs, nullptr, nullptr);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(h, SDS_INVALID_VALUE,
+                    "sds_solve_noalloc: n = " + std::to_string(h->n) + " with nrhs = " + std::to_string(nrhs) +
+                        " needs " + std::to_string(need) + " bytes of solve workspace, more than its " +
+                        std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte scratch; use sds_solve or fewer right-hand sides");
+    void* base = nullptr;
+    if ((rc = scratch_base(h, &base))) return rc;
+    SolveWork W;
+    noalloc_layout(h, nrhs, static_cast<char*>(base), &W);
+    // profiling records HIP events, which it may create: off for this call
+    const bool prof = h->prof;
+    h->prof = false;
+    const int tr = h->method == M_LU && solve_mode != 0;
+    rc = h->dtype == SDS_R64
+             ? solve_t<double>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps)
+             : solve_t<float>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
+    h->prof = prof;
+    return rc;
 }
 
 int sds_adjoint_scatter(sds_handle_t h, const double* lambda, int64_t ldl, const double* X, int64_t ldx, int32_t nrhs,

@@ -5,8 +5,9 @@ c_i32, c_i64, c_dbl, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctype
 
 # the exported entry points, in the order of include/sds_hip.h
 EXPORTS = ("sds_create", "sds_destroy", "sds_last_error", "sds_set_stream", "sds_upload_symbolic",
-           "sds_alloc_numeric", "sds_factorize", "sds_solve", "sds_get_stats", "sds_get_pivots", "sds_get_diag",
-           "sds_get_factor", "sds_memory_estimates", "sds_profile", "sds_profile_read", "sds_adjoint_scatter",
+           "sds_alloc_numeric", "sds_factorize", "sds_solve", "sds_solve_noalloc", "sds_get_stats",
+           "sds_get_pivots", "sds_get_diag", "sds_get_factor", "sds_memory_estimates", "sds_profile",
+           "sds_profile_read", "sds_adjoint_scatter",
            "sds_autotune", "sds_graph_capture", "sds_graph_launch", "sds_graph_info", "sds_graph_destroy",
            "sds_set_ir_mode", "sds_set_tiles")
 
@@ -42,6 +43,7 @@ def bind(lib):
     lib.sds_alloc_numeric.argtypes = [vp, c_i32, c_i32]
     lib.sds_factorize.argtypes = [vp, vp, P(FactorParams), P(c_i64)]
     lib.sds_solve.argtypes = [vp, c_i32, c_i32, vp, c_i64, vp, c_i64, c_i32, vp, c_i32, c_dbl, P(c_i32)]
+    lib.sds_solve_noalloc.argtypes = lib.sds_solve.argtypes
     lib.sds_get_stats.argtypes = [vp, vp, vp]
     lib.sds_get_pivots.argtypes = [vp, vp, vp]
     lib.sds_autotune.argtypes = [vp, vp, P(FactorParams), c_i32, vp]
