@@ -11,7 +11,9 @@
 #include <cstring>
 #include <string>
 #include <chrono>
+#include <map>
 #include <mutex>
+#include <set>
 #include <vector>
 
 #include "../../include/sds_hip.h"
@@ -185,6 +187,8 @@ struct sds_handle {
     int64_t cb_len = 0, uv_len = 0;
     int64_t* d_cb_off = nullptr;
     int64_t* d_uv_off = nullptr;
+    bool placed = false;                      // cb_place / uv_place hold this analysis' placements
+    std::vector<int64_t> cb_place, uv_place;  // element offsets of the contribution blocks / update vectors
     void* factor = nullptr;
     void* ufactor = nullptr;  // U^T panels of "G"
     void* cbpool = nullptr;
@@ -324,9 +328,11 @@ struct IdxReader {
     }
 };
 
-// first-fit placement of blocks with lifetimes [first, last] (levels) on an unbounded buffer:
-// level by level, release the blocks whose last level has passed, then place the new ones
-// (largest first). Returns the high-water mark.
+// best-fit placement of blocks with lifetimes [first, last] (levels) on an unbounded buffer:
+// level by level, release the blocks whose last level has passed (merging adjacent holes), then
+// place the new ones, largest first, each in the smallest hole that holds it (or on top).
+// Holes are kept by offset (for merging) and by (length, offset) (for the fit): O(b log b) for
+// b blocks. Returns the high-water mark.
 int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& first, const std::vector<int>& last,
                      int nlev, std::vector<int64_t>& off) {
     int64_t nb = (int64_t)len.size();
@@ -337,24 +343,34 @@ int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& fi
             born[first[b]].push_back(b);
             dies[last[b] + 1].push_back(b);
         }
-    std::vector<std::pair<int64_t, int64_t>> freel;  // sorted (offset, length) holes
+    std::map<int64_t, int64_t> holes;              // offset -> length
+    std::set<std::pair<int64_t, int64_t>> bylen;   // (length, offset)
     int64_t top = 0;
+    auto drop = [&](std::map<int64_t, int64_t>::iterator it) {
+        bylen.erase({it->second, it->first});
+        holes.erase(it);
+    };
     auto release = [&](int64_t o, int64_t l) {
-        auto it = std::lower_bound(freel.begin(), freel.end(), std::make_pair(o, l));
-        it = freel.insert(it, {o, l});
-        size_t k = it - freel.begin();
-        if (k + 1 < freel.size() && freel[k].first + freel[k].second == freel[k + 1].first) {
-            freel[k].second += freel[k + 1].second;
-            freel.erase(freel.begin() + k + 1);
+        auto nx = holes.find(o + l);
+        if (nx != holes.end()) {
+            l += nx->second;
+            drop(nx);
         }
-        if (k > 0 && freel[k - 1].first + freel[k - 1].second == freel[k].first) {
-            freel[k - 1].second += freel[k].second;
-            freel.erase(freel.begin() + k);
+        auto it = holes.lower_bound(o);
+        if (it != holes.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == o) {
+                o = pv->first;
+                l += pv->second;
+                drop(pv);
+            }
         }
-        if (!freel.empty() && freel.back().first + freel.back().second == top) {
-            top = freel.back().first;
-            freel.pop_back();
+        if (o + l == top) {
+            top = o;
+            return;
         }
+        holes[o] = l;
+        bylen.insert({l, o});
     };
     int64_t hw = 0;
     for (int L = 0; L <= nlev; ++L) {
@@ -362,19 +378,18 @@ int64_t place_blocks(const std::vector<int64_t>& len, const std::vector<int>& fi
         std::vector<int64_t>& nbk = born[L];
         std::sort(nbk.begin(), nbk.end(), [&](int64_t a, int64_t b) { return len[a] > len[b]; });
         for (int64_t b : nbk) {
-            int64_t l = len[b];
-            bool placed = false;
-            for (size_t k = 0; k < freel.size(); ++k)
-                if (freel[k].second >= l) {
-                    off[b] = freel[k].first;
-                    if (freel[k].second == l)
-                        freel.erase(freel.begin() + k);
-                    else
-                        freel[k] = {freel[k].first + l, freel[k].second - l};
-                    placed = true;
-                    break;
+            const int64_t l = len[b];
+            auto it = bylen.lower_bound({l, INT64_MIN});
+            if (it != bylen.end()) {
+                const int64_t hl = it->first, ho = it->second;
+                bylen.erase(it);
+                holes.erase(ho);
+                off[b] = ho;
+                if (hl > l) {
+                    holes[ho + l] = hl - l;
+                    bylen.insert({hl - l, ho + l});
                 }
-            if (!placed) {
+            } else {
                 off[b] = top;
                 top += l;
             }
@@ -451,9 +466,14 @@ int build_schedule(sds_handle* h, int dtype) {
         if (m > 0 && (h->snparent[s] >= 0 || big[s])) cblen[s] = m * m;
         if (m > 0 && h->snparent[s] >= 0) uvlen[s] = m;
     }
-    std::vector<int64_t> cb_off, uv_off;
-    h->cb_len = place_blocks(cblen, first, last, nlev, cb_off);
-    h->uv_len = place_blocks(uvlen, first, last, nlev, uv_off);
+    // the placements depend on the analysis only (element offsets): computed once per upload
+    if (!h->placed) {
+        h->cb_len = place_blocks(cblen, first, last, nlev, h->cb_place);
+        h->uv_len = place_blocks(uvlen, first, last, nlev, h->uv_place);
+        h->placed = true;
+    }
+    const std::vector<int64_t>& cb_off = h->cb_place;
+    const std::vector<int64_t>& uv_off = h->uv_place;
     // front lists
     std::vector<int32_t> flist;
     for (auto& L : h->levels) {
@@ -1516,6 +1536,7 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     free_numeric(h);
     free_all(h->symbufs);
     h->have_sym = false;
+    h->placed = false;
     const int b = d->index_bits;
     const int64_t n = d->n, ns = d->ns, nnz = d->nnz;
     h->n = n;

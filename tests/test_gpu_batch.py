@@ -4,9 +4,11 @@ same fronts, same kernels), in FP64 and with the FP32 factor + FP64 refinement."
 import numpy as np
 import pytest
 
+import oracle
 import sparsedirectsolver as sds
 from sparsedirectsolver import matrices as M
 from sparsedirectsolver.batch import UnevenBatch
+from sparsedirectsolver.solver import numeric_factor
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +54,13 @@ def test_batch_members_equal_individual_solves(fp):
         assert torch.equal(batch.split(X)[k], x), f"member {k}"
     A = batch.A + batch.A.T - __import__("scipy").sparse.diags(batch.A.diagonal())
     Xh, Bh = X.cpu().numpy(), B.cpu().numpy()
+    # the CPU oracle on the whole forest: its factor (FP64) and its solution
+    info, F, _ = oracle.cholesky(batch.solver.symbolic, batch.solver.nzval.cpu().numpy())
+    assert info == 0
+    if fp is None:
+        assert np.abs(numeric_factor(batch.solver) - F).max() <= 100 * np.finfo(np.float64).eps * np.abs(F).max()
+    Xr = oracle.solve_cholesky(batch.solver.symbolic, F, Bh)
+    assert np.abs(Xh - Xr).max() <= (1e-10 if fp is None else 1e-8) * np.abs(Xr).max()
     an = abs(A).sum(axis=1).max()
     berr = np.max(np.abs(Bh - A @ Xh).max(axis=0) / (an * np.abs(Xh).max(axis=0) + np.abs(Bh).max(axis=0)))
     assert berr <= (1e-14 if fp is None else 1e-10)

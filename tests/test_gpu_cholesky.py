@@ -38,6 +38,9 @@ CASES = [
     ("varlap3d_14_nd", lambda: (M.variable_laplacian((14, 13, 12), seed=3), sds.grid_nd_perm(14, 13, 12),
                                 (128, 0.25, 8))),
     ("lap3d_20_natural", lambda: (M.laplacian3d(20, 20, 20), None, (64, 0.25, 8))),
+    # a 784-column root front: wide solve steps (more than 512 columns) and a 13-step chain,
+    # element by element
+    ("lap3d_28_nd_wide", lambda: (M.laplacian3d(28, 28, 28), sds.grid_nd_perm(28, 28, 28), (512, 0.25, 8))),
     # the reference's CPU-runnable configuration: 2-D 100 x 100, the build's AMD ordering
     ("lap2d_100x100_amd", lambda: (M.laplacian2d(100, 100), None, (32, 0.25, 8))),
 ]

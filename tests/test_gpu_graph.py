@@ -5,10 +5,11 @@ Cholesky (FP64 and FP32 + refinement), LDL^T and LU."""
 import numpy as np
 import pytest
 
+import oracle
 import sparsedirectsolver as sds
 from generators import kkt_matrix
 from sparsedirectsolver import matrices as M
-from sparsedirectsolver.solver import graph_capture, graph_replay
+from sparsedirectsolver.solver import graph_capture, graph_replay, numeric_factor, numeric_pivots
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +40,31 @@ def _case(kind):
     return A, s, nz
 
 
+def _oracle_check(kind, s, nz, Bh, Xg):
+    """The replay's factor and solution against the CPU oracle on the same values (the solver's
+    default pivoting parameters): the Cholesky factor to 100 eps, the LDL^T and LU factors, pivot
+    order and kinds bitwise; the solution to 1e-10 relative, 1e-8 after the FP32 factor's
+    refinement, and 1e-6 for the KKT system (delta = 1e-8 makes it ill-conditioned enough that
+    two backward-stable solves with different summation orders differ at about 1e-9)."""
+    S = s.symbolic
+    if kind in ("spd", "spd32"):
+        info, F, _ = oracle.cholesky(S, nz)
+        assert info == 0
+        if kind == "spd":
+            assert np.abs(numeric_factor(s) - F).max() <= 100 * np.finfo(np.float64).eps * np.abs(F).max()
+        Xr = oracle.solve_cholesky(S, F, Bh)
+    else:
+        F = oracle.ldlt(S, nz) if kind == "ldlt" else oracle.lu(S, nz)
+        piv, pkind = numeric_pivots(s)
+        assert np.array_equal(piv, F["piv"]) and np.array_equal(pkind, F["kind"])
+        assert np.array_equal(numeric_factor(s), F["factor"])
+        if kind == "lu":
+            assert np.array_equal(numeric_factor(s, 1), F["ufactor"])
+        Xr = oracle.solve_ldlt(S, F, Bh) if kind == "ldlt" else oracle.solve_lu(S, F, Bh)
+    tol = {"spd": 1e-10, "spd32": 1e-8, "ldlt": 1e-6, "lu": 1e-10}[kind]
+    assert np.abs(Xg - Xr).max() <= tol * np.abs(Xr).max()
+
+
 @pytest.mark.parametrize("kind", ["spd", "spd32", "ldlt", "lu"])
 def test_graph_replays_match_eager(kind):
     import torch
@@ -58,6 +84,8 @@ def test_graph_replays_match_eager(kind):
         info = graph_replay(s, Xg, B, check_info=True)
         assert info == 0
         torch.cuda.synchronize()
+        # the replay's factor and solution against the oracle on the same values
+        _oracle_check(kind, s, (base * scale).cpu().numpy(), B.cpu().numpy(), Xg.cpu().numpy())
         # the eager path on the same values
         Xe = torch.zeros_like(B)
         sds.execute("refactorization", s)

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+import oracle
 import sparsedirectsolver as sds
 from sparsedirectsolver import matrices as M
 from sparsedirectsolver.errors import InvalidValueError
@@ -72,6 +73,11 @@ def test_members_bitwise_as_solved_alone(matrix_values):
         xk = torch.zeros((n, r), dtype=torch.float64, device="cuda")
         sds.execute("solve", q, xk, B[:, k * r:(k + 1) * r].contiguous())
         assert torch.equal(X[:, k * r:(k + 1) * r], xk), k
+        # and the CPU oracle's solution of the member
+        info, F, _ = oracle.cholesky(q.symbolic, mats[k].data)
+        assert info == 0
+        Xr = oracle.solve_cholesky(q.symbolic, F, B[:, k * r:(k + 1) * r].cpu().numpy())
+        assert np.abs(xk.cpu().numpy() - Xr).max() <= 1e-10 * np.abs(Xr).max(), k
     assert sds.getparam(s, "info") == [0] * nb
     assert sds.getparam(s, "inertia") == [(n, 0)] * nb
 
