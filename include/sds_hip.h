@@ -189,25 +189,27 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
 /*
  * Tile-shape autotuner (no counterpart in the reference, whose kernels have fixed shapes;
  * the chosen form of level and batch launches whose tile shapes come from an autotuner):
- * times `reps` factorizations of the values nzval for each candidate shape of the large-front
+ * times `reps` factorizations of the values nzval for each candidate shape of the front
  * kernels (child rows per extend-add task: 512, 1024, 2048, 4096; for Cholesky also the K
  * chunk of the contribution-block update: 16, 32 or 64, the number of trailing-update tiles of a
- * step from which they get a launch of their own: never, 256, 512, 1024, and the number of
+ * step from which they get a launch of their own: never, 256, 512, 1024, the number of
  * workgroups from which a step launch uses the build for two workgroups per CU: never, 1, 64,
- * 128, 256, 512), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
- * are not kept: factorize again). tiles (may be NULL, else 4 entries) receives {extend-add
- * rows, K chunk, trailing split, two-per-CU steps}. Synchronous; needs sds_alloc_numeric
- * first.
+ * 128, 256, 512, and the small-front kernels: 0 = fronts of at most 32 rows one wave each and
+ * the others in blocked panels, 1 = one wave each / column by column, 2 = all column by column),
+ * keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
+ * are not kept: factorize again). tiles (may be NULL, else 5 entries) receives {extend-add
+ * rows, K chunk, trailing split, two-per-CU steps, small-front kernels}. Every candidate gives
+ * the same factor bit for bit. Synchronous; needs sds_alloc_numeric first.
  */
 int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
                  int32_t* tiles);
 
 /*
- * Set the tile shapes of the large-front kernels directly, as sds_autotune reports them: tiles =
+ * Set the tile shapes of the front kernels directly, as sds_autotune reports them: tiles =
  * {extend-add rows (512, 1024, 2048 or 4096), K chunk (16, 32 or 64), trailing split (0 = never,
- * else >= 1), two-per-CU steps (0 = never, else >= 1)}. No counterpart in the reference (whose
- * kernels have fixed shapes); it restores a saved tuning without timing again. Takes effect at
- * the next sds_alloc_numeric (the launch plan depends on it).
+ * else >= 1), two-per-CU steps (0 = never, else >= 1), small-front kernels (0, 1 or 2)}. No
+ * counterpart in the reference (whose kernels have fixed shapes); it restores a saved tuning
+ * without timing again. Takes effect at the next sds_alloc_numeric (the launch plan depends on it).
  */
 int sds_set_tiles(sds_handle_t handle, const int32_t* tiles);
 

@@ -14,6 +14,7 @@
 // Reference counterparts: regime B fused kernel src/numeric/front.jl:154-206,
 // assembly src/numeric/assembly.jl:18-252, regime C panel src/numeric/factorize.jl:240-269,
 // statistics src/numeric/factorize.jl:13-48.
+#include "k_dense.h"
 #include "k_front.h"
 #include "sds_common.h"
 #include "sds_kernels.h"
@@ -32,10 +33,6 @@
 
 namespace sds {
 
-// Cholesky pivot acceptance on the FP64 value of the pivot (src/numeric/front.jl:46-50):
-// a pivot in [0, eps] is raised to eps (cuDSS pivot_epsilon), negative or NaN fails.
-__device__ inline bool chol_accept(double a, double eps) { return a > eps || (eps > 0.0 && a >= 0.0); }
-
 // 1/sqrt(a), a > 0: the hardware estimate refined by two Newton steps (the pivots of the
 // large fronts' diagonal blocks; the result is within an ulp or two, well inside the
 // factor's accuracy)
@@ -53,8 +50,44 @@ __device__ inline double rsqrt_nr(double a) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_front_small_chol(DevSym S, const int32_t* __restrict__ fronts,
                                                          const double* __restrict__ nz, T* __restrict__ factor,
-                                                         T* __restrict__ cbpool, int32_t* __restrict__ status,
-                                                         double eps) {
+                                                         T* __restrict__ cbpool, T* __restrict__ DB,
+                                                         int32_t* __restrict__ status, double eps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
+    __shared__ int sh;
+    const int s = fronts[blockIdx.x];
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    const int tid = threadIdx.x;
+    assemble_dense<T>(S, s, nz, cbpool, F, f, 0);
+    // blocked Cholesky of the w pivot columns over the whole front (f <= 128)
+    const int st = front_chol_lds<T, 2>(F, f, w, eps, &sh);
+    // the panel (strict upper triangle zero) and the contribution block's lower triangle, a wave
+    // per column, lanes along the rows
+    const int lane = tid & 63, wv = tid >> 6;
+    T* P = factor + S.panel_off[s];
+    for (int j = wv; j < w; j += 4)
+        for (int i = lane; i < f; i += 64) P[(int64_t)j * f + i] = i >= j ? F[j * f + i] : T(0);
+    const int64_t co = S.cb_off[s];
+    if (co >= 0) {
+        T* C = cbpool + co;
+        for (int jj = wv; jj < m; jj += 4)
+            for (int ii = jj + lane; ii < m; ii += 64) C[(int64_t)jj * m + ii] = F[(w + jj) * f + w + ii];
+    }
+    // the diagonal block's inverse for the sweeps (fronts of at most FUSED_INV_MAX columns)
+    if (w <= FUSED_INV_MAX) diag_inverse_lds<T>(F, f, w, DB + (int64_t)S.super_ptr[s] * NB);
+    if (tid == 0) status[s] = st;
+}
+
+// The same small fronts factored column by column: per pivot column two barriers, the column
+// scaled by 1/d, then the whole trailing lower triangle updated by all threads (the reference's
+// regime-B form, src/numeric/front.jl:154-206). The entries receive exactly the operations of
+// k_front_small_chol in the same order, so both give the same factor bit for bit; which one is
+// faster depends on the fronts (the autotuner chooses).
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_col_chol(DevSym S, const int32_t* __restrict__ fronts,
+                                                       const double* __restrict__ nz, T* __restrict__ factor,
+                                                       T* __restrict__ cbpool, T* __restrict__ DB,
+                                                       int32_t* __restrict__ status, double eps) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);
     __shared__ int32_t st;
@@ -63,7 +96,6 @@ __global__ void __launch_bounds__(256) k_front_small_chol(DevSym S, const int32_
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0) st = 0;
     assemble_dense<T>(S, s, nz, cbpool, F, f, 0);
-    // right-looking Cholesky of the w pivot columns over the whole front
     for (int j = 0; j < w; ++j) {
         if (tid == 0) {
             const double a = (double)F[j * f + j];
@@ -74,31 +106,145 @@ __global__ void __launch_bounds__(256) k_front_small_chol(DevSym S, const int32_
         }
         __syncthreads();
         if (st) break;
-        const T d = F[j * f + j];
-        for (int i = j + 1 + tid; i < f; i += nt) F[j * f + i] /= d;
+        const T rd = T(1) / F[j * f + j];
+        for (int i = j + 1 + tid; i < f; i += nt) F[j * f + i] = F[j * f + i] * rd;
         __syncthreads();
         const int r = f - j - 1;
         for (int q = tid; q < r * r; q += nt) {
             const int kk = q / r, ii = q - kk * r;
             if (ii < kk) continue;
-            F[(j + 1 + kk) * f + j + 1 + ii] -= F[j * f + j + 1 + ii] * F[j * f + j + 1 + kk];
+            const T prod = F[j * f + j + 1 + ii] * F[j * f + j + 1 + kk];
+            F[(j + 1 + kk) * f + j + 1 + ii] = F[(j + 1 + kk) * f + j + 1 + ii] - prod;
         }
         __syncthreads();
     }
+    const int lane = tid & 63, wv = tid >> 6;
     T* P = factor + S.panel_off[s];
-    for (int q = tid; q < f * w; q += nt) {
-        const int j = q / f, i = q - j * f;
-        P[q] = i >= j ? F[q] : T(0);
-    }
+    for (int j = wv; j < w; j += 4)
+        for (int i = lane; i < f; i += 64) P[(int64_t)j * f + i] = i >= j ? F[j * f + i] : T(0);
     const int64_t co = S.cb_off[s];
     if (co >= 0) {
         T* C = cbpool + co;
-        for (int q = tid; q < m * m; q += nt) {
-            const int jj = q / m, ii = q - jj * m;
-            if (ii >= jj) C[q] = F[(w + jj) * f + w + ii];
+        for (int jj = wv; jj < m; jj += 4)
+            for (int ii = jj + lane; ii < m; ii += 64) C[(int64_t)jj * m + ii] = F[(w + jj) * f + w + ii];
+    }
+    if (w <= FUSED_INV_MAX) diag_inverse_lds<T>(F, f, w, DB + (int64_t)S.super_ptr[s] * NB);
+    if (tid == 0) status[s] = st;
+}
+
+// fronts of at most TINY_F rows: one wave per front, four per workgroup (no block barriers). The
+// wave assembles the front in its own LDS square (zero, A, then the children in child_list
+// order, as assemble_dense), then each lane takes one row into registers and the wave factors
+// the w pivot columns one by one: the pivot broadcast from its lane, the rows below scaled by
+// 1/d, every later column k of row i given -= L(i, j) L(k, j) with L(k, j) read from lane k. An
+// entry's updates come in column order as separate products and differences, the arithmetic of
+// the blocked kernel. The panel and contribution block are written from the registers, and for
+// fronts of at most FUSED_INV_MAX columns the diagonal block's inverse (column c by lane c, the
+// arithmetic of the per-block inverse kernel).
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_tiny_chol(DevSym S, const int32_t* __restrict__ fronts, int nfronts,
+                                                        const double* __restrict__ nz, T* __restrict__ factor,
+                                                        T* __restrict__ cbpool, T* __restrict__ DB,
+                                                        int32_t* __restrict__ status, double eps) {
+    __shared__ T Fs[4][TINY_F * TINY_F];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int id = blockIdx.x * 4 + wv;
+    if (id >= nfronts) return;  // the whole wave: no block barriers below
+    T* F = Fs[wv];
+    const int s = fronts[id];
+    const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
+    for (int q = lane; q < f * f; q += 64) F[q] = T(0);
+    wave_sync();
+    // A: one lane sums each run of entries with one destination, in nzval order (distinct runs
+    // have distinct destinations)
+    const int64_t a0 = S.asm_ptr[s], a1 = S.asm_ptr[s + 1];
+    for (int64_t k = a0 + lane; k < a1; k += 64) {
+        const int64_t d = S.asm_dst[k];
+        if (k > a0 && S.asm_dst[k - 1] == d) continue;
+        double v = 0.0;
+        for (int64_t q = k; q < a1 && S.asm_dst[q] == d; ++q) v += nz[S.asm_src[q]];
+        F[d >= 0 ? d : -(d + 1)] += T(v);
+    }
+    wave_sync();
+    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
+        const int c = S.child_list[k];
+        const int64_t co = S.cb_off[c];
+        if (co < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* C = cbpool + co;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        const int rl = lane < mc ? ri[lane] : 0;
+        for (int jj = 0; jj < mc; ++jj) {
+            const int pj = __builtin_amdgcn_readlane(rl, jj);
+            if (lane >= jj && lane < mc) F[pj * f + rl] += C[(int64_t)jj * mc + lane];
+        }
+        wave_sync();
+    }
+    // row `lane` into registers
+    T r[TINY_F];
+#pragma unroll
+    for (int k = 0; k < TINY_F; ++k) r[k] = (k < f && lane < f) ? F[k * f + lane] : T(0);
+    int st = 0;
+#pragma unroll
+    for (int j = 0; j < TINY_F; ++j) {
+        if (j < w && !st) {
+            const double a = (double)lane_value(r[j], j);
+            if (!chol_accept(a, eps)) {
+                st = j + 1;
+            } else {
+                const T d = T(sqrt(fmax(a, eps)));
+                const T rd = T(1) / d;
+                if (lane == j) r[j] = d;
+                else if (lane > j) r[j] = r[j] * rd;
+#pragma unroll
+                for (int k = j + 1; k < TINY_F; ++k) {
+                    if (k < f) {
+                        const T lk = lane_value(r[j], k);
+                        if (lane >= k) {
+                            const T prod = r[j] * lk;
+                            r[k] = r[k] - prod;
+                        }
+                    }
+                }
+            }
         }
     }
-    if (tid == 0) status[s] = st;
+    T* P = factor + S.panel_off[s];
+#pragma unroll
+    for (int j = 0; j < TINY_F; ++j)
+        if (j < w && lane < f) P[(int64_t)j * f + lane] = lane >= j ? r[j] : T(0);
+    const int64_t co = S.cb_off[s];
+    if (co >= 0) {
+        T* C = cbpool + co;
+#pragma unroll
+        for (int k = 0; k < TINY_F; ++k)
+            if (k >= w && k < f && lane >= k && lane < f) C[(int64_t)(k - w) * m + lane - w] = r[k];
+    }
+    if (w <= FUSED_INV_MAX) {
+        // column `lane` of L11^-1: L(rr, mm) is row rr's register mm
+        T x[FUSED_INV_MAX];
+#pragma unroll
+        for (int q = 0; q < FUSED_INV_MAX; ++q) x[q] = q == lane ? T(1) : T(0);
+#pragma unroll
+        for (int mm = 0; mm < FUSED_INV_MAX; ++mm) {
+            if (mm < w) {
+                const T rinv = T(1) / lane_value(r[mm], mm);
+                x[mm] = x[mm] * rinv;
+                const T xm = x[mm];
+#pragma unroll
+                for (int rr = mm + 1; rr < FUSED_INV_MAX; ++rr)
+                    if (rr < w) x[rr] = fma(-lane_value(r[mm], rr), xm, x[rr]);
+            }
+        }
+        if (lane < w) {
+            T* o = DB + (int64_t)(S.super_ptr[s] + lane) * NB;
+#pragma unroll
+            for (int q = 0; q < FUSED_INV_MAX; ++q) o[q] = q < w ? x[q] : T(0);
+#pragma unroll
+            for (int q = FUSED_INV_MAX; q < NB; ++q) o[q] = T(0);
+        }
+    }
+    if (lane == 0) status[s] = st;
 }
 
 // ---------------------------------------------------------------------------
@@ -784,20 +930,35 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
 }
 
 
-// per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and the smallest
-// failed factor column (1-based, 0 = none) into info[0]
+// per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and, per block, the
+// smallest failed factor column (1-based, INT64_MAX = none) into part[blockIdx.x]
 __global__ void __launch_bounds__(256) k_chol_stats(DevSym S, const int32_t* __restrict__ status,
-                                                   int64_t* __restrict__ stats, int64_t* __restrict__ info) {
+                                                   int64_t* __restrict__ stats, int64_t* __restrict__ part) {
     __shared__ int64_t best[256];
     int64_t m = INT64_MAX;
-    for (int s = threadIdx.x; s < S.ns; s += blockDim.x) {
+    for (int64_t s = blockIdx.x * 256 + threadIdx.x; s < S.ns; s += (int64_t)gridDim.x * 256) {
         const int64_t fi = status[s];
-        int64_t* st = stats + (int64_t)s * NF;
+        int64_t* st = stats + s * NF;
         st[0] = fi == 0 ? (int64_t)S.fncols[s] : fi - 1;
         st[1] = st[2] = st[3] = st[4] = 0;
         st[5] = fi;
         if (fi > 0) m = min(m, (int64_t)S.super_ptr[s] + fi);
     }
+    best[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) best[threadIdx.x] = min(best[threadIdx.x], best[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = best[0];
+}
+
+// info[0] = the smallest of the n block minima, 0 for none (one block)
+__global__ void __launch_bounds__(256) k_info_final(const int64_t* __restrict__ part, int n,
+                                                   int64_t* __restrict__ info) {
+    __shared__ int64_t best[256];
+    int64_t m = INT64_MAX;
+    for (int b = threadIdx.x; b < n; b += 256) m = min(m, part[b]);
     best[threadIdx.x] = m;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
@@ -817,10 +978,25 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
 
 template <typename T>
 void launch_front_small_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
-                             const double* nz, T* factor, T* cbpool, int32_t* status, double eps) {
+                             const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps) {
     size_t lds = (size_t)fmax * fmax * sizeof(T);
-    hipLaunchKernelGGL(k_front_small_chol<T>, dim3(nfronts), dim3(256), lds, st, S, fronts, nz, factor, cbpool,
+    hipLaunchKernelGGL(k_front_small_chol<T>, dim3(nfronts), dim3(256), lds, st, S, fronts, nz, factor, cbpool, DB,
                        status, eps);
+}
+
+template <typename T>
+void launch_front_col_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                           const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps) {
+    size_t lds = (size_t)fmax * fmax * sizeof(T);
+    hipLaunchKernelGGL(k_front_col_chol<T>, dim3(nfronts), dim3(256), lds, st, S, fronts, nz, factor, cbpool, DB,
+                       status, eps);
+}
+
+template <typename T>
+void launch_front_tiny_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const double* nz,
+                            T* factor, T* cbpool, T* DB, int32_t* status, double eps) {
+    hipLaunchKernelGGL(k_front_tiny_chol<T>, dim3((nfronts + 3) / 4), dim3(256), 0, st, S, fronts, nfronts, nz, factor,
+                       cbpool, DB, status, eps);
 }
 
 template <typename T>
@@ -864,7 +1040,14 @@ void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n
 }
 
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info) {
-    hipLaunchKernelGGL(k_chol_stats, dim3(1), dim3(256), 0, st, S, status, stats, info);
+    // info[8 ..] holds the block minima (INFO_BLOCKS entries)
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(INFO_BLOCKS, (S.ns + 255) / 256));
+    hipLaunchKernelGGL(k_chol_stats, dim3(blocks), dim3(256), 0, st, S, status, stats, info + 8);
+    hipLaunchKernelGGL(k_info_final, dim3(1), dim3(256), 0, st, info + 8, blocks, info);
+}
+
+void launch_info_final(hipStream_t st, int blocks, int64_t* info) {
+    hipLaunchKernelGGL(k_info_final, dim3(1), dim3(256), 0, st, info + 8, blocks, info);
 }
 
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
@@ -875,7 +1058,11 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
 
 #define INST(T)                                                                                                   \
     template void launch_front_small_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
-                                             T*, T*, int32_t*, double);                                           \
+                                             T*, T*, T*, int32_t*, double);                                       \
+    template void launch_front_col_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, T*, \
+                                           T*, T*, int32_t*, double);                                             \
+    template void launch_front_tiny_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, const double*, T*, T*, \
+                                            T*, int32_t*, double);                                                \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);            \
     template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \

@@ -817,11 +817,11 @@ __global__ void k_absmax(int64_t nnz, const double* __restrict__ nz, unsigned lo
 // info[0] = the smallest failed factor column (1-based; 0 = none) over the fronts'
 // statistics (ldlt.jl:208-216)
 __global__ void __launch_bounds__(256) k_info_from_stats(DevSym S, const int64_t* __restrict__ stats,
-                                                        int64_t* __restrict__ info) {
+                                                        int64_t* __restrict__ part) {
     __shared__ int64_t best[256];
     int64_t mn = INT64_MAX;
-    for (int s = threadIdx.x; s < S.ns; s += blockDim.x) {
-        const int64_t fi = stats[(int64_t)s * NF + ST_INFO];
+    for (int64_t s = blockIdx.x * 256 + threadIdx.x; s < S.ns; s += (int64_t)gridDim.x * 256) {
+        const int64_t fi = stats[s * NF + ST_INFO];
         if (fi > 0) mn = min(mn, (int64_t)S.super_ptr[s] + fi);
     }
     best[threadIdx.x] = mn;
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(256) k_info_from_stats(DevSym S, const int64_t
         if (threadIdx.x < h) best[threadIdx.x] = min(best[threadIdx.x], best[threadIdx.x + h]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) info[0] = best[0] == INT64_MAX ? 0 : best[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = best[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -877,7 +877,10 @@ void launch_absmax(hipStream_t st, int64_t nnz, const double* nz, double* aux) {
 }
 
 void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stats, int64_t* info) {
-    hipLaunchKernelGGL(k_info_from_stats, dim3(1), dim3(256), 0, st, S, stats, info);
+    // block minima at info + 8, then the smallest into info[0]
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(INFO_BLOCKS, (S.ns + 255) / 256));
+    hipLaunchKernelGGL(k_info_from_stats, dim3(blocks), dim3(256), 0, st, S, stats, info + 8);
+    launch_info_final(st, blocks, info);
 }
 
 #define INST(T)                                                                                                    \

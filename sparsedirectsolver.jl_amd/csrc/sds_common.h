@@ -40,6 +40,11 @@ struct Task {
 enum { KIND_1X1 = 1, KIND_2X2_FIRST = 2, KIND_2X2_SECOND = 3, KIND_PERTURBED = 4 };
 
 constexpr int NB = 64;     // pivot block of the blocked front factorization
+// widest small front whose diagonal-block inverse its factorization kernel forms (wider ones: the
+// per-block inverse kernel, whose register footprint would cost the front kernels their occupancy)
+constexpr int FUSED_INV_MAX = 16;
+constexpr int INFO_BLOCKS = 256;
+constexpr int TINY_F = 32;  // fronts of at most this many rows: one wave each (Cholesky)  // block minima of the info reduction (after the 8 info words)
 constexpr int TILE = 64;   // Schur-update tile
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int EA_COLS = 8;      // child columns per extend-add task

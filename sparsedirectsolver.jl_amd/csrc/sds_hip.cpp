@@ -87,6 +87,11 @@ struct Level {
     std::vector<int32_t> ssmall;        // fronts the solve handles one workgroup each
     int fmax_small = 0;
     int64_t off_small = 0, off_all = 0, off_ssmall = 0;  // offsets into the device front lists
+    // Cholesky: the small fronts of at most TINY_F rows (one wave each) and the others
+    std::vector<int32_t> tiny, mid;
+    int64_t off_tiny = 0, off_mid = 0;
+    int fmax_mid = 0;
+    double tiny_flops = 0, tiny_bytes = 0;
     std::vector<Launch> plan;           // large-front launches of this level
     std::vector<Launch> fwd_plan, bwd_plan;  // large-front solve launches (per right-hand side work)
     double small_flops = 0, small_bytes = 0;
@@ -201,6 +206,8 @@ struct sds_handle {
     int rest_split = 0;  // trailing tiles of a step from which they get a launch of their own (0: never)
     int max_rest = 0;    // most trailing tiles of one step in the current plan
     int step_occ2 = 0;   // step launches of at least this many workgroups use the two-per-CU build (0: none)
+    int small_kind = 0;  // small-front kernels (Cholesky): 0 one wave per tiny front + blocked,
+                         // 1 one wave per tiny front + column by column, 2 column by column
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -450,6 +457,12 @@ int build_schedule(sds_handle* h, int dtype) {
         if (h->fnrows[s] <= fsmall) {
             L.small.push_back((int32_t)s);
             L.fmax_small = std::max(L.fmax_small, h->fnrows[s]);
+            if (h->fnrows[s] <= TINY_F) {
+                L.tiny.push_back((int32_t)s);
+            } else {
+                L.mid.push_back((int32_t)s);
+                L.fmax_mid = std::max(L.fmax_mid, h->fnrows[s]);
+            }
         } else {
             L.big.push_back((int32_t)s);
             big[s] = 1;
@@ -479,6 +492,10 @@ int build_schedule(sds_handle* h, int dtype) {
     for (auto& L : h->levels) {
         L.off_small = (int64_t)flist.size();
         flist.insert(flist.end(), L.small.begin(), L.small.end());
+        L.off_tiny = (int64_t)flist.size();
+        flist.insert(flist.end(), L.tiny.begin(), L.tiny.end());
+        L.off_mid = (int64_t)flist.size();
+        flist.insert(flist.end(), L.mid.begin(), L.mid.end());
     }
     for (auto& L : h->levels) {
         L.off_all = (int64_t)flist.size();
@@ -514,6 +531,10 @@ int build_schedule(sds_handle* h, int dtype) {
         for (int32_t s : L.small) {
             L.small_flops += front_flops(s);
             L.small_bytes += front_bytes(s);
+            if (h->fnrows[s] <= TINY_F) {
+                L.tiny_flops += front_flops(s);
+                L.tiny_bytes += front_bytes(s);
+            }
         }
         for (int32_t s : L.ssmall) {
             double f = h->fnrows[s], w = h->fncols[s];
@@ -841,8 +862,10 @@ int build_schedule(sds_handle* h, int dtype) {
     if (h->method == M_CHOLESKY) {
         std::vector<Task> t[3];
         double fl[3] = {0, 0, 0}, by[3] = {0, 0, 0};
+        // big fronts: done by k_big_diag; small fronts of at most FUSED_INV_MAX columns: by their
+        // factorization
         for (int64_t s = 0; s < ns; ++s)
-            for (int k = 0; k < h->fncols[s] && !big[s]; k += NB) {  // big fronts: done by k_big_diag
+            for (int k = 0; k < h->fncols[s] && !big[s] && h->fncols[s] > FUSED_INV_MAX; k += NB) {
                 const int nbi = std::min(NB, h->fncols[s] - k), cls = nbi <= 16 ? 0 : nbi <= 32 ? 1 : 2;
                 t[cls].push_back({(int32_t)s, k, 0, 0});
                 const double nb = nbi;
@@ -944,11 +967,28 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     launch_fill_i32(st, h->status, h->ns, 0);
     for (auto& L : h->levels) {
         if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
-        if (!L.small.empty())
-            timed(h, SDS_K_FRONT_SMALL, L.small_flops, L.small_bytes, [&] {
-                launch_front_small_chol<T>(st, h->dsym, h->d_fronts + L.off_small, (int)L.small.size(), L.fmax_small,
-                                           nz, factor, cb, h->status, eps);
-            });
+        if (!L.small.empty()) {
+            // small_kind 0: fronts of at most TINY_F rows one wave each, the others blocked;
+            // 1: one wave each / column by column; 2: all column by column
+            T* DB = static_cast<T*>(h->dbuf);
+            const bool tiny = h->small_kind != 2;
+            if (tiny && !L.tiny.empty())
+                timed(h, SDS_K_FRONT_SMALL, L.tiny_flops, L.tiny_bytes, [&] {
+                    launch_front_tiny_chol<T>(st, h->dsym, h->d_fronts + L.off_tiny, (int)L.tiny.size(), nz, factor, cb,
+                                              DB, h->status, eps);
+                });
+            const int32_t* fl = h->d_fronts + (tiny ? L.off_mid : L.off_small);
+            const int nf = (int)(tiny ? L.mid.size() : L.small.size());
+            const int fm = tiny ? L.fmax_mid : L.fmax_small;
+            if (nf)
+                timed(h, SDS_K_FRONT_SMALL, tiny ? L.small_flops - L.tiny_flops : L.small_flops,
+                      tiny ? L.small_bytes - L.tiny_bytes : L.small_bytes, [&] {
+                          if (h->small_kind == 0)
+                              launch_front_small_chol<T>(st, h->dsym, fl, nf, fm, nz, factor, cb, DB, h->status, eps);
+                          else
+                              launch_front_col_chol<T>(st, h->dsym, fl, nf, fm, nz, factor, cb, DB, h->status, eps);
+                      });
+        }
         for (const Launch& ln : L.plan) {
             const Task* t = h->d_tasks + ln.off;
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
@@ -1736,7 +1776,7 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     h->aux = static_cast<double*>(p);
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * NF * (h->ns + 1), &p))) return rc;
     h->stats = static_cast<int64_t*>(p);
-    if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * 8, &p))) return rc;
+    if ((rc = dev_alloc(h, h->numbufs, sizeof(int64_t) * (8 + INFO_BLOCKS), &p))) return rc;
     h->d_info = static_cast<int64_t*>(p);
     HIPCHK(h, hipHostMalloc((void**)&h->h_info, sizeof(int64_t) * 8, hipHostMallocDefault));
     HIPCHK(h, hipHostMalloc((void**)&h->h_norms, sizeof(double) * 2 * MAX_NRHS, hipHostMallocDefault));
@@ -1845,19 +1885,20 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     const int dtype = h->dtype;
     // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
     struct Cand {
-        int ea, kc, rs, oc;
+        int ea, kc, rs, oc, sk;
     };
     std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
         for (int kc : {16, 32, 64})
-            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0});
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0, 0});
     if (h->method == M_CHOLESKY) {  // with the best of the above
         for (int rs : {256, 512, 1024})
-            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0});
-        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc});
+            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0, 0});
+        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc, 0});
+        for (int sk : {1, 2}) cand.push_back({-1, -1, -1, -1, sk});
     }
     double best = 1e300;
-    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2};
+    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2, h->small_kind};
     int64_t info = 0;
     int rc;
     for (Cand c : cand) {
@@ -1866,10 +1907,12 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
             c.kc = pick.kc;
         }
         if (c.rs < 0) c.rs = pick.rs;
+        if (c.oc < 0) c.oc = pick.oc;
         h->ea_rows = c.ea;
         h->upd_kc = c.kc;
         h->rest_split = c.rs;
         h->step_occ2 = c.oc;
+        h->small_kind = c.sk;
         if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
         if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
@@ -1885,12 +1928,14 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     h->upd_kc = pick.kc;
     h->rest_split = pick.rs;
     h->step_occ2 = pick.oc;
+    h->small_kind = pick.sk;
     if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
     if (tiles) {
         tiles[0] = pick.ea;
         tiles[1] = pick.kc;
         tiles[2] = pick.rs;
         tiles[3] = pick.oc;
+        tiles[4] = pick.sk;
     }
     return SDS_SUCCESS;
 }
@@ -1898,15 +1943,17 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
 int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     if (!h) return SDS_INVALID_VALUE;
     if (!tiles) return fail(h, SDS_INVALID_VALUE, "tiles is NULL");
-    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3];
+    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4];
     if (ea != 512 && ea != 1024 && ea != 2048 && ea != 4096)
         return fail(h, SDS_INVALID_VALUE, "extend-add rows must be 512, 1024, 2048 or 4096");
     if (kc != 16 && kc != 32 && kc != 64) return fail(h, SDS_INVALID_VALUE, "the K chunk must be 16, 32 or 64");
     if (rs < 0 || oc < 0) return fail(h, SDS_INVALID_VALUE, "the trailing split and two-per-CU steps must be >= 0");
+    if (sk < 0 || sk > 2) return fail(h, SDS_INVALID_VALUE, "the small-front kernel must be 0, 1 or 2");
     h->ea_rows = ea;
     h->upd_kc = kc;
     h->rest_split = rs;
     h->step_occ2 = oc;
+    h->small_kind = sk;
     return SDS_SUCCESS;
 }
 

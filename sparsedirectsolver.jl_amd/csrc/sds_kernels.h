@@ -5,8 +5,14 @@
 namespace sds {
 
 template <typename T>
+void launch_front_col_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
+                           const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps);
+template <typename T>
+void launch_front_tiny_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const double* nz,
+                            T* factor, T* cbpool, T* DB, int32_t* status, double eps);
+template <typename T>
 void launch_front_small_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
-                             const double* nz, T* factor, T* cbpool, int32_t* status, double eps);
+                             const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps);
 template <typename T>
 void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks, int n, const double* nz, T* factor,
                              T* cbpool);
@@ -23,6 +29,8 @@ template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode, int kc);
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info);
+// info[0] = the smallest of the `blocks` block minima at info + 8 (0 for none)
+void launch_info_final(hipStream_t st, int blocks, int64_t* info);
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 
 // LDL^T (k_ldlt.hip)

@@ -153,24 +153,36 @@ def test_every_tile_candidate_gives_the_same_factor():
     through sds_set_tiles) gives bitwise the factor of the default shape. The 56 x 56 x 40
     Laplacian in nested dissection with (512, 0.25, 8) has a step with 1,282 trailing tiles over
     its level and step launches of 1,564 workgroups, so every trailing-split threshold (256, 512,
-    1024) and every two-per-CU threshold (1 to 512) switches some launch over."""
+    1024) and every two-per-CU threshold (1 to 512) switches some launch over. Its small fronts
+    have from 8 to 90 rows, so each small-front kernel choice (one wave per front of at most 32
+    rows with blocked or column-by-column panels above, or column by column for all) factors some;
+    a solve with the same right-hand sides is bitwise the same too (the diagonal-block inverses
+    formed in the front kernels or by the inverse kernel)."""
+    import torch
     A = M.laplacian3d(56, 56, 40)
     s, nz = _solver(A, sds.grid_nd_perm(56, 56, 40), (512, 0.25, 8))
-    base = dict(extend_add_rows=1024, cb_k_chunk=32, trailing_split=0, step_two_per_cu=0)
+    base = dict(extend_add_rows=1024, cb_k_chunk=32, trailing_split=0, step_two_per_cu=0, small_front_kernel=0)
     sds.set_tiles(s, base)
     sds.execute("factorization", s)
     assert sds.getparam(s, "info") == 0
     ref = numeric_factor(s)
+    B = torch.tensor(np.random.default_rng(4).standard_normal((A.shape[0], 2)), dtype=torch.float64, device="cuda")
+    Xref = torch.zeros_like(B)
+    sds.execute("solve", s, Xref, B)
     cands = [dict(base, extend_add_rows=ea, cb_k_chunk=kc) for ea in (512, 1024, 2048, 4096) for kc in (16, 32, 64)]
     cands += [dict(base, trailing_split=rs) for rs in (256, 512, 1024)]
     cands += [dict(base, step_two_per_cu=oc) for oc in (1, 64, 128, 256, 512)]
     cands += [dict(base, trailing_split=256, step_two_per_cu=1)]
+    cands += [dict(base, small_front_kernel=k) for k in (1, 2)]
     for c in cands:
         print("tiles", c, flush=True)
         sds.set_tiles(s, c)
         sds.execute("factorization", s)
         assert sds.getparam(s, "info") == 0
         assert np.array_equal(numeric_factor(s), ref), f"tile shape {c} changes the factor"
+        X = torch.zeros_like(B)
+        sds.execute("solve", s, X, B)
+        assert torch.equal(X, Xref), f"tile shape {c} changes the solve"
 
 
 def test_autotune_keeps_the_factor():
