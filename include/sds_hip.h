@@ -194,12 +194,14 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  * chunk of the contribution-block update: 16, 32 or 64, the number of trailing-update tiles of a
  * step from which they get a launch of their own: never, 256, 512, 1024, the number of
  * workgroups from which a step launch uses the build for two workgroups per CU: never, 1, 64,
- * 128, 256, 512, and the small-front kernels: 0 = fronts of at most 32 rows one wave each and
- * the others in blocked panels, 1 = one wave each / column by column, 2 = all column by column),
- * keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
- * are not kept: factorize again). tiles (may be NULL, else 5 entries) receives {extend-add
- * rows, K chunk, trailing split, two-per-CU steps, small-front kernels}. Every candidate gives
- * the same factor bit for bit. Synchronous; needs sds_alloc_numeric first.
+ * 128, 256, 512, the small-front kernels: 0 = fronts of at most 32 rows one wave each and
+ * the others in blocked panels, 1 = one wave each / column by column, 2 = all column by column,
+ * and the LDS budget of the leaf-subtree kernel (regime A: one workgroup walks a leaf subtree with
+ * its fronts and contribution blocks in LDS, src/numeric/subtree.jl:274-327): 0 = none, 16, 32 or
+ * 64 KiB), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
+ * are not kept: factorize again). tiles (may be NULL, else 6 entries) receives {extend-add
+ * rows, K chunk, trailing split, two-per-CU steps, small-front kernels, subtree KiB}. Every
+ * candidate gives the same factor bit for bit. Synchronous; needs sds_alloc_numeric first.
  */
 int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
                  int32_t* tiles);
@@ -207,7 +209,8 @@ int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_para
 /*
  * Set the tile shapes of the front kernels directly, as sds_autotune reports them: tiles =
  * {extend-add rows (512, 1024, 2048 or 4096), K chunk (16, 32 or 64), trailing split (0 = never,
- * else >= 1), two-per-CU steps (0 = never, else >= 1), small-front kernels (0, 1 or 2)}. No
+ * else >= 1), two-per-CU steps (0 = never, else >= 1), small-front kernels (0, 1 or 2), subtree
+ * KiB (0, 16, 32 or 64)}. No
  * counterpart in the reference (whose kernels have fixed shapes); it restores a saved tuning
  * without timing again. Takes effect at the next sds_alloc_numeric (the launch plan depends on it).
  */
@@ -291,6 +294,7 @@ enum {
     SDS_K_RESIDUAL,        /* refinement residual (FP64 SpMV) */
     SDS_K_PIVOT_PANEL,     /* large fronts: pivoted panel factorization (LDL^T, LU) */
     SDS_K_INVERSE,         /* Cholesky: inverses of the diagonal blocks (for the sweeps) */
+    SDS_K_SUBTREE,         /* Cholesky regime A: leaf subtrees, one workgroup each */
     SDS_K_COUNT
 };
 int sds_profile(sds_handle_t handle, int32_t enable);

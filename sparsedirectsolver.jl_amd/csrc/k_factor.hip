@@ -132,6 +132,92 @@ __global__ void __launch_bounds__(256) k_front_col_chol(DevSym S, const int32_t*
     if (tid == 0) status[s] = st;
 }
 
+// Regime A: one workgroup per leaf subtree, walking its fronts in postorder (sub_nodes) with the
+// multifrontal stack in LDS (src/numeric/subtree.jl:274-316 is the reference's form over packed
+// triangles; here every front is a dense f x f square and every contribution block an m x m
+// square). A front at lfront[v] sits just above its children's blocks: zeroed, A added (one
+// thread per run of entries with one destination, in nzval order), the children's blocks added
+// in child_list order (the order of assemble_dense, so the front is bitwise the same as in the
+// level kernels), factored by front_chol_lds; its panel and diagonal-block inverse go to global
+// memory. The subtree root's block goes to the contribution-block pool for its parent; any other
+// front's block moves down to lcb[v], where its first child's block began: every destination lies
+// below its source and the gap grows along the copy, so a chunk read whole before it is written
+// never overwrites a source still to be read.
+template <typename T>
+__global__ void __launch_bounds__(256) k_subtree_chol(DevSym S, const int32_t* __restrict__ sub_ptr,
+                                                     const int32_t* __restrict__ sub_nodes, int sub0,
+                                                     const int32_t* __restrict__ lfront, const int32_t* __restrict__ lcb,
+                                                     const double* __restrict__ nz, T* __restrict__ factor,
+                                                     T* __restrict__ cbpool, T* __restrict__ DB,
+                                                     int32_t* __restrict__ status, double eps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* stack = reinterpret_cast<T*>(smem_raw);
+    __shared__ int sh;
+    const int t = sub0 + blockIdx.x;
+    const int n0 = sub_ptr[t], n1 = sub_ptr[t + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int kn = n0; kn < n1; ++kn) {
+        const int v = sub_nodes[kn];
+        const int f = S.fnrows[v], w = S.fncols[v], m = f - w;
+        T* F = stack + lfront[v];
+        for (int q = tid; q < f * f; q += 256) F[q] = T(0);
+        __syncthreads();
+        const int64_t a0 = S.asm_ptr[v], a1 = S.asm_ptr[v + 1];
+        for (int64_t k = a0 + tid; k < a1; k += 256) {
+            const int64_t d = S.asm_dst[k];
+            if (k > a0 && S.asm_dst[k - 1] == d) continue;
+            double val = 0.0;
+            for (int64_t q = k; q < a1 && S.asm_dst[q] == d; ++q) val += nz[S.asm_src[q]];
+            F[d >= 0 ? d : -(d + 1)] += T(val);
+        }
+        __syncthreads();
+        for (int k = S.child_ptr[v]; k < S.child_ptr[v + 1]; ++k) {
+            const int c = S.child_list[k];
+            const int mc = S.fnrows[c] - S.fncols[c];
+            const T* C = stack + lcb[c];
+            const int32_t* ri = S.relind + S.relind_ptr[c];
+            for (int q = tid; q < mc * mc; q += 256) {
+                const int jj = q / mc, ii = q - jj * mc;
+                if (ii >= jj) F[ri[jj] * f + ri[ii]] += C[q];
+            }
+            __syncthreads();
+        }
+        const int st = front_chol_lds<T, 2>(F, f, w, eps, &sh);
+        T* P = factor + S.panel_off[v];
+        for (int j = wv; j < w; j += 4)
+            for (int i = lane; i < f; i += 64) P[(int64_t)j * f + i] = i >= j ? F[j * f + i] : T(0);
+        if (w <= FUSED_INV_MAX) diag_inverse_lds<T>(F, f, w, DB + (int64_t)S.super_ptr[v] * NB);
+        if (kn == n1 - 1) {
+            const int64_t co = S.cb_off[v];
+            if (co >= 0) {
+                T* C = cbpool + co;
+                for (int jj = wv; jj < m; jj += 4)
+                    for (int ii = jj + lane; ii < m; ii += 64) C[(int64_t)jj * m + ii] = F[(w + jj) * f + w + ii];
+            }
+        } else {
+            T* D = stack + lcb[v];
+            for (int q0 = 0; q0 < m * m; q0 += 1024) {
+                T val[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = q0 + tid + 256 * u;
+                    const int jj = q / max(m, 1), ii = q - jj * m;
+                    val[u] = q < m * m ? F[(w + jj) * f + w + ii] : T(0);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = q0 + tid + 256 * u;
+                    if (q < m * m) D[q] = val[u];
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) status[v] = st;
+        __syncthreads();
+    }
+}
+
 // fronts of at most TINY_F rows: one wave per front, four per workgroup (no block barriers). The
 // wave assembles the front in its own LDS square (zero, A, then the children in child_list
 // order, as assemble_dense), then each lane takes one row into registers and the wave factors
@@ -985,6 +1071,14 @@ void launch_front_small_chol(hipStream_t st, const DevSym& S, const int32_t* fro
 }
 
 template <typename T>
+void launch_subtree_chol(hipStream_t st, const DevSym& S, const int32_t* sub_ptr, const int32_t* sub_nodes, int first,
+                         int count, int lds, const int32_t* lfront, const int32_t* lcb, const double* nz, T* factor,
+                         T* cbpool, T* DB, int32_t* status, double eps) {
+    hipLaunchKernelGGL(k_subtree_chol<T>, dim3(count), dim3(256), (size_t)lds, st, S, sub_ptr, sub_nodes, first,
+                       lfront, lcb, nz, factor, cbpool, DB, status, eps);
+}
+
+template <typename T>
 void launch_front_col_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
                            const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps) {
     size_t lds = (size_t)fmax * fmax * sizeof(T);
@@ -1059,6 +1153,9 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
 #define INST(T)                                                                                                   \
     template void launch_front_small_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, \
                                              T*, T*, T*, int32_t*, double);                                       \
+    template void launch_subtree_chol<T>(hipStream_t, const DevSym&, const int32_t*, const int32_t*, int, int, int, \
+                                         const int32_t*, const int32_t*, const double*, T*, T*, T*, int32_t*,     \
+                                         double);                                                                  \
     template void launch_front_col_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, T*, \
                                            T*, T*, int32_t*, double);                                             \
     template void launch_front_tiny_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, const double*, T*, T*, \

@@ -87,6 +87,7 @@ struct Level {
     std::vector<int32_t> ssmall;        // fronts the solve handles one workgroup each
     int fmax_small = 0;
     int64_t off_small = 0, off_all = 0, off_ssmall = 0;  // offsets into the device front lists
+    int nsub = 0;  // fronts factored by the subtree kernel (Cholesky regime A)
     // Cholesky: the small fronts of at most TINY_F rows (one wave each) and the others
     std::vector<int32_t> tiny, mid;
     int64_t off_tiny = 0, off_mid = 0;
@@ -193,6 +194,18 @@ struct sds_handle {
     int64_t* d_cb_off = nullptr;
     int64_t* d_uv_off = nullptr;
     bool placed = false;                      // cb_place / uv_place hold this analysis' placements
+    int place_key = 0;                        // ... for this element type and subtree budget
+    // regime A (Cholesky): subtree budget (KiB of LDS, 0 = off), subtrees (node ranges of
+    // sub_nodes, in postorder), per supernode the stack offsets of its front and of its
+    // contribution block, and one launch per LDS class {first subtree, count, LDS bytes}
+    int sub_kib = 0;
+    std::vector<int32_t> sub_ptr, sub_nodes, sub_lfront, sub_lcb;
+    struct SubLaunch {
+        int first, count, lds;
+        double flops, bytes;  // algorithmic work: the fronts' flops; panels written, A read, roots' blocks
+    };
+    std::vector<SubLaunch> sub_launch;
+    int32_t *d_sub_ptr = nullptr, *d_sub_nodes = nullptr, *d_sub_lfront = nullptr, *d_sub_lcb = nullptr;
     std::vector<int64_t> cb_place, uv_place;  // element offsets of the contribution blocks / update vectors
     void* factor = nullptr;
     void* ufactor = nullptr;  // U^T panels of "G"
@@ -451,10 +464,90 @@ int build_schedule(sds_handle* h, int dtype) {
     h->levels.assign(nlev, Level{});
     h->max_rest = 0;
     std::vector<char> big(ns, 0);
+    // regime A (Cholesky, sub_kib > 0): leaf subtrees of small fronts whose LDS stack fits the
+    // budget, one workgroup each, walked in postorder before the levels. A front sits above its
+    // children's contribution blocks (each an m x m square), which it consumes; its own block then
+    // moves down to where its first child's block began. peak(v) = max(max_i(base_i + peak(c_i)),
+    // sum_c m_c^2 + f^2) with base_i the blocks of the earlier children.
+    std::vector<int32_t> in_sub(ns, -1);
+    h->sub_ptr.assign(1, 0);
+    h->sub_nodes.clear();
+    h->sub_lfront.assign(ns, 0);
+    h->sub_lcb.assign(ns, 0);
+    h->sub_launch.clear();
+    if (h->method == M_CHOLESKY && h->sub_kib > 0) {
+        // elements of the stack: the budget less the kernel's static words (at most 64 KiB in all)
+        const int64_t cap = ((int64_t)h->sub_kib * 1024 - 256) / (int64_t)es;
+        std::vector<int64_t> peak(ns, 0);
+        std::vector<char> ok(ns, 0);
+        for (int64_t s = 0; s < ns; ++s) {
+            const int64_t f = h->fnrows[s];
+            bool good = f <= fsmall;
+            int64_t base = 0, pk = 0;
+            for (int k = h->child_ptr[s]; k < h->child_ptr[s + 1]; ++k) {
+                const int32_t c = h->child_list[k];
+                const int64_t mc = h->fnrows[c] - h->fncols[c];
+                good = good && ok[c];
+                pk = std::max(pk, base + peak[c]);
+                base += mc * mc;
+            }
+            pk = std::max(pk, base + f * f);
+            peak[s] = pk;
+            ok[s] = good && pk <= cap;
+        }
+        // subtree roots (at least two fronts: a lone front gains nothing), their postorder and the
+        // stack offsets, grouped by the LDS their peak needs (16, 32 or 64 KiB classes)
+        std::vector<std::vector<int32_t>> byclass(3);
+        for (int64_t s = 0; s < ns; ++s) {
+            const int32_t p = h->snparent[s];
+            if (!ok[s] || (p >= 0 && ok[p]) || h->child_ptr[s + 1] == h->child_ptr[s]) continue;
+            const int64_t bytes = peak[s] * (int64_t)es + 256;
+            byclass[bytes <= 16384 ? 0 : bytes <= 32768 ? 1 : 2].push_back((int32_t)s);
+        }
+        for (int cls = 0; cls < 3; ++cls) {
+            int64_t pmax = 0;
+            const int first_sub = (int)h->sub_ptr.size() - 1;
+            for (int32_t root : byclass[cls]) {
+                pmax = std::max(pmax, peak[root]);
+                // postorder with children in child_list order (iterative), offsets on the way down
+                std::vector<std::pair<int32_t, int>> stk{{root, 0}};
+                h->sub_lcb[root] = 0;
+                while (!stk.empty()) {
+                    auto& [v, ci] = stk.back();
+                    const int nc = h->child_ptr[v + 1] - h->child_ptr[v];
+                    if (ci == 0) {
+                        int64_t b = h->sub_lcb[v];
+                        for (int k = h->child_ptr[v]; k < h->child_ptr[v + 1]; ++k) {
+                            const int32_t c = h->child_list[k];
+                            const int64_t mc = h->fnrows[c] - h->fncols[c];
+                            h->sub_lcb[c] = (int32_t)b;
+                            b += mc * mc;
+                        }
+                        h->sub_lfront[v] = (int32_t)b;
+                    }
+                    if (ci < nc) {
+                        const int32_t c = h->child_list[h->child_ptr[v] + ci];
+                        ++ci;
+                        stk.push_back({c, 0});
+                    } else {
+                        h->sub_nodes.push_back(v);
+                        in_sub[v] = (int32_t)h->sub_ptr.size() - 1;
+                        stk.pop_back();
+                    }
+                }
+                h->sub_ptr.push_back((int32_t)h->sub_nodes.size());
+            }
+            const int cnt = (int)h->sub_ptr.size() - 1 - first_sub;
+            if (cnt) h->sub_launch.push_back({first_sub, cnt, (int)(pmax * (int64_t)es), 0.0, 0.0});
+        }
+    }
     for (int64_t s = 0; s < ns; ++s) {
         Level& L = h->levels[lev[s]];
         L.all.push_back((int32_t)s);
-        if (h->fnrows[s] <= fsmall) {
+        if (in_sub[s] >= 0) {
+            // factored by its subtree's workgroup; still a small front for the sweeps
+            L.nsub += 1;
+        } else if (h->fnrows[s] <= fsmall) {
             L.small.push_back((int32_t)s);
             L.fmax_small = std::max(L.fmax_small, h->fnrows[s]);
             if (h->fnrows[s] <= TINY_F) {
@@ -478,12 +571,22 @@ int build_schedule(sds_handle* h, int dtype) {
         last[s] = h->snparent[s] >= 0 ? lev[h->snparent[s]] : lev[s];
         if (m > 0 && (h->snparent[s] >= 0 || big[s])) cblen[s] = m * m;
         if (m > 0 && h->snparent[s] >= 0) uvlen[s] = m;
+        if (in_sub[s] >= 0) {
+            // inside a subtree the blocks stay in LDS; a subtree root's block is written before the
+            // levels begin
+            const int32_t p = h->snparent[s];
+            if (p >= 0 && in_sub[p] == in_sub[s]) cblen[s] = 0;
+            else first[s] = 0;
+        }
     }
-    // the placements depend on the analysis only (element offsets): computed once per upload
-    if (!h->placed) {
+    // the placements depend on the analysis, the big/small split (element type) and the subtree
+    // budget: computed again only when one of them changes
+    const int place_key = (int)dtype * 1000 + h->sub_kib;
+    if (!h->placed || h->place_key != place_key) {
         h->cb_len = place_blocks(cblen, first, last, nlev, h->cb_place);
         h->uv_len = place_blocks(uvlen, first, last, nlev, h->uv_place);
         h->placed = true;
+        h->place_key = place_key;
     }
     const std::vector<int64_t>& cb_off = h->cb_place;
     const std::vector<int64_t>& uv_off = h->uv_place;
@@ -527,6 +630,15 @@ int build_schedule(sds_handle* h, int dtype) {
         }
         return b;
     };
+    for (auto& sl : h->sub_launch)
+        for (int t = sl.first; t < sl.first + sl.count; ++t)
+            for (int k = h->sub_ptr[t]; k < h->sub_ptr[t + 1]; ++k) {
+                const int32_t v = h->sub_nodes[k];
+                const double f = h->fnrows[v], w = h->fncols[v], m = f - w;
+                sl.flops += front_flops(v);
+                sl.bytes += dbl_es * (f * w - w * (w - 1) / 2) + 16.0 * (double)h->asm_count[v];
+                if (k == h->sub_ptr[t + 1] - 1) sl.bytes += dbl_es * m * (m + 1) / 2;
+            }
     for (auto& L : h->levels) {
         for (int32_t s : L.small) {
             L.small_flops += front_flops(s);
@@ -951,6 +1063,12 @@ int build_schedule(sds_handle* h, int dtype) {
     }
     if ((rc = upload(h, h->numbufs, flist, &h->d_fronts))) return rc;
     if ((rc = upload(h, h->numbufs, tasks, &h->d_tasks))) return rc;
+    if (!h->sub_launch.empty()) {
+        if ((rc = upload(h, h->numbufs, h->sub_ptr, &h->d_sub_ptr))) return rc;
+        if ((rc = upload(h, h->numbufs, h->sub_nodes, &h->d_sub_nodes))) return rc;
+        if ((rc = upload(h, h->numbufs, h->sub_lfront, &h->d_sub_lfront))) return rc;
+        if ((rc = upload(h, h->numbufs, h->sub_lcb, &h->d_sub_lcb))) return rc;
+    }
     if ((rc = upload(h, h->numbufs, cb_off, &h->d_cb_off))) return rc;
     if ((rc = upload(h, h->numbufs, uv_off, &h->d_uv_off))) return rc;
     h->dsym.cb_off = h->d_cb_off;
@@ -965,6 +1083,13 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
     T* factor = static_cast<T*>(h->factor);
     T* cb = static_cast<T*>(h->cbpool);
     launch_fill_i32(st, h->status, h->ns, 0);
+    // regime A: the leaf subtrees, one workgroup each, one launch per LDS class
+    for (const auto& sl : h->sub_launch)
+        timed(h, SDS_K_SUBTREE, sl.flops, sl.bytes, [&] {
+            launch_subtree_chol<T>(st, h->dsym, h->d_sub_ptr, h->d_sub_nodes, sl.first, sl.count, sl.lds,
+                                   h->d_sub_lfront, h->d_sub_lcb, nz, factor, cb, static_cast<T*>(h->dbuf), h->status,
+                                   eps);
+        });
     for (auto& L : h->levels) {
         if (interrupt && *interrupt) return fail(h, SDS_INTERRUPTED, "factorization interrupted");
         if (!L.small.empty()) {
@@ -1885,20 +2010,21 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     const int dtype = h->dtype;
     // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
     struct Cand {
-        int ea, kc, rs, oc, sk;
+        int ea, kc, rs, oc, sk, sb;
     };
     std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
         for (int kc : {16, 32, 64})
-            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0, 0});
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0, 0, 0});
     if (h->method == M_CHOLESKY) {  // with the best of the above
         for (int rs : {256, 512, 1024})
-            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0, 0});
-        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc, 0});
-        for (int sk : {1, 2}) cand.push_back({-1, -1, -1, -1, sk});
+            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0, 0, 0});
+        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc, 0, 0});
+        for (int sk : {1, 2}) cand.push_back({-1, -1, -1, -1, sk, 0});
+        for (int sb : {16, 32, 64}) cand.push_back({-1, -1, -1, -1, -1, sb});
     }
     double best = 1e300;
-    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2, h->small_kind};
+    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2, h->small_kind, h->sub_kib};
     int64_t info = 0;
     int rc;
     for (Cand c : cand) {
@@ -1908,11 +2034,13 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
         }
         if (c.rs < 0) c.rs = pick.rs;
         if (c.oc < 0) c.oc = pick.oc;
+        if (c.sk < 0) c.sk = pick.sk;
         h->ea_rows = c.ea;
         h->upd_kc = c.kc;
         h->rest_split = c.rs;
         h->step_occ2 = c.oc;
         h->small_kind = c.sk;
+        h->sub_kib = c.sb;
         if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
         if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
@@ -1929,6 +2057,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     h->rest_split = pick.rs;
     h->step_occ2 = pick.oc;
     h->small_kind = pick.sk;
+    h->sub_kib = pick.sb;
     if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
     if (tiles) {
         tiles[0] = pick.ea;
@@ -1936,6 +2065,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
         tiles[2] = pick.rs;
         tiles[3] = pick.oc;
         tiles[4] = pick.sk;
+        tiles[5] = pick.sb;
     }
     return SDS_SUCCESS;
 }
@@ -1943,17 +2073,20 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
 int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     if (!h) return SDS_INVALID_VALUE;
     if (!tiles) return fail(h, SDS_INVALID_VALUE, "tiles is NULL");
-    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4];
+    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4], sb = tiles[5];
     if (ea != 512 && ea != 1024 && ea != 2048 && ea != 4096)
         return fail(h, SDS_INVALID_VALUE, "extend-add rows must be 512, 1024, 2048 or 4096");
     if (kc != 16 && kc != 32 && kc != 64) return fail(h, SDS_INVALID_VALUE, "the K chunk must be 16, 32 or 64");
     if (rs < 0 || oc < 0) return fail(h, SDS_INVALID_VALUE, "the trailing split and two-per-CU steps must be >= 0");
     if (sk < 0 || sk > 2) return fail(h, SDS_INVALID_VALUE, "the small-front kernel must be 0, 1 or 2");
+    if (sb != 0 && sb != 16 && sb != 32 && sb != 64)
+        return fail(h, SDS_INVALID_VALUE, "the subtree budget must be 0, 16, 32 or 64 (KiB)");
     h->ea_rows = ea;
     h->upd_kc = kc;
     h->rest_split = rs;
     h->step_occ2 = oc;
     h->small_kind = sk;
+    h->sub_kib = sb;
     return SDS_SUCCESS;
 }
 

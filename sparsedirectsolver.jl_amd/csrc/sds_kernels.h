@@ -5,6 +5,10 @@
 namespace sds {
 
 template <typename T>
+void launch_subtree_chol(hipStream_t st, const DevSym& S, const int32_t* sub_ptr, const int32_t* sub_nodes, int first,
+                         int count, int lds, const int32_t* lfront, const int32_t* lcb, const double* nz, T* factor,
+                         T* cbpool, T* DB, int32_t* status, double eps);
+template <typename T>
 void launch_front_col_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
                            const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps);
 template <typename T>

@@ -13,7 +13,7 @@ EXPORTS = ("sds_create", "sds_destroy", "sds_last_error", "sds_set_stream", "sds
 
 # profiling kinds (SDS_K_* of include/sds_hip.h)
 PROFILE_KINDS = ("front_small", "zero_scatter", "extend_add", "potrf", "trsm", "update_panel", "update_cb", "stats",
-                 "permute", "fwd", "bwd", "diag", "residual", "pivot_panel", "inverse")
+                 "permute", "fwd", "bwd", "diag", "residual", "pivot_panel", "inverse", "subtree")
 
 SDS_SUCCESS, SDS_INVALID_VALUE, SDS_NOT_SUPPORTED, SDS_FACTORIZATION_ERROR, SDS_DEVICE_ERROR, SDS_INTERRUPTED = range(6)
 SDS_R64, SDS_R32 = 0, 1

@@ -11,7 +11,7 @@ import pytest
 import oracle
 import sparsedirectsolver as sds
 from sparsedirectsolver import matrices as M
-from sparsedirectsolver.solver import numeric_factor, numeric_stats
+from sparsedirectsolver.solver import numeric_factor, numeric_stats, profile, profile_read
 
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
@@ -174,6 +174,7 @@ def test_every_tile_candidate_gives_the_same_factor():
     cands += [dict(base, step_two_per_cu=oc) for oc in (1, 64, 128, 256, 512)]
     cands += [dict(base, trailing_split=256, step_two_per_cu=1)]
     cands += [dict(base, small_front_kernel=k) for k in (1, 2)]
+    cands += [dict(base, subtree_kib=b) for b in (16, 32, 64)]
     for c in cands:
         print("tiles", c, flush=True)
         sds.set_tiles(s, c)
@@ -183,6 +184,50 @@ def test_every_tile_candidate_gives_the_same_factor():
         X = torch.zeros_like(B)
         sds.execute("solve", s, X, B)
         assert torch.equal(X, Xref), f"tile shape {c} changes the solve"
+
+
+SUBTREE_CASES = [
+    # the reference's CPU-runnable configuration: 2-D 100 x 100, the build's AMD ordering
+    ("lap2d_100x100_amd", lambda: (M.laplacian2d(100, 100), None, (32, 0.25, 8))),
+    ("varlap3d_14_nd", lambda: (M.variable_laplacian((14, 13, 12), seed=3), sds.grid_nd_perm(14, 13, 12),
+                                (128, 0.25, 8))),
+    ("lap3d_20_natural", lambda: (M.laplacian3d(20, 20, 20), None, (64, 0.25, 8))),
+]
+
+
+@pytest.mark.parametrize("fp", [None, "Float32"])
+@pytest.mark.parametrize("name,make", SUBTREE_CASES, ids=[c[0] for c in SUBTREE_CASES])
+def test_subtree_kernel_matches_oracle(name, make, fp):
+    """Regime A (leaf subtrees, one workgroup each, the stack in LDS) against the CPU oracle: the
+    factor within 100 eps (FP64; FP32 within 1e-5 relative of the oracle's FP64 factor), the
+    statistics equal, the factor and the solve bitwise those of the level kernels, and the
+    subtree launches actually taking the subtrees (fewer level fronts)."""
+    import torch
+    A, perm, amal = make()
+    s, nz = _solver(A, perm, amal, fp=fp)
+    tiles = dict(extend_add_rows=1024, cb_k_chunk=32, trailing_split=0, step_two_per_cu=0, small_front_kernel=0)
+    n = A.shape[0]
+    B = torch.tensor(np.random.default_rng(8).standard_normal((n, 2)), dtype=torch.float64, device="cuda")
+    out = {}
+    for kib in (0, 16, 64):
+        sds.set_tiles(s, dict(tiles, subtree_kib=kib))
+        profile(s, True)
+        sds.execute("factorization", s)
+        prof = profile_read(s)
+        profile(s, False)
+        assert sds.getparam(s, "info") == 0
+        X = torch.zeros_like(B)
+        sds.execute("solve", s, X, B)
+        out[kib] = (numeric_factor(s), numeric_stats(s)[0], X.cpu().numpy(), prof.get("subtree", (0, 0))[1])
+        print(f"{name} fp={fp} subtree {kib} KiB: {out[kib][3]} subtree launches", flush=True)
+    assert out[0][3] == 0 and out[64][3] >= 1
+    info, ref, st_ref = oracle.cholesky(s.symbolic, nz)
+    for kib in (16, 64):
+        got, st, X, _ = out[kib]
+        assert np.array_equal(got, out[0][0]) and np.array_equal(st, st_ref)
+        assert np.array_equal(X, out[0][2])
+        err = np.abs(got.astype(np.float64) - ref).max() / np.abs(ref).max()
+        assert err <= (100 * EPS if fp is None else 1e-5), err
 
 
 def test_autotune_keeps_the_factor():
