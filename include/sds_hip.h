@@ -152,6 +152,33 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
               int64_t ldx, int32_t nrhs, const double* nzval, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps);
 
 /*
+ * Schur complement mode (`schur_mode = 1` with `user_schur_indices`, src/numeric/schur.jl:1-10,
+ * src/solver.jl:400-416): after sds_upload_symbolic of an analysis that ordered the Schur
+ * indices last as one dense root supernode of nschur columns, sds_set_schur(h, nschur) makes that
+ * root the Schur block (0 turns the mode off); call sds_alloc_numeric afterwards. sds_factorize
+ * then factors every other front and only assembles the root (`assemble_schur!`,
+ * src/numeric/schur.jl:52-78): its panel holds S = A22 - A21 A11^-1 A12, its columns get the
+ * identity pivot order, unit 1x1 pivots and zero statistics. The solve phases read B and write X
+ * in factor order unless they permute (the last nschur entries are the Schur block, in increasing
+ * original order): SDS_PHASE_SOLVE_FWD_SCHUR leaves the condensed right-hand side
+ * b2 - A21 A11^-1 b1 there, SDS_PHASE_SOLVE_DIAG applies D^-1 to the factored part, and
+ * SDS_PHASE_SOLVE_BWD_SCHUR, with the solution x2 of S x2 = b2' in those entries of B, gives the
+ * solution of A x = b. A solve phase with both sweeps, or with refinement, is SDS_NOT_SUPPORTED.
+ */
+#define SDS_PHASE_SOLVE_FWD_SCHUR 48   /* fwd_perm | fwd */
+#define SDS_PHASE_SOLVE_BWD_SCHUR 384  /* bwd | bwd_perm */
+int sds_set_schur(sds_handle_t handle, int64_t nschur);
+
+/*
+ * The Schur complement S of the last factorization (Schur complement mode) into the device
+ * buffer S: nschur x nschur, column-major, leading dimension lds, Float64 (the full matrix: the
+ * symmetric structures mirror the lower triangle, "G" assembles the whole square).
+ * Replaces `schur_matrix!` with a dense destination (src/numeric/schur.jl:112-150).
+ * Asynchronous on the handle's stream.
+ */
+int sds_schur_export(sds_handle_t handle, double* S, int64_t lds);
+
+/*
  * sds_solve without device allocation. No counterpart in the reference: it is for callers that
  * cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid
  * allocation). Same arguments, kernels and results as sds_solve (bitwise), but the solve

@@ -47,6 +47,7 @@ struct Analysis {
     i64 n = 0, nnz = 0, ns = 0;
     int structure = 0, view = 0;
     i64 nnz_stored = 0, nnz_L = 0;
+    i64 nschur = 0;  // Schur complement mode: the last supernode's columns (0 = off)
     double flops = 0.0, flops_true = 0.0;
     std::map<std::string, V> f;  // named 1-based integer arrays
     std::string err;
@@ -233,7 +234,9 @@ V relabel_parent(const V& parent, const V& order) {
 
 i64 trapezoid(i64 w, i64 f) { return w * f - (w * (w - 1)) / 2; }
 
-ColumnPartition fundamental_supernodes(const V& parent, const V& post, const V& counts) {
+// nschur > 0: the last nschur columns of the order (the Schur complement's) form one supernode of
+// their own, and no other supernode reaches into them
+ColumnPartition fundamental_supernodes(const V& parent, const V& post, const V& counts, i64 nschur = 0) {
     i64 n = (i64)parent.size();
     V par = relabel_parent(parent, post);
     V cnt(n);
@@ -246,6 +249,7 @@ ColumnPartition fundamental_supernodes(const V& parent, const V& post, const V& 
     cp.super_ptr.push_back(0);
     for (i64 k = 1; k < n; ++k) {
         bool joins = par[k - 1] == k && nchild[k] == 1 && cnt[k - 1] == cnt[k] + 1;
+        if (nschur > 0 && k >= n - nschur) joins = k > n - nschur;
         if (!joins) cp.super_ptr.push_back(k);
     }
     if (n > 0) cp.super_ptr.push_back(n);
@@ -259,8 +263,9 @@ ColumnPartition fundamental_supernodes(const V& parent, const V& post, const V& 
     return cp;
 }
 
+// schur_root: the last supernode is the Schur complement's, which takes no children's columns
 ColumnPartition amalgamate(const ColumnPartition& sn, const V& parent, const V& counts, i64 max_width,
-                           double zero_fraction, i64 min_width) {
+                           double zero_fraction, i64 min_width, bool schur_root = false) {
     if (max_width < 1) throw std::runtime_error("amalgamate: max_width must be >= 1");
     i64 n = (i64)sn.order.size();
     V cnt(n);
@@ -293,6 +298,7 @@ ColumnPartition amalgamate(const ColumnPartition& sn, const V& parent, const V& 
     for (i64 u = 0; u < nu; ++u)
         if (uparent[u] >= 0) children[uparent[u]].push_back(u);
     for (i64 p = 0; p < nu; ++p) {
+        if (schur_root && p == nu - 1) continue;
         V& kids = children[p];
         // most shared rows first, then narrow first, then by index
         std::sort(kids.begin(), kids.end(), [&](i64 a, i64 b) {
@@ -572,7 +578,7 @@ V amd_order(const Pattern& P) {
 
 Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structure, int view, int index_base,
                   const i64* perm_in, int ordering, i64 max_width, double zero_fraction, i64 min_width,
-                  int use_superpanels) {
+                  int use_superpanels, const int8_t* schur) {
     auto A = new Analysis();
     A->n = n;
     A->structure = structure;
@@ -606,14 +612,43 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
     } else {
         std::iota(perm0.begin(), perm0.end(), 0);
     }
+    // Schur complement mode (src/numeric/schur.jl:1-10): the flagged rows and columns go last, in
+    // increasing original order, after the others in the ordering's order
+    i64 nschur = 0;
+    if (schur) {
+        V rest, last;
+        for (i64 k = 0; k < n; ++k)
+            if (!schur[perm0[k]]) rest.push_back(perm0[k]);
+        for (i64 i = 0; i < n; ++i)
+            if (schur[i]) last.push_back(i);
+        nschur = (i64)last.size();
+        if (nschur == 0) throw std::runtime_error("schur_mode = 1 needs user_schur_indices with at least one flag");
+        rest.insert(rest.end(), last.begin(), last.end());
+        perm0 = rest;
+    }
+    A->nschur = nschur;
     V iperm0(n);
     for (i64 k = 0; k < n; ++k) iperm0[perm0[k]] = k;
 
     V parent = etree(P, perm0, iperm0);
     V post = postorder(parent);
     V counts = colcounts(P, perm0, iperm0, parent, post);
-    ColumnPartition cp = fundamental_supernodes(parent, post, counts);
-    if (use_superpanels) cp = amalgamate(cp, parent, counts, max_width, zero_fraction, min_width);
+    if (nschur > 0) {
+        // the Schur columns (etree numbering n - nschur .. n - 1) after every other column, in that
+        // numbering, with the counts of a dense lower triangle: children before parents still holds
+        // (a parent's etree number is larger than its child's)
+        V p2;
+        p2.reserve(n);
+        for (i64 k = 0; k < n; ++k)
+            if (post[k] < n - nschur) p2.push_back(post[k]);
+        for (i64 j = n - nschur; j < n; ++j) {
+            p2.push_back(j);
+            counts[j] = n - j;
+        }
+        post = p2;
+    }
+    ColumnPartition cp = fundamental_supernodes(parent, post, counts, nschur);
+    if (use_superpanels) cp = amalgamate(cp, parent, counts, max_width, zero_fraction, min_width, nschur > 0);
 
     // supernodal symbolic factorization
     V fperm(n), fiperm(n);
@@ -835,12 +870,14 @@ extern "C" {
 // Run the analysis. ordering: 0 natural, 1 user permutation (perm must be
 // given), 3 approximate minimum degree. Returns NULL on error
 // (sdsa_last_error() holds the message).
+// schur: NULL, or n flags (original rows) of the Schur complement mode, whose rows and columns go
+// last and form the last supernode.
 void* sdsa_analyze(int64_t n, const int64_t* rowptr, const int64_t* colval, int structure, int view,
                    int index_base, const int64_t* perm, int ordering, int64_t max_width, double zero_fraction,
-                   int64_t min_width, int use_superpanels) {
+                   int64_t min_width, int use_superpanels, const int8_t* schur) {
     try {
         return analyze(n, rowptr, colval, structure, view, index_base, perm, ordering, max_width, zero_fraction,
-                       min_width, use_superpanels);
+                       min_width, use_superpanels, schur);
     } catch (const std::exception& e) {
         g_err = e.what();
         return nullptr;
@@ -866,7 +903,8 @@ int sdsa_field_copy(void* h, const char* name, int64_t* dst) {
     return 0;
 }
 
-// scalars: 0 n, 1 nnz, 2 ns, 3 nnz_stored, 4 nnz_L, 5 stored-panel flops, 6 true flops (sum c_j^2)
+// scalars: 0 n, 1 nnz, 2 ns, 3 nnz_stored, 4 nnz_L, 5 stored-panel flops, 6 true flops (sum c_j^2),
+// 7 Schur complement size (0 = no Schur mode)
 double sdsa_scalar(void* h, int which) {
     auto A = static_cast<Analysis*>(h);
     switch (which) {
@@ -877,6 +915,7 @@ double sdsa_scalar(void* h, int which) {
         case 4: return (double)A->nnz_L;
         case 5: return A->flops;
         case 6: return A->flops_true;
+        case 7: return (double)A->nschur;
     }
     return -1;
 }

@@ -1060,6 +1060,44 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
 }
 
 // ---------------------------------------------------------------------------
+// Schur complement mode: the root front s (the Schur block) is assembled, not factored
+
+// its factor columns: the identity local pivot order (global columns, as the pivoting kernels
+// leave piv), unit 1x1 pivots with d = 1, zero statistics (src/numeric/schur.jl:15-28)
+template <typename T>
+__global__ void k_schur_meta(DevSym S, int s, int32_t* __restrict__ piv, int8_t* __restrict__ kind,
+                             T* __restrict__ dvec, int64_t* __restrict__ stats) {
+    const int c0 = S.super_ptr[s], w = S.fncols[s];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < max(w, NF); q += gridDim.x * blockDim.x) {
+        if (q < w) {
+            if (piv) piv[c0 + q] = c0 + q;
+            if (kind) kind[c0 + q] = KIND_1X1;
+            if (dvec) {
+                dvec[c0 + q] = T(1);
+                dvec[S.n + c0 + q] = T(0);
+            }
+        }
+        if (q < NF) stats[(int64_t)s * NF + q] = 0;
+    }
+}
+
+// the Schur complement S (ns x ns, column-major, leading dimension lds, Float64) from the root's
+// panel: the symmetric structures assemble the lower triangle (the upper is mirrored); "G" (full:
+// a large LU front assembles its whole fully summed square in the L panel, U^T holds only the rows
+// past it, and the root has none) the whole square
+template <typename T>
+__global__ void k_schur_export(DevSym S, int s, const T* __restrict__ factor, int full, double* __restrict__ out,
+                               int64_t lds) {
+    const int ns = S.fncols[s];
+    const T* P = factor + S.panel_off[s];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < (int64_t)ns * ns;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q / ns), i = (int)(q - (int64_t)j * ns);
+        out[(int64_t)j * lds + i] = (double)(i >= j || full ? P[(int64_t)j * ns + i] : P[(int64_t)i * ns + j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 
 template <typename T>
@@ -1133,6 +1171,21 @@ void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n
         hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
 }
 
+template <typename T>
+void launch_schur_meta(hipStream_t st, const DevSym& S, int s, int w, int32_t* piv, int8_t* kind, T* dvec,
+                       int64_t* stats) {
+    hipLaunchKernelGGL(k_schur_meta<T>, dim3((std::max(w, NF) + 255) / 256), dim3(256), 0, st, S, s, piv, kind, dvec,
+                       stats);
+}
+
+template <typename T>
+void launch_schur_export(hipStream_t st, const DevSym& S, int s, int ns, const T* factor, int full, double* out,
+                         int64_t lds) {
+    const int64_t total = (int64_t)ns * ns;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_schur_export<T>, dim3(blocks), dim3(256), 0, st, S, s, factor, full, out, lds);
+}
+
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info) {
     // info[8 ..] holds the block minima (INFO_BLOCKS entries)
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(INFO_BLOCKS, (S.ns + 255) / 256));
@@ -1156,6 +1209,8 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
     template void launch_subtree_chol<T>(hipStream_t, const DevSym&, const int32_t*, const int32_t*, int, int, int, \
                                          const int32_t*, const int32_t*, const double*, T*, T*, T*, int32_t*,     \
                                          double);                                                                  \
+    template void launch_schur_meta<T>(hipStream_t, const DevSym&, int, int, int32_t*, int8_t*, T*, int64_t*);    \
+    template void launch_schur_export<T>(hipStream_t, const DevSym&, int, int, const T*, int, double*, int64_t); \
     template void launch_front_col_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, int, const double*, T*, \
                                            T*, T*, int32_t*, double);                                             \
     template void launch_front_tiny_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, const double*, T*, T*, \

@@ -23,7 +23,28 @@ __global__ void k_permute_in(int n, int nrhs, const int32_t* __restrict__ perm, 
     const int64_t total = (int64_t)n * nrhs;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
-        Y[q] = T(B[perm[k] + r * ldb]);
+        Y[q] = T(B[(perm ? perm[k] : k) + r * ldb]);
+    }
+}
+
+// Schur complement mode, forward sweep: the Schur root is not solved; its rows of Y get their
+// children's update vectors (child_list order), leaving the condensed right-hand side
+// Y2 - L21 Y1 (src/solve/sweeps.jl:919-925). One workgroup; each row has one writer per child.
+template <typename T>
+__global__ void __launch_bounds__(256) k_schur_pull(DevSym S, int s, T* __restrict__ Y, const T* __restrict__ U,
+                                                   int nrhs) {
+    const int cg = S.super_ptr[s];
+    for (int k = S.child_ptr[s]; k < S.child_ptr[s + 1]; ++k) {
+        const int c = S.child_list[k];
+        if (S.uv_off[c] < 0) continue;
+        const int mc = S.fnrows[c] - S.fncols[c];
+        const T* uc = U + S.uv_off[c] * nrhs;
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        for (int q = threadIdx.x; q < mc * nrhs; q += 256) {
+            const int ii = q / nrhs, r = q - ii * nrhs;
+            Y[(int64_t)(cg + ri[ii]) * nrhs + r] -= uc[q];
+        }
+        __syncthreads();
     }
 }
 
@@ -34,7 +55,7 @@ __global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm,
     const int64_t total = (int64_t)n * nrhs;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
-        double* x = X + perm[k] + r * ldx;
+        double* x = X + (perm ? perm[k] : k) + r * ldx;
         *x = add ? *x + (double)Y[q] : (double)Y[q];
     }
 }
@@ -529,6 +550,11 @@ void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, con
 }
 
 template <typename T>
+void launch_schur_pull(hipStream_t st, const DevSym& S, int s, T* Y, const T* U, int nrhs) {
+    hipLaunchKernelGGL(k_schur_pull<T>, dim3(1), dim3(256), 0, st, S, s, Y, U, nrhs);
+}
+
+template <typename T>
 void launch_permute_out(hipStream_t st, int n, int nrhs, const int32_t* perm, const T* Y, double* X, int64_t ldx,
                         int add) {
     hipLaunchKernelGGL(k_permute_out<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, Y, X,
@@ -616,6 +642,7 @@ void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, con
 
 #define INST(T)                                                                                                     \
     template void launch_permute_in<T>(hipStream_t, int, int, const int32_t*, const double*, int64_t, T*);          \
+    template void launch_schur_pull<T>(hipStream_t, const DevSym&, int, T*, const T*, int);                        \
     template void launch_permute_out<T>(hipStream_t, int, int, const int32_t*, const T*, double*, int64_t, int);    \
     template void launch_fwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, T*, int, int,    \
                                 const int32_t*);                                                                    \

@@ -199,6 +199,10 @@ struct sds_handle {
     // sub_nodes, in postorder), per supernode the stack offsets of its front and of its
     // contribution block, and one launch per LDS class {first subtree, count, LDS bytes}
     int sub_kib = 0;
+    // Schur complement mode: the root supernode holding the Schur block (assembled, not factored),
+    // -1 = off, and its size
+    int32_t schur_s = -1;
+    int64_t nschur = 0;
     std::vector<int32_t> sub_ptr, sub_nodes, sub_lfront, sub_lcb;
     struct SubLaunch {
         int first, count, lds;
@@ -482,7 +486,7 @@ int build_schedule(sds_handle* h, int dtype) {
         std::vector<char> ok(ns, 0);
         for (int64_t s = 0; s < ns; ++s) {
             const int64_t f = h->fnrows[s];
-            bool good = f <= fsmall;
+            bool good = f <= fsmall && s != h->schur_s;
             int64_t base = 0, pk = 0;
             for (int k = h->child_ptr[s]; k < h->child_ptr[s + 1]; ++k) {
                 const int32_t c = h->child_list[k];
@@ -542,6 +546,10 @@ int build_schedule(sds_handle* h, int dtype) {
         }
     }
     for (int64_t s = 0; s < ns; ++s) {
+        if (s == h->schur_s) {
+            big[s] = 1;  // assembled by the last pseudo-level below; in no sweep plan
+            continue;
+        }
         Level& L = h->levels[lev[s]];
         L.all.push_back((int32_t)s);
         if (in_sub[s] >= 0) {
@@ -889,6 +897,30 @@ int build_schedule(sds_handle* h, int dtype) {
         push_launch(L, h->method == M_CHOLESKY ? L_UPDATE_CB : lu ? L_LU_CB : L_LDLT_CB, t, fl, by);
         if (chol) push_extend(L, true, L_EXTEND_ADD);
     }
+    if (h->schur_s >= 0) {
+        // Schur complement mode: one more level with the root's assembly only (zero, A, the
+        // children's blocks; for LU the L and U^T structures), after every factored front
+        // (src/numeric/schur.jl:52-78). Its front lists stay empty: the sweeps skip it.
+        Level LS;
+        const int32_t s = h->schur_s;
+        const double f = h->fnrows[s], w = h->fncols[s];
+        LS.big.push_back(s);
+        std::vector<Task> t;
+        const bool lu = h->method == M_LU;
+        for (int c = 0; c < h->fnrows[s]; c += 8) t.push_back({s, c, 8, 0});
+        const double by = dbl_es * f * w * (lu ? 2.0 : 1.0) + 16.0 * (double)h->asm_count[s];
+        if (lu) {
+            push_launch(LS, L_LU_ZERO, t, 0, by);
+            t.clear();
+            for (int c = 0; c < h->fncols[s]; c += 8) t.push_back({s, c, 8, 0});
+            push_launch(LS, L_LU_SCATTER, t, 0, 0);
+        } else {
+            push_launch(LS, L_ZERO_SCATTER, t, 0, by);
+        }
+        push_extend(LS, false, lu ? L_LU_EXTEND : L_EXTEND_ADD);
+        LS.big.clear();
+        h->levels.push_back(LS);
+    }
     void* p;
     int rc;
     // solve plans of the fronts that are not solve_small: per level, the forward sweep is
@@ -1139,6 +1171,9 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
         }
     }
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_chol_stats(st, h->dsym, h->status, h->stats, h->d_info); });
+    if (h->schur_s >= 0)
+        launch_schur_meta<T>(st, h->dsym, h->schur_s, (int)h->nschur, nullptr, nullptr, static_cast<T*>(h->dvec),
+                             h->stats);
     // the inverses of the diagonal blocks for the sweeps (a failed factorization gives
     // inverses of whatever the panels hold; its solves are not meaningful anyway)
     for (int cls = 0; cls < 3; ++cls) {
@@ -1196,6 +1231,9 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
             });
         }
     }
+    if (h->schur_s >= 0)
+        launch_schur_meta<T>(st, h->dsym, h->schur_s, (int)h->nschur, h->piv, h->pkind, static_cast<T*>(h->dvec),
+                             h->stats);
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_info_from_stats(st, h->dsym, h->stats, h->d_info); });
     HIPCHK(h, hipGetLastError());
     return SDS_SUCCESS;
@@ -1240,6 +1278,9 @@ int factorize_lu(sds_handle* h, const double* nz, int pivoting, double u, double
             });
         }
     }
+    if (h->schur_s >= 0)
+        launch_schur_meta<T>(st, h->dsym, h->schur_s, (int)h->nschur, h->piv, h->pkind, static_cast<T*>(h->dvec),
+                             h->stats);
     timed(h, SDS_K_STATS, 0, 0, [&] { launch_info_from_stats(st, h->dsym, h->stats, h->d_info); });
     HIPCHK(h, hipGetLastError());
     return SDS_SUCCESS;
@@ -1378,6 +1419,8 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                             launch_fwd_rows<T>(st, h->dsym, stp, tk + G.frows.off, G.frows.n, L, Y, U, nrhs);
                         });
                 }
+        if (fwd && h->schur_s >= 0)
+            timed(h, SDS_K_FWD, 0, 0, [&] { launch_schur_pull<T>(st, h->dsym, h->schur_s, Y, U, nrhs); });
         if (bwd)
             for (auto it = h->levels.rbegin(); it != h->levels.rend(); ++it)
                 for (auto G = it->groups.rbegin(); G != it->groups.rend(); ++G) {
@@ -1426,6 +1469,8 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                 });
             for (const Launch& ln : L.fwd_plan) run(ln, SDS_K_FWD);
         }
+    if (fwd && h->schur_s >= 0)
+        timed(h, SDS_K_FWD, 0, 0, [&] { launch_schur_pull<T>(st, h->dsym, h->schur_s, Y, U, nrhs); });
     if (diag && h->method != M_CHOLESKY)
         timed(h, SDS_K_DIAG, 2.0 * h->n * nrhs, es * (2.0 * h->n + 2.0 * h->n * nrhs) + h->n, [&] {
             launch_diag<T>(st, (int)h->n, nrhs, static_cast<const T*>(h->dvec), lu ? nullptr : h->pkind, Y);
@@ -1599,12 +1644,21 @@ int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, co
     }
     const int64_t ldbs = Bsrc == B ? ldb : n;
     const double es = sizeof(T), pbytes = (double)n * nrhs * (8.0 + es + 4.0);
+    // Schur complement mode (src/solver.jl:400-416): every phase reads B and writes X, in factor
+    // order unless it permutes (the last nschur entries are the Schur block)
+    const bool schur = h->schur_s >= 0;
     if (phase & SDS_PHASE_SOLVE_FWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y); });
-    sweeps<T>(h, W, nrhs, phase & SDS_PHASE_SOLVE_FWD, phase & SDS_PHASE_SOLVE_DIAG, phase & SDS_PHASE_SOLVE_BWD,
-              transposed);
+    else if (schur)
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, nullptr, B, ldb, Y); });
+    // Schur mode, LU: the diagonal goes with the backward phase and the diagonal phase is the
+    // identity, as for Cholesky (src/solver.jl:409-411)
+    const int diag = schur && h->method == M_LU ? (phase & SDS_PHASE_SOLVE_BWD) : (phase & SDS_PHASE_SOLVE_DIAG);
+    sweeps<T>(h, W, nrhs, phase & SDS_PHASE_SOLVE_FWD, diag, phase & SDS_PHASE_SOLVE_BWD, transposed);
     if (phase & SDS_PHASE_SOLVE_BWD_PERM)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0); });
+    else if (schur)
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, nullptr, Y, X, ldx, 0); });
     int done = 0;
     if (refine) {
         if (!h->full_rowptr) return fail(h, SDS_INVALID_VALUE, "refinement needs the full-pattern map at upload");
@@ -1702,6 +1756,8 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     free_all(h->symbufs);
     h->have_sym = false;
     h->placed = false;
+    h->schur_s = -1;
+    h->nschur = 0;
     const int b = d->index_bits;
     const int64_t n = d->n, ns = d->ns, nnz = d->nnz;
     h->n = n;
@@ -1985,6 +2041,11 @@ int check_solve_args(sds_handle* h, int32_t phase, int32_t solve_mode, int64_t l
     if (solve_mode != 0 && solve_mode != 1 && solve_mode != 2)
         return fail(h, SDS_INVALID_VALUE, "solve_mode must be 0, 1 or 2");
     if ((phase & ~SDS_PHASE_SOLVE) != 0 || phase == 0) return fail(h, SDS_INVALID_VALUE, "not a solve phase");
+    if (h->schur_s >= 0 && ((phase & SDS_PHASE_SOLVE_REFINEMENT) ||
+                            ((phase & SDS_PHASE_SOLVE_FWD) && (phase & SDS_PHASE_SOLVE_BWD))))
+        return fail(h, SDS_NOT_SUPPORTED,
+                    "Schur complement mode: the factorization is partial; solve in the phases fwd_perm | fwd, diag, "
+                    "bwd | bwd_perm");
     return SDS_SUCCESS;
 }
 
@@ -2199,6 +2260,42 @@ int sds_solve_noalloc(sds_handle_t h, int32_t phase, int32_t solve_mode, const d
              : solve_t<float>(h, W, phase, tr, B, ldb, X, ldx, nrhs, nzval, ir_n_steps, ir_tol, ir_steps);
     h->prof = prof;
     return rc;
+}
+
+int sds_set_schur(sds_handle_t h, int64_t nschur) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->have_sym) return fail(h, SDS_FACTORIZATION_ERROR, "no analysis uploaded");
+    if (nschur < 0) return fail(h, SDS_INVALID_VALUE, "nschur must be >= 0");
+    free_numeric(h);  // the launch plan changes: sds_alloc_numeric again
+    h->placed = false;
+    if (nschur == 0) {
+        h->schur_s = -1;
+        h->nschur = 0;
+        return SDS_SUCCESS;
+    }
+    const int64_t s = h->ns - 1;
+    if (s < 0 || h->fncols[s] != nschur || h->fnrows[s] != nschur || h->snparent[s] >= 0)
+        return fail(h, SDS_INVALID_VALUE,
+                    "the last supernode is not a dense root of nschur columns: analyse with the Schur indices");
+    h->schur_s = (int32_t)s;
+    h->nschur = nschur;
+    return SDS_SUCCESS;
+}
+
+int sds_schur_export(sds_handle_t h, double* S, int64_t lds) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (h->schur_s < 0) return fail(h, SDS_INVALID_VALUE, "not in Schur complement mode");
+    if (!h->factorized) return fail(h, SDS_FACTORIZATION_ERROR, "schur_matrix before a factorization");
+    if (!S || lds < h->nschur) return fail(h, SDS_INVALID_VALUE, "null buffer or leading dimension below nschur");
+    const int full = h->method == M_LU;
+    if (h->dtype == SDS_R64)
+        launch_schur_export<double>(h->stream, h->dsym, h->schur_s, (int)h->nschur, static_cast<double*>(h->factor),
+                                    full, S, lds);
+    else
+        launch_schur_export<float>(h->stream, h->dsym, h->schur_s, (int)h->nschur, static_cast<float*>(h->factor),
+                                   full, S, lds);
+    HIPCHK(h, hipGetLastError());
+    return SDS_SUCCESS;
 }
 
 int sds_adjoint_scatter(sds_handle_t h, const double* lambda, int64_t ldl, const double* X, int64_t ldx, int32_t nrhs,

@@ -8,6 +8,16 @@ template <typename T>
 void launch_subtree_chol(hipStream_t st, const DevSym& S, const int32_t* sub_ptr, const int32_t* sub_nodes, int first,
                          int count, int lds, const int32_t* lfront, const int32_t* lcb, const double* nz, T* factor,
                          T* cbpool, T* DB, int32_t* status, double eps);
+// Schur complement mode: the root's pivot order, unit pivots and zero statistics; the export of S
+template <typename T>
+void launch_schur_meta(hipStream_t st, const DevSym& S, int s, int w, int32_t* piv, int8_t* kind, T* dvec,
+                       int64_t* stats);
+template <typename T>
+void launch_schur_export(hipStream_t st, const DevSym& S, int s, int ns, const T* factor, int full, double* out,
+                         int64_t lds);
+// its rows of the forward sweep: the children's update vectors only
+template <typename T>
+void launch_schur_pull(hipStream_t st, const DevSym& S, int s, T* Y, const T* U, int nrhs);
 template <typename T>
 void launch_front_col_chol(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, int fmax,
                            const double* nz, T* factor, T* cbpool, T* DB, int32_t* status, double eps);

@@ -115,6 +115,19 @@ function upload_symbolic!(h::Handle, S::Symbolic{INT}, full) where {INT}
     return nothing
 end
 
+# Schur complement mode (src/solver.jl:400-416): the reference's analysis with schur_mode = 1
+# orders the Schur indices last as the root supernode; the library then assembles that root
+# without factoring it. Before alloc_numeric!.
+set_schur!(h::Handle, nschur::Integer) =
+    _check(h, ccall((:sds_set_schur, libsds_hip), Cint, (Ptr{Cvoid}, Int64), h.ptr, Int64(nschur)))
+
+# getparam(solver, "schur_matrix") into a dense device matrix (schur_matrix!, src/numeric/schur.jl:112-150)
+function schur_matrix!(S::ROCMatrix{Float64}, h::Handle)
+    _check(h, ccall((:sds_schur_export, libsds_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), h.ptr, pointer(S),
+                    stride(S, 2)))
+    return S
+end
+
 alloc_numeric!(h::Handle, ::Type{T}) where {T} =
     _check(h, ccall((:sds_alloc_numeric, libsds_hip), Cint, (Ptr{Cvoid}, Cint, Cint), h.ptr,
                     T === Float32 ? SDS_R32 : SDS_R64, Cint(1)))

@@ -25,7 +25,8 @@ def analysis_lib():
         i64p = ctypes.POINTER(ctypes.c_int64)
         lib.sdsa_analyze.restype = ctypes.c_void_p
         lib.sdsa_analyze.argtypes = [ctypes.c_int64, i64p, i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p,
-                                     ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_int64, ctypes.c_int]
+                                     ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_int64, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int8)]
         lib.sdsa_last_error.restype = ctypes.c_char_p
         lib.sdsa_free.argtypes = [ctypes.c_void_p]
         lib.sdsa_field_len.restype = ctypes.c_int64

@@ -34,6 +34,7 @@ class Symbolic:
     flops: float          # stored-panel flops, sum over panel columns of rows^2 (src/symbolic/supernodes.jl:336-339)
     flops_true: float     # sum_j c_j^2 over the true column counts (src/symbolic/etree.jl:153-160)
     maps: dict = field(default_factory=dict)
+    nschur: int = 0       # Schur complement mode: the size of the last supernode, the Schur block
 
     def __getattr__(self, name):
         maps = self.__dict__.get("maps")
@@ -66,12 +67,14 @@ def grid_nd_perm(nx, ny=1, nz=1, leaf=8):
 
 
 def symbolic_analysis(rowptr, colval, n, structure, view="F", index="O", perm=None, ordering="default",
-                      amalgamation=(32, 0.25, 8), use_superpanels=1):
+                      amalgamation=(32, 0.25, 8), use_superpanels=1, schur=None):
     """Run the analysis of the CSR pattern (rowptr, colval) of an n x n matrix.
 
     `perm` (1-based, length n) is the user permutation (``user_perm``); otherwise
     `ordering` selects "algo3" (approximate minimum degree, also the default here)
-    or "algo5" (natural)."""
+    or "algo5" (natural). `schur` (n flags 0/1, ``user_schur_indices`` of Schur complement mode,
+    src/numeric/schur.jl:1-10): the flagged rows and columns go last, in increasing original
+    order, and form the last supernode, which the numeric phase assembles without factoring."""
     lib = analysis_lib()
     rp = np.ascontiguousarray(rowptr, dtype=np.int64)
     cv = np.ascontiguousarray(colval, dtype=np.int64)
@@ -88,9 +91,15 @@ def symbolic_analysis(rowptr, colval, n, structure, view="F", index="O", perm=No
             raise ValueError("user_perm must have length n")
         code = 1
     P = ctypes.POINTER(ctypes.c_int64)
+    flags = None
+    if schur is not None:
+        flags = np.ascontiguousarray(schur, dtype=np.int8)
+        if flags.shape != (n,) or np.any((flags != 0) & (flags != 1)):
+            raise ValueError("user_schur_indices must hold n flags 0/1")
     h = lib.sdsa_analyze(n, rp.ctypes.data_as(P), cv.ctypes.data_as(P), structure_code(structure),
                          view_code(view), base, p.ctypes.data_as(P) if p is not None else None, code,
-                         int(amalgamation[0]), float(amalgamation[1]), int(amalgamation[2]), int(use_superpanels))
+                         int(amalgamation[0]), float(amalgamation[1]), int(amalgamation[2]), int(use_superpanels),
+                         flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)) if flags is not None else None)
     if not h:
         raise ValueError(lib.sdsa_last_error().decode())
     try:
@@ -101,11 +110,12 @@ def symbolic_analysis(rowptr, colval, n, structure, view="F", index="O", perm=No
             if ln > 0:
                 lib.sdsa_field_copy(h, name.encode(), a.ctypes.data_as(P))
             maps[name] = a
-        sc = [lib.sdsa_scalar(h, k) for k in range(7)]
+        sc = [lib.sdsa_scalar(h, k) for k in range(8)]
     finally:
         lib.sdsa_free(h)
     return Symbolic(n=int(sc[0]), nnz=int(sc[1]), ns=int(sc[2]), structure=structure, view=view,
-                    nnz_stored=int(sc[3]), nnz_L=int(sc[4]), flops=sc[5], flops_true=sc[6], maps=maps)
+                    nnz_stored=int(sc[3]), nnz_L=int(sc[4]), flops=sc[5], flops_true=sc[6], maps=maps,
+                    nschur=int(sc[7]))
 
 
 def merge_symbolic(parts):
@@ -118,6 +128,8 @@ def merge_symbolic(parts):
     st, vw = parts[0].structure, parts[0].view
     if any(p.structure != st or p.view != vw for p in parts):
         raise ValueError("the members of a batch must share structure and view")
+    if any(p.nschur for p in parts):
+        raise ValueError("Schur complement mode is for single matrices")
     gen = st == "G"
     cat = {k: [] for k in _FIELDS}
     N = NS = NZ = FL = RV = CL = RI = AS = 0

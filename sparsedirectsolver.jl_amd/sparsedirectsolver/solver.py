@@ -172,8 +172,18 @@ class DirectSolver:
         o = self.options
         if self.structure not in ("SPD", "HPD", "S", "H", "G"):
             raise NotSupportedError(f'structure "{self.structure}" is not available in this build')
+        schur = None
         if o["schur_mode"] != 0:
-            raise NotSupportedError("schur_mode = 1 is not available in this build")
+            # Schur complement mode (src/solver.jl:400-416): the flagged rows and columns last, one
+            # dense root supernode that the factorization assembles without factoring
+            if self.nbatch > 1 or self._analysis is not None:
+                raise NotSupportedError("schur_mode = 1 is not supported for batches")
+            schur = o["user_schur_indices"]
+            if schur is None:
+                raise InvalidValueError("schur_mode = 1 needs user_schur_indices (n flags 0/1)")
+            schur = np.ascontiguousarray(schur, dtype=np.int8)
+            if schur.shape != (self.n,) or np.any((schur != 0) & (schur != 1)) or not schur.any():
+                raise InvalidValueError("user_schur_indices must hold n flags 0/1, at least one of them 1")
         if o["matching_alg"] not in ("default", "none"):
             raise NotSupportedError("matching is not available in this build")
         if self._analysis is not None:
@@ -197,7 +207,8 @@ class DirectSolver:
         else:
             self.symbolic = symbolic_analysis(self.rowptr, self.colval, self.n, self.structure, self.view,
                                               self.index, perm=o["user_perm"], ordering=ordering or "default",
-                                              amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"])
+                                              amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"],
+                                              schur=schur)
         self.stage = STAGE_REORDERED
 
     def _symbolic_factorization(self):
@@ -224,6 +235,7 @@ class DirectSolver:
         d.full_rowptr, d.full_colval, d.full_src = (keep["full_rowptr"].ctypes.data, keep["full_colval"].ctypes.data,
                                                     keep["full_src"].ctypes.data)
         _check(lib, self._h, lib.sds_upload_symbolic(self._h, ctypes.byref(d)))
+        _check(lib, self._h, lib.sds_set_schur(self._h, int(getattr(S, "nschur", 0))))
         fp = self.options["factor_precision"]
         dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64
         _check(lib, self._h, lib.sds_alloc_numeric(self._h, dtype, 1))
@@ -324,6 +336,8 @@ class DirectSolver:
     def _solve(self, phase, X, B):
         if self.stage < STAGE_FACTORIZED:
             raise FactorizationError(0, f'phase "{phase}": needs a "factorization"')
+        if phase in ("solve_fwd_schur", "solve_bwd_schur") and not getattr(self.symbolic, "nschur", 0):
+            raise FactorizationError(0, f'phase "{phase}" needs an analysis with schur_mode = 1')
         if self.nbatch > 1:
             return self._solve_batch(phase, X, B)
         return self._solve_one(phase, X, B)
@@ -543,6 +557,13 @@ def getparam(solver, name):
         return S.flops
     if name == "lu_nnz":
         return S.nnz_stored
+    if name in ("schur_shape", "schur_matrix") and (S is None or not getattr(S, "nschur", 0)):
+        raise InvalidValueError(f'the data parameter "{name}" needs an analysis with schur_mode = 1')
+    if name == "schur_shape":
+        # (rows, columns, nonzeros of the pattern): the analysis keeps S dense, one triangle with the
+        # diagonal for the symmetric structures (src/solver.jl:1067)
+        ns = S.nschur
+        return (ns, ns, ns * ns if solver.structure == "G" else ns * (ns + 1) // 2)
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, f'"{name}" needs the analysis')
     lib, h = solver._lib, solver._h
@@ -567,6 +588,14 @@ def getparam(solver, name):
         if name == "npivots":
             return int(tot[3])
         return dict(npos=int(tot[0]), nneg=int(tot[1]), nzero=int(tot[2]), nperturbed=int(tot[3]), n2x2=int(tot[4]))
+    if name == "schur_matrix":
+        # S = A22 - A21 A11^-1 A12 of the flagged rows and columns in increasing order, a new dense
+        # ns x ns device matrix (src/solver.jl:1068)
+        torch = _torch()
+        ns = S.nschur
+        buf = torch.empty((ns, ns), dtype=torch.float64, device="cuda")
+        _check(lib, h, lib.sds_schur_export(h, ctypes.c_void_p(buf.data_ptr()), ns))
+        return buf.t().contiguous()  # the export is column-major
     if name == "diag":
         d = np.zeros(2 * solver.n, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
         _check(lib, h, lib.sds_get_diag(h, d.ctypes.data))
