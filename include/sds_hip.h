@@ -225,10 +225,14 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  * the others in blocked panels, 1 = one wave each / column by column, 2 = all column by column,
  * and the LDS budget of the leaf-subtree kernel (regime A: one workgroup walks a leaf subtree with
  * its fronts and contribution blocks in LDS, src/numeric/subtree.jl:274-327): 0 = none, 16, 32 or
- * 64 KiB), keeps the fastest, and leaves the numeric storage reallocated for it (factor contents
- * are not kept: factorize again). tiles (may be NULL, else 6 entries) receives {extend-add
- * rows, K chunk, trailing split, two-per-CU steps, small-front kernels, subtree KiB}. Every
- * candidate gives the same factor bit for bit. Synchronous; needs sds_alloc_numeric first.
+ * 64 KiB, and the panel block of the large fronts: 0 = the 64-column step chain runs over the
+ * whole panel, else 128, 256 or 512 columns: the chain runs inside each block and the columns
+ * right of a block get its update in one pass with K = the block), keeps the fastest, and leaves
+ * the numeric storage reallocated for it (factor contents are not kept: factorize again). tiles
+ * (may be NULL, else 7 entries) receives {extend-add rows, K chunk, trailing split, two-per-CU
+ * steps, small-front kernels, subtree KiB, panel block}. Except for the panel block, which
+ * changes the order in which a panel entry receives its updates, every candidate gives the same
+ * factor bit for bit. Synchronous; needs sds_alloc_numeric first.
  */
 int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_params* params, int32_t reps,
                  int32_t* tiles);
@@ -237,7 +241,7 @@ int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_para
  * Set the tile shapes of the front kernels directly, as sds_autotune reports them: tiles =
  * {extend-add rows (512, 1024, 2048 or 4096), K chunk (16, 32 or 64), trailing split (0 = never,
  * else >= 1), two-per-CU steps (0 = never, else >= 1), small-front kernels (0, 1 or 2), subtree
- * KiB (0, 16, 32 or 64)}. No
+ * KiB (0, 16, 32 or 64), panel block (0 or a multiple of 64 up to 4096)}. No
  * counterpart in the reference (whose kernels have fixed shapes); it restores a saved tuning
  * without timing again. Takes effect at the next sds_alloc_numeric (the launch plan depends on it).
  */

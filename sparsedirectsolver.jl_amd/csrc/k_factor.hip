@@ -625,15 +625,19 @@ template <typename T, int KC, bool EARLY>
 __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& S, const Task t, T* __restrict__ factor,
                                                                   T* __restrict__ cbpool,
                                                                   const int32_t* __restrict__ status, int mode,
-                                                                  T (*As)[TILE], T (*Bs)[TILE]) {
+                                                                  T (*As)[TILE], T (*Bs)[TILE], int pblk,
+                                                                  T (*Dout)[NB + 1] = nullptr, int nbd = 0) {
     constexpr int PER = KC * TILE / 256;  // chunk rows per wave
     constexpr int EVERY = KC / 4 / PER;   // products per issued pair of next-chunk loads
     typedef typename Mfma<T>::acc_t acc_t;
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
     if (status[s] != 0) return;
     const int I0 = t.a, J0 = t.b;
-    const int k0 = mode == 0 ? t.c : 0;
-    const int k1 = mode == 0 ? min(t.c + NB, w) : w;
+    // mode 0: K starts at column c; it spans pblk columns (a whole panel block) when the BIGK bit
+    // of c is set, one step (NB) otherwise
+    const int kc0 = t.c & ~BIGK;
+    const int k0 = mode == 0 ? kc0 : 0;
+    const int k1 = mode == 0 ? min(kc0 + ((t.c & BIGK) ? pblk : NB), w) : w;
     const int jlim = mode == 0 ? w : f;
     const int jmin = mode == 0 ? k1 : w;
     T* P = factor + S.panel_off[s];
@@ -719,7 +723,19 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
             for (int q = 0; q < 4; ++q) {
                 const int i = I0 + wm * 32 + a * 16 + l16;
                 const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
-                if (j < jlim && j >= jmin && i < f && i >= j) {
+                const bool in = j < jlim && j >= jmin && i < f && i >= j;
+                if (Dout) {
+                    // a diagonal tile about to be factored: its lower nbd x nbd block goes to LDS
+                    // (upper part zero, identity past nbd), rows below the block to the panel
+                    const int ii = i - I0, jj = j - J0;
+                    if (ii < nbd && jj < nbd) {
+                        Dout[jj][ii] = in ? (EARLY ? old[a][b][q] : *front_at(P, C, f, w, i, j)) - acc[a][b][q] : T(0);
+                        continue;
+                    }
+                    if (jj >= nbd) Dout[jj][ii] = ii == jj ? T(1) : T(0);
+                    else Dout[jj][ii] = T(0);
+                }
+                if (in) {
                     // mode 1 writes the contribution block whole (the children's entries are
                     // added to it afterwards)
                     T* d = front_at(P, C, f, w, i, j);
@@ -731,10 +747,10 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
 template <typename T, int KC>
 __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
                                                         T* __restrict__ factor, T* __restrict__ cbpool,
-                                                        const int32_t* __restrict__ status, int mode) {
+                                                        const int32_t* __restrict__ status, int mode, int pb) {
     __shared__ T As[KC][TILE];
     __shared__ T Bs[KC][TILE];
-    update_tile<T, KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs);
+    update_tile<T, KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs, pb);
 }
 
 // the rows [k + 64, k + 128) of a step's columns, kept in LN by k_big_step, into the panel.
@@ -771,6 +787,12 @@ __device__ inline void copy_ln(const DevSym& S, const Task t, T* __restrict__ fa
 // The next step's launch also carries this step's copies of those rows (copy_ln) and its
 // trailing update of the blocks three on (update_tile, mode 0): they touch no block that
 // step reads or writes, and they fill the machine while the step's chain runs.
+// Panel blocks (pb > 0, a multiple of NB): the chain above runs inside blocks of pb columns
+// [K0, K1) only. The look-ahead stops at K1; the columns from K1 on get the block's whole
+// update with K = pb from k_big_block (the next block's first strip, whose launch also factors
+// the next diagonal block) and from update tiles with the BIGK flag that ride in the next
+// block's step launches (the strips after it). A block's first step has no previous step in
+// the chain: k_big_block applied everything before K0.
 // tasks: nstep steps (s, k, I0), then ncopy copies (s, k), then update tiles (s, I0, J0, k)
 // OCC: workgroups per CU the kernel is built for (1: registers unbounded; 2: at most 256 VGPRs
 // per lane, for the launches with many workgroups). The two LDS tiles (67 KiB in FP64) allow two.
@@ -779,7 +801,7 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
                                                       T* __restrict__ factor, T* __restrict__ cbpool,
                                                       T* __restrict__ DB, T* __restrict__ LN,
                                                       int32_t* __restrict__ status, double eps, int nstep,
-                                                      int ncopy) {
+                                                      int ncopy, int pb) {
     typedef typename Mfma<T>::acc_t acc_t;
     // Bs[kk][c] = Linv_b(c, kk); later L(J + r, b - 1) as Bs[r][kk]; later L(I0 + r, k + kk) (the
     // rows after next) or D
@@ -790,7 +812,7 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
     const Task t = tasks[blockIdx.x];
     if ((int)blockIdx.x >= nstep + ncopy) {
         update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, reinterpret_cast<T(*)[TILE]>(&Bs[0][0]),
-                                 reinterpret_cast<T(*)[TILE]>(&Ln[0][0]));
+                                 reinterpret_cast<T(*)[TILE]>(&Ln[0][0]), pb);
         return;
     }
     if ((int)blockIdx.x >= nstep) {
@@ -801,8 +823,10 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
     if (status[s] != 0) return;
     SDS_STEP_MARK(0);
     const int nb = min(NB, w - k);
-    const bool la = k + NB < w;  // a next block exists (then nb = NB)
-    const int J = k + NB, nbn = la ? min(NB, w - J) : 0;
+    // the panel block of this step: [K0, K1)
+    const int K0 = pb > 0 ? k - k % pb : 0, K1 = pb > 0 ? min(w, K0 + pb) : w;
+    const bool la = k + NB < K1;  // a next block exists in the chain (then nb = NB)
+    const int J = k + NB, nbn = la ? min(NB, K1 - J) : 0;
     T* P = factor + S.panel_off[s];
     const int cg = S.super_ptr[s];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
@@ -835,7 +859,7 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
     // solve: the previous step's rows J and I0, and the tile's old values of the next block's
     // columns (all final before this launch: the trailing-update tiles of this launch touch
     // neither)
-    const bool prev = k >= NB;
+    const bool prev = k >= K0 + NB;
     const int kp = max(k - NB, 0);
     T pj[16], ap[16];
     acc_t accC[4];  // transposed tile: element q of lane l is C(I0 + 16 wv + l % 16, J + 16 nt + row(l / 16, q))
@@ -939,11 +963,11 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
                 if (i < f && j < J + nbn) P[(int64_t)j * f + i] = accC[nt][q];
             }
         const int I = J + NB;  // the block after next
-        if (I0 != I || I >= w) return;
+        if (I0 != I || I >= K1) return;
         // this workgroup's rows are the block after next: its diagonal tile gets steps b and
         // b - 1 here (the trailing-update tiles of step b - 1 in this launch leave it out), so
         // that the next step's diagonal workgroup only applies its own step
-        const int nb3 = min(NB, w - I);
+        const int nb3 = min(NB, K1 - I);
         acc_t accD[4];  // element q of lane l is C(I + 16 wv + l % 16, I + 16 nt + row(l / 16, q))
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
@@ -1015,6 +1039,40 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
     SDS_STEP_MARK(8);
 }
 
+
+// The first strip of the next panel block: tiles (s, I0, J0, K0 | BIGK) with J0 in [K1, K1 + pb)
+// get C(I, J) -= L(I, K0:K1) L(J, K0:K1)^T with K = pb (K1 = K0 + pb), on the matrix cores
+// (update_tile). The first ndiag tasks are the next block's diagonal tiles (I0 = J0 = K1): their
+// workgroup keeps the updated block in LDS and factors and inverts it (as k_big_diag) for the
+// next block's step chain; its rows below a narrower last block go to the panel.
+template <typename T>
+__global__ void __launch_bounds__(256, 2) k_big_block(DevSym S, const Task* __restrict__ tasks,
+                                                     T* __restrict__ factor, T* __restrict__ cbpool,
+                                                     T* __restrict__ DB, int32_t* __restrict__ status, double eps,
+                                                     int pb, int ndiag) {
+    __shared__ T D[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];
+    __shared__ T rd[NB];
+    __shared__ int fail;
+    const Task t = tasks[blockIdx.x];
+    T(*As)[TILE] = reinterpret_cast<T(*)[TILE]>(&D[0][0]);
+    T(*Bs)[TILE] = reinterpret_cast<T(*)[TILE]>(&X[0][0]);
+    if ((int)blockIdx.x >= ndiag) {
+        update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, As, Bs, pb);
+        return;
+    }
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], k = t.b;
+    if (status[s] != 0) return;
+    const int nb = min(NB, w - k);
+    update_tile<T, NB, true>(S, t, factor, cbpool, status, 0, As, Bs, pb, D, nb);
+    __syncthreads();
+    block_potrf_inv<T>(D, rd, &fail, nb, eps, X);
+    if (fail) {
+        if (threadIdx.x == 0) status[s] = k + fail;
+        return;
+    }
+    block_store<T>(D, X, factor + S.panel_off[s], f, k, nb, DB + (int64_t)(S.super_ptr[s] + k) * NB);
+}
 
 // per-front statistics of a Cholesky (src/numeric/factorize.jl:13-48) and, per block, the
 // smallest failed factor column (1-based, INT64_MAX = none) into part[blockIdx.x]
@@ -1151,24 +1209,34 @@ void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, 
 
 template <typename T>
 void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
-                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ) {
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ, int pb) {
     if (occ == 2)
         hipLaunchKernelGGL((k_big_step<T, 2>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps,
-                           nstep, ncopy);
+                           nstep, ncopy, pb);
     else
         hipLaunchKernelGGL((k_big_step<T, 1>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, LN, status, eps,
-                           nstep, ncopy);
+                           nstep, ncopy, pb);
+}
+
+template <typename T>
+void launch_big_block(hipStream_t st, const DevSym& S, const Task* tasks, int n, int ndiag, T* factor, T* cbpool,
+                      T* DB, int32_t* status, double eps, int pb) {
+    hipLaunchKernelGGL(k_big_block<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, DB, status, eps, pb,
+                       ndiag);
 }
 
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
-                       const int32_t* status, int mode, int kc) {
+                       const int32_t* status, int mode, int kc, int pb) {
     if (kc == 64)
-        hipLaunchKernelGGL((k_big_update_mfma<T, 64>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+        hipLaunchKernelGGL((k_big_update_mfma<T, 64>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
+                           pb);
     else if (kc == 16)
-        hipLaunchKernelGGL((k_big_update_mfma<T, 16>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+        hipLaunchKernelGGL((k_big_update_mfma<T, 16>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
+                           pb);
     else
-        hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode);
+        hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
+                           pb);
 }
 
 template <typename T>
@@ -1219,9 +1287,11 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
     template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);            \
     template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \
     template void launch_big_step<T>(hipStream_t, const DevSym&, const Task*, int, int, int, T*, T*, T*, T*,      \
-                                     int32_t*, double, int);                                                       \
+                                     int32_t*, double, int, int);                                                  \
+    template void launch_big_block<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, T*, int32_t*,    \
+                                      double, int);                                                                \
     template void launch_big_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, const int32_t*, int, \
-                                       int);
+                                       int, int);
 INST(double)
 INST(float)
 #undef INST

@@ -46,6 +46,9 @@ constexpr int FUSED_INV_MAX = 16;
 constexpr int INFO_BLOCKS = 256;
 constexpr int TINY_F = 32;  // fronts of at most this many rows: one wave each (Cholesky)  // block minima of the info reduction (after the 8 info words)
 constexpr int TILE = 64;   // Schur-update tile
+// flag in field c of a panel update tile (c = the first column of K): K spans a whole panel
+// block (the block width is an argument of the launch) instead of one NB-column step
+constexpr int32_t BIGK = 1 << 30;
 constexpr int PANEL_ROWS = 256;  // panel rows per workgroup of the fused block step
 constexpr int EA_COLS = 8;      // child columns per extend-add task
 constexpr int EA_ROWS = 1024;   // child rows per extend-add task (default; the autotuner may pick another)

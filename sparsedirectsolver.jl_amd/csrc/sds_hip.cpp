@@ -50,6 +50,7 @@ enum LaunchKind {
     L_TRSM,
     L_UPDATE_PANEL,
     L_STEP,          // k_big_step: a step, with the previous step's copies and trailing update
+    L_BLOCK,         // k_big_block: a panel block's update of the next strip, the next diagonal blocks
     L_UPDATE_CB,
     L_LDLT_CB,
     L_LU_ZERO,
@@ -118,6 +119,7 @@ int prof_kind(int kind) {
         case L_TRSM: return SDS_K_TRSM;
         case L_UPDATE_PANEL: return SDS_K_UPDATE_PANEL;
         case L_STEP: return SDS_K_TRSM;
+        case L_BLOCK: return SDS_K_UPDATE_PANEL;
         case L_LU_PIVOT: return SDS_K_PIVOT_PANEL;
         case L_LDLT_PIVOT: return SDS_K_PIVOT_PANEL;
         case L_LDLT_UPDATE: return SDS_K_PIVOT_PANEL;
@@ -225,6 +227,7 @@ struct sds_handle {
     int step_occ2 = 0;   // step launches of at least this many workgroups use the two-per-CU build (0: none)
     int small_kind = 0;  // small-front kernels (Cholesky): 0 one wave per tiny front + blocked,
                          // 1 one wave per tiny front + column by column, 2 column by column
+    int panel_block = 0;  // Cholesky large fronts: panel block width (a multiple of NB; 0 = one block)
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -809,31 +812,91 @@ int build_schedule(sds_handle* h, int dtype) {
         // per 64-column step b (k = 64 b): one k_big_step launch with the step's row tiles
         // (row solve, look-ahead update of block b + 1, the next diagonal block), the previous
         // step's copies and its trailing update of blocks b + 2 on (the first step's diagonal
-        // block by k_big_diag before; the last step's copies and update in a launch of their own)
+        // block by k_big_diag before; the last step's copies and update in a launch of their own).
+        // With panel blocks of pb columns the chain and the trailing tiles stay inside a block
+        // [K0, K0 + pb); at each block start K1 = K0 + pb a k_big_block launch updates the next
+        // strip [K1, K1 + pb) with K = pb (and factors its diagonal block), and the tiles of the
+        // block's update right of that strip ride in the next block's step launches, a share in
+        // each (they touch no column those launches read or write).
+        const int pb = h->panel_block;
         std::vector<Task> pc, pr;  // the previous step's copies and update tiles
-        double pfr = 0, pbr = 0;
+        std::vector<Task> bigq;    // the previous block's update tiles right of the current block
+        double pfr = 0, pbr = 0, bigf = 0, bigb = 0;
+        int big_left = 0;          // step launches of the current block still to take a share
         // rest_split: a step whose trailing tiles number at least rest_split sends them to a
         // launch of their own right after it (k_big_update_mfma, several workgroups per CU)
         // instead of riding in the step launch (one workgroup per CU beside the chain)
         auto push_step = [&](std::vector<Task>& st, double fs, double bs) {
-            if (st.empty() && pc.empty() && pr.empty()) return;
-            h->max_rest = std::max(h->max_rest, (int)pr.size());
-            const bool split = h->rest_split > 0 && (int)pr.size() >= h->rest_split;
+            std::vector<Task> rest(pr);
+            double rf = pfr, rb = pbr;
+            if (big_left > 0) {  // this launch's share of the previous block's update
+                const size_t take = (bigq.size() + big_left - 1) / big_left;
+                const double frac = bigq.empty() ? 0.0 : (double)take / (double)bigq.size();
+                rest.insert(rest.end(), bigq.begin(), bigq.begin() + take);
+                rf += bigf * frac;
+                rb += bigb * frac;
+                bigf -= bigf * frac;
+                bigb -= bigb * frac;
+                bigq.erase(bigq.begin(), bigq.begin() + take);
+                --big_left;
+            }
+            if (st.empty() && pc.empty() && rest.empty()) return;
+            h->max_rest = std::max(h->max_rest, (int)rest.size());
+            const bool split = h->rest_split > 0 && (int)rest.size() >= h->rest_split;
             std::vector<Task> all(st);
             all.insert(all.end(), pc.begin(), pc.end());
-            if (!split) all.insert(all.end(), pr.begin(), pr.end());
-            push_launch(L, L_STEP, all, fs + (split ? 0.0 : pfr),
-                        bs + (split ? 0.0 : pbr) + dbl_es * 2.0 * NB * NB * (double)pc.size());
+            if (!split) all.insert(all.end(), rest.begin(), rest.end());
+            push_launch(L, L_STEP, all, fs + (split ? 0.0 : rf), bs + (split ? 0.0 : rb) + dbl_es * 2.0 * NB * NB * (double)pc.size());
             L.plan.back().nstep = (int)st.size();
             L.plan.back().ncopy = (int)pc.size();
-            if (split) push_launch(L, L_UPDATE_PANEL, pr, pfr, pbr);
+            if (split) push_launch(L, L_UPDATE_PANEL, rest, rf, rb);
         };
         for (int k = 0; k < wmax; k += NB) {
+            if (pb > 0 && k > 0 && k % pb == 0) {
+                // block [k - pb, k) done: the next strip and diagonal blocks, and the tiles right of
+                // the strip for the next block's step launches
+                std::vector<Task> none;
+                push_step(none, 0, 0);  // (nothing is left over at a block's end)
+                std::vector<Task> tdg, tst;
+                double fb = 0, bb = 0;
+                bigq.clear();
+                bigf = bigb = 0;
+                const int32_t kc = (int32_t)(k - pb) | BIGK;
+                for (int32_t s : L.big) {
+                    const int f = h->fnrows[s], w = h->fncols[s];
+                    if (k >= w) continue;
+                    const int nb = std::min(NB, w - k);
+                    tdg.push_back({s, k, k, kc});
+                    fb += (double)nb * nb * nb / 3.0 * 2.0;
+                    bb += dbl_es * ((double)nb * (nb + 1) + (double)NB * nb);
+                    for (int J0 = k; J0 < w; J0 += TILE)
+                        for (int I0 = J0; I0 < f; I0 += TILE) {
+                            const double e = tile_entries(I0, J0, f, k, w);
+                            const double fl2 = 2.0 * e * pb, by2 = 2.0 * dbl_es * e;
+                            if (J0 < k + pb) {
+                                fb += fl2;
+                                bb += by2;
+                                if (I0 != k || J0 != k) tst.push_back({s, I0, J0, kc});
+                            } else {
+                                bigq.push_back({s, I0, J0, kc});
+                                bigf += fl2;
+                                bigb += by2;
+                            }
+                        }
+                    bb += dbl_es * (double)(f - k + pb) * pb;  // the block's columns read once
+                }
+                const int ndg = (int)tdg.size();
+                tdg.insert(tdg.end(), tst.begin(), tst.end());
+                push_launch(L, L_BLOCK, tdg, fb, bb);
+                if (!tdg.empty()) L.plan.back().nstep = ndg;
+                big_left = std::max(1, (std::min(wmax, k + pb) - k + NB - 1) / NB);
+            }
             std::vector<Task> td, tt, tr, tc;
             double fp = 0, ft = 0, fr = 0, bp = 0, bt = 0, br = 0;
             for (int32_t s : L.big) {
                 int f = h->fnrows[s], w = h->fncols[s];
                 if (k >= w) continue;
+                const int K0 = pb > 0 ? k - k % pb : 0, K1 = pb > 0 ? std::min(w, K0 + pb) : w;
                 int nb = std::min(NB, w - k);
                 if (k == 0) td.push_back({s, 0, 0, 0});
                 // the diagonal block's factorization and inverse
@@ -842,28 +905,29 @@ int build_schedule(sds_handle* h, int dtype) {
                 for (int r0 = k + nb; r0 < f; r0 += 64) tt.push_back({s, k, r0, 0});
                 ft += 2.0 * (double)(f - k - nb) * nb * nb;
                 bt += dbl_es * 2.0 * (double)(f - k - nb) * nb;
-                if (k + NB < w) {  // the look-ahead update of block b + 1 (and b's predecessor's part)
+                const bool prev = k >= K0 + NB;
+                if (k + NB < K1) {  // the look-ahead update of block b + 1 (and b's predecessor's part)
                     const int J = k + NB;
                     tc.push_back({s, k, 0, 0});
                     for (int I0 = J; I0 < f; I0 += TILE) {
-                        double e = tile_entries(I0, J, f, J, w);
-                        ft += 2.0 * e * (k >= NB ? 2 * NB : NB);
+                        double e = tile_entries(I0, J, f, J, K1);
+                        ft += 2.0 * e * (prev ? 2 * NB : NB);
                         bt += 2.0 * dbl_es * e;
                     }
-                    if (J + NB < w) {  // the diagonal tile after next, by the row tile of its rows
-                        double e = tile_entries(J + NB, J + NB, f, J + NB, w);
-                        ft += 2.0 * e * (k >= NB ? 2 * NB : NB);
+                    if (J + NB < K1) {  // the diagonal tile after next, by the row tile of its rows
+                        double e = tile_entries(J + NB, J + NB, f, J + NB, K1);
+                        ft += 2.0 * e * (prev ? 2 * NB : NB);
                         bt += 2.0 * dbl_es * e;
                     }
                 }
                 const int base = k + 3 * NB;
                 for (int I0 = base; I0 < f; I0 += TILE)
-                    for (int J0 = base; J0 <= I0 && J0 < w; J0 += TILE) {
+                    for (int J0 = base; J0 <= I0 && J0 < K1; J0 += TILE) {
                         // the first diagonal tile gets this step's update from the next step's
                         // row tile of the same rows (with that step's own update)
                         if (I0 == base && J0 == base) continue;
                         tr.push_back({s, I0, J0, k});
-                        double e = tile_entries(I0, J0, f, base, w);
+                        double e = tile_entries(I0, J0, f, base, K1);
                         fr += 2.0 * e * nb;
                         br += 2.0 * dbl_es * e;
                     }
@@ -878,6 +942,10 @@ int build_schedule(sds_handle* h, int dtype) {
         {
             std::vector<Task> none;
             push_step(none, 0, 0);
+            pc.clear();
+            pr.clear();
+            pfr = pbr = 0;
+            while (big_left > 0 && !bigq.empty()) push_step(none, 0, 0);
         }
         t.clear();
         fl = by = 0;
@@ -1158,13 +1226,19 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
                     case L_STEP:
                         launch_big_step<T>(st, h->dsym, t, ln.n, ln.nstep, ln.ncopy, factor, cb,
                                            static_cast<T*>(h->dbuf), static_cast<T*>(h->lnbuf), h->status, eps,
-                                           h->step_occ2 > 0 && ln.n >= h->step_occ2 ? 2 : 1);
+                                           h->step_occ2 > 0 && ln.n >= h->step_occ2 ? 2 : 1, h->panel_block);
+                        break;
+                    case L_BLOCK:
+                        launch_big_block<T>(st, h->dsym, t, ln.n, ln.nstep, factor, cb, static_cast<T*>(h->dbuf),
+                                            h->status, eps, h->panel_block);
                         break;
                     case L_UPDATE_PANEL:
-                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0, h->upd_kc);
+                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 0, h->upd_kc,
+                                             h->panel_block);
                         break;
                     case L_UPDATE_CB:
-                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1, h->upd_kc);
+                        launch_big_update<T>(st, h->dsym, t, ln.n, factor, cb, h->status, 1, h->upd_kc,
+                                             h->panel_block);
                         break;
                 }
             });
@@ -2071,21 +2145,23 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     const int dtype = h->dtype;
     // candidate tile shapes; the K chunk only matters to the Cholesky contribution-block update
     struct Cand {
-        int ea, kc, rs, oc, sk, sb;
+        int ea, kc, rs, oc, sk, sb, pb;
     };
     std::vector<Cand> cand;
     for (int ea : {512, 1024, 2048, 4096})
         for (int kc : {16, 32, 64})
-            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0, 0, 0});
+            if (kc == 32 || h->method == M_CHOLESKY) cand.push_back({ea, kc, 0, 0, 0, 0, h->panel_block});
     if (h->method == M_CHOLESKY) {  // with the best of the above
+        for (int pb : {0, 128, 256, 512})
+            if (pb != h->panel_block) cand.push_back({-1, -1, 0, 0, 0, 0, pb});
         for (int rs : {256, 512, 1024})
-            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0, 0, 0});
-        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc, 0, 0});
-        for (int sk : {1, 2}) cand.push_back({-1, -1, -1, -1, sk, 0});
-        for (int sb : {16, 32, 64}) cand.push_back({-1, -1, -1, -1, -1, sb});
+            if (rs <= h->max_rest) cand.push_back({-1, -1, rs, 0, 0, 0, -1});
+        for (int oc : {1, 64, 128, 256, 512}) cand.push_back({-1, -1, -1, oc, 0, 0, -1});
+        for (int sk : {1, 2}) cand.push_back({-1, -1, -1, -1, sk, 0, -1});
+        for (int sb : {16, 32, 64}) cand.push_back({-1, -1, -1, -1, -1, sb, -1});
     }
     double best = 1e300;
-    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2, h->small_kind, h->sub_kib};
+    Cand pick{h->ea_rows, h->upd_kc, h->rest_split, h->step_occ2, h->small_kind, h->sub_kib, h->panel_block};
     int64_t info = 0;
     int rc;
     for (Cand c : cand) {
@@ -2096,12 +2172,14 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
         if (c.rs < 0) c.rs = pick.rs;
         if (c.oc < 0) c.oc = pick.oc;
         if (c.sk < 0) c.sk = pick.sk;
+        if (c.pb < 0) c.pb = pick.pb;
         h->ea_rows = c.ea;
         h->upd_kc = c.kc;
         h->rest_split = c.rs;
         h->step_occ2 = c.oc;
         h->small_kind = c.sk;
         h->sub_kib = c.sb;
+        h->panel_block = c.pb;
         if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
         if ((rc = sds_factorize(h, nzval, prm, &info))) return rc;  // warm-up
         const auto t0 = std::chrono::steady_clock::now();
@@ -2119,6 +2197,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
     h->step_occ2 = pick.oc;
     h->small_kind = pick.sk;
     h->sub_kib = pick.sb;
+    h->panel_block = pick.pb;
     if ((rc = sds_alloc_numeric(h, dtype, 1))) return rc;
     if (tiles) {
         tiles[0] = pick.ea;
@@ -2127,6 +2206,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
         tiles[3] = pick.oc;
         tiles[4] = pick.sk;
         tiles[5] = pick.sb;
+        tiles[6] = pick.pb;
     }
     return SDS_SUCCESS;
 }
@@ -2134,7 +2214,8 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
 int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     if (!h) return SDS_INVALID_VALUE;
     if (!tiles) return fail(h, SDS_INVALID_VALUE, "tiles is NULL");
-    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4], sb = tiles[5];
+    const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4], sb = tiles[5],
+              pb = tiles[6];
     if (ea != 512 && ea != 1024 && ea != 2048 && ea != 4096)
         return fail(h, SDS_INVALID_VALUE, "extend-add rows must be 512, 1024, 2048 or 4096");
     if (kc != 16 && kc != 32 && kc != 64) return fail(h, SDS_INVALID_VALUE, "the K chunk must be 16, 32 or 64");
@@ -2142,12 +2223,15 @@ int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     if (sk < 0 || sk > 2) return fail(h, SDS_INVALID_VALUE, "the small-front kernel must be 0, 1 or 2");
     if (sb != 0 && sb != 16 && sb != 32 && sb != 64)
         return fail(h, SDS_INVALID_VALUE, "the subtree budget must be 0, 16, 32 or 64 (KiB)");
+    if (pb < 0 || pb % NB != 0 || pb > 4096)
+        return fail(h, SDS_INVALID_VALUE, "the panel block must be 0 or a multiple of 64 up to 4096");
     h->ea_rows = ea;
     h->upd_kc = kc;
     h->rest_split = rs;
     h->step_occ2 = oc;
     h->small_kind = sk;
     h->sub_kib = sb;
+    h->panel_block = pb;
     return SDS_SUCCESS;
 }
 

@@ -36,12 +36,17 @@ void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, i
 template <typename T>
 void launch_big_diag(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* DB, int32_t* status,
                      double eps);
+// pb: panel block width (0: the step chain runs over the whole panel)
 template <typename T>
 void launch_big_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, int nstep, int ncopy, T* factor,
-                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ);
+                     T* cbpool, T* DB, T* LN, int32_t* status, double eps, int occ, int pb);
+// the next strip of a panel block (first ndiag tasks: the next diagonal blocks, factored too)
+template <typename T>
+void launch_big_block(hipStream_t st, const DevSym& S, const Task* tasks, int n, int ndiag, T* factor, T* cbpool,
+                      T* DB, int32_t* status, double eps, int pb);
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
-                       const int32_t* status, int mode, int kc);
+                       const int32_t* status, int mode, int kc, int pb);
 void launch_chol_stats(hipStream_t st, const DevSym& S, const int32_t* status, int64_t* stats, int64_t* info);
 // info[0] = the smallest of the `blocks` block minima at info + 8 (0 for none)
 void launch_info_final(hipStream_t st, int blocks, int64_t* info);

@@ -467,29 +467,31 @@ def autotune(solver, reps=2):
     """Choose the front kernels' tile shapes for this analysis by timing `reps` factorizations of
     the current values per candidate (libsds_hip's sds_autotune; no counterpart in the reference).
     Returns {"extend_add_rows": r, "cb_k_chunk": kc, "trailing_split": t, "step_two_per_cu": o,
-    "small_front_kernel": k, "subtree_kib": b}; the solver then needs a "factorization" again."""
+    "small_front_kernel": k, "subtree_kib": b, "panel_block": p}; the solver then needs a
+    "factorization" again."""
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, 'autotune needs "analysis"')
     p, keep = solver._factor_params()
-    tiles = (ctypes.c_int32 * 6)()
+    tiles = (ctypes.c_int32 * 7)()
     _check(solver._lib, solver._h, solver._lib.sds_autotune(solver._h, ctypes.c_void_p(solver.nzval.data_ptr()),
                                                              ctypes.byref(p), int(reps), tiles))
     del keep
     solver.stage = STAGE_ANALYZED
     return {"extend_add_rows": int(tiles[0]), "cb_k_chunk": int(tiles[1]), "trailing_split": int(tiles[2]),
-            "step_two_per_cu": int(tiles[3]), "small_front_kernel": int(tiles[4]), "subtree_kib": int(tiles[5])}
+            "step_two_per_cu": int(tiles[3]), "small_front_kernel": int(tiles[4]), "subtree_kib": int(tiles[5]),
+            "panel_block": int(tiles[6])}
 
 
 def set_tiles(solver, tiles):
     """Set the front kernels' tile shapes as `autotune` returns them (a dict with the keys
     "extend_add_rows", "cb_k_chunk", "trailing_split", "step_two_per_cu" and optionally
-    "small_front_kernel" and "subtree_kib", default 0; libsds_hip's sds_set_tiles) and reallocate the
-    numeric storage for them: the solver then needs a "factorization" again."""
+    "small_front_kernel", "subtree_kib" and "panel_block", default 0; libsds_hip's sds_set_tiles) and
+    reallocate the numeric storage for them: the solver then needs a "factorization" again."""
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, 'set_tiles needs "analysis"')
-    t = (ctypes.c_int32 * 6)(int(tiles["extend_add_rows"]), int(tiles["cb_k_chunk"]), int(tiles["trailing_split"]),
+    t = (ctypes.c_int32 * 7)(int(tiles["extend_add_rows"]), int(tiles["cb_k_chunk"]), int(tiles["trailing_split"]),
                              int(tiles["step_two_per_cu"]), int(tiles.get("small_front_kernel", 0)),
-                             int(tiles.get("subtree_kib", 0)))
+                             int(tiles.get("subtree_kib", 0)), int(tiles.get("panel_block", 0)))
     _check(solver._lib, solver._h, solver._lib.sds_set_tiles(solver._h, t))
     fp = solver.options["factor_precision"]
     dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64
