@@ -229,8 +229,9 @@ int sds_set_ir_mode(sds_handle_t handle, int32_t mode);
  * whole panel, else 128, 256 or 512 columns: the chain runs inside each block and the columns
  * right of a block get its update in one pass with K = the block), keeps the fastest, and leaves
  * the numeric storage reallocated for it (factor contents are not kept: factorize again). tiles
- * (may be NULL, else 7 entries) receives {extend-add rows, K chunk, trailing split, two-per-CU
- * steps, small-front kernels, subtree KiB, panel block}. Except for the panel block, which
+ * (may be NULL, else 8 entries) receives {extend-add rows, K chunk, trailing split, two-per-CU
+ * steps, small-front kernels, subtree KiB, panel block, pivot panels (not tuned: the setting in
+ * place)}. Except for the panel block, which
  * changes the order in which a panel entry receives its updates, every candidate gives the same
  * factor bit for bit. Synchronous; needs sds_alloc_numeric first.
  */
@@ -241,7 +242,9 @@ int sds_autotune(sds_handle_t handle, const double* nzval, const sds_factor_para
  * Set the tile shapes of the front kernels directly, as sds_autotune reports them: tiles =
  * {extend-add rows (512, 1024, 2048 or 4096), K chunk (16, 32 or 64), trailing split (0 = never,
  * else >= 1), two-per-CU steps (0 = never, else >= 1), small-front kernels (0, 1 or 2), subtree
- * KiB (0, 16, 32 or 64), panel block (0 or a multiple of 64 up to 4096)}. No
+ * KiB (0, 16, 32 or 64), panel block (0 or a multiple of 64 up to 4096), pivot panels (LDL^T large
+ * fronts: 0 = one pivot per pair of launches, the default; 1 = panels of 32 columns with deferred
+ * updates, one workgroup per front; both give the same factor bit for bit)}. No
  * counterpart in the reference (whose kernels have fixed shapes); it restores a saved tuning
  * without timing again. Takes effect at the next sds_alloc_numeric (the launch plan depends on it).
  */

@@ -722,6 +722,603 @@ __global__ void __launch_bounds__(256) k_ldlt_update(DevSym S, const Task* __res
     }
 }
 
+// ---------------------------------------------------------------------------
+// large fronts with deferred updates: k_ldlt_panel (one workgroup per front) takes the pivots
+// of a panel of at least LP_NB columns one after the other. The front's columns from the current
+// pivot on keep, in the panel memory, their values of the panel's start; the columns the pivot
+// choice reads are formed when it reads them, by applying the panel's pivots to them in the
+// reference's order: x - L(i, p) U(p, j) for a 1x1 pivot p, x - (L(i, p) U(p, j) + L(i, p + 1)
+// U(p + 1, j)) for a 2x2 one, pivot after pivot. U(p, j), the row of pivot p in column j at
+// the time of pivot p (the upper mirror entry the reference's update reads), is formed once per
+// pivot for every later column. k_ldlt_trail then applies the panel's pivots to the columns
+// after it in the same way over the GPU. Every entry thus goes through the operations of
+// ldlt_front in the same order, later: the factor is bitwise that of one pivot at a time. The
+// symmetric interchanges act on the stored values, as in the reference; a row interchange of
+// two positions at or after the current pivot commutes with the pending updates, since it
+// exchanges the L rows of the panel's pivots too.
+
+constexpr int LP_NB = 32;           // columns per panel (a 2x2 pivot at its end takes one more)
+constexpr int LP_MAXP = LP_NB + 1;  // pivot rows of one panel
+
+// scratch of one front during a panel (element offsets from its base): the current columns X0,
+// X1, X2 (f entries each), the pivot rows U (LP_MAXP x w, U[q w + j] = U(k0 + q, j)) and the
+// current diagonal DG (w)
+__host__ __device__ inline int64_t lp_scratch(int f, int w) { return 3 * (int64_t)f + (int64_t)(LP_MAXP + 1) * w; }
+
+template <typename T, int NT>
+struct PanelFront {
+    T* P;
+    int f, w, k0;
+    int* lp;
+    RedBuf<NT>* rb;
+    T *X0, *X1, *X2, *U, *DG;
+    // LDS: per column q of the panel (k0 + q) the kind of its pivot step: 0 none yet, 1 a 1x1,
+    // 2 the first and 3 the second column of a 2x2
+    int* ctype;
+    T* Uc;  // U(., j) of the column being formed
+    T* Lr;  // L(r, .) of the pivot row whose U row is being formed
+    double* skey;
+    int* sidx;
+    int scap;
+
+    // x minus the panel's pivots in order, lv[q] = L(row, k0 + q) and u(q) = U(k0 + q, column)
+    // (the operations and their order of ldlt_front's updates)
+    template <class UF>
+    __device__ __attribute__((always_inline)) T apply(T x, const T (&lv)[LP_MAXP], UF u) const {
+#pragma unroll
+        for (int q = 0; q < LP_MAXP; ++q) {
+            const int ct = ctype[q];
+            if (ct == 1) {
+                x = x - lv[q] * u(q);
+            } else if (ct == 2 && q + 1 < LP_MAXP) {
+                x = x - (lv[q] * u(q) + lv[q + 1] * u(q + 1));
+            }
+        }
+        return x;
+    }
+    // L(i, k0 .. k0 + np - 1) into registers, all loads issued before any is used
+    __device__ __attribute__((always_inline)) void load_l(int i, int np, T (&lv)[LP_MAXP]) const {
+#pragma unroll
+        for (int q = 0; q < LP_MAXP; ++q) lv[q] = q < np ? P[(int64_t)(k0 + q) * f + i] : T(0);
+    }
+    // current values of column j at rows k..f-1 into X (all threads; ends with a barrier)
+    __device__ __attribute__((always_inline)) void form_col(int j, int k, T* X) const {
+        const int np = k - k0;
+        for (int q = threadIdx.x; q < np; q += NT) Uc[q] = U[(int64_t)q * w + j];
+        __syncthreads();
+        const T* cj = P + (int64_t)j * f;
+        for (int i = k + threadIdx.x; i < f; i += NT) {
+            T lv[LP_MAXP];
+            load_l(i, np, lv);
+            X[i] = apply(cj[i], lv, [&](int q) { return Uc[q]; });
+        }
+        __syncthreads();
+    }
+    // current diagonal entries of the columns k..w-1 into DG (all threads; ends with a barrier)
+    __device__ __attribute__((always_inline)) void form_diag(int k) const {
+        const int np = k - k0;
+        for (int j = k + threadIdx.x; j < w; j += NT) {
+            T lv[LP_MAXP], uv[LP_MAXP];
+            load_l(j, np, lv);
+#pragma unroll
+            for (int q = 0; q < LP_MAXP; ++q) uv[q] = q < np ? U[(int64_t)q * w + j] : T(0);
+            DG[j] = apply(P[(int64_t)j * f + j], lv, [&](int q) { return uv[q]; });
+        }
+        __syncthreads();
+    }
+    // U row of the pivot row r (a position at or after the current pivot, before the current
+    // pivot's own step is recorded) for the columns j0..w-1 (all threads; ends with a barrier)
+    __device__ __attribute__((always_inline)) void form_urow(int r, int j0) const {
+        const int np = r - k0;
+        for (int q = threadIdx.x; q < LP_MAXP; q += NT) Lr[q] = q < np ? P[(int64_t)(k0 + q) * f + r] : T(0);
+        __syncthreads();
+        T lv[LP_MAXP];
+#pragma unroll
+        for (int q = 0; q < LP_MAXP; ++q) lv[q] = Lr[q];
+        T* ur = U + (int64_t)(r - k0) * w;
+        for (int j = j0 + threadIdx.x; j < w; j += NT) {
+            T uv[LP_MAXP];
+#pragma unroll
+            for (int q = 0; q < LP_MAXP; ++q) uv[q] = q < np ? U[(int64_t)q * w + j] : T(0);
+            ur[j] = apply(P[(int64_t)j * f + r], lv, [&](int q) { return uv[q]; });
+        }
+        __syncthreads();
+    }
+    // largest |X[i]| over the rows k..f-1 but s1 and s2, NaN-propagating (ldlt.jl:240-260)
+    __device__ __attribute__((always_inline)) double colmax(const T* X, int k, int s1, int s2) const {
+        double m = 0.0;
+        for (int i = k + threadIdx.x; i < f; i += NT)
+            if (i != s1 && i != s2) m = jmax(m, fabs((double)X[i]));
+        return block_jmax<NT>(m, *rb);
+    }
+    // the same for the current column j, formed by one wave (lanes along the rows)
+    __device__ __attribute__((always_inline)) double wave_colmax(int j, int k, int s1) const {
+        const int lane = threadIdx.x & 63, np = k - k0;
+        T uv[LP_MAXP];
+#pragma unroll
+        for (int q = 0; q < LP_MAXP; ++q) uv[q] = q < np ? U[(int64_t)q * w + j] : T(0);
+        double m = 0.0;
+        const T* cj = P + (int64_t)j * f;
+        for (int i = k + lane; i < f; i += 64) {
+            T lv[LP_MAXP];
+            load_l(i, np, lv);
+            const T x = apply(cj[i], lv, [&](int q) { return uv[q]; });
+            if (i != s1) m = jmax(m, fabs((double)x));
+        }
+        return wave_jmax(m);
+    }
+    __device__ __attribute__((always_inline)) bool accept_1x1(const T* X, int c, int k, const LdltPrm& p) const {
+        const double d = fabs((double)X[c]);
+        if (!(d >= p.eps)) return false;
+        return d >= p.u * colmax(X, k, c, -1);
+    }
+    // the 2x2 block (k, r) from the current columns k (Xk) and r (Xr)
+    __device__ __attribute__((always_inline)) bool nonsingular_2x2(const T* Xk, const T* Xr, int k, int r, const LdltPrm& p) const {
+        const double akk = Xk[k], arr = Xr[r], akr = Xr[k], ark = Xk[r];
+        const double det = akk * arr - akr * ark;
+        const double mx = jmax(jmax(fabs(akk), fabs(ark)), fabs(arr));
+        return det != 0.0 && isfinite(det) && mx >= p.eps;
+    }
+    __device__ __attribute__((always_inline)) bool accept_2x2(const T* Xk, const T* Xr, int k, int r, const LdltPrm& p) const {
+        if (!nonsingular_2x2(Xk, Xr, k, r, p)) return false;
+        const double akk = Xk[k], arr = Xr[r], akr = Xr[k], ark = Xk[r];
+        const double det = akk * arr - akr * ark;
+        const double e11 = fabs(arr / det), e12 = fabs(akr / det), e21 = fabs(ark / det), e22 = fabs(akk / det);
+        const double m1 = colmax(Xk, k, k, r), m2 = colmax(Xr, k, r, k);
+        return p.u * (e11 * m1 + e21 * m2) <= 1.0 && p.u * (e12 * m1 + e22 * m2) <= 1.0;
+    }
+    // best_1x1 (ldlt.jl:264-276) on the current diagonal DG (formed here): the acceptable 1x1
+    // pivot of k..w-1 with the largest |a_jj|, first on ties, -1 if none. The largest
+    // candidate is tested first by the whole block (its column formed in X2, which holds it on
+    // return); else the candidates in sorted order, sixteen at a time, one wave per column.
+    __device__ __attribute__((always_inline)) int best_1x1(int k, const LdltPrm& p) const {
+        form_diag(k);
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        double v = -1.0;
+        int bi = -1;
+        for (int j = k + tid; j < w; j += NT) {
+            const double a = fabs((double)DG[j]);
+            if (a >= p.eps && vi_better(a, j, v, bi)) {
+                v = a;
+                bi = j;
+            }
+        }
+        block_argmax<NT>(v, bi, *rb);
+        if (bi < 0) return -1;
+        form_col(bi, k, X2);
+        if (v >= p.u * colmax(X2, k, bi, -1)) return bi;
+        const int n = w - k;
+        if (n <= scap) {
+            int np = 1;
+            while (np < n) np <<= 1;
+            for (int q = tid; q < np; q += NT) {
+                double key = -1.0;
+                if (q < n) {
+                    const double a = fabs((double)DG[k + q]);
+                    key = a >= p.eps ? a : -1.0;
+                }
+                skey[q] = key;
+                sidx[q] = k + q;
+            }
+            __syncthreads();
+            for (int size = 2; size <= np; size <<= 1)
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = tid; i < np; i += NT) {
+                        const int l = i ^ stride;
+                        if (l <= i) continue;
+                        const bool lbefore = skey[l] > skey[i] || (skey[l] == skey[i] && sidx[l] < sidx[i]);
+                        const bool ibefore = skey[i] > skey[l] || (skey[i] == skey[l] && sidx[i] < sidx[l]);
+                        if ((i & size) == 0 ? lbefore : ibefore) {
+                            const double tk = skey[i];
+                            skey[i] = skey[l];
+                            skey[l] = tk;
+                            const int ti = sidx[i];
+                            sidx[i] = sidx[l];
+                            sidx[l] = ti;
+                        }
+                    }
+                    __syncthreads();
+                }
+            // the first entry is bi, already rejected
+            for (int base = 1; base < n && skey[base] >= 0.0; base += NT / 64) {
+                const int q = base + wv;
+                int found = 0x7fffffff;
+                if (q < n && skey[q] >= 0.0) {
+                    const int j = sidx[q];
+                    const double m = wave_colmax(j, k, j);
+                    if (skey[q] >= p.u * m) found = q;
+                }
+                if (lane == 0) rb->i[wv] = found;
+                __syncthreads();
+                int best = rb->i[0];
+                for (int w2 = 1; w2 < NT / 64; ++w2) best = min(best, rb->i[w2]);
+                __syncthreads();
+                if (best != 0x7fffffff) {
+                    const int j = sidx[best];
+                    form_col(j, k, X2);
+                    return j;
+                }
+            }
+            return -1;
+        }
+        // too many candidates to sort: every column, one wave each; a column that cannot beat
+        // its wave's current best is not tested
+        double bv = 0.0;
+        int bj = -1;
+        for (int j = k + wv; j < w; j += NT / 64) {
+            const double a = fabs((double)DG[j]);
+            if (!(a >= p.eps)) continue;
+            if (bj >= 0 && !(a > bv)) continue;
+            if (!(a >= p.u * wave_colmax(j, k, j))) continue;
+            bv = a;
+            bj = j;
+        }
+        if (lane == 0) {
+            rb->v[wv] = bv;
+            rb->i[wv] = bj;
+        }
+        __syncthreads();
+        double rv = rb->v[0];
+        int ri = rb->i[0];
+        for (int q = 1; q < NT / 64; ++q)
+            if (vi_better(rb->v[q], rb->i[q], rv, ri)) {
+                rv = rb->v[q];
+                ri = rb->i[q];
+            }
+        __syncthreads();
+        if (ri >= 0) form_col(ri, k, X2);
+        return ri;
+    }
+    // pivot choice at column k (ldlt.jl:280-325), as Front::choose on the current columns:
+    // c = pivot column, r = 2x2 partner or -1; on return the current column c is in *Xc (1x1),
+    // or the columns k and r are in X0 and X1 (2x2)
+    __device__ __attribute__((always_inline)) void choose(int k, const LdltPrm& p, int& c, int& r, T*& Xc) const {
+        r = -1;
+        form_col(k, k, X0);
+        Xc = X0;
+        if (p.pivot == PV_NONE) {
+            c = k;
+            return;
+        }
+        if (p.pivot == PV_DIAG) {
+            if (accept_1x1(X0, k, k, p)) {
+                c = k;
+                return;
+            }
+            const int best = best_1x1(k, p);
+            if (best >= 0) {
+                c = best;
+                Xc = X2;
+                return;
+            }
+            // the largest diagonal entry, first on ties; a NaN |a_kk| keeps k (DG was formed by
+            // best_1x1)
+            double v = fabs((double)X0[k]);
+            int bi = k;
+            for (int j = k + 1 + threadIdx.x; j < w; j += NT) {
+                const double a = fabs((double)DG[j]);
+                if (a > v) {
+                    v = a;
+                    bi = j;
+                }
+            }
+            block_argmax<NT>(v, bi, *rb);
+            c = fabs((double)DG[bi]) >= p.eps ? bi : k;
+            if (c != k) {
+                form_col(c, k, X2);
+                Xc = X2;
+            }
+            return;
+        }
+        const double alpha = (1.0 + sqrt(17.0)) / 8.0;  // Bunch-Kaufman constant (ldlt.jl:22)
+        double lambda = 0.0;
+        int rr = -1;
+        for (int i = k + 1 + threadIdx.x; i < w; i += NT) {
+            const double a = fabs((double)X0[i]);
+            if (a > lambda) {
+                lambda = a;
+                rr = i;
+            }
+        }
+        block_argmax<NT>(lambda, rr, *rb);
+        if (rr < 0) lambda = 0.0;
+        const double akk = fabs((double)X0[k]);
+        int bc = k, br = -1;
+        T* Xb = X0;
+        if (rr < 0 || akk >= alpha * lambda) {
+            bc = k;
+        } else {
+            form_col(rr, k, X1);
+            double sigma = 0.0;
+            for (int i = k + threadIdx.x; i < w; i += NT)
+                if (i != rr) sigma = jmax(sigma, fabs((double)X1[i]));
+            sigma = block_jmax<NT>(sigma, *rb);
+            if (akk * sigma >= alpha * lambda * lambda) {
+                bc = k;
+            } else if (fabs((double)X1[rr]) >= alpha * sigma) {
+                bc = rr;
+                Xb = X1;
+            } else {
+                bc = k;
+                br = rr;
+            }
+        }
+        const bool ok = br < 0 ? accept_1x1(Xb, bc, k, p) : accept_2x2(X0, X1, k, br, p);
+        if (ok) {
+            c = bc;
+            r = br;
+            Xc = Xb;
+            return;
+        }
+        const int best = best_1x1(k, p);
+        if (best >= 0) {
+            c = best;
+            Xc = X2;
+            return;
+        }
+        if (br >= 0 && nonsingular_2x2(X0, X1, k, br, p)) {
+            c = bc;
+            r = br;
+            return;
+        }
+        c = bc;
+        Xc = Xb;
+    }
+    // symmetric interchange of positions a and b (ldlt.jl:328-340) on the stored values, the
+    // pivot rows' entries of those columns, and the rows of the current columns held in Xa, Xb
+    __device__ __attribute__((always_inline)) void swap(int a, int b, int k, T* Xa, T* Xb) const {
+        __syncthreads();  // the pivot choice's last reads are done
+        if (a == b) return;
+        for (int i = threadIdx.x; i < f; i += NT) {
+            const T t = P[(int64_t)a * f + i];
+            P[(int64_t)a * f + i] = P[(int64_t)b * f + i];
+            P[(int64_t)b * f + i] = t;
+        }
+        for (int q = threadIdx.x; q < k - k0; q += NT) {
+            const T t = U[(int64_t)q * w + a];
+            U[(int64_t)q * w + a] = U[(int64_t)q * w + b];
+            U[(int64_t)q * w + b] = t;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < w; j += NT) {
+            const T t = P[(int64_t)j * f + a];
+            P[(int64_t)j * f + a] = P[(int64_t)j * f + b];
+            P[(int64_t)j * f + b] = t;
+        }
+        if (threadIdx.x == 0) {
+            const int t = lp[a];
+            lp[a] = lp[b];
+            lp[b] = t;
+            if (Xa) {
+                const T x = Xa[a];
+                Xa[a] = Xa[b];
+                Xa[b] = x;
+            }
+            if (Xb) {
+                const T x = Xb[a];
+                Xb[a] = Xb[b];
+                Xb[b] = x;
+            }
+        }
+        __syncthreads();
+    }
+};
+
+// panel e of every large front of a level: its pivots until LP_NB columns are taken or the
+// front is done (the first panel also sets up the front). task: (s, scratch offset)
+template <typename T>
+__global__ void __launch_bounds__(512) k_ldlt_panel(DevSym S, const Task* __restrict__ tasks, int e,
+                                                    T* __restrict__ factor, T* __restrict__ dvec,
+                                                    int32_t* __restrict__ piv, int8_t* __restrict__ pkind,
+                                                    int64_t* __restrict__ stats, int32_t* __restrict__ lstep,
+                                                    T* __restrict__ scratch, LdltPrm prm, int scaled,
+                                                    const double* __restrict__ aux, const int8_t* __restrict__ psign) {
+    constexpr int NT = 512;
+    constexpr int SCAP = 2048;
+    __shared__ RedBuf<NT> rb;
+    __shared__ double sh[4];
+    __shared__ double skey[SCAP];
+    __shared__ int sidx[SCAP];
+    __shared__ int ctype[LP_MAXP];
+    __shared__ T Uc[LP_MAXP], Lr[LP_MAXP];
+    const Task tk = tasks[blockIdx.x];
+    const int s = tk.s;
+    const int f = S.fnrows[s], w = S.fncols[s];
+    const int c0 = S.super_ptr[s];
+    const int tid = threadIdx.x;
+    prm.eps = eps_eff<T>(prm.eps, scaled, aux);
+    T* P = factor + S.panel_off[s];
+    int* lp = piv + c0;
+    int8_t* kind = pkind + c0;
+    T* dd = dvec + c0;
+    T* ee = dvec + S.n + c0;
+    int64_t* st = stats + (int64_t)s * NF;
+    int32_t* ls = lstep + 3 * s;
+    int k;
+    if (e == 0) {
+        for (int q = tid; q < w; q += NT) {
+            lp[q] = q;
+            kind[q] = 0;
+            dd[q] = ee[q] = T(0);
+        }
+        if (tid < NF) st[tid] = 0;
+        // the upper triangle of the fully summed square mirrors the lower (the reference's
+        // full square)
+        for (int q = tid; q < w * w; q += NT) {
+            const int j = q / w, i = q - j * w;
+            if (i > j) P[(int64_t)i * f + j] = P[(int64_t)j * f + i];
+        }
+        k = 0;
+    } else {
+        k = ls[2];
+    }
+    __syncthreads();
+    if (k >= w) {
+        if (tid == 0) ls[1] = 0;
+        return;
+    }
+    const int k0 = k;
+    for (int q = tid; q < LP_MAXP; q += NT) ctype[q] = 0;
+    T* base = scratch + tk.a;
+    PanelFront<T, NT> Z{P, f, w, k0, lp, &rb, base, base + f, base + 2 * (int64_t)f, base + 3 * (int64_t)f,
+                        base + 3 * (int64_t)f + (int64_t)LP_MAXP * w, ctype, Uc, Lr, skey, sidx, SCAP};
+    const int8_t* sreq_row = psign;
+    const int32_t* perm_c0 = S.perm + c0;
+    __syncthreads();
+    while (k < w && k - k0 < LP_NB) {
+        int c, r;
+        T* Xc;
+        Z.choose(k, prm, c, r, Xc);
+        if (r < 0) {
+            Z.swap(k, c, k, Xc, nullptr);
+            if (tid == 0) {
+                double d = (double)Xc[k];
+                int8_t kd = KIND_1X1;
+                if (!(fabs(d) >= prm.eps) || d == 0.0) {
+                    if (d == 0.0) st[ST_NZERO]++;
+                    const int req = sreq_row ? (int)sreq_row[perm_c0[lp[k]]] : 0;
+                    const double sg = req != 0 ? (double)req : (d < 0.0 ? -1.0 : 1.0);
+                    d = sg * prm.eps;
+                    kd = KIND_PERTURBED;
+                    st[ST_NPERT]++;
+                    if (d == 0.0 && st[ST_INFO] == 0) st[ST_INFO] = lp[k] + 1;
+                }
+                P[(int64_t)k * f + k] = T(d);
+                kind[k] = kd;
+                dd[k] = T(d);
+                ee[k] = T(0);
+                if (d > 0.0)
+                    st[ST_NPOS]++;
+                else if (d < 0.0)
+                    st[ST_NNEG]++;
+                sh[0] = d;
+            }
+            __syncthreads();
+            const T d = T(sh[0]);
+            for (int i = k + 1 + tid; i < f; i += NT) P[(int64_t)k * f + i] = Xc[i] / d;
+            __syncthreads();
+            Z.form_urow(k, k + 1);
+            if (tid == 0) ctype[k - k0] = 1;
+            __syncthreads();
+            k += 1;
+        } else {
+            Z.swap(k + 1, r, k, Z.X0, Z.X1);
+            if (tid == 0) {
+                const T a = Z.X0[k], b = Z.X0[k + 1], c2 = Z.X1[k + 1];
+                const T up = b;
+                const T det = a * c2 - up * b;
+                sh[0] = (double)(c2 / det);
+                sh[1] = (double)(-up / det);
+                sh[2] = (double)(-b / det);
+                sh[3] = (double)(a / det);
+                kind[k] = KIND_2X2_FIRST;
+                kind[k + 1] = KIND_2X2_SECOND;
+                dd[k] = a;
+                dd[k + 1] = c2;
+                ee[k] = b;
+                ee[k + 1] = T(0);
+                st[ST_N2X2]++;
+                if (det < T(0)) {
+                    st[ST_NPOS]++;
+                    st[ST_NNEG]++;
+                } else if (a > T(0)) {
+                    st[ST_NPOS] += 2;
+                } else {
+                    st[ST_NNEG] += 2;
+                }
+                P[(int64_t)k * f + k] = a;
+                P[(int64_t)(k + 1) * f + k + 1] = c2;
+                P[(int64_t)k * f + k + 1] = T(0);  // the reference clears it
+            }
+            __syncthreads();
+            const T e11 = T(sh[0]), e12 = T(sh[1]), e21 = T(sh[2]), e22 = T(sh[3]);
+            for (int i = k + 2 + tid; i < f; i += NT) {
+                const T x1 = Z.X0[i], x2 = Z.X1[i];
+                P[(int64_t)k * f + i] = x1 * e11 + x2 * e21;
+                P[(int64_t)(k + 1) * f + i] = x1 * e12 + x2 * e22;
+            }
+            __syncthreads();
+            Z.form_urow(k, k + 2);
+            Z.form_urow(k + 1, k + 2);
+            if (tid == 0) {
+                ctype[k - k0] = 2;
+                ctype[k + 1 - k0] = 3;
+            }
+            __syncthreads();
+            k += 2;
+        }
+    }
+    if (tid == 0) {
+        ls[0] = k0;
+        ls[1] = k - k0;
+        ls[2] = k;
+    }
+}
+
+// the update of panel e's pivots on the columns after it: the tile rows [I0, I0 + 64) x columns
+// [J0, J0 + 64) of the entries at or after the panel's end (every row below, the fully summed
+// square's upper mirror included), each entry applying the panel's pivots in order as ldlt_front
+// does. task: (s, I0, J0, scratch offset)
+template <typename T>
+__global__ void __launch_bounds__(256) k_ldlt_trail(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor,
+                                                   const int8_t* __restrict__ pkind, const int32_t* __restrict__ lstep,
+                                                   const T* __restrict__ scratch) {
+    __shared__ T Ls[LP_MAXP][TILE];
+    __shared__ T Us[LP_MAXP][TILE];
+    __shared__ int stp[LP_MAXP], ssz[LP_MAXP], nsh;
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    const int32_t* ls = lstep + 3 * s;
+    const int np = ls[1];
+    if (np == 0) return;
+    const int k0 = ls[0], kend = ls[2];
+    const int I0 = t.a, J0 = t.b;
+    if (I0 + TILE <= kend || J0 + TILE <= kend || I0 >= f || J0 >= w) return;
+    const T* P = factor + S.panel_off[s];
+    T* Pw = factor + S.panel_off[s];
+    const int8_t* kind = pkind + S.super_ptr[s];
+    const T* U = scratch + t.c + 3 * (int64_t)f;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < np * TILE; q += 256) {
+        const int pp = q / TILE, c = q - pp * TILE;
+        const int i = I0 + c, j = J0 + c;
+        Ls[pp][c] = i < f ? P[(int64_t)(k0 + pp) * f + i] : T(0);
+        Us[pp][c] = j < w ? U[(int64_t)pp * w + j] : T(0);
+    }
+    if (tid == 0) {
+        int n = 0;
+        for (int p = k0; p < kend; ++p) {
+            if (kind[p] == KIND_2X2_SECOND) continue;
+            stp[n] = p - k0;
+            ssz[n] = kind[p] == KIND_2X2_FIRST ? 2 : 1;
+            ++n;
+        }
+        nsh = n;
+    }
+    __syncthreads();
+    const int ns = nsh;
+    const int tx = tid & 15, ty = tid >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int r = 4 * ty + a, i = I0 + r;
+        if (i < kend || i >= f) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = 4 * tx + b, j = J0 + c;
+            if (j < kend || j >= w) continue;
+            T x = P[(int64_t)j * f + i];
+            for (int q = 0; q < ns; ++q) {
+                const int pp = stp[q];
+                if (ssz[q] == 1)
+                    x = x - Ls[pp][r] * Us[pp][c];
+                else
+                    x = x - (Ls[pp][r] * Us[pp][c] + Ls[pp + 1][r] * Us[pp + 1][c]);
+            }
+            Pw[(int64_t)j * f + i] = x;
+        }
+    }
+}
+
 // after the last pair: unit diagonal, empty upper triangle and the global pivot order.
 // task: (s)
 template <typename T>
@@ -856,6 +1453,24 @@ void launch_ldlt_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n
 }
 
 template <typename T>
+void launch_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, int e, T* factor, T* dvec,
+                       int32_t* piv, int8_t* kind, int64_t* stats, int32_t* lstep, T* scratch, int pivot, double u,
+                       double eps, int scaled, const double* aux, const int8_t* psign) {
+    LdltPrm p{pivot, u, eps};
+    hipLaunchKernelGGL(k_ldlt_panel<T>, dim3(n), dim3(512), 0, st, S, tasks, e, factor, dvec, piv, kind, stats,
+                       lstep, scratch, p, scaled, aux, psign);
+}
+
+template <typename T>
+void launch_ldlt_trail(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int8_t* kind,
+                       const int32_t* lstep, const T* scratch) {
+    hipLaunchKernelGGL(k_ldlt_trail<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, kind, lstep, scratch);
+}
+
+int64_t ldlt_panel_scratch(int f, int w) { return lp_scratch(f, w); }
+int ldlt_panel_cols() { return LP_NB; }
+
+template <typename T>
 void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep) {
     hipLaunchKernelGGL(k_ldlt_update<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, lstep);
 }
@@ -893,6 +1508,11 @@ void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stat
                                        int8_t*, int64_t*, int32_t*, int, double, double, int, const double*,       \
                                        const int8_t*);                                                             \
     template void launch_ldlt_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int32_t*);        \
+    template void launch_ldlt_panel<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, int32_t*,        \
+                                       int8_t*, int64_t*, int32_t*, T*, int, double, double, int, const double*,   \
+                                       const int8_t*);                                                             \
+    template void launch_ldlt_trail<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int8_t*,           \
+                                       const int32_t*, const T*);                                                  \
     template void launch_ldlt_finish<T>(hipStream_t, const DevSym&, const Task*, int, T*, int32_t*);
 INST(double)
 INST(float)

@@ -60,6 +60,8 @@ enum LaunchKind {
     L_LDLT_PIVOT,    // k_ldlt_pivot: launch pair k's pivot of every large front of the level
     L_LDLT_UPDATE,   // k_ldlt_update: that pivot's update tiles
     L_LDLT_FINISH,   // k_ldlt_finish: unit diagonals and the pivot order
+    L_LDLT_PANEL,    // k_ldlt_panel: a panel of pivots of every large front of the level
+    L_LDLT_TRAIL,    // k_ldlt_trail: that panel's update of the columns after it
     L_LU_PIVOT,      // k_lu_pivot: step k's pivot of every large front of the level
     L_LU_UPDATE,     // k_lu_update: step k's rank-1 update tiles
     L_LU_FINISH,     // k_lu_finish: the last deferred scaling, U^T and unit diagonals
@@ -124,6 +126,8 @@ int prof_kind(int kind) {
         case L_LDLT_PIVOT: return SDS_K_PIVOT_PANEL;
         case L_LDLT_UPDATE: return SDS_K_PIVOT_PANEL;
         case L_LDLT_FINISH: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_PANEL: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_TRAIL: return SDS_K_UPDATE_PANEL;
         case L_LU_UPDATE: return SDS_K_PIVOT_PANEL;
         case L_LU_FINISH: return SDS_K_PIVOT_PANEL;
         case L_LU_ZERO: return SDS_K_ZERO_SCATTER;
@@ -228,6 +232,9 @@ struct sds_handle {
     int small_kind = 0;  // small-front kernels (Cholesky): 0 one wave per tiny front + blocked,
                          // 1 one wave per tiny front + column by column, 2 column by column
     int panel_block = 0;  // Cholesky large fronts: panel block width (a multiple of NB; 0 = one block)
+    int pivot_panels = 0;  // LDL^T large fronts: 1 deferred-update panels, 0 one pivot per launch pair
+    int64_t ldlt_scr = 0;  // ... the panels' scratch (elements)
+    void* lscratch = nullptr;
     int32_t* status = nullptr;
     void* dvec = nullptr;        // D: 2n entries of the factor type (diagonal, 2x2 subdiagonals)
     int32_t* piv = nullptr;      // n: local pivot order as 0-based factor columns
@@ -470,6 +477,7 @@ int build_schedule(sds_handle* h, int dtype) {
     for (int64_t s = 0; s < ns; ++s) nlev = std::max(nlev, lev[s] + 1);
     h->levels.assign(nlev, Level{});
     h->max_rest = 0;
+    h->ldlt_scr = 0;
     std::vector<char> big(ns, 0);
     // regime A (Cholesky, sub_kib > 0): leaf subtrees of small fronts whose LDS stack fits the
     // budget, one workgroup each, walked in postorder before the levels. A front sits above its
@@ -772,6 +780,44 @@ int build_schedule(sds_handle* h, int dtype) {
                 }
             }
             L.plan.push_back({L_LU_FINISH, off_p, (int)tp.size(), 0.0, 0.0});
+            wmax = 0;
+        } else if (h->method != M_CHOLESKY && h->pivot_panels) {
+            // LDL^T of the large fronts in panels of deferred updates: per panel e one launch takes
+            // each front's next pivots until ldlt_panel_cols() columns are done (one workgroup per
+            // front, its scratch at the offset in the task), then one launch applies them to the
+            // entries after the panel (64 x 64 tiles). A panel takes at least that many columns,
+            // so ceil(wmax / cols) panels finish every front of the level.
+            const int nbp = ldlt_panel_cols();
+            std::vector<Task> tp;
+            int64_t off = 0;
+            for (int32_t s : L.big) {
+                tp.push_back({s, (int32_t)off, 0, 0});
+                off += ldlt_panel_scratch(h->fnrows[s], h->fncols[s]);
+            }
+            h->ldlt_scr = std::max(h->ldlt_scr, off);
+            const int64_t off_p = (int64_t)tasks.size();
+            tasks.insert(tasks.end(), tp.begin(), tp.end());
+            const int npan = (wmax + nbp - 1) / nbp;
+            for (int e = 0; e < npan; ++e) {
+                std::vector<Task> tu;
+                double fl1 = 0, by1 = 0;
+                const int kmin = (e + 1) * nbp, lo = kmin / TILE * TILE;
+                for (size_t q = 0; q < L.big.size(); ++q) {
+                    const int32_t s = L.big[q];
+                    const int f = h->fnrows[s], w = h->fncols[s];
+                    if (kmin >= w) continue;
+                    for (int J0 = lo; J0 < w; J0 += TILE)
+                        for (int I0 = lo; I0 < f; I0 += TILE) tu.push_back({s, I0, J0, tp[q].a});
+                    // a panel of nbp 1x1 pivots on the (f - kmin) x (w - kmin) entries after it
+                    const double ent = (double)(f - kmin) * (w - kmin);
+                    fl1 += 2.0 * nbp * ent;
+                    by1 += 2.0 * dbl_es * ent;
+                }
+                L.plan.push_back({L_LDLT_PANEL, off_p, (int)tp.size(), 0.0, 0.0});
+                L.plan.back().k = e;
+                push_launch(L, L_LDLT_TRAIL, tu, fl1, by1);
+            }
+            L.plan.push_back({L_LDLT_FINISH, off_p, (int)tp.size(), 0.0, 0.0});
             wmax = 0;
         } else if (h->method != M_CHOLESKY) {
             // LDL^T of the large fronts, one pivot (1x1 or 2x2) per launch pair over all of them:
@@ -1299,6 +1345,15 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
                                              h->lstep, pivot, u, eps, scaled, h->aux, psign);
                         break;
                     case L_LDLT_UPDATE: launch_ldlt_update<T>(st, h->dsym, t, ln.n, factor, h->lstep); break;
+                    case L_LDLT_PANEL:
+                        launch_ldlt_panel<T>(st, h->dsym, t, ln.n, ln.k, factor, dv, h->piv, h->pkind, h->stats,
+                                             h->lstep, static_cast<T*>(h->lscratch), pivot, u, eps, scaled, h->aux,
+                                             psign);
+                        break;
+                    case L_LDLT_TRAIL:
+                        launch_ldlt_trail<T>(st, h->dsym, t, ln.n, factor, h->pkind, h->lstep,
+                                             static_cast<const T*>(h->lscratch));
+                        break;
                     case L_LDLT_FINISH: launch_ldlt_finish<T>(st, h->dsym, t, ln.n, factor, h->piv); break;
                     case L_LDLT_CB: launch_ldlt_cb<T>(st, h->dsym, t, ln.n, factor, cb, dv, h->pkind); break;
                 }
@@ -2018,6 +2073,8 @@ int sds_alloc_numeric(sds_handle_t h, int32_t dtype, int32_t nbatch) {
     if ((rc = dev_alloc(h, h->numbufs, es * NB * std::max<int64_t>(h->n, 1), &h->lnbuf))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * 3 * (h->ns + 1), &p))) return rc;
     h->lstep = static_cast<int32_t*>(p);
+    h->lscratch = nullptr;
+    if (h->ldlt_scr > 0 && (rc = dev_alloc(h, h->numbufs, es * h->ldlt_scr, &h->lscratch))) return rc;
     if ((rc = dev_alloc(h, h->numbufs, sizeof(int32_t) * (h->ns + 1), &p))) return rc;
     h->status = static_cast<int32_t*>(p);
     if ((rc = dev_alloc(h, h->numbufs, es * 2 * std::max<int64_t>(h->n, 1), &h->dvec))) return rc;
@@ -2207,6 +2264,7 @@ int sds_autotune(sds_handle_t h, const double* nzval, const sds_factor_params* p
         tiles[4] = pick.sk;
         tiles[5] = pick.sb;
         tiles[6] = pick.pb;
+        tiles[7] = h->pivot_panels;
     }
     return SDS_SUCCESS;
 }
@@ -2215,7 +2273,7 @@ int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     if (!h) return SDS_INVALID_VALUE;
     if (!tiles) return fail(h, SDS_INVALID_VALUE, "tiles is NULL");
     const int ea = tiles[0], kc = tiles[1], rs = tiles[2], oc = tiles[3], sk = tiles[4], sb = tiles[5],
-              pb = tiles[6];
+              pb = tiles[6], pp = tiles[7];
     if (ea != 512 && ea != 1024 && ea != 2048 && ea != 4096)
         return fail(h, SDS_INVALID_VALUE, "extend-add rows must be 512, 1024, 2048 or 4096");
     if (kc != 16 && kc != 32 && kc != 64) return fail(h, SDS_INVALID_VALUE, "the K chunk must be 16, 32 or 64");
@@ -2225,6 +2283,7 @@ int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
         return fail(h, SDS_INVALID_VALUE, "the subtree budget must be 0, 16, 32 or 64 (KiB)");
     if (pb < 0 || pb % NB != 0 || pb > 4096)
         return fail(h, SDS_INVALID_VALUE, "the panel block must be 0 or a multiple of 64 up to 4096");
+    if (pp != 0 && pp != 1) return fail(h, SDS_INVALID_VALUE, "pivot panels must be 0 or 1");
     h->ea_rows = ea;
     h->upd_kc = kc;
     h->rest_split = rs;
@@ -2232,6 +2291,7 @@ int sds_set_tiles(sds_handle_t h, const int32_t* tiles) {
     h->small_kind = sk;
     h->sub_kib = sb;
     h->panel_block = pb;
+    h->pivot_panels = pp;
     return SDS_SUCCESS;
 }
 

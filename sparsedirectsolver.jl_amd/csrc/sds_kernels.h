@@ -64,6 +64,18 @@ void launch_ldlt_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n
                        int scaled, const double* aux, const int8_t* psign);
 template <typename T>
 void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep);
+// deferred-update panels of the large fronts: panel e of every front (task (s, scratch offset)),
+// then its update of the columns after it (task (s, I0, J0, scratch offset))
+template <typename T>
+void launch_ldlt_panel(hipStream_t st, const DevSym& S, const Task* tasks, int n, int e, T* factor, T* dvec,
+                       int32_t* piv, int8_t* kind, int64_t* stats, int32_t* lstep, T* scratch, int pivot, double u,
+                       double eps, int scaled, const double* aux, const int8_t* psign);
+template <typename T>
+void launch_ldlt_trail(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int8_t* kind,
+                       const int32_t* lstep, const T* scratch);
+// scratch elements of one front's panels, and the columns of a panel
+int64_t ldlt_panel_scratch(int f, int w);
+int ldlt_panel_cols();
 template <typename T>
 void launch_ldlt_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, int32_t* piv);
 template <typename T>

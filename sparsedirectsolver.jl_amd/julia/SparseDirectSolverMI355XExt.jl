@@ -173,10 +173,10 @@ function solve!(h::Handle, phase::Phase, X::ROCMatrix{Float64}, B::ROCMatrix{Flo
 end
 
 # the library's tile-shape autotuner: {extend-add rows, K chunk, trailing split, two-per-CU steps,
-# small-front kernels, subtree KiB, panel block}; the numeric storage is reallocated, so factorize
-# again afterwards
+# small-front kernels, subtree KiB, panel block, pivot panels}; the numeric storage is reallocated,
+# so factorize again afterwards
 function autotune!(h::Handle, nzval::ROCVector{Float64}, opts; reps::Integer = 3)
-    tiles = zeros(Int32, 7)
+    tiles = zeros(Int32, 8)
     GC.@preserve opts tiles begin
         p = _params(opts)
         _check(h, ccall((:sds_autotune, libsds_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ref{FactorParams}, Cint,

@@ -472,26 +472,28 @@ def autotune(solver, reps=2):
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, 'autotune needs "analysis"')
     p, keep = solver._factor_params()
-    tiles = (ctypes.c_int32 * 7)()
+    tiles = (ctypes.c_int32 * 8)()
     _check(solver._lib, solver._h, solver._lib.sds_autotune(solver._h, ctypes.c_void_p(solver.nzval.data_ptr()),
                                                              ctypes.byref(p), int(reps), tiles))
     del keep
     solver.stage = STAGE_ANALYZED
     return {"extend_add_rows": int(tiles[0]), "cb_k_chunk": int(tiles[1]), "trailing_split": int(tiles[2]),
             "step_two_per_cu": int(tiles[3]), "small_front_kernel": int(tiles[4]), "subtree_kib": int(tiles[5]),
-            "panel_block": int(tiles[6])}
+            "panel_block": int(tiles[6]), "pivot_panels": int(tiles[7])}
 
 
 def set_tiles(solver, tiles):
     """Set the front kernels' tile shapes as `autotune` returns them (a dict with the keys
     "extend_add_rows", "cb_k_chunk", "trailing_split", "step_two_per_cu" and optionally
-    "small_front_kernel", "subtree_kib" and "panel_block", default 0; libsds_hip's sds_set_tiles) and
-    reallocate the numeric storage for them: the solver then needs a "factorization" again."""
+    "small_front_kernel", "subtree_kib", "panel_block" and "pivot_panels", default 0;
+    libsds_hip's sds_set_tiles) and reallocate the numeric storage for them: the solver then needs a
+    "factorization" again."""
     if solver.stage < STAGE_ANALYZED:
         raise FactorizationError(0, 'set_tiles needs "analysis"')
-    t = (ctypes.c_int32 * 7)(int(tiles["extend_add_rows"]), int(tiles["cb_k_chunk"]), int(tiles["trailing_split"]),
+    t = (ctypes.c_int32 * 8)(int(tiles["extend_add_rows"]), int(tiles["cb_k_chunk"]), int(tiles["trailing_split"]),
                              int(tiles["step_two_per_cu"]), int(tiles.get("small_front_kernel", 0)),
-                             int(tiles.get("subtree_kib", 0)), int(tiles.get("panel_block", 0)))
+                             int(tiles.get("subtree_kib", 0)), int(tiles.get("panel_block", 0)),
+                             int(tiles.get("pivot_panels", 0)))
     _check(solver._lib, solver._h, solver._lib.sds_set_tiles(solver._h, t))
     fp = solver.options["factor_precision"]
     dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64

@@ -159,3 +159,38 @@ def test_ldlt_fp32_refinement():
     sds.execute("solve", s, X, B)
     berr = _berr(K, X.cpu().numpy(), B.cpu().numpy())
     assert berr <= 1e-10, berr
+
+
+PANEL_CASES = [c for c in CASES if c[0] in ("symindef_300_amd_big", "symindef_200_diag", "spd_150_none",
+                                            "kkt_400_200_delta0_amd", "kkt_config_1500_750_amd",
+                                            "symrand_400_bk_big", "symrand_250_diag_big", "symindef_160_one_front")]
+
+
+@pytest.mark.parametrize("fp", [None, "Float32"])
+@pytest.mark.parametrize("name,make,perm,amal,ptype,thr", PANEL_CASES, ids=[c[0] for c in PANEL_CASES])
+def test_pivot_panels_bitwise_as_one_pivot_per_launch(name, make, perm, amal, ptype, thr, fp):
+    """The large fronts in panels of deferred updates against one pivot per pair of launches (the
+    default): pivot order, kinds, D, statistics and panels bitwise the same, in FP64 and in FP32
+    (whose decisions read FP32 values): every entry receives the same operations in the same
+    order, only later. Also against the CPU oracle in FP64."""
+    A = make()
+    print(f"case {name} n={A.shape[0]} {fp or 'Float64'}", flush=True)
+    s, nz = _solver(A, perm, amal, ptype, thr, fp=fp)
+    out = {}
+    for pp in (0, 1):
+        sds.set_tiles(s, dict(extend_add_rows=1024, cb_k_chunk=32, trailing_split=0, step_two_per_cu=0,
+                              pivot_panels=pp))
+        sds.execute("factorization", s)
+        out[pp] = (numeric_pivots(s), numeric_diag(s), numeric_stats(s)[0], numeric_factor(s),
+                   sds.getparam(s, "info"))
+    (p0, k0), d0, st0, f0, i0 = out[0]
+    (p1, k1), d1, st1, f1, i1 = out[1]
+    assert i0 == i1
+    assert np.array_equal(p0, p1) and np.array_equal(k0, k1), "pivots differ"
+    assert np.array_equal(d0, d1), "D differs"
+    assert np.array_equal(st0, st1)
+    assert np.array_equal(f0, f1), f"panels differ by {np.abs(f0.astype(float) - f1).max():.3e}"
+    if fp is None:
+        ref = oracle.ldlt(s.symbolic, nz, pivot_type=ptype, pivot_threshold=thr, eps=1e-13)
+        assert np.array_equal(p1, ref["piv"]) and np.array_equal(k1, ref["kind"])
+        assert np.array_equal(f1, ref["factor"]) and np.array_equal(d1, ref["d"])
