@@ -170,6 +170,22 @@ int sds_solve(sds_handle_t handle, int32_t phase, int32_t solve_mode, const doub
 int sds_set_schur(sds_handle_t handle, int64_t nschur);
 
 /*
+ * Matching and scaling (`matching_alg` = "algo1".."algo6", src/solver.jl:543-604,
+ * src/matching/mc64.jl:463-470): after sds_upload_symbolic of the analysis of the matched matrix
+ * M = (Dr A Dc)[:, q] ("G": the stored pattern with column c relabeled q^-1(c); the symmetric
+ * structures: D A D, the pattern unchanged), weights (host, nnz: r_i c_j of each stored entry
+ * (i, j) in the original numbering), rscale and cscale (host, n: Dr and Dc in the original
+ * numbering; equal for the symmetric structures) and colperm (host, n, 1-based: q, NULL for the
+ * symmetric structures). sds_factorize then factors the values w_p a_p (its statistics, inertia
+ * and diag are those of M); the solve phases solve A x = b as M y = Dr b, x = Dc q y (A^T x = b
+ * as M^T y = q^T Dc b, x = Dr y), and the refinement residual is that of A with the caller's
+ * values (the full-pattern map must then be that of A, in the original numbering). weights =
+ * NULL clears the matching. Not with the Schur complement mode.
+ */
+int sds_set_matching(sds_handle_t handle, const double* weights, const double* rscale, const double* cscale,
+                     const int64_t* colperm);
+
+/*
  * The Schur complement S of the last factorization (Schur complement mode) into the device
  * buffer S: nschur x nschur, column-major, leading dimension lds, Float64 (the full matrix: the
  * symmetric structures mirror the lower triangle, "G" assembles the whole square).

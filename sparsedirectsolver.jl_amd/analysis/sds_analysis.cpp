@@ -48,6 +48,7 @@ struct Analysis {
     int structure = 0, view = 0;
     i64 nnz_stored = 0, nnz_L = 0;
     i64 nschur = 0;  // Schur complement mode: the last supernode's columns (0 = off)
+    i64 pair_rounds = 0;  // rounds of the "default" pair search that added pairs
     double flops = 0.0, flops_true = 0.0;
     std::map<std::string, V> f;  // named 1-based integer arrays
     std::string err;
@@ -575,10 +576,666 @@ V amd_order(const Pattern& P) {
 }
 
 // ----------------------------------------------------------------------------
+// 2x2 pivot pairs of "S"/"H" (src/symbolic/pairs.jl): a row whose diagonal is absent or
+// negligible is matched with a neighbour; the ordering runs on the graph compressed by the pairs
+// (the partner eliminated right before its candidate) and the symbolic factorization on the pair
+// pattern, where both columns of a pair carry the union structure, so that the pair lies in one
+// supernode and in-front Bunch-Kaufman can take it as a 2x2 pivot.
+
+// pivot_pairs settings (src/types.jl:196): none, "default" (the structurally zero pivots),
+// "all" (every candidate)
+enum PairMode { PAIRS_NONE = 0, PAIRS_DEFAULT = 1, PAIRS_ALL = 2 };
+constexpr int PAIR_MAX_ROUNDS = 8;  // src/symbolic/pairs.jl:164
+
+using Pairs = std::vector<std::pair<i64, i64>>;  // (partner, candidate), 0-based
+
+// position of row r in column c's sorted adjacency (r must be a neighbour)
+i64 adj_pos(const Pattern& P, i64 c, i64 r) {
+    const i64* b = P.nb(c);
+    return P.colptr[c] + (std::lower_bound(b, b + P.deg(c), r) - b);
+}
+
+struct PairCands {
+    std::vector<char> cand;    // absent diagonal or |a_ii| <= tol max_{j != i} |a_ij|
+    std::vector<double> w;     // |a_ij| aligned with P.rowval, duplicates summed first
+    std::vector<double> amax;  // max_{j != i} |a_ij| (0 without off-diagonal entries)
+};
+
+// pivot_candidates (pairs.jl:60-99): the values read with the view rules of the pattern
+PairCands pivot_candidates(const Pattern& P, const V& rp, const V& cv, const double* nz, int structure, int view,
+                           double tol) {
+    const i64 n = P.n;
+    std::vector<double> dval(n, 0.0), wval(P.rowval.size(), 0.0);
+    std::vector<char> dpres(n, 0);
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            const i64 c = cv[p];
+            if (!keep_entry(structure, view, r, c)) continue;
+            if (r == c) {
+                dval[r] += nz[p];
+                dpres[r] = 1;
+            } else {
+                wval[adj_pos(P, c, r)] += nz[p];
+                wval[adj_pos(P, r, c)] += nz[p];
+            }
+        }
+    PairCands pc;
+    pc.w.resize(wval.size());
+    for (size_t p = 0; p < wval.size(); ++p) pc.w[p] = std::fabs(wval[p]);
+    pc.amax.assign(n, 0.0);
+    pc.cand.assign(n, 0);
+    for (i64 i = 0; i < n; ++i) {
+        for (i64 p = P.colptr[i]; p < P.colptr[i + 1]; ++p) pc.amax[i] = std::max(pc.amax[i], pc.w[p]);
+        pc.cand[i] = !dpres[i] || std::fabs(dval[i]) <= tol * pc.amax[i];
+    }
+    return pc;
+}
+
+// pivot_pairs "all" (pairs.jl:126-151): a greedy matching of the candidates to non-candidate
+// neighbours with a_ij != 0, candidates by increasing count of such neighbours (then index),
+// each taking the free neighbour of largest |a_ij| (ties: smaller degree, then smaller index)
+Pairs pivot_pairs_all(const Pattern& P, const PairCands& pc) {
+    const i64 n = P.n;
+    auto eligible = [&](i64 p) { return !pc.cand[P.rowval[p]] && pc.w[p] > 0; };
+    std::vector<std::pair<i64, i64>> order;  // (eligible count, candidate)
+    for (i64 i = 0; i < n; ++i) {
+        if (!pc.cand[i]) continue;
+        i64 c = 0;
+        for (i64 p = P.colptr[i]; p < P.colptr[i + 1]; ++p) c += eligible(p);
+        order.push_back({c, i});
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<char> matched(n, 0);
+    Pairs pairs;
+    for (auto [cnt, i] : order) {
+        (void)cnt;
+        i64 best = -1;
+        for (i64 p = P.colptr[i]; p < P.colptr[i + 1]; ++p) {
+            const i64 j = P.rowval[p];
+            if (!eligible(p) || matched[j]) continue;
+            if (best < 0 || pc.w[p] > pc.w[best] || (pc.w[p] == pc.w[best] && P.deg(j) < P.deg(P.rowval[best])))
+                best = p;
+        }
+        if (best < 0) continue;
+        matched[P.rowval[best]] = 1;
+        pairs.push_back({P.rowval[best], i});
+    }
+    std::sort(pairs.begin(), pairs.end(), [](auto& a, auto& b) { return a.second < b.second; });
+    return pairs;
+}
+
+// structural_zero_pivots (pairs.jl:184-249): the candidates, not in a pair, whose pivot is
+// structurally zero when A[perm, perm] is factored column by column, in elimination order. A
+// perfect matching of the leading block is kept: a non-candidate takes its own diagonal, a pair
+// (a, c) with c right after a is matched crosswise, a candidate searches an augmenting path from
+// its row to its column through earlier columns (|a_ij| > 0 edges, and the diagonal of a
+// non-candidate row that an earlier path moved off it); one without is reported (and matched to
+// itself, as if a partner fixed it).
+V structural_zero_pivots(const PairCands& pc, const Pattern& P, const V& perm, const Pairs& pairs) {
+    const i64 n = P.n;
+    V pos(n);
+    for (i64 k = 0; k < n; ++k) pos[perm[k]] = k;
+    V mate(n, -1);
+    for (auto& [a, b] : pairs) {
+        mate[a] = b;
+        mate[b] = a;
+    }
+    V colm(n, -1), seen(n, -1), out;
+    V srow, sptr, scol;
+    for (i64 k = 0; k < n; ++k) {
+        const i64 v = perm[k];
+        const i64 m = mate[v];
+        if (m >= 0 && pos[m] == k - 1) {  // the second column of a pair: the 2x2 block, crosswise
+            colm[v] = m;
+            colm[m] = v;
+            continue;
+        }
+        if (!pc.cand[v] || m >= 0) {  // a nonzero diagonal, or the first column of a pair
+            colm[v] = v;
+            continue;
+        }
+        srow.assign(1, v);
+        sptr.assign(1, P.colptr[v]);
+        scol.assign(1, -1);
+        bool found = false;
+        while (!srow.empty()) {
+            const i64 r = srow.back(), p = sptr.back();
+            if (p > P.colptr[r + 1]) {  // row r exhausted: backtrack
+                srow.pop_back();
+                sptr.pop_back();
+                scol.pop_back();
+                continue;
+            }
+            sptr.back() = p + 1;
+            i64 j;
+            if (p == P.colptr[r + 1]) {  // the last slot: the diagonal of a non-candidate row
+                if (pc.cand[r]) continue;
+                j = r;
+            } else {
+                j = P.rowval[p];
+                if (!(pc.w[p] > 0)) continue;
+            }
+            if (!(pos[j] <= k && seen[j] != k)) continue;
+            seen[j] = k;
+            scol.back() = j;
+            if (j == v) {  // reached the free column: flip the path
+                for (size_t t = 0; t < srow.size(); ++t) colm[scol[t]] = srow[t];
+                found = true;
+                break;
+            }
+            srow.push_back(colm[j]);
+            sptr.push_back(P.colptr[colm[j]]);
+            scol.push_back(-1);
+        }
+        if (!found) {
+            out.push_back(v);
+            colm[v] = v;
+        }
+    }
+    return out;
+}
+
+// one round of pivot_pairs "default" (pairs.jl:283-318): pair the structurally zero pivots under
+// perm with a free neighbour j, |a_ij| >= u max_k |a_ik| (another candidate allowed), taken by
+// increasing count of such partners, then by position in perm, each the one of largest |a_ij|
+// (ties: earliest in perm). Returns the pairs added.
+i64 zero_pivot_pairs(Pairs& pairs, const PairCands& pc, const Pattern& P, const V& perm, double u) {
+    const V zs = structural_zero_pivots(pc, P, perm, pairs);
+    const i64 n = P.n;
+    std::vector<char> inpair(n, 0);
+    for (auto& [a, b] : pairs) inpair[a] = inpair[b] = 1;
+    V pos(n);
+    for (i64 k = 0; k < n; ++k) pos[perm[k]] = k;
+    auto eligible = [&](i64 i, i64 p) { return !inpair[P.rowval[p]] && pc.w[p] > 0 && pc.w[p] >= u * pc.amax[i]; };
+    std::vector<std::pair<std::pair<i64, i64>, i64>> order;  // ((free count, position), candidate)
+    for (i64 i : zs) {
+        i64 c = 0;
+        for (i64 p = P.colptr[i]; p < P.colptr[i + 1]; ++p) c += eligible(i, p);
+        order.push_back({{c, pos[i]}, i});
+    }
+    std::sort(order.begin(), order.end());
+    i64 added = 0;
+    for (auto& o : order) {
+        const i64 i = o.second;
+        if (inpair[i]) continue;  // taken as the partner of an earlier candidate
+        i64 best = -1;
+        for (i64 p = P.colptr[i]; p < P.colptr[i + 1]; ++p) {
+            if (!eligible(i, p)) continue;
+            if (best < 0 || pc.w[p] > pc.w[best] || (pc.w[p] == pc.w[best] && pos[P.rowval[p]] < pos[P.rowval[best]]))
+                best = p;
+        }
+        if (best < 0) continue;
+        const i64 j = P.rowval[best];
+        inpair[j] = inpair[i] = 1;
+        pairs.push_back({j, i});
+        ++added;
+    }
+    std::sort(pairs.begin(), pairs.end(), [](auto& a, auto& b) { return a.second < b.second; });
+    return added;
+}
+
+// symmetric pattern from (row, column) pairs (both triangles given), sorted and unique
+Pattern pattern_from_edges(i64 n, std::vector<std::pair<i64, i64>>& e) {
+    std::sort(e.begin(), e.end(), [](auto& a, auto& b) { return a.second != b.second ? a.second < b.second : a.first < b.first; });
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    Pattern Q;
+    Q.n = n;
+    Q.colptr.assign(n + 1, 0);
+    Q.rowval.reserve(e.size());
+    for (auto& [r, c] : e) {
+        Q.colptr[c + 1]++;
+        Q.rowval.push_back(r);
+    }
+    for (i64 j = 0; j < n; ++j) Q.colptr[j + 1] += Q.colptr[j];
+    return Q;
+}
+
+// the groups of the compressed graph (pairs.jl:388-411): a pair is one vertex (partner first),
+// every other column one vertex, numbered by increasing first column
+void pair_groups(i64 n, const Pairs& pairs, V& gptr, V& members) {
+    V mate(n, -1);
+    std::vector<char> paired(n, 0);
+    for (auto& [a, b] : pairs) {
+        if (a < 0 || a >= n || b < 0 || b >= n || a == b || paired[a] || paired[b])
+            throw std::runtime_error("pivot pairs must be disjoint pairs of distinct indices");
+        mate[a] = b;
+        paired[a] = paired[b] = 1;
+    }
+    gptr.assign(1, 0);
+    members.clear();
+    for (i64 v = 0; v < n; ++v) {
+        if (mate[v] >= 0) {
+            members.push_back(v);
+            members.push_back(mate[v]);
+        } else if (!paired[v]) {
+            members.push_back(v);
+        } else {
+            continue;  // a candidate: emitted with its partner
+        }
+        gptr.push_back((i64)members.size());
+    }
+}
+
+// compressed_pattern (pairs.jl:421-438): a pair's adjacency is the union of its rows
+Pattern compressed_pattern(const Pattern& P, const Pairs& pairs, V& gptr, V& members) {
+    pair_groups(P.n, pairs, gptr, members);
+    const i64 ng = (i64)gptr.size() - 1;
+    V group(P.n);
+    for (i64 g = 0; g < ng; ++g)
+        for (i64 k = gptr[g]; k < gptr[g + 1]; ++k) group[members[k]] = g;
+    std::vector<std::pair<i64, i64>> e;
+    for (i64 j = 0; j < P.n; ++j)
+        for (i64 t = 0; t < P.deg(j); ++t) {
+            const i64 gi = group[P.nb(j)[t]], gj = group[j];
+            if (gi != gj) e.push_back({gi, gj});
+        }
+    return pattern_from_edges(ng, e);
+}
+
+// expand_permutation (pairs.jl:447-456): the columns of each vertex in turn, partner first
+V expand_permutation(const V& cperm, const V& gptr, const V& members) {
+    V perm;
+    perm.reserve(members.size());
+    for (i64 g : cperm)
+        for (i64 k = gptr[g]; k < gptr[g + 1]; ++k) perm.push_back(members[k]);
+    return perm;
+}
+
+// pair_pattern (pairs.jl:469-485): the expansion of the compressed pattern, both columns of a pair
+// with the union structure and adjacent to each other
+Pattern pair_pattern(const Pattern& P, const Pairs& pairs) {
+    if (pairs.empty()) return P;
+    V gptr, members;
+    const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
+    std::vector<std::pair<i64, i64>> e;
+    for (i64 g = 0; g < Pc.n; ++g) {
+        for (i64 a = gptr[g]; a < gptr[g + 1]; ++a)
+            for (i64 b = gptr[g]; b < gptr[g + 1]; ++b)
+                if (a != b) e.push_back({members[a], members[b]});
+        for (i64 t = 0; t < Pc.deg(g); ++t) {
+            const i64 h = Pc.nb(g)[t];
+            for (i64 a = gptr[g]; a < gptr[g + 1]; ++a)
+                for (i64 b = gptr[h]; b < gptr[h + 1]; ++b) e.push_back({members[b], members[a]});
+        }
+    }
+    return pattern_from_edges(P.n, e);
+}
+
+// ----------------------------------------------------------------------------
+// MC64-style matching and scaling (src/matching/mc64.jl, Duff & Koster 2001): the rows of the
+// stored matrix matched to its columns, on the host, from the values
+
+// the binary heap of (key, vertex) with lazy deletion of the successive-shortest-path search
+struct MinHeap {
+    std::vector<double> k;
+    V v;
+    void push(double key, i64 x) {
+        k.push_back(key);
+        v.push_back(x);
+        size_t i = k.size() - 1;
+        while (i > 0) {
+            const size_t p = (i - 1) / 2;
+            if (k[p] <= k[i]) break;
+            std::swap(k[p], k[i]);
+            std::swap(v[p], v[i]);
+            i = p;
+        }
+    }
+    void pop(double& key, i64& x) {
+        key = k[0];
+        x = v[0];
+        const double lk = k.back();
+        const i64 lv = v.back();
+        k.pop_back();
+        v.pop_back();
+        const size_t m = k.size();
+        if (m == 0) return;
+        k[0] = lk;
+        v[0] = lv;
+        size_t i = 0;
+        for (;;) {
+            const size_t l = 2 * i + 1;
+            if (l >= m) break;
+            const size_t c = (l + 1 < m && k[l + 1] < k[l]) ? l + 1 : l;
+            if (k[i] <= k[c]) break;
+            std::swap(k[i], k[c]);
+            std::swap(v[i], v[c]);
+            i = c;
+        }
+    }
+};
+
+// min_cost_matching (mc64.jl:101-181): minimum-cost matching of maximum cardinality of the left
+// vertices 0..n-1 (adjacency adj[ptr[i] .. ptr[i+1]), costs >= 0) to the right ones, by successive
+// shortest augmenting paths (Dijkstra on the reduced costs cost - u_i - v_j >= 0); lmate[i] the
+// right vertex of i (-1 none), u and v the duals (u_i + v_j <= cost, equality on matched edges)
+void min_cost_matching(const V& ptr, const V& adj, const std::vector<double>& cost, i64 n, V& lmate,
+                       std::vector<double>& u, std::vector<double>& v) {
+    lmate.assign(n, -1);
+    V rmate(n, -1);
+    u.assign(n, 0.0);
+    v.assign(n, 0.0);
+    for (i64 i = 0; i < n; ++i) {  // feasible start: u = the row minimum; the tight edges greedily
+        double mn = 0.0;
+        bool first = true;
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p) {
+            mn = first ? cost[p] : std::min(mn, cost[p]);
+            first = false;
+        }
+        u[i] = first ? 0.0 : std::min(0.0, mn);  // the minimum with the initial value 0
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p) {
+            const i64 j = adj[p];
+            if (rmate[j] < 0 && cost[p] - u[i] <= 0) {
+                lmate[i] = j;
+                rmate[j] = i;
+                break;
+            }
+        }
+    }
+    std::vector<double> dist(n, INFINITY);
+    V pred(n, -1), touched, finals;
+    std::vector<char> done(n, 0);
+    MinHeap hp;
+    for (i64 i0 = 0; i0 < n; ++i0) {
+        if (lmate[i0] >= 0) continue;
+        touched.clear();
+        finals.clear();
+        hp.k.clear();
+        hp.v.clear();
+        i64 cur = i0, jfree = -1;
+        double dcur = 0.0, L = INFINITY;
+        for (;;) {
+            for (i64 p = ptr[cur]; p < ptr[cur + 1]; ++p) {
+                const i64 j = adj[p];
+                if (done[j]) continue;
+                const double nd = dcur + std::max(cost[p] - u[cur] - v[j], 0.0);
+                if (nd < dist[j]) {
+                    if (std::isinf(dist[j])) touched.push_back(j);
+                    dist[j] = nd;
+                    pred[j] = cur;
+                    hp.push(nd, j);
+                }
+            }
+            i64 j = -1;
+            while (!hp.k.empty()) {
+                double d;
+                i64 jj;
+                hp.pop(d, jj);
+                if (done[jj] || d > dist[jj]) continue;
+                j = jj;
+                break;
+            }
+            if (j < 0) break;  // no augmenting path: i0 stays unmatched
+            done[j] = 1;
+            finals.push_back(j);
+            if (rmate[j] < 0) {
+                jfree = j;
+                L = dist[j];
+                break;
+            }
+            cur = rmate[j];
+            dcur = dist[j];  // the matched edge has reduced cost 0
+        }
+        if (jfree >= 0) {
+            u[i0] += L;
+            for (i64 j : finals) {
+                if (j == jfree) continue;
+                u[rmate[j]] += L - dist[j];
+                v[j] -= L - dist[j];
+            }
+            i64 j = jfree;
+            for (;;) {  // flip the path
+                const i64 i = pred[j];
+                const i64 nj = lmate[i];
+                lmate[i] = j;
+                rmate[j] = i;
+                if (i == i0) break;
+                j = nj;
+            }
+        }
+        for (i64 j : touched) {
+            dist[j] = INFINITY;
+            done[j] = 0;
+        }
+    }
+}
+
+// the full matrix (both triangles of a symmetric structure) as rows -> columns with |a_ij|,
+// duplicates summed first, explicit zeros kept (mc64.jl:184-207)
+void abs_csr(i64 n, const V& rp, const V& cv, const double* nz, int structure, int view, V& ptr, V& adj,
+             std::vector<double>& w) {
+    std::vector<std::map<i64, double>> rows(n);
+    for (i64 r = 0; r < n; ++r)
+        for (i64 p = rp[r]; p < rp[r + 1]; ++p) {
+            const i64 c = cv[p];
+            if (structure == ST_G) {
+                rows[r][c] += nz[p];
+                continue;
+            }
+            if (!keep_entry(structure, view, r, c)) continue;
+            rows[r][c] += nz[p];
+            if (r != c) rows[c][r] += nz[p];
+        }
+    ptr.assign(n + 1, 0);
+    adj.clear();
+    w.clear();
+    for (i64 r = 0; r < n; ++r) {
+        for (auto& [c, x] : rows[r]) {
+            adj.push_back(c);
+            w.push_back(std::fabs(x));
+        }
+        ptr[r + 1] = (i64)adj.size();
+    }
+}
+
+// maximum cardinality matching on the kept edges (zero costs)
+V cardinality_matching(const V& ptr, const V& adj, const std::vector<char>& keep, i64 n) {
+    V sp(n + 1, 0), sa;
+    for (i64 i = 0; i < n; ++i) {
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p)
+            if (keep[p]) sa.push_back(adj[p]);
+        sp[i + 1] = (i64)sa.size();
+    }
+    V lm;
+    std::vector<double> u, v;
+    min_cost_matching(sp, sa, std::vector<double>(sa.size(), 0.0), n, lm, u, v);
+    return lm;
+}
+
+i64 matched_count(const V& lm) {
+    i64 c = 0;
+    for (i64 x : lm) c += x >= 0;
+    return c;
+}
+
+// mc64 (mc64.jl:265-312): job 1 maximum cardinality; jobs 2 and 3 the largest smallest matched
+// |a_ij| (bisection over the distinct values); job 4 the largest sum; job 5 the largest product
+// with the scalings r_i = exp(u_i) / max_j |a_ij|, c_j = exp(v_j) (|r_i a_ij c_j| <= 1, equality
+// on the matched entries). Jobs 2-5 use the nonzero entries only.
+void mc64(const V& ptr, const V& adj, const std::vector<double>& w, i64 n, int job, V& lmate,
+          std::vector<double>& rs, std::vector<double>& cs) {
+    rs.assign(n, 1.0);
+    cs.assign(n, 1.0);
+    if (job == 1) {
+        lmate = cardinality_matching(ptr, adj, std::vector<char>(w.size(), 1), n);
+        return;
+    }
+    if (job == 2 || job == 3) {
+        const V full = cardinality_matching(ptr, adj, std::vector<char>(w.size(), 1), n);
+        const i64 target = matched_count(full);
+        std::vector<double> vals;
+        for (double x : w)
+            if (x > 0) vals.push_back(x);
+        std::sort(vals.begin(), vals.end());
+        vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+        if (vals.empty()) {
+            lmate = full;
+            return;
+        }
+        auto with = [&](double t) {
+            std::vector<char> keep(w.size());
+            for (size_t p = 0; p < w.size(); ++p) keep[p] = w[p] >= t;
+            return cardinality_matching(ptr, adj, keep, n);
+        };
+        V best = with(vals[0]);
+        if (matched_count(best) != target) {
+            lmate = full;
+            return;
+        }
+        i64 lo = 0, hi = (i64)vals.size() - 1;
+        while (lo < hi) {
+            const i64 mid = (lo + hi + 1) / 2;
+            V m = with(vals[mid]);
+            if (matched_count(m) == target) {
+                lo = mid;
+                best = m;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        lmate = best;
+        return;
+    }
+    if (job != 4 && job != 5) throw std::runtime_error("unknown matching job; expected 1 to 5");
+    std::vector<double> rowmax(n, 0.0);
+    for (i64 i = 0; i < n; ++i)
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p) rowmax[i] = std::max(rowmax[i], w[p]);
+    V sp(n + 1, 0), sa;
+    std::vector<double> cost;
+    for (i64 i = 0; i < n; ++i) {
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p) {
+            if (!(w[p] > 0)) continue;
+            sa.push_back(adj[p]);
+            cost.push_back(job == 5 ? std::log(rowmax[i]) - std::log(w[p]) : rowmax[i] - w[p]);
+        }
+        sp[i + 1] = (i64)sa.size();
+    }
+    std::vector<double> u, v;
+    min_cost_matching(sp, sa, cost, n, lmate, u, v);
+    if (job == 5) {
+        for (i64 i = 0; i < n; ++i) {
+            rs[i] = rowmax[i] > 0 ? std::exp(u[i]) / rowmax[i] : 1.0;
+            cs[i] = std::exp(v[i]);
+        }
+        for (auto* s : {&rs, &cs})
+            for (double& x : *s)
+                if (!(std::isfinite(x) && x > 0)) x = 1.0;
+    }
+}
+
+// compute_matching (mc64.jl:370-392): "G" the row/column scalings of the job; the symmetric
+// structures d_i = sqrt(r_i c_i) (Duff & Pralet 2005), an unmatched index (structurally
+// singular) getting 1 / max_j |a_ij| d_j over its matched neighbours; the matching completed to a
+// permutation (unmatched rows take the unmatched columns in increasing order)
+void compute_matching(i64 n, const V& rp, const V& cv, const double* nz, int structure, int view, int job, V& perm,
+                      std::vector<char>& matched, std::vector<double>& rs, std::vector<double>& cs) {
+    V ptr, adj, lmate;
+    std::vector<double> w;
+    abs_csr(n, rp, cv, nz, structure, view, ptr, adj, w);
+    mc64(ptr, adj, w, n, job, lmate, rs, cs);
+    matched.assign(n, 0);
+    std::vector<char> used(n, 0);
+    for (i64 i = 0; i < n; ++i)
+        if (lmate[i] >= 0) {
+            matched[i] = 1;
+            used[lmate[i]] = 1;
+        }
+    perm = lmate;
+    i64 nxt = 0;
+    for (i64 i = 0; i < n; ++i) {
+        if (perm[i] >= 0) continue;
+        while (used[nxt]) ++nxt;
+        perm[i] = nxt;
+        used[nxt] = 1;
+    }
+    if (structure == ST_G) return;
+    std::vector<double> d(n);
+    for (i64 i = 0; i < n; ++i) d[i] = std::sqrt(rs[i] * cs[i]);
+    for (i64 i = 0; i < n; ++i) {
+        if (matched[i]) continue;
+        double m = 0.0;
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p)
+            if (matched[adj[p]]) m = std::max(m, w[p] * d[adj[p]]);
+        d[i] = m > 0 ? 1.0 / m : 1.0;
+    }
+    for (double& x : d)
+        if (!(std::isfinite(x) && x > 0)) x = 1.0;
+    rs = d;
+    cs = d;
+}
+
+// matching_pairs (mc64.jl:413-461): the 2x2 pivot pairs of "S"/"H" from the cycles of the
+// symmetric matching (matched indices only): a 2-cycle is a pair, a longer cycle is split into
+// pairs of consecutive indices (even length: the alternative with the larger product of scaled
+// |d_i a_ij d_j|; odd: leaving out the index with the largest scaled diagonal); "default" keeps a
+// pair only when one of its indices is a 2x2 candidate of the scaled matrix; the candidate is the
+// index with the smaller relative scaled diagonal
+Pairs matching_pairs(i64 n, const V& rp, const V& cv, const double* nz, int structure, int view, const V& perm,
+                     const std::vector<char>& matched, const std::vector<double>& d, int mode, double tol) {
+    Pairs pairs;
+    if (mode == PAIRS_NONE) return pairs;
+    V ptr, adj;
+    std::vector<double> w;
+    abs_csr(n, rp, cv, nz, structure, view, ptr, adj, w);
+    std::vector<double> dg(n, 0.0), offmax(n, 0.0);
+    for (i64 i = 0; i < n; ++i)
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p) {
+            const double x = d[i] * w[p] * d[adj[p]];
+            if (adj[p] == i) dg[i] = x;
+            else offmax[i] = std::max(offmax[i], x);
+        }
+    auto entry = [&](i64 i, i64 j) {
+        for (i64 p = ptr[i]; p < ptr[i + 1]; ++p)
+            if (adj[p] == j) return d[i] * w[p] * d[j];
+        return 0.0;
+    };
+    auto ratio = [&](i64 i) { return offmax[i] > 0 ? dg[i] / offmax[i] : INFINITY; };
+    auto candidate = [&](i64 i) { return dg[i] <= tol * offmax[i]; };
+    std::vector<char> seen(n, 0);
+    V cyc;
+    for (i64 i0 = 0; i0 < n; ++i0) {
+        if (seen[i0] || !matched[i0]) continue;
+        cyc.clear();
+        i64 i = i0;
+        bool ok = true;
+        while (!seen[i]) {
+            seen[i] = 1;
+            cyc.push_back(i);
+            if (!matched[i]) ok = false;
+            i = perm[i];
+        }
+        const i64 L = (i64)cyc.size();
+        if (!(ok && i == i0 && L >= 2)) continue;
+        i64 start;  // 0-based position of the first pair's first index
+        if (L % 2 == 0) {
+            double p0 = 1.0, p1 = 1.0;
+            for (i64 k = 0; k < L; k += 2) {
+                p0 *= entry(cyc[k], cyc[k + 1]);
+                p1 *= entry(cyc[k + 1], cyc[(k + 2) % L]);
+            }
+            start = p1 > p0 ? 1 : 0;
+        } else {
+            i64 am = 0;  // the largest scaled diagonal, first on ties
+            for (i64 k = 1; k < L; ++k)
+                if (dg[cyc[k]] > dg[cyc[am]]) am = k;
+            start = (am + 1) % L;  // the index after the singleton
+        }
+        for (i64 t = 0; t + 1 < L; t += 2) {
+            const i64 a = cyc[(start + t) % L], b = cyc[(start + t + 1) % L];
+            if (mode != PAIRS_ALL && !candidate(a) && !candidate(b)) continue;
+            pairs.push_back(ratio(a) < ratio(b) ? std::make_pair(b, a) : std::make_pair(a, b));
+        }
+    }
+    std::sort(pairs.begin(), pairs.end(), [](auto& a, auto& b) { return a.second < b.second; });
+    return pairs;
+}
+
+// ----------------------------------------------------------------------------
 
 Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structure, int view, int index_base,
                   const i64* perm_in, int ordering, i64 max_width, double zero_fraction, i64 min_width,
-                  int use_superpanels, const int8_t* schur) {
+                  int use_superpanels, const int8_t* schur, const double* values, int pair_mode, double pair_tol,
+                  double pivot_u, const i64* fixed_pairs, i64 nfixed) {
     auto A = new Analysis();
     A->n = n;
     A->structure = structure;
@@ -600,6 +1257,8 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
 
     // ordering: etree numbering column k = original perm0[k]
     V perm0(n);
+    Pairs pairs;
+    int pair_rounds = 0;
     if (perm_in) {
         std::vector<char> seen(n, 0);
         for (i64 k = 0; k < n; ++k) {
@@ -608,7 +1267,37 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
             seen[perm0[k]] = 1;
         }
     } else if (ordering == 3) {
-        perm0 = amd_order(P);
+        // 2x2 pivot pairs of "S"/"H" (src/symbolic/ordering.jl:283-295, pairs.jl:330-343): not with a
+        // user permutation, the natural ordering or the Schur complement mode
+        const bool sym = structure == ST_S || structure == ST_H;
+        if (sym && !schur && nfixed > 0) {
+            // fixed pairs (those of a matching): the compressed graph ordered, then expanded
+            for (i64 t = 0; t < nfixed; ++t) pairs.push_back({fixed_pairs[2 * t] - 1, fixed_pairs[2 * t + 1] - 1});
+            V gptr, members;
+            const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
+            perm0 = expand_permutation(amd_order(Pc), gptr, members);
+        } else if (sym && !schur && pair_mode != PAIRS_NONE && values) {
+            const PairCands pc = pivot_candidates(P, rp, cv, values, structure, view, pair_tol);
+            if (pair_mode == PAIRS_ALL) {
+                pairs = pivot_pairs_all(P, pc);
+                V gptr, members;
+                const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
+                perm0 = expand_permutation(amd_order(Pc), gptr, members);
+            } else {
+                // "default": pair the structurally zero pivots of the ordering, re-order the
+                // compressed graph, until a round adds no pair (src/symbolic/pairs.jl:322-335)
+                perm0 = amd_order(P);
+                while (pair_rounds < PAIR_MAX_ROUNDS) {
+                    if (zero_pivot_pairs(pairs, pc, P, perm0, pivot_u) == 0) break;
+                    ++pair_rounds;
+                    V gptr, members;
+                    const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
+                    perm0 = expand_permutation(amd_order(Pc), gptr, members);
+                }
+            }
+        } else {
+            perm0 = amd_order(P);
+        }
     } else {
         std::iota(perm0.begin(), perm0.end(), 0);
     }
@@ -629,6 +1318,17 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
     A->nschur = nschur;
     V iperm0(n);
     for (i64 k = 0; k < n; ++k) iperm0[perm0[k]] = k;
+    A->pair_rounds = pair_rounds;
+    {
+        V pv;
+        for (auto& [a, b] : pairs) {
+            pv.push_back(a + 1);
+            pv.push_back(b + 1);
+        }
+        A->f["pairs"] = pv;
+    }
+    // the symbolic factorization runs on the pair pattern (the pattern itself without pairs)
+    if (!pairs.empty()) P = pair_pattern(P, pairs);
 
     V parent = etree(P, perm0, iperm0);
     V post = postorder(parent);
@@ -872,12 +1572,18 @@ extern "C" {
 // (sdsa_last_error() holds the message).
 // schur: NULL, or n flags (original rows) of the Schur complement mode, whose rows and columns go
 // last and form the last supernode.
+// values (nzval of the first member, may be NULL), pair_mode (0 none, 1 "default", 2 "all"),
+// pair_tol (pivot_pair_tolerance) and pivot_u (pivot_threshold, the partners' acceptance):
+// 2x2 pivot pairs of "S"/"H" with the AMD ordering; fixed_pairs (nfixed (partner, candidate)
+// pairs, 1-based, e.g. those of a matching) replace the search. The pairs used are the field
+// "pairs" (partner, candidate, ... 1-based).
 void* sdsa_analyze(int64_t n, const int64_t* rowptr, const int64_t* colval, int structure, int view,
                    int index_base, const int64_t* perm, int ordering, int64_t max_width, double zero_fraction,
-                   int64_t min_width, int use_superpanels, const int8_t* schur) {
+                   int64_t min_width, int use_superpanels, const int8_t* schur, const double* values, int pair_mode,
+                   double pair_tol, double pivot_u, const int64_t* fixed_pairs, int64_t nfixed) {
     try {
         return analyze(n, rowptr, colval, structure, view, index_base, perm, ordering, max_width, zero_fraction,
-                       min_width, use_superpanels, schur);
+                       min_width, use_superpanels, schur, values, pair_mode, pair_tol, pivot_u, fixed_pairs, nfixed);
     } catch (const std::exception& e) {
         g_err = e.what();
         return nullptr;
@@ -916,8 +1622,65 @@ double sdsa_scalar(void* h, int which) {
         case 5: return A->flops;
         case 6: return A->flops_true;
         case 7: return (double)A->nschur;
+        case 8: return (double)A->pair_rounds;
     }
     return -1;
+}
+
+// MC64-style matching (src/matching/mc64.jl:370-392) of the CSR matrix (index_base 0 or 1) with
+// job 1..5: perm (1-based, row i matched to column perm[i], completed to a permutation), matched
+// (n flags), rscale / cscale (n; equal for the symmetric structures). 0 on success, -1 on error.
+int sdsa_matching(int64_t n, const int64_t* rowptr, const int64_t* colval, const double* nzval, int structure,
+                  int view, int index_base, int job, int64_t* perm, int8_t* matched, double* rscale, double* cscale) {
+    try {
+        V rp(n + 1), cv;
+        for (i64 i = 0; i <= n; ++i) rp[i] = rowptr[i] - index_base;
+        cv.resize(rp[n]);
+        for (i64 p = 0; p < rp[n]; ++p) {
+            cv[p] = colval[p] - index_base;
+            if (cv[p] < 0 || cv[p] >= n) throw std::runtime_error("column index out of range");
+        }
+        V pm;
+        std::vector<char> mt;
+        std::vector<double> rs, cs;
+        compute_matching(n, rp, cv, nzval, structure, view, job, pm, mt, rs, cs);
+        for (i64 i = 0; i < n; ++i) {
+            perm[i] = pm[i] + 1;
+            matched[i] = mt[i];
+            rscale[i] = rs[i];
+            cscale[i] = cs[i];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// matching_pairs (src/matching/mc64.jl:413-461): the 2x2 pivot pairs (partner, candidate; 1-based,
+// at most n entries into pairs) from the cycles of a symmetric matching (perm 1-based, matched,
+// d the symmetric scaling); mode 0 none, 1 "default", 2 "all". Returns the number of pairs, -1 on error.
+int64_t sdsa_matching_pairs(int64_t n, const int64_t* rowptr, const int64_t* colval, const double* nzval,
+                            int structure, int view, int index_base, const int64_t* perm, const int8_t* matched,
+                            const double* d, int mode, double tol, int64_t* pairs) {
+    try {
+        V rp(n + 1), cv, pm(n);
+        for (i64 i = 0; i <= n; ++i) rp[i] = rowptr[i] - index_base;
+        cv.resize(rp[n]);
+        for (i64 p = 0; p < rp[n]; ++p) cv[p] = colval[p] - index_base;
+        for (i64 i = 0; i < n; ++i) pm[i] = perm[i] - 1;
+        std::vector<char> mt(matched, matched + n);
+        std::vector<double> dv(d, d + n);
+        const Pairs pr = matching_pairs(n, rp, cv, nzval, structure, view, pm, mt, dv, mode, tol);
+        for (size_t t = 0; t < pr.size(); ++t) {
+            pairs[2 * t] = pr[t].first + 1;
+            pairs[2 * t + 1] = pr[t].second + 1;
+        }
+        return (int64_t)pr.size();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 // geometric nested dissection of an nx x ny x nz grid; perm (1-based) of length nx ny nz

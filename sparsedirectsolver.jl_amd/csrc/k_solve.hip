@@ -17,14 +17,28 @@
 
 namespace sds {
 
+// Y[k nrhs + r] = sc_i B[i + r ldb] with i = cq[perm[k]] (cq: the matching's column permutation
+// of "G" on the side that needs it, else none; sc: the matching's scaling of that side, or none)
+// (src/solve/permute.jl:68-150 with the SolveScaling of src/solver.jl:583-604)
 template <typename T>
-__global__ void k_permute_in(int n, int nrhs, const int32_t* __restrict__ perm, const double* __restrict__ B,
-                             int64_t ldb, T* __restrict__ Y) {
+__global__ void k_permute_in(int n, int nrhs, const int32_t* __restrict__ perm, const int32_t* __restrict__ cq,
+                             const double* __restrict__ sc, const double* __restrict__ B, int64_t ldb,
+                             T* __restrict__ Y) {
     const int64_t total = (int64_t)n * nrhs;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
-        Y[q] = T(B[(perm ? perm[k] : k) + r * ldb]);
+        int i = perm ? perm[k] : k;
+        if (cq) i = cq[i];
+        const double b = B[i + r * ldb];
+        Y[q] = T(sc ? sc[i] * b : b);
     }
+}
+
+// the values the factorization reads with a matching: w_p a_p (w_p = r_i c_j of the stored entry)
+__global__ void k_scale_values(int64_t nnz, const double* __restrict__ w, const double* __restrict__ nz,
+                               double* __restrict__ out) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x)
+        out[q] = w[q] * nz[q];
 }
 
 // Schur complement mode, forward sweep: the Schur root is not solved; its rows of Y get their
@@ -49,14 +63,19 @@ __global__ void __launch_bounds__(256) k_schur_pull(DevSym S, int s, T* __restri
 }
 
 // X[perm[k], r] = Y[k, r] (add = 0) or X[perm[k], r] += Y[k, r] (add = 1)
+// X[i + r ldx] (+)= sc_i Y[k nrhs + r] with i = cq[perm[k]] (as k_permute_in)
 template <typename T>
-__global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm, const T* __restrict__ Y,
-                              double* __restrict__ X, int64_t ldx, int add) {
+__global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm, const int32_t* __restrict__ cq,
+                              const double* __restrict__ sc, const T* __restrict__ Y, double* __restrict__ X,
+                              int64_t ldx, int add) {
     const int64_t total = (int64_t)n * nrhs;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / nrhs), r = (int)(q - (int64_t)k * nrhs);
-        double* x = X + (perm ? perm[k] : k) + r * ldx;
-        *x = add ? *x + (double)Y[q] : (double)Y[q];
+        int i = perm ? perm[k] : k;
+        if (cq) i = cq[i];
+        double* x = X + i + r * ldx;
+        const double y = sc ? sc[i] * (double)Y[q] : (double)Y[q];
+        *x = add ? *x + y : y;
     }
 }
 
@@ -545,8 +564,14 @@ __global__ void k_adjoint_scatter(int64_t nnz, int nrhs, const int32_t* __restri
 static int grid_for(int64_t total) { return (int)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 4096); }
 
 template <typename T>
-void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y) {
-    hipLaunchKernelGGL(k_permute_in<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, B, ldb, Y);
+void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y,
+                       const int32_t* cq, const double* sc) {
+    hipLaunchKernelGGL(k_permute_in<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, cq, sc, B,
+                       ldb, Y);
+}
+
+void launch_scale_values(hipStream_t st, int64_t nnz, const double* w, const double* nz, double* out) {
+    hipLaunchKernelGGL(k_scale_values, dim3(grid_for(nnz)), dim3(256), 0, st, nnz, w, nz, out);
 }
 
 template <typename T>
@@ -556,9 +581,9 @@ void launch_schur_pull(hipStream_t st, const DevSym& S, int s, T* Y, const T* U,
 
 template <typename T>
 void launch_permute_out(hipStream_t st, int n, int nrhs, const int32_t* perm, const T* Y, double* X, int64_t ldx,
-                        int add) {
-    hipLaunchKernelGGL(k_permute_out<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, Y, X,
-                       ldx, add);
+                        int add, const int32_t* cq, const double* sc) {
+    hipLaunchKernelGGL(k_permute_out<T>, dim3(grid_for((int64_t)n * nrhs)), dim3(256), 0, st, n, nrhs, perm, cq, sc, Y,
+                       X, ldx, add);
 }
 
 template <typename T>
@@ -641,9 +666,11 @@ void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, con
 }
 
 #define INST(T)                                                                                                     \
-    template void launch_permute_in<T>(hipStream_t, int, int, const int32_t*, const double*, int64_t, T*);          \
+    template void launch_permute_in<T>(hipStream_t, int, int, const int32_t*, const double*, int64_t, T*,          \
+                                       const int32_t*, const double*);                                             \
     template void launch_schur_pull<T>(hipStream_t, const DevSym&, int, T*, const T*, int);                        \
-    template void launch_permute_out<T>(hipStream_t, int, int, const int32_t*, const T*, double*, int64_t, int);    \
+    template void launch_permute_out<T>(hipStream_t, int, int, const int32_t*, const T*, double*, int64_t, int,     \
+                                        const int32_t*, const double*);                                             \
     template void launch_fwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, T*, int, int,    \
                                 const int32_t*);                                                                    \
     template void launch_bwd<T>(hipStream_t, const DevSym&, const int32_t*, int, const T*, T*, T*, int, int,        \

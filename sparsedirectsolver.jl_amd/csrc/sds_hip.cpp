@@ -180,6 +180,12 @@ struct sds_handle {
     int64_t part_units = 0;                  // partial-sum buffer, units of nrhs
     DevBuf part;
     std::vector<DevBuf> symbufs;  // device maps
+    // matching and scaling (matching_alg, sds_set_matching): the entry weights r_i c_j (nnz), the
+    // scaled values the factorization reads, the row and column scalings (n) and the column
+    // permutation q of "G" (0-based, none for the symmetric structures)
+    std::vector<DevBuf> mtbufs;
+    double *mt_w = nullptr, *mt_vals = nullptr, *mt_rs = nullptr, *mt_cs = nullptr;
+    int32_t* mt_q = nullptr;
     DevSym dsym{};
     int64_t full_nrows = 0;
     int64_t* full_rowptr = nullptr;
@@ -1626,6 +1632,12 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
 template <typename T>
 int fgmres_t(sds_handle* h, const SolveWork& W, int transposed, const double* Bsrc, int64_t ldbs, double* X,
              int64_t ldx, int nrhs, const double* nz, int m, double tol, int* iters) {
+    // the matching's sides (src/solver.jl:583-604): A x = b solves M y = Dr b with M = (Dr A Dc)[:, q]
+    // and x = Dc q y; A^T x = b the transposed way round (no matching: none of them)
+    const int32_t* IN_CQ = transposed ? h->mt_q : nullptr;
+    const double* IN_SC = transposed ? h->mt_cs : h->mt_rs;
+    const int32_t* OUT_CQ = transposed ? nullptr : h->mt_q;
+    const double* OUT_SC = transposed ? h->mt_rs : h->mt_cs;
     hipStream_t st = h->stream;
     const int n = (int)h->n;
     const int64_t N = (int64_t)n * nrhs;
@@ -1703,9 +1715,9 @@ int fgmres_t(sds_handle* h, const SolveWork& W, int transposed, const double* Bs
         double* Vn = V + (int64_t)(j + 1) * N;
         double* Zj = Zk + (int64_t)j * N;
         // Z_j = op(A)^-1 V_j by the factorization, then V_{j+1} = op(A) Z_j, orthogonalized
-        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, Vj, n, Y); });
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, Vj, n, Y, IN_CQ, IN_SC); });
         sweeps<T>(h, W, nrhs, 1, 1, 1, transposed);
-        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, Zj, n, 0); });
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, Zj, n, 0, OUT_CQ, OUT_SC); });
         spmv(Zj, n, nullptr, 0, Vn);
         for (int i = 0; i <= j; ++i) {
             launch_dot(st, N, Vn, V + (int64_t)i * N, part, H + (int64_t)j * hc + i);
@@ -1761,6 +1773,12 @@ double max_relative_residual(const double* nr, int nrhs) {
 template <typename T>
 int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, const double* B, int64_t ldb, double* X,
             int64_t ldx, int nrhs, const double* nz, int32_t ir_n_steps, double ir_tol, int32_t* ir_steps) {
+    // the matching's sides (src/solver.jl:583-604): A x = b solves M y = Dr b with M = (Dr A Dc)[:, q]
+    // and x = Dc q y; A^T x = b the transposed way round (no matching: none of them)
+    const int32_t* IN_CQ = transposed ? h->mt_q : nullptr;
+    const double* IN_SC = transposed ? h->mt_cs : h->mt_rs;
+    const int32_t* OUT_CQ = transposed ? nullptr : h->mt_q;
+    const double* OUT_SC = transposed ? h->mt_rs : h->mt_cs;
     hipStream_t st = h->stream;
     T* Y = static_cast<T*>(W.Y);
     const int n = (int)h->n;
@@ -1777,7 +1795,7 @@ int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, co
     // order unless it permutes (the last nschur entries are the Schur block)
     const bool schur = h->schur_s >= 0;
     if (phase & SDS_PHASE_SOLVE_FWD_PERM)
-        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y); });
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, B, ldb, Y, IN_CQ, IN_SC); });
     else if (schur)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, nullptr, B, ldb, Y); });
     // Schur mode, LU: the diagonal goes with the backward phase and the diagonal phase is the
@@ -1785,7 +1803,7 @@ int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, co
     const int diag = schur && h->method == M_LU ? (phase & SDS_PHASE_SOLVE_BWD) : (phase & SDS_PHASE_SOLVE_DIAG);
     sweeps<T>(h, W, nrhs, phase & SDS_PHASE_SOLVE_FWD, diag, phase & SDS_PHASE_SOLVE_BWD, transposed);
     if (phase & SDS_PHASE_SOLVE_BWD_PERM)
-        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0); });
+        timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 0, OUT_CQ, OUT_SC); });
     else if (schur)
         timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_out<T>(st, n, nrhs, nullptr, Y, X, ldx, 0); });
     int done = 0;
@@ -1822,10 +1840,10 @@ int solve_t(sds_handle* h, const SolveWork& W, int32_t phase, int transposed, co
                 HIPCHK(h, hipStreamSynchronize(st));
                 if (max_relative_residual(W.hnorms, nrhs) <= ir_tol) break;
             }
-            timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y); });
+            timed(h, SDS_K_PERMUTE, 0, pbytes, [&] { launch_permute_in<T>(st, n, nrhs, h->dsym.perm, R, n, Y, IN_CQ, IN_SC); });
             sweeps<T>(h, W, nrhs, 1, 1, 1, transposed);
             timed(h, SDS_K_PERMUTE, 0, pbytes + 8.0 * n * nrhs,
-                  [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1); });
+                  [&] { launch_permute_out<T>(st, n, nrhs, h->dsym.perm, Y, X, ldx, 1, OUT_CQ, OUT_SC); });
             ++done;
         }
     }
@@ -1856,6 +1874,7 @@ int sds_destroy(sds_handle_t h) {
 
     free_numeric(h);
     free_all(h->symbufs);
+    free_all(h->mtbufs);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
     delete h;
     return SDS_SUCCESS;
@@ -1883,6 +1902,9 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
         return fail(h, SDS_INVALID_VALUE, "unknown structure");
     free_numeric(h);
     free_all(h->symbufs);
+    free_all(h->mtbufs);
+    h->mt_w = h->mt_vals = h->mt_rs = h->mt_cs = nullptr;
+    h->mt_q = nullptr;
     h->have_sym = false;
     h->placed = false;
     h->schur_s = -1;
@@ -2128,6 +2150,11 @@ int factorize_enqueue(sds_handle* h, const double* nzval, const sds_factor_param
     }
     h->factorized = false;
     int rc;
+    if (h->mt_w) {
+        // the matching's scaled values r_i a_ij c_j are what gets factored (src/solver.jl:595-601)
+        launch_scale_values(h->stream, h->nnz, h->mt_w, nzval, h->mt_vals);
+        nzval = h->mt_vals;
+    }
     if (h->method == M_LDLT) {
         // pivot_type (src/types.jl:84-92): 'N' none, 'D' diagonal, 'A'/'L'/'B' Bunch-Kaufman
         // inside the fully summed block (src/reference/ldlt.jl:44-51)
@@ -2423,6 +2450,40 @@ int sds_set_schur(sds_handle_t h, int64_t nschur) {
                     "the last supernode is not a dense root of nschur columns: analyse with the Schur indices");
     h->schur_s = (int32_t)s;
     h->nschur = nschur;
+    return SDS_SUCCESS;
+}
+
+int sds_set_matching(sds_handle_t h, const double* weights, const double* rscale, const double* cscale,
+                     const int64_t* colperm) {
+    if (!h) return SDS_INVALID_VALUE;
+    if (!h->have_sym) return fail(h, SDS_FACTORIZATION_ERROR, "no analysis uploaded");
+    free_all(h->mtbufs);
+    h->mt_w = h->mt_vals = h->mt_rs = h->mt_cs = nullptr;
+    h->mt_q = nullptr;
+    h->factorized = false;
+    if (!weights) return SDS_SUCCESS;
+    if (!rscale || !cscale) return fail(h, SDS_INVALID_VALUE, "a matching needs its row and column scalings");
+    if (h->schur_s >= 0) return fail(h, SDS_NOT_SUPPORTED, "matching is not supported with schur_mode");
+    const int64_t n = h->n, nnz = h->nnz;
+    std::vector<double> w(weights, weights + nnz), rs(rscale, rscale + n), cs(cscale, cscale + n);
+    int rc;
+    void* p;
+    if ((rc = upload(h, h->mtbufs, w, &h->mt_w))) return rc;
+    if ((rc = dev_alloc(h, h->mtbufs, sizeof(double) * std::max<int64_t>(nnz, 1), &p))) return rc;
+    h->mt_vals = static_cast<double*>(p);
+    if ((rc = upload(h, h->mtbufs, rs, &h->mt_rs))) return rc;
+    if ((rc = upload(h, h->mtbufs, cs, &h->mt_cs))) return rc;
+    if (colperm) {
+        std::vector<int32_t> q(n);
+        std::vector<char> seen(n, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t c = colperm[i] - 1;
+            if (c < 0 || c >= n || seen[c]) return fail(h, SDS_INVALID_VALUE, "colperm is not a permutation");
+            seen[c] = 1;
+            q[i] = (int32_t)c;
+        }
+        if ((rc = upload(h, h->mtbufs, q, &h->mt_q))) return rc;
+    }
     return SDS_SUCCESS;
 }
 

@@ -112,10 +112,13 @@ void launch_lu_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n,
 void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stats, int64_t* info);
 
 template <typename T>
-void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y);
+void launch_permute_in(hipStream_t st, int n, int nrhs, const int32_t* perm, const double* B, int64_t ldb, T* Y,
+                       const int32_t* cq = nullptr, const double* sc = nullptr);
 template <typename T>
 void launch_permute_out(hipStream_t st, int n, int nrhs, const int32_t* perm, const T* Y, double* X, int64_t ldx,
-                        int add);
+                        int add, const int32_t* cq = nullptr, const double* sc = nullptr);
+// out = w .* nz (the matching's scaled values)
+void launch_scale_values(hipStream_t st, int64_t nnz, const double* w, const double* nz, double* out);
 template <typename T>
 void launch_fwd(hipStream_t st, const DevSym& S, const int32_t* fronts, int nfronts, const T* factor, T* Y, T* U, T* Z,
                 int nrhs, int unit, const int32_t* piv);

@@ -121,6 +121,21 @@ end
 set_schur!(h::Handle, nschur::Integer) =
     _check(h, ccall((:sds_set_schur, libsds_hip), Cint, (Ptr{Cvoid}, Int64), h.ptr, Int64(nschur)))
 
+# Matching and scaling (matching_alg, src/solver.jl:543-604): the reference's host Matching `m`
+# (compute_matching, src/matching/mc64.jl:370-392) and its entry weights (entry_scaling,
+# :463-470); "G" passes the column permutation m.perm, the symmetric structures nothing. After
+# upload_symbolic! of the matched analysis; the full-pattern map must be that of A itself.
+function set_matching!(h::Handle, w::Vector{Float64}, m)
+    rs, cs = Vector{Float64}(m.rscale), Vector{Float64}(m.cscale)
+    q = m.symmetric ? nothing : Vector{Int64}(m.perm)
+    GC.@preserve w rs cs q begin
+        _check(h, ccall((:sds_set_matching, libsds_hip), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}),
+                        h.ptr, pointer(w), pointer(rs), pointer(cs), q === nothing ? C_NULL : pointer(q)))
+    end
+    return nothing
+end
+
 # getparam(solver, "schur_matrix") into a dense device matrix (schur_matrix!, src/numeric/schur.jl:112-150)
 function schur_matrix!(S::ROCMatrix{Float64}, h::Handle)
     _check(h, ccall((:sds_schur_export, libsds_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), h.ptr, pointer(S),

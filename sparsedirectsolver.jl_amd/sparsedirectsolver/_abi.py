@@ -9,7 +9,7 @@ EXPORTS = ("sds_create", "sds_destroy", "sds_last_error", "sds_set_stream", "sds
            "sds_get_pivots", "sds_get_diag", "sds_get_factor", "sds_memory_estimates", "sds_profile",
            "sds_profile_read", "sds_adjoint_scatter",
            "sds_autotune", "sds_graph_capture", "sds_graph_launch", "sds_graph_info", "sds_graph_destroy",
-           "sds_set_ir_mode", "sds_set_tiles", "sds_set_schur", "sds_schur_export")
+           "sds_set_ir_mode", "sds_set_tiles", "sds_set_schur", "sds_schur_export", "sds_set_matching")
 
 # profiling kinds (SDS_K_* of include/sds_hip.h)
 PROFILE_KINDS = ("front_small", "zero_scatter", "extend_add", "potrf", "trsm", "update_panel", "update_cb", "stats",
@@ -46,6 +46,7 @@ def bind(lib):
     lib.sds_solve_noalloc.argtypes = lib.sds_solve.argtypes
     lib.sds_get_stats.argtypes = [vp, vp, vp]
     lib.sds_set_schur.argtypes = [vp, c_i64]
+    lib.sds_set_matching.argtypes = [vp, vp, vp, vp, vp]
     lib.sds_schur_export.argtypes = [vp, vp, c_i64]
     lib.sds_get_pivots.argtypes = [vp, vp, vp]
     lib.sds_autotune.argtypes = [vp, vp, P(FactorParams), c_i32, vp]

@@ -24,9 +24,17 @@ def analysis_lib():
         lib = ctypes.CDLL(ANALYSIS_SO)
         i64p = ctypes.POINTER(ctypes.c_int64)
         lib.sdsa_analyze.restype = ctypes.c_void_p
+        f64p = ctypes.POINTER(ctypes.c_double)
+        i8p = ctypes.POINTER(ctypes.c_int8)
         lib.sdsa_analyze.argtypes = [ctypes.c_int64, i64p, i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p,
                                      ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_int64, ctypes.c_int,
-                                     ctypes.POINTER(ctypes.c_int8)]
+                                     i8p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, i64p, ctypes.c_int64]
+        lib.sdsa_matching.restype = ctypes.c_int
+        lib.sdsa_matching.argtypes = [ctypes.c_int64, i64p, i64p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, i64p, i8p, f64p, f64p]
+        lib.sdsa_matching_pairs.restype = ctypes.c_int64
+        lib.sdsa_matching_pairs.argtypes = [ctypes.c_int64, i64p, i64p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            i64p, i8p, f64p, ctypes.c_int, ctypes.c_double, i64p]
         lib.sdsa_last_error.restype = ctypes.c_char_p
         lib.sdsa_free.argtypes = [ctypes.c_void_p]
         lib.sdsa_field_len.restype = ctypes.c_int64

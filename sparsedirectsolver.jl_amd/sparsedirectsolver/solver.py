@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _abi
 from ._native import hip_lib
-from .analysis import merge_symbolic, symbolic_analysis
+from .analysis import (MATCHING_JOBS, compute_matching, entry_scaling, full_maps, matching_pairs, merge_symbolic,
+                       symbolic_analysis)
 from .enums import PHASES, PIVOT_TYPES, STRUCTURES, VIEWS
 from .errors import FactorizationError, InterruptedError_, InvalidValueError, NotSupportedError
 
@@ -184,8 +185,18 @@ class DirectSolver:
             schur = np.ascontiguousarray(schur, dtype=np.int8)
             if schur.shape != (self.n,) or np.any((schur != 0) & (schur != 1)) or not schur.any():
                 raise InvalidValueError("user_schur_indices must hold n flags 0/1, at least one of them 1")
+        # matching and scaling (src/solver.jl:543-579), from the values at analysis
+        self.matching = None
         if o["matching_alg"] not in ("default", "none"):
-            raise NotSupportedError("matching is not available in this build")
+            job = MATCHING_JOBS.get(o["matching_alg"])
+            if job is None:
+                raise InvalidValueError(f"matching_alg = {o['matching_alg']!r} is unknown")
+            if schur is not None:
+                raise NotSupportedError(f"matching_alg = {o['matching_alg']!r} is not supported with schur_mode = 1")
+            if self.nbatch > 1 or self._analysis is not None:
+                raise NotSupportedError("matching is not supported for batches in this build")
+            self.matching = compute_matching(self.rowptr, self.colval, self._host_values(), self.n, self.structure,
+                                             self.view, self.index, job)
         if self._analysis is not None:
             # an analysis assembled elsewhere (the block-diagonal forest of an uneven batch)
             self.symbolic = self._analysis(o)
@@ -205,11 +216,40 @@ class DirectSolver:
             self.symbolic = merge_symbolic([S1] * self.nbatch)
             self._ns_member = S1.ns
         else:
-            self.symbolic = symbolic_analysis(self.rowptr, self.colval, self.n, self.structure, self.view,
+            m = self.matching
+            # 2x2 pivot pairs (src/symbolic/pairs.jl:27-35): "S"/"H", the AMD ordering, no user_perm
+            pp = o["pivot_pairs"]
+            pairs_on = (self.structure in ("S", "H") and o["user_perm"] is None and ordering in ("default", "algo3")
+                        and pp != "none" and schur is None)
+            colval, values, fixed = self.colval, None, None
+            if m is not None and self.structure == "G":
+                # the analysis runs on M = (Dr A Dc)[:, q]: column c becomes column q^-1(c)
+                qinv = np.empty(self.n, dtype=np.int64)
+                qinv[m.perm - 1] = np.arange(1, self.n + 1)
+                base = 0 if self.index in ("Z", 0) else 1
+                colval = qinv[np.asarray(self.colval, dtype=np.int64) - base] - 1 + base
+            elif m is not None and m.job == 5 and pairs_on:
+                # pairs from the cycles of the scaled symmetric matching (src/solver.jl:560-566)
+                fixed = matching_pairs(m, self.rowptr, self.colval, self._host_values(), self.n, self.structure,
+                                       self.view, self.index, pp, o["pivot_pair_tolerance"])
+            elif pairs_on:
+                values = self._host_values()
+            self.symbolic = symbolic_analysis(self.rowptr, colval, self.n, self.structure, self.view,
                                               self.index, perm=o["user_perm"], ordering=ordering or "default",
                                               amalgamation=o["amalgamation"], use_superpanels=o["use_superpanels"],
-                                              schur=schur)
+                                              schur=schur, values=values, pivot_pairs=pp if pairs_on else "none",
+                                              pair_tolerance=o["pivot_pair_tolerance"],
+                                              pivot_threshold=o["pivot_threshold"], fixed_pairs=fixed)
+            if m is not None and self.structure == "G":
+                # the refinement residual and the adjoint scatter read A itself, in its own numbering
+                fr, fc, fs = full_maps(self.rowptr, self.colval, self.n, self.structure, self.view, self.index)
+                self.symbolic.maps.update(full_rowptr=fr, full_colval=fc, full_src=fs)
         self.stage = STAGE_REORDERED
+
+    def _host_values(self):
+        """The first batch member's values on the host: the analysis reads them once (pivot pairs,
+        matching; src/solver.jl:545-572)."""
+        return self.nzval[:len(self.colval)].detach().cpu().numpy().astype(np.float64)
 
     def _symbolic_factorization(self):
         if self.stage < STAGE_REORDERED:
@@ -236,6 +276,15 @@ class DirectSolver:
                                                     keep["full_src"].ctypes.data)
         _check(lib, self._h, lib.sds_upload_symbolic(self._h, ctypes.byref(d)))
         _check(lib, self._h, lib.sds_set_schur(self._h, int(getattr(S, "nschur", 0))))
+        m = getattr(self, "matching", None)
+        if m is not None and (not m.symmetric or m.job == 5):
+            # jobs 1-4 of the symmetric structures have no scaling: nothing to apply
+            w = np.ascontiguousarray(entry_scaling(m, self.rowptr, self.colval, self.n, self.index))
+            rs = np.ascontiguousarray(m.rscale, dtype=np.float64)
+            cs = np.ascontiguousarray(m.cscale, dtype=np.float64)
+            q = None if m.symmetric else np.ascontiguousarray(m.perm, dtype=np.int64)
+            _check(lib, self._h, lib.sds_set_matching(self._h, w.ctypes.data, rs.ctypes.data, cs.ctypes.data,
+                                                      q.ctypes.data if q is not None else None))
         fp = self.options["factor_precision"]
         dtype = _abi.SDS_R32 if fp in ("Float32", "float32", np.float32) else _abi.SDS_R64
         _check(lib, self._h, lib.sds_alloc_numeric(self._h, dtype, 1))
@@ -600,6 +649,11 @@ def getparam(solver, name):
         buf = torch.empty((ns, ns), dtype=torch.float64, device="cuda")
         _check(lib, h, lib.sds_schur_export(h, ctypes.c_void_p(buf.data_ptr()), ns))
         return buf.t().contiguous()  # the export is column-major
+    if name in ("perm_matching", "scale_row", "scale_col"):
+        m = getattr(solver, "matching", None)
+        if m is None:
+            raise InvalidValueError(f'"{name}" needs a matching (matching_alg other than "default" at the analysis)')
+        return {"perm_matching": m.perm, "scale_row": m.rscale, "scale_col": m.cscale}[name].copy()
     if name == "diag":
         d = np.zeros(2 * solver.n, dtype=np.float64 if solver._dtype == _abi.SDS_R64 else np.float32)
         _check(lib, h, lib.sds_get_diag(h, d.ctypes.data))
