@@ -35,16 +35,30 @@ KIND_KERNEL = {"trsm": "k_big_step", "update_cb": "k_big_update_mfma", "extend_a
 
 def measured_traffic(kind):
     """HBM bytes per launch of the class's kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated), or None when it holds no figure for that kernel."""
+    WRITE_SIZE summaries (copy-calibrated, profiles/*_hbm_traffic.txt), or None when none holds a
+    figure for that kernel."""
+    import glob
     name = KIND_KERNEL.get(kind)
-    path = os.path.join(ROOT, "profiles", "k_big_step_laplacian3d_64_hbm_traffic.txt")
-    if name is None or not os.path.exists(path):
+    if name is None:
         return None
-    for line in open(path):
-        f = line.split()
-        if len(f) == 3 and f[0] == name and f[1] == "traffic_bytes_per_launch":
-            return float(f[2])
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.txt"))):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == name and f[1] == "traffic_bytes_per_launch":
+                return float(f[2])
     return None
+
+
+def bench_copy(args):
+    """Calibration of the HBM counters: a plain device copy of 512 MiB, args.steps times (the
+    FETCH_SIZE / WRITE_SIZE reading of a copy of known size scales the other kernels' readings)."""
+    import torch
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_per_copy": a.numel() * 8, "copies": args.steps}), flush=True)
 
 
 def backward_error(A, X, B):
@@ -65,8 +79,13 @@ def make_workload(k, nrhs, seed=666):
     return A, (rp, cv, nz), perm, B
 
 
-def make_solver(csr, n, perm, amalgamation, factor_precision=None):
-    """Analysis, the tile-shape autotuner on these values, then the first factorization."""
+TILE_KEYS = ("extend_add_rows", "cb_k_chunk", "trailing_split", "step_two_per_cu", "small_front_kernel",
+             "subtree_kib", "panel_block", "pivot_panels")
+
+
+def make_solver(csr, n, perm, amalgamation, factor_precision=None, tiles=None):
+    """Analysis, the tile-shape autotuner on these values (or the given tile shapes), then the
+    first factorization."""
     import sparsedirectsolver as sds
     rp, cv, nz = csr
     s = sds.DirectSolver((rp, cv, nz, n), "SPD", "L")
@@ -75,9 +94,26 @@ def make_solver(csr, n, perm, amalgamation, factor_precision=None):
     if factor_precision:
         sds.setparam(s, "factor_precision", factor_precision)
     sds.execute("analysis", s)
-    s.tiles = sds.autotune(s, reps=2)
+    if tiles:
+        s.tiles = dict(zip(TILE_KEYS, tiles))
+        sds.set_tiles(s, s.tiles)
+    else:
+        s.tiles = sds.autotune(s, reps=2)
     sds.execute("factorization", s)
     return s
+
+
+def solve_inverse_entries(S):
+    """Entries of the diagonal-block inverses the Cholesky sweeps read: per front, per solve step of
+    at most 512 columns, nw^2 (the inverse of the step's diagonal block: 64 x 64 blocks for the
+    narrow steps, the whole nw x nw for the wide ones)."""
+    tot = 0
+    for w in S.front_ncols:
+        w = int(w)
+        for c0 in range(0, w, 512):
+            nw = min(w, c0 + 512) - c0
+            tot += nw * nw
+    return tot
 
 
 def time_steps(s, X, B, steps, warmup, ir_steps):
@@ -481,7 +517,9 @@ def main():
     ap.add_argument("--nrhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint", "kkt"), default="laplacian3d")
+    ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint", "kkt", "copy"), default="laplacian3d")
+    ap.add_argument("--tiles", type=lambda t: [int(x) for x in t.split(",")], default=None,
+                    help="laplacian3d: tile shapes instead of the autotuner (" + ",".join(TILE_KEYS) + ")")
     ap.add_argument("--kkt-nh", type=int, default=8000, help="kkt workload: order of H (J has half as many rows)")
     ap.add_argument("--fd-entries", type=int, default=64, help="adjoint workload: finite-difference checks")
     ap.add_argument("--lu-grid", type=int, default=48, help="adjoint workload: grid side of the LU case")
@@ -498,6 +536,9 @@ def main():
     if args.workload == "adjoint":
         bench_adjoint(args)
         return
+    if args.workload == "copy":
+        bench_copy(args)
+        return
     if args.workload == "kkt":
         bench_kkt(args)
         return
@@ -512,7 +553,7 @@ def main():
     X = torch.zeros_like(B)
 
     # -- FP64 form
-    s64 = make_solver(csr, n, perm, amal)
+    s64 = make_solver(csr, n, perm, amal, tiles=args.tiles)
     S = s64.symbolic
     t_step64 = time_steps(s64, X, B, args.steps, args.warmup, 0)
     berr64 = backward_error(A, X.cpu().numpy(), Bh)
@@ -559,11 +600,20 @@ def main():
                            "traffic": measured_traffic(dom), "algorithmic_bytes_per_launch": round(by / nl),
                            "launches": nl, "avg_launch_ms": round(ms / nl, 4)}
     res["profile_ms"] = {k: round(v[0] / args.steps, 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    # the solve against the HBM roofline: its algorithmic bytes are the factor panels and the
+    # diagonal-block inverses the sweeps multiply by (read once forward, once backward) and the
+    # right-hand sides (B read, Y written and read per sweep, X written)
+    ninv = solve_inverse_entries(S)
+    sbytes = 8.0 * (2.0 * (S.nnz_stored + ninv) + 6.0 * n * args.nrhs)
+    sach = sbytes / (t_solve64 * 1e-3) / 1e9
+    res["solve_roofline"] = {"bound": "hbm", "achieved": round(sach, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                             "frac": round(sach / PEAK["hbm_gbs"], 4), "algorithmic_bytes": round(sbytes),
+                             "factor_entries": S.nnz_stored, "inverse_entries": ninv, "ms": round(t_solve64, 3)}
 
     # -- FP32 factor + FP64 refinement
     f32_factor = None
     if not args.no_fp32:
-        s32 = make_solver(csr, n, perm, amal, "Float32")
+        s32 = make_solver(csr, n, perm, amal, "Float32", tiles=args.tiles)
         steps_needed = None
         for k in range(1, 11):
             sds.setparam(s32, "ir_n_steps", k)

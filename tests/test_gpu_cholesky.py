@@ -318,6 +318,39 @@ def test_panel_blocks_match_oracle(pb, rs, oc, fp):
     assert berr <= (1e-14 if fp is None else 1e-10), f"backward error {berr:.2e}"
 
 
+@pytest.mark.parametrize("sigma", [2.0, 4.0])
+def test_ill_conditioned_solve_matches_the_oracle(sigma):
+    """The sweeps multiply by inverted diagonal blocks (64 x 64, and whole 512-column steps for the
+    wide fronts) where the oracle substitutes (src/reference/cholesky.jl:127-170). On a
+    variable-coefficient Laplacian with conductivities exp(N(0, sigma^2)) (sigma = 4: a condition
+    number of order 1e10 and more) over a 784-column root, the GPU solve's backward error must stay
+    within 10x the oracle's (plus 1e-15), with and without refinement."""
+    import torch
+    k = 28
+    A = M.variable_laplacian((k, k, k), seed=17, sigma=sigma)
+    s, nz = _solver(A, sds.grid_nd_perm(k, k, k), (512, 0.25, 8))
+    sds.execute("factorization", s)
+    assert sds.getparam(s, "info") == 0
+    info, ref, _ = oracle.cholesky(s.symbolic, nz)
+    assert info == 0
+    n = A.shape[0]
+    Bh = np.random.default_rng(3).standard_normal((n, 2))
+    B = torch.tensor(Bh, dtype=torch.float64, device="cuda")
+    Xr = oracle.solve_cholesky(s.symbolic, ref, Bh)
+    an = abs(A).sum(axis=1).max()
+
+    def berr(X):
+        return float(np.max(np.abs(Bh - A @ X).max(axis=0) / (an * np.abs(X).max(axis=0) + np.abs(Bh).max(axis=0))))
+
+    for steps in (0, 2):
+        sds.setparam(s, "ir_n_steps", steps)
+        X = torch.zeros_like(B)
+        sds.execute("solve", s, X, B)
+        got, want = berr(X.cpu().numpy()), berr(Xr)
+        print(f"sigma {sigma} ir {steps}: GPU backward error {got:.2e}, oracle {want:.2e}", flush=True)
+        assert got <= 10 * want + 1e-15, (got, want)
+
+
 def test_interrupt_flag_stops_the_factorization():
     """user_host_interrupt (a one-byte flag) set before the phase: the library's level loop sees
     it before the first launch group and the phase raises InterruptedError
