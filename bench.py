@@ -517,6 +517,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-default-amalgamation", action="store_true",
+                    help="laplacian3d: skip the step at the reference's default amalgamation (32, 0.25, 8)")
     ap.add_argument("--workload", choices=("laplacian3d", "batch", "adjoint", "kkt", "copy"), default="laplacian3d")
     ap.add_argument("--tiles", type=lambda t: [int(x) for x in t.split(",")], default=None,
                     help="laplacian3d: tile shapes instead of the autotuner (" + ",".join(TILE_KEYS) + ")")
@@ -609,6 +611,17 @@ def main():
     res["solve_roofline"] = {"bound": "hbm", "achieved": round(sach, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                              "frac": round(sach / PEAK["hbm_gbs"], 4), "algorithmic_bytes": round(sbytes),
                              "factor_entries": S.nnz_stored, "inverse_entries": ninv, "ms": round(t_solve64, 3)}
+
+    # -- the same FP64 step at the reference's default amalgamation (32, 0.25, 8): what a user of
+    # the reference gets without tuning it (more, narrower supernodes and tree levels)
+    if not args.no_default_amalgamation:
+        sd = make_solver(csr, n, perm, (32, 0.25, 8), tiles=args.tiles)
+        t_def = time_steps(sd, X, B, max(3, args.steps // 4), 1, 0)
+        res["default_amalgamation"] = {"amalgamation": [32, 0.25, 8], "ms_per_step": round(t_def, 3),
+                                       "backward_error": backward_error(A, X.cpu().numpy(), Bh),
+                                       "supernodes": sd.symbolic.ns, "nnz_stored": sd.symbolic.nnz_stored,
+                                       "autotuned_tiles": sd.tiles}
+        del sd
 
     # -- FP32 factor + FP64 refinement
     f32_factor = None
