@@ -576,6 +576,195 @@ V amd_order(const Pattern& P) {
 }
 
 // ----------------------------------------------------------------------------
+// nested dissection of a general graph (reordering_alg "algo4"; the reference takes a library
+// nested dissection there, src/symbolic/ordering.jl:198-204): recursive bisection by level-
+// structure separators (George & Liu, "An automatic nested dissection algorithm for irregular
+// finite element problems", SIAM J. Numer. Anal. 15, 1978). From a pseudo-peripheral vertex, the
+// breadth-first levels; the level where half the vertices are reached, less its vertices with no
+// neighbour in the next level, separates the levels before it from those after; both parts are
+// dissected in turn and the separator is numbered after them. Parts of at most ND_LEAF vertices,
+// and parts whose level structure is too shallow to split, are ordered by minimum degree.
+
+constexpr i64 ND_LEAF = 128;
+
+struct NdWork {
+    const Pattern& P;
+    V loc;  // global vertex -> index in the current part, -1 outside
+    V lev;  // breadth-first level within the current part, -1 unreached
+    V out;  // the ordering, first eliminated first
+};
+
+// minimum-degree order of the part verts (loc must mark it), appended to out
+void nd_leaf(NdWork& W, const V& verts) {
+    const i64 m = (i64)verts.size();
+    Pattern Q;
+    Q.n = m;
+    Q.colptr.assign(m + 1, 0);
+    for (i64 a = 0; a < m; ++a) {
+        const i64 v = verts[a];
+        for (i64 t = 0; t < W.P.deg(v); ++t) {
+            const i64 u = W.P.nb(v)[t];
+            if (W.loc[u] >= 0) Q.rowval.push_back(W.loc[u]);
+        }
+        Q.colptr[a + 1] = (i64)Q.rowval.size();
+        std::sort(Q.rowval.begin() + Q.colptr[a], Q.rowval.end());
+    }
+    for (i64 p : amd_order(Q)) W.out.push_back(verts[p]);
+}
+
+// breadth-first levels of the part from root (lev must be -1 on the part); returns the vertices
+// in level order and the level starts
+void nd_bfs(NdWork& W, i64 root, V& order, V& lstart) {
+    order.assign(1, root);
+    lstart.assign(1, 0);
+    W.lev[root] = 0;
+    for (size_t h = 0; h < order.size(); ++h) {
+        const i64 v = order[h];
+        if (W.lev[v] >= (i64)lstart.size()) lstart.push_back((i64)h);
+        for (i64 t = 0; t < W.P.deg(v); ++t) {
+            const i64 u = W.P.nb(v)[t];
+            if (W.loc[u] >= 0 && W.lev[u] < 0) {
+                W.lev[u] = W.lev[v] + 1;
+                order.push_back(u);
+            }
+        }
+    }
+    lstart.push_back((i64)order.size());
+}
+
+i64 nd_part_degree(const NdWork& W, i64 v) {
+    i64 d = 0;
+    for (i64 t = 0; t < W.P.deg(v); ++t) d += W.loc[W.P.nb(v)[t]] >= 0;
+    return d;
+}
+
+void nd_rec(NdWork& W, V verts, int depth) {
+    const i64 m = (i64)verts.size();
+    if (m == 0) return;
+    for (i64 a = 0; a < m; ++a) W.loc[verts[a]] = a;
+    auto release = [&](const V& vs) {
+        for (i64 v : vs) {
+            W.loc[v] = -1;
+            W.lev[v] = -1;
+        }
+    };
+    if (m <= ND_LEAF || depth > 200) {
+        nd_leaf(W, verts);
+        release(verts);
+        return;
+    }
+    // connected components: dissected one after the other
+    V order, lstart;
+    nd_bfs(W, verts[0], order, lstart);
+    if ((i64)order.size() < m) {
+        std::vector<V> comps;
+        comps.push_back(order);
+        for (i64 v : verts)
+            if (W.lev[v] < 0) {
+                nd_bfs(W, v, order, lstart);
+                comps.push_back(order);
+            }
+        release(verts);
+        for (auto& c : comps) nd_rec(W, c, depth + 1);
+        return;
+    }
+    // pseudo-peripheral root: from a vertex of smallest degree, the last level's vertex of smallest
+    // degree, while the eccentricity grows
+    for (i64 v : verts) W.lev[v] = -1;
+    i64 root = verts[0], best = nd_part_degree(W, root);
+    for (i64 v : verts) {
+        const i64 d = nd_part_degree(W, v);
+        if (d < best) {
+            best = d;
+            root = v;
+        }
+    }
+    nd_bfs(W, root, order, lstart);
+    for (int it = 0; it < 8; ++it) {
+        const i64 ecc = (i64)lstart.size() - 2;
+        i64 cand = order[lstart[ecc]], cd = nd_part_degree(W, cand);
+        for (i64 h = lstart[ecc]; h < lstart[ecc + 1]; ++h)
+            if (nd_part_degree(W, order[h]) < cd) {
+                cand = order[h];
+                cd = nd_part_degree(W, cand);
+            }
+        for (i64 v : verts) W.lev[v] = -1;
+        V o2, l2;
+        nd_bfs(W, cand, o2, l2);
+        if ((i64)l2.size() - 2 <= ecc) {
+            for (i64 v : verts) W.lev[v] = -1;
+            nd_bfs(W, root, order, lstart);
+            break;
+        }
+        root = cand;
+        order.swap(o2);
+        lstart.swap(l2);
+    }
+    const i64 nlev = (i64)lstart.size() - 1;
+    if (nlev < 3) {
+        nd_leaf(W, verts);
+        release(verts);
+        return;
+    }
+    // the level reaching half the vertices, kept off the first and the last level
+    i64 mid = 1;
+    while (mid < nlev - 2 && lstart[mid + 1] < (m + 1) / 2) ++mid;
+    V A, B, sep;
+    for (i64 h = 0; h < m; ++h) {
+        const i64 v = order[h], l = W.lev[v];
+        if (l < mid) {
+            A.push_back(v);
+        } else if (l > mid) {
+            B.push_back(v);
+        } else {
+            bool up = false;
+            for (i64 t = 0; t < W.P.deg(v) && !up; ++t) {
+                const i64 u = W.P.nb(v)[t];
+                up = W.loc[u] >= 0 && W.lev[u] == mid + 1;
+            }
+            (up ? sep : A).push_back(v);
+        }
+    }
+    release(verts);
+    nd_rec(W, A, depth + 1);
+    nd_rec(W, B, depth + 1);
+    for (i64 v : sep) W.out.push_back(v);
+}
+
+V nd_order(const Pattern& P) {
+    NdWork W{P, V(P.n, -1), V(P.n, -1), {}};
+    W.out.reserve(P.n);
+    V all(P.n);
+    std::iota(all.begin(), all.end(), 0);
+    nd_rec(W, all, 0);
+    return W.out;
+}
+
+// ordering_cost (src/symbolic/ordering.jl:86-129): flops (sum of the squared column counts) plus
+// 1e8 flops per level of the amalgamated supernodal schedule, the automatic choice's model
+double ordering_cost(const Pattern& Q, const V& perm, i64 max_width, double zero_fraction, i64 min_width,
+                     int use_superpanels) {
+    const i64 n = Q.n;
+    V iperm(n);
+    for (i64 k = 0; k < n; ++k) iperm[perm[k]] = k;
+    const V parent = etree(Q, perm, iperm);
+    const V post = postorder(parent);
+    const V counts = colcounts(Q, perm, iperm, parent, post);
+    double fl = 0.0;
+    for (i64 c : counts) fl += (double)c * (double)c;
+    ColumnPartition cp = fundamental_supernodes(parent, post, counts);
+    if (use_superpanels) cp = amalgamate(cp, parent, counts, max_width, zero_fraction, min_width);
+    const i64 ns = (i64)cp.snparent.size();
+    V h(ns, 1);
+    i64 depth = ns > 0 ? 1 : 0;
+    for (i64 s = 0; s < ns; ++s) {  // a parent comes after its children
+        depth = std::max(depth, h[s]);
+        if (cp.snparent[s] >= 0) h[cp.snparent[s]] = std::max(h[cp.snparent[s]], h[s] + 1);
+    }
+    return fl + 1e8 * (double)depth;
+}
+
+// ----------------------------------------------------------------------------
 // 2x2 pivot pairs of "S"/"H" (src/symbolic/pairs.jl): a row whose diagonal is absent or
 // negligible is matched with a neighbour; the ordering runs on the graph compressed by the pairs
 // (the partner eliminated right before its candidate) and the symbolic factorization on the pair
@@ -1266,38 +1455,69 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
             if (perm0[k] < 0 || perm0[k] >= n || seen[perm0[k]]) throw std::runtime_error("user_perm is not a permutation");
             seen[perm0[k]] = 1;
         }
-    } else if (ordering == 3) {
-        // 2x2 pivot pairs of "S"/"H" (src/symbolic/ordering.jl:283-295, pairs.jl:330-343): not with a
-        // user permutation, the natural ordering or the Schur complement mode
+    } else if (ordering == 3 || ordering == 4 || ordering == 6) {
+        // 3 approximate minimum degree, 4 nested dissection, 6 the automatic choice between the two
+        // (src/symbolic/ordering.jl:279-326): per algorithm its ordering, with the 2x2 pivot pairs
+        // of "S"/"H" (not with the Schur complement mode; pairs.jl:27-35): fixed pairs (those of a
+        // matching) order the compressed graph; "all" pairs every candidate first; "default" pairs
+        // the structurally zero pivots of the ordering and re-orders the compressed graph until a
+        // round adds no pair. The choice takes the fewest structurally zero pivots left, then the
+        // lower cost (ties keep the minimum degree).
         const bool sym = structure == ST_S || structure == ST_H;
-        if (sym && !schur && nfixed > 0) {
-            // fixed pairs (those of a matching): the compressed graph ordered, then expanded
+        const bool fixedp = sym && !schur && nfixed > 0;
+        const bool search = sym && !schur && !fixedp && pair_mode != PAIRS_NONE && values;
+        PairCands pc;
+        if (search) pc = pivot_candidates(P, rp, cv, values, structure, view, pair_tol);
+        if (fixedp)
             for (i64 t = 0; t < nfixed; ++t) pairs.push_back({fixed_pairs[2 * t] - 1, fixed_pairs[2 * t + 1] - 1});
+        auto base = [&](const Pattern& G, int alg) { return alg == 4 ? nd_order(G) : amd_order(G); };
+        auto compressed_order = [&](const Pairs& prs, int alg) {
             V gptr, members;
-            const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
-            perm0 = expand_permutation(amd_order(Pc), gptr, members);
-        } else if (sym && !schur && pair_mode != PAIRS_NONE && values) {
-            const PairCands pc = pivot_candidates(P, rp, cv, values, structure, view, pair_tol);
-            if (pair_mode == PAIRS_ALL) {
-                pairs = pivot_pairs_all(P, pc);
-                V gptr, members;
-                const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
-                perm0 = expand_permutation(amd_order(Pc), gptr, members);
+            const Pattern Pc = compressed_pattern(P, prs, gptr, members);
+            return expand_permutation(base(Pc, alg), gptr, members);
+        };
+        struct Cand {
+            V perm;
+            Pairs prs;
+            int rounds = 0;
+        };
+        auto order_on = [&](int alg) {
+            Cand c;
+            if (fixedp) {
+                c.prs = pairs;
+                c.perm = compressed_order(c.prs, alg);
+            } else if (search && pair_mode == PAIRS_ALL) {
+                c.prs = pivot_pairs_all(P, pc);
+                c.perm = compressed_order(c.prs, alg);
             } else {
-                // "default": pair the structurally zero pivots of the ordering, re-order the
-                // compressed graph, until a round adds no pair (src/symbolic/pairs.jl:322-335)
-                perm0 = amd_order(P);
-                while (pair_rounds < PAIR_MAX_ROUNDS) {
-                    if (zero_pivot_pairs(pairs, pc, P, perm0, pivot_u) == 0) break;
-                    ++pair_rounds;
-                    V gptr, members;
-                    const Pattern Pc = compressed_pattern(P, pairs, gptr, members);
-                    perm0 = expand_permutation(amd_order(Pc), gptr, members);
+                c.perm = base(P, alg);
+                while (search && c.rounds < PAIR_MAX_ROUNDS) {
+                    if (zero_pivot_pairs(c.prs, pc, P, c.perm, pivot_u) == 0) break;
+                    ++c.rounds;
+                    c.perm = compressed_order(c.prs, alg);
                 }
             }
-        } else {
-            perm0 = amd_order(P);
+            return c;
+        };
+        const std::vector<int> algs = ordering == 6 ? std::vector<int>{3, 4} : std::vector<int>{ordering};
+        Cand best;
+        size_t best_zeros = 0;
+        double best_cost = 0.0;
+        for (size_t q = 0; q < algs.size(); ++q) {
+            Cand c = order_on(algs[q]);
+            if (algs.size() > 1) {
+                const size_t zeros = search ? structural_zero_pivots(pc, P, c.perm, c.prs).size() : 0;
+                const double cost = ordering_cost(pair_pattern(P, c.prs), c.perm, max_width, zero_fraction, min_width,
+                                                  use_superpanels);
+                if (q > 0 && !(zeros < best_zeros || (zeros == best_zeros && cost < best_cost))) continue;
+                best_zeros = zeros;
+                best_cost = cost;
+            }
+            best = std::move(c);
         }
+        perm0 = std::move(best.perm);
+        pairs = std::move(best.prs);
+        pair_rounds = best.rounds;
     } else {
         std::iota(perm0.begin(), perm0.end(), 0);
     }
@@ -1568,7 +1788,8 @@ Analysis* analyze(i64 n, const i64* rowptr_in, const i64* colval_in, int structu
 extern "C" {
 
 // Run the analysis. ordering: 0 natural, 1 user permutation (perm must be
-// given), 3 approximate minimum degree. Returns NULL on error
+// given), 3 approximate minimum degree, 4 nested dissection, 6 the automatic choice between 3 and
+// 4 by the reference's cost model. Returns NULL on error
 // (sdsa_last_error() holds the message).
 // schur: NULL, or n flags (original rows) of the Schur complement mode, whose rows and columns go
 // last and form the last supernode.

@@ -23,7 +23,9 @@ PIVOT_PAIRS = {"none": 0, "default": 1, "all": 2}
 MATCHING_JOBS = {"algo1": 1, "algo2": 2, "algo3": 3, "algo4": 4, "algo5": 5, "algo6": 5}
 
 # "reordering_alg" spellings (src/types.jl:100-107) and what this analysis does for them
-ORDERINGS = {"default": 3, "algo3": 3, "algo5": 0, "natural": 0, "user": 1}
+# (src/types.jl:98-113): "default" the automatic minimum degree / nested dissection choice, "algo1" and
+# "algo2" (COLAMD variants) fall back to the symmetric-pattern minimum degree as in the reference
+ORDERINGS = {"default": 6, "algo1": 3, "algo2": 3, "algo3": 3, "algo4": 4, "algo5": 0, "natural": 0, "user": 1}
 
 
 @dataclass

@@ -205,7 +205,8 @@ class DirectSolver:
         if o["ubatch_size"] not in (0, self.nbatch):
             raise InvalidValueError(f"ubatch_size = {o['ubatch_size']}, but the matrix values hold {self.nbatch} "
                                     "batch members")
-        ordering = {"default": "default", "algo3": "algo3", "algo5": "algo5"}.get(o["reordering_alg"])
+        ordering = o["reordering_alg"] if o["reordering_alg"] in ("default", "algo1", "algo2", "algo3", "algo4",
+                                                                  "algo5") else None
         if ordering is None and o["user_perm"] is None:
             raise NotSupportedError(f"reordering_alg = {o['reordering_alg']!r} is not available in this build")
         if self.nbatch > 1:
@@ -219,7 +220,7 @@ class DirectSolver:
             m = self.matching
             # 2x2 pivot pairs (src/symbolic/pairs.jl:27-35): "S"/"H", the AMD ordering, no user_perm
             pp = o["pivot_pairs"]
-            pairs_on = (self.structure in ("S", "H") and o["user_perm"] is None and ordering in ("default", "algo3")
+            pairs_on = (self.structure in ("S", "H") and o["user_perm"] is None and ordering != "algo5"
                         and pp != "none" and schur is None)
             colval, values, fixed = self.colval, None, None
             if m is not None and self.structure == "G":

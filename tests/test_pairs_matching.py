@@ -302,3 +302,43 @@ def test_mutations_are_caught():
         return out
 
     assert len(slipped()) > len(good)
+
+
+# ------------------------------------------------------------------------------------------------
+# nested dissection and the automatic ordering choice (src/symbolic/ordering.jl:86-326)
+
+def _levels(S):
+    """Depth of the supernodal tree of an analysis (its schedule depth)."""
+    par = S.snparent.astype(np.int64) - 1
+    h = np.ones(S.ns, dtype=np.int64)
+    for s in range(S.ns):
+        if par[s] >= 0:
+            h[par[s]] = max(h[par[s]], h[s] + 1)
+    return int(h.max())
+
+
+@pytest.mark.parametrize("name,make", [("lap3d_20", lambda: M.laplacian3d(20, 20, 20)),
+                                       ("lap2d_60x50", lambda: M.laplacian2d(60, 50)),
+                                       ("kkt_2000_1000", lambda: M.kkt_matrix(2000, 1000)),
+                                       ("two_blocks", lambda: sp.block_diag([M.laplacian3d(9, 9, 9),
+                                                                             M.laplacian2d(30, 30)]).tocsr())],
+                         ids=["lap3d_20", "lap2d_60x50", "kkt_2000_1000", "two_blocks"])
+def test_nested_dissection_and_the_automatic_choice(name, make):
+    """Nested dissection ("algo4") is a permutation whose analysis passes the structural checks;
+    on 3-D grids it needs fewer flops than minimum degree; "default" takes the ordering of lower
+    cost = flops + 1e8 x schedule depth (ties: minimum degree)."""
+    A = make()
+    n = A.shape[0]
+    st = "S" if name.startswith("kkt") else "SPD"
+    L = M.lower_csr(A)
+    rp, cv, nz = M.csr_arrays(L)
+    res = {}
+    for o in ("algo3", "algo4", "default"):
+        S = symbolic_analysis(rp, cv, n, st, "L", ordering=o, amalgamation=(32, 0.25, 8), values=nz,
+                              pivot_pairs="none")
+        assert sorted(S.etree_perm.tolist()) == list(range(1, n + 1))
+        res[o] = (S.flops_true + 1e8 * _levels(S), S)
+    if name.startswith("lap3d"):
+        assert res["algo4"][1].flops_true < res["algo3"][1].flops_true
+    best = "algo4" if res["algo4"][0] < res["algo3"][0] else "algo3"
+    assert np.array_equal(res["default"][1].etree_perm, res[best][1].etree_perm)
