@@ -2,7 +2,7 @@
 
 Workload (one "step" = refactorization + solve with 8 right-hand sides):
 3-D 7-point Laplacian on a 64^3 grid (n = 262,144), SPD, Float64 values,
-geometric nested dissection, amalgamation (512, 0.25, 8), B ~ N(0, 1) n x 8.
+nested dissection of the matrix graph (reordering_alg "algo4"), amalgamation (512, 0.25, 8), B ~ N(0, 1) n x 8.
 Two forms are timed: the native FP64 factor + solve, and the FP32 factor with
 FP64 iterative refinement (the residual and the update of x in FP64) run for
 the number of steps that reaches a normwise backward error <= 1e-10.
@@ -69,14 +69,12 @@ def backward_error(A, X, B):
 
 
 def make_workload(k, nrhs, seed=666):
-    import sparsedirectsolver as sds
     from sparsedirectsolver import matrices as M
     A = M.laplacian3d(k, k, k)
     L = M.lower_csr(A)
     rp, cv, nz = M.csr_arrays(L)
-    perm = sds.grid_nd_perm(k, k, k)
     B = np.random.default_rng(seed).standard_normal((A.shape[0], nrhs))
-    return A, (rp, cv, nz), perm, B
+    return A, (rp, cv, nz), None, B
 
 
 TILE_KEYS = ("extend_add_rows", "cb_k_chunk", "trailing_split", "step_two_per_cu", "small_front_kernel",
@@ -89,7 +87,10 @@ def make_solver(csr, n, perm, amalgamation, factor_precision=None, tiles=None):
     import sparsedirectsolver as sds
     rp, cv, nz = csr
     s = sds.DirectSolver((rp, cv, nz, n), "SPD", "L")
-    sds.setparam(s, "user_perm", perm)
+    if perm is not None:
+        sds.setparam(s, "user_perm", perm)
+    else:
+        sds.setparam(s, "reordering_alg", "algo4")  # nested dissection of the matrix graph
     sds.setparam(s, "amalgamation", amalgamation)
     if factor_precision:
         sds.setparam(s, "factor_precision", factor_precision)
@@ -578,7 +579,8 @@ def main():
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"3-D 7-point Laplacian {args.grid}^3 (n = {n}), SPD Cholesky, "
                                f"{args.nrhs} right-hand sides, refactorization + solve",
-                   "ordering": "geometric nested dissection", "amalgamation": list(amal), "nrhs": args.nrhs,
+                   "ordering": "nested dissection of the matrix graph (reordering_alg algo4)",
+                   "amalgamation": list(amal), "nrhs": args.nrhs,
                    "nnz_L": S.nnz_L, "nnz_stored": S.nnz_stored, "supernodes": S.ns,
                    "factor_flops": flops_true, "parallelism": "single GPU", "autotuned_tiles": s64.tiles},
         "fp64": {"refactor_ms": round(t_refact64, 3), "solve_ms": round(t_solve64, 3),

@@ -14,7 +14,7 @@
 // inverse Linv of its single diagonal block; a wide step uses X, the inverse of its whole
 // diagonal block, formed after the factorization by k_step_inverse. Per tree level and step:
 //   forward : pull the children's update vectors and form y_step = inverse * t_step:
-//             k_fwd_diag (narrow, one workgroup per front; a narrow front with at most FUSE_ROWS
+//             k_fwd_diag (narrow, one workgroup per step; a narrow front with at most FUSE_ROWS
 //             rows below its columns forms its update vector there too) or k_fwd_inv (wide,
 //             16 rows per workgroup, into Z, then k_copy_rows); then k_fwd_rows adds
 //             L(rows, step) y_step to the rows below the step, 64 per workgroup (the update
@@ -137,8 +137,9 @@ __device__ __attribute__((always_inline)) inline T wave_reduce_scatter64(T (&a)[
 // ---------------------------------------------------------------------------
 // forward
 
-// one workgroup per narrow front (one solve step of at most NB columns): task (step, fused).
-// Pull the children's update vectors; y = Linv t (entry (j, r) per thread, Linv rows read
+// one workgroup per narrow step (at most NB columns: a whole narrow front, or the last columns
+// [c0, c1) of a wide front): task (step, fused). At c0 = 0 pull the children's update vectors;
+// y = Linv t over the step's columns (entry (j, r) per thread, Linv rows read
 // coalesced, t in LDS); a fused front (at most FUSE_ROWS rows below) also forms its update
 // vector u_i = L(i, front) y, one row per thread. LDS: 2 NB 8 entries.
 template <typename T>
@@ -151,17 +152,19 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
     const int tid = threadIdx.x, nt = blockDim.x;
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
-    const int s = st.s;
+    const int s = st.s, c0 = st.c0, nw = st.c1 - c0;
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
     const int cg = S.super_ptr[s];
     const T* P = factor + S.panel_off[s];
-    const bool fused = tk.a != 0;
+    const bool fused = tk.a != 0;  // only a whole narrow front (c0 = 0, nw = w) is fused
     T* us = fused && m > 0 && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
     if (us)
         for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
     __syncthreads();
-    // the children's update vectors, in child_list order (only the rows < w unless fused)
-    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
+    // the children's update vectors, in child_list order (only the rows < w unless fused), at the
+    // front's first step; the last narrow step of a wide front (c0 > 0) got its rows' entries
+    // from k_fwd_rows at the first step
+    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1] && c0 == 0; ++kc) {
         const int c = S.child_list[kc];
         if (S.uv_off[c] < 0) continue;
         const int mc = S.fnrows[c] - S.fncols[c];
@@ -178,13 +181,13 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
         }
         __syncthreads();
     }
-    T* y = Y + (int64_t)cg * nrhs;
-    const T* Li = DB + (int64_t)cg * NB;
+    T* y = Y + (int64_t)(cg + c0) * nrhs;
+    const T* Li = DB + (int64_t)(cg + c0) * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
-        for (int q = tid; q < w * 8; q += nt) tb[q] = (q & 7) < nr ? y[(q >> 3) * nrhs + rb + (q & 7)] : T(0);
+        for (int q = tid; q < nw * 8; q += nt) tb[q] = (q & 7) < nr ? y[(q >> 3) * nrhs + rb + (q & 7)] : T(0);
         __syncthreads();
-        for (int q = tid; q < w * 8; q += nt) {
+        for (int q = tid; q < nw * 8; q += nt) {
             const int j = q >> 3, r = q & 7;
             T acc = T(0);
 #pragma unroll 8
@@ -199,7 +202,7 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
 #pragma unroll
                 for (int r = 0; r < 8; ++r) a[r] = T(0);
 #pragma unroll 8
-                for (int j = 0; j < w; ++j) {
+                for (int j = 0; j < nw; ++j) {
                     const T lv = P[(int64_t)j * f + i];
 #pragma unroll
                     for (int r = 0; r < 8; ++r) a[r] += lv * yx[j * 8 + r];
@@ -349,8 +352,9 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
     }
 }
 
-// one workgroup per narrow front: task (step, 0). t = y - the L21^T x partial sums (chunk
-// order), x = Linv^T t (entry (j, r) per thread). LDS: NB 8 entries.
+// one workgroup per narrow step (a whole narrow front, or the last columns [c0, c1) of a wide
+// front): task (step, 0). t = y - the L21^T x partial sums (chunk order), x = Linv^T t over the
+// step's columns (entry (j, r) per thread). LDS: NB 8 entries.
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
                                                  const Task* __restrict__ tasks, const T* __restrict__ factor,
@@ -360,28 +364,28 @@ __global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __r
     const int tid = threadIdx.x, nt = blockDim.x;
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
-    const int s = st.s, w = st.c1;
+    const int s = st.s, c0 = st.c0, nw = st.c1 - c0;
     const int cg = S.super_ptr[s];
-    T* y = Y + (int64_t)cg * nrhs;
+    T* y = Y + (int64_t)(cg + c0) * nrhs;
     const T* pp = part + st.poff * nrhs;
-    const T* Li = DB + (int64_t)cg * NB;
+    const T* Li = DB + (int64_t)(cg + c0) * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
-        for (int q = tid; q < w * 8; q += nt) {
+        for (int q = tid; q < nw * 8; q += nt) {
             const int j = q >> 3, r = q & 7;
             T v = T(0);
             if (r < nr) {
                 v = y[j * nrhs + rb + r];
-                for (int ch = 0; ch < st.nch; ++ch) v -= pp[((int64_t)ch * w + j) * nrhs + rb + r];
+                for (int ch = 0; ch < st.nch; ++ch) v -= pp[((int64_t)ch * nw + j) * nrhs + rb + r];
             }
             tb[q] = v;
         }
         __syncthreads();
-        for (int q = tid; q < w * 8; q += nt) {
+        for (int q = tid; q < nw * 8; q += nt) {
             const int j = q >> 3, r = q & 7;
             T acc = T(0);
 #pragma unroll 8
-            for (int i = j; i < w; ++i) acc += Li[(int64_t)j * NB + i] * tb[i * 8 + r];
+            for (int i = j; i < nw; ++i) acc += Li[(int64_t)j * NB + i] * tb[i * 8 + r];
             if (r < nr) y[j * nrhs + rb + r] = acc;
         }
         __syncthreads();

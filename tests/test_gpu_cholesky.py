@@ -17,17 +17,46 @@ pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
 
 
-def _solver(A, perm=None, amalgamation=(32, 0.25, 8), fp=None):
+def _solver(A, perm=None, amalgamation=(32, 0.25, 8), fp=None, ordering=None):
     L = M.lower_csr(A)
     rp, cv, nz = M.csr_arrays(L)
     s = sds.DirectSolver((rp, cv, nz, A.shape[0]), "SPD", "L")
     if perm is not None:
         sds.setparam(s, "user_perm", perm)
+    if ordering is not None:
+        sds.setparam(s, "reordering_alg", ordering)
     sds.setparam(s, "amalgamation", amalgamation)
     if fp:
         sds.setparam(s, "factor_precision", fp)
     sds.execute("analysis", s)
     return s, nz
+
+
+def _block_arrow(nb=2, bs=540, tail=530, link=100, seed=5):
+    """nb dense bs x bs blocks whose first columns couple to `link` random columns of a dense
+    tail block, diagonally dominant. In natural order every block is one bs-column front with
+    `link` rows below and the tail is the root."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    n = nb * bs + tail
+    rows, cols, vals = [], [], []
+
+    def dense(o, k):
+        r, c = np.tril_indices(k, -1)
+        rows.extend(r + o)
+        cols.extend(c + o)
+        vals.extend(rng.uniform(-1, 1, r.size))
+
+    for b in range(nb):
+        dense(b * bs, bs)
+        t = nb * bs + rng.choice(tail, link, replace=False)
+        rows.extend(t)
+        cols.extend([b * bs] * link)
+        vals.extend(rng.uniform(-1, 1, link))
+    dense(nb * bs, tail)
+    L = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    A = L + L.T
+    return (A + sp.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)).tocsr()
 
 
 CASES = [
@@ -43,6 +72,9 @@ CASES = [
     ("lap3d_28_nd_wide", lambda: (M.laplacian3d(28, 28, 28), sds.grid_nd_perm(28, 28, 28), (512, 0.25, 8))),
     # the reference's CPU-runnable configuration: 2-D 100 x 100, the build's AMD ordering
     ("lap2d_100x100_amd", lambda: (M.laplacian2d(100, 100), None, (32, 0.25, 8))),
+    # fronts of 540 columns with 100 rows below and a 526-column root: a wide solve step of 512
+    # columns followed by a narrow one (28 and 14 columns) that starts at column 512
+    ("block_arrow_540_narrow_last_step", lambda: (_block_arrow(), np.arange(1, 1611), (512, 0.25, 8))),
 ]
 
 
@@ -123,15 +155,20 @@ def test_refinement_fp64_and_fp32():
         assert berr.max() <= tol, (fp, berr.max())
 
 
-def test_laplacian3d_64_full_size_backward_error():
-    """The benchmark's configuration at full size (3-D 64^3, nested dissection, 8 right-hand
-    sides): size-independent checks, as the oracle would take minutes here. FP64 backward
-    error <= 1e-14, inertia-free stats (every pivot positive), bitwise-reproducible
-    refactorization and solve."""
+@pytest.mark.parametrize("ordering", ["grid_nd", "algo4"])
+def test_laplacian3d_64_full_size_backward_error(ordering):
+    """The benchmark's configuration at full size (3-D 64^3, nested dissection of the grid or of
+    the graph, 8 right-hand sides): size-independent checks, as the oracle would take minutes
+    here. FP64 backward error <= 1e-14, inertia-free stats (every pivot positive),
+    bitwise-reproducible refactorization and solve. The graph dissection gives wide fronts with
+    rows below whose last solve step is narrow."""
     import torch
     k = 64
     A = M.laplacian3d(k, k, k)
-    s, nz = _solver(A, sds.grid_nd_perm(k, k, k), (512, 0.25, 8))
+    if ordering == "grid_nd":
+        s, nz = _solver(A, sds.grid_nd_perm(k, k, k), (512, 0.25, 8))
+    else:
+        s, nz = _solver(A, None, (512, 0.25, 8), ordering=ordering)
     sds.execute("factorization", s)
     assert sds.getparam(s, "info") == 0
     n = A.shape[0]
