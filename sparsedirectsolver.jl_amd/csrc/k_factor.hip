@@ -376,44 +376,53 @@ __global__ void __launch_bounds__(256) k_big_zero_scatter(DevSym S, const Task* 
     }
 }
 
-// extend-add of one child per front (the launch covers child rank r of every front);
-// task: (s, child, jj0, njj + 16 r0): child columns jj0 .. jj0 + njj - 1 (njj <= EA_COLS),
-// child rows r0 .. r0 + ea_rows - 1 (and below the diagonal). A thread takes one child row at
-// a time (its parent row read once) and all the task's columns of that row: the child reads
-// and the parent read-modify-writes are coalesced along the rows, several in flight per thread.
+// extend-add, owner-pull: task (s, pc0, e0, ne) owns the parent entries of the columns
+// [pc0, pc1) (at most EA_COLS, never across w) in one range of rows, and adds into them the
+// entries of the children that land there, child by child in child_list order (a barrier
+// between children: two children can hit one parent entry). Its ne entries, tasks[e0 ..] from
+// the launch's first record: (child c, first child column jj0 landing in the block, child rows
+// [ii0, ii1)). A thread takes one child row at a time (its parent row read once) and the
+// block's columns of that row: the child reads and the parent read-modify-writes are coalesced
+// along the rows, several in flight per thread. Each parent entry gets its children's entries
+// in child_list order, as in the reference's owner-pull extend-add (src/numeric/assembly.jl:
+// 180-198), so the sums are bitwise those of a child-by-child sweep.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_extend_add(DevSym S, const Task* __restrict__ tasks,
-                                                       T* __restrict__ factor, T* __restrict__ cbpool, int ea_rows) {
+                                                       T* __restrict__ factor, T* __restrict__ cbpool) {
     const Task t = tasks[blockIdx.x];
     const int s = t.s, f = S.fnrows[s], w = S.fncols[s], m = f - w;
-    const int c = t.a, mc = S.fnrows[c] - S.fncols[c];
+    const int pc0 = t.a, pc1 = min(pc0 + EA_COLS, pc0 < w ? w : f);
     T* P = factor + S.panel_off[s];
     T* Cs = cbpool + (S.cb_off[s] >= 0 ? S.cb_off[s] : 0);
-    const T* Cc = cbpool + S.cb_off[c];
-    const int32_t* ri = S.relind + S.relind_ptr[c];
-    const int jj0 = t.b, nj = min(t.c & 15, mc - jj0), r0 = t.c >> 4;
-    const int iend = min(mc, r0 + ea_rows);
-    int pj[EA_COLS];
+    for (int e = 0; e < t.c; ++e) {
+        const Task E = tasks[t.b + e];
+        const int c = E.s, jj0 = E.a, mc = S.fnrows[c] - S.fncols[c];
+        const T* Cc = cbpool + S.cb_off[c];
+        const int32_t* ri = S.relind + S.relind_ptr[c];
+        int pj[EA_COLS];
 #pragma unroll
-    for (int q = 0; q < EA_COLS; ++q) pj[q] = ri[jj0 + min(q, nj - 1)];
-    for (int ii = max(jj0, r0) + threadIdx.x; ii < iend; ii += 256) {
-        const int pi = ri[ii];
-        T v[EA_COLS], old[EA_COLS];
-        T* dst[EA_COLS];
+        for (int q = 0; q < EA_COLS; ++q) pj[q] = jj0 + q < mc ? ri[jj0 + q] : f;  // f: past the block
+        for (int ii = E.b + threadIdx.x; ii < E.c; ii += 256) {
+            const int pi = ri[ii];
+            T v[EA_COLS], old[EA_COLS];
+            T* dst[EA_COLS];
 #pragma unroll
-        for (int q = 0; q < EA_COLS; ++q) {
-            const int jj = jj0 + q;
-            const bool in = q < nj && ii >= jj;
-            // parent entry (pi, pj): panel column pj < w, else the contribution block
-            dst[q] = !in ? nullptr : pj[q] < w ? P + (int64_t)pj[q] * f + pi : Cs + (int64_t)(pj[q] - w) * m + (pi - w);
-            // the child's block is read once: keep it out of the caches the parent's entries use
-            v[q] = in ? __builtin_nontemporal_load(Cc + (int64_t)jj * mc + ii) : T(0);
+            for (int q = 0; q < EA_COLS; ++q) {
+                const int jj = jj0 + q;
+                const bool in = pj[q] < pc1 && ii >= jj;
+                // parent entry (pi, pj): panel column pj < w, else the contribution block
+                dst[q] = !in ? nullptr
+                             : pj[q] < w ? P + (int64_t)pj[q] * f + pi : Cs + (int64_t)(pj[q] - w) * m + (pi - w);
+                // the child's block is read once: keep it out of the caches the parent's entries use
+                v[q] = in ? __builtin_nontemporal_load(Cc + (int64_t)jj * mc + ii) : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < EA_COLS; ++q) old[q] = dst[q] ? *dst[q] : T(0);
+#pragma unroll
+            for (int q = 0; q < EA_COLS; ++q)
+                if (dst[q]) *dst[q] = old[q] + v[q];
         }
-#pragma unroll
-        for (int q = 0; q < EA_COLS; ++q) old[q] = dst[q] ? *dst[q] : T(0);
-#pragma unroll
-        for (int q = 0; q < EA_COLS; ++q)
-            if (dst[q]) *dst[q] = old[q] + v[q];
+        __syncthreads();  // the next child may add into the same parent entries
     }
 }
 
@@ -1196,9 +1205,8 @@ void launch_big_zero_scatter(hipStream_t st, const DevSym& S, const Task* tasks,
 }
 
 template <typename T>
-void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
-                           int ea_rows) {
-    hipLaunchKernelGGL(k_big_extend_add<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, ea_rows);
+void launch_big_extend_add(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool) {
+    hipLaunchKernelGGL(k_big_extend_add<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool);
 }
 
 template <typename T>
@@ -1284,7 +1292,7 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
     template void launch_front_tiny_chol<T>(hipStream_t, const DevSym&, const int32_t*, int, const double*, T*, T*, \
                                             T*, int32_t*, double);                                                \
     template void launch_big_zero_scatter<T>(hipStream_t, const DevSym&, const Task*, int, const double*, T*, T*); \
-    template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int);            \
+    template void launch_big_extend_add<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*);                 \
     template void launch_big_diag<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, int32_t*, double);     \
     template void launch_big_step<T>(hipStream_t, const DevSym&, const Task*, int, int, int, T*, T*, T*, T*,      \
                                      int32_t*, double, int, int);                                                  \

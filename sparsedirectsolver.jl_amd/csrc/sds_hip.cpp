@@ -172,6 +172,8 @@ struct sds_handle {
     std::vector<int32_t> perm, super_ptr, fnrows, fncols, snparent, child_ptr, child_list;
     std::vector<int64_t> panel_off, asm_count;
     std::vector<int32_t> cb_split;  // per child: its contribution-block columns that land in the parent's panel
+    std::vector<int64_t> relind_ptr;  // per child: positions of its contribution-block rows among its parent's
+    std::vector<int32_t> relind;      // rows (increasing), as on the device
     // inverses of the diagonal blocks (Cholesky): blocks of at most 16, 32 and NB columns
     Launch inv_launch[3]{{L_POTRF, 0, 0, 0, 0}, {L_POTRF, 0, 0, 0, 0}, {L_POTRF, 0, 0, 0, 0}};
     SolveStep* d_steps = nullptr;            // solve steps of the Cholesky sweeps
@@ -701,6 +703,57 @@ int build_schedule(sds_handle* h, int dtype) {
     // before the factorization, the contribution-block columns after the update
     const bool chol = h->method == M_CHOLESKY;
     auto push_extend = [&](Level& L, bool cb_part, int kind) {
+        if (kind == L_EXTEND_ADD) {
+            // owner-pull, one launch: a task owns the parent entries of EA_COLS parent columns
+            // (never across w) and ea_rows parent rows from the block's first column, and adds
+            // the children's entries that land there in child_list order. Its entry list (after
+            // the launch's task records, in the same task array): (child, first child column of
+            // the block, child rows [ii0, ii1)). Cholesky: the panel columns [0, w) before the
+            // factorization, the contribution-block columns [w, f) after the update; otherwise
+            // all of them before the factorization.
+            std::vector<Task> rec, ent;
+            double by = 0;
+            auto lb = [&](int32_t c, int v) {
+                const int32_t* r0 = h->relind.data() + h->relind_ptr[c];
+                const int32_t* r1 = h->relind.data() + h->relind_ptr[c + 1];
+                return (int)(std::lower_bound(r0, r1, v) - r0);
+            };
+            for (int32_t s : L.big) {
+                const int f = h->fnrows[s], w = h->fncols[s];
+                const int p0 = chol && cb_part ? w : 0, p1 = chol && !cb_part ? w : f;
+                for (int kc = h->child_ptr[s]; kc < h->child_ptr[s + 1]; ++kc) {
+                    const int32_t c = h->child_list[kc];
+                    if (cb_off[c] < 0) continue;
+                    const double mc = h->fnrows[c] - h->fncols[c];
+                    const double j0 = lb(c, p0), j1 = lb(c, p1);
+                    // read the child's block, read + write the parent
+                    by += 3.0 * dbl_es * ((mc - j0) * (mc - j0 + 1) - (mc - j1) * (mc - j1 + 1)) / 2;
+                }
+                // column blocks never straddle w (the kernel ends a block at w)
+                for (int pc0 = p0; pc0 < p1; pc0 = pc0 < w && pc0 + EA_COLS > w ? w : pc0 + EA_COLS) {
+                    const int pc1 = std::min(pc0 + EA_COLS, pc0 < w ? w : p1);
+                    for (int pr0 = pc0; pr0 < f; pr0 += h->ea_rows) {
+                        const int pr1 = std::min(pr0 + h->ea_rows, f);
+                        const int32_t e0 = (int32_t)ent.size();
+                        for (int kc = h->child_ptr[s]; kc < h->child_ptr[s + 1]; ++kc) {
+                            const int32_t c = h->child_list[kc];
+                            if (cb_off[c] < 0) continue;
+                            const int jlo = lb(c, pc0), jhi = lb(c, pc1);
+                            if (jlo == jhi) continue;
+                            const int ilo = std::max(jlo, lb(c, pr0)), ihi = lb(c, pr1);
+                            if (ilo < ihi) ent.push_back({c, jlo, ilo, ihi});
+                        }
+                        if ((int32_t)ent.size() > e0) rec.push_back({s, pc0, e0, (int32_t)ent.size() - e0});
+                    }
+                }
+            }
+            if (rec.empty()) return;
+            for (Task& r : rec) r.b += (int32_t)rec.size();  // entry offsets from the launch's first record
+            L.plan.push_back({kind, (int64_t)tasks.size(), (int)rec.size(), 0, by});
+            tasks.insert(tasks.end(), rec.begin(), rec.end());
+            tasks.insert(tasks.end(), ent.begin(), ent.end());
+            return;
+        }
         int maxch = 0;
         for (int32_t s : L.big) maxch = std::max(maxch, h->child_ptr[s + 1] - h->child_ptr[s]);
         for (int r = 0; r < maxch; ++r) {  // one launch per child rank: each parent entry has one writer
@@ -711,18 +764,8 @@ int build_schedule(sds_handle* h, int dtype) {
                 int32_t c = h->child_list[h->child_ptr[s] + r];
                 if (cb_off[c] < 0) continue;
                 const int mc = h->fnrows[c] - h->fncols[c];
-                const int j0 = chol && cb_part ? h->cb_split[c] : 0, j1 = chol && !cb_part ? h->cb_split[c] : mc;
-                if (kind == L_LU_EXTEND) {
-                    for (int jj = j0; jj < j1; jj += 4) t.push_back({s, c, jj, std::min(4, j1 - jj)});
-                } else {
-                    for (int jj = j0; jj < j1; jj += EA_COLS)
-                        for (int r0 = jj - jj % h->ea_rows; r0 < mc; r0 += h->ea_rows)
-                            t.push_back({s, c, jj, std::min(EA_COLS, j1 - jj) + 16 * r0});
-                }
-                // read the child's block, read + write the parent
-                const double e = kind == L_LU_EXTEND ? (double)mc * mc
-                                                     : ((double)(mc - j0) * (mc - j0 + 1) - (double)(mc - j1) * (mc - j1 + 1)) / 2;
-                by += 3.0 * dbl_es * e;
+                for (int jj = 0; jj < mc; jj += 4) t.push_back({s, c, jj, std::min(4, mc - jj)});
+                by += 3.0 * dbl_es * (double)mc * mc;  // read the child's block, read + write the parent
             }
             push_launch(L, kind, t, 0, by);
         }
@@ -1271,7 +1314,7 @@ int factorize_cholesky(sds_handle* h, const double* nz, double eps, const volati
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
-                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb, h->ea_rows); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
                     case L_POTRF:
                         launch_big_diag<T>(st, h->dsym, t, ln.n, factor, static_cast<T*>(h->dbuf), h->status, eps);
                         break;
@@ -1345,7 +1388,7 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
             timed(h, prof_kind(ln.kind), ln.flops, ln.bytes, [&] {
                 switch (ln.kind) {
                     case L_ZERO_SCATTER: launch_big_zero_scatter<T>(st, h->dsym, t, ln.n, nz, factor, cb); break;
-                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb, h->ea_rows); break;
+                    case L_EXTEND_ADD: launch_big_extend_add<T>(st, h->dsym, t, ln.n, factor, cb); break;
                     case L_LDLT_PIVOT:
                         launch_ldlt_pivot<T>(st, h->dsym, t, ln.n, ln.k, factor, dv, h->piv, h->pkind, h->stats,
                                              h->lstep, pivot, u, eps, scaled, h->aux, psign);
@@ -1947,9 +1990,11 @@ int sds_upload_symbolic(sds_handle_t h, const sds_symbolic_desc* d) {
     for (int64_t s = 0; s <= ns; ++s) h->child_ptr[s] = (int32_t)(cptr[s] - 1);
     h->child_list.resize(h->child_ptr[ns]);
     for (int64_t q = 0; q < h->child_ptr[ns]; ++q) h->child_list[q] = (int32_t)(clist[q] - 1);
-    std::vector<int64_t> relind_ptr(ns + 1);
+    std::vector<int64_t>& relind_ptr = h->relind_ptr;
+    relind_ptr.assign(ns + 1, 0);
     for (int64_t s = 0; s <= ns; ++s) relind_ptr[s] = riptr[s] - 1;
-    std::vector<int32_t> relind(relind_ptr[ns]);
+    std::vector<int32_t>& relind = h->relind;
+    relind.assign(relind_ptr[ns], 0);
     for (int64_t q = 0; q < relind_ptr[ns]; ++q) relind[q] = (int32_t)(rind[q] - 1);
     // relind is increasing: a child's first cb_split columns land in the parent's panel
     // columns, the rest in its contribution block
