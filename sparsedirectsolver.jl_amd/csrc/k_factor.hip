@@ -604,9 +604,19 @@ __global__ void __launch_bounds__(256) k_big_diag(DevSym S, const Task* __restri
     if (status[s] != 0) return;
     const int nb = min(NB, w - k);
     T* P = factor + S.panel_off[s];
-    for (int q = threadIdx.x; q < NB * NB; q += 256) {
-        const int j = q / NB, ii = q - j * NB;
-        D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? P[(int64_t)(k + j) * f + k + ii] : T(0)) : (ii == j ? T(1) : T(0));
+    // the block's sixteen loads per thread in flight together (clamped into the block), then the
+    // masking
+    T v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u, j = q / NB, ii = q - j * NB;
+        v[u] = P[(int64_t)(k + min(j, nb - 1)) * f + k + min(ii, nb - 1)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u, j = q / NB, ii = q - j * NB;
+        D[j][ii] = (ii < nb && j < nb) ? (ii >= j ? v[u] : T(0)) : (ii == j ? T(1) : T(0));
     }
     block_potrf_inv<T>(D, rd, &fail, nb, eps, X);
     if (fail) {
@@ -840,27 +850,31 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
     const int cg = S.super_ptr[s];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
     const T* Li = DB + (int64_t)(cg + k) * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int kk = e / NB, c = e - kk * NB;
-        Bs[kk][c] = kk < nb && c < nb ? Li[e] : T(0);
-    }
     const int rI = I0 + 16 * wv + l16, rJ = J + 16 * wv + l16;
     // the diagonal workgroup's rows are the rows J: it reads them once
     const bool two = la && I0 != J;
-    T aI[16], aJ[16];
+    // every load of the phase is issued (from clamped, in-bounds addresses) before any of them is
+    // used: a use next to its load makes the compiler wait for each load before the next one.
+    // Linv_b (its first nb columns), the step's columns of the tile's rows (and of the rows J)
+    T lv[16], aI[16], aJ[16];
 #pragma unroll
-    for (int ks = 0; ks < 16; ++ks) {
-        const int col = 4 * ks + l4, cc = k + min(col, nb - 1);
-        const T vi = P[(int64_t)cc * f + min(rI, f - 1)];
-        aI[ks] = rI < f && col < nb ? vi : T(0);
-    }
+    for (int u = 0; u < 16; ++u) lv[u] = Li[min(tid + 256 * u, nb * NB - 1)];
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) aI[ks] = P[(int64_t)(k + min(4 * ks + l4, nb - 1)) * f + min(rI, f - 1)];
     if (two)
 #pragma unroll
-        for (int ks = 0; ks < 16; ++ks) {
-            const int col = 4 * ks + l4, cc = k + min(col, nb - 1);
-            const T vj = P[(int64_t)cc * f + min(rJ, f - 1)];
-            aJ[ks] = rJ < f && col < nb ? vj : T(0);
-        }
+        for (int ks = 0; ks < 16; ++ks) aJ[ks] = P[(int64_t)(k + min(4 * ks + l4, nb - 1)) * f + min(rJ, f - 1)];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * u, kk = e / NB, c = e - kk * NB;
+        Bs[kk][c] = kk < nb && c < nb ? lv[u] : T(0);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) aI[ks] = rI < f && 4 * ks + l4 < nb ? aI[ks] : T(0);
+    if (two)
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) aJ[ks] = rJ < f && 4 * ks + l4 < nb ? aJ[ks] : T(0);
     __syncthreads();
     SDS_STEP_MARK(1);
     // the look-ahead's operands that do not depend on this step, read after the barrier (which
@@ -876,23 +890,20 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int e = tid + 256 * u, kk = e / NB, r = e - kk * NB;
-            const T v = P[(int64_t)(kp + kk) * f + min(J + r, f - 1)];
-            pj[u] = prev && J + r < f ? v : T(0);
+            pj[u] = P[(int64_t)(kp + kk) * f + min(J + r, f - 1)];
         }
     if (two)
 #pragma unroll
-        for (int ks = 0; ks < 16; ++ks) {
-            const T v = P[(int64_t)(kp + 4 * ks + l4) * f + min(rI, f - 1)];
-            ap[ks] = prev && rI < f ? v : T(0);
-        }
+        for (int ks = 0; ks < 16; ++ks) ap[ks] = P[(int64_t)(kp + 4 * ks + l4) * f + min(rI, f - 1)];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = rI, j = J + 16 * nt + Mfma<T>::row(l4, q);
-            const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
-            accC[nt][q] = i < f && j < J + nbn && i >= j ? v : T(0);
+            const int j = J + 16 * nt + Mfma<T>::row(l4, q);
+            accC[nt][q] = P[(int64_t)min(j, w - 1) * f + min(rI, f - 1)];
         }
+    // (masked after the row solve, so that the loads stay in flight under it)
+    __builtin_amdgcn_sched_barrier(0);
     // the row solve as L(I0, b)^T = Linv_b A(I0, b)^T: transposed products, so that each
     // accumulator register holds 16 consecutive rows of one column (coalesced stores); element
     // q of lane l is L(I0 + 16 wv + l % 16, k + 16 nt + row(l / 16, q))
@@ -930,6 +941,23 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
                     P[(int64_t)(k + c) * f + I0 + r] = accI[nt][q];
             }
             if (la) Ln[r][c] = J + r < f ? accJ[nt][q] : T(0);
+        }
+    __builtin_amdgcn_sched_barrier(0);
+    if (two)
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = tid + 256 * u, rr = e - (e / NB) * NB;
+            pj[u] = prev && J + rr < f ? pj[u] : T(0);
+        }
+    if (two)
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) ap[ks] = prev && rI < f ? ap[ks] : T(0);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = J + 16 * nt + Mfma<T>::row(l4, q);
+            accC[nt][q] = rI < f && j < J + nbn && rI >= j ? accC[nt][q] : T(0);
         }
     SDS_STEP_MARK(2);
     if (!la) return;
@@ -981,10 +1009,15 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
+            for (int q = 0; q < 4; ++q)
+                accD[nt][q] = P[(int64_t)min(I + 16 * nt + Mfma<T>::row(l4, q), w - 1) * f + min(rI, f - 1)];
+        __builtin_amdgcn_sched_barrier(0);  // the sixteen loads in flight together
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = rI, j = I + 16 * nt + Mfma<T>::row(l4, q);
-                const T v = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
-                accD[nt][q] = i < f && j < I + nb3 && i >= j ? v : T(0);
+                accD[nt][q] = i < f && j < I + nb3 && i >= j ? accD[nt][q] : T(0);
             }
         __syncthreads();  // Bs and Ln are free: L(I, b) and L(I, b - 1) into them as rows
 #pragma unroll

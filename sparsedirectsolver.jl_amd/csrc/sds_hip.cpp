@@ -770,15 +770,26 @@ int build_schedule(sds_handle* h, int dtype) {
             push_launch(L, kind, t, 0, by);
         }
     };
+    // Cholesky: zeroing a large front's panel and adding A into it reads nothing the
+    // factorization writes, and nothing else writes that panel before the front's own level: the
+    // panels of every level are prepared by one launch at the head of the first level with
+    // large fronts (one launch instead of one per level on the level chain)
+    bool zs_done = false;
     for (auto& L : h->levels) {
         if (L.big.empty()) continue;
         std::vector<Task> t;
         double fl = 0, by = 0;
-        for (int32_t s : L.big) {
-            double f = h->fnrows[s], w = h->fncols[s], m = f - w;
-            const int cols = chol ? h->fncols[s] : h->fnrows[s];
-            for (int c = 0; c < cols; c += 8) t.push_back({s, c, 8, 0});
-            by += dbl_es * (f * w + (chol ? 0.0 : m * (m + 1) / 2)) + 16.0 * (double)h->asm_count[s];
+        for (auto& L2 : h->levels) {
+            if (!(chol && !zs_done) && &L2 != &L) continue;
+            for (int32_t s : L2.big) {
+                double f = h->fnrows[s], w = h->fncols[s], m = f - w;
+                // Cholesky: the panel columns only (a task past w would zero the head of the
+                // front's contribution-block slot, which another front's live block may hold
+                // before this front's level)
+                const int cols = chol ? h->fncols[s] : h->fnrows[s];
+                for (int c = 0; c < cols; c += 8) t.push_back({s, c, std::min(8, cols - c), 0});
+                by += dbl_es * (f * w + (chol ? 0.0 : m * (m + 1) / 2)) + 16.0 * (double)h->asm_count[s];
+            }
         }
         const bool lu = h->method == M_LU;
         if (lu) {
@@ -787,8 +798,9 @@ int build_schedule(sds_handle* h, int dtype) {
             for (int32_t s : L.big)
                 for (int c = 0; c < h->fncols[s]; c += 8) t.push_back({s, c, 8, 0});
             push_launch(L, L_LU_SCATTER, t, 0, 0);
-        } else {
+        } else if (!chol || !zs_done) {
             push_launch(L, L_ZERO_SCATTER, t, 0, by);
+            zs_done = chol;
         }
         push_extend(L, false, lu ? L_LU_EXTEND : L_EXTEND_ADD);
         int wmax = 0;
