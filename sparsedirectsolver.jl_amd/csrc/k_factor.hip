@@ -26,10 +26,6 @@
 #ifndef SDS_POTRF_MARK
 #define SDS_POTRF_MARK(i)
 #endif
-// broadcasts of a panel column read ahead together before a scheduling barrier
-#ifndef SDS_BCAST_GROUP
-#define SDS_BCAST_GROUP 8
-#endif
 
 namespace sds {
 
@@ -471,24 +467,22 @@ __device__ __attribute__((always_inline)) inline void inverse_row(const T (*D)[N
 }
 
 // Cholesky of the 64 x 64 block D in LDS (D[j][i] = L(i, j), lower part; padded with the
-// identity past nb) and its inverse X = L^-1 (X[j][i] = X(i, j)), four waves: per 16-column
-// panel, wave 0 factors the panel's rows in registers (lane i holds row kb + i; the pivot and
-// the column entries move between lanes by v_readlane; 1/sqrt is the hardware estimate with
-// two Newton steps) while waves 1-3 form the previous panel's block row of X; then the four
-// waves apply the panel to the trailing lower 16 x 16 tiles on the matrix cores (K = 16). The
-// last block row of X follows the loop. The upper triangle of D holds scratch values
-// afterwards. rd gets the pivots' reciprocals (1 past nb); a failed pivot (1-based within the
-// block) is left in fail, and X is then incomplete. Ends with a barrier.
+// identity past nb) and its inverse X = L^-1 (X[j][i] = X(i, j); its blocks above the diagonal
+// blocks are left unwritten), four waves: per 16-column panel, wave 0 factors the panel's rows
+// in registers (lane i holds row kb + i; the pivot and the next column's entry move between
+// lanes by v_readlane, each finished column goes to D and the lanes read its further entries
+// back as LDS broadcasts; 1/sqrt is
+// the hardware estimate with two Newton steps) while waves 1-3 form the previous panel's block
+// row of X; then the four waves apply the panel to the trailing lower 16 x 16 tiles on the
+// matrix cores (K = 16). The last block row of X follows the loop. The upper triangle of D
+// holds scratch values afterwards. rd gets the pivots' reciprocals (1 past nb); a failed pivot
+// (1-based within the block) is left in fail, and X is then incomplete. Ends with a barrier.
 template <typename T>
 __device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB + 1], T* rd, int* fail, int nb,
                                                                       double eps, T (*X)[NB + 1]) {
     typedef typename Mfma<T>::acc_t acc_t;
     const int tid = threadIdx.x, wv = tid >> 6;
     if (tid == 0) *fail = 0;
-    for (int e = tid; e < NB * NB; e += blockDim.x) {
-        const int j = e / NB, i = e - j * NB;
-        if (i / 16 < j / 16) X[j][i] = T(0);
-    }
     __syncthreads();
     int p = 0;  // the panel of this round
     for (int kb = 0; kb < nb; kb += 16, ++p) {
@@ -517,18 +511,15 @@ __device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB 
                 p16[c] = (isc ? T(pe) : p16[c]) * r;
                 myr = isc ? r : myr;
                 mypiv = isc ? piv : mypiv;
-                // the broadcasts are read ahead in groups of SDS_BCAST_GROUP, which keeps the
-                // scalar registers they pass through few (the scheduler would otherwise read
-                // them all ahead and spill them) while their latencies overlap
+                // column c is final: it goes to its place in D. The entry L(kb + c + 1, kb + c)
+                // that the next pivot waits for moves by v_readlane; the lanes read the entries
+                // further down, needed only by later columns, back from D as LDS broadcasts (two
+                // per read, their latency under the next pivot's; one wave's LDS accesses
+                // complete in order)
+                if (row < NB) D[kb + c][row] = p16[c];
+                if (c + 1 < 16) p16[c + 1] = fma(-p16[c], lane_bcast(p16[c], c + 1), p16[c + 1]);
 #pragma unroll
-                for (int cc = c + 1; cc < 16; ++cc) {
-                    p16[cc] = fma(-p16[c], lane_bcast(p16[c], cc), p16[cc]);
-                    if ((cc - c) % SDS_BCAST_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-            if (row < NB) {
-#pragma unroll
-                for (int c = 0; c < 16; ++c) D[kb + c][row] = p16[c];
+                for (int cc = c + 2; cc < 16; ++cc) p16[cc] = fma(-p16[c], D[kb + c][kb + cc], p16[cc]);
             }
             if (ii < 16) rd[kb + ii] = myr;
             // the first pivot of the panel that fails the acceptance test (1-based), if any
@@ -586,7 +577,8 @@ __device__ __attribute__((always_inline)) inline void block_store(const T (*D)[N
     for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
         const int j = q / NB, i = q - j * NB;
         if (j < nb && i >= j && i < nb) P[(int64_t)(k + j) * f + k + i] = D[j][i];
-        if (j < nb) out[q] = i < nb ? X[j][i] : T(0);
+        // X is lower block triangular: its blocks above the diagonal blocks are never formed
+        if (j < nb) out[q] = i < nb && i / 16 >= j / 16 ? X[j][i] : T(0);
     }
 }
 
