@@ -13,12 +13,14 @@
 // [c0, c1) (one step unless the front is wider). A narrow step (at most NB columns) uses the
 // inverse Linv of its single diagonal block; a wide step uses X, the inverse of its whole
 // diagonal block, formed after the factorization by k_step_inverse. Per tree level and step:
-//   forward : pull the children's update vectors and form y_step = inverse * t_step:
+//   forward : pull the children's update vectors (at a front's first step; each row gathers its
+//             entries, children in child_list order, from the host's pull map) and form
+//             y_step = inverse * t_step:
 //             k_fwd_diag (narrow, one workgroup per step; a narrow front with at most FUSE_ROWS
 //             rows below its columns forms its update vector there too) or k_fwd_inv (wide,
 //             16 rows per workgroup, into Z, then k_copy_rows); then k_fwd_rows adds
 //             L(rows, step) y_step to the rows below the step, 64 per workgroup (the update
-//             vector rows are zeroed and get the children's entries of their rows first).
+//             vector rows start from the children's entries of their rows).
 //   backward: k_bwd_l21t, partial sums of L(rows below the step, step)^T x per chunk of
 //             L21T_CHUNK rows (one wave per 8 columns, lanes along the rows, a reduce-scatter
 //             across the lanes at the end); then x_step = inverse^T (y_step - partials):
@@ -94,18 +96,6 @@ __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __re
 
 extern __shared__ __align__(16) unsigned char sweep_lds[];
 
-// first index in the ascending ri[0, n) with ri[idx] >= v
-__device__ inline int lower_bound_i32(const int32_t* __restrict__ ri, int n, int v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ri[mid] < v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 // one recursive-halving exchange at lane distance D: the lanes with bit D set keep the upper
 // halves a[D, 2D), the others the lower, each adding its partner's copy
@@ -158,29 +148,30 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
     const T* P = factor + S.panel_off[s];
     const bool fused = tk.a != 0;  // only a whole narrow front (c0 = 0, nw = w) is fused
     T* us = fused && m > 0 && S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
-    if (us)
-        for (int q = tid; q < m * nrhs; q += nt) us[q] = T(0);
-    __syncthreads();
-    // the children's update vectors, in child_list order (only the rows < w unless fused), at the
-    // front's first step; the last narrow step of a wide front (c0 > 0) got its rows' entries
-    // from k_fwd_rows at the first step
-    for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1] && c0 == 0; ++kc) {
-        const int c = S.child_list[kc];
-        if (S.uv_off[c] < 0) continue;
-        const int mc = S.fnrows[c] - S.fncols[c];
-        const T* uc = U + S.uv_off[c] * nrhs;
-        const int32_t* ri = S.relind + S.relind_ptr[c];
-        const int hi = us ? mc : lower_bound_i32(ri, mc, w);
-        for (int q = tid; q < hi * nrhs; q += nt) {
-            const int ii = q / nrhs, r = q - ii * nrhs;
-            const int pos = ri[ii];
-            if (pos < w)
-                Y[(int64_t)(cg + pos) * nrhs + r] -= uc[q];
-            else
-                us[(pos - w) * nrhs + r] += uc[q];
+    // the children's update vectors at the front's first step, row by row from the pull map
+    // (child_list order; only the rows < w unless fused: the update vector rows start from zero).
+    // The last narrow step of a wide front (c0 > 0) got its rows' entries from k_fwd_rows at the
+    // first step
+    if (c0 == 0) {
+        const int32_t* pp = S.pm_ptr + S.pm_row[s];
+        const int rows = us ? f : w;
+        for (int q = tid; q < rows * nrhs; q += nt) {
+            const int i = q / nrhs, r = q - i * nrhs;
+            const int e0 = pp[i], e1 = pp[i + 1];
+            if (i < w) {
+                if (e0 == e1) continue;
+                T* yp = Y + (int64_t)(cg + i) * nrhs + r;
+                T v = *yp;
+                for (int e = e0; e < e1; ++e) v -= U[(int64_t)S.pm_src[e] * nrhs + r];
+                *yp = v;
+            } else {
+                T v = T(0);
+                for (int e = e0; e < e1; ++e) v += U[(int64_t)S.pm_src[e] * nrhs + r];
+                us[(i - w) * nrhs + r] = v;
+            }
         }
-        __syncthreads();
     }
+    __syncthreads();
     T* y = Y + (int64_t)(cg + c0) * nrhs;
     const T* Li = DB + (int64_t)(cg + c0) * NB;
     for (int rb = 0; rb < nrhs; rb += 8) {
@@ -239,26 +230,24 @@ __global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __r
     const bool below = r0 >= w;
     T* us = below ? U + S.uv_off[s] * nrhs : nullptr;
     if (c0 == 0) {
-        if (below)
-            for (int q = tid; q < (r1 - r0) * nrhs; q += nt) us[(int64_t)(r0 - w) * nrhs + q] = T(0);
-        __syncthreads();
-        for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-            const int c = S.child_list[kc];
-            if (S.uv_off[c] < 0) continue;
-            const int mc = S.fnrows[c] - S.fncols[c];
-            const T* uc = U + S.uv_off[c] * nrhs;
-            const int32_t* ri = S.relind + S.relind_ptr[c];
-            const int lo = lower_bound_i32(ri, mc, r0), hi = lower_bound_i32(ri, mc, r1);
-            for (int q = tid; q < (hi - lo) * nrhs; q += nt) {
-                const int ii = lo + q / nrhs, r = q % nrhs;
-                const T v = uc[(int64_t)ii * nrhs + r];
-                if (below)
-                    us[(int64_t)(ri[ii] - w) * nrhs + r] += v;
-                else
-                    Y[(int64_t)(cg + ri[ii]) * nrhs + r] -= v;
+        // the rows' entries of the children's update vectors, row by row from the pull map
+        // (child_list order; update vector rows start from zero)
+        const int32_t* pp = S.pm_ptr + S.pm_row[s];
+        for (int q = tid; q < (r1 - r0) * nrhs; q += nt) {
+            const int di = q / nrhs, i = r0 + di, r = q - di * nrhs;
+            const int e0 = pp[i], e1 = pp[i + 1];
+            if (below) {
+                T v = T(0);
+                for (int e = e0; e < e1; ++e) v += U[(int64_t)S.pm_src[e] * nrhs + r];
+                us[(int64_t)(i - w) * nrhs + r] = v;
+            } else if (e0 < e1) {
+                T* yp = Y + (int64_t)(cg + i) * nrhs + r;
+                T v = *yp;
+                for (int e = e0; e < e1; ++e) v -= U[(int64_t)S.pm_src[e] * nrhs + r];
+                *yp = v;
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
     const int nw = c1 - c0;
     const int i = min(r0 + lane, r1 - 1);
@@ -536,26 +525,22 @@ __global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __re
     const int r = ri0 + min(rl, nbi - 1);
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
+        // t = the step's rows of y, less (at the front's first step) the children's update entries
+        // of those rows, row by row from the pull map in child_list order
+        const int32_t* pp = S.pm_ptr + S.pm_row[s];
         for (int q = tid; q < ncols * 8; q += 256) {
             const int i = q >> 3, rr = q & 7;
-            t[q] = rr < nr ? Y[(int64_t)(cg + c0 + i) * nrhs + rb + rr] : T(0);
+            T v = T(0);
+            if (rr < nr) {
+                v = Y[(int64_t)(cg + c0 + i) * nrhs + rb + rr];
+                if (c0 == 0) {
+                    const int e1 = pp[i + 1];
+                    for (int e = pp[i]; e < e1; ++e) v -= U[(int64_t)S.pm_src[e] * nrhs + rb + rr];
+                }
+            }
+            t[q] = v;
         }
         __syncthreads();
-        if (c0 == 0) {
-            for (int kc = S.child_ptr[s]; kc < S.child_ptr[s + 1]; ++kc) {
-                const int c = S.child_list[kc];
-                if (S.uv_off[c] < 0) continue;
-                const int mc = S.fnrows[c] - S.fncols[c];
-                const T* uc = U + S.uv_off[c] * nrhs;
-                const int32_t* ri = S.relind + S.relind_ptr[c];
-                const int hi = lower_bound_i32(ri, mc, ncols);
-                for (int q = tid; q < hi * 8; q += 256) {
-                    const int ii = q >> 3, rr = q & 7;
-                    if (rr < nr) t[ri[ii] * 8 + rr] -= uc[(int64_t)ii * nrhs + rb + rr];
-                }
-                __syncthreads();
-            }
-        }
         T acc[8];
 #pragma unroll
         for (int rr = 0; rr < 8; ++rr) acc[rr] = T(0);

@@ -27,6 +27,11 @@ struct DevSym {
     const int64_t* acol_ptr;    // n + 1: the entries of global factor column j start at acol_ptr[j]
     const int64_t* cb_off;      // ns: contribution block (m x m, ld m) in the CB pool, -1 = none
     const int64_t* uv_off;      // ns: solve update vector (m entries per right-hand side), -1 = none
+    // Cholesky sweeps, the forward pull map: row i of front s has the update-vector rows
+    // pm_src[pm_ptr[pm_row[s] + i] .. pm_ptr[pm_row[s] + i + 1]) of its children (child_list order)
+    const int64_t* pm_row;      // ns: first map row of the front
+    const int32_t* pm_ptr;      // sum of f + 1
+    const int32_t* pm_src;      // uv_off[c] + ii for the child rows ii that land there
     int32_t unsym;              // structure "G": full fronts, L and U^T panels
 };
 

@@ -207,6 +207,9 @@ struct sds_handle {
     int64_t cb_len = 0, uv_len = 0;
     int64_t* d_cb_off = nullptr;
     int64_t* d_uv_off = nullptr;
+    int64_t* d_pm_row = nullptr;  // the forward pull map of the Cholesky sweeps (DevSym)
+    int32_t* d_pm_ptr = nullptr;
+    int32_t* d_pm_src = nullptr;
     bool placed = false;                      // cb_place / uv_place hold this analysis' placements
     int place_key = 0;                        // ... for this element type and subtree budget
     // regime A (Cholesky): subtree budget (KiB of LDS, 0 = off), subtrees (node ranges of
@@ -1280,6 +1283,37 @@ int build_schedule(sds_handle* h, int dtype) {
     if ((rc = upload(h, h->numbufs, uv_off, &h->d_uv_off))) return rc;
     h->dsym.cb_off = h->d_cb_off;
     h->dsym.uv_off = h->d_uv_off;
+    if (h->method == M_CHOLESKY) {
+        // the forward pull map: per front row, its children's update-vector rows in child_list
+        // order (one gather per row in the forward sweep, no search and no barrier per child)
+        std::vector<int64_t> pm_row(ns + 1, 0);
+        for (int64_t s = 0; s < ns; ++s) pm_row[s + 1] = pm_row[s] + h->fnrows[s];
+        std::vector<int64_t> cnt(pm_row[ns] + 1, 0);
+        for (int64_t s = 0; s < ns; ++s)
+            for (int kc = h->child_ptr[s]; kc < h->child_ptr[s + 1]; ++kc) {
+                const int32_t c = h->child_list[kc];
+                if (uv_off[c] < 0) continue;
+                for (int64_t q = h->relind_ptr[c]; q < h->relind_ptr[c + 1]; ++q) cnt[pm_row[s] + h->relind[q] + 1] += 1;
+            }
+        for (int64_t r = 0; r < pm_row[ns]; ++r) cnt[r + 1] += cnt[r];
+        if (cnt[pm_row[ns]] >= ((int64_t)1 << 31) || h->uv_len >= ((int64_t)1 << 31))
+            return fail(h, SDS_NOT_SUPPORTED, "solve pull map beyond 2^31 entries");
+        std::vector<int32_t> pm_ptr(cnt.begin(), cnt.end()), pm_src(std::max<int64_t>(cnt[pm_row[ns]], 1));
+        for (int64_t s = 0; s < ns; ++s)
+            for (int kc = h->child_ptr[s]; kc < h->child_ptr[s + 1]; ++kc) {
+                const int32_t c = h->child_list[kc];
+                if (uv_off[c] < 0) continue;
+                for (int64_t q = h->relind_ptr[c]; q < h->relind_ptr[c + 1]; ++q)
+                    pm_src[cnt[pm_row[s] + h->relind[q]]++] = (int32_t)(uv_off[c] + (q - h->relind_ptr[c]));
+            }
+        pm_row.resize(ns);
+        if ((rc = upload(h, h->numbufs, pm_row, &h->d_pm_row))) return rc;
+        if ((rc = upload(h, h->numbufs, pm_ptr, &h->d_pm_ptr))) return rc;
+        if ((rc = upload(h, h->numbufs, pm_src, &h->d_pm_src))) return rc;
+        h->dsym.pm_row = h->d_pm_row;
+        h->dsym.pm_ptr = h->d_pm_ptr;
+        h->dsym.pm_src = h->d_pm_src;
+    }
     (void)p;
     return SDS_SUCCESS;
 }
