@@ -477,9 +477,17 @@ __device__ __attribute__((always_inline)) inline void inverse_row(const T (*D)[N
 // matrix cores (K = 16). The last block row of X follows the loop. The upper triangle of D
 // holds scratch values afterwards. rd gets the pivots' reciprocals (1 past nb); a failed pivot
 // (1-based within the block) is left in fail, and X is then incomplete. Ends with a barrier.
-template <typename T>
+struct NoRound0 {
+    __device__ void operator()(int) const {}
+};
+
+// (pre: work for waves 1-3 during the first panel, which only reads and writes D's first 16
+// columns; it may write D's other columns, and read X, which is not written before the second
+// panel)
+template <typename T, typename Pre = NoRound0>
 __device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB + 1], T* rd, int* fail, int nb,
-                                                                      double eps, T (*X)[NB + 1]) {
+                                                                      double eps, T (*X)[NB + 1],
+                                                                      Pre pre = Pre()) {
     typedef typename Mfma<T>::acc_t acc_t;
     const int tid = threadIdx.x, wv = tid >> 6;
     if (tid == 0) *fail = 0;
@@ -529,6 +537,8 @@ __device__ __attribute__((always_inline)) inline void block_potrf_inv(T (*D)[NB 
             // block row p - 1 of X: wave w forms piece j = w - 1 (wave 1 writes the diagonal block)
             const int pr = p - 1, j = wv - 1;
             if (j < pr || wv == 1) inverse_row<T>(D, rd, X, pr, min(j, pr), wv == 1);
+        } else {
+            pre(wv);
         }
         SDS_POTRF_MARK(0);
         __syncthreads();
@@ -966,6 +976,63 @@ __global__ void __launch_bounds__(256, OCC) k_big_step(DevSym S, const Task* __r
         __syncthreads();
     }
     SDS_STEP_MARK(3);
+    if (I0 == J && nbn == NB) {
+        // the next diagonal block, a full one: its old values (lower part) go to D, and its
+        // update by step b, D -= L(J, b) L(J, b)^T, is made per 16-column block from Ln (rows J
+        // of step b, all in LDS): the first column block by the four waves before the
+        // factorization, the others by waves 1-3 while wave 0 factors the first 16 columns. Each
+        // block's products run in the order of the whole tile's, so the values are the same.
+        T(*D)[NB + 1] = Bs;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = 16 * nt + Mfma<T>::row(l4, q);
+                D[j][r] = r >= j ? accC[nt][q] : T(0);
+            }
+        __syncthreads();
+        // block (rows 16 rb.., columns 16 cb..) of the tile, rb >= cb
+        auto upd = [&](int rb, int cb) {
+            acc_t acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = D[16 * cb + Mfma<T>::row(l4, q)][16 * rb + l16];
+#pragma unroll
+            for (int ntp = 0; ntp < 4; ++ntp)
+#pragma unroll
+                for (int qp = 0; qp < 4; ++qp) {
+                    const int kx = 16 * ntp + Mfma<T>::row(l4, qp);
+                    acc = Mfma<T>::mac(-Ln[16 * cb + l16][kx], Ln[16 * rb + l16][kx], acc);
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = 16 * rb + l16, j = 16 * cb + Mfma<T>::row(l4, q);
+                D[j][i] = i >= j ? acc[q] : T(0);
+            }
+        };
+        upd(wv, 0);
+        __syncthreads();
+        SDS_STEP_MARK(5);
+        // waves 1-3: the six blocks of column blocks 1-3 on or below the diagonal, two each
+        // (wave 1: (1, 1) and (3, 3); wave 2: (2, 1) and (2, 2); wave 3: (3, 1) and (3, 2))
+        block_potrf_inv<T>(D, rd, &fail, NB, eps, Ln, [&](int w) {
+            upd(w, 1);
+            if (w == 1)
+                upd(3, 3);
+            else if (w == 2)
+                upd(2, 2);
+            else
+                upd(3, 2);
+        });
+        SDS_STEP_MARK(6);
+        if (fail) {
+            if (tid == 0) status[s] = J + fail;
+            return;
+        }
+        SDS_STEP_MARK(7);
+        block_store<T>(D, Ln, P, f, J, NB, DB + (int64_t)(cg + J) * NB);
+        SDS_STEP_MARK(8);
+        return;
+    }
     // C(I0.., J..)^T -= L(J, b) L(I0, b)^T (+ L(J, b - 1) L(I0, b - 1)^T off the diagonal tile);
     // the K index of the first product runs over accI's registers (element q of accI[nt'] is
     // L(I0 + r, k + 16 nt' + row(l / 16, q)), the B operand of this wave's rows as it stands)
