@@ -96,6 +96,25 @@ __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __re
 
 extern __shared__ __align__(16) unsigned char sweep_lds[];
 
+// row[r] = row[r] + a[r] (or - a[r] when sub) for r < nr (at most 8): the loads issued together
+// (clamped into the row), the results held in registers by empty asm statements, then the
+// stores. Written as one read-modify-write per r, the compiler waited for every memory
+// operation before each store.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void rmw8(T* row, const T (&a)[8], int nr, bool sub) {
+    T o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) o[r] = row[min(r, nr - 1)];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        o[r] = sub ? o[r] - a[r] : o[r] + a[r];
+        asm volatile("" : "+v"(o[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+        if (r < nr) row[r] = o[r];
+}
+
 
 // one recursive-halving exchange at lane distance D: the lanes with bit D set keep the upper
 // halves a[D, 2D), the others the lower, each adding its partner's copy
@@ -198,9 +217,7 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
 #pragma unroll
                     for (int r = 0; r < 8; ++r) a[r] += lv * yx[j * 8 + r];
                 }
-#pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if (r < nr) us[(i - w) * nrhs + rb + r] += a[r];
+                rmw8<T>(us + (i - w) * nrhs + rb, a, nr, false);
             }
         __syncthreads();
     }
@@ -279,17 +296,14 @@ __global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __r
         for (int r = 0; r < 8; ++r) red[(wv * 64 + lane) * 8 + r] = acc[r];
         __syncthreads();
         if (wv == 0 && r0 + lane < r1) {
+            T v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                if (r >= nr) break;
-                T v = red[lane * 8 + r];
+                v[r] = red[lane * 8 + r];
 #pragma unroll
-                for (int u = 1; u < NWV; ++u) v += red[(u * 64 + lane) * 8 + r];
-                if (below)
-                    us[(int64_t)(i - w) * nrhs + rb + r] += v;
-                else
-                    Y[(int64_t)(cg + i) * nrhs + rb + r] -= v;
+                for (int u = 1; u < NWV; ++u) v[r] += red[(u * 64 + lane) * 8 + r];
             }
+            rmw8<T>(below ? us + (int64_t)(i - w) * nrhs + rb : Y + (int64_t)(cg + i) * nrhs + rb, v, nr, !below);
         }
         __syncthreads();
     }
