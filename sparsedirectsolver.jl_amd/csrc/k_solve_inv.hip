@@ -59,12 +59,19 @@ __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __re
     const int nb = min(BS, w - k);
     T* P = factor + S.panel_off[s];
     T* blk = DB + (int64_t)(S.super_ptr[s] + k) * NB;
+    {
+        // the column's BS loads issued together (clamped into the block), then the masking
+        const T* src = t.c ? blk + min(c, nb - 1) : P + (int64_t)k * f + k + min(c, nb - 1);
+        const int64_t ld = t.c ? NB : f;
+        T lv[BS];
 #pragma unroll
-    for (int m = 0; m < BS; ++m) {
-        const int r = c;
-        T v = r == m ? T(1) : T(0);
-        if (m < nb && r < nb) v = r >= m ? (t.c ? blk[(int64_t)m * NB + r] : P[(int64_t)(k + m) * f + k + r]) : T(0);
-        Lb[g][m][r] = v;
+        for (int m = 0; m < BS; ++m) lv[m] = src[(int64_t)min(m, nb - 1) * ld];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = 0; m < BS; ++m) {
+            const int r = c;
+            Lb[g][m][r] = m < nb && r < nb ? (r >= m ? lv[m] : T(0)) : (r == m ? T(1) : T(0));
+        }
     }
     __syncthreads();
     if (live && t.c && c < nb)
@@ -341,13 +348,17 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
 #pragma unroll
             for (int r = 0; r < 8; ++r) x[r] = yr[min(rb + r, nrhs - 1)];
             const T* pc = P + (int64_t)col0 * f + i;  // column col0 + c; columns past c1 read c1 - 1
+            T l[8];  // the row's eight loads issued together (the FP32 build otherwise waits per load)
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const T l = __builtin_nontemporal_load(pc);  // the factor streams through once per sweep
+                l[c] = __builtin_nontemporal_load(pc);  // the factor streams through once per sweep
                 pc += col0 + c + 1 < c1 ? f : 0;
-#pragma unroll
-                for (int r = 0; r < 8; ++r) a[c * 8 + r] += l * x[r];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) a[c * 8 + r] += l[c] * x[r];
         }
         const T v = wave_reduce_scatter64<T>(a);
         const int c = lane >> 3, r = lane & 7;
@@ -442,20 +453,32 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
     const int row0 = 32 * (wv >> 1), col0 = 32 * (wv & 1);
     const int lr = tid % NB, lc = tid / NB;
+    {
+        // X_kk = Linv_kk: the loads issued together (clamped into the block), then the stores
+        T xv[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int c = lc + 4 * i;
-        if (c < nbk && lr < nbk) X[(int64_t)(kc + c) * nw + kc + lr] = DB[(int64_t)(cg + c0 + kc + c) * NB + lr];
-    }
-    // the operands of product (ib, jb): A = L(ic + r, jc + c), B = X(jc + r, kc + c)
-    T ra[PER], rb[PER];
-    auto load = [&](int ib, int jb) {
-        const int ic = ib * NB, jc = jb * NB, nbi = min(NB, nw - ic);
+        for (int i = 0; i < PER; ++i)
+            xv[i] = DB[(int64_t)(cg + c0 + kc + min(lc + 4 * i, nbk - 1)) * NB + min(lr, nbk - 1)];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) asm volatile("" : "+v"(xv[i]));
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int c = lc + 4 * i;
-            ra[i] = lr < nbi ? P[(int64_t)(c0 + jc + c) * f + c0 + ic + lr] : T(0);
-            rb[i] = c < nbk ? X[(int64_t)(kc + c) * nw + jc + lr] : T(0);
+            if (c < nbk && lr < nbk) X[(int64_t)(kc + c) * nw + kc + lr] = xv[i];
+        }
+    }
+    // the operands of product (ib, jb): A = L(ic + r, jc + c), B = X(jc + r, kc + c); loaded from
+    // clamped in-bounds addresses (jb < ib < nblk: block jb is full), masked at the use
+    T ra[PER], rb[PER];
+    int nbl = NB;  // rows of block ib of the loaded operands
+    auto load = [&](int ib, int jb) {
+        const int ic = ib * NB, jc = jb * NB;
+        nbl = min(NB, nw - ic);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int c = lc + 4 * i;
+            ra[i] = P[(int64_t)(c0 + jc + c) * f + c0 + ic + min(lr, nbl - 1)];
+            rb[i] = X[(int64_t)(kc + min(c, nbk - 1)) * nw + jc + lr];
         }
     };
     __syncthreads();  // X_kk is read back as an operand below
@@ -470,8 +493,8 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
         for (int jb = tk.a; jb < ib; ++jb) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
-                A[lr][lc + 4 * i] = ra[i];
-                B[lr][lc + 4 * i] = rb[i];
+                A[lr][lc + 4 * i] = lr < nbl ? ra[i] : T(0);
+                B[lr][lc + 4 * i] = lc + 4 * i < nbk ? rb[i] : T(0);
             }
             __syncthreads();
             // the next product of this block row; X(jb + 1, kk) was written before this row
@@ -481,12 +504,10 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
         }
         // X_ik = -Linv_ii S: Linv_ii into A, the sum S into B
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int c = lc + 4 * i;
-            ra[i] = lr < nbi && c < nbi ? DB[(int64_t)(cg + c0 + ic + c) * NB + lr] : T(0);
-        }
+        for (int i = 0; i < PER; ++i)
+            ra[i] = DB[(int64_t)(cg + c0 + ic + min(lc + 4 * i, nbi - 1)) * NB + min(lr, nbi - 1)];
 #pragma unroll
-        for (int i = 0; i < PER; ++i) A[lr][lc + 4 * i] = ra[i];
+        for (int i = 0; i < PER; ++i) A[lr][lc + 4 * i] = lr < nbi && lc + 4 * i < nbi ? ra[i] : T(0);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -621,13 +642,17 @@ __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __res
 #pragma unroll
             for (int rr = 0; rr < 8; ++rr) x[rr] = ti[rr];
             const T* pc = X + (int64_t)col0 * nw + i;  // column col0 + c; columns past cend read cend - 1
+            T l[8];  // the row's eight loads issued together (the FP32 build otherwise waits per load)
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const T l = *pc;
+                l[c] = *pc;
                 pc += col0 + c + 1 < cend ? nw : 0;
-#pragma unroll
-                for (int rr = 0; rr < 8; ++rr) a[c * 8 + rr] += l * x[rr];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int rr = 0; rr < 8; ++rr) a[c * 8 + rr] += l[c] * x[rr];
         }
         const T v = wave_reduce_scatter64<T>(a);
         const int col = col0 + (lane >> 3), rr = lane & 7;
