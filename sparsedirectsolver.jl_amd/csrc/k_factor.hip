@@ -736,6 +736,27 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
         }
         __syncthreads();
     }
+    if (!EARLY && mode == 0) {
+        // the tile's old values (panel columns only in mode 0), read in one batch from clamped
+        // addresses after the products: read one at a time next to their stores, each read
+        // waited for the stores before it
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = I0 + wm * 32 + a * 16 + l16;
+                    const int j = J0 + wn * 32 + b * 16 + Mfma<T>::row(l4, q);
+                    old[a][b][q] = P[(int64_t)min(j, w - 1) * f + min(i, f - 1)];
+                }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) asm volatile("" : "+v"(old[a][b][q]));
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -750,7 +771,7 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
                     // (upper part zero, identity past nbd), rows below the block to the panel
                     const int ii = i - I0, jj = j - J0;
                     if (ii < nbd && jj < nbd) {
-                        Dout[jj][ii] = in ? (EARLY ? old[a][b][q] : *front_at(P, C, f, w, i, j)) - acc[a][b][q] : T(0);
+                        Dout[jj][ii] = in ? (EARLY || mode == 0 ? old[a][b][q] : *front_at(P, C, f, w, i, j)) - acc[a][b][q] : T(0);
                         continue;
                     }
                     if (jj >= nbd) Dout[jj][ii] = ii == jj ? T(1) : T(0);
@@ -760,18 +781,20 @@ __device__ __attribute__((always_inline)) inline void update_tile(const DevSym& 
                     // mode 1 writes the contribution block whole (the children's entries are
                     // added to it afterwards)
                     T* d = front_at(P, C, f, w, i, j);
-                    *d = (EARLY ? old[a][b][q] : mode == 1 ? T(0) : *d) - acc[a][b][q];
+                    *d = (EARLY || mode == 0 ? old[a][b][q] : T(0)) - acc[a][b][q];
                 }
             }
 }
 
-template <typename T, int KC>
+// MODE a template parameter: each mode gets its own register allocation (mode 0 holds the
+// tile's old values; the contribution-block update, mode 1, keeps four workgroups per CU)
+template <typename T, int KC, int MODE>
 __global__ void __launch_bounds__(256) k_big_update_mfma(DevSym S, const Task* __restrict__ tasks,
                                                         T* __restrict__ factor, T* __restrict__ cbpool,
-                                                        const int32_t* __restrict__ status, int mode, int pb) {
+                                                        const int32_t* __restrict__ status, int pb) {
     __shared__ T As[KC][TILE];
     __shared__ T Bs[KC][TILE];
-    update_tile<T, KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, mode, As, Bs, pb);
+    update_tile<T, KC, false>(S, tasks[blockIdx.x], factor, cbpool, status, MODE, As, Bs, pb);
 }
 
 // the rows [k + 64, k + 128) of a step's columns, kept in LN by k_big_step, into the panel.
@@ -1328,15 +1351,21 @@ void launch_big_block(hipStream_t st, const DevSym& S, const Task* tasks, int n,
 template <typename T>
 void launch_big_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, T* cbpool,
                        const int32_t* status, int mode, int kc, int pb) {
-    if (kc == 64)
-        hipLaunchKernelGGL((k_big_update_mfma<T, 64>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
-                           pb);
-    else if (kc == 16)
-        hipLaunchKernelGGL((k_big_update_mfma<T, 16>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
-                           pb);
-    else
-        hipLaunchKernelGGL((k_big_update_mfma<T, 32>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, mode,
-                           pb);
+    if (mode == 0) {
+        if (kc == 64)
+            hipLaunchKernelGGL((k_big_update_mfma<T, 64, 0>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+        else if (kc == 16)
+            hipLaunchKernelGGL((k_big_update_mfma<T, 16, 0>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+        else
+            hipLaunchKernelGGL((k_big_update_mfma<T, 32, 0>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+    } else {
+        if (kc == 64)
+            hipLaunchKernelGGL((k_big_update_mfma<T, 64, 1>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+        else if (kc == 16)
+            hipLaunchKernelGGL((k_big_update_mfma<T, 16, 1>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+        else
+            hipLaunchKernelGGL((k_big_update_mfma<T, 32, 1>), dim3(n), dim3(256), 0, st, S, tasks, factor, cbpool, status, pb);
+    }
 }
 
 template <typename T>
