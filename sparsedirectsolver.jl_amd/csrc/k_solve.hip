@@ -84,10 +84,19 @@ __global__ void k_permute_out(int n, int nrhs, const int32_t* __restrict__ perm,
 // has all its loads in flight at once (blockDim.x = 256)
 template <typename T>
 __device__ inline void load_block(T (*B)[NB + 1], const T* __restrict__ P, int f, int i0, int j0, int ni, int nj) {
+    // unconditional loads from clamped addresses, then the masked LDS stores: a guarded load
+    // made the compiler wait for each load before issuing the next
+    T v[NB * NB / 256];
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
         const int q = threadIdx.x + 256 * u, jj = q / NB, ii = q - jj * NB;
-        if (ii < ni && jj < nj) B[jj][ii] = P[(int64_t)(j0 + jj) * f + i0 + ii];
+        v[u] = P[(int64_t)(j0 + min(jj, nj - 1)) * f + i0 + min(ii, ni - 1)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+        const int q = threadIdx.x + 256 * u, jj = q / NB, ii = q - jj * NB;
+        if (ii < ni && jj < nj) B[jj][ii] = v[u];
     }
 }
 
@@ -319,27 +328,28 @@ __global__ void __launch_bounds__(256) k_fwd_step(DevSym S, const Task* __restri
     if (t.c)
         for (int q = tid; q < nb * nrhs; q += 256) Z[(int64_t)(c0 + k) * nrhs + q] = xs[q];
     const int i = row0 + tid;
-    if (i < min(row0 + 256, f)) {
-        T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    T* us = S.uv_off[s] >= 0 ? U + S.uv_off[s] * nrhs : nullptr;
+    if (i < min(row0 + 256, f) && (i < w || us)) {
         for (int rb = 0; rb < nrhs; rb += 8) {
+            const int nr = min(8, nrhs - rb);
             T acc[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) acc[r] = T(0);
-#pragma unroll 16
-            for (int j = 0; j < nb; ++j) {
-                const T l = P[(int64_t)(k + j) * f + i];
+            for (int j0 = 0; j0 < nb; j0 += 16) {
+                // sixteen panel entries of the row in flight together (clamped to the block)
+                T l[16];
 #pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if (rb + r < nrhs) acc[r] += l * xs[j * nrhs + rb + r];
-            }
+                for (int jj = 0; jj < 16; ++jj) l[jj] = P[(int64_t)(k + min(j0 + jj, nb - 1)) * f + i];
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                if (rb + r >= nrhs) continue;
-                if (i < w)
-                    Y[(int64_t)(c0 + i) * nrhs + rb + r] -= acc[r];
-                else if (us)
-                    us[(int64_t)(i - w) * nrhs + rb + r] += acc[r];
+                for (int jj = 0; jj < 16; ++jj) {
+                    if (j0 + jj >= nb) break;
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        if (r < nr) acc[r] += l[jj] * xs[(j0 + jj) * nrhs + rb + r];
+                }
             }
+            rmw8<T>(i < w ? Y + (int64_t)(c0 + i) * nrhs + rb : us + (int64_t)(i - w) * nrhs + rb, acc, nr, i < w);
         }
     }
 }
@@ -379,10 +389,21 @@ __global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restric
         T acc0 = T(0), acc1 = T(0);
         for (int i0 = w; i0 < f; i0 += 64) {
             const int ni = min(64, f - i0);
+            {
+                // the tile's sixteen loads per thread in flight together (clamped), then the
+                // masked LDS stores (zeros outside the tile)
+                T v[16];
 #pragma unroll
-            for (int q = tid; q < 64 * 64; q += 256) {
-                const int c = q >> 6, ii = q & 63;
-                Lt[c][ii] = (c < ncol && ii < ni) ? P[(int64_t)(col0 + c) * f + i0 + ii] : T(0);
+                for (int u = 0; u < 16; ++u) {
+                    const int q = tid + 256 * u, c = q >> 6, ii = q & 63;
+                    v[u] = P[(int64_t)(col0 + min(c, ncol - 1)) * f + i0 + min(ii, ni - 1)];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int q = tid + 256 * u, c = q >> 6, ii = q & 63;
+                    Lt[c][ii] = (c < ncol && ii < ni) ? v[u] : T(0);
+                }
             }
             for (int q = tid; q < 64 * 8; q += 256) {
                 const int ii = q >> 3, r = q & 7;

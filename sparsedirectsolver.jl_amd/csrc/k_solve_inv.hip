@@ -103,24 +103,6 @@ __global__ void __launch_bounds__(64) k_block_inverse(DevSym S, const Task* __re
 
 extern __shared__ __align__(16) unsigned char sweep_lds[];
 
-// row[r] = row[r] + a[r] (or - a[r] when sub) for r < nr (at most 8): the loads issued together
-// (clamped into the row), the results held in registers by empty asm statements, then the
-// stores. Written as one read-modify-write per r, the compiler waited for every memory
-// operation before each store.
-template <typename T>
-__device__ __attribute__((always_inline)) inline void rmw8(T* row, const T (&a)[8], int nr, bool sub) {
-    T o[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) o[r] = row[min(r, nr - 1)];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        o[r] = sub ? o[r] - a[r] : o[r] + a[r];
-        asm volatile("" : "+v"(o[r]));
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-        if (r < nr) row[r] = o[r];
-}
 
 
 // one recursive-halving exchange at lane distance D: the lanes with bit D set keep the upper

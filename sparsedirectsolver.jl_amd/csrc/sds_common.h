@@ -86,6 +86,25 @@ __device__ inline float lane_bcast(float v, int lane) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
+// row[r] = row[r] + a[r] (or - a[r] when sub) for r < nr (at most 8): the loads issued together
+// (clamped into the row), the results held in registers by empty asm statements, then the
+// stores. Written as one read-modify-write per r, the compiler waited for every memory
+// operation before each store.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void rmw8(T* row, const T (&a)[8], int nr, bool sub) {
+    T o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) o[r] = row[min(r, nr - 1)];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        o[r] = sub ? o[r] - a[r] : o[r] + a[r];
+        asm volatile("" : "+v"(o[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+        if (r < nr) row[r] = o[r];
+}
+
 #define SDS_HIP_CHECK(x)                                      \
     do {                                                      \
         hipError_t e_ = (x);                                  \
