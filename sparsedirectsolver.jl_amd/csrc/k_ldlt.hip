@@ -600,12 +600,8 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
             dd[q] = ee[q] = T(0);
         }
         if (tid < NF) st[tid] = 0;
-        // the upper triangle of the fully summed square mirrors the lower (the reference's
-        // full square)
-        for (int q = tid; q < w * w; q += NT) {
-            const int j = q / w, i = q - j * w;
-            if (i > j) P[(int64_t)i * f + j] = P[(int64_t)j * f + i];
-        }
+        // (the upper triangle of the fully summed square, the reference's full square, was
+        // mirrored from the lower one by k_ldlt_mirror)
         k = 0;
     } else {
         k = ls[2];
@@ -616,8 +612,68 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
         return;
     }
     Front<T, NT> Fr{P, f, w, lp, &rb, skey, sidx, SCAP};
-    int c, r;
-    Fr.choose(k, prm, c, r);
+    // The usual step in one pass over column k: its entries k..f-1 in registers (the loads
+    // issued together), one combined reduction giving the Bunch-Kaufman partner search (the
+    // largest |a_ik| over rows k+1..w-1, first index on ties) and the threshold test's column
+    // maximum (rows k+1..f-1, NaN-propagating). When column k is its own acceptable 1x1 pivot
+    // (the first outcome of choose: no partner needed, the threshold test passes) the step
+    // takes it and scales the column from the registers; otherwise choose decides from the
+    // panel with the same predicates.
+    constexpr int CR = 8;  // column entries per thread held in registers
+    __shared__ double smx[NT / 64];
+    T x[CR];
+    bool fast = false;
+    if (f - k <= CR * NT) {
+        const T* col = P + (int64_t)k * f;
+#pragma unroll
+        for (int u = 0; u < CR; ++u) x[u] = col[min(k + tid + NT * u, f - 1)];
+        const double akk = fabs((double)col[k]);
+        __builtin_amdgcn_sched_barrier(0);
+        double lam = 0.0, m = 0.0;
+        int rr = -1;
+#pragma unroll
+        for (int u = 0; u < CR; ++u) {
+            const int i = k + tid + NT * u;
+            if (i > k && i < f) {
+                const double a = fabs((double)x[u]);
+                m = jmax(m, a);
+                if (i < w && a > lam) {
+                    lam = a;
+                    rr = i;
+                }
+            }
+        }
+        wave_argmax(lam, rr);
+        m = wave_jmax(m);
+        const int lane = tid & 63, wv = tid >> 6;
+        if (lane == 0) {
+            rb.v[wv] = lam;
+            rb.i[wv] = rr;
+            smx[wv] = m;
+        }
+        __syncthreads();
+        lam = rb.v[0];
+        rr = rb.i[0];
+        m = smx[0];
+        for (int q = 1; q < NT / 64; ++q) {
+            if (vi_better(rb.v[q], rb.i[q], lam, rr)) {
+                lam = rb.v[q];
+                rr = rb.i[q];
+            }
+            m = jmax(m, smx[q]);
+        }
+        __syncthreads();
+        if (rr < 0) lam = 0.0;
+        if (prm.pivot == PV_NONE) {
+            fast = true;
+        } else {
+            const double alpha = (1.0 + sqrt(17.0)) / 8.0;  // Bunch-Kaufman constant (ldlt.jl:22)
+            const bool self = prm.pivot == PV_DIAG || rr < 0 || akk >= alpha * lam;
+            fast = self && akk >= prm.eps && akk >= prm.u * m;
+        }
+    }
+    int c = k, r = -1;
+    if (!fast) Fr.choose(k, prm, c, r);
     const int8_t* sreq_row = psign;
     const int32_t* perm_c0 = S.perm + c0;
     if (r < 0) {
@@ -649,7 +705,16 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
         }
         __syncthreads();
         const T d = T(sh[0]);
-        for (int i = k + 1 + tid; i < f; i += NT) Fr.ref(i, k) = Fr.ref(i, k) / d;
+        if (fast) {
+            T* col = P + (int64_t)k * f;
+#pragma unroll
+            for (int u = 0; u < CR; ++u) {
+                const int i = k + tid + NT * u;
+                if (i > k && i < f) col[i] = x[u] / d;
+            }
+        } else {
+            for (int i = k + 1 + tid; i < f; i += NT) Fr.ref(i, k) = Fr.ref(i, k) / d;
+        }
     } else {
         Fr.swap(k + 1, r);
         if (tid == 0) {
@@ -691,6 +756,36 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
     }
 }
 
+// before the pivot launches: the upper triangle of each large front's fully summed square
+// mirrors the lower one (the reference's full square). task: (s, I0, J0), the 64 x 64 tile of
+// rows I0.. and columns J0.. with I0 >= J0, written transposed through LDS (both sides
+// coalesced)
+template <typename T>
+__global__ void __launch_bounds__(256) k_ldlt_mirror(DevSym S, const Task* __restrict__ tasks, T* __restrict__ factor) {
+    __shared__ T tl[64][65];
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s];
+    const int I0 = t.a, J0 = t.b, ni = min(64, w - I0), nj = min(64, w - J0);
+    T* P = factor + S.panel_off[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    {
+        T v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            v[q] = P[(int64_t)(J0 + min(wv + 4 * q, nj - 1)) * f + I0 + min(lane, ni - 1)];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tl[wv + 4 * q][lane] = v[q];  // tl[j][i] = A(I0 + i, J0 + j)
+    }
+    __syncthreads();
+    // A(J0 + j, I0 + i) = A(I0 + i, J0 + j) for I0 + i > J0 + j: lanes along j
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = wv + 4 * q, j = lane;
+        if (i < ni && j < nj && I0 + i > J0 + j) P[(int64_t)(I0 + i) * f + J0 + j] = tl[j][i];
+    }
+}
+
 // launch pair t: the update of each front's pivot of that pair on the fully summed columns
 // after it (every row below). task: (s, I0, J0), a 64 x 64 tile, lanes along the rows
 template <typename T>
@@ -708,11 +803,22 @@ __global__ void __launch_bounds__(256) k_ldlt_update(DevSym S, const Task* __res
     const T* ck = P + (int64_t)k * f;
     const int j1 = min(t.b + 64, w);
     if (sz == 1) {
+        // the thread's sixteen columns: every load in flight together (clamped to the tile's
+        // last column), then the updates and the stores; row k is not written here
+        const int jb = max(t.b, k2) + (threadIdx.x >> 6);
+        if (jb >= j1) return;
         const T lk = ck[i];
-        for (int j = max(t.b, k2) + (threadIdx.x >> 6); j < j1; j += 4) {
-            T& a = P[(int64_t)j * f + i];
-            a = a - lk * P[(int64_t)j * f + k];
+        T a[16], u[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t o = (int64_t)min(jb + 4 * q, j1 - 1) * f;
+            a[q] = P[o + i];
+            u[q] = P[o + k];
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (jb + 4 * q < j1) P[(int64_t)(jb + 4 * q) * f + i] = a[q] - lk * u[q];
     } else {
         const T l1 = ck[i], l2 = ck[f + i];
         for (int j = max(t.b, k2) + (threadIdx.x >> 6); j < j1; j += 4) {
@@ -1471,6 +1577,11 @@ int64_t ldlt_panel_scratch(int f, int w) { return lp_scratch(f, w); }
 int ldlt_panel_cols() { return LP_NB; }
 
 template <typename T>
+void launch_ldlt_mirror(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor) {
+    hipLaunchKernelGGL(k_ldlt_mirror<T>, dim3(n), dim3(256), 0, st, S, tasks, factor);
+}
+
+template <typename T>
 void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep) {
     hipLaunchKernelGGL(k_ldlt_update<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, lstep);
 }
@@ -1508,6 +1619,7 @@ void launch_info_from_stats(hipStream_t st, const DevSym& S, const int64_t* stat
                                        int8_t*, int64_t*, int32_t*, int, double, double, int, const double*,       \
                                        const int8_t*);                                                             \
     template void launch_ldlt_update<T>(hipStream_t, const DevSym&, const Task*, int, T*, const int32_t*);        \
+    template void launch_ldlt_mirror<T>(hipStream_t, const DevSym&, const Task*, int, T*);                        \
     template void launch_ldlt_panel<T>(hipStream_t, const DevSym&, const Task*, int, int, T*, T*, int32_t*,        \
                                        int8_t*, int64_t*, int32_t*, T*, int, double, double, int, const double*,   \
                                        const int8_t*);                                                             \

@@ -370,33 +370,46 @@ __global__ void __launch_bounds__(256) k_finish(DevSym S, const Task* __restrict
 }
 
 // backward, before the block steps: y_j -= sum_{i >= w} L[i, j] y[rows[i]] for the columns
-// [col0, col0 + 64) of the front; one wave per column. task: (s, col0)
-// The 64 x 64 tiles of L21 and the matching 64 rows of Y (gathered through rowval, 8
-// right-hand sides at a time) are staged in LDS, so L21 is read from memory once and Y once
-// per 64 columns; thread (column jj, rhs pair) accumulates over the rows.
+// [col0, col0 + 64) of the front, in two launches. k_bwd_l21: the partial sums over one chunk of
+// L21_CHUNK rows below the columns. task: (s, col0, chunk, poff); the 64 x 64 tiles of L21 and
+// the matching 64 rows of Y (gathered through rowval, 8 right-hand sides at a time) are staged
+// in LDS, so L21 is read from memory once; thread (column jj, rhs pair) accumulates over the
+// chunk's rows, into part[(poff + chunk w + col) nrhs + r]. k_bwd_l21_sum then subtracts the
+// chunks' sums from Y in chunk order (a fixed order: the result does not depend on the schedule).
+constexpr int L21_CHUNK = 256;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restrict__ tasks,
-                                                const T* __restrict__ factor, T* __restrict__ Y, int nrhs) {
+                                                const T* __restrict__ factor, const T* __restrict__ Y,
+                                                T* __restrict__ part, int nrhs) {
     __shared__ T Lt[64][65];  // Lt[jj][ii] = L(i0 + ii, col0 + jj)
     __shared__ T Yc[64][8];
     const Task t = tasks[blockIdx.x];
-    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], c0 = S.super_ptr[s], col0 = t.a;
+    const int s = t.s, f = S.fnrows[s], w = S.fncols[s], col0 = t.a;
     const int ncol = min(64, w - col0);
+    const int r0 = w + t.b * L21_CHUNK, r1 = min(f, r0 + L21_CHUNK);
     const int tid = threadIdx.x, jj = tid & 63, rr = (tid >> 6) * 2;
     const T* P = factor + S.panel_off[s];
     const int32_t* rows = S.rowval + S.rowptr[s];
+    T* pt = part + ((int64_t)t.c + (int64_t)t.b * w + col0) * nrhs;
     for (int rb = 0; rb < nrhs; rb += 8) {
         T acc0 = T(0), acc1 = T(0);
-        for (int i0 = w; i0 < f; i0 += 64) {
-            const int ni = min(64, f - i0);
+        for (int i0 = r0; i0 < r1; i0 += 64) {
+            const int ni = min(64, r1 - i0);
             {
-                // the tile's sixteen loads per thread in flight together (clamped), then the
-                // masked LDS stores (zeros outside the tile)
+                // the tile's sixteen loads per thread and its Y rows in flight together
+                // (clamped), then the masked LDS stores (zeros outside the tile)
                 T v[16];
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
                     const int q = tid + 256 * u, c = q >> 6, ii = q & 63;
                     v[u] = P[(int64_t)(col0 + min(c, ncol - 1)) * f + i0 + min(ii, ni - 1)];
+                }
+                T yv[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int q = tid + 256 * u, ii = q >> 3, r = q & 7;
+                    yv[u] = Y[(int64_t)rows[i0 + min(ii, ni - 1)] * nrhs + min(rb + r, nrhs - 1)];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -404,10 +417,11 @@ __global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restric
                     const int q = tid + 256 * u, c = q >> 6, ii = q & 63;
                     Lt[c][ii] = (c < ncol && ii < ni) ? v[u] : T(0);
                 }
-            }
-            for (int q = tid; q < 64 * 8; q += 256) {
-                const int ii = q >> 3, r = q & 7;
-                Yc[ii][r] = (ii < ni && rb + r < nrhs) ? Y[(int64_t)rows[i0 + ii] * nrhs + rb + r] : T(0);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int q = tid + 256 * u, ii = q >> 3, r = q & 7;
+                    Yc[ii][r] = (ii < ni && rb + r < nrhs) ? yv[u] : T(0);
+                }
             }
             __syncthreads();
 #pragma unroll 8
@@ -419,9 +433,26 @@ __global__ void __launch_bounds__(256) k_bwd_l21(DevSym S, const Task* __restric
             __syncthreads();
         }
         if (jj < ncol) {
-            if (rb + rr < nrhs) Y[(int64_t)(c0 + col0 + jj) * nrhs + rb + rr] -= acc0;
-            if (rb + rr + 1 < nrhs) Y[(int64_t)(c0 + col0 + jj) * nrhs + rb + rr + 1] -= acc1;
+            if (rb + rr < nrhs) pt[(int64_t)jj * nrhs + rb + rr] = acc0;
+            if (rb + rr + 1 < nrhs) pt[(int64_t)jj * nrhs + rb + rr + 1] = acc1;
         }
+    }
+}
+
+// y_j -= sum over the front's chunks (in order) of the partial sums. task: (s, col0, nch, poff),
+// the columns [col0, col0 + 64)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_l21_sum(DevSym S, const Task* __restrict__ tasks, T* __restrict__ Y,
+                                                    const T* __restrict__ part, int nrhs) {
+    const Task t = tasks[blockIdx.x];
+    const int s = t.s, w = S.fncols[s], col0 = t.a, nch = t.b;
+    const int ncol = min(64, w - col0);
+    T* y = Y + (int64_t)(S.super_ptr[s] + col0) * nrhs;
+    const T* pt = part + ((int64_t)t.c + col0) * nrhs;
+    for (int q = threadIdx.x; q < ncol * nrhs; q += 256) {
+        T acc = pt[q];
+        for (int c = 1; c < nch; ++c) acc += pt[(int64_t)c * w * nrhs + q];
+        y[q] -= acc;
     }
 }
 
@@ -639,9 +670,17 @@ void launch_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T*
 }
 
 template <typename T>
-void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, int nrhs) {
-    hipLaunchKernelGGL(k_bwd_l21<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, nrhs);
+void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* part,
+                    int nrhs) {
+    hipLaunchKernelGGL(k_bwd_l21<T>, dim3(n), dim3(256), 0, st, S, tasks, factor, Y, part, nrhs);
 }
+
+template <typename T>
+void launch_bwd_l21_sum(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, const T* part, int nrhs) {
+    hipLaunchKernelGGL(k_bwd_l21_sum<T>, dim3(n), dim3(256), 0, st, S, tasks, Y, part, nrhs);
+}
+
+int bwd_l21_chunk() { return L21_CHUNK; }
 
 template <typename T>
 void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
@@ -700,7 +739,8 @@ void launch_residual(hipStream_t st, int n, int nrhs, const int64_t* rowptr, con
     template void launch_fwd_pull<T>(hipStream_t, const DevSym&, const Task*, int, T*, T*, T*, int, const int32_t*); \
     template void launch_fwd_step<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, T*, T*, int, int);  \
     template void launch_finish<T>(hipStream_t, const DevSym&, const Task*, int, T*, const T*, int, const int32_t*); \
-    template void launch_bwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, int);               \
+    template void launch_bwd_l21<T>(hipStream_t, const DevSym&, const Task*, int, const T*, const T*, T*, int);     \
+    template void launch_bwd_l21_sum<T>(hipStream_t, const DevSym&, const Task*, int, T*, const T*, int);          \
     template void launch_bwd_step<T>(hipStream_t, const DevSym&, const Task*, int, const T*, T*, T*, int, int);
 INST(double)
 INST(float)

@@ -57,6 +57,7 @@ enum LaunchKind {
     L_LU_SCATTER,
     L_LU_EXTEND,
     L_LU_CB,
+    L_LDLT_MIRROR,   // k_ldlt_mirror: the upper triangle of the fully summed squares
     L_LDLT_PIVOT,    // k_ldlt_pivot: launch pair k's pivot of every large front of the level
     L_LDLT_UPDATE,   // k_ldlt_update: that pivot's update tiles
     L_LDLT_FINISH,   // k_ldlt_finish: unit diagonals and the pivot order
@@ -69,6 +70,7 @@ enum LaunchKind {
     S_FWD_STEP,
     S_FWD_FINISH,
     S_BWD_L21,
+    S_BWD_L21_SUM,
     S_BWD_STEP,
     S_BWD_FINISH
 };
@@ -123,6 +125,7 @@ int prof_kind(int kind) {
         case L_STEP: return SDS_K_TRSM;
         case L_BLOCK: return SDS_K_UPDATE_PANEL;
         case L_LU_PIVOT: return SDS_K_PIVOT_PANEL;
+        case L_LDLT_MIRROR: return SDS_K_PIVOT_PANEL;
         case L_LDLT_PIVOT: return SDS_K_PIVOT_PANEL;
         case L_LDLT_UPDATE: return SDS_K_PIVOT_PANEL;
         case L_LDLT_FINISH: return SDS_K_PIVOT_PANEL;
@@ -887,9 +890,16 @@ int build_schedule(sds_handle* h, int dtype) {
             // LDL^T of the large fronts, one pivot (1x1 or 2x2) per launch pair over all of them:
             // the pivot (one workgroup per front) and its update tiles (the remaining fully summed
             // columns, all rows, 64 x 64 tiles; one tile list per 64 pairs; a pair takes at least
-            // one column, so pair t works on columns >= t)
-            std::vector<Task> tp;
-            for (int32_t s : L.big) tp.push_back({s, 0, 0, 0});
+            // one column, so pair t works on columns >= t), after the mirror tiles of the fully
+            // summed squares
+            std::vector<Task> tp, tm;
+            for (int32_t s : L.big) {
+                tp.push_back({s, 0, 0, 0});
+                const int w = h->fncols[s];
+                for (int J0 = 0; J0 < w; J0 += 64)
+                    for (int I0 = J0; I0 < w; I0 += 64) tm.push_back({s, I0, J0, 0});
+            }
+            push_launch(L, L_LDLT_MIRROR, tm, 0.0, 0.0);
             const int64_t off_p = (int64_t)tasks.size();
             tasks.insert(tasks.end(), tp.begin(), tp.end());
             for (int e = 0; e * 64 < wmax; ++e) {
@@ -1111,6 +1121,7 @@ int build_schedule(sds_handle* h, int dtype) {
         plan.push_back({kind, (int64_t)tasks.size(), (int)t.size(), flops, bytes});
         tasks.insert(tasks.end(), t.begin(), t.end());
     };
+    int64_t l21_units = 0;  // the largest level's backward partial sums, units of nrhs
     for (auto& L : h->levels) {
         std::vector<int32_t> bs;
         for (int32_t s : L.all)
@@ -1145,17 +1156,28 @@ int build_schedule(sds_handle* h, int dtype) {
         for (int32_t s : bs)
             for (int c = 0; c < h->fncols[s]; c += 256) t.push_back({s, c, 0, 0});
         push_solve(L.fwd_plan, S_FWD_FINISH, t, 0, 0);
-        // backward
+        // backward: the partial sums of L21^T x per chunk of rows (into the partial-sum buffer,
+        // a front's chunks from poff on), then their sum per column in chunk order
         t.clear();
+        std::vector<Task> tsum;
         double fl = 0, by = 0;
+        int64_t poff = 0;
+        const int rc = bwd_l21_chunk();
         for (int32_t s : bs) {
             int f = h->fnrows[s], w = h->fncols[s];
             if (f == w) continue;
-            for (int c = 0; c < w; c += 64) t.push_back({s, c, 0, 0});
+            const int nch = (f - w + rc - 1) / rc;
+            for (int c = 0; c < w; c += 64) {
+                for (int q = 0; q < nch; ++q) t.push_back({s, c, q, (int32_t)poff});
+                tsum.push_back({s, c, nch, (int32_t)poff});
+            }
+            poff += (int64_t)nch * w;
             fl += 2.0 * (double)(f - w) * w;
             by += dbl_es * (double)(f - w) * w;
         }
+        l21_units = std::max(l21_units, poff);
         push_solve(L.bwd_plan, S_BWD_L21, t, fl, by);
+        push_solve(L.bwd_plan, S_BWD_L21_SUM, tsum, 0, 0);
         for (int k = ((wmax - 1) / NB) * NB; k >= 0; k -= NB) {
             t.clear();
             fl = by = 0;
@@ -1180,7 +1202,7 @@ int build_schedule(sds_handle* h, int dtype) {
         push_solve(L.bwd_plan, S_BWD_FINISH, t, 0, 0);
     }
     // Cholesky: the diagonal-block inverses and the inverse-block sweep plans
-    h->part_units = 0;
+    h->part_units = h->method == M_CHOLESKY ? 0 : l21_units;
     if (h->method == M_CHOLESKY) {
         std::vector<Task> t[3];
         double fl[3] = {0, 0, 0}, by[3] = {0, 0, 0};
@@ -1440,6 +1462,7 @@ int factorize_ldlt(sds_handle* h, const double* nz, int pivot, double u, double 
                                              h->lstep, pivot, u, eps, scaled, h->aux, psign);
                         break;
                     case L_LDLT_UPDATE: launch_ldlt_update<T>(st, h->dsym, t, ln.n, factor, h->lstep); break;
+                    case L_LDLT_MIRROR: launch_ldlt_mirror<T>(st, h->dsym, t, ln.n, factor); break;
                     case L_LDLT_PANEL:
                         launch_ldlt_panel<T>(st, h->dsym, t, ln.n, ln.k, factor, dv, h->piv, h->pkind, h->stats,
                                              h->lstep, static_cast<T*>(h->lscratch), pivot, u, eps, scaled, h->aux,
@@ -1678,7 +1701,8 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                 case S_FWD_PULL: launch_fwd_pull<T>(st, h->dsym, t, ln.n, Y, U, Z, nrhs, fpiv); break;
                 case S_FWD_STEP: launch_fwd_step<T>(st, h->dsym, t, ln.n, ffac, Y, U, Z, nrhs, unit); break;
                 case S_FWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, nullptr); break;
-                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, bfac, Y, nrhs); break;
+                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, bfac, Y, static_cast<T*>(W.part), nrhs); break;
+                case S_BWD_L21_SUM: launch_bwd_l21_sum<T>(st, h->dsym, t, ln.n, Y, static_cast<const T*>(W.part), nrhs); break;
                 case S_BWD_STEP: launch_bwd_step<T>(st, h->dsym, t, ln.n, bfac, Y, Z, nrhs, unit); break;
                 case S_BWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, bpiv); break;
             }

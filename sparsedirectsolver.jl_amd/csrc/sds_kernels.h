@@ -63,6 +63,10 @@ void launch_ldlt_pivot(hipStream_t st, const DevSym& S, const Task* tasks, int n
                        int scaled, const double* aux, const int8_t* psign);
 template <typename T>
 void launch_ldlt_update(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor, const int32_t* lstep);
+// the upper triangle of each large front's fully summed square from the lower (task (s, I0, J0),
+// a 64 x 64 tile with I0 >= J0), before its pivot launches
+template <typename T>
+void launch_ldlt_mirror(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* factor);
 // deferred-update panels of the large fronts: panel e of every front (task (s, scratch offset)),
 // then its update of the columns after it (task (s, I0, J0, scratch offset))
 template <typename T>
@@ -136,7 +140,11 @@ template <typename T>
 void launch_finish(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, const T* Z, int nrhs,
                    const int32_t* piv);
 template <typename T>
-void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, int nrhs);
+void launch_bwd_l21(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, const T* Y, T* part,
+                    int nrhs);
+template <typename T>
+void launch_bwd_l21_sum(hipStream_t st, const DevSym& S, const Task* tasks, int n, T* Y, const T* part, int nrhs);
+int bwd_l21_chunk();  // rows per partial sum of launch_bwd_l21
 template <typename T>
 void launch_bwd_step(hipStream_t st, const DevSym& S, const Task* tasks, int n, const T* factor, T* Y, T* Z, int nrhs,
                      int unit);
