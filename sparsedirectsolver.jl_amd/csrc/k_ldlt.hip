@@ -679,7 +679,9 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
     if (r < 0) {
         Fr.swap(k, c);
         if (tid == 0) {
-            double d = Fr.at(k, k);
+            // (thread 0 holds a_kk in x[0] on the one-pass step; the sign counts are added by
+            // atomics whose result is not waited for)
+            double d = fast ? (double)x[0] : Fr.at(k, k);
             int8_t kd = KIND_1X1;
             if (!(fabs(d) >= prm.eps) || d == 0.0) {
                 if (d == 0.0) st[ST_NZERO]++;
@@ -695,9 +697,9 @@ __global__ void __launch_bounds__(1024) k_ldlt_pivot(DevSym S, const Task* __res
             dd[k] = T(d);
             ee[k] = T(0);
             if (d > 0.0)
-                st[ST_NPOS]++;
+                atomicAdd(reinterpret_cast<unsigned long long*>(st + ST_NPOS), 1ull);
             else if (d < 0.0)
-                st[ST_NNEG]++;
+                atomicAdd(reinterpret_cast<unsigned long long*>(st + ST_NNEG), 1ull);
             sh[0] = d;
             ls[0] = k;
             ls[1] = 1;
@@ -1467,17 +1469,42 @@ __global__ void __launch_bounds__(256) k_ldlt_cb(DevSym S, const Task* __restric
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int i = I0 + 4 * ty + a, j = J0 + 4 * tx + b;
-            acc[a][b] = (i < f && j <= i) ? C[(int64_t)(j - w) * m + (i - w)] : T(0);
+            const int i = min(I0 + 4 * ty + a, f - 1), j = min(J0 + 4 * tx + b, f - 1);
+            acc[a][b] = C[(int64_t)(j - w) * m + (i - w)];  // (clamped; entries j > i are not stored)
         }
+    // the thread's four (kk, c) staging slots, kk = (tid >> 6) + 4 u, c = tid & 63: the raw
+    // values of chunk k0 + KC (the L entries W(i, k) may combine, D, the pivot kinds, L(j, k))
+    // are loaded, clamped, while chunk k0 is multiplied; W(i, k) is formed as w_entry forms it
+    const int c = tid & 63, ic = min(I0 + c, f - 1), jc = min(J0 + c, f - 1);
+    T l0[4], lm[4], lq[4], d0[4], e0[4], em[4], lj[4];
+    int kd[4];
+    auto load_raw = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kc = min(k0 + (tid >> 6) + 4 * u, w - 1), km = max(kc - 1, 0), kq = min(kc + 1, w - 1);
+            kd[u] = kind[kc];
+            l0[u] = L[(int64_t)kc * f + ic];
+            lm[u] = L[(int64_t)km * f + ic];
+            lq[u] = L[(int64_t)kq * f + ic];
+            d0[u] = dd[kc];
+            e0[u] = ee[kc];
+            em[u] = ee[km];
+            lj[u] = L[(int64_t)kc * f + jc];
+        }
+    };
+    load_raw(0);
     for (int k0 = 0; k0 < w; k0 += KC) {
-        for (int q = tid; q < KC * TILE; q += 256) {
-            const int kk = q / TILE, c = q - kk * TILE, k = k0 + kk;
-            const int i = I0 + c, j = J0 + c;
-            Ws[kk][c] = (k < w && i < f) ? w_entry<T>(L, f, i, k, kind, dd, ee) : T(0);
-            Ls[kk][c] = (k < w && j < f) ? L[(int64_t)k * f + j] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = (tid >> 6) + 4 * u, k = k0 + kk;
+            const T wv = kd[u] == KIND_2X2_FIRST    ? l0[u] * d0[u] + lq[u] * e0[u]
+                         : kd[u] == KIND_2X2_SECOND ? lm[u] * em[u] + l0[u] * d0[u]
+                                                    : l0[u] * d0[u];
+            Ws[kk][c] = (k < w && I0 + c < f) ? wv : T(0);
+            Ls[kk][c] = (k < w && J0 + c < f) ? lj[u] : T(0);
         }
         __syncthreads();
+        if (k0 + KC < w) load_raw(k0 + KC);
         const int kn = min(KC, w - k0);
         for (int kk = 0; kk < kn; ++kk) {
             T wv[4], lv[4];

@@ -515,14 +515,29 @@ __global__ void __launch_bounds__(256) k_lu_cb(DevSym S, const Task* __restrict_
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    // the thread's four (kk, c) staging slots, kk = (tid >> 6) + 4 u, c = tid & 63: the raw
+    // values of chunk k0 + KC are loaded, clamped, while chunk k0 is multiplied
+    const int c = tid & 63, ic = min(I0 + c, f - 1), jc = min(J0 + c, f - 1);
+    T lv[4], dv[4], uv0[4];
+    auto load_raw = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kc = min(k0 + (tid >> 6) + 4 * u, w - 1);
+            lv[u] = L[(int64_t)kc * f + ic];
+            dv[u] = dd[kc];
+            uv0[u] = Q[(int64_t)kc * f + jc];
+        }
+    };
+    load_raw(0);
     for (int k0 = 0; k0 < w; k0 += KC) {
-        for (int q = tid; q < KC * TILE; q += 256) {
-            const int kk = q / TILE, c = q - kk * TILE, k = k0 + kk;
-            const int i = I0 + c, j = J0 + c;
-            Ws[kk][c] = (k < w && i < f) ? L[(int64_t)k * f + i] * dd[k] : T(0);
-            Us[kk][c] = (k < w && j < f) ? Q[(int64_t)k * f + j] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = (tid >> 6) + 4 * u, k = k0 + kk;
+            Ws[kk][c] = (k < w && I0 + c < f) ? lv[u] * dv[u] : T(0);
+            Us[kk][c] = (k < w && J0 + c < f) ? uv0[u] : T(0);
         }
         __syncthreads();
+        if (k0 + KC < w) load_raw(k0 + KC);
         const int kn = min(KC, w - k0);
         for (int kk = 0; kk < kn; ++kk) {
             T wv[4], uv[4];
@@ -537,15 +552,22 @@ __global__ void __launch_bounds__(256) k_lu_cb(DevSym S, const Task* __restrict_
         }
         __syncthreads();
     }
+    // the tile's old values loaded together (clamped), then the masked stores
+    T old[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = min(I0 + 4 * ty + a, f - 1), j = min(J0 + 4 * tx + b, f - 1);
+            old[a][b] = C[(int64_t)(j - w) * m + (i - w)];
+        }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int i = I0 + 4 * ty + a, j = J0 + 4 * tx + b;
-            if (i < f && j < f) {
-                T* x = C + (int64_t)(j - w) * m + (i - w);
-                *x = *x - acc[a][b];
-            }
+            if (i < f && j < f) C[(int64_t)(j - w) * m + (i - w)] = old[a][b] - acc[a][b];
         }
 }
 
