@@ -341,38 +341,55 @@ __global__ void __launch_bounds__(1024) k_lu_pivot(DevSym S, const Task* __restr
             dvec[S.n + c0 + q] = T(0);
         }
         if (tid < NF) st[tid] = 0;
-    } else {
-        // step k - 1's column and row, divided by its pivot
-        const T dp = dd[k - 1];
-        for (int i = k + tid; i < f; i += NT) Fr.l(i, k - 1) = Fr.l(i, k - 1) / dp;
-        for (int j = k + tid; j < f; j += NT) {
-            T& x = Fr.at(k - 1, j);
-            x = x / dp;
-        }
     }
-    __syncthreads();
+    // The usual step in one pass: column k's entries k..f-1 in registers, loaded together with
+    // step k - 1's column and row (which are divided by its pivot here) and reduced once for the
+    // threshold test's column maximum and the row search; the general loops below serve fronts
+    // with more than CR x NT rows.
+    constexpr int CR = 8;
+    __shared__ double sg[NT / 64];
     int r = k;
-    if (pivoting) {
-        double g = 0.0;
-        for (int i = k + tid; i < f; i += NT) g = lu_jmax(g, fabs((double)Fr.l(i, k)));
-        for (int o = 32; o > 0; o >>= 1) g = lu_jmax(g, __shfl_xor(g, o));
-        if (lane == 0) rb.v[wv] = g;
-        __syncthreads();
-        g = rb.v[0];
-        for (int q = 1; q < NT / 64; ++q) g = lu_jmax(g, rb.v[q]);
-        __syncthreads();
-        const double akk = fabs((double)Fr.l(k, k));
-        if (!(akk >= u * g)) {
-            double bv = akk;
+    if (f - k <= CR * NT) {
+        T x[CR], cv[CR], rv[CR];
+        const T* colk = &Fr.l(0, k);
+#pragma unroll
+        for (int q = 0; q < CR; ++q) x[q] = colk[min(k + tid + NT * q, f - 1)];
+        const double akk = fabs((double)colk[k]);
+        if (k > 0) {
+            const T dp = dd[k - 1];
+#pragma unroll
+            for (int q = 0; q < CR; ++q) {
+                const int i = min(k + tid + NT * q, f - 1);
+                cv[q] = Fr.l(i, k - 1);
+                rv[q] = Fr.at(k - 1, i);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < CR; ++q) {
+                const int i = k + tid + NT * q;
+                if (i < f) {
+                    Fr.l(i, k - 1) = cv[q] / dp;
+                    Fr.at(k - 1, i) = rv[q] / dp;
+                }
+            }
+        }
+        if (pivoting) {
+            double g = 0.0, bv = akk;
             int bi = k;
-            for (int i = k + 1 + tid; i < w; i += NT) {
-                const double a = fabs((double)Fr.l(i, k));
-                if (a > bv) {
-                    bv = a;
-                    bi = i;
+#pragma unroll
+            for (int q = 0; q < CR; ++q) {
+                const int i = k + tid + NT * q;
+                if (i < f) {
+                    const double a = fabs((double)x[q]);
+                    g = lu_jmax(g, a);
+                    if (i > k && i < w && a > bv) {
+                        bv = a;
+                        bi = i;
+                    }
                 }
             }
             for (int o = 32; o > 0; o >>= 1) {
+                g = lu_jmax(g, __shfl_xor(g, o));
                 const double ov = __shfl_xor(bv, o);
                 const int oi = __shfl_xor(bi, o);
                 if (ov > bv || (ov == bv && oi < bi)) {
@@ -381,20 +398,79 @@ __global__ void __launch_bounds__(1024) k_lu_pivot(DevSym S, const Task* __restr
                 }
             }
             if (lane == 0) {
+                sg[wv] = g;
                 rb.v[wv] = bv;
                 rb.i[wv] = bi;
             }
             __syncthreads();
+            g = sg[0];
             bv = rb.v[0];
             bi = rb.i[0];
-            for (int q = 1; q < NT / 64; ++q)
+            for (int q = 1; q < NT / 64; ++q) {
+                g = lu_jmax(g, sg[q]);
                 if (rb.v[q] > bv || (rb.v[q] == bv && rb.i[q] < bi)) {
                     bv = rb.v[q];
                     bi = rb.i[q];
                 }
-            r = bi;
+            }
+            if (!(akk >= u * g)) r = bi;
         }
         __syncthreads();
+    } else {
+        if (k > 0) {
+            // step k - 1's column and row, divided by its pivot
+            const T dp = dd[k - 1];
+            for (int i = k + tid; i < f; i += NT) Fr.l(i, k - 1) = Fr.l(i, k - 1) / dp;
+            for (int j = k + tid; j < f; j += NT) {
+                T& x = Fr.at(k - 1, j);
+                x = x / dp;
+            }
+        }
+        __syncthreads();
+        if (pivoting) {
+            double g = 0.0;
+            for (int i = k + tid; i < f; i += NT) g = lu_jmax(g, fabs((double)Fr.l(i, k)));
+            for (int o = 32; o > 0; o >>= 1) g = lu_jmax(g, __shfl_xor(g, o));
+            if (lane == 0) rb.v[wv] = g;
+            __syncthreads();
+            g = rb.v[0];
+            for (int q = 1; q < NT / 64; ++q) g = lu_jmax(g, rb.v[q]);
+            __syncthreads();
+            const double akk = fabs((double)Fr.l(k, k));
+            if (!(akk >= u * g)) {
+                double bv = akk;
+                int bi = k;
+                for (int i = k + 1 + tid; i < w; i += NT) {
+                    const double a = fabs((double)Fr.l(i, k));
+                    if (a > bv) {
+                        bv = a;
+                        bi = i;
+                    }
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double ov = __shfl_xor(bv, o);
+                    const int oi = __shfl_xor(bi, o);
+                    if (ov > bv || (ov == bv && oi < bi)) {
+                        bv = ov;
+                        bi = oi;
+                    }
+                }
+                if (lane == 0) {
+                    rb.v[wv] = bv;
+                    rb.i[wv] = bi;
+                }
+                __syncthreads();
+                bv = rb.v[0];
+                bi = rb.i[0];
+                for (int q = 1; q < NT / 64; ++q)
+                    if (rb.v[q] > bv || (rb.v[q] == bv && rb.i[q] < bi)) {
+                        bv = rb.v[q];
+                        bi = rb.i[q];
+                    }
+                r = bi;
+            }
+            __syncthreads();
+        }
     }
     if (r != k) {
         for (int j = tid; j < f; j += NT) {
@@ -444,17 +520,40 @@ __global__ void __launch_bounds__(256) k_lu_update(DevSym S, const Task* __restr
         // lanes along the rows: column-major panel
         const int i = I0 + tx;
         if (i <= k || i >= f) return;
+        // the thread's sixteen columns: loads in flight together (clamped), then the stores;
+        // row k is not written here
+        const int jb = max(J0, k + 1) + ty, j1 = min(J0 + 64, w);
+        if (jb >= j1) return;
         const T lk = Fr.l(i, k) / dk;
-        for (int j = max(J0, k + 1) + ty; j < min(J0 + 64, w); j += 4) Fr.l(i, j) = Fr.l(i, j) - lk * Fr.l(k, j);
+        T a[16], b[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = min(jb + 4 * q, j1 - 1);
+            a[q] = Fr.l(i, j);
+            b[q] = Fr.l(k, j);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (jb + 4 * q < j1) Fr.l(i, jb + 4 * q) = a[q] - lk * b[q];
     } else {
-        // lanes along the columns: U12 is stored transposed
+        // lanes along the columns: U12 is stored transposed; the thread's sixteen rows as above
         const int j = J0 + tx;
         if (j >= f) return;
+        const int ib = max(I0, k + 1) + ty, i1 = min(I0 + 64, w);
+        if (ib >= i1) return;
         const T rk = Fr.u(k, j);
-        for (int i = max(I0, k + 1) + ty; i < min(I0 + 64, w); i += 4) {
-            const T lk = Fr.l(i, k) / dk;
-            Fr.u(i, j) = Fr.u(i, j) - lk * rk;
+        T a[16], l[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = min(ib + 4 * q, i1 - 1);
+            l[q] = Fr.l(i, k);
+            a[q] = Fr.u(i, j);
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (ib + 4 * q < i1) Fr.u(ib + 4 * q, j) = a[q] - (l[q] / dk) * rk;
     }
 }
 
