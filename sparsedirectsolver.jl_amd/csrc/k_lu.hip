@@ -576,9 +576,22 @@ __global__ void __launch_bounds__(1024) k_lu_finish(DevSym S, const Task* __rest
         x = x / dp;
     }
     __syncthreads();
-    for (int q = tid; q < w * w; q += NT) {
-        const int j = q / w, i = q - j * w;
-        if (i > j) Q[(int64_t)j * f + i] = P[(int64_t)i * f + j];
+    // U^T: eight entries per thread per pass, their loads in flight together (clamped to the
+    // square), then the stores of those below the diagonal
+    const int ww = w * w;
+    for (int q0 = 0; q0 < ww; q0 += 8 * NT) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = min(q0 + tid + u * NT, ww - 1), j = q / w, i = q - j * w;
+            v[u] = P[(int64_t)i * f + j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + tid + u * NT, j = q / w, i = q - j * w;
+            if (q < ww && i > j) Q[(int64_t)j * f + i] = v[u];
+        }
     }
     __syncthreads();
     for (int q = tid; q < w * w; q += NT) {
