@@ -481,6 +481,11 @@ def bench_kkt(args):
                       "backward_error": berr, "inertia": [int(npos), int(nneg), int(n - npos - nneg)],
                       "inertia_expected": [nh, nj, 0], "npivots": int(sds.getparam(s, "npivots")),
                       "pivots_2x2": sds.getparam(s, "pivot_stats")["n2x2"]})
+        if fp == "Float64":
+            # the same refactorization + solve replayed from one HIP graph (launches pre-built)
+            xg = torch.zeros(n, 1, dtype=torch.float64, device="cuda")
+            cases[-1]["graph_ms_per_step"] = round(time_graph(s, xg, b.view(n, 1), args.steps, args.warmup), 3)
+            cases[-1]["graph_backward_error"] = backward_error(K, xg.cpu().numpy(), bh[:, None])
         print(json.dumps(cases[-1]), file=sys.stderr, flush=True)
         if fp == "Float64":
             s64 = s
