@@ -105,12 +105,16 @@ __global__ void __launch_bounds__(256) k_front_col_chol(DevSym S, const int32_t*
         const T rd = T(1) / F[j * f + j];
         for (int i = j + 1 + tid; i < f; i += nt) F[j * f + i] = F[j * f + i] * rd;
         __syncthreads();
+        // the lower triangle of the trailing square: one wave per column, lanes along the rows
         const int r = f - j - 1;
-        for (int q = tid; q < r * r; q += nt) {
-            const int kk = q / r, ii = q - kk * r;
-            if (ii < kk) continue;
-            const T prod = F[j * f + j + 1 + ii] * F[j * f + j + 1 + kk];
-            F[(j + 1 + kk) * f + j + 1 + ii] = F[(j + 1 + kk) * f + j + 1 + ii] - prod;
+        const T* lj = F + j * f + j + 1;
+        for (int kk = tid >> 6; kk < r; kk += nt >> 6) {
+            const T lk = lj[kk];
+            T* col = F + (j + 1 + kk) * f + j + 1;
+            for (int ii = kk + (tid & 63); ii < r; ii += 64) {
+                const T prod = lj[ii] * lk;
+                col[ii] = col[ii] - prod;
+            }
         }
         __syncthreads();
     }
