@@ -216,12 +216,15 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
 // vector): y_i -= L(i, step) y_step, or u_i += L(i, step) y_step. At the front's first step the
 // rows get the children's entries of those rows first (child_list order; update vector rows are
 // zeroed before). Wave v takes the step's 16-column batches v, v + 8, ...; y_step is staged in
-// LDS; the eight partial sums are added in wave order. task: (step, r0).
+// LDS; the eight partial sums are added in wave order. task: (step, r0, copy, wide): a wide
+// step's y_step is read from Z, where k_fwd_inv left it, and the task with copy = 1 also copies it
+// to Y (no task of the launch reads those rows of Y).
 // LDS: SOLVE_STEP 8 + 8 64 8 entries.
 template <typename T>
 __global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __restrict__ steps,
                                                  const Task* __restrict__ tasks, const T* __restrict__ factor,
-                                                 T* __restrict__ Y, T* __restrict__ U, int nrhs) {
+                                                 T* __restrict__ Y, const T* __restrict__ Z, T* __restrict__ U,
+                                                 int nrhs) {
     constexpr int NWV = 8;
     T* ys = reinterpret_cast<T*>(sweep_lds);  // step columns x 8
     T* red = ys + SOLVE_STEP * 8;              // NWV waves x 64 rows x 8
@@ -258,11 +261,14 @@ __global__ void __launch_bounds__(512) k_fwd_rows(DevSym S, const SolveStep* __r
     const int nw = c1 - c0;
     const int i = min(r0 + lane, r1 - 1);
     const T* pi = P + i;
+    const T* ysrc = tk.c ? Z : Y;
+    if (tk.b)
+        for (int q = tid; q < nw * nrhs; q += nt) Y[(int64_t)(cg + c0) * nrhs + q] = Z[(int64_t)(cg + c0) * nrhs + q];
     for (int rb = 0; rb < nrhs; rb += 8) {
         const int nr = min(8, nrhs - rb);
         for (int q = tid; q < nw * 8; q += nt) {
             const int j = q >> 3, r = q & 7;
-            ys[q] = r < nr ? Y[(int64_t)(cg + c0 + j) * nrhs + rb + r] : T(0);
+            ys[q] = r < nr ? ysrc[(int64_t)(cg + c0 + j) * nrhs + rb + r] : T(0);
         }
         __syncthreads();
         T acc[8];
@@ -695,9 +701,9 @@ void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, co
 
 template <typename T>
 void launch_fwd_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
-                     const T* factor, T* Y, T* U, int nrhs) {
+                     const T* factor, T* Y, const T* Z, T* U, int nrhs) {
     const size_t lds = (SOLVE_STEP * 8 + 8 * 64 * 8) * sizeof(T);
-    hipLaunchKernelGGL(k_fwd_rows<T>, dim3(n), dim3(512), lds, st, S, steps, tasks, factor, Y, U, nrhs);
+    hipLaunchKernelGGL(k_fwd_rows<T>, dim3(n), dim3(512), lds, st, S, steps, tasks, factor, Y, Z, U, nrhs);
 }
 
 template <typename T>
@@ -750,7 +756,7 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
     template void launch_fwd_diag<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, int, const T*, \
                                      const T*, T*, T*, int);                                                       \
     template void launch_fwd_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*, T*, \
-                                     T*, int);                                                                     \
+                                     const T*, T*, int);                                                           \
     template void launch_bwd_l21t<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,     \
                                      const T*, T*, int);                                                           \
     template void launch_bwd_diag<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, int, const T*, \

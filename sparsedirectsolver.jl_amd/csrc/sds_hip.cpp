@@ -109,6 +109,7 @@ struct Level {
     struct StepGroup {
         Launch fdiag, frows, bl21t, bdiag;  // narrow steps (one block), rows below, partial sums
         Launch fwide, bwide, copy;           // wide steps: the inverse products, the copies of their rows
+        Launch fcopy;                        // forward copies of the wide steps with no rows below
         int threads;
     };
     std::vector<StepGroup> groups;
@@ -1233,7 +1234,7 @@ int build_schedule(sds_handle* h, int dtype) {
             int wmax = 0;
             for (int32_t s : L.all) wmax = std::max(wmax, h->fncols[s]);
             for (int g = 0; g * SOLVE_STEP < wmax; ++g) {
-                std::vector<Task> td, tr, tt, tfw, tbw, tcp;
+                std::vector<Task> td, tr, tt, tfw, tbw, tcp, tcpf;
                 double e11s = 0, efused = 0, erows = 0;  // factor entries read (see below)
                 int threads = 64;  // narrow steps: one diagonal block (more threads for fused rows)
                 int64_t units = 0;
@@ -1249,10 +1250,14 @@ int build_schedule(sds_handle* h, int dtype) {
                     const bool wide = nw > NB;
                     steps.push_back({s, c0, c1, nch, units, wide ? xlen : -1});
                     units += (int64_t)nch * nw;
+                    // a wide step's rows below read y_step from Z and the first of their tasks copies it
+                    // to Y (no copy launch in the forward sweep); without rows below, a copy task
+                    const bool has_rows = !fused && (c1 < w || (m > 0 && uv_off[s] >= 0));
                     if (wide) {
                         for (int b = 0; b * NB < nw; ++b) {
                             ts.push_back({id, b, 0, 0});
                             tcp.push_back({id, b * NB, std::min(NB, nw - b * NB), 0});
+                            if (!has_rows) tcpf.push_back(tcp.back());
                         }
                         for (int r = 0; r < nw; r += 16) tfw.push_back({id, r, 0, 0});
                         for (int c = 0; c < nw; c += 8) tbw.push_back({id, c, 0, 0});
@@ -1268,9 +1273,11 @@ int build_schedule(sds_handle* h, int dtype) {
                     e11s += (double)nw * (nw + 1) / 2 + (double)NB * nw;
                     (fused ? efused : erows) += (double)mb * nw;
                     if (!fused) {
-                        for (int r0 = c1; r0 < w; r0 += 64) tr.push_back({id, r0, 0, 0});
+                        const size_t t0 = tr.size();
+                        for (int r0 = c1; r0 < w; r0 += 64) tr.push_back({id, r0, 0, wide ? 1 : 0});
                         if (m > 0 && uv_off[s] >= 0)
-                            for (int r0 = w; r0 < f; r0 += 64) tr.push_back({id, r0, 0, 0});
+                            for (int r0 = w; r0 < f; r0 += 64) tr.push_back({id, r0, 0, wide ? 1 : 0});
+                        if (wide && tr.size() > t0) tr[t0].b = 1;
                     }
                     for (int c = c0; c < c1 && nch; c += 8)
                         for (int ch = 0; ch < nch; ++ch) tt.push_back({id, c, ch, 0});
@@ -1286,6 +1293,7 @@ int build_schedule(sds_handle* h, int dtype) {
                 G.fwide = put(tfw, S_FWD_STEP, 0, 0);
                 G.bwide = put(tbw, S_BWD_STEP, 0, 0);
                 G.copy = put(tcp, S_FWD_FINISH, 0, 0);
+                G.fcopy = put(tcpf, S_FWD_FINISH, 0, 0);
                 L.groups.push_back(G);
             }
         }
@@ -1657,13 +1665,14 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                         timed(h, SDS_K_FWD, 0, 0, [&] {
                             launch_fwd_inv<T>(st, h->dsym, stp, tk + G.fwide.off, G.fwide.n, Xb, Y, Z, U, nrhs);
                         });
-                        timed(h, SDS_K_FWD, 0, 0, [&] {
-                            launch_copy_rows<T>(st, h->dsym, stp, tk + G.copy.off, G.copy.n, Y, Z, nrhs);
-                        });
+                        if (G.fcopy.n)
+                            timed(h, SDS_K_FWD, 0, 0, [&] {
+                                launch_copy_rows<T>(st, h->dsym, stp, tk + G.fcopy.off, G.fcopy.n, Y, Z, nrhs);
+                            });
                     }
                     if (G.frows.n)
                         timed(h, SDS_K_FWD, G.frows.flops * nrhs, G.frows.bytes, [&] {
-                            launch_fwd_rows<T>(st, h->dsym, stp, tk + G.frows.off, G.frows.n, L, Y, U, nrhs);
+                            launch_fwd_rows<T>(st, h->dsym, stp, tk + G.frows.off, G.frows.n, L, Y, Z, U, nrhs);
                         });
                 }
         if (fwd && h->schur_s >= 0)

@@ -156,7 +156,7 @@ void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, co
                      const T* factor, const T* DB, T* Y, T* U, int nrhs);
 template <typename T>
 void launch_fwd_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
-                     const T* factor, T* Y, T* U, int nrhs);
+                     const T* factor, T* Y, const T* Z, T* U, int nrhs);
 template <typename T>
 void launch_bwd_l21t(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
                      const T* factor, const T* Y, T* part, int nrhs);
