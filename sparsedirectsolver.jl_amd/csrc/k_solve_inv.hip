@@ -18,14 +18,15 @@
 //             y_step = inverse * t_step:
 //             k_fwd_diag (narrow, one workgroup per step; a narrow front with at most FUSE_ROWS
 //             rows below its columns forms its update vector there too) or k_fwd_inv (wide,
-//             16 rows per workgroup, into Z, then k_copy_rows); then k_fwd_rows adds
-//             L(rows, step) y_step to the rows below the step, 64 per workgroup (the update
-//             vector rows start from the children's entries of their rows).
+//             16 rows per workgroup, into Z); then k_fwd_rows adds L(rows, step) y_step to the
+//             rows below the step, 64 per workgroup (the update vector rows start from the
+//             children's entries of their rows; a wide step's y_step is read from Z and copied
+//             to y by one of its tasks, or by k_copy_rows when it has no rows below).
 //   backward: k_bwd_l21t, partial sums of L(rows below the step, step)^T x per chunk of
 //             L21T_CHUNK rows (one wave per 8 columns, lanes along the rows, a reduce-scatter
 //             across the lanes at the end); then x_step = inverse^T (y_step - partials):
-//             k_bwd_diag (narrow) or k_sub_partials + k_bwd_inv (wide, one wave per 8 columns,
-//             into Z) + k_copy_rows.
+//             k_bwd_diag (narrow) or k_sub_partials (y_step less the partials, into Z) +
+//             k_bwd_inv (wide, one wave per 8 columns, from Z into y).
 #include "sds_common.h"
 #include "sds_kernels.h"
 
@@ -599,12 +600,13 @@ __global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __re
 // backward, wide steps: x_j = sum_{i >= j} X_ij t_i for 8 columns [col0, col0 + 8) of a step,
 // one wave per (step, col0); t = the step's rows of y (k_sub_partials has subtracted the
 // L21^T x partial sums), rows >= col0 staged in LDS. Lanes along the rows of X (coalesced); the 64 sums are
-// reduce-scattered across the lanes. The result goes to Z; k_copy_rows moves it to y.
+// reduce-scattered across the lanes. t (y_step less the partial sums) comes from Z, where
+// k_sub_partials formed it, and x goes to y directly (no task of the launch reads those rows of y).
 // task: (step, col0). LDS: SOLVE_STEP 8 entries.
 template <typename T>
 __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __restrict__ steps,
                                                const Task* __restrict__ tasks, const T* __restrict__ XB,
-                                               const T* __restrict__ Y, T* __restrict__ Z, int nrhs) {
+                                               T* __restrict__ Y, const T* __restrict__ Z, int nrhs) {
     T* t = reinterpret_cast<T*>(sweep_lds);  // rows [col0, nw) of the step x 8
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
@@ -617,7 +619,7 @@ __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __res
         const int nr = min(8, nrhs - rb);
         for (int q = lane; q < (nw - col0) * 8; q += 64) {
             const int i = col0 + (q >> 3), rr = q & 7;
-            t[q] = rr < nr ? Y[(int64_t)(cg + c0 + i) * nrhs + rb + rr] : T(0);
+            t[q] = rr < nr ? Z[(int64_t)(cg + c0 + i) * nrhs + rb + rr] : T(0);
         }
         __syncthreads();
         T a[64];
@@ -644,27 +646,26 @@ __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __res
         }
         const T v = wave_reduce_scatter64<T>(a);
         const int col = col0 + (lane >> 3), rr = lane & 7;
-        if (col < cend && rr < nr) Z[(int64_t)(cg + c0 + col) * nrhs + rb + rr] = v;
+        if (col < cend && rr < nr) Y[(int64_t)(cg + c0 + col) * nrhs + rb + rr] = v;
         __syncthreads();
     }
 }
 
-// y rows [r0, r0 + nr) of a wide step less the L21^T x partial sums of those rows, in chunk
-// order. task: (step, r0, rows)
+// rows [r0, r0 + nr) of a wide step: Z = y less the L21^T x partial sums of those rows, in chunk
+// order (the input of k_bwd_inv). task: (step, r0, rows)
 template <typename T>
 __global__ void __launch_bounds__(256) k_sub_partials(DevSym S, const SolveStep* __restrict__ steps,
-                                                     const Task* __restrict__ tasks, T* __restrict__ Y,
-                                                     const T* __restrict__ part, int nrhs) {
+                                                     const Task* __restrict__ tasks, const T* __restrict__ Y,
+                                                     T* __restrict__ Z, const T* __restrict__ part, int nrhs) {
     const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
-    if (st.nch == 0) return;
     const int nw = st.c1 - st.c0;
-    T* y = Y + (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
+    const int64_t o = (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
     const T* pp = part + (st.poff + tk.a) * nrhs;
     for (int q = threadIdx.x; q < tk.b * nrhs; q += 256) {
-        T v = y[q];
+        T v = Y[o + q];
         for (int ch = 0; ch < st.nch; ++ch) v -= pp[(int64_t)ch * nw * nrhs + q];
-        y[q] = v;
+        Z[o + q] = v;
     }
 }
 
@@ -734,15 +735,15 @@ void launch_fwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, con
 
 template <typename T>
 void launch_bwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
-                    const T* Y, T* Z, int nrhs) {
+                    T* Y, const T* Z, int nrhs) {
     const size_t lds = SOLVE_STEP * 8 * sizeof(T);
     hipLaunchKernelGGL(k_bwd_inv<T>, dim3(n), dim3(64), lds, st, S, steps, tasks, XB, Y, Z, nrhs);
 }
 
 template <typename T>
-void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
-                         const T* part, int nrhs) {
-    hipLaunchKernelGGL(k_sub_partials<T>, dim3(n), dim3(256), 0, st, S, steps, tasks, Y, part, nrhs);
+void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                         const T* Y, T* Z, const T* part, int nrhs) {
+    hipLaunchKernelGGL(k_sub_partials<T>, dim3(n), dim3(256), 0, st, S, steps, tasks, Y, Z, part, nrhs);
 }
 
 template <typename T>
@@ -766,9 +767,9 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
     template void launch_fwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
                                     const T*, T*, const T*, int);                                                            \
     template void launch_bwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
-                                    const T*, T*, int);                                                            \
-    template void launch_sub_partials<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, T*,      \
-                                         const T*, int);                                                           \
+                                    T*, const T*, int);                                                            \
+    template void launch_sub_partials<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int,          \
+                                         const T*, T*, const T*, int);                                             \
     template void launch_copy_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, T*,         \
                                       const T*, int);
 INST(double)

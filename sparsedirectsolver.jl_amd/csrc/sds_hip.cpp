@@ -1690,14 +1690,13 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                                                nrhs);
                         });
                     if (G->bwide.n) {
+                        // t = y_step less the partial sums into Z, then x = inverse^T t from Z into y
                         timed(h, SDS_K_BWD, 0, 0, [&] {
-                            launch_sub_partials<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, part, nrhs);
+                            launch_sub_partials<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, Z, part,
+                                                   nrhs);
                         });
                         timed(h, SDS_K_BWD, 0, 0, [&] {
                             launch_bwd_inv<T>(st, h->dsym, stp, tk + G->bwide.off, G->bwide.n, Xb, Y, Z, nrhs);
-                        });
-                        timed(h, SDS_K_BWD, 0, 0, [&] {
-                            launch_copy_rows<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, Z, nrhs);
                         });
                     }
                 }

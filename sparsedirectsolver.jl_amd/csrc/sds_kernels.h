@@ -171,10 +171,10 @@ void launch_fwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, con
                     const T* Y, T* Z, const T* U, int nrhs);
 template <typename T>
 void launch_bwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
-                    const T* Y, T* Z, int nrhs);
+                    T* Y, const T* Z, int nrhs);
 template <typename T>
-void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
-                         const T* part, int nrhs);
+void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
+                         const T* Y, T* Z, const T* part, int nrhs);
 template <typename T>
 void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, T* Y,
                       const T* Z, int nrhs);
