@@ -142,14 +142,10 @@ __device__ __attribute__((always_inline)) inline T wave_reduce_scatter64(T (&a)[
 // coalesced, t in LDS); a fused front (at most FUSE_ROWS rows below) also forms its update
 // vector u_i = L(i, front) y, one row per thread. LDS: 2 NB 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __restrict__ steps,
-                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
-                                                 const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U,
-                                                 int nrhs) {
+__device__ __attribute__((always_inline)) inline void fwd_diag_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ factor, const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U, int nrhs) {
     T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8: t
     T* yx = tb + NB * 8;                       // NB x 8: y
     const int tid = threadIdx.x, nt = blockDim.x;
-    const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
     const int s = st.s, c0 = st.c0, nw = st.c1 - c0;
     const int f = S.fnrows[s], w = S.fncols[s], m = f - w;
@@ -211,6 +207,14 @@ __global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __r
             }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                 const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U,
+                                                 int nrhs) {
+    fwd_diag_body<T>(S, steps, tasks[blockIdx.x], factor, DB, Y, U, nrhs);
 }
 
 // rows [r0, r1) below a front step (64 per workgroup of 8 waves, within L11 or within the update
@@ -359,13 +363,9 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
 // front): task (step, 0). t = y - the L21^T x partial sums (chunk order), x = Linv^T t over the
 // step's columns (entry (j, r) per thread). LDS: NB 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
-                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
-                                                 const T* __restrict__ DB, T* __restrict__ Y,
-                                                 const T* __restrict__ part, int nrhs) {
+__device__ __attribute__((always_inline)) inline void bwd_diag_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ factor, const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ part, int nrhs) {
     T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8
     const int tid = threadIdx.x, nt = blockDim.x;
-    const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
     const int s = st.s, c0 = st.c0, nw = st.c1 - c0;
     const int cg = S.super_ptr[s];
@@ -393,6 +393,14 @@ __global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __r
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_diag(DevSym S, const SolveStep* __restrict__ steps,
+                                                 const Task* __restrict__ tasks, const T* __restrict__ factor,
+                                                 const T* __restrict__ DB, T* __restrict__ Y,
+                                                 const T* __restrict__ part, int nrhs) {
+    bwd_diag_body<T>(S, steps, tasks[blockIdx.x], factor, DB, Y, part, nrhs);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,13 +541,9 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
 // to Z (the other workgroups still read y); k_copy_rows moves it to y. task: (step, ri0).
 // LDS: SOLVE_STEP 8 + 16 16 8 entries.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __restrict__ steps,
-                                                const Task* __restrict__ tasks, const T* __restrict__ XB,
-                                                const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ U,
-                                                int nrhs) {
+__device__ __attribute__((always_inline)) inline void fwd_inv_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ XB, const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ U, int nrhs) {
     T* t = reinterpret_cast<T*>(sweep_lds);  // step rows x 8
     T* red = t + SOLVE_STEP * 8;              // 16 groups x 16 rows x 8
-    const Task tk = tasks[blockIdx.x];
     const SolveStep st = steps[tk.s];
     const int s = st.s, c0 = st.c0, c1 = st.c1, nw = c1 - c0;
     const int cg = S.super_ptr[s];
@@ -595,6 +599,14 @@ __global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __re
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_inv(DevSym S, const SolveStep* __restrict__ steps,
+                                                const Task* __restrict__ tasks, const T* __restrict__ XB,
+                                                const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ U,
+                                                int nrhs) {
+    fwd_inv_body<T>(S, steps, tasks[blockIdx.x], XB, Y, Z, U, nrhs);
 }
 
 // backward, wide steps: x_j = sum_{i >= j} X_ij t_i for 8 columns [col0, col0 + 8) of a step,
@@ -654,10 +666,7 @@ __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __res
 // rows [r0, r0 + nr) of a wide step: Z = y less the L21^T x partial sums of those rows, in chunk
 // order (the input of k_bwd_inv). task: (step, r0, rows)
 template <typename T>
-__global__ void __launch_bounds__(256) k_sub_partials(DevSym S, const SolveStep* __restrict__ steps,
-                                                     const Task* __restrict__ tasks, const T* __restrict__ Y,
-                                                     T* __restrict__ Z, const T* __restrict__ part, int nrhs) {
-    const Task tk = tasks[blockIdx.x];
+__device__ __attribute__((always_inline)) inline void sub_partials_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ part, int nrhs) {
     const SolveStep st = steps[tk.s];
     const int nw = st.c1 - st.c0;
     const int64_t o = (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
@@ -667,6 +676,13 @@ __global__ void __launch_bounds__(256) k_sub_partials(DevSym S, const SolveStep*
         for (int ch = 0; ch < st.nch; ++ch) v -= pp[(int64_t)ch * nw * nrhs + q];
         Z[o + q] = v;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sub_partials(DevSym S, const SolveStep* __restrict__ steps,
+                                                     const Task* __restrict__ tasks, const T* __restrict__ Y,
+                                                     T* __restrict__ Z, const T* __restrict__ part, int nrhs) {
+    sub_partials_body<T>(S, steps, tasks[blockIdx.x], Y, Z, part, nrhs);
 }
 
 // y rows [r0, r0 + nr) of a wide step from Z (the result of k_fwd_inv / k_bwd_inv).
@@ -679,6 +695,36 @@ __global__ void __launch_bounds__(256) k_copy_rows(DevSym S, const SolveStep* __
     const SolveStep st = steps[tk.s];
     const int64_t o = (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
     for (int q = threadIdx.x; q < tk.b * nrhs; q += 256) Y[o + q] = Z[o + q];
+}
+
+// ---------------------------------------------------------------------------
+
+// One launch for the two independent halves of a step group (different fronts, disjoint rows):
+// forward, the narrow steps (the first ndiag workgroups, tasks of k_fwd_diag) and the wide steps'
+// inverse products (tasks of k_fwd_inv); backward, the narrow steps (k_bwd_diag) and the wide
+// steps' t = y less the partial sums (k_sub_partials). Each workgroup runs its kernel's body.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_group(DevSym S, const SolveStep* __restrict__ steps,
+                                                  const Task* __restrict__ tdiag, int ndiag,
+                                                  const Task* __restrict__ tinv, const T* __restrict__ factor,
+                                                  const T* __restrict__ DB, const T* __restrict__ XB,
+                                                  T* __restrict__ Y, T* __restrict__ Z, T* __restrict__ U, int nrhs) {
+    if ((int)blockIdx.x < ndiag)
+        fwd_diag_body<T>(S, steps, tdiag[blockIdx.x], factor, DB, Y, U, nrhs);
+    else
+        fwd_inv_body<T>(S, steps, tinv[blockIdx.x - ndiag], XB, Y, Z, U, nrhs);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_group(DevSym S, const SolveStep* __restrict__ steps,
+                                                  const Task* __restrict__ tdiag, int ndiag,
+                                                  const Task* __restrict__ tsub, const T* __restrict__ factor,
+                                                  const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ Z,
+                                                  const T* __restrict__ part, int nrhs) {
+    if ((int)blockIdx.x < ndiag)
+        bwd_diag_body<T>(S, steps, tdiag[blockIdx.x], factor, DB, Y, part, nrhs);
+    else
+        sub_partials_body<T>(S, steps, tsub[blockIdx.x - ndiag], Y, Z, part, nrhs);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,6 +744,24 @@ void launch_fwd_diag(hipStream_t st, const DevSym& S, const SolveStep* steps, co
                      const T* factor, const T* DB, T* Y, T* U, int nrhs) {
     const size_t lds = 2 * NB * 8 * sizeof(T);
     hipLaunchKernelGGL(k_fwd_diag<T>, dim3(n), dim3(threads), lds, st, S, steps, tasks, factor, DB, Y, U, nrhs);
+}
+
+template <typename T>
+void launch_fwd_group(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tdiag, int ndiag,
+                      const Task* tinv, int ninv, const T* factor, const T* DB, const T* XB, T* Y, T* Z, T* U,
+                      int nrhs) {
+    const size_t lds = std::max(2 * NB * 8, SOLVE_STEP * 8 + 16 * 16 * 8) * sizeof(T);
+    hipLaunchKernelGGL(k_fwd_group<T>, dim3(ndiag + ninv), dim3(256), lds, st, S, steps, tdiag, ndiag, tinv, factor,
+                       DB, XB, Y, Z, U, nrhs);
+}
+
+template <typename T>
+void launch_bwd_group(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tdiag, int ndiag,
+                      const Task* tsub, int nsub, const T* factor, const T* DB, T* Y, T* Z, const T* part,
+                      int nrhs) {
+    const size_t lds = NB * 8 * sizeof(T);
+    hipLaunchKernelGGL(k_bwd_group<T>, dim3(ndiag + nsub), dim3(256), lds, st, S, steps, tdiag, ndiag, tsub, factor,
+                       DB, Y, Z, part, nrhs);
 }
 
 template <typename T>
@@ -771,7 +835,11 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
     template void launch_sub_partials<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int,          \
                                          const T*, T*, const T*, int);                                             \
     template void launch_copy_rows<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, T*,         \
-                                      const T*, int);
+                                      const T*, int);                                                              \
+    template void launch_fwd_group<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const Task*, \
+                                      int, const T*, const T*, const T*, T*, T*, T*, int);                         \
+    template void launch_bwd_group<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const Task*, \
+                                      int, const T*, const T*, T*, T*, const T*, int);
 INST(double)
 INST(float)
 #undef INST

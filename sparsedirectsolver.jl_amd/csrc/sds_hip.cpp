@@ -1656,20 +1656,26 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
         if (fwd)
             for (auto& Lv : h->levels)
                 for (auto& G : Lv.groups) {
-                    if (G.fdiag.n)
+                    if (G.fdiag.n && G.fwide.n) {
+                        // narrow and wide steps of the group (different fronts) in one launch
+                        timed(h, SDS_K_FWD, G.fdiag.flops * nrhs, G.fdiag.bytes, [&] {
+                            launch_fwd_group<T>(st, h->dsym, stp, tk + G.fdiag.off, G.fdiag.n, tk + G.fwide.off,
+                                                G.fwide.n, L, Li, Xb, Y, Z, U, nrhs);
+                        });
+                    } else if (G.fdiag.n) {
                         timed(h, SDS_K_FWD, G.fdiag.flops * nrhs, G.fdiag.bytes, [&] {
                             launch_fwd_diag<T>(st, h->dsym, stp, tk + G.fdiag.off, G.fdiag.n, G.threads, L, Li, Y, U,
                                                nrhs);
                         });
-                    if (G.fwide.n) {
+                    } else if (G.fwide.n) {
                         timed(h, SDS_K_FWD, 0, 0, [&] {
                             launch_fwd_inv<T>(st, h->dsym, stp, tk + G.fwide.off, G.fwide.n, Xb, Y, Z, U, nrhs);
                         });
-                        if (G.fcopy.n)
-                            timed(h, SDS_K_FWD, 0, 0, [&] {
-                                launch_copy_rows<T>(st, h->dsym, stp, tk + G.fcopy.off, G.fcopy.n, Y, Z, nrhs);
-                            });
                     }
+                    if (G.fcopy.n)
+                        timed(h, SDS_K_FWD, 0, 0, [&] {
+                            launch_copy_rows<T>(st, h->dsym, stp, tk + G.fcopy.off, G.fcopy.n, Y, Z, nrhs);
+                        });
                     if (G.frows.n)
                         timed(h, SDS_K_FWD, G.frows.flops * nrhs, G.frows.bytes, [&] {
                             launch_fwd_rows<T>(st, h->dsym, stp, tk + G.frows.off, G.frows.n, L, Y, Z, U, nrhs);
@@ -1684,17 +1690,25 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                         timed(h, SDS_K_BWD, G->bl21t.flops * nrhs, G->bl21t.bytes, [&] {
                             launch_bwd_l21t<T>(st, h->dsym, stp, tk + G->bl21t.off, G->bl21t.n, L, Y, part, nrhs);
                         });
-                    if (G->bdiag.n)
+                    // narrow steps, and the wide steps' t = y_step less the partial sums into Z (one
+                    // launch when the group has both), then the wide steps' x = inverse^T t into y
+                    if (G->bdiag.n && G->bwide.n) {
+                        timed(h, SDS_K_BWD, G->bdiag.flops * nrhs, G->bdiag.bytes, [&] {
+                            launch_bwd_group<T>(st, h->dsym, stp, tk + G->bdiag.off, G->bdiag.n, tk + G->copy.off,
+                                                G->copy.n, L, Li, Y, Z, part, nrhs);
+                        });
+                    } else if (G->bdiag.n) {
                         timed(h, SDS_K_BWD, G->bdiag.flops * nrhs, G->bdiag.bytes, [&] {
                             launch_bwd_diag<T>(st, h->dsym, stp, tk + G->bdiag.off, G->bdiag.n, 64, L, Li, Y, part,
                                                nrhs);
                         });
-                    if (G->bwide.n) {
-                        // t = y_step less the partial sums into Z, then x = inverse^T t from Z into y
+                    } else if (G->bwide.n) {
                         timed(h, SDS_K_BWD, 0, 0, [&] {
                             launch_sub_partials<T>(st, h->dsym, stp, tk + G->copy.off, G->copy.n, Y, Z, part,
                                                    nrhs);
                         });
+                    }
+                    if (G->bwide.n) {
                         timed(h, SDS_K_BWD, 0, 0, [&] {
                             launch_bwd_inv<T>(st, h->dsym, stp, tk + G->bwide.off, G->bwide.n, Xb, Y, Z, nrhs);
                         });

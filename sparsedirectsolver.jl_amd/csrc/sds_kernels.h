@@ -172,6 +172,16 @@ void launch_fwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, con
 template <typename T>
 void launch_bwd_inv(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n, const T* XB,
                     T* Y, const T* Z, int nrhs);
+// one launch for a step group's narrow steps and wide steps (forward: k_fwd_diag + k_fwd_inv tasks;
+// backward: k_bwd_diag + k_sub_partials tasks), 256 threads
+template <typename T>
+void launch_fwd_group(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tdiag, int ndiag,
+                      const Task* tinv, int ninv, const T* factor, const T* DB, const T* XB, T* Y, T* Z, T* U,
+                      int nrhs);
+template <typename T>
+void launch_bwd_group(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tdiag, int ndiag,
+                      const Task* tsub, int nsub, const T* factor, const T* DB, T* Y, T* Z, const T* part,
+                      int nrhs);
 template <typename T>
 void launch_sub_partials(hipStream_t st, const DevSym& S, const SolveStep* steps, const Task* tasks, int n,
                          const T* Y, T* Z, const T* part, int nrhs);
