@@ -142,7 +142,11 @@ __device__ __attribute__((always_inline)) inline T wave_reduce_scatter64(T (&a)[
 // coalesced, t in LDS); a fused front (at most FUSE_ROWS rows below) also forms its update
 // vector u_i = L(i, front) y, one row per thread. LDS: 2 NB 8 entries.
 template <typename T>
-__device__ __attribute__((always_inline)) inline void fwd_diag_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ factor, const T* __restrict__ DB, T* __restrict__ Y, T* __restrict__ U, int nrhs) {
+__device__ __attribute__((always_inline)) inline void fwd_diag_body(const DevSym& S,
+                                                                    const SolveStep* __restrict__ steps, const Task tk,
+                                                                    const T* __restrict__ factor,
+                                                                    const T* __restrict__ DB, T* __restrict__ Y,
+                                                                    T* __restrict__ U, int nrhs) {
     T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8: t
     T* yx = tb + NB * 8;                       // NB x 8: y
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -363,7 +367,11 @@ __global__ void __launch_bounds__(64) k_bwd_l21t(DevSym S, const SolveStep* __re
 // front): task (step, 0). t = y - the L21^T x partial sums (chunk order), x = Linv^T t over the
 // step's columns (entry (j, r) per thread). LDS: NB 8 entries.
 template <typename T>
-__device__ __attribute__((always_inline)) inline void bwd_diag_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ factor, const T* __restrict__ DB, T* __restrict__ Y, const T* __restrict__ part, int nrhs) {
+__device__ __attribute__((always_inline)) inline void bwd_diag_body(const DevSym& S,
+                                                                    const SolveStep* __restrict__ steps, const Task tk,
+                                                                    const T* __restrict__ factor,
+                                                                    const T* __restrict__ DB, T* __restrict__ Y,
+                                                                    const T* __restrict__ part, int nrhs) {
     T* tb = reinterpret_cast<T*>(sweep_lds);  // NB x 8
     const int tid = threadIdx.x, nt = blockDim.x;
     const SolveStep st = steps[tk.s];
@@ -541,7 +549,10 @@ __global__ void __launch_bounds__(256) k_step_inverse(DevSym S, const SolveStep*
 // to Z (the other workgroups still read y); k_copy_rows moves it to y. task: (step, ri0).
 // LDS: SOLVE_STEP 8 + 16 16 8 entries.
 template <typename T>
-__device__ __attribute__((always_inline)) inline void fwd_inv_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ XB, const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ U, int nrhs) {
+__device__ __attribute__((always_inline)) inline void fwd_inv_body(const DevSym& S, const SolveStep* __restrict__ steps,
+                                                                   const Task tk, const T* __restrict__ XB,
+                                                                   const T* __restrict__ Y, T* __restrict__ Z,
+                                                                   const T* __restrict__ U, int nrhs) {
     T* t = reinterpret_cast<T*>(sweep_lds);  // step rows x 8
     T* red = t + SOLVE_STEP * 8;              // 16 groups x 16 rows x 8
     const SolveStep st = steps[tk.s];
@@ -666,7 +677,11 @@ __global__ void __launch_bounds__(64) k_bwd_inv(DevSym S, const SolveStep* __res
 // rows [r0, r0 + nr) of a wide step: Z = y less the L21^T x partial sums of those rows, in chunk
 // order (the input of k_bwd_inv). task: (step, r0, rows)
 template <typename T>
-__device__ __attribute__((always_inline)) inline void sub_partials_body(const DevSym& S, const SolveStep* __restrict__ steps, const Task tk, const T* __restrict__ Y, T* __restrict__ Z, const T* __restrict__ part, int nrhs) {
+__device__ __attribute__((always_inline)) inline void sub_partials_body(const DevSym& S,
+                                                                        const SolveStep* __restrict__ steps,
+                                                                        const Task tk, const T* __restrict__ Y,
+                                                                        T* __restrict__ Z, const T* __restrict__ part,
+                                                                        int nrhs) {
     const SolveStep st = steps[tk.s];
     const int nw = st.c1 - st.c0;
     const int64_t o = (int64_t)(S.super_ptr[st.s] + st.c0 + tk.a) * nrhs;
@@ -829,7 +844,7 @@ void launch_copy_rows(hipStream_t st, const DevSym& S, const SolveStep* steps, c
     template void launch_step_inverse<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int,          \
                                          const T*, const T*, T*);                                                  \
     template void launch_fwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
-                                    const T*, T*, const T*, int);                                                            \
+                                    const T*, T*, const T*, int);                                                  \
     template void launch_bwd_inv<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int, const T*,      \
                                     T*, const T*, int);                                                            \
     template void launch_sub_partials<T>(hipStream_t, const DevSym&, const SolveStep*, const Task*, int,          \

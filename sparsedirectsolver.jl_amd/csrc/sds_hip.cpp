@@ -1723,8 +1723,12 @@ void sweeps(sds_handle* h, const SolveWork& W, int nrhs, int fwd, int diag, int 
                 case S_FWD_PULL: launch_fwd_pull<T>(st, h->dsym, t, ln.n, Y, U, Z, nrhs, fpiv); break;
                 case S_FWD_STEP: launch_fwd_step<T>(st, h->dsym, t, ln.n, ffac, Y, U, Z, nrhs, unit); break;
                 case S_FWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, nullptr); break;
-                case S_BWD_L21: launch_bwd_l21<T>(st, h->dsym, t, ln.n, bfac, Y, static_cast<T*>(W.part), nrhs); break;
-                case S_BWD_L21_SUM: launch_bwd_l21_sum<T>(st, h->dsym, t, ln.n, Y, static_cast<const T*>(W.part), nrhs); break;
+                case S_BWD_L21:
+                    launch_bwd_l21<T>(st, h->dsym, t, ln.n, bfac, Y, static_cast<T*>(W.part), nrhs);
+                    break;
+                case S_BWD_L21_SUM:
+                    launch_bwd_l21_sum<T>(st, h->dsym, t, ln.n, Y, static_cast<const T*>(W.part), nrhs);
+                    break;
                 case S_BWD_STEP: launch_bwd_step<T>(st, h->dsym, t, ln.n, bfac, Y, Z, nrhs, unit); break;
                 case S_BWD_FINISH: launch_finish<T>(st, h->dsym, t, ln.n, Y, Z, nrhs, bpiv); break;
             }
