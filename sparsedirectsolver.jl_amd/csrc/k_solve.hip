@@ -90,7 +90,7 @@ __device__ inline void load_block(T (*B)[NB + 1], const T* __restrict__ P, int f
 #pragma unroll
     for (int u = 0; u < NB * NB / 256; ++u) {
         const int q = threadIdx.x + 256 * u, jj = q / NB, ii = q - jj * NB;
-        v[u] = P[(int64_t)(j0 + min(jj, nj - 1)) * f + i0 + min(ii, ni - 1)];
+        v[u] = P[(int64_t)(j0 + min(jj, max(nj - 1, 0))) * f + i0 + min(ii, max(ni - 1, 0))];  // (nj may be 0)
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
